@@ -1,0 +1,506 @@
+// dress_build.hip -- device-side graph construction for libdress_hip.so.
+//
+// Follows the construction of libdress v0.8.3 (libdress/src/dress.c:103-296,
+// 307-363) but runs it on the GPU:
+//   1. half-edges of the variant adjacency are generated per edge record
+//      (UNDIRECTED both directions with w-bar = 2w, DIRECTED out-edges plus
+//      in-edges without a reciprocal, reciprocal weights summed; FORWARD
+//      out-edges; BACKWARD in-edges), keyed (row, neighbour) and radix sorted.
+//      The sort is stable and half-edges are generated in edge order, so
+//      ties on the neighbour end up ordered by edge index -- the order the
+//      reference's stable qsort leaves them in.
+//   2. the intersection index: for every edge, the multiset intersection of
+//      its two sorted rows.  A group of lanes walks the shorter row and
+//      binary-searches each element in the longer one; the i-th copy of a
+//      repeated neighbour pairs with the i-th copy on the other side, which
+//      is what the reference's merge walk produces.  Matches are emitted in
+//      ascending neighbour order with a ballot + prefix popcount, so the
+//      index equals the reference's intercept arrays entry for entry.
+#include "dress_hip_internal.h"
+
+#include <algorithm>
+
+namespace dh {
+
+namespace {
+
+constexpr int TPB = 256;
+
+inline int grid_for(int64_t threads, int cap_blocks = 1 << 20)
+{
+    int64_t b = (threads + TPB - 1) / TPB;
+    if (b < 1) b = 1;
+    if (b > cap_blocks) b = cap_blocks;
+    return (int)b;
+}
+
+__global__ void k_fill_i32(int *p, int64_t n, int v)
+{
+    for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n;
+         i += (int64_t)gridDim.x * blockDim.x)
+        p[i] = v;
+}
+
+__global__ void k_fill_f64(double *p, int64_t n, double v)
+{
+    for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n;
+         i += (int64_t)gridDim.x * blockDim.x)
+        p[i] = v;
+}
+
+// raw out-adjacency keys for DIRECTED (reciprocal lookup only)
+__global__ void k_raw_keys(const int *U, const int *V, int E, uint64_t *key, int *val, int *deg)
+{
+    for (int e = blockIdx.x * blockDim.x + threadIdx.x; e < E; e += gridDim.x * blockDim.x) {
+        key[e] = ((uint64_t)(uint32_t)U[e] << 32) | (uint32_t)V[e];
+        val[e] = e;
+        atomicAdd(&deg[U[e]], 1);
+    }
+}
+
+// Binary search for neighbour v in raw row u with the probe sequence of
+// dress.c:70-87, so repeated edges resolve to the same entry.
+__device__ int raw_find(const int64_t *roff, const uint64_t *rkey, int u, int v)
+{
+    int64_t lo = roff[u], hi = roff[u + 1] - 1;
+    while (lo <= hi) {
+        int64_t mid = lo + (hi - lo) / 2;
+        int x = (int)(uint32_t)(rkey[mid] & 0xffffffffu);
+        if (x == v) return (int)mid;
+        if (x < v) lo = mid + 1; else hi = mid - 1;
+    }
+    return -1;
+}
+
+// variant half-edges: slot 2e is the out/primary entry, slot 2e+1 the
+// in/secondary one; unused slots get the all-ones key and sort last.
+__global__ void k_half_keys(const int *U, const int *V, const double *W, int E, int variant,
+                            const int64_t *roff, const uint64_t *rkey, const int *rval,
+                            uint64_t *key, int *val, double *ew, int *deg)
+{
+    for (int e = blockIdx.x * blockDim.x + threadIdx.x; e < E; e += gridDim.x * blockDim.x) {
+        int u = U[e], v = V[e];
+        double w = W ? W[e] : 1.0;
+        uint64_t k0 = ~0ull, k1 = ~0ull;
+        double wb = w;
+        if (variant == DRESS_VARIANT_UNDIRECTED) {
+            k0 = ((uint64_t)(uint32_t)u << 32) | (uint32_t)v;
+            k1 = ((uint64_t)(uint32_t)v << 32) | (uint32_t)u;
+            wb = 2.0 * w;
+        } else if (variant == DRESS_VARIANT_DIRECTED) {
+            k0 = ((uint64_t)(uint32_t)u << 32) | (uint32_t)v;
+            int r = raw_find(roff, rkey, v, u);
+            if (r >= 0) wb = w + (W ? W[rval[r]] : 1.0);
+            else k1 = ((uint64_t)(uint32_t)v << 32) | (uint32_t)u;
+        } else if (variant == DRESS_VARIANT_FORWARD) {
+            k0 = ((uint64_t)(uint32_t)u << 32) | (uint32_t)v;
+        } else {
+            k0 = ((uint64_t)(uint32_t)v << 32) | (uint32_t)u;
+        }
+        key[2 * (int64_t)e] = k0;
+        key[2 * (int64_t)e + 1] = k1;
+        val[2 * (int64_t)e] = 2 * e;
+        val[2 * (int64_t)e + 1] = 2 * e + 1;
+        ew[e] = wb;
+        if (k0 != ~0ull) atomicAdd(&deg[(int)(k0 >> 32)], 1);
+        if (k1 != ~0ull) atomicAdd(&deg[(int)(k1 >> 32)], 1);
+    }
+}
+
+__global__ void k_unpack_half(const uint64_t *key, const int *val, int64_t S, int *adj_x,
+                              int *adj_e, int2 *pos)
+{
+    for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < S;
+         i += (int64_t)gridDim.x * blockDim.x) {
+        int e = val[i] >> 1;
+        adj_x[i] = (int)(uint32_t)(key[i] & 0xffffffffu);
+        adj_e[i] = e;
+        if (atomicCAS(&pos[e].x, -1, (int)i) != -1) pos[e].y = (int)i;
+    }
+}
+
+__global__ void k_i64_to_i32(const int64_t *a, int *b, int64_t n)
+{
+    for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n;
+         i += (int64_t)gridDim.x * blockDim.x)
+        b[i] = (int)a[i];
+}
+
+__global__ void k_edge_consts(const int *U, const int *V, const double *nw, int E, int variant,
+                              double *c4)
+{
+    for (int e = blockIdx.x * blockDim.x + threadIdx.x; e < E; e += gridDim.x * blockDim.x) {
+        double a = 4.0 * nw[U[e]];
+        c4[e] = (variant == DRESS_VARIANT_FORWARD || variant == DRESS_VARIANT_BACKWARD)
+                    ? a : a + 4.0 * nw[V[e]];
+    }
+}
+
+__global__ void k_max_deg(const int *row_off, int N, int *out)
+{
+    int m = 0;
+    for (int u = blockIdx.x * blockDim.x + threadIdx.x; u < N; u += gridDim.x * blockDim.x)
+        m = max(m, row_off[u + 1] - row_off[u]);
+    for (int o = 32; o > 0; o >>= 1) m = max(m, __shfl_xor(m, o));
+    if ((threadIdx.x & 63) == 0) atomicMax(out, m);
+}
+
+// first index in [lo, hi) whose value is >= x
+__device__ __forceinline__ int lower_bound_i(const int *a, int lo, int hi, int x)
+{
+    while (lo < hi) {
+        int mid = (lo + hi) >> 1;
+        if (a[mid] < x) lo = mid + 1; else hi = mid;
+    }
+    return lo;
+}
+
+// Multiset intersection of rows U[e], V[e].  GS lanes per edge walk the
+// shorter row; FILL=false counts, FILL=true writes (pos in row U, pos in
+// row V) pairs at ioff[e] in ascending neighbour order.
+template <int GS, bool FILL>
+__global__ void k_isect(const int *U, const int *V, int E, const int *row_off, const int *adj_x,
+                        int *cnt, const int64_t *ioff, int2 *ipair)
+{
+    const int lane = threadIdx.x % GS;
+    const int wlane = threadIdx.x & 63;
+    const int gbase = wlane - lane;              // first lane of my group in the wave
+    const int64_t groups = (int64_t)gridDim.x * blockDim.x / GS;
+    for (int64_t g = (blockIdx.x * (int64_t)blockDim.x + threadIdx.x) / GS; g < E; g += groups) {
+        int e = (int)g;
+        int u = U[e], v = V[e];
+        int ub = row_off[u], ue = row_off[u + 1];
+        int vb = row_off[v], ve = row_off[v + 1];
+        bool a_is_u = (ue - ub) <= (ve - vb);
+        int ab = a_is_u ? ub : vb, ae = a_is_u ? ue : ve;
+        int bb = a_is_u ? vb : ub, be = a_is_u ? ve : ue;
+        int64_t out = FILL ? ioff[e] : 0;
+        int found = 0;
+        for (int base = ab; base < ae; base += GS) {
+            int p = base + lane;
+            bool hit = false;
+            int q = 0;
+            if (p < ae) {
+                int x = adj_x[p];
+                int first = (p > ab && adj_x[p - 1] == x) ? lower_bound_i(adj_x, ab, p, x) : p;
+                int r = p - first;
+                int lo = lower_bound_i(adj_x, bb, be, x);
+                int have = 0;
+                while (lo + have < be && adj_x[lo + have] == x && have <= r) have++;
+                if (r < have) { hit = true; q = lo + r; }
+            }
+            if (FILL) {
+                unsigned long long m = __ballot(hit);
+                unsigned long long gm = (GS == 64) ? m : ((m >> gbase) & ((1ull << GS) - 1));
+                if (hit) {
+                    int before = __popcll(gm & ((1ull << lane) - 1));
+                    int2 pr;
+                    pr.x = a_is_u ? p : q;
+                    pr.y = a_is_u ? q : p;
+                    ipair[out + before] = pr;
+                }
+                out += __popcll(gm);
+            } else {
+                found += hit ? 1 : 0;
+            }
+        }
+        if (!FILL) {
+            for (int o = GS / 2; o > 0; o >>= 1) found += __shfl_xor(found, o, GS);
+            if (lane == 0) cnt[e] = found;
+        }
+    }
+}
+
+// bin key: (bin << 32) | item, bin chosen from the number of summed terms
+__global__ void k_bin_keys_rows(const int *row_off, int N, uint64_t *key)
+{
+    for (int u = blockIdx.x * blockDim.x + threadIdx.x; u < N; u += gridDim.x * blockDim.x) {
+        int terms = row_off[u + 1] - row_off[u] + 1;
+        int b = terms <= 4 ? 0 : terms <= 32 ? 1 : terms <= 256 ? 2 : terms <= 4096 ? 3 : 4;
+        key[u] = ((uint64_t)b << 32) | (uint32_t)u;
+    }
+}
+
+__global__ void k_bin_keys_edges(const int64_t *ioff, int E, uint64_t *key)
+{
+    for (int e = blockIdx.x * blockDim.x + threadIdx.x; e < E; e += gridDim.x * blockDim.x) {
+        int64_t terms = ioff[e + 1] - ioff[e] + 1;
+        int b = terms <= 4 ? 0 : terms <= 32 ? 1 : terms <= 256 ? 2 : terms <= 4096 ? 3 : 4;
+        key[e] = ((uint64_t)b << 32) | (uint32_t)e;
+    }
+}
+
+__global__ void k_bin_unpack(const uint64_t *key, int n, int *list, int *counts)
+{
+    __shared__ int c[Bins::NB];
+    if (threadIdx.x < Bins::NB) c[threadIdx.x] = 0;
+    __syncthreads();
+    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
+        list[i] = (int)(uint32_t)(key[i] & 0xffffffffu);
+        atomicAdd(&c[(int)(key[i] >> 32)], 1);
+    }
+    __syncthreads();
+    if (threadIdx.x < Bins::NB) atomicAdd(&counts[threadIdx.x], c[threadIdx.x]);
+}
+
+bool make_bins(DevGraph *G, Bins *B, int n, bool rows)
+{
+    hipStream_t s = G->stream;
+    uint64_t *k0 = nullptr, *k1 = nullptr;
+    int *cnt = nullptr;
+    if (!dalloc(&k0, (size_t)n + 1, "bin keys") || !dalloc(&k1, (size_t)n + 1, "bin keys") ||
+        !dalloc(&cnt, Bins::NB, "bin counts")) {
+        dfree(k0); dfree(k1); dfree(cnt);
+        return false;
+    }
+    bool ok = true;
+    if (n > 0) {
+        if (rows)
+            hipLaunchKernelGGL(k_bin_keys_rows, dim3(grid_for(n, 4096)), dim3(TPB), 0, s, G->row_off, n, k0);
+        else
+            hipLaunchKernelGGL(k_bin_keys_edges, dim3(grid_for(n, 4096)), dim3(TPB), 0, s, G->ioff, n, k0);
+        ok = prim_sort_u64(k0, k1, (size_t)n, s);
+    }
+    if (ok && !dalloc(&B->list, (size_t)n + 1, "bin list")) ok = false;
+    if (ok) {
+        hipError_t e1 = hipMemsetAsync(cnt, 0, Bins::NB * sizeof(int), s);
+        if (e1 != hipSuccess) { set_error("bin count reset failed"); ok = false; }
+    }
+    if (ok && n > 0)
+        hipLaunchKernelGGL(k_bin_unpack, dim3(grid_for(n, 2048)), dim3(TPB), 0, s, k1, n, B->list, cnt);
+    int hc[Bins::NB] = {0, 0, 0, 0, 0};
+    if (ok) {
+        hipError_t e2 = hipMemcpyAsync(hc, cnt, sizeof(hc), hipMemcpyDeviceToHost, s);
+        hipError_t e3 = hipStreamSynchronize(s);
+        if (e2 != hipSuccess || e3 != hipSuccess) { set_error("bin download failed"); ok = false; }
+    }
+    if (ok) {
+        int st = 0;
+        for (int b = 0; b < Bins::NB; b++) { B->start[b] = st; B->count[b] = hc[b]; st += hc[b]; }
+    }
+    dfree(k0); dfree(k1); dfree(cnt);
+    return ok;
+}
+
+}  // namespace
+
+bool build_graph(DevGraph *G, int N, int E, const int *U, const int *V, const double *W,
+                 const double *NW, int variant)
+{
+    if (N < 0 || E < 0) { set_error("negative graph size"); return false; }
+    if ((int64_t)E * 2 >= (int64_t)INT32_MAX) { set_error("too many edges for int32 CSR"); return false; }
+    if (variant < 0 || variant > 3) { set_error("unknown variant"); return false; }
+    G->N = N; G->E = E; G->variant = variant; G->weighted = W != nullptr;
+    if (!G->stream) DH_CHECK(hipStreamCreateWithFlags(&G->stream, hipStreamNonBlocking));
+    hipStream_t s = G->stream;
+
+    if (!dalloc(&G->U, E, "U") || !dalloc(&G->V, E, "V") || !dalloc(&G->ew, E, "edge_weight") ||
+        !dalloc(&G->nw, (size_t)N, "NW") || !dalloc(&G->c4, E, "self term") ||
+        !dalloc(&G->pos, E, "half-edge positions") || !dalloc(&G->row_off, (size_t)N + 1, "row offsets"))
+        return false;
+    if (E > 0) {
+        DH_CHECK(hipMemcpyAsync(G->U, U, (size_t)E * sizeof(int), hipMemcpyHostToDevice, s));
+        DH_CHECK(hipMemcpyAsync(G->V, V, (size_t)E * sizeof(int), hipMemcpyHostToDevice, s));
+    }
+    double *dW = nullptr;
+    if (W && E > 0) {
+        if (!dalloc(&dW, E, "W")) return false;
+        DH_CHECK(hipMemcpyAsync(dW, W, (size_t)E * sizeof(double), hipMemcpyHostToDevice, s));
+    }
+    if (NW && N > 0) DH_CHECK(hipMemcpyAsync(G->nw, NW, (size_t)N * sizeof(double), hipMemcpyHostToDevice, s));
+    else if (N > 0) hipLaunchKernelGGL(k_fill_f64, dim3(grid_for(N, 4096)), dim3(TPB), 0, s, G->nw, (int64_t)N, 1.0);
+
+    // raw out-adjacency, only needed to find reciprocal edges (DIRECTED)
+    int64_t *roff = nullptr;
+    uint64_t *rkey = nullptr;
+    int *rval = nullptr;
+    int *deg = nullptr;
+    int64_t *deg64 = nullptr;
+    if (!dalloc(&deg, (size_t)N + 1, "degrees") || !dalloc(&deg64, (size_t)N + 1, "degrees")) return false;
+    bool ok = true;
+    if (variant == DRESS_VARIANT_DIRECTED && E > 0) {
+        uint64_t *k0 = nullptr;
+        int *v0 = nullptr;
+        ok = dalloc(&k0, E, "raw keys") && dalloc(&v0, E, "raw vals") && dalloc(&rkey, E, "raw keys") &&
+             dalloc(&rval, E, "raw vals") && dalloc(&roff, (size_t)N + 1, "raw offsets");
+        if (ok) {
+            hipLaunchKernelGGL(k_fill_i32, dim3(grid_for(N + 1, 4096)), dim3(TPB), 0, s, deg, (int64_t)N + 1, 0);
+            hipLaunchKernelGGL(k_raw_keys, dim3(grid_for(E, 8192)), dim3(TPB), 0, s, G->U, G->V, E, k0, v0, deg);
+            ok = prim_sort_pairs_u64_i32(k0, rkey, v0, rval, (size_t)E, 64, s) &&
+                 prim_exclusive_scan_i32_to_i64(deg, roff, (size_t)N + 1, s);
+        }
+        dfree(k0); dfree(v0);
+        if (!ok) { dfree(rkey); dfree(rval); dfree(roff); dfree(deg); dfree(deg64); dfree(dW); return false; }
+    }
+
+    // variant half-edges
+    uint64_t *hk0 = nullptr, *hk1 = nullptr;
+    int *hv0 = nullptr, *hv1 = nullptr;
+    int64_t slots = 2 * (int64_t)E;
+    ok = dalloc(&hk0, (size_t)slots + 1, "half keys") && dalloc(&hk1, (size_t)slots + 1, "half keys") &&
+         dalloc(&hv0, (size_t)slots + 1, "half vals") && dalloc(&hv1, (size_t)slots + 1, "half vals");
+    if (ok) {
+        hipLaunchKernelGGL(k_fill_i32, dim3(grid_for(N + 1, 4096)), dim3(TPB), 0, s, deg, (int64_t)N + 1, 0);
+        if (E > 0)
+            hipLaunchKernelGGL(k_half_keys, dim3(grid_for(E, 8192)), dim3(TPB), 0, s, G->U, G->V,
+                               (const double *)dW, E, variant, (const int64_t *)roff,
+                               (const uint64_t *)rkey, (const int *)rval, hk0, hv0, G->ew, deg);
+        ok = prim_sort_pairs_u64_i32(hk0, hk1, hv0, hv1, (size_t)slots, 64, s) &&
+             prim_exclusive_scan_i32_to_i64(deg, deg64, (size_t)N + 1, s);
+    }
+    if (ok) {
+        int64_t S = 0;
+        hipError_t e1 = hipMemcpyAsync(&S, deg64 + N, sizeof(int64_t), hipMemcpyDeviceToHost, s);
+        hipError_t e2 = hipStreamSynchronize(s);
+        if (e1 != hipSuccess || e2 != hipSuccess) { set_error("half-edge count download failed"); ok = false; }
+        G->S = S;
+        if (ok && (!dalloc(&G->adj_x, (size_t)S + 1, "adj_target") || !dalloc(&G->adj_e, (size_t)S + 1, "adj_edge_idx")))
+            ok = false;
+        if (ok) {
+            hipLaunchKernelGGL(k_i64_to_i32, dim3(grid_for(N + 1, 4096)), dim3(TPB), 0, s, deg64, G->row_off, (int64_t)N + 1);
+            hipLaunchKernelGGL(k_fill_i32, dim3(grid_for(2 * (int64_t)E, 8192)), dim3(TPB), 0, s,
+                               (int *)G->pos, 2 * (int64_t)E, -1);
+            if (S > 0)
+                hipLaunchKernelGGL(k_unpack_half, dim3(grid_for(S, 8192)), dim3(TPB), 0, s, hk1, hv1, S,
+                                   G->adj_x, G->adj_e, G->pos);
+            if (E > 0)
+                hipLaunchKernelGGL(k_edge_consts, dim3(grid_for(E, 8192)), dim3(TPB), 0, s, G->U, G->V,
+                                   G->nw, E, variant, G->c4);
+            int *dmax = deg;  // reuse: degrees no longer needed
+            hipLaunchKernelGGL(k_fill_i32, dim3(1), dim3(64), 0, s, dmax, (int64_t)1, 0);
+            if (N > 0)
+                hipLaunchKernelGGL(k_max_deg, dim3(grid_for(N, 2048)), dim3(TPB), 0, s, G->row_off, N, dmax);
+            int md = 0;
+            hipError_t e3 = hipMemcpyAsync(&md, dmax, sizeof(int), hipMemcpyDeviceToHost, s);
+            hipError_t e4 = hipStreamSynchronize(s);
+            if (e3 != hipSuccess || e4 != hipSuccess) { set_error("graph build failed: " + std::string(hipGetErrorString(e4))); ok = false; }
+            G->max_degree = md;
+        }
+    }
+    dfree(hk0); dfree(hk1); dfree(hv0); dfree(hv1);
+    dfree(rkey); dfree(rval); dfree(roff); dfree(deg); dfree(deg64); dfree(dW);
+    return ok;
+}
+
+bool build_index(DevGraph *G)
+{
+    if (G->have_index) return true;
+    hipStream_t s = G->stream;
+    int E = G->E;
+    int *cnt = nullptr;
+    int64_t *cnt64 = nullptr;
+    if (!dalloc(&cnt, (size_t)E + 1, "isect counts") || !dalloc(&G->ioff, (size_t)E + 1, "isect offsets")) {
+        dfree(cnt);
+        return false;
+    }
+    const int GS = 16;
+    int grid = grid_for((int64_t)E * GS, num_cus() * 32);
+    hipLaunchKernelGGL(k_fill_i32, dim3(grid_for(E + 1, 4096)), dim3(TPB), 0, s, cnt, (int64_t)E + 1, 0);
+    if (E > 0)
+        hipLaunchKernelGGL((k_isect<GS, false>), dim3(grid), dim3(TPB), 0, s, G->U, G->V, E, G->row_off,
+                           G->adj_x, cnt, (const int64_t *)nullptr, (int2 *)nullptr);
+    bool ok = prim_exclusive_scan_i32_to_i64(cnt, G->ioff, (size_t)E + 1, s);
+    dfree(cnt);
+    (void)cnt64;
+    if (!ok) return false;
+    int64_t T = 0;
+    DH_CHECK(hipMemcpyAsync(&T, G->ioff + E, sizeof(int64_t), hipMemcpyDeviceToHost, s));
+    DH_CHECK(hipStreamSynchronize(s));
+    G->T = T;
+    if (!dalloc(&G->ipair, (size_t)T + 1, "intersection index")) return false;
+    if (E > 0 && T > 0)
+        hipLaunchKernelGGL((k_isect<GS, true>), dim3(grid), dim3(TPB), 0, s, G->U, G->V, E, G->row_off,
+                           G->adj_x, (int *)nullptr, (const int64_t *)G->ioff, G->ipair);
+    DH_CHECK(hipGetLastError());
+    DH_CHECK(hipStreamSynchronize(s));
+    G->have_index = true;
+    return true;
+}
+
+bool build_bins(DevGraph *G)
+{
+    if (G->have_bins) return true;
+    if (!build_index(G)) return false;
+    if (!make_bins(G, &G->vbins, G->N, true)) return false;
+    if (!make_bins(G, &G->ebins, G->E, false)) return false;
+    G->have_bins = true;
+    return true;
+}
+
+bool download_csr(DevGraph *G, int *adj_offset, int *adj_target, int *adj_edge_idx,
+                  double *edge_weight)
+{
+    hipStream_t s = G->stream;
+    DH_CHECK(hipMemcpyAsync(adj_offset, G->row_off, ((size_t)G->N + 1) * sizeof(int), hipMemcpyDeviceToHost, s));
+    if (G->S > 0) {
+        DH_CHECK(hipMemcpyAsync(adj_target, G->adj_x, (size_t)G->S * sizeof(int), hipMemcpyDeviceToHost, s));
+        DH_CHECK(hipMemcpyAsync(adj_edge_idx, G->adj_e, (size_t)G->S * sizeof(int), hipMemcpyDeviceToHost, s));
+    }
+    if (G->E > 0)
+        DH_CHECK(hipMemcpyAsync(edge_weight, G->ew, (size_t)G->E * sizeof(double), hipMemcpyDeviceToHost, s));
+    DH_CHECK(hipStreamSynchronize(s));
+    return true;
+}
+
+namespace {
+__global__ void k_index_to_edges(const int2 *ipair, int64_t T, const int *adj_e, int *ux, int *vx)
+{
+    for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < T;
+         i += (int64_t)gridDim.x * blockDim.x) {
+        int2 p = ipair[i];
+        ux[i] = adj_e[p.x];
+        vx[i] = adj_e[p.y];
+    }
+}
+}  // namespace
+
+// Intercept arrays in the reference's form (edge ids, int offsets), malloc'd.
+bool download_index(DevGraph *G, int **ioff_out, int **ux_out, int **vx_out)
+{
+    if (!build_index(G)) return false;
+    if (G->T >= (int64_t)INT32_MAX) { set_error("intersection index exceeds int32 offsets of dress_graph_t"); return false; }
+    hipStream_t s = G->stream;
+    int E = G->E;
+    int64_t T = G->T;
+    int *io = (int *)malloc(((size_t)E + 1) * sizeof(int));
+    int *ux = (int *)malloc(((size_t)T + 1) * sizeof(int));
+    int *vx = (int *)malloc(((size_t)T + 1) * sizeof(int));
+    int64_t *io64 = (int64_t *)malloc(((size_t)E + 1) * sizeof(int64_t));
+    int *dux = nullptr, *dvx = nullptr;
+    bool ok = io && ux && vx && io64 && dalloc(&dux, (size_t)T + 1, "ux") && dalloc(&dvx, (size_t)T + 1, "vx");
+    if (!io || !ux || !vx || !io64) set_error("host allocation failed for intercept arrays");
+    if (ok) {
+        if (T > 0)
+            hipLaunchKernelGGL(k_index_to_edges, dim3(grid_for(T, 8192)), dim3(TPB), 0, s, G->ipair, T,
+                               G->adj_e, dux, dvx);
+        hipError_t e1 = hipMemcpyAsync(io64, G->ioff, ((size_t)E + 1) * sizeof(int64_t), hipMemcpyDeviceToHost, s);
+        hipError_t e2 = T > 0 ? hipMemcpyAsync(ux, dux, (size_t)T * sizeof(int), hipMemcpyDeviceToHost, s) : hipSuccess;
+        hipError_t e3 = T > 0 ? hipMemcpyAsync(vx, dvx, (size_t)T * sizeof(int), hipMemcpyDeviceToHost, s) : hipSuccess;
+        hipError_t e4 = hipStreamSynchronize(s);
+        if (e1 != hipSuccess || e2 != hipSuccess || e3 != hipSuccess || e4 != hipSuccess) {
+            set_error("intercept download failed");
+            ok = false;
+        }
+    }
+    if (ok) for (int e = 0; e <= E; e++) io[e] = (int)io64[e];
+    free(io64);
+    dfree(dux); dfree(dvx);
+    if (!ok) { free(io); free(ux); free(vx); return false; }
+    *ioff_out = io; *ux_out = ux; *vx_out = vx;
+    return true;
+}
+
+void free_graph(DevGraph *G)
+{
+    if (!G) return;
+    if (G->stream) hipStreamSynchronize(G->stream);
+    dfree(G->U); dfree(G->V); dfree(G->row_off); dfree(G->adj_x); dfree(G->adj_e);
+    dfree(G->ew); dfree(G->nw); dfree(G->c4); dfree(G->pos); dfree(G->ioff); dfree(G->ipair);
+    dfree(G->vbins.list); dfree(G->ebins.list);
+    for (int i = 0; i < 2; i++) { dfree(G->d[i]); dfree(G->wh[i]); dfree(G->d32[i]); dfree(G->wh32[i]); }
+    dfree(G->vd); dfree(G->state);
+    if (G->stream) hipStreamDestroy(G->stream);
+    delete G;
+}
+
+}  // namespace dh

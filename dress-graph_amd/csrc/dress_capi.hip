@@ -1,0 +1,370 @@
+// dress_capi.hip -- the C ABI of libdress_hip.so: libdress's core API
+// (libdress/include/dress/dress.h:80-107) and the dress_fit_hip entry point
+// (counterpart of dress_fit_cuda, libdress/src/cuda/dress_cuda.cu:445-707).
+//
+// Device state of a graph lives in a side table keyed by the dress_graph_t
+// pointer, so the struct layout the bindings read stays untouched.
+#include "dress_hip_internal.h"
+#include "dress_capi_internal.h"
+
+#include <cmath>
+#include <cstdio>
+#include <cstdlib>
+#include <cstring>
+#include <mutex>
+#include <unordered_map>
+
+#include "dress/hip/dress_hip.h"
+
+namespace dh {
+
+// ---------------------------------------------------------------- errors
+static thread_local std::string t_err;
+static thread_local dress_hip_fit_stats_t t_stats;
+
+void set_error(const std::string &msg)
+{
+    if (t_err.empty()) t_err = msg;      // keep the first (root) cause
+    if (getenv("DRESS_HIP_QUIET") == nullptr) fprintf(stderr, "libdress_hip: %s\n", msg.c_str());
+}
+void clear_error() { t_err.clear(); }
+bool has_error() { return !t_err.empty(); }
+
+bool dmalloc(void **p, size_t bytes, const char *what)
+{
+    *p = nullptr;
+    hipError_t e = hipMalloc(p, bytes ? bytes : 16);
+    if (e != hipSuccess) {
+        *p = nullptr;
+        hipGetLastError();
+        set_error(std::string("device allocation of ") + std::to_string(bytes) + " bytes for " + what +
+                  " failed: " + hipGetErrorString(e));
+        return false;
+    }
+    return true;
+}
+
+void dfree(void *p)
+{
+    if (p) hipFree(p);
+}
+
+int current_device()
+{
+    int d = 0;
+    hipGetDevice(&d);
+    return d;
+}
+
+int num_cus()
+{
+    static int cus[64] = {0};
+    int d = current_device();
+    if (d < 0 || d >= 64) d = 0;
+    if (!cus[d]) {
+        hipDeviceProp_t p;
+        if (hipGetDeviceProperties(&p, d) == hipSuccess) cus[d] = p.multiProcessorCount;
+        if (cus[d] <= 0) cus[d] = 256;
+    }
+    return cus[d];
+}
+
+// ---------------------------------------------------------------- side table
+static std::mutex g_mu;
+static std::unordered_map<const dress_graph_t *, DevGraph *> g_table;
+static int g_precision = -1;
+
+int default_precision()
+{
+    std::lock_guard<std::mutex> lk(g_mu);
+    if (g_precision < 0) {
+        const char *e = getenv("DRESS_HIP_PRECISION");
+        g_precision = (e && (!strcmp(e, "mixed") || !strcmp(e, "fp32_fp64"))) ? PREC_FP32_FP64 : PREC_FP64;
+    }
+    return g_precision;
+}
+
+DevGraph *lookup(const dress_graph_t *g)
+{
+    std::lock_guard<std::mutex> lk(g_mu);
+    auto it = g_table.find(g);
+    return it == g_table.end() ? nullptr : it->second;
+}
+
+static void table_put(const dress_graph_t *g, DevGraph *G)
+{
+    std::lock_guard<std::mutex> lk(g_mu);
+    g_table[g] = G;
+}
+
+static DevGraph *table_take(const dress_graph_t *g)
+{
+    std::lock_guard<std::mutex> lk(g_mu);
+    auto it = g_table.find(g);
+    if (it == g_table.end()) return nullptr;
+    DevGraph *G = it->second;
+    g_table.erase(it);
+    return G;
+}
+
+// Device graph for g: the one dress_init_graph built, or one built now from
+// g's edge list (graphs constructed by another libdress build).
+DevGraph *device_graph(p_dress_graph_t g)
+{
+    DevGraph *G = lookup(g);
+    if (G) return G;
+    G = new DevGraph();
+    if (!build_graph(G, g->N, g->E, g->U, g->V, g->W, g->NW, (int)g->variant)) {
+        free_graph(G);
+        return nullptr;
+    }
+    table_put(g, G);
+    return G;
+}
+
+void record_stats(DevGraph *G, const FitResult &r)
+{
+    t_stats.iterations_executed = r.executed;
+    t_stats.fp32_iterations = r.fp32_iterations;
+    t_stats.fp64_reruns = r.reruns;
+    t_stats.device_ms = r.ms;
+    t_stats.intersections = G->T;
+    t_stats.half_edges = G->S;
+}
+
+}  // namespace dh
+
+using namespace dh;
+
+namespace {
+
+__global__ void k_self_consts(const int *U, const int *V, const double *nw, int E, int directed,
+                              double *c4)
+{
+    for (int e = blockIdx.x * blockDim.x + threadIdx.x; e < E; e += gridDim.x * blockDim.x) {
+        double a = 4.0 * nw[U[e]];
+        c4[e] = directed ? a : a + 4.0 * nw[V[e]];
+    }
+}
+
+// refresh vertex weights from the host struct (the caller may edit g->NW)
+bool refresh_vertex_weights(DevGraph *G, p_dress_graph_t g)
+{
+    if (!g->NW || G->N == 0) return true;
+    DH_CHECK(hipMemcpyAsync(G->nw, g->NW, (size_t)G->N * sizeof(double), hipMemcpyHostToDevice, G->stream));
+    int directed = (G->variant == DRESS_VARIANT_FORWARD || G->variant == DRESS_VARIANT_BACKWARD);
+    if (G->E > 0) {
+        int grid = (int)std::min<int64_t>(((int64_t)G->E + 255) / 256, 8192);
+        hipLaunchKernelGGL(k_self_consts, dim3(grid), dim3(256), 0, G->stream, G->U, G->V, G->nw, G->E,
+                           directed, G->c4);
+    }
+    DH_CHECK(hipGetLastError());
+    return true;
+}
+
+int fit_impl(p_dress_graph_t g, int max_iterations, double epsilon, int precision, int *iterations,
+             double *delta)
+{
+    clear_error();
+    if (!g) { set_error("dress_fit_hip: NULL graph"); return -1; }
+    if (max_iterations <= 0) {
+        // dress.c:607: no iteration runs; *delta is left untouched
+        if (iterations) *iterations = max_iterations;
+        return 0;
+    }
+    DevGraph *G = device_graph(g);
+    if (!G) return -1;
+    FitResult r;
+    if (!refresh_vertex_weights(G, g) || !fit_upload_values(G, g->edge_dress) ||
+        !fit_run(G, max_iterations, epsilon, precision, &r) ||
+        !fit_download(G, g->edge_dress, g->vertex_dress))
+        return -1;
+    record_stats(G, r);
+    if (iterations) *iterations = r.iterations;
+    if (delta) *delta = r.delta;
+    return 0;
+}
+
+}  // namespace
+
+extern "C" {
+
+const char *dress_hip_last_error(void) { return t_err.c_str(); }
+
+int dress_hip_device_count(void)
+{
+    int n = 0;
+    if (hipGetDeviceCount(&n) != hipSuccess) {
+        hipGetLastError();
+        return 0;
+    }
+    return n;
+}
+
+void dress_hip_set_precision(dress_hip_precision_t p)
+{
+    default_precision();
+    std::lock_guard<std::mutex> lk(g_mu);
+    g_precision = (p == DRESS_HIP_FP32_FP64) ? PREC_FP32_FP64 : PREC_FP64;
+}
+
+dress_hip_precision_t dress_hip_get_precision(void)
+{
+    return default_precision() == PREC_FP32_FP64 ? DRESS_HIP_FP32_FP64 : DRESS_HIP_FP64;
+}
+
+void dress_hip_last_fit_stats(dress_hip_fit_stats_t *out)
+{
+    if (out) *out = t_stats;
+}
+
+// dress.h:80-83 / dress.c:378-427.  The CSR and (when requested) the
+// intercept arrays are built on the device and mirrored into the struct.
+p_dress_graph_t dress_init_graph(int N, int E, int *U, int *V, double *W, double *NW,
+                                 dress_variant_t variant, int precompute_intercepts)
+{
+    clear_error();
+    p_dress_graph_t g = (p_dress_graph_t)calloc(1, sizeof(dress_graph_t));
+    if (!g) { set_error("host allocation of dress_graph_t failed"); return nullptr; }
+    g->variant = variant;
+    g->N = N; g->E = E; g->U = U; g->V = V; g->W = W;
+    g->precompute_intercepts = precompute_intercepts;
+    if (NW) g->NW = NW;
+    else {
+        g->NW = (double *)malloc(((size_t)N + 1) * sizeof(double));
+        if (g->NW) for (int i = 0; i < N; i++) g->NW[i] = 1.0;
+    }
+    g->vertex_dress = (double *)calloc((size_t)N + 1, sizeof(double));
+    g->edge_dress = (double *)malloc(((size_t)E + 1) * sizeof(double));
+    g->edge_dress_next = (double *)malloc(((size_t)E + 1) * sizeof(double));
+    g->adj_offset = (int *)malloc(((size_t)N + 1) * sizeof(int));
+    g->edge_weight = (double *)malloc(((size_t)E + 1) * sizeof(double));
+    DevGraph *G = new DevGraph();
+    bool ok = g->NW && g->vertex_dress && g->edge_dress && g->edge_dress_next && g->adj_offset &&
+              g->edge_weight;
+    if (!ok) set_error("host allocation failed in dress_init_graph");
+    ok = ok && build_graph(G, N, E, U, V, W, g->NW, (int)variant);
+    if (ok) {
+        g->adj_target = (int *)malloc(((size_t)G->S + 1) * sizeof(int));
+        g->adj_edge_idx = (int *)malloc(((size_t)G->S + 1) * sizeof(int));
+        ok = g->adj_target && g->adj_edge_idx;
+        if (!ok) set_error("host allocation failed in dress_init_graph");
+    }
+    ok = ok && download_csr(G, g->adj_offset, g->adj_target, g->adj_edge_idx, g->edge_weight);
+    if (ok) {
+        g->max_degree = G->max_degree;
+        for (int e = 0; e < E; e++) g->edge_dress[e] = g->edge_dress_next[e] = 1.0;
+        if (precompute_intercepts)
+            ok = download_index(G, &g->intercept_offset, &g->intercept_edge_ux, &g->intercept_edge_vx);
+    }
+    if (!ok) {
+        free_graph(G);
+        free(g->adj_offset); free(g->adj_target); free(g->adj_edge_idx); free(g->edge_weight);
+        free(g->edge_dress); free(g->edge_dress_next); free(g->vertex_dress);
+        if (!NW) free(g->NW);
+        free(g);
+        return nullptr;
+    }
+    table_put(g, G);
+    return g;
+}
+
+// dress.h:93 / dress.c:615-643, plus the device state
+void dress_free_graph(p_dress_graph_t g)
+{
+    if (!g) return;
+    DevGraph *G = table_take(g);
+    if (G) free_graph(G);
+    free(g->adj_offset); free(g->adj_target); free(g->adj_edge_idx);
+    free(g->W); free(g->edge_weight); free(g->edge_dress); free(g->edge_dress_next);
+    free(g->vertex_dress); free(g->NW);
+    if (g->precompute_intercepts) {
+        free(g->intercept_offset); free(g->intercept_edge_ux); free(g->intercept_edge_vx);
+    }
+    free(g->U); free(g->V);
+    free(g);
+}
+
+void dress_fit_hip(p_dress_graph_t g, int max_iterations, double epsilon, int *iterations,
+                   double *delta)
+{
+    if (fit_impl(g, max_iterations, epsilon, default_precision(), iterations, delta) != 0) {
+        if (iterations) *iterations = -1;
+        if (delta) *delta = NAN;
+    }
+}
+
+// In libdress_hip.so the plain name runs on the GPU as well.
+void dress_fit(p_dress_graph_t g, int max_iterations, double epsilon, int *iterations, double *delta)
+{
+    dress_fit_hip(g, max_iterations, epsilon, iterations, delta);
+}
+
+int dress_fit_hip_ex(p_dress_graph_t g, int max_iterations, double epsilon,
+                     dress_hip_precision_t precision, int *iterations, double *delta)
+{
+    int r = fit_impl(g, max_iterations, epsilon,
+                     precision == DRESS_HIP_FP32_FP64 ? PREC_FP32_FP64 : PREC_FP64, iterations, delta);
+    if (r != 0) {
+        if (iterations) *iterations = -1;
+        if (delta) *delta = NAN;
+    }
+    return r;
+}
+
+int dress_hip_resident_reset(p_dress_graph_t g)
+{
+    clear_error();
+    if (!g) { set_error("NULL graph"); return -1; }
+    DevGraph *G = device_graph(g);
+    if (!G) return -1;
+    if (!refresh_vertex_weights(G, g) || !fit_upload_values(G, g->edge_dress) || !build_bins(G)) return -1;
+    if (hipStreamSynchronize(G->stream) != hipSuccess) { set_error("stream sync failed"); return -1; }
+    return 0;
+}
+
+int dress_hip_fit_resident(p_dress_graph_t g, int max_iterations, double epsilon,
+                           dress_hip_precision_t precision, int *iterations, double *delta)
+{
+    clear_error();
+    DevGraph *G = g ? lookup(g) : nullptr;
+    if (!G || !G->d[0]) { set_error("dress_hip_fit_resident: call dress_hip_resident_reset first"); return -1; }
+    FitResult r;
+    if (!fit_run(G, max_iterations, epsilon,
+                 precision == DRESS_HIP_FP32_FP64 ? PREC_FP32_FP64 : PREC_FP64, &r))
+        return -1;
+    record_stats(G, r);
+    if (iterations) *iterations = r.iterations;
+    if (delta) *delta = r.delta;
+    return 0;
+}
+
+int dress_hip_resident_download(p_dress_graph_t g)
+{
+    clear_error();
+    DevGraph *G = g ? lookup(g) : nullptr;
+    if (!G || !G->d[0]) { set_error("dress_hip_resident_download: no resident state"); return -1; }
+    return fit_download(G, g->edge_dress, g->vertex_dress) ? 0 : -1;
+}
+
+int64_t dress_delta_binom(int n, int k)
+{
+    // delta_dress_impl.c:194-204
+    if (k < 0 || k > n) return 0;
+    if (k == 0 || k == n) return 1;
+    if (k > n - k) k = n - k;
+    int64_t r = 1;
+    for (int i = 0; i < k; i++) r = r * (n - i) / (i + 1);
+    return r;
+}
+
+int64_t dress_nabla_perm_count(int n, int k)
+{
+    // nabla_dress_impl.c:111-118
+    if (k < 0 || k > n) return 0;
+    int64_t r = 1;
+    for (int i = 0; i < k; i++) r *= (n - i);
+    return r;
+}
+
+}  // extern "C"

@@ -1,0 +1,563 @@
+// dress_fit.hip -- the DRESS fixed-point iteration on one device-resident graph.
+//
+// One iteration (libdress/src/dress.c:546-608) is three kernels:
+//   vertex phase   vd[u] = sqrt( sum{4nw[u]} + sum_{i in row u} wh[i] )
+//   edge phase     d'[e] = ( sum_k (wh[a_k] + wh[b_k]) + t0_e ) / (vd[U]*vd[V])
+//                  t0_e = c4[e] + 2*ew[e]*d[e]   (c4[e] + ew[e]*d[e] FWD/BWD)
+//                  with max|d - d'| folded into one device word, and
+//                  wh'[pos(e)] = ew[e]*d'[e] scattered to the edge's half-edges
+//   end            convergence bookkeeping on the device (strict < epsilon)
+// so the host only checks the state every few iterations.
+//
+// Sums are exact-rounding compensated sums (TwoSum per term, pairwise
+// TwoSum merges across lanes): the result is the correctly rounded sum of
+// the terms, whatever order the lanes add them in.  The reference's sort +
+// Kahan-Babuska-Neumaier sum (dress.c:51-66) produces the same correctly
+// rounded value, so values match it bit for bit (except where an exact sum
+// lies within ~n^2 u^2 of a rounding midpoint), and they are invariant
+// under relabelling like the reference's.  All FP64 code is compiled with
+// -ffp-contract=off: the reference computes w*d + w*d with two roundings.
+//
+// Work is degree binned: rows/edges with <= 4, 32, 256, 4096 terms use groups
+// of 1, 4, 16, 64 lanes; larger ones a 256-lane workgroup.
+#include "dress_hip_internal.h"
+
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+
+namespace dh {
+
+struct FitState {
+    int iter;                // iterations finished
+    int done;
+    int iterations;          // reference convention result
+    int cur;                 // buffer index holding the current values
+    double delta;            // max change of the last finished iteration
+    unsigned long long maxd; // running max |change| (bit pattern, >= 0)
+    int fp32;                // 1 while values live in the FP32 buffers,
+                             // 2 during the iteration that converts them
+    int switched_at;         // iteration at which FP32 -> FP64 happened
+    double switch_thr;       // hand over once the FP32 change is below this
+    int switch_by;           // ... or at this iteration at the latest
+    int mixed;               // fit started in FP32
+    int ambiguous;           // an FP64-phase change fell within 0.1% of
+                             // epsilon: the stop step may differ from a
+                             // pure FP64 run, so the host reruns in FP64
+};
+
+namespace {
+
+constexpr int TPB = 256;
+
+__device__ __forceinline__ void two_sum_add(double &s, double &c, double x)
+{
+    double t = s + x;
+    double z = t - s;
+    double e = (s - (t - z)) + (x - z);
+    s = t;
+    c += e;
+}
+
+__device__ __forceinline__ void two_sum_merge(double &s, double &c, double s2, double c2)
+{
+    double t = s + s2;
+    double z = t - s;
+    double e = (s - (t - z)) + (s2 - z);
+    s = t;
+    c = (c + c2) + e;
+}
+
+template <int G>
+__device__ __forceinline__ void group_reduce(double &s, double &c)
+{
+#pragma unroll
+    for (int o = G / 2; o > 0; o >>= 1) {
+        double s2 = __shfl_xor(s, o, G);
+        double c2 = __shfl_xor(c, o, G);
+        two_sum_merge(s, c, s2, c2);
+    }
+}
+
+// workgroup-wide merge of (s, c); result valid in thread 0
+__device__ __forceinline__ void block_reduce(double &s, double &c)
+{
+    __shared__ double ss[TPB / 64], sc[TPB / 64];
+    group_reduce<64>(s, c);
+    int w = threadIdx.x >> 6;
+    if ((threadIdx.x & 63) == 0) { ss[w] = s; sc[w] = c; }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        for (int i = 1; i < TPB / 64; i++) two_sum_merge(s, c, ss[i], sc[i]);
+    }
+    __syncthreads();
+}
+
+__device__ __forceinline__ void wave_max_atomic(double m, unsigned long long *out)
+{
+    for (int o = 32; o > 0; o >>= 1) m = fmax(m, __shfl_xor(m, o));
+    if ((threadIdx.x & 63) == 0 && m > 0.0)
+        atomicMax(out, (unsigned long long)__double_as_longlong(m));
+}
+
+// ------------------------------------------------------------- FP64 phase
+
+template <int G>
+__global__ void __launch_bounds__(TPB) k_vertex(const int *__restrict__ list, int count,
+                                                const int *__restrict__ row_off,
+                                                const double *__restrict__ wh0,
+                                                const double *__restrict__ wh1,
+                                                const double *__restrict__ nw, double *__restrict__ vd,
+                                                const FitState *__restrict__ st)
+{
+    if (st->done || st->fp32) return;
+    const double *__restrict__ wh = st->cur ? wh1 : wh0;
+    const int lane = (G == TPB) ? threadIdx.x : (threadIdx.x % G);
+    const int64_t groups = (int64_t)gridDim.x * (TPB / G);
+    for (int64_t g = (int64_t)blockIdx.x * (TPB / G) + threadIdx.x / G; g < count; g += groups) {
+        int u = list[g];
+        int b = row_off[u], e = row_off[u + 1];
+        double s = 0.0, c = 0.0;
+        if (lane == 0) s = 4.0 * nw[u];
+        for (int i = b + lane; i < e; i += G) two_sum_add(s, c, wh[i]);
+        if (G == TPB) block_reduce(s, c);
+        else group_reduce<G>(s, c);
+        if (lane == 0) vd[u] = sqrt(s + c);
+    }
+}
+
+template <int G>
+__global__ void __launch_bounds__(TPB) k_edge(const int *__restrict__ list, int count,
+                                              const int *__restrict__ U, const int *__restrict__ V,
+                                              const double *__restrict__ ew,
+                                              const double *__restrict__ c4,
+                                              const int2 *__restrict__ pos,
+                                              const int64_t *__restrict__ ioff,
+                                              const int2 *__restrict__ ipair,
+                                              double *d0, double *d1, double *wh0, double *wh1,
+                                              const double *__restrict__ vd, int directed,
+                                              FitState *st)
+{
+    if (st->done || st->fp32) return;
+    const int cur = st->cur;
+    const double *__restrict__ dc = cur ? d1 : d0;
+    const double *__restrict__ wc = cur ? wh1 : wh0;
+    double *__restrict__ dn = cur ? d0 : d1;
+    double *__restrict__ wn = cur ? wh0 : wh1;
+    const int lane = (G == TPB) ? threadIdx.x : (threadIdx.x % G);
+    const int64_t groups = (int64_t)gridDim.x * (TPB / G);
+    double mymax = 0.0;
+    for (int64_t g = (int64_t)blockIdx.x * (TPB / G) + threadIdx.x / G; g < count; g += groups) {
+        int e = list[g];
+        int64_t kb = ioff[e], ke = ioff[e + 1];
+        double s = 0.0, c = 0.0;
+        for (int64_t k = kb + lane; k < ke; k += G) {
+            int2 p = ipair[k];
+            double t = wc[p.x] + wc[p.y];
+            two_sum_add(s, c, t);
+        }
+        double de = 0.0, w = 0.0;
+        if (lane == 0) {
+            de = dc[e];
+            w = ew[e];
+            double own = w * de;
+            double t0 = directed ? (c4[e] + own) : (c4[e] + 2.0 * own);
+            two_sum_add(s, c, t0);
+        }
+        if (G == TPB) block_reduce(s, c);
+        else group_reduce<G>(s, c);
+        if (lane == 0) {
+            double num = s + c;
+            double den = vd[U[e]] * vd[V[e]];
+            double x = den > 0.0 ? num / den : 0.0;
+            mymax = fmax(mymax, fabs(de - x));
+            dn[e] = x;
+            double wx = w * x;
+            int2 q = pos[e];
+            wn[q.x] = wx;
+            if (q.y >= 0) wn[q.y] = wx;
+        }
+    }
+    wave_max_atomic(mymax, &st->maxd);
+}
+
+// ------------------------------------------------------------- FP32 phase
+// Values and half-edge copies stored in FP32 (half the gather bytes); terms
+// are added in FP64, which is exact for FP32 inputs within 2^29 of each
+// other, so the phase stays order independent; norms, division and the
+// change are computed in FP64 and the new value rounded to FP32.
+
+template <int G>
+__global__ void __launch_bounds__(TPB) k_vertex32(const int *__restrict__ list, int count,
+                                                  const int *__restrict__ row_off,
+                                                  const float *__restrict__ wh0,
+                                                  const float *__restrict__ wh1,
+                                                  const double *__restrict__ nw, double *__restrict__ vd,
+                                                  const FitState *__restrict__ st)
+{
+    if (st->done || !st->fp32) return;
+    const float *__restrict__ wh = st->cur ? wh1 : wh0;
+    const int lane = (G == TPB) ? threadIdx.x : (threadIdx.x % G);
+    const int64_t groups = (int64_t)gridDim.x * (TPB / G);
+    for (int64_t g = (int64_t)blockIdx.x * (TPB / G) + threadIdx.x / G; g < count; g += groups) {
+        int u = list[g];
+        int b = row_off[u], e = row_off[u + 1];
+        double s = 0.0, c = 0.0;
+        if (lane == 0) s = 4.0 * nw[u];
+        for (int i = b + lane; i < e; i += G) two_sum_add(s, c, (double)wh[i]);
+        if (G == TPB) block_reduce(s, c);
+        else group_reduce<G>(s, c);
+        if (lane == 0) vd[u] = sqrt(s + c);
+    }
+}
+
+template <int G>
+__global__ void __launch_bounds__(TPB) k_edge32(const int *__restrict__ list, int count,
+                                                const int *__restrict__ U, const int *__restrict__ V,
+                                                const double *__restrict__ ew,
+                                                const double *__restrict__ c4,
+                                                const int2 *__restrict__ pos,
+                                                const int64_t *__restrict__ ioff,
+                                                const int2 *__restrict__ ipair,
+                                                float *d0, float *d1, float *wh0, float *wh1,
+                                                const double *__restrict__ vd, int directed,
+                                                FitState *st)
+{
+    if (st->done || !st->fp32) return;
+    const int cur = st->cur;
+    const float *__restrict__ dc = cur ? d1 : d0;
+    const float *__restrict__ wc = cur ? wh1 : wh0;
+    float *__restrict__ dn = cur ? d0 : d1;
+    float *__restrict__ wn = cur ? wh0 : wh1;
+    const int lane = (G == TPB) ? threadIdx.x : (threadIdx.x % G);
+    const int64_t groups = (int64_t)gridDim.x * (TPB / G);
+    double mymax = 0.0;
+    for (int64_t g = (int64_t)blockIdx.x * (TPB / G) + threadIdx.x / G; g < count; g += groups) {
+        int e = list[g];
+        int64_t kb = ioff[e], ke = ioff[e + 1];
+        double s = 0.0, c = 0.0;
+        for (int64_t k = kb + lane; k < ke; k += G) {
+            int2 p = ipair[k];
+            two_sum_add(s, c, (double)wc[p.x] + (double)wc[p.y]);
+        }
+        double de = 0.0, w = 0.0;
+        if (lane == 0) {
+            de = (double)dc[e];
+            w = ew[e];
+            double own = w * de;
+            double t0 = directed ? (c4[e] + own) : (c4[e] + 2.0 * own);
+            two_sum_add(s, c, t0);
+        }
+        if (G == TPB) block_reduce(s, c);
+        else group_reduce<G>(s, c);
+        if (lane == 0) {
+            double num = s + c;
+            double den = vd[U[e]] * vd[V[e]];
+            double x = den > 0.0 ? num / den : 0.0;
+            mymax = fmax(mymax, fabs(de - x));
+            float xf = (float)x;
+            dn[e] = xf;
+            float wx = (float)(w * x);
+            int2 q = pos[e];
+            wn[q.x] = wx;
+            if (q.y >= 0) wn[q.y] = wx;
+        }
+    }
+    wave_max_atomic(mymax, &st->maxd);
+}
+
+// ------------------------------------------------------------- glue kernels
+
+// wh[cur] = ew * d[cur] on every half-edge
+__global__ void k_init_wh(int E, const double *ew, const int2 *pos, double *d0, double *d1,
+                          double *wh0, double *wh1, const FitState *st)
+{
+    const double *d = st->cur ? d1 : d0;
+    double *wh = st->cur ? wh1 : wh0;
+    for (int e = blockIdx.x * blockDim.x + threadIdx.x; e < E; e += gridDim.x * blockDim.x) {
+        double x = ew[e] * d[e];
+        int2 q = pos[e];
+        wh[q.x] = x;
+        if (q.y >= 0) wh[q.y] = x;
+    }
+}
+
+// FP64 values -> FP32 buffers (start of a mixed-precision fit)
+__global__ void k_to32(int E, const double *ew, const int2 *pos, const double *d0, const double *d1,
+                       float *f0, float *f1, float *w0, float *w1, const FitState *st)
+{
+    const double *d = st->cur ? d1 : d0;
+    float *f = st->cur ? f1 : f0;
+    float *w = st->cur ? w1 : w0;
+    for (int e = blockIdx.x * blockDim.x + threadIdx.x; e < E; e += gridDim.x * blockDim.x) {
+        f[e] = (float)d[e];
+        float x = (float)(ew[e] * d[e]);
+        int2 q = pos[e];
+        w[q.x] = x;
+        if (q.y >= 0) w[q.y] = x;
+    }
+}
+
+// FP32 values -> FP64 buffers once the FP32 phase has converged to its
+// switch threshold; runs only in the iteration where the end kernel flagged
+// the switch (st->fp32 == 2).
+__global__ void k_to64(int E, const double *ew, const int2 *pos, double *d0, double *d1,
+                       double *wh0, double *wh1, const float *f0, const float *f1, FitState *st)
+{
+    if (st->done || st->fp32 != 2) return;
+    const float *f = st->cur ? f1 : f0;
+    double *d = st->cur ? d1 : d0;
+    double *wh = st->cur ? wh1 : wh0;
+    for (int e = blockIdx.x * blockDim.x + threadIdx.x; e < E; e += gridDim.x * blockDim.x) {
+        double x = (double)f[e];
+        d[e] = x;
+        double wx = ew[e] * x;
+        int2 q = pos[e];
+        wh[q.x] = wx;
+        if (q.y >= 0) wh[q.y] = wx;
+    }
+}
+
+__global__ void k_switch_done(FitState *st)
+{
+    if (threadIdx.x == 0 && blockIdx.x == 0 && st->fp32 == 2) st->fp32 = 0;
+}
+
+// convergence bookkeeping for the iteration just finished
+__global__ void k_end(FitState *st, double eps, int max_it)
+{
+    if (threadIdx.x != 0 || blockIdx.x != 0 || st->done) return;
+    double md = __longlong_as_double((long long)st->maxd);
+    st->maxd = 0ull;
+    st->delta = md;
+    st->cur ^= 1;
+    int it = st->iter;
+    st->iter = it + 1;
+    if (st->fp32) {
+        // FP32 phase: never decide convergence on FP32 values; hand over to
+        // FP64 while enough iterations remain to contract the FP32 error
+        if (md < st->switch_thr || it + 1 >= st->switch_by) {
+            st->fp32 = 2;
+            st->switched_at = it + 1;
+        }
+        if (it + 1 >= max_it) { st->done = 1; st->iterations = max_it; st->ambiguous = 1; }
+        return;
+    }
+    if (st->mixed && fabs(md - eps) <= 1e-3 * md) st->ambiguous = 1;
+    if (md < eps) { st->done = 1; st->iterations = it; }
+    else if (it + 1 >= max_it) { st->done = 1; st->iterations = max_it; }
+}
+
+inline int grid_for(int64_t threads, int cap)
+{
+    int64_t b = (threads + TPB - 1) / TPB;
+    if (b < 1) b = 1;
+    if (b > cap) b = cap;
+    return (int)b;
+}
+
+template <int G>
+void launch_vertex(DevGraph *Gr, int bin, int cap, bool fp32)
+{
+    const Bins &B = Gr->vbins;
+    int n = B.count[bin];
+    if (!n) return;
+    int grid = grid_for((int64_t)n * G, cap);
+    FitState *st = (FitState *)Gr->state;
+    if (fp32)
+        hipLaunchKernelGGL((k_vertex32<G>), dim3(grid), dim3(TPB), 0, Gr->stream, B.list + B.start[bin], n,
+                           Gr->row_off, Gr->wh32[0], Gr->wh32[1], Gr->nw, Gr->vd, st);
+    else
+        hipLaunchKernelGGL((k_vertex<G>), dim3(grid), dim3(TPB), 0, Gr->stream, B.list + B.start[bin], n,
+                           Gr->row_off, Gr->wh[0], Gr->wh[1], Gr->nw, Gr->vd, st);
+}
+
+template <int G>
+void launch_edge(DevGraph *Gr, int bin, int cap, bool fp32)
+{
+    const Bins &B = Gr->ebins;
+    int n = B.count[bin];
+    if (!n) return;
+    int grid = grid_for((int64_t)n * G, cap);
+    FitState *st = (FitState *)Gr->state;
+    int directed = (Gr->variant == DRESS_VARIANT_FORWARD || Gr->variant == DRESS_VARIANT_BACKWARD);
+    if (fp32)
+        hipLaunchKernelGGL((k_edge32<G>), dim3(grid), dim3(TPB), 0, Gr->stream, B.list + B.start[bin], n,
+                           Gr->U, Gr->V, Gr->ew, Gr->c4, Gr->pos, Gr->ioff, Gr->ipair,
+                           Gr->d32[0], Gr->d32[1], Gr->wh32[0], Gr->wh32[1], Gr->vd, directed, st);
+    else
+        hipLaunchKernelGGL((k_edge<G>), dim3(grid), dim3(TPB), 0, Gr->stream, B.list + B.start[bin], n,
+                           Gr->U, Gr->V, Gr->ew, Gr->c4, Gr->pos, Gr->ioff, Gr->ipair,
+                           Gr->d[0], Gr->d[1], Gr->wh[0], Gr->wh[1], Gr->vd, directed, st);
+}
+
+void launch_phase(DevGraph *Gr, bool fp32)
+{
+    int cap = num_cus() * 16;
+    launch_vertex<1>(Gr, 0, cap, fp32);
+    launch_vertex<4>(Gr, 1, cap, fp32);
+    launch_vertex<16>(Gr, 2, cap, fp32);
+    launch_vertex<64>(Gr, 3, cap, fp32);
+    launch_vertex<TPB>(Gr, 4, cap, fp32);
+    launch_edge<1>(Gr, 0, cap, fp32);
+    launch_edge<4>(Gr, 1, cap, fp32);
+    launch_edge<16>(Gr, 2, cap, fp32);
+    launch_edge<64>(Gr, 3, cap, fp32);
+    launch_edge<TPB>(Gr, 4, cap, fp32);
+}
+
+}  // namespace
+
+bool fit_alloc_state(DevGraph *G, bool fp32)
+{
+    if (!G->state && !dmalloc(&G->state, sizeof(FitState), "fit state")) return false;
+    for (int i = 0; i < 2; i++) {
+        if (!G->d[i] && !dalloc(&G->d[i], (size_t)G->E + 1, "edge values")) return false;
+        if (!G->wh[i] && !dalloc(&G->wh[i], (size_t)G->S + 1, "half-edge values")) return false;
+    }
+    if (!G->vd && !dalloc(&G->vd, (size_t)G->N + 1, "vertex norms")) return false;
+    if (fp32) {
+        for (int i = 0; i < 2; i++) {
+            if (!G->d32[i] && !dalloc(&G->d32[i], (size_t)G->E + 1, "edge values fp32")) return false;
+            if (!G->wh32[i] && !dalloc(&G->wh32[i], (size_t)G->S + 1, "half-edge values fp32")) return false;
+        }
+    }
+    return true;
+}
+
+bool fit_upload_values(DevGraph *G, const double *edge_dress)
+{
+    if (!fit_alloc_state(G, false)) return false;
+    G->cur = 0;
+    if (G->E > 0)
+        DH_CHECK(hipMemcpyAsync(G->d[0], edge_dress, (size_t)G->E * sizeof(double), hipMemcpyHostToDevice, G->stream));
+    return true;
+}
+
+namespace {
+
+bool fit_run_once(DevGraph *G, int max_iterations, double epsilon, bool mixed, FitResult *res,
+                  bool *ambiguous)
+{
+    hipStream_t s = G->stream;
+    if (!fit_alloc_state(G, mixed)) return false;
+    FitState h;
+    memset(&h, 0, sizeof(h));
+    h.cur = G->cur;
+    h.fp32 = mixed ? 1 : 0;
+    h.mixed = mixed ? 1 : 0;
+    // The FP32 phase carries a relative error of ~1.5e-7 (u32 / (1 - rho)).
+    // The FP64 phase contracts that error and the change at the same rate,
+    // so handing over at change ~ 3000 * epsilon leaves the error below
+    // 1.5e-7 * epsilon / (3000 epsilon) = 5e-11 when the change reaches
+    // epsilon.  At least 16 FP64 iterations are kept before max_iterations.
+    h.switch_thr = 3000.0 * epsilon;
+    h.switch_by = max_iterations - 16;
+    FitState *st = (FitState *)G->state;
+    DH_CHECK(hipMemcpyAsync(st, &h, sizeof(h), hipMemcpyHostToDevice, s));
+    hipEvent_t ev0, ev1;
+    DH_CHECK(hipEventCreate(&ev0));
+    DH_CHECK(hipEventCreate(&ev1));
+    int egrid = grid_for(G->E, num_cus() * 8);
+    DH_CHECK(hipEventRecord(ev0, s));
+    if (G->E > 0)
+        hipLaunchKernelGGL(k_init_wh, dim3(egrid), dim3(TPB), 0, s, G->E, G->ew, G->pos, G->d[0], G->d[1],
+                           G->wh[0], G->wh[1], st);
+    if (mixed && G->E > 0)
+        hipLaunchKernelGGL(k_to32, dim3(egrid), dim3(TPB), 0, s, G->E, G->ew, G->pos, G->d[0], G->d[1],
+                           G->d32[0], G->d32[1], G->wh32[0], G->wh32[1], st);
+    int launched = 0;
+    int chunk = 4;
+    while (launched < max_iterations) {
+        int n = std::min(chunk, max_iterations - launched);
+        for (int j = 0; j < n; j++) {
+            if (mixed) launch_phase(G, true);
+            launch_phase(G, false);
+            hipLaunchKernelGGL(k_end, dim3(1), dim3(64), 0, s, st, epsilon, max_iterations);
+            if (mixed) {
+                hipLaunchKernelGGL(k_to64, dim3(egrid), dim3(TPB), 0, s, G->E, G->ew, G->pos, G->d[0],
+                                   G->d[1], G->wh[0], G->wh[1], G->d32[0], G->d32[1], st);
+                hipLaunchKernelGGL(k_switch_done, dim3(1), dim3(64), 0, s, st);
+            }
+        }
+        launched += n;
+        DH_CHECK(hipGetLastError());
+        DH_CHECK(hipMemcpyAsync(&h, st, sizeof(h), hipMemcpyDeviceToHost, s));
+        DH_CHECK(hipStreamSynchronize(s));
+        if (h.done) break;
+        chunk = std::min(chunk * 2, 16);
+    }
+    DH_CHECK(hipEventRecord(ev1, s));
+    DH_CHECK(hipEventSynchronize(ev1));
+    float ms = 0.f;
+    hipEventElapsedTime(&ms, ev0, ev1);
+    hipEventDestroy(ev0);
+    hipEventDestroy(ev1);
+    if (!h.done) { set_error("fit did not finish within the launched iterations"); return false; }
+    G->cur = h.cur;
+    res->iterations = h.iterations;
+    res->delta = h.delta;
+    res->executed = h.iter;
+    res->fp32_iterations = mixed ? h.switched_at : 0;
+    res->ms += ms;
+    *ambiguous = h.ambiguous != 0;
+    return true;
+}
+
+__global__ void k_copy_f64(const double *a, double *b, int64_t n)
+{
+    for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n;
+         i += (int64_t)gridDim.x * blockDim.x)
+        b[i] = a[i];
+}
+
+}  // namespace
+
+bool fit_run(DevGraph *G, int max_iterations, double epsilon, int precision, FitResult *res)
+{
+    bool mixed = (precision == PREC_FP32_FP64);
+    res->ms = 0.f;
+    res->reruns = 0;
+    if (!build_bins(G)) return false;
+    if (max_iterations <= 0) {
+        res->iterations = max_iterations;
+        res->executed = 0;
+        return true;
+    }
+    // mixed mode keeps the start values so an ambiguous stop can be redone
+    double *start = nullptr;
+    if (mixed) {
+        if (!fit_alloc_state(G, true) || !dalloc(&start, (size_t)G->E + 1, "start values")) return false;
+        if (G->E > 0)
+            hipLaunchKernelGGL(k_copy_f64, dim3(grid_for(G->E, num_cus() * 8)), dim3(TPB), 0, G->stream,
+                               G->d[G->cur], start, (int64_t)G->E);
+    }
+    bool amb = false;
+    int start_cur = G->cur;
+    bool ok = fit_run_once(G, max_iterations, epsilon, mixed, res, &amb);
+    if (ok && mixed && amb) {
+        // redo in FP64 from the same start: bitwise the reference trajectory
+        G->cur = start_cur;
+        if (G->E > 0)
+            hipLaunchKernelGGL(k_copy_f64, dim3(grid_for(G->E, num_cus() * 8)), dim3(TPB), 0, G->stream,
+                               start, G->d[start_cur], (int64_t)G->E);
+        res->reruns = 1;
+        ok = fit_run_once(G, max_iterations, epsilon, false, res, &amb);
+        res->fp32_iterations = 0;
+    }
+    dfree(start);
+    return ok;
+}
+
+bool fit_download(DevGraph *G, double *edge_dress, double *vertex_dress)
+{
+    hipStream_t s = G->stream;
+    if (edge_dress && G->E > 0)
+        DH_CHECK(hipMemcpyAsync(edge_dress, G->d[G->cur], (size_t)G->E * sizeof(double), hipMemcpyDeviceToHost, s));
+    if (vertex_dress && G->N > 0)
+        DH_CHECK(hipMemcpyAsync(vertex_dress, G->vd, (size_t)G->N * sizeof(double), hipMemcpyDeviceToHost, s));
+    DH_CHECK(hipStreamSynchronize(s));
+    return true;
+}
+
+}  // namespace dh

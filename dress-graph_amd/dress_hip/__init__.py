@@ -1,0 +1,320 @@
+"""dress_hip -- the MI355X (HIP) backend of DRESS, selected by import.
+
+Mirrors the reference's ``dress.cuda`` module
+(python/src/dress/cuda/__init__.py:1-643)::
+
+    from dress_hip import fit, delta_fit, nabla_fit, DRESS
+
+    r = fit(N, sources, targets)                 # runs on the GPU
+    h = delta_fit(N, sources, targets, k=1)      # one batched GPU launch
+
+Same parameters, defaults and result fields as ``dress.fit`` /
+``dress.delta_fit`` / ``dress.nabla_fit``.  Every call goes through
+libdress_hip.so's C ABI; there is no CPU path.  ``is_available()`` probes
+for the library and a GPU.
+"""
+import ctypes
+from dataclasses import dataclass, field
+from typing import List, Optional, Tuple
+
+import numpy as np
+
+from . import _lib
+from ._lib import HipError, c_dbl_p, c_int_p
+
+UNDIRECTED = 0
+DIRECTED = 1
+FORWARD = 2
+BACKWARD = 3
+
+FP64 = 0
+FP32_FP64 = 1
+
+
+@dataclass
+class DRESSResult:
+    """Fields of the reference's DRESSResult (python/src/dress/core.py)."""
+    sources: List[int]
+    targets: List[int]
+    edge_dress: np.ndarray
+    edge_weight: np.ndarray
+    vertex_dress: np.ndarray
+    iterations: int
+    delta: float
+
+
+@dataclass
+class DeltaDRESSResult:
+    histogram: List[Tuple[float, int]]
+    multisets: Optional[np.ndarray]
+    num_subgraphs: int
+
+
+@dataclass
+class NablaDRESSResult:
+    histogram: List[Tuple[float, int]]
+    multisets: Optional[np.ndarray]
+    num_tuples: int
+
+
+@dataclass
+class FitStats:
+    iterations_executed: int
+    fp32_iterations: int
+    fp64_reruns: int
+    device_ms: float
+    intersections: int
+    half_edges: int
+
+
+def is_available():
+    try:
+        return _lib.lib().dress_hip_device_count() > 0
+    except (HipError, OSError):
+        return False
+
+
+def set_precision(precision):
+    _lib.lib().dress_hip_set_precision(int(precision))
+
+
+def get_precision():
+    return _lib.lib().dress_hip_get_precision()
+
+
+def last_fit_stats():
+    s = _lib.FitStats()
+    _lib.lib().dress_hip_last_fit_stats(ctypes.byref(s))
+    return FitStats(s.iterations_executed, s.fp32_iterations, s.fp64_reruns, s.device_ms,
+                    s.intersections, s.half_edges)
+
+
+class Graph:
+    """Owns one dress_graph_t built by libdress_hip.so (device-resident)."""
+
+    def __init__(self, n_vertices, sources, targets, weights=None, vertex_weights=None,
+                 variant=UNDIRECTED, precompute_intercepts=False):
+        L = _lib.lib()
+        U = np.ascontiguousarray(sources, dtype=np.int32)
+        V = np.ascontiguousarray(targets, dtype=np.int32)
+        if U.shape != V.shape:
+            raise ValueError("sources and targets differ in length")
+        self.N = int(n_vertices)
+        self.E = int(U.size)
+        pW = _lib.malloc_copy(weights, ctypes.c_double) if weights is not None else c_dbl_p()
+        pNW = _lib.malloc_copy(vertex_weights, ctypes.c_double) if vertex_weights is not None else c_dbl_p()
+        g = L.dress_init_graph(self.N, self.E, _lib.malloc_copy(U, ctypes.c_int),
+                               _lib.malloc_copy(V, ctypes.c_int), pW, pNW, int(variant),
+                               1 if precompute_intercepts else 0)
+        if not g:
+            raise HipError(f"dress_init_graph failed: {_lib.last_error()}")
+        self.g = g
+
+    def close(self):
+        if getattr(self, "g", None):
+            _lib.lib().dress_free_graph(self.g)
+            self.g = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *a):
+        self.close()
+
+    # -- views of the host struct --------------------------------------
+    @property
+    def edge_dress(self):
+        return _lib.as_np(self.g.contents.edge_dress, self.E, np.float64)
+
+    @property
+    def vertex_dress(self):
+        return _lib.as_np(self.g.contents.vertex_dress, self.N, np.float64)
+
+    @property
+    def edge_weight(self):
+        return _lib.as_np(self.g.contents.edge_weight, self.E, np.float64)
+
+    def csr(self):
+        c = self.g.contents
+        S = c.adj_offset[c.N] if c.N > 0 else 0
+        out = {
+            "adj_offset": _lib.as_np(c.adj_offset, c.N + 1, np.int32),
+            "adj_target": _lib.as_np(c.adj_target, S, np.int32),
+            "adj_edge_idx": _lib.as_np(c.adj_edge_idx, S, np.int32),
+            "edge_weight": _lib.as_np(c.edge_weight, c.E, np.float64),
+            "max_degree": c.max_degree,
+        }
+        if c.precompute_intercepts:
+            T = c.intercept_offset[c.E]
+            out["intercept_offset"] = _lib.as_np(c.intercept_offset, c.E + 1, np.int32)
+            out["intercept_edge_ux"] = _lib.as_np(c.intercept_edge_ux, T, np.int32)
+            out["intercept_edge_vx"] = _lib.as_np(c.intercept_edge_vx, T, np.int32)
+        return out
+
+    def set_edge_dress(self, values):
+        a = np.ascontiguousarray(values, dtype=np.float64)
+        ctypes.memmove(self.g.contents.edge_dress, a.ctypes.data, 8 * self.E)
+
+    # -- fits -------------------------------------------------------------
+    def fit(self, max_iterations=100, epsilon=1e-6, precision=None):
+        it = ctypes.c_int(0)
+        dl = ctypes.c_double(0.0)
+        L = _lib.lib()
+        if precision is None:
+            L.dress_fit_hip(self.g, int(max_iterations), float(epsilon), ctypes.byref(it),
+                            ctypes.byref(dl))
+            if it.value == -1 and _lib.last_error():
+                raise HipError(f"dress_fit_hip failed: {_lib.last_error()}")
+        else:
+            r = L.dress_fit_hip_ex(self.g, int(max_iterations), float(epsilon), int(precision),
+                                   ctypes.byref(it), ctypes.byref(dl))
+            _lib.check(r == 0, "dress_fit_hip_ex")
+        return it.value, dl.value
+
+    def resident_reset(self):
+        _lib.check(_lib.lib().dress_hip_resident_reset(self.g) == 0, "dress_hip_resident_reset")
+
+    def fit_resident(self, max_iterations=100, epsilon=1e-6, precision=FP64):
+        it = ctypes.c_int(0)
+        dl = ctypes.c_double(0.0)
+        r = _lib.lib().dress_hip_fit_resident(self.g, int(max_iterations), float(epsilon),
+                                              int(precision), ctypes.byref(it), ctypes.byref(dl))
+        _lib.check(r == 0, "dress_hip_fit_resident")
+        return it.value, dl.value
+
+    def resident_download(self):
+        _lib.check(_lib.lib().dress_hip_resident_download(self.g) == 0, "dress_hip_resident_download")
+
+    def get(self, u, v, max_iterations=100, epsilon=1e-6, edge_weight=1.0):
+        L = _lib.lib()
+        val = L.dress_get(self.g, int(u), int(v), int(max_iterations), float(epsilon),
+                          float(edge_weight))
+        err = _lib.last_error()
+        if err:
+            raise HipError(f"dress_get failed: {err}")
+        return val
+
+    def lifted(self, kind, k, max_iterations=100, epsilon=1e-6, n_samples=0, seed=0,
+               keep_multisets=False, compute_histogram=True, offset=0, stride=1):
+        """delta^k / nabla^k through dress_{delta,nabla}_fit_hip_strided_flat.
+        Returns (values, counts, multisets or None, count)."""
+        L = _lib.lib()
+        f = L.dress_delta_fit_hip_strided_flat if kind == "delta" else L.dress_nabla_fit_hip_strided_flat
+        hs = ctypes.c_int(0)
+        ns = ctypes.c_int64(0)
+        ms = c_dbl_p()
+        flat = f(self.g, int(k), int(max_iterations), float(epsilon), int(n_samples),
+                 ctypes.c_uint(seed), ctypes.byref(hs) if compute_histogram else None,
+                 1 if keep_multisets else 0, ctypes.byref(ms), ctypes.byref(ns),
+                 int(offset), int(stride))
+        err = _lib.last_error()
+        if err:
+            _lib.c_free(flat)
+            _lib.c_free(ms)
+            raise HipError(f"{kind} fit failed: {err}")
+        n = hs.value
+        vals = np.empty(0)
+        cnts = np.empty(0, dtype=np.int64)
+        if flat and n > 0:
+            arr = np.ctypeslib.as_array(flat, shape=(n,)).copy()
+            vals = arr[0::2].view(np.float64).copy()
+            cnts = arr[1::2].copy()
+        _lib.c_free(flat)
+        mset = None
+        if keep_multisets and ms:
+            mset = np.ctypeslib.as_array(ms, shape=(ns.value * self.E,)).copy().reshape(ns.value, self.E)
+            _lib.c_free(ms)
+        return vals, cnts, mset, ns.value
+
+
+# ---------------------------------------------------------------------------
+#  Unified API -- same signatures as dress.fit / dress.delta_fit / nabla_fit
+# ---------------------------------------------------------------------------
+
+def fit(n_vertices, sources, targets, weights=None, vertex_weights=None, variant=UNDIRECTED,
+        max_iterations=100, epsilon=1e-6, precompute_intercepts=False, precision=None):
+    """GPU DRESS fit -- drop-in for ``dress.fit`` (cuda/__init__.py:451-494)."""
+    src = list(sources) if not isinstance(sources, np.ndarray) else sources
+    tgt = list(targets) if not isinstance(targets, np.ndarray) else targets
+    with Graph(n_vertices, src, tgt, weights, vertex_weights, variant,
+               precompute_intercepts) as g:
+        it, dl = g.fit(max_iterations, epsilon, precision)
+        return DRESSResult(sources=list(np.asarray(src).tolist()), targets=list(np.asarray(tgt).tolist()),
+                           edge_dress=g.edge_dress, edge_weight=g.edge_weight,
+                           vertex_dress=g.vertex_dress, iterations=it, delta=dl)
+
+
+def _hist_list(vals, cnts):
+    return [(float(v), int(c)) for v, c in zip(vals, cnts)]
+
+
+def delta_fit(n_vertices, sources, targets, weights=None, vertex_weights=None, k=0,
+              variant=UNDIRECTED, max_iterations=100, epsilon=1e-6, precompute=False,
+              keep_multisets=False, offset=0, stride=1, n_samples=0, seed=0,
+              compute_histogram=True):
+    """GPU delta^k-DRESS -- drop-in for ``dress.delta_fit`` (cuda/__init__.py:497-574)."""
+    with Graph(n_vertices, sources, targets, weights, vertex_weights, variant, precompute) as g:
+        vals, cnts, ms, ns = g.lifted("delta", k, max_iterations, epsilon, n_samples, seed,
+                                      keep_multisets, compute_histogram, offset, stride)
+    return DeltaDRESSResult(histogram=_hist_list(vals, cnts), multisets=ms, num_subgraphs=ns)
+
+
+def nabla_fit(n_vertices, sources, targets, weights=None, vertex_weights=None, k=0,
+              variant=UNDIRECTED, max_iterations=100, epsilon=1e-6, precompute=False,
+              keep_multisets=False, n_samples=0, seed=0, compute_histogram=True):
+    """GPU nabla^k-DRESS -- drop-in for ``dress.nabla_fit`` (cuda/__init__.py:306-375)."""
+    with Graph(n_vertices, sources, targets, weights, vertex_weights, variant, precompute) as g:
+        vals, cnts, ms, ns = g.lifted("nabla", k, max_iterations, epsilon, n_samples, seed,
+                                      keep_multisets, compute_histogram, 0, 1)
+    return NablaDRESSResult(histogram=_hist_list(vals, cnts), multisets=ms, num_tuples=ns)
+
+
+class DRESS:
+    """Object API of ``dress.cuda.DRESS`` (cuda/__init__.py:586-643):
+    ``fit`` / ``delta_fit`` / ``nabla_fit`` / ``get`` on one graph."""
+
+    def __init__(self, n_vertices, sources, targets, weights=None, vertex_weights=None,
+                 variant=UNDIRECTED, precompute_intercepts=False):
+        self._g = Graph(n_vertices, sources, targets, weights, vertex_weights, variant,
+                        precompute_intercepts)
+        self.n_vertices = int(n_vertices)
+        self.n_edges = self._g.E
+
+    def fit(self, max_iterations=100, epsilon=1e-6):
+        it, dl = self._g.fit(max_iterations, epsilon)
+        return it, dl
+
+    def delta_fit(self, k=0, max_iterations=100, epsilon=1e-6, n_samples=0, seed=0,
+                  keep_multisets=False, compute_histogram=True, offset=0, stride=1):
+        vals, cnts, ms, ns = self._g.lifted("delta", k, max_iterations, epsilon, n_samples, seed,
+                                            keep_multisets, compute_histogram, offset, stride)
+        return DeltaDRESSResult(histogram=_hist_list(vals, cnts), multisets=ms, num_subgraphs=ns)
+
+    def nabla_fit(self, k=0, max_iterations=100, epsilon=1e-6, n_samples=0, seed=0,
+                  keep_multisets=False, compute_histogram=True):
+        vals, cnts, ms, ns = self._g.lifted("nabla", k, max_iterations, epsilon, n_samples, seed,
+                                            keep_multisets, compute_histogram)
+        return NablaDRESSResult(histogram=_hist_list(vals, cnts), multisets=ms, num_tuples=ns)
+
+    def get(self, u, v, max_iterations=100, epsilon=1e-6, edge_weight=1.0):
+        return self._g.get(u, v, max_iterations, epsilon, edge_weight)
+
+    @property
+    def edge_dress(self):
+        return self._g.edge_dress
+
+    @property
+    def vertex_dress(self):
+        return self._g.vertex_dress
+
+    def close(self):
+        self._g.close()
+
+    def __repr__(self):
+        return f"DRESS(n_vertices={self.n_vertices}, n_edges={self.n_edges}, backend=hip)"

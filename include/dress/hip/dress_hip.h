@@ -1,0 +1,159 @@
+/*
+ * dress/hip/dress_hip.h -- the HIP (MI355X, gfx950) backend of libdress.
+ *
+ * Drop-in counterpart of libdress/include/dress/cuda/dress_cuda.h: the same
+ * entry points with the suffix _hip, the same arguments, results and
+ * ownership rules (histograms and multisets are malloc'd; the caller frees
+ * them with free()).  Include dress/hip/dress.h instead to redirect the plain
+ * names (dress_fit, dress_delta_fit, ...) to these.
+ *
+ * What differs from the CUDA backend:
+ *  - the graph lives on the device: dress_init_graph builds its CSR and the
+ *    neighbourhood-intersection index there, and later fits reuse them; the
+ *    host struct is filled with the same arrays the CPU library would build;
+ *  - delta^k / nabla^k subgraphs are not rebuilt on the host: one persistent
+ *    kernel runs every subgraph of the request with the parent graph masked
+ *    (deleted vertices) or re-weighted (marked vertices) in LDS;
+ *  - errors do not exit(): the call leaves its outputs unset (fits set
+ *    *iterations = -1 and *delta = NaN, lifted fits return NULL with
+ *    *hist_size = 0) and dress_hip_last_error() describes what failed.
+ *
+ * Numerics: FP64 results equal the CPU library's bit for bit, except in the
+ * rare case where an exact sum lies within ~n^2*2^-106 of a rounding midpoint
+ * (the CPU sorts the terms and adds them with Kahan-Babuska-Neumaier; the GPU
+ * adds them in any order with error-free transformations and rounds once).
+ * The optional FP32->FP64 precision keeps every value within 1e-9 of the
+ * FP64 result (see dress_hip_set_precision).
+ */
+#ifndef DRESS_HIP_H
+#define DRESS_HIP_H
+
+#include "dress/dress.h"
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ---- counterparts of libdress/include/dress/cuda/dress_cuda.h -------- */
+
+/* dress_cuda.h:71-72 */
+void dress_fit_hip(p_dress_graph_t g, int max_iterations, double epsilon,
+                   int *iterations, double *delta);
+
+/* dress_cuda.h:80-85 */
+dress_hist_pair_t *dress_delta_fit_hip(p_dress_graph_t g, int k, int iterations,
+                                       double epsilon,
+                                       int n_samples, unsigned int seed,
+                                       int *hist_size,
+                                       int keep_multisets, double **multisets,
+                                       int64_t *num_subgraphs);
+
+/* dress_cuda.h:93-99 */
+dress_hist_pair_t *dress_delta_fit_hip_strided(p_dress_graph_t g, int k,
+                                               int iterations, double epsilon,
+                                               int n_samples, unsigned int seed,
+                                               int *hist_size,
+                                               int keep_multisets, double **multisets,
+                                               int64_t *num_subgraphs,
+                                               int offset, int stride);
+
+/* dress_cuda.h:105-111 (and the unstrided flat form delta_dress_cuda.c:24-36) */
+int64_t *dress_delta_fit_hip_flat(p_dress_graph_t g, int k, int iterations,
+                                  double epsilon,
+                                  int n_samples, unsigned int seed,
+                                  int *hist_size,
+                                  int keep_multisets, double **multisets,
+                                  int64_t *num_subgraphs);
+int64_t *dress_delta_fit_hip_strided_flat(p_dress_graph_t g, int k,
+                                          int iterations, double epsilon,
+                                          int n_samples, unsigned int seed,
+                                          int *hist_size,
+                                          int keep_multisets, double **multisets,
+                                          int64_t *num_subgraphs,
+                                          int offset, int stride);
+
+/* dress_cuda.h:115-143 */
+dress_hist_pair_t *dress_nabla_fit_hip(p_dress_graph_t g, int k, int iterations,
+                                       double epsilon,
+                                       int n_samples, unsigned int seed,
+                                       int *hist_size,
+                                       int keep_multisets, double **multisets,
+                                       int64_t *num_tuples);
+dress_hist_pair_t *dress_nabla_fit_hip_strided(p_dress_graph_t g, int k,
+                                               int iterations, double epsilon,
+                                               int n_samples, unsigned int seed,
+                                               int *hist_size,
+                                               int keep_multisets, double **multisets,
+                                               int64_t *num_tuples,
+                                               int offset, int stride);
+int64_t *dress_nabla_fit_hip_flat(p_dress_graph_t g, int k, int iterations,
+                                  double epsilon,
+                                  int n_samples, unsigned int seed,
+                                  int *hist_size,
+                                  int keep_multisets, double **multisets,
+                                  int64_t *num_tuples);
+int64_t *dress_nabla_fit_hip_strided_flat(p_dress_graph_t g, int k,
+                                          int iterations, double epsilon,
+                                          int n_samples, unsigned int seed,
+                                          int *hist_size,
+                                          int keep_multisets, double **multisets,
+                                          int64_t *num_tuples,
+                                          int offset, int stride);
+
+/* ---- backend extensions (no counterpart in the reference) ------------ */
+
+/* Last error of the calling thread ("" when the last call succeeded). */
+const char *dress_hip_last_error(void);
+
+/* Number of visible HIP devices (0 when no GPU / no runtime). */
+int dress_hip_device_count(void);
+
+/* Arithmetic of the *_hip fits.
+ *   DRESS_HIP_FP64      FP64 throughout (default; the CPU library's values)
+ *   DRESS_HIP_FP32_FP64 early iterations store values in FP32 (half the
+ *                       gathered bytes) and hand over to FP64 while enough
+ *                       iterations remain to bring every value within 1e-9
+ *                       of the FP64 result; a fit whose stopping decision
+ *                       is within 0.1% of epsilon is redone in FP64.
+ * The initial default can also be set with DRESS_HIP_PRECISION=fp64|mixed. */
+typedef enum {
+    DRESS_HIP_FP64 = 0,
+    DRESS_HIP_FP32_FP64 = 1
+} dress_hip_precision_t;
+void dress_hip_set_precision(dress_hip_precision_t precision);
+dress_hip_precision_t dress_hip_get_precision(void);
+
+/* Statistics of the calling thread's last dress_fit_hip / dress_hip_fit_resident. */
+typedef struct {
+    int    iterations_executed;   /* iterations actually run                  */
+    int    fp32_iterations;       /* of which with FP32 storage               */
+    int    fp64_reruns;           /* mixed fits redone in FP64                */
+    double device_ms;             /* device time of the iteration loop        */
+    int64_t intersections;        /* T: entries of the intersection index     */
+    int64_t half_edges;           /* S: CSR entries                           */
+} dress_hip_fit_stats_t;
+void dress_hip_last_fit_stats(dress_hip_fit_stats_t *out);
+
+/* Fit with explicit precision; returns 0 on success, -1 on error.
+ * Same semantics as dress_fit_hip otherwise. */
+int dress_fit_hip_ex(p_dress_graph_t g, int max_iterations, double epsilon,
+                     dress_hip_precision_t precision,
+                     int *iterations, double *delta);
+
+/* Device-resident fits for callers that keep the graph on the GPU between
+ * calls (benchmarks, servers): dress_hip_resident_reset() loads
+ * g->edge_dress onto the device; dress_hip_fit_resident() then iterates the
+ * device copy without any host transfer; dress_hip_resident_download()
+ * copies values and norms back into g.  Return 0 on success, -1 on error. */
+int dress_hip_resident_reset(p_dress_graph_t g);
+int dress_hip_fit_resident(p_dress_graph_t g, int max_iterations, double epsilon,
+                           dress_hip_precision_t precision,
+                           int *iterations, double *delta);
+int dress_hip_resident_download(p_dress_graph_t g);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* DRESS_HIP_H */
