@@ -19,6 +19,9 @@
 #include "dress_hip_internal.h"
 
 #include <algorithm>
+#include <cmath>
+#include <cstdlib>
+#include <cstring>
 
 namespace dh {
 
@@ -108,14 +111,12 @@ __global__ void k_half_keys(const int *U, const int *V, const double *W, int E, 
 }
 
 __global__ void k_unpack_half(const uint64_t *key, const int *val, int64_t S, int *adj_x,
-                              int *adj_e, int2 *pos)
+                              int *adj_e)
 {
     for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < S;
          i += (int64_t)gridDim.x * blockDim.x) {
-        int e = val[i] >> 1;
         adj_x[i] = (int)(uint32_t)(key[i] & 0xffffffffu);
-        adj_e[i] = e;
-        if (atomicCAS(&pos[e].x, -1, (int)i) != -1) pos[e].y = (int)i;
+        adj_e[i] = val[i] >> 1;
     }
 }
 
@@ -156,26 +157,28 @@ __device__ __forceinline__ int lower_bound_i(const int *a, int lo, int hi, int x
 }
 
 // Multiset intersection of rows U[e], V[e].  GS lanes per edge walk the
-// shorter row; FILL=false counts, FILL=true writes (pos in row U, pos in
-// row V) pairs at ioff[e] in ascending neighbour order.
+// shorter row.  FILL=false: count for input edge e.  FILL=true: for internal
+// edge i (input edge ord[i]) write the half-edge pair (a in row U, b in row V)
+// of each common neighbour at ioff[i], ascending in the neighbour, packed as
+// row-relative 16-bit positions (ipk) or absolute (ipair).
 template <int GS, bool FILL>
 __global__ void k_isect(const int *U, const int *V, int E, const int *row_off, const int *adj_x,
-                        int *cnt, const int64_t *ioff, int2 *ipair)
+                        const int *ord, int64_t *cnt, const int64_t *ioff, uint32_t *ipk, int2 *ipair)
 {
     const int lane = threadIdx.x % GS;
     const int wlane = threadIdx.x & 63;
     const int gbase = wlane - lane;              // first lane of my group in the wave
     const int64_t groups = (int64_t)gridDim.x * blockDim.x / GS;
     for (int64_t g = (blockIdx.x * (int64_t)blockDim.x + threadIdx.x) / GS; g < E; g += groups) {
-        int e = (int)g;
+        int e = FILL ? ord[g] : (int)g;
         int u = U[e], v = V[e];
         int ub = row_off[u], ue = row_off[u + 1];
         int vb = row_off[v], ve = row_off[v + 1];
         bool a_is_u = (ue - ub) <= (ve - vb);
         int ab = a_is_u ? ub : vb, ae = a_is_u ? ue : ve;
         int bb = a_is_u ? vb : ub, be = a_is_u ? ve : ue;
-        int64_t out = FILL ? ioff[e] : 0;
-        int found = 0;
+        int64_t out = FILL ? ioff[g] : 0;
+        int64_t found = 0;
         for (int base = ab; base < ae; base += GS) {
             int p = base + lane;
             bool hit = false;
@@ -194,10 +197,15 @@ __global__ void k_isect(const int *U, const int *V, int E, const int *row_off, c
                 unsigned long long gm = (GS == 64) ? m : ((m >> gbase) & ((1ull << GS) - 1));
                 if (hit) {
                     int before = __popcll(gm & ((1ull << lane) - 1));
-                    int2 pr;
-                    pr.x = a_is_u ? p : q;
-                    pr.y = a_is_u ? q : p;
-                    ipair[out + before] = pr;
+                    int pu = a_is_u ? p : q, pv = a_is_u ? q : p;
+                    if (ipk)
+                        ipk[out + before] = (uint32_t)(pu - ub) | ((uint32_t)(pv - vb) << 16);
+                    else {
+                        int2 pr;
+                        pr.x = pu;
+                        pr.y = pv;
+                        ipair[out + before] = pr;
+                    }
                 }
                 out += __popcll(gm);
             } else {
@@ -211,22 +219,84 @@ __global__ void k_isect(const int *U, const int *V, int E, const int *row_off, c
     }
 }
 
-// bin key: (bin << 32) | item, bin chosen from the number of summed terms
-__global__ void k_bin_keys_rows(const int *row_off, int N, uint64_t *key)
+// half-edges of each input edge (at most two), ordered x < y
+__global__ void k_edge_positions(const int *adj_e, int64_t S, int2 *pos_in)
 {
-    for (int u = blockIdx.x * blockDim.x + threadIdx.x; u < N; u += gridDim.x * blockDim.x) {
-        int terms = row_off[u + 1] - row_off[u] + 1;
-        int b = terms <= 4 ? 0 : terms <= 32 ? 1 : terms <= 256 ? 2 : terms <= 4096 ? 3 : 4;
-        key[u] = ((uint64_t)b << 32) | (uint32_t)u;
+    for (int64_t s = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; s < S;
+         s += (int64_t)gridDim.x * blockDim.x) {
+        int e = adj_e[s];
+        if (atomicCAS(&pos_in[e].x, -1, (int)s) != -1) pos_in[e].y = (int)s;
     }
 }
 
-__global__ void k_bin_keys_edges(const int64_t *ioff, int E, uint64_t *key)
+__global__ void k_order_positions(int2 *pos_in, int E)
 {
     for (int e = blockIdx.x * blockDim.x + threadIdx.x; e < E; e += gridDim.x * blockDim.x) {
-        int64_t terms = ioff[e + 1] - ioff[e] + 1;
-        int b = terms <= 4 ? 0 : terms <= 32 ? 1 : terms <= 256 ? 2 : terms <= 4096 ? 3 : 4;
-        key[e] = ((uint64_t)b << 32) | (uint32_t)e;
+        int2 p = pos_in[e];
+        if (p.y >= 0 && p.y < p.x) {
+            int t = p.x;
+            p.x = p.y;
+            p.y = t;
+            pos_in[e] = p;
+        }
+    }
+}
+
+// internal order key: (bin, first half-edge)
+__global__ void k_order_keys(const int64_t *cnt_in, const int2 *pos_in, int E, uint64_t *key, int *val)
+{
+    for (int e = blockIdx.x * blockDim.x + threadIdx.x; e < E; e += gridDim.x * blockDim.x) {
+        key[e] = ((uint64_t)term_bin(cnt_in[e] + 1) << 32) | (uint32_t)pos_in[e].x;
+        val[e] = e;
+    }
+}
+
+__global__ void k_count_bins(const int *ord, const int64_t *cnt_in, int E, int *counts)
+{
+    __shared__ int c[Bins::NB];
+    if (threadIdx.x < Bins::NB) c[threadIdx.x] = 0;
+    __syncthreads();
+    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < E; i += gridDim.x * blockDim.x)
+        atomicAdd(&c[term_bin(cnt_in[ord[i]] + 1)], 1);
+    __syncthreads();
+    if (threadIdx.x < Bins::NB) atomicAdd(&counts[threadIdx.x], c[threadIdx.x]);
+}
+
+__global__ void k_internal_edges(const int *ord, int E, const int *U, const int *V, const int *row_off,
+                                 const double *ew_in, const double *c4_in, const int64_t *cnt_in,
+                                 const int2 *pos_in, int *inv, int2 *uv, int2 *rs, int2 *pos,
+                                 double *ew, double *c4, int64_t *cnt_i)
+{
+    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < E; i += gridDim.x * blockDim.x) {
+        int e = ord[i];
+        inv[e] = i;
+        int2 p;
+        p.x = U[e];
+        p.y = V[e];
+        uv[i] = p;
+        int2 r;
+        r.x = row_off[p.x];
+        r.y = row_off[p.y];
+        rs[i] = r;
+        pos[i] = pos_in[e];
+        ew[i] = ew_in[e];
+        c4[i] = c4_in[e];
+        if (cnt_i) cnt_i[i] = cnt_in[e];
+    }
+}
+
+__global__ void k_self_terms(const int *ord, int E, const double *c4_in, double *c4)
+{
+    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < E; i += gridDim.x * blockDim.x)
+        c4[i] = c4_in[ord[i]];
+}
+
+// bin key of vertex rows: (bin << 32) | u
+__global__ void k_bin_keys_rows(const int *row_off, int N, uint64_t *key)
+{
+    for (int u = blockIdx.x * blockDim.x + threadIdx.x; u < N; u += gridDim.x * blockDim.x) {
+        int b = term_bin(row_off[u + 1] - row_off[u] + 1);
+        key[u] = ((uint64_t)b << 32) | (uint32_t)u;
     }
 }
 
@@ -243,43 +313,59 @@ __global__ void k_bin_unpack(const uint64_t *key, int n, int *list, int *counts)
     if (threadIdx.x < Bins::NB) atomicAdd(&counts[threadIdx.x], c[threadIdx.x]);
 }
 
-bool make_bins(DevGraph *G, Bins *B, int n, bool rows)
+// vertex rows sorted by (bin, id)
+bool vertex_bins(hipStream_t s, int n, const int *row_off, int *list, int counts_out[Bins::NB])
 {
-    hipStream_t s = G->stream;
     uint64_t *k0 = nullptr, *k1 = nullptr;
-    int *cnt = nullptr;
+    int *cnt_d = nullptr;
     if (!dalloc(&k0, (size_t)n + 1, "bin keys") || !dalloc(&k1, (size_t)n + 1, "bin keys") ||
-        !dalloc(&cnt, Bins::NB, "bin counts")) {
-        dfree(k0); dfree(k1); dfree(cnt);
+        !dalloc(&cnt_d, Bins::NB, "bin counts")) {
+        dfree(k0); dfree(k1); dfree(cnt_d);
         return false;
     }
     bool ok = true;
     if (n > 0) {
-        if (rows)
-            hipLaunchKernelGGL(k_bin_keys_rows, dim3(grid_for(n, 4096)), dim3(TPB), 0, s, G->row_off, n, k0);
-        else
-            hipLaunchKernelGGL(k_bin_keys_edges, dim3(grid_for(n, 4096)), dim3(TPB), 0, s, G->ioff, n, k0);
+        hipLaunchKernelGGL(k_bin_keys_rows, dim3(grid_for(n, 4096)), dim3(TPB), 0, s, row_off, n, k0);
         ok = prim_sort_u64(k0, k1, (size_t)n, s);
     }
-    if (ok && !dalloc(&B->list, (size_t)n + 1, "bin list")) ok = false;
-    if (ok) {
-        hipError_t e1 = hipMemsetAsync(cnt, 0, Bins::NB * sizeof(int), s);
-        if (e1 != hipSuccess) { set_error("bin count reset failed"); ok = false; }
+    if (ok && hipMemsetAsync(cnt_d, 0, Bins::NB * sizeof(int), s) != hipSuccess) {
+        set_error("bin count reset failed");
+        ok = false;
     }
     if (ok && n > 0)
-        hipLaunchKernelGGL(k_bin_unpack, dim3(grid_for(n, 2048)), dim3(TPB), 0, s, k1, n, B->list, cnt);
-    int hc[Bins::NB] = {0, 0, 0, 0, 0};
+        hipLaunchKernelGGL(k_bin_unpack, dim3(grid_for(n, 2048)), dim3(TPB), 0, s, k1, n, list, cnt_d);
     if (ok) {
-        hipError_t e2 = hipMemcpyAsync(hc, cnt, sizeof(hc), hipMemcpyDeviceToHost, s);
+        hipError_t e2 = hipMemcpyAsync(counts_out, cnt_d, Bins::NB * sizeof(int), hipMemcpyDeviceToHost, s);
         hipError_t e3 = hipStreamSynchronize(s);
         if (e2 != hipSuccess || e3 != hipSuccess) { set_error("bin download failed"); ok = false; }
     }
-    if (ok) {
-        int st = 0;
-        for (int b = 0; b < Bins::NB; b++) { B->start[b] = st; B->count[b] = hc[b]; st += hc[b]; }
-    }
-    dfree(k0); dfree(k1); dfree(cnt);
+    dfree(k0); dfree(k1); dfree(cnt_d);
     return ok;
+}
+
+// reference-layout intercept arrays (input edge ids, input order offsets)
+__global__ void k_index_to_reference(const int *ord, int E, const int64_t *ioff, const int2 *rs,
+                                     const uint32_t *ipk, const int2 *ipair, const int *adj_e,
+                                     const int64_t *ioff_in, int *ux, int *vx)
+{
+    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < E; i += gridDim.x * blockDim.x) {
+        int e = ord[i];
+        int64_t src = ioff[i], n = ioff[i + 1] - ioff[i], dst = ioff_in[e];
+        int2 r = rs[i];
+        for (int64_t k = 0; k < n; k++) {
+            int a, b;
+            if (ipk) {
+                uint32_t q = ipk[src + k];
+                a = r.x + (int)(q & 0xffffu);
+                b = r.y + (int)(q >> 16);
+            } else {
+                a = ipair[src + k].x;
+                b = ipair[src + k].y;
+            }
+            ux[dst + k] = adj_e[a];
+            vx[dst + k] = adj_e[b];
+        }
+    }
 }
 
 }  // namespace
@@ -294,9 +380,9 @@ bool build_graph(DevGraph *G, int N, int E, const int *U, const int *V, const do
     if (!G->stream) DH_CHECK(hipStreamCreateWithFlags(&G->stream, hipStreamNonBlocking));
     hipStream_t s = G->stream;
 
-    if (!dalloc(&G->U, E, "U") || !dalloc(&G->V, E, "V") || !dalloc(&G->ew, E, "edge_weight") ||
-        !dalloc(&G->nw, (size_t)N, "NW") || !dalloc(&G->c4, E, "self term") ||
-        !dalloc(&G->pos, E, "half-edge positions") || !dalloc(&G->row_off, (size_t)N + 1, "row offsets"))
+    if (!dalloc(&G->U, E, "U") || !dalloc(&G->V, E, "V") || !dalloc(&G->ew_in, E, "edge_weight") ||
+        !dalloc(&G->nw, (size_t)N, "NW") || !dalloc(&G->c4_in, E, "self term") ||
+        !dalloc(&G->row_off, (size_t)N + 1, "row offsets"))
         return false;
     if (E > 0) {
         DH_CHECK(hipMemcpyAsync(G->U, U, (size_t)E * sizeof(int), hipMemcpyHostToDevice, s));
@@ -316,7 +402,10 @@ bool build_graph(DevGraph *G, int N, int E, const int *U, const int *V, const do
     int *rval = nullptr;
     int *deg = nullptr;
     int64_t *deg64 = nullptr;
-    if (!dalloc(&deg, (size_t)N + 1, "degrees") || !dalloc(&deg64, (size_t)N + 1, "degrees")) return false;
+    if (!dalloc(&deg, (size_t)N + 1, "degrees") || !dalloc(&deg64, (size_t)N + 1, "degrees")) {
+        dfree(deg); dfree(dW);
+        return false;
+    }
     bool ok = true;
     if (variant == DRESS_VARIANT_DIRECTED && E > 0) {
         uint64_t *k0 = nullptr;
@@ -344,7 +433,7 @@ bool build_graph(DevGraph *G, int N, int E, const int *U, const int *V, const do
         if (E > 0)
             hipLaunchKernelGGL(k_half_keys, dim3(grid_for(E, 8192)), dim3(TPB), 0, s, G->U, G->V,
                                (const double *)dW, E, variant, (const int64_t *)roff,
-                               (const uint64_t *)rkey, (const int *)rval, hk0, hv0, G->ew, deg);
+                               (const uint64_t *)rkey, (const int *)rval, hk0, hv0, G->ew_in, deg);
         ok = prim_sort_pairs_u64_i32(hk0, hk1, hv0, hv1, (size_t)slots, 64, s) &&
              prim_exclusive_scan_i32_to_i64(deg, deg64, (size_t)N + 1, s);
     }
@@ -358,14 +447,12 @@ bool build_graph(DevGraph *G, int N, int E, const int *U, const int *V, const do
             ok = false;
         if (ok) {
             hipLaunchKernelGGL(k_i64_to_i32, dim3(grid_for(N + 1, 4096)), dim3(TPB), 0, s, deg64, G->row_off, (int64_t)N + 1);
-            hipLaunchKernelGGL(k_fill_i32, dim3(grid_for(2 * (int64_t)E, 8192)), dim3(TPB), 0, s,
-                               (int *)G->pos, 2 * (int64_t)E, -1);
             if (S > 0)
                 hipLaunchKernelGGL(k_unpack_half, dim3(grid_for(S, 8192)), dim3(TPB), 0, s, hk1, hv1, S,
-                                   G->adj_x, G->adj_e, G->pos);
+                                   G->adj_x, G->adj_e);
             if (E > 0)
                 hipLaunchKernelGGL(k_edge_consts, dim3(grid_for(E, 8192)), dim3(TPB), 0, s, G->U, G->V,
-                                   G->nw, E, variant, G->c4);
+                                   G->nw, E, variant, G->c4_in);
             int *dmax = deg;  // reuse: degrees no longer needed
             hipLaunchKernelGGL(k_fill_i32, dim3(1), dim3(64), 0, s, dmax, (int64_t)1, 0);
             if (N > 0)
@@ -386,31 +473,77 @@ bool build_index(DevGraph *G)
 {
     if (G->have_index) return true;
     hipStream_t s = G->stream;
-    int E = G->E;
-    int *cnt = nullptr;
-    int64_t *cnt64 = nullptr;
-    if (!dalloc(&cnt, (size_t)E + 1, "isect counts") || !dalloc(&G->ioff, (size_t)E + 1, "isect offsets")) {
-        dfree(cnt);
-        return false;
-    }
+    const int E = G->E;
     const int GS = 16;
     int grid = grid_for((int64_t)E * GS, num_cus() * 32);
-    hipLaunchKernelGGL(k_fill_i32, dim3(grid_for(E + 1, 4096)), dim3(TPB), 0, s, cnt, (int64_t)E + 1, 0);
-    if (E > 0)
+    int64_t *cnt_i = nullptr;
+    if (!dalloc(&G->cnt_in, (size_t)E + 1, "isect counts") || !dalloc(&cnt_i, (size_t)E + 1, "isect counts") ||
+        !dalloc(&G->ioff, (size_t)E + 1, "isect offsets") || !dalloc(&G->ord, (size_t)E + 1, "edge order") ||
+        !dalloc(&G->inv, (size_t)E + 1, "edge order") || !dalloc(&G->uv, (size_t)E + 1, "edge ends") ||
+        !dalloc(&G->rs, (size_t)E + 1, "row starts") || !dalloc(&G->pos, (size_t)E + 1, "edge half-edges") ||
+        !dalloc(&G->pos_in, (size_t)E + 1, "edge half-edges") ||
+        !dalloc(&G->ew, (size_t)E + 1, "edge weights") || !dalloc(&G->c4, (size_t)E + 1, "self terms")) {
+        dfree(cnt_i);
+        return false;
+    }
+    // 1. half-edges of every edge, |N(u) & N(v)| per input edge
+    if (E > 0) {
+        hipLaunchKernelGGL(k_fill_i32, dim3(grid_for(2 * (int64_t)E, 8192)), dim3(TPB), 0, s, (int *)G->pos_in,
+                           2 * (int64_t)E, -1);
+        if (G->S > 0)
+            hipLaunchKernelGGL(k_edge_positions, dim3(grid_for(G->S, 8192)), dim3(TPB), 0, s, G->adj_e, G->S,
+                               G->pos_in);
+        hipLaunchKernelGGL(k_order_positions, dim3(grid_for(E, 8192)), dim3(TPB), 0, s, G->pos_in, E);
         hipLaunchKernelGGL((k_isect<GS, false>), dim3(grid), dim3(TPB), 0, s, G->U, G->V, E, G->row_off,
-                           G->adj_x, cnt, (const int64_t *)nullptr, (int2 *)nullptr);
-    bool ok = prim_exclusive_scan_i32_to_i64(cnt, G->ioff, (size_t)E + 1, s);
-    dfree(cnt);
-    (void)cnt64;
+                           G->adj_x, (const int *)nullptr, G->cnt_in, (const int64_t *)nullptr,
+                           (uint32_t *)nullptr, (int2 *)nullptr);
+    }
+    // 2. internal order: by bin, then by first half-edge (CSR order)
+    int counts[Bins::NB] = {0, 0, 0, 0, 0};
+    if (E > 0) {
+        uint64_t *k0 = nullptr, *k1 = nullptr;
+        int *v0 = nullptr, *cd = nullptr;
+        bool ok2 = dalloc(&k0, (size_t)E + 1, "order keys") && dalloc(&k1, (size_t)E + 1, "order keys") &&
+                   dalloc(&v0, (size_t)E + 1, "order vals") && dalloc(&cd, Bins::NB, "bin counts");
+        if (ok2) {
+            hipLaunchKernelGGL(k_order_keys, dim3(grid_for(E, 8192)), dim3(TPB), 0, s, G->cnt_in, G->pos_in, E, k0, v0);
+            ok2 = prim_sort_pairs_u64_i32(k0, k1, v0, G->ord, (size_t)E, 64, s);
+        }
+        ok2 = ok2 && hipMemsetAsync(cd, 0, Bins::NB * sizeof(int), s) == hipSuccess;
+        if (ok2)
+            hipLaunchKernelGGL(k_count_bins, dim3(grid_for(E, 2048)), dim3(TPB), 0, s, G->ord, G->cnt_in, E, cd);
+        ok2 = ok2 && hipMemcpyAsync(counts, cd, sizeof(counts), hipMemcpyDeviceToHost, s) == hipSuccess &&
+              hipStreamSynchronize(s) == hipSuccess;
+        dfree(k0); dfree(k1); dfree(v0); dfree(cd);
+        if (!ok2) {
+            if (!has_error()) set_error("edge ordering failed");
+            dfree(cnt_i);
+            return false;
+        }
+        hipLaunchKernelGGL(k_internal_edges, dim3(grid_for(E, 8192)), dim3(TPB), 0, s, G->ord, E, G->U, G->V,
+                           G->row_off, G->ew_in, G->c4_in, G->cnt_in, G->pos_in, G->inv, G->uv, G->rs, G->pos,
+                           G->ew, G->c4, cnt_i);
+    }
+    G->ebin[0] = 0;
+    for (int b = 0; b < Bins::NB; b++) G->ebin[b + 1] = G->ebin[b] + counts[b];
+    hipLaunchKernelGGL(k_fill_i32, dim3(1), dim3(64), 0, s, (int *)(cnt_i + E), (int64_t)2, 0);
+    bool ok = prim_exclusive_scan_i64(cnt_i, G->ioff, (size_t)E + 1, s);
+    dfree(cnt_i);
     if (!ok) return false;
     int64_t T = 0;
     DH_CHECK(hipMemcpyAsync(&T, G->ioff + E, sizeof(int64_t), hipMemcpyDeviceToHost, s));
     DH_CHECK(hipStreamSynchronize(s));
     G->T = T;
-    if (!dalloc(&G->ipair, (size_t)T + 1, "intersection index")) return false;
+    // 3. fill, in internal order
+    G->packed = G->max_degree <= 65535;
+    if (G->packed) {
+        if (!dalloc(&G->ipk, (size_t)T + 1, "intersection index")) return false;
+    } else {
+        if (!dalloc(&G->ipair, (size_t)T + 1, "intersection index")) return false;
+    }
     if (E > 0 && T > 0)
         hipLaunchKernelGGL((k_isect<GS, true>), dim3(grid), dim3(TPB), 0, s, G->U, G->V, E, G->row_off,
-                           G->adj_x, (int *)nullptr, (const int64_t *)G->ioff, G->ipair);
+                           G->adj_x, G->ord, (int64_t *)nullptr, (const int64_t *)G->ioff, G->ipk, G->ipair);
     DH_CHECK(hipGetLastError());
     DH_CHECK(hipStreamSynchronize(s));
     G->have_index = true;
@@ -421,9 +554,28 @@ bool build_bins(DevGraph *G)
 {
     if (G->have_bins) return true;
     if (!build_index(G)) return false;
-    if (!make_bins(G, &G->vbins, G->N, true)) return false;
-    if (!make_bins(G, &G->ebins, G->E, false)) return false;
+    if (!dalloc(&G->vbins.list, (size_t)G->N + 1, "vertex bins")) return false;
+    int counts[Bins::NB] = {0, 0, 0, 0, 0};
+    if (!vertex_bins(G->stream, G->N, G->row_off, G->vbins.list, counts)) return false;
+    int st = 0;
+    for (int b = 0; b < Bins::NB; b++) { G->vbins.start[b] = st; G->vbins.count[b] = counts[b]; st += counts[b]; }
     G->have_bins = true;
+    return true;
+}
+
+bool refresh_self_terms(DevGraph *G, const double *NW_host)
+{
+    hipStream_t s = G->stream;
+    if (NW_host && G->N > 0)
+        DH_CHECK(hipMemcpyAsync(G->nw, NW_host, (size_t)G->N * sizeof(double), hipMemcpyHostToDevice, s));
+    if (G->E > 0) {
+        hipLaunchKernelGGL(k_edge_consts, dim3(grid_for(G->E, 8192)), dim3(TPB), 0, s, G->U, G->V, G->nw,
+                           G->E, G->variant, G->c4_in);
+        if (G->have_index)
+            hipLaunchKernelGGL(k_self_terms, dim3(grid_for(G->E, 8192)), dim3(TPB), 0, s, G->ord, G->E, G->c4_in,
+                               G->c4);
+    }
+    DH_CHECK(hipGetLastError());
     return true;
 }
 
@@ -437,43 +589,40 @@ bool download_csr(DevGraph *G, int *adj_offset, int *adj_target, int *adj_edge_i
         DH_CHECK(hipMemcpyAsync(adj_edge_idx, G->adj_e, (size_t)G->S * sizeof(int), hipMemcpyDeviceToHost, s));
     }
     if (G->E > 0)
-        DH_CHECK(hipMemcpyAsync(edge_weight, G->ew, (size_t)G->E * sizeof(double), hipMemcpyDeviceToHost, s));
+        DH_CHECK(hipMemcpyAsync(edge_weight, G->ew_in, (size_t)G->E * sizeof(double), hipMemcpyDeviceToHost, s));
     DH_CHECK(hipStreamSynchronize(s));
     return true;
 }
 
-namespace {
-__global__ void k_index_to_edges(const int2 *ipair, int64_t T, const int *adj_e, int *ux, int *vx)
-{
-    for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < T;
-         i += (int64_t)gridDim.x * blockDim.x) {
-        int2 p = ipair[i];
-        ux[i] = adj_e[p.x];
-        vx[i] = adj_e[p.y];
-    }
-}
-}  // namespace
-
-// Intercept arrays in the reference's form (edge ids, int offsets), malloc'd.
+// Intercept arrays in the reference's form (input edge ids, int offsets in
+// input order), malloc'd.
 bool download_index(DevGraph *G, int **ioff_out, int **ux_out, int **vx_out)
 {
     if (!build_index(G)) return false;
     if (G->T >= (int64_t)INT32_MAX) { set_error("intersection index exceeds int32 offsets of dress_graph_t"); return false; }
     hipStream_t s = G->stream;
-    int E = G->E;
-    int64_t T = G->T;
+    const int E = G->E;
+    const int64_t T = G->T;
     int *io = (int *)malloc(((size_t)E + 1) * sizeof(int));
     int *ux = (int *)malloc(((size_t)T + 1) * sizeof(int));
     int *vx = (int *)malloc(((size_t)T + 1) * sizeof(int));
     int64_t *io64 = (int64_t *)malloc(((size_t)E + 1) * sizeof(int64_t));
     int *dux = nullptr, *dvx = nullptr;
-    bool ok = io && ux && vx && io64 && dalloc(&dux, (size_t)T + 1, "ux") && dalloc(&dvx, (size_t)T + 1, "vx");
+    int64_t *dio = nullptr, *dcnt = nullptr;
     if (!io || !ux || !vx || !io64) set_error("host allocation failed for intercept arrays");
+    bool ok = io && ux && vx && io64 && dalloc(&dux, (size_t)T + 1, "ux") && dalloc(&dvx, (size_t)T + 1, "vx") &&
+              dalloc(&dio, (size_t)E + 1, "input offsets") && dalloc(&dcnt, (size_t)E + 1, "input counts");
     if (ok) {
-        if (T > 0)
-            hipLaunchKernelGGL(k_index_to_edges, dim3(grid_for(T, 8192)), dim3(TPB), 0, s, G->ipair, T,
-                               G->adj_e, dux, dvx);
-        hipError_t e1 = hipMemcpyAsync(io64, G->ioff, ((size_t)E + 1) * sizeof(int64_t), hipMemcpyDeviceToHost, s);
+        hipError_t e0 = hipMemcpyAsync(dcnt, G->cnt_in, (size_t)E * sizeof(int64_t), hipMemcpyDeviceToDevice, s);
+        if (e0 != hipSuccess) { set_error("intercept copy failed"); ok = false; }
+        hipLaunchKernelGGL(k_fill_i32, dim3(1), dim3(64), 0, s, (int *)(dcnt + E), (int64_t)2, 0);
+        ok = ok && prim_exclusive_scan_i64(dcnt, dio, (size_t)E + 1, s);
+    }
+    if (ok) {
+        if (E > 0)
+            hipLaunchKernelGGL(k_index_to_reference, dim3(grid_for(E, 8192)), dim3(TPB), 0, s, G->ord, E, G->ioff,
+                               G->rs, G->ipk, G->ipair, G->adj_e, dio, dux, dvx);
+        hipError_t e1 = hipMemcpyAsync(io64, dio, ((size_t)E + 1) * sizeof(int64_t), hipMemcpyDeviceToHost, s);
         hipError_t e2 = T > 0 ? hipMemcpyAsync(ux, dux, (size_t)T * sizeof(int), hipMemcpyDeviceToHost, s) : hipSuccess;
         hipError_t e3 = T > 0 ? hipMemcpyAsync(vx, dvx, (size_t)T * sizeof(int), hipMemcpyDeviceToHost, s) : hipSuccess;
         hipError_t e4 = hipStreamSynchronize(s);
@@ -484,7 +633,7 @@ bool download_index(DevGraph *G, int **ioff_out, int **ux_out, int **vx_out)
     }
     if (ok) for (int e = 0; e <= E; e++) io[e] = (int)io64[e];
     free(io64);
-    dfree(dux); dfree(dvx);
+    dfree(dux); dfree(dvx); dfree(dio); dfree(dcnt);
     if (!ok) { free(io); free(ux); free(vx); return false; }
     *ioff_out = io; *ux_out = ux; *vx_out = vx;
     return true;
@@ -495,10 +644,12 @@ void free_graph(DevGraph *G)
     if (!G) return;
     if (G->stream) hipStreamSynchronize(G->stream);
     dfree(G->U); dfree(G->V); dfree(G->row_off); dfree(G->adj_x); dfree(G->adj_e);
-    dfree(G->ew); dfree(G->nw); dfree(G->c4); dfree(G->pos); dfree(G->ioff); dfree(G->ipair);
-    dfree(G->vbins.list); dfree(G->ebins.list);
+    dfree(G->ew_in); dfree(G->c4_in); dfree(G->nw); dfree(G->pos_in);
+    dfree(G->ord); dfree(G->inv); dfree(G->uv); dfree(G->rs); dfree(G->pos); dfree(G->ew); dfree(G->c4);
+    dfree(G->ioff); dfree(G->ipk); dfree(G->ipair); dfree(G->cnt_in);
+    dfree(G->vbins.list);
     for (int i = 0; i < 2; i++) { dfree(G->d[i]); dfree(G->wh[i]); dfree(G->d32[i]); dfree(G->wh32[i]); }
-    dfree(G->vd); dfree(G->state);
+    dfree(G->vd); dfree(G->stage); dfree(G->state);
     if (G->stream) hipStreamDestroy(G->stream);
     delete G;
 }

@@ -138,28 +138,17 @@ using namespace dh;
 
 namespace {
 
-__global__ void k_self_consts(const int *U, const int *V, const double *nw, int E, int directed,
-                              double *c4)
+__global__ void k_bin_terms(const int64_t *cnt, int E, unsigned long long *out)
 {
+    __shared__ unsigned long long acc[Bins::NB];
+    if (threadIdx.x < Bins::NB) acc[threadIdx.x] = 0;
+    __syncthreads();
     for (int e = blockIdx.x * blockDim.x + threadIdx.x; e < E; e += gridDim.x * blockDim.x) {
-        double a = 4.0 * nw[U[e]];
-        c4[e] = directed ? a : a + 4.0 * nw[V[e]];
+        int64_t c = cnt[e];
+        atomicAdd(&acc[term_bin(c + 1)], (unsigned long long)c);
     }
-}
-
-// refresh vertex weights from the host struct (the caller may edit g->NW)
-bool refresh_vertex_weights(DevGraph *G, p_dress_graph_t g)
-{
-    if (!g->NW || G->N == 0) return true;
-    DH_CHECK(hipMemcpyAsync(G->nw, g->NW, (size_t)G->N * sizeof(double), hipMemcpyHostToDevice, G->stream));
-    int directed = (G->variant == DRESS_VARIANT_FORWARD || G->variant == DRESS_VARIANT_BACKWARD);
-    if (G->E > 0) {
-        int grid = (int)std::min<int64_t>(((int64_t)G->E + 255) / 256, 8192);
-        hipLaunchKernelGGL(k_self_consts, dim3(grid), dim3(256), 0, G->stream, G->U, G->V, G->nw, G->E,
-                           directed, G->c4);
-    }
-    DH_CHECK(hipGetLastError());
-    return true;
+    __syncthreads();
+    if (threadIdx.x < Bins::NB) atomicAdd(&out[threadIdx.x], acc[threadIdx.x]);
 }
 
 int fit_impl(p_dress_graph_t g, int max_iterations, double epsilon, int precision, int *iterations,
@@ -175,7 +164,7 @@ int fit_impl(p_dress_graph_t g, int max_iterations, double epsilon, int precisio
     DevGraph *G = device_graph(g);
     if (!G) return -1;
     FitResult r;
-    if (!refresh_vertex_weights(G, g) || !fit_upload_values(G, g->edge_dress) ||
+    if (!refresh_self_terms(G, g->NW) || !fit_upload_values(G, g->edge_dress) ||
         !fit_run(G, max_iterations, epsilon, precision, &r) ||
         !fit_download(G, g->edge_dress, g->vertex_dress))
         return -1;
@@ -312,13 +301,40 @@ int dress_fit_hip_ex(p_dress_graph_t g, int max_iterations, double epsilon,
     return r;
 }
 
+int dress_hip_graph_info(p_dress_graph_t g, int64_t *info, int n)
+{
+    clear_error();
+    DevGraph *G = g ? device_graph(g) : nullptr;
+    if (!G || !build_bins(G)) return -1;
+    int64_t v[32] = {0};
+    v[0] = G->N; v[1] = G->E; v[2] = G->S; v[3] = G->T; v[4] = G->max_degree;
+    v[5] = G->packed ? 1 : 0;
+    for (int b = 0; b < Bins::NB; b++) {
+        v[6 + b] = G->ebin[b + 1] - G->ebin[b];
+        v[11 + b] = G->vbins.count[b];
+    }
+    unsigned long long *bt = nullptr;
+    if (!dalloc(&bt, Bins::NB, "bin terms")) return -1;
+    unsigned long long hb[Bins::NB] = {0, 0, 0, 0, 0};
+    bool ok = hipMemsetAsync(bt, 0, sizeof(hb), G->stream) == hipSuccess;
+    if (ok && G->E > 0)
+        hipLaunchKernelGGL(k_bin_terms, dim3(1024), dim3(256), 0, G->stream, G->cnt_in, G->E, bt);
+    ok = ok && hipMemcpyAsync(hb, bt, sizeof(hb), hipMemcpyDeviceToHost, G->stream) == hipSuccess &&
+         hipStreamSynchronize(G->stream) == hipSuccess;
+    dfree(bt);
+    if (!ok) { set_error("dress_hip_graph_info failed"); return -1; }
+    for (int b = 0; b < Bins::NB; b++) v[16 + b] = (int64_t)hb[b];
+    for (int i = 0; i < n && i < 21; i++) info[i] = v[i];
+    return 0;
+}
+
 int dress_hip_resident_reset(p_dress_graph_t g)
 {
     clear_error();
     if (!g) { set_error("NULL graph"); return -1; }
     DevGraph *G = device_graph(g);
     if (!G) return -1;
-    if (!refresh_vertex_weights(G, g) || !fit_upload_values(G, g->edge_dress) || !build_bins(G)) return -1;
+    if (!refresh_self_terms(G, g->NW) || !fit_upload_values(G, g->edge_dress) || !build_bins(G)) return -1;
     if (hipStreamSynchronize(G->stream) != hipSuccess) { set_error("stream sync failed"); return -1; }
     return 0;
 }

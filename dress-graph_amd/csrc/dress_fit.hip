@@ -1,13 +1,17 @@
 // dress_fit.hip -- the DRESS fixed-point iteration on one device-resident graph.
 //
 // One iteration (libdress/src/dress.c:546-608) is three kernels:
-//   vertex phase   vd[u] = sqrt( sum{4nw[u]} + sum_{i in row u} wh[i] )
+//   vertex phase   vd[u] = sqrt( 4nw[u] + sum_{s in row u} wh[s] )
 //   edge phase     d'[e] = ( sum_k (wh[a_k] + wh[b_k]) + t0_e ) / (vd[U]*vd[V])
-//                  t0_e = c4[e] + 2*ew[e]*d[e]   (c4[e] + ew[e]*d[e] FWD/BWD)
-//                  with max|d - d'| folded into one device word, and
-//                  wh'[pos(e)] = ew[e]*d'[e] scattered to the edge's half-edges
+//                  t0_e = c4[e] + 2*w[e]*d[e]   (c4[e] + w[e]*d[e] FWD/BWD)
+//                  with max|d - d'| folded into one device word, and the new
+//                  w[e]*d'[e] written onto the edge's half-edges
 //   end            convergence bookkeeping on the device (strict < epsilon)
-// so the host only checks the state every few iterations.
+// so the host only reads the state every few iterations.  wh[] holds one
+// copy of w-bar*d per CSR entry: the vertex phase streams rows, and the
+// common-neighbour terms of an edge are read at increasing positions of its
+// two rows, so neighbouring lanes share cache lines.  Index and per-edge
+// streams use non-temporal loads.
 //
 // Sums are exact-rounding compensated sums (TwoSum per term, pairwise
 // TwoSum merges across lanes): the result is the correctly rounded sum of
@@ -19,7 +23,8 @@
 // -ffp-contract=off: the reference computes w*d + w*d with two roundings.
 //
 // Work is degree binned: rows/edges with <= 4, 32, 256, 4096 terms use groups
-// of 1, 4, 16, 64 lanes; larger ones a 256-lane workgroup.
+// of 1, 4, 16, 64 lanes; larger ones a 256-lane workgroup.  Edges of one bin
+// are a contiguous range of internal ids.
 #include "dress_hip_internal.h"
 
 #include <algorithm>
@@ -100,165 +105,141 @@ __device__ __forceinline__ void wave_max_atomic(double m, unsigned long long *ou
         atomicMax(out, (unsigned long long)__double_as_longlong(m));
 }
 
-// ------------------------------------------------------------- FP64 phase
+template <typename T>
+__device__ __forceinline__ T ntl(const T *p) { return __builtin_nontemporal_load(p); }
 
-template <int G>
+typedef int v2i __attribute__((ext_vector_type(2)));
+__device__ __forceinline__ int2 ntl2(const int2 *p)
+{
+    v2i v = __builtin_nontemporal_load(reinterpret_cast<const v2i *>(p));
+    int2 r;
+    r.x = v.x;
+    r.y = v.y;
+    return r;
+}
+
+struct EdgeArgs {
+    const int2 *uv;
+    const int2 *rs;
+    const int2 *pos;
+    const double *ew;
+    const double *c4;
+    const int64_t *ioff;
+    const uint32_t *ipk;
+    const int2 *ipair;
+    const double *vd;
+    int directed;
+};
+
+// half-edge pair of index entry k of an edge whose rows start at r
+template <bool PK>
+__device__ __forceinline__ void entry(const EdgeArgs &a, int64_t k, int2 r, int &x, int &y)
+{
+    if (PK) {
+        uint32_t q = ntl(a.ipk + k);
+        x = r.x + (int)(q & 0xffffu);
+        y = r.y + (int)(q >> 16);
+    } else {
+        int2 p = ntl2(a.ipair + k);
+        x = p.x;
+        y = p.y;
+    }
+}
+
+// Vertex phase: vd[u] = sqrt(4nw[u] + sum of w-bar*d over row u).
+// VT = double (FP64 phase) or float (FP32 phase: values stored in FP32,
+// summed in FP64, exact for FP32 terms within 2^29 of each other).
+template <int G, typename VT>
 __global__ void __launch_bounds__(TPB) k_vertex(const int *__restrict__ list, int count,
-                                                const int *__restrict__ row_off,
-                                                const double *__restrict__ wh0,
-                                                const double *__restrict__ wh1,
+                                                const int *__restrict__ row_off, const VT *w0, const VT *w1,
                                                 const double *__restrict__ nw, double *__restrict__ vd,
                                                 const FitState *__restrict__ st)
 {
-    if (st->done || st->fp32) return;
-    const double *__restrict__ wh = st->cur ? wh1 : wh0;
+    const bool f32 = sizeof(VT) == 4;
+    if (st->done || (st->fp32 != 0) != f32) return;
+    const VT *__restrict__ wh = st->cur ? w1 : w0;
     const int lane = (G == TPB) ? threadIdx.x : (threadIdx.x % G);
     const int64_t groups = (int64_t)gridDim.x * (TPB / G);
     for (int64_t g = (int64_t)blockIdx.x * (TPB / G) + threadIdx.x / G; g < count; g += groups) {
-        int u = list[g];
+        int u = ntl(list + g);
         int b = row_off[u], e = row_off[u + 1];
         double s = 0.0, c = 0.0;
         if (lane == 0) s = 4.0 * nw[u];
-        for (int i = b + lane; i < e; i += G) two_sum_add(s, c, wh[i]);
+        int i = b + lane;
+        for (; i + G < e; i += 2 * G) {
+            double x0 = (double)ntl(wh + i), x1 = (double)ntl(wh + i + G);
+            two_sum_add(s, c, x0);
+            two_sum_add(s, c, x1);
+        }
+        for (; i < e; i += G) two_sum_add(s, c, (double)ntl(wh + i));
         if (G == TPB) block_reduce(s, c);
         else group_reduce<G>(s, c);
         if (lane == 0) vd[u] = sqrt(s + c);
     }
 }
 
-template <int G>
-__global__ void __launch_bounds__(TPB) k_edge(const int *__restrict__ list, int count,
-                                              const int *__restrict__ U, const int *__restrict__ V,
-                                              const double *__restrict__ ew,
-                                              const double *__restrict__ c4,
-                                              const int2 *__restrict__ pos,
-                                              const int64_t *__restrict__ ioff,
-                                              const int2 *__restrict__ ipair,
-                                              double *d0, double *d1, double *wh0, double *wh1,
-                                              const double *__restrict__ vd, int directed,
+// Edge phase over internal edges [lo, hi).
+template <int G, bool PK, typename VT>
+__global__ void __launch_bounds__(TPB) k_edge(int lo, int hi, EdgeArgs a, VT *d0, VT *d1, VT *w0, VT *w1,
                                               FitState *st)
 {
-    if (st->done || st->fp32) return;
+    const bool f32 = sizeof(VT) == 4;
+    if (st->done || (st->fp32 != 0) != f32) return;
     const int cur = st->cur;
-    const double *__restrict__ dc = cur ? d1 : d0;
-    const double *__restrict__ wc = cur ? wh1 : wh0;
-    double *__restrict__ dn = cur ? d0 : d1;
-    double *__restrict__ wn = cur ? wh0 : wh1;
+    const VT *__restrict__ dc = cur ? d1 : d0;
+    const VT *__restrict__ wc = cur ? w1 : w0;
+    VT *__restrict__ dn = cur ? d0 : d1;
+    VT *__restrict__ wn = cur ? w0 : w1;
     const int lane = (G == TPB) ? threadIdx.x : (threadIdx.x % G);
     const int64_t groups = (int64_t)gridDim.x * (TPB / G);
     double mymax = 0.0;
-    for (int64_t g = (int64_t)blockIdx.x * (TPB / G) + threadIdx.x / G; g < count; g += groups) {
-        int e = list[g];
-        int64_t kb = ioff[e], ke = ioff[e + 1];
+    for (int64_t g = lo + (int64_t)blockIdx.x * (TPB / G) + threadIdx.x / G; g < hi; g += groups) {
+        const int e = (int)g;
+        int64_t kb = ntl(a.ioff + e), ke = ntl(a.ioff + e + 1);
+        int2 r = PK ? ntl2(a.rs + e) : make_int2(0, 0);
         double s = 0.0, c = 0.0;
-        for (int64_t k = kb + lane; k < ke; k += G) {
-            int2 p = ipair[k];
-            double t = wc[p.x] + wc[p.y];
-            two_sum_add(s, c, t);
-        }
-        double de = 0.0, w = 0.0;
-        if (lane == 0) {
-            de = dc[e];
-            w = ew[e];
-            double own = w * de;
-            double t0 = directed ? (c4[e] + own) : (c4[e] + 2.0 * own);
+        int64_t k = kb + lane;
+        // four index entries and eight gathers in flight per lane
+        for (; k + 3 * G < ke; k += 4 * G) {
+            int x0, y0, x1, y1, x2, y2, x3, y3;
+            entry<PK>(a, k, r, x0, y0);
+            entry<PK>(a, k + G, r, x1, y1);
+            entry<PK>(a, k + 2 * G, r, x2, y2);
+            entry<PK>(a, k + 3 * G, r, x3, y3);
+            double t0 = (double)wc[x0] + (double)wc[y0];
+            double t1 = (double)wc[x1] + (double)wc[y1];
+            double t2 = (double)wc[x2] + (double)wc[y2];
+            double t3 = (double)wc[x3] + (double)wc[y3];
             two_sum_add(s, c, t0);
+            two_sum_add(s, c, t1);
+            two_sum_add(s, c, t2);
+            two_sum_add(s, c, t3);
         }
-        if (G == TPB) block_reduce(s, c);
-        else group_reduce<G>(s, c);
-        if (lane == 0) {
-            double num = s + c;
-            double den = vd[U[e]] * vd[V[e]];
-            double x = den > 0.0 ? num / den : 0.0;
-            mymax = fmax(mymax, fabs(de - x));
-            dn[e] = x;
-            double wx = w * x;
-            int2 q = pos[e];
-            wn[q.x] = wx;
-            if (q.y >= 0) wn[q.y] = wx;
-        }
-    }
-    wave_max_atomic(mymax, &st->maxd);
-}
-
-// ------------------------------------------------------------- FP32 phase
-// Values and half-edge copies stored in FP32 (half the gather bytes); terms
-// are added in FP64, which is exact for FP32 inputs within 2^29 of each
-// other, so the phase stays order independent; norms, division and the
-// change are computed in FP64 and the new value rounded to FP32.
-
-template <int G>
-__global__ void __launch_bounds__(TPB) k_vertex32(const int *__restrict__ list, int count,
-                                                  const int *__restrict__ row_off,
-                                                  const float *__restrict__ wh0,
-                                                  const float *__restrict__ wh1,
-                                                  const double *__restrict__ nw, double *__restrict__ vd,
-                                                  const FitState *__restrict__ st)
-{
-    if (st->done || !st->fp32) return;
-    const float *__restrict__ wh = st->cur ? wh1 : wh0;
-    const int lane = (G == TPB) ? threadIdx.x : (threadIdx.x % G);
-    const int64_t groups = (int64_t)gridDim.x * (TPB / G);
-    for (int64_t g = (int64_t)blockIdx.x * (TPB / G) + threadIdx.x / G; g < count; g += groups) {
-        int u = list[g];
-        int b = row_off[u], e = row_off[u + 1];
-        double s = 0.0, c = 0.0;
-        if (lane == 0) s = 4.0 * nw[u];
-        for (int i = b + lane; i < e; i += G) two_sum_add(s, c, (double)wh[i]);
-        if (G == TPB) block_reduce(s, c);
-        else group_reduce<G>(s, c);
-        if (lane == 0) vd[u] = sqrt(s + c);
-    }
-}
-
-template <int G>
-__global__ void __launch_bounds__(TPB) k_edge32(const int *__restrict__ list, int count,
-                                                const int *__restrict__ U, const int *__restrict__ V,
-                                                const double *__restrict__ ew,
-                                                const double *__restrict__ c4,
-                                                const int2 *__restrict__ pos,
-                                                const int64_t *__restrict__ ioff,
-                                                const int2 *__restrict__ ipair,
-                                                float *d0, float *d1, float *wh0, float *wh1,
-                                                const double *__restrict__ vd, int directed,
-                                                FitState *st)
-{
-    if (st->done || !st->fp32) return;
-    const int cur = st->cur;
-    const float *__restrict__ dc = cur ? d1 : d0;
-    const float *__restrict__ wc = cur ? wh1 : wh0;
-    float *__restrict__ dn = cur ? d0 : d1;
-    float *__restrict__ wn = cur ? wh0 : wh1;
-    const int lane = (G == TPB) ? threadIdx.x : (threadIdx.x % G);
-    const int64_t groups = (int64_t)gridDim.x * (TPB / G);
-    double mymax = 0.0;
-    for (int64_t g = (int64_t)blockIdx.x * (TPB / G) + threadIdx.x / G; g < count; g += groups) {
-        int e = list[g];
-        int64_t kb = ioff[e], ke = ioff[e + 1];
-        double s = 0.0, c = 0.0;
-        for (int64_t k = kb + lane; k < ke; k += G) {
-            int2 p = ipair[k];
-            two_sum_add(s, c, (double)wc[p.x] + (double)wc[p.y]);
+        for (; k < ke; k += G) {
+            int x, y;
+            entry<PK>(a, k, r, x, y);
+            two_sum_add(s, c, (double)wc[x] + (double)wc[y]);
         }
         double de = 0.0, w = 0.0;
         if (lane == 0) {
             de = (double)dc[e];
-            w = ew[e];
+            w = ntl(a.ew + e);
             double own = w * de;
-            double t0 = directed ? (c4[e] + own) : (c4[e] + 2.0 * own);
+            double c4 = ntl(a.c4 + e);
+            double t0 = a.directed ? (c4 + own) : (c4 + 2.0 * own);
             two_sum_add(s, c, t0);
         }
         if (G == TPB) block_reduce(s, c);
         else group_reduce<G>(s, c);
         if (lane == 0) {
             double num = s + c;
-            double den = vd[U[e]] * vd[V[e]];
+            int2 p = ntl2(a.uv + e);
+            double den = a.vd[p.x] * a.vd[p.y];
             double x = den > 0.0 ? num / den : 0.0;
             mymax = fmax(mymax, fabs(de - x));
-            float xf = (float)x;
-            dn[e] = xf;
-            float wx = (float)(w * x);
-            int2 q = pos[e];
+            dn[e] = (VT)x;
+            VT wx = (VT)(w * x);
+            int2 q = ntl2(a.pos + e);
             wn[q.x] = wx;
             if (q.y >= 0) wn[q.y] = wx;
         }
@@ -268,53 +249,57 @@ __global__ void __launch_bounds__(TPB) k_edge32(const int *__restrict__ list, in
 
 // ------------------------------------------------------------- glue kernels
 
-// wh[cur] = ew * d[cur] on every half-edge
-__global__ void k_init_wh(int E, const double *ew, const int2 *pos, double *d0, double *d1,
-                          double *wh0, double *wh1, const FitState *st)
+// current buffer from input-order staging: d[i] = stage[ord[i]], and the
+// half-edge copies wh = w-bar*d
+__global__ void k_gather_in(int E, const int *ord, const double *stage, double *d0, double *d1,
+                            const FitState *st)
+{
+    double *d = st->cur ? d1 : d0;
+    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < E; i += gridDim.x * blockDim.x)
+        d[i] = stage[ord[i]];
+}
+
+// stage[ord[i]] = d[cur][i] (input order out)
+__global__ void k_scatter_out(int E, const int *ord, const double *d, double *stage)
+{
+    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < E; i += gridDim.x * blockDim.x)
+        stage[ord[i]] = d[i];
+}
+
+// half-edge copies of the current values
+template <typename VT>
+__global__ void k_init_wh(int E, const double *ew, const int2 *pos, const double *d0, const double *d1,
+                          VT *f0, VT *f1, VT *w0, VT *w1, const FitState *st)
 {
     const double *d = st->cur ? d1 : d0;
-    double *wh = st->cur ? wh1 : wh0;
+    VT *f = st->cur ? f1 : f0;
+    VT *w = st->cur ? w1 : w0;
     for (int e = blockIdx.x * blockDim.x + threadIdx.x; e < E; e += gridDim.x * blockDim.x) {
-        double x = ew[e] * d[e];
+        double x = d[e];
+        if (f) f[e] = (VT)x;
+        VT wx = (VT)(ew[e] * x);
         int2 q = pos[e];
-        wh[q.x] = x;
-        if (q.y >= 0) wh[q.y] = x;
+        w[q.x] = wx;
+        if (q.y >= 0) w[q.y] = wx;
     }
 }
 
-// FP64 values -> FP32 buffers (start of a mixed-precision fit)
-__global__ void k_to32(int E, const double *ew, const int2 *pos, const double *d0, const double *d1,
-                       float *f0, float *f1, float *w0, float *w1, const FitState *st)
-{
-    const double *d = st->cur ? d1 : d0;
-    float *f = st->cur ? f1 : f0;
-    float *w = st->cur ? w1 : w0;
-    for (int e = blockIdx.x * blockDim.x + threadIdx.x; e < E; e += gridDim.x * blockDim.x) {
-        f[e] = (float)d[e];
-        float x = (float)(ew[e] * d[e]);
-        int2 q = pos[e];
-        w[q.x] = x;
-        if (q.y >= 0) w[q.y] = x;
-    }
-}
-
-// FP32 values -> FP64 buffers once the FP32 phase has converged to its
-// switch threshold; runs only in the iteration where the end kernel flagged
-// the switch (st->fp32 == 2).
-__global__ void k_to64(int E, const double *ew, const int2 *pos, double *d0, double *d1,
-                       double *wh0, double *wh1, const float *f0, const float *f1, FitState *st)
+// FP32 values -> FP64 buffers once the FP32 phase reached its hand-over
+// point; runs only in the iteration where the end kernel flagged it.
+__global__ void k_to64(int E, const double *ew, const int2 *pos, double *d0, double *d1, double *w0,
+                       double *w1, const float *f0, const float *f1, FitState *st)
 {
     if (st->done || st->fp32 != 2) return;
     const float *f = st->cur ? f1 : f0;
     double *d = st->cur ? d1 : d0;
-    double *wh = st->cur ? wh1 : wh0;
+    double *w = st->cur ? w1 : w0;
     for (int e = blockIdx.x * blockDim.x + threadIdx.x; e < E; e += gridDim.x * blockDim.x) {
         double x = (double)f[e];
         d[e] = x;
         double wx = ew[e] * x;
         int2 q = pos[e];
-        wh[q.x] = wx;
-        if (q.y >= 0) wh[q.y] = wx;
+        w[q.x] = wx;
+        if (q.y >= 0) w[q.y] = wx;
     }
 }
 
@@ -356,61 +341,65 @@ inline int grid_for(int64_t threads, int cap)
     return (int)b;
 }
 
-template <int G>
-void launch_vertex(DevGraph *Gr, int bin, int cap, bool fp32)
+template <int G, typename VT>
+void launch_vertex_t(DevGraph *Gr, int bin, int cap, VT *w0, VT *w1)
 {
     const Bins &B = Gr->vbins;
     int n = B.count[bin];
     if (!n) return;
     int grid = grid_for((int64_t)n * G, cap);
-    FitState *st = (FitState *)Gr->state;
-    if (fp32)
-        hipLaunchKernelGGL((k_vertex32<G>), dim3(grid), dim3(TPB), 0, Gr->stream, B.list + B.start[bin], n,
-                           Gr->row_off, Gr->wh32[0], Gr->wh32[1], Gr->nw, Gr->vd, st);
-    else
-        hipLaunchKernelGGL((k_vertex<G>), dim3(grid), dim3(TPB), 0, Gr->stream, B.list + B.start[bin], n,
-                           Gr->row_off, Gr->wh[0], Gr->wh[1], Gr->nw, Gr->vd, st);
+    hipLaunchKernelGGL((k_vertex<G, VT>), dim3(grid), dim3(TPB), 0, Gr->stream, B.list + B.start[bin], n,
+                       Gr->row_off, w0, w1, Gr->nw, Gr->vd, (const FitState *)Gr->state);
 }
 
-template <int G>
-void launch_edge(DevGraph *Gr, int bin, int cap, bool fp32)
+template <int G, bool PK, typename VT>
+void launch_edge_t(DevGraph *Gr, int bin, int cap, VT *d0, VT *d1, VT *w0, VT *w1)
 {
-    const Bins &B = Gr->ebins;
-    int n = B.count[bin];
-    if (!n) return;
-    int grid = grid_for((int64_t)n * G, cap);
-    FitState *st = (FitState *)Gr->state;
-    int directed = (Gr->variant == DRESS_VARIANT_FORWARD || Gr->variant == DRESS_VARIANT_BACKWARD);
-    if (fp32)
-        hipLaunchKernelGGL((k_edge32<G>), dim3(grid), dim3(TPB), 0, Gr->stream, B.list + B.start[bin], n,
-                           Gr->U, Gr->V, Gr->ew, Gr->c4, Gr->pos, Gr->ioff, Gr->ipair,
-                           Gr->d32[0], Gr->d32[1], Gr->wh32[0], Gr->wh32[1], Gr->vd, directed, st);
-    else
-        hipLaunchKernelGGL((k_edge<G>), dim3(grid), dim3(TPB), 0, Gr->stream, B.list + B.start[bin], n,
-                           Gr->U, Gr->V, Gr->ew, Gr->c4, Gr->pos, Gr->ioff, Gr->ipair,
-                           Gr->d[0], Gr->d[1], Gr->wh[0], Gr->wh[1], Gr->vd, directed, st);
+    int lo = Gr->ebin[bin], hi = Gr->ebin[bin + 1];
+    if (hi <= lo) return;
+    int grid = grid_for((int64_t)(hi - lo) * G, cap);
+    EdgeArgs a;
+    a.uv = Gr->uv; a.rs = Gr->rs; a.pos = Gr->pos; a.ew = Gr->ew; a.c4 = Gr->c4; a.ioff = Gr->ioff;
+    a.ipk = Gr->ipk; a.ipair = Gr->ipair; a.vd = Gr->vd;
+    a.directed = (Gr->variant == DRESS_VARIANT_FORWARD || Gr->variant == DRESS_VARIANT_BACKWARD);
+    hipLaunchKernelGGL((k_edge<G, PK, VT>), dim3(grid), dim3(TPB), 0, Gr->stream, lo, hi, a, d0, d1, w0, w1,
+                       (FitState *)Gr->state);
+}
+
+template <bool PK, typename VT>
+void launch_phase_t(DevGraph *Gr, VT *d0, VT *d1, VT *w0, VT *w1)
+{
+    int cap = num_cus() * 32;
+    launch_vertex_t<1, VT>(Gr, 0, cap, w0, w1);
+    launch_vertex_t<4, VT>(Gr, 1, cap, w0, w1);
+    launch_vertex_t<16, VT>(Gr, 2, cap, w0, w1);
+    launch_vertex_t<64, VT>(Gr, 3, cap, w0, w1);
+    launch_vertex_t<TPB, VT>(Gr, 4, cap, w0, w1);
+    launch_edge_t<1, PK, VT>(Gr, 0, cap, d0, d1, w0, w1);
+    launch_edge_t<4, PK, VT>(Gr, 1, cap, d0, d1, w0, w1);
+    launch_edge_t<16, PK, VT>(Gr, 2, cap, d0, d1, w0, w1);
+    launch_edge_t<64, PK, VT>(Gr, 3, cap, d0, d1, w0, w1);
+    launch_edge_t<TPB, PK, VT>(Gr, 4, cap, d0, d1, w0, w1);
 }
 
 void launch_phase(DevGraph *Gr, bool fp32)
 {
-    int cap = num_cus() * 16;
-    launch_vertex<1>(Gr, 0, cap, fp32);
-    launch_vertex<4>(Gr, 1, cap, fp32);
-    launch_vertex<16>(Gr, 2, cap, fp32);
-    launch_vertex<64>(Gr, 3, cap, fp32);
-    launch_vertex<TPB>(Gr, 4, cap, fp32);
-    launch_edge<1>(Gr, 0, cap, fp32);
-    launch_edge<4>(Gr, 1, cap, fp32);
-    launch_edge<16>(Gr, 2, cap, fp32);
-    launch_edge<64>(Gr, 3, cap, fp32);
-    launch_edge<TPB>(Gr, 4, cap, fp32);
+    if (fp32) {
+        if (Gr->packed) launch_phase_t<true, float>(Gr, Gr->d32[0], Gr->d32[1], Gr->wh32[0], Gr->wh32[1]);
+        else launch_phase_t<false, float>(Gr, Gr->d32[0], Gr->d32[1], Gr->wh32[0], Gr->wh32[1]);
+    } else {
+        if (Gr->packed) launch_phase_t<true, double>(Gr, Gr->d[0], Gr->d[1], Gr->wh[0], Gr->wh[1]);
+        else launch_phase_t<false, double>(Gr, Gr->d[0], Gr->d[1], Gr->wh[0], Gr->wh[1]);
+    }
 }
 
 }  // namespace
 
 bool fit_alloc_state(DevGraph *G, bool fp32)
 {
+    if (!build_bins(G)) return false;
     if (!G->state && !dmalloc(&G->state, sizeof(FitState), "fit state")) return false;
+    if (!G->stage && !dalloc(&G->stage, (size_t)G->E + 1, "staging")) return false;
     for (int i = 0; i < 2; i++) {
         if (!G->d[i] && !dalloc(&G->d[i], (size_t)G->E + 1, "edge values")) return false;
         if (!G->wh[i] && !dalloc(&G->wh[i], (size_t)G->S + 1, "half-edge values")) return false;
@@ -425,12 +414,22 @@ bool fit_alloc_state(DevGraph *G, bool fp32)
     return true;
 }
 
+// host edge values (input order) -> current device buffer (internal order)
 bool fit_upload_values(DevGraph *G, const double *edge_dress)
 {
     if (!fit_alloc_state(G, false)) return false;
     G->cur = 0;
-    if (G->E > 0)
-        DH_CHECK(hipMemcpyAsync(G->d[0], edge_dress, (size_t)G->E * sizeof(double), hipMemcpyHostToDevice, G->stream));
+    if (G->E == 0) return true;
+    hipStream_t s = G->stream;
+    DH_CHECK(hipMemcpyAsync(G->stage, edge_dress, (size_t)G->E * sizeof(double), hipMemcpyHostToDevice, s));
+    FitState h;
+    memset(&h, 0, sizeof(h));
+    h.cur = 0;
+    DH_CHECK(hipMemcpyAsync(G->state, &h, sizeof(h), hipMemcpyHostToDevice, s));
+    hipLaunchKernelGGL(k_gather_in, dim3(grid_for(G->E, num_cus() * 8)), dim3(TPB), 0, s, G->E, G->ord, G->stage,
+                       G->d[0], G->d[1], (const FitState *)G->state);
+    DH_CHECK(hipGetLastError());
+    DH_CHECK(hipStreamSynchronize(s));
     return true;
 }
 
@@ -460,12 +459,14 @@ bool fit_run_once(DevGraph *G, int max_iterations, double epsilon, bool mixed, F
     DH_CHECK(hipEventCreate(&ev1));
     int egrid = grid_for(G->E, num_cus() * 8);
     DH_CHECK(hipEventRecord(ev0, s));
-    if (G->E > 0)
-        hipLaunchKernelGGL(k_init_wh, dim3(egrid), dim3(TPB), 0, s, G->E, G->ew, G->pos, G->d[0], G->d[1],
-                           G->wh[0], G->wh[1], st);
-    if (mixed && G->E > 0)
-        hipLaunchKernelGGL(k_to32, dim3(egrid), dim3(TPB), 0, s, G->E, G->ew, G->pos, G->d[0], G->d[1],
-                           G->d32[0], G->d32[1], G->wh32[0], G->wh32[1], st);
+    if (G->E > 0) {
+        if (mixed)
+            hipLaunchKernelGGL((k_init_wh<float>), dim3(egrid), dim3(TPB), 0, s, G->E, G->ew, G->pos, G->d[0],
+                               G->d[1], G->d32[0], G->d32[1], G->wh32[0], G->wh32[1], st);
+        else
+            hipLaunchKernelGGL((k_init_wh<double>), dim3(egrid), dim3(TPB), 0, s, G->E, G->ew, G->pos, G->d[0],
+                               G->d[1], (double *)nullptr, (double *)nullptr, G->wh[0], G->wh[1], st);
+    }
     int launched = 0;
     int chunk = 4;
     while (launched < max_iterations) {
@@ -552,8 +553,11 @@ bool fit_run(DevGraph *G, int max_iterations, double epsilon, int precision, Fit
 bool fit_download(DevGraph *G, double *edge_dress, double *vertex_dress)
 {
     hipStream_t s = G->stream;
-    if (edge_dress && G->E > 0)
-        DH_CHECK(hipMemcpyAsync(edge_dress, G->d[G->cur], (size_t)G->E * sizeof(double), hipMemcpyDeviceToHost, s));
+    if (edge_dress && G->E > 0) {
+        hipLaunchKernelGGL(k_scatter_out, dim3(grid_for(G->E, num_cus() * 8)), dim3(TPB), 0, s, G->E, G->ord,
+                           G->d[G->cur], G->stage);
+        DH_CHECK(hipMemcpyAsync(edge_dress, G->stage, (size_t)G->E * sizeof(double), hipMemcpyDeviceToHost, s));
+    }
     if (vertex_dress && G->N > 0)
         DH_CHECK(hipMemcpyAsync(vertex_dress, G->vd, (size_t)G->N * sizeof(double), hipMemcpyDeviceToHost, s));
     DH_CHECK(hipStreamSynchronize(s));

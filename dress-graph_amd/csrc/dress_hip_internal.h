@@ -3,21 +3,29 @@
 // Device layout of one graph (all arrays in HBM, built by dress_build.hip):
 //
 //   row_off[N+1]  int32   CSR row offsets of the variant adjacency
-//   adj_x[S]      int32   neighbour id of half-edge i            (sorted (x, e))
-//   adj_e[S]      int32   input edge of half-edge i
-//   U[E], V[E]    int32   edge endpoints
-//   ew[E]         f64     variant edge weight  w-bar
-//   c4[E]         f64     self term constant 4nw_u + 4nw_v (4nw_u for FWD/BWD)
-//   pos[E]        int2    the (at most two) half-edges that carry edge e
-//   ioff[E+1]     int64   intersection index offsets
-//   ipair[T]      int2    (half-edge in row U[e], half-edge in row V[e]) per
-//                         common neighbour, in ascending neighbour order
+//   adj_x[S]      int32   neighbour id of half-edge s, rows sorted by (x, edge)
+//   adj_e[S]      int32   input edge of half-edge s
 //
-// Fit state: d[2][E] (edge values, double buffer), wh[2][S] (w-bar*d copied
-// onto every half-edge, double buffer), vd[N] (vertex norms).  Because every
-// term of a row or intersection sum is read from wh[] by half-edge position,
-// the vertex phase streams contiguous rows and the edge phase gathers inside
-// the two rows of the edge.
+// Edges are renumbered internally: internal ids run bin by bin (bin chosen by
+// the intersection size) and, inside a bin, in CSR order of the edge's first
+// half-edge, so every bin is a contiguous range read with unit stride and
+// consecutive edges share a row.
+//   ord[i] / inv[e]       internal -> input / input -> internal
+//   uv[i]         int2    (U, V) of internal edge i
+//   rs[i]         int2    (row_off[U], row_off[V])
+//   pos[i]        int2    the half-edges that carry edge i (y = -1 if one)
+//   ew[i], c4[i]  f64     w-bar and the self-term constant 4nw_u + 4nw_v
+//                         (4nw_u for FORWARD/BACKWARD)
+//   ioff[i]       int64   intersection index offsets
+//   ipk[k]        u32     (a - rs.x) | (b - rs.y) << 16 for the half-edges a
+//                         in row U and b in row V of each common neighbour,
+//                         ascending in the neighbour (max degree < 65536)
+//   ipair[k]      int2    (a, b) absolute, for graphs with larger degrees
+//
+// Fit state: d[2][E] edge values (internal order, double buffer); wh[2][S]
+// the value w-bar*d copied onto each of the edge's half-edges (double buffer),
+// so the vertex phase streams rows and the common-neighbour gathers of one
+// edge fall inside its two rows, close together; vd[N] norms.
 #pragma once
 
 #include <hip/hip_runtime.h>
@@ -56,12 +64,18 @@ void dfree(void *p);
 // ---------------------------------------------------------------- graph
 enum Precision { PREC_FP64 = 0, PREC_FP32_FP64 = 1 };
 
-struct Bins {                      // degree-binned work lists
+struct Bins {                      // degree-binned vertex work list
     static constexpr int NB = 5;   // group sizes 1, 4, 16, 64, 256
-    int *list = nullptr;           // concatenated item ids, bin by bin
+    int *list = nullptr;           // vertex ids, bin by bin
     int count[NB] = {0, 0, 0, 0, 0};
     int start[NB] = {0, 0, 0, 0, 0};
 };
+
+// number of summed terms -> bin (group of 1, 4, 16, 64 lanes or a workgroup)
+__host__ __device__ inline int term_bin(int64_t terms)
+{
+    return terms <= 4 ? 0 : terms <= 32 ? 1 : terms <= 256 ? 2 : terms <= 4096 ? 3 : 4;
+}
 
 struct DevGraph {
     int N = 0, E = 0, variant = 0;
@@ -70,23 +84,33 @@ struct DevGraph {
     bool weighted = false;
     hipStream_t stream = nullptr;
 
-    int *U = nullptr, *V = nullptr;
+    int *U = nullptr, *V = nullptr;        // input order
     int *row_off = nullptr, *adj_x = nullptr, *adj_e = nullptr;
-    double *ew = nullptr, *nw = nullptr, *c4 = nullptr;
-    int2 *pos = nullptr;
+    double *ew_in = nullptr, *c4_in = nullptr, *nw = nullptr;   // input order
+    int2 *pos_in = nullptr;                // half-edges of input edge e
+
+    // internal edge order (valid once have_index)
+    int *ord = nullptr, *inv = nullptr;
+    int2 *uv = nullptr, *rs = nullptr, *pos = nullptr;
+    double *ew = nullptr, *c4 = nullptr;
     int64_t *ioff = nullptr;
-    int2 *ipair = nullptr;
+    uint32_t *ipk = nullptr;       // packed index (max degree < 65536)
+    int2 *ipair = nullptr;         // absolute index otherwise
+    bool packed = false;
+    int64_t *cnt_in = nullptr;     // |intersection| per input edge
+    int ebin[Bins::NB + 1] = {0, 0, 0, 0, 0, 0};  // bin ranges of internal ids
     bool have_index = false;
 
-    Bins vbins, ebins;             // vertex rows by degree, edges by |isect|
+    Bins vbins;
     bool have_bins = false;
 
     // fit state
-    double *d[2] = {nullptr, nullptr};
-    double *wh[2] = {nullptr, nullptr};
+    double *d[2] = {nullptr, nullptr};     // internal order
+    double *wh[2] = {nullptr, nullptr};    // per half-edge
     double *vd = nullptr;
     float *d32[2] = {nullptr, nullptr};
     float *wh32[2] = {nullptr, nullptr};
+    double *stage = nullptr;       // input-order staging for uploads/downloads
     void *state = nullptr;         // FitState (device)
     int cur = 0;                   // buffer holding the current values
 };
@@ -99,6 +123,7 @@ bool build_bins(DevGraph *G);
 bool download_csr(DevGraph *G, int *adj_offset, int *adj_target, int *adj_edge_idx,
                   double *edge_weight);
 bool download_index(DevGraph *G, int **ioff_out, int **ux_out, int **vx_out);
+bool refresh_self_terms(DevGraph *G, const double *NW_host);
 void free_graph(DevGraph *G);
 
 // dress_fit.hip

@@ -177,6 +177,17 @@ class Graph:
             _lib.check(r == 0, "dress_fit_hip_ex")
         return it.value, dl.value
 
+    def info(self):
+        """Device graph shape: N, E, S, T, max degree, common weight, per-bin
+        edge/vertex counts and intersection entries."""
+        v = np.zeros(21, dtype=np.int64)
+        r = _lib.lib().dress_hip_graph_info(self.g, v.ctypes.data_as(_lib.c_i64_p), 21)
+        _lib.check(r == 0, "dress_hip_graph_info")
+        return {"N": int(v[0]), "E": int(v[1]), "S": int(v[2]), "T": int(v[3]),
+                "max_degree": int(v[4]), "packed_index": bool(v[5]),
+                "edge_bins": v[6:11].tolist(), "vertex_bins": v[11:16].tolist(),
+                "bin_terms": v[16:21].tolist()}
+
     def resident_reset(self):
         _lib.check(_lib.lib().dress_hip_resident_reset(self.g) == 0, "dress_hip_resident_reset")
 

@@ -81,6 +81,8 @@ def _bind(L):
     L.dress_hip_set_precision.argtypes = [ctypes.c_int]
     L.dress_hip_get_precision.restype = ctypes.c_int
     L.dress_hip_last_fit_stats.argtypes = [ctypes.POINTER(FitStats)]
+    L.dress_hip_graph_info.restype = ctypes.c_int
+    L.dress_hip_graph_info.argtypes = [PG, c_i64_p, ctypes.c_int]
     L.dress_hip_resident_reset.restype = ctypes.c_int
     L.dress_hip_resident_reset.argtypes = [PG]
     L.dress_hip_fit_resident.restype = ctypes.c_int

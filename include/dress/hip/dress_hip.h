@@ -141,6 +141,13 @@ int dress_fit_hip_ex(p_dress_graph_t g, int max_iterations, double epsilon,
                      dress_hip_precision_t precision,
                      int *iterations, double *delta);
 
+/* Shape of the device graph: info[0..20] = N, E, S (CSR entries), T
+ * (intersection entries), max degree, 1 if the index is packed (16-bit row
+ * positions), edges per work bin (5), vertices per work bin (5), intersection
+ * entries per edge bin (5).  Bins group items whose sums
+ * have <= 4, 32, 256, 4096, more terms.  Returns 0, or -1 on error. */
+int dress_hip_graph_info(p_dress_graph_t g, int64_t *info, int n);
+
 /* Device-resident fits for callers that keep the graph on the GPU between
  * calls (benchmarks, servers): dress_hip_resident_reset() loads
  * g->edge_dress onto the device; dress_hip_fit_resident() then iterates the
