@@ -1,0 +1,67 @@
+// dress_batch.h -- interfaces of the batched delta^k / nabla^k kernel.
+#pragma once
+
+#include "dress_hip_internal.h"
+
+namespace dh {
+
+enum { LIFT_DELTA = 0, LIFT_NABLA = 1 };
+constexpr int LIFT_KMAX = 64;      // largest k a work item may carry
+
+// one parent graph, as offsets into the concatenated pools
+struct LiftParent {
+    int N, E, S, T;
+    int64_t row_ofs, adj_ofs, edge_ofs, ioff_ofs, pair_ofs;
+    int directed;                  // FORWARD/BACKWARD self-term form
+    int pad;
+};
+
+// one subgraph: parent + deleted vertices (delta) or marked tuple (nabla)
+struct LiftItem {
+    int parent;
+    int kind;
+    int k;
+    int mixed;                     // start in FP32
+    int64_t vofs;                  // item vertices at item_verts[vofs .. vofs+k)
+    int64_t val_ofs;               // live values at vals[val_ofs ...]
+    int64_t ms_row;                // multiset row
+};
+
+struct LiftArgs {
+    const LiftParent *parents;
+    const LiftItem *items;
+    const int *order;              // queue order (item ids), or null
+    int n_items;
+    const int *item_verts;
+    const double *marks;           // nabla marker weights [LIFT_KMAX]
+    const int *row_off;            // per parent: N+1 local offsets
+    const int *adj_e;              // per parent: S local edge ids
+    const int2 *uv;                // per parent: E (U, V)
+    const double *ew;              // per parent: E variant weights
+    const int *ioff;               // per parent: E+1 local offsets
+    const uint32_t *pairs;         // per parent: T (edge(u,x) | edge(v,x) << 16)
+    int *queue;                    // work counter (zeroed before launch)
+    int ecap, ncap;                // LDS capacity in edges / vertices
+    int max_it;
+    double eps;
+    double *vals;                  // packed live values (or null)
+    double *ms;                    // multiset rows (or null)
+    int64_t ms_stride;
+    int *iters;                    // per item (or null)
+    double *deltas;                // per item (or null)
+    int *flags;                    // per item: 1 = redo in FP64 (or null)
+    // quantised histogram: open-addressing table keyed by
+    // (graph << 41) | (boundary << 40) | floor(d / eps), integer counts
+    unsigned long long *qkeys;     // capacity qcap (power of two), or null
+    unsigned long long *qcounts;
+    int64_t qcap;
+    double qtol;                   // values within qtol of a bin edge are boundary values
+    const int *parent_graph;       // graph id of each parent (for the key)
+};
+
+constexpr unsigned long long QEMPTY = ~0ull;
+
+size_t lifted_lds_bytes(int ecap, int ncap);
+bool lifted_launch(const LiftArgs &a, int block, int grid, hipStream_t s);
+
+}  // namespace dh

@@ -1,0 +1,322 @@
+// dress_batch.hip -- delta^k / nabla^k subgraph fits in one persistent kernel.
+//
+// The reference fits every subgraph separately: it rebuilds the subgraph on
+// the host (libdress/src/delta_dress_impl.c:95-169, nabla_dress_impl.c:61-91)
+// and calls the fit function on it (delta_dress_impl.c:349,
+// nabla_dress_impl.c:280).  Here a subgraph is its parent graph plus an item
+// descriptor:
+//   delta^k  the k deleted vertices: an edge is live when neither end is
+//            deleted; vertex weights are all 1.0 (delta_dress_impl.c:166-168
+//            passes NW = NULL), edge weights are the parent's;
+//   nabla^k  the ordered k-tuple: vertex weights are 1.0 except tuple[i],
+//            which gets marker[i] (nabla_dress_impl.c:82-86).
+// Because relabelling the surviving vertices keeps every row's order and the
+// intersection of two rows in the subgraph is the parent's intersection
+// without the deleted neighbours, the live terms of every sum are exactly the
+// subgraph's terms; sums are rounded once from their exact value, so each
+// subgraph value equals the one the reference computes on the rebuilt graph.
+//
+// One workgroup runs one item at a time, taking items from a queue sorted by
+// decreasing size.  The item's state (edge values, double buffered, and vertex
+// norms) lives in LDS; the parent's structure is read through the cache.
+// Mixed precision: FP32 edge values, terms summed exactly in FP64, until the
+// change is below 3000*epsilon, then FP64 with compensated sums; an item
+// whose stopping decision falls within 0.1% of epsilon is flagged and redone
+// in FP64 by the host.
+#include "dress_batch.h"
+
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+
+namespace dh {
+
+namespace {
+
+__device__ __forceinline__ void two_sum_add(double &s, double &c, double x)
+{
+    double t = s + x;
+    double z = t - s;
+    double e = (s - (t - z)) + (x - z);
+    s = t;
+    c += e;
+}
+
+// vertex weight of u for the current item
+__device__ __forceinline__ double item_nw(int kind, int u, const int *tup, const double *mark, int k)
+{
+    if (kind == LIFT_NABLA) {
+        for (int i = 0; i < k; i++)
+            if (tup[i] == u) return mark[i];
+    }
+    return 1.0;
+}
+
+template <int B>
+__device__ __forceinline__ double block_max(double m, double *red)
+{
+    for (int o = 32; o > 0; o >>= 1) m = fmax(m, __shfl_xor(m, o));
+    if (B > 64) {
+        if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = m;
+        __syncthreads();
+        m = red[0];
+        for (int i = 1; i < B / 64; i++) m = fmax(m, red[i]);
+    }
+    return m;
+}
+
+// One persistent kernel over all items.  LDS layout (dynamic):
+//   double dA[ecap], dB[ecap], vd[ncap]; int alive[ncap] (bytes as int)
+// FP32 phase uses the first halves of dA / dB as float arrays.
+template <int B>
+__global__ void __launch_bounds__(B) k_lifted(LiftArgs a)
+{
+    extern __shared__ double smem[];
+    double *dA = smem;
+    double *dB = dA + a.ecap;
+    double *vd = dB + a.ecap;
+    int *alive = (int *)(vd + a.ncap);
+    __shared__ int s_item;
+    __shared__ double s_red[B / 64];
+    __shared__ int s_tup[LIFT_KMAX];
+    __shared__ double s_mark[LIFT_KMAX];
+
+    for (;;) {
+        if (threadIdx.x == 0) s_item = atomicAdd(a.queue, 1);
+        __syncthreads();
+        const int qi = s_item;
+        __syncthreads();
+        if (qi >= a.n_items) break;
+        const int it = a.order ? a.order[qi] : qi;
+        const LiftItem I = a.items[it];
+        const LiftParent P = a.parents[I.parent];
+        const int N = P.N, E = P.E;
+        const int k = I.k;
+        const int *__restrict__ row_off = a.row_off + P.row_ofs;
+        const int *__restrict__ adj_e = a.adj_e + P.adj_ofs;
+        const int2 *__restrict__ uv = a.uv + P.edge_ofs;
+        const double *__restrict__ ew = a.ew + P.edge_ofs;
+        const int *__restrict__ ioff = a.ioff + P.ioff_ofs;
+        const uint32_t *__restrict__ pairs = a.pairs + P.pair_ofs;
+        const int directed = P.directed;
+
+        if (threadIdx.x < k) {
+            s_tup[threadIdx.x] = a.item_verts[I.vofs + threadIdx.x];
+            s_mark[threadIdx.x] = a.marks[threadIdx.x];
+        }
+        for (int u = threadIdx.x; u < N; u += B) alive[u] = 1;
+        __syncthreads();
+        if (I.kind == LIFT_DELTA && threadIdx.x < k) alive[s_tup[threadIdx.x]] = 0;
+        __syncthreads();
+        // d = 1.0 on live edges (delta_dress_impl.c:166 -> dress.c:418-421);
+        // a dead edge keeps value 0 so it adds nothing to a row sum
+        bool fp32 = I.mixed != 0;
+        float *fA = (float *)dA, *fB = (float *)dB;
+        for (int e = threadIdx.x; e < E; e += B) {
+            int2 p = uv[e];
+            bool live = alive[p.x] && alive[p.y];
+            if (fp32) { fA[e] = live ? 1.0f : 0.0f; fB[e] = 0.0f; }
+            else { dA[e] = live ? 1.0 : 0.0; dB[e] = 0.0; }
+        }
+        __syncthreads();
+        const double thr = 3000.0 * a.eps;
+        int cur = 0;            // 0: A holds the current values, 1: B
+        int iters = a.max_it;   // reference convention result
+        double last = 0.0;
+        int ambiguous = 0;
+        for (int iter = 0; iter < a.max_it; iter++) {
+            // ---- vertex phase
+            for (int u = threadIdx.x; u < N; u += B) {
+                if (!alive[u]) continue;
+                int rb = row_off[u], re = row_off[u + 1];
+                const double self = 4.0 * item_nw(I.kind, u, s_tup, s_mark, k);
+                double s = 0.0, c = 0.0;
+                if (fp32) {
+                    // float products, summed exactly in FP64 (independent of
+                    // order), then the self term
+                    const float *dc = cur ? fB : fA;
+                    for (int i = rb; i < re; i++) {
+                        int e = adj_e[i];
+                        s += (double)((float)ew[e] * dc[e]);
+                    }
+                    vd[u] = sqrt(s + self);
+                } else {
+                    const double *dc = cur ? dB : dA;
+                    s = self;
+                    for (int i = rb; i < re; i++) {
+                        int e = adj_e[i];
+                        two_sum_add(s, c, ew[e] * dc[e]);
+                    }
+                    vd[u] = sqrt(s + c);
+                }
+            }
+            __syncthreads();
+            // ---- edge phase
+            double mymax = 0.0;
+            for (int e = threadIdx.x; e < E; e += B) {
+                int2 p = uv[e];
+                if (!alive[p.x] || !alive[p.y]) continue;
+                double cu = 4.0 * item_nw(I.kind, p.x, s_tup, s_mark, k);
+                double c4 = directed ? cu : cu + 4.0 * item_nw(I.kind, p.y, s_tup, s_mark, k);
+                int kb = ioff[e], ke = ioff[e + 1];
+                double w = ew[e];
+                double x, de;
+                if (fp32) {
+                    const float *dc = cur ? fB : fA;
+                    float *dn = cur ? fA : fB;
+                    double s = 0.0;
+                    for (int q = kb; q < ke; q++) {
+                        uint32_t pr = pairs[q];
+                        int ea = (int)(pr & 0xffffu), eb = (int)(pr >> 16);
+                        // a dead neighbour kills both of its edges: adds 0
+                        s += (double)((float)ew[ea] * dc[ea]) + (double)((float)ew[eb] * dc[eb]);
+                    }
+                    de = (double)dc[e];
+                    double own = (double)((float)w * dc[e]);
+                    s += directed ? (c4 + own) : (c4 + 2.0 * own);
+                    double den = vd[p.x] * vd[p.y];
+                    x = den > 0.0 ? s / den : 0.0;
+                    dn[e] = (float)x;
+                } else {
+                    const double *dc = cur ? dB : dA;
+                    double *dn = cur ? dA : dB;
+                    double s = 0.0, c = 0.0;
+                    for (int q = kb; q < ke; q++) {
+                        uint32_t pr = pairs[q];
+                        int ea = (int)(pr & 0xffffu), eb = (int)(pr >> 16);
+                        double ta = dc[ea], tb = dc[eb];
+                        if (ta == 0.0 && tb == 0.0) continue;   // deleted common neighbour
+                        two_sum_add(s, c, ew[ea] * ta + ew[eb] * tb);
+                    }
+                    de = dc[e];
+                    double own = w * de;
+                    two_sum_add(s, c, directed ? (c4 + own) : (c4 + 2.0 * own));
+                    double den = vd[p.x] * vd[p.y];
+                    x = den > 0.0 ? (s + c) / den : 0.0;
+                    dn[e] = x;
+                }
+                mymax = fmax(mymax, fabs(de - x));
+            }
+            double md = block_max<B>(mymax, s_red);
+            __syncthreads();
+            cur ^= 1;
+            last = md;
+            if (fp32) {
+                // hand over to FP64 (never decide convergence on FP32 values)
+                if (md < thr || iter + 1 >= a.max_it - 16) {
+                    // float values of buffer cur -> doubles in the other
+                    // buffer, then copied back so both buffers hold doubles
+                    const float *fc = cur ? fB : fA;
+                    double *dn = cur ? dA : dB;
+                    double *dm = cur ? dB : dA;
+                    for (int e = threadIdx.x; e < E; e += B) dn[e] = (double)fc[e];
+                    __syncthreads();
+                    for (int e = threadIdx.x; e < E; e += B) dm[e] = dn[e];
+                    __syncthreads();
+                    cur ^= 1;
+                    fp32 = false;
+                }
+                if (iter + 1 >= a.max_it) { ambiguous = 1; iters = a.max_it; break; }
+                continue;
+            }
+            if (I.mixed && fabs(md - a.eps) <= 1e-3 * md) ambiguous = 1;
+            if (md < a.eps) { iters = iter; break; }
+        }
+        // ---- outputs
+        const double *dc = cur ? dB : dA;
+        if (a.ms) {
+            double *row = a.ms + (int64_t)I.ms_row * a.ms_stride;
+            for (int e = threadIdx.x; e < E; e += B) {
+                int2 p = uv[e];
+                row[e] = (alive[p.x] && alive[p.y]) ? dc[e] : __longlong_as_double(0x7ff8000000000000ll);
+            }
+        }
+        if (a.vals) {
+            // live values packed in edge order at vals[I.val_ofs ...]
+            __shared__ int s_base;
+            if (threadIdx.x == 0) s_base = 0;
+            __syncthreads();
+            for (int e0 = 0; e0 < E; e0 += B) {
+                int e = e0 + threadIdx.x;
+                bool live = false;
+                if (e < E) {
+                    int2 p = uv[e];
+                    live = alive[p.x] && alive[p.y];
+                }
+                // block-wide exclusive prefix of live flags, wave ballots
+                unsigned long long m = __ballot(live);
+                int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+                int before = __popcll(m & ((1ull << lane) - 1));
+                __shared__ int s_wave[B / 64];
+                if (lane == 0) s_wave[wv] = __popcll(m);
+                __syncthreads();
+                int off = s_base;
+                for (int i = 0; i < wv; i++) off += s_wave[i];
+                if (live) a.vals[I.val_ofs + off + before] = dc[e];
+                __syncthreads();
+                if (threadIdx.x == 0) {
+                    int tot = 0;
+                    for (int i = 0; i < B / 64; i++) tot += s_wave[i];
+                    s_base += tot;
+                }
+                __syncthreads();
+            }
+        }
+        if (a.qkeys) {
+            // quantised histogram in integer atomics: key floor(d / eps) of
+            // every live value; values within qtol of a bin edge counted apart
+            const unsigned long long gid = (unsigned long long)a.parent_graph[I.parent];
+            for (int e = threadIdx.x; e < E; e += B) {
+                int2 p = uv[e];
+                if (!(alive[p.x] && alive[p.y])) continue;
+                double v = dc[e];
+                double r = v / a.eps;
+                double q = floor(r);
+                bool bnd = (r - q) * a.eps < a.qtol || (q + 1.0 - r) * a.eps < a.qtol;
+                unsigned long long qq = (q < 0.0) ? 0ull : (q >= 1099511627775.0 ? 1099511627775ull
+                                                                                  : (unsigned long long)q);
+                unsigned long long key = (gid << 41) | ((unsigned long long)bnd << 40) | qq;
+                unsigned long long h = key * 0x9E3779B97F4A7C15ull;
+                int64_t slot = (int64_t)(h >> 20) & (a.qcap - 1);
+                for (int64_t probe = 0; probe < a.qcap; probe++) {
+                    unsigned long long prev = atomicCAS(&a.qkeys[slot], QEMPTY, key);
+                    if (prev == QEMPTY || prev == key) {
+                        atomicAdd(&a.qcounts[slot], 1ull);
+                        break;
+                    }
+                    slot = (slot + 1) & (a.qcap - 1);
+                }
+            }
+        }
+        if (threadIdx.x == 0) {
+            if (a.iters) a.iters[it] = iters;
+            if (a.deltas) a.deltas[it] = last;
+            if (a.flags) a.flags[it] = ambiguous;
+        }
+        __syncthreads();
+    }
+}
+
+}  // namespace
+
+size_t lifted_lds_bytes(int ecap, int ncap)
+{
+    return (size_t)(2 * ecap + ncap) * sizeof(double) + (size_t)ncap * sizeof(int) + 16;
+}
+
+bool lifted_launch(const LiftArgs &a, int block, int grid, hipStream_t s)
+{
+    size_t lds = lifted_lds_bytes(a.ecap, a.ncap);
+    if (block == 64)
+        hipLaunchKernelGGL(k_lifted<64>, dim3(grid), dim3(64), lds, s, a);
+    else if (block == 128)
+        hipLaunchKernelGGL(k_lifted<128>, dim3(grid), dim3(128), lds, s, a);
+    else if (block == 256)
+        hipLaunchKernelGGL(k_lifted<256>, dim3(grid), dim3(256), lds, s, a);
+    else
+        hipLaunchKernelGGL(k_lifted<512>, dim3(grid), dim3(512), lds, s, a);
+    DH_CHECK(hipGetLastError());
+    return true;
+}
+
+}  // namespace dh

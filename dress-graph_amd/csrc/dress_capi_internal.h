@@ -12,5 +12,6 @@ DevGraph *device_graph(p_dress_graph_t g);
 DevGraph *lookup(const dress_graph_t *g);
 int default_precision();
 void record_stats(DevGraph *G, const FitResult &r);
+void record_lifted(int reruns, double device_ms, int64_t subgraphs, int64_t values);
 
 }  // namespace dh

@@ -145,6 +145,10 @@ bool fit_download(DevGraph *G, double *edge_dress, double *vertex_dress);
 bool prim_sort_pairs_u64_i32(const uint64_t *kin, uint64_t *kout, const int *vin, int *vout,
                              size_t n, int end_bit, hipStream_t s);
 bool prim_sort_u64(const uint64_t *kin, uint64_t *kout, size_t n, hipStream_t s);
+bool prim_sort_pairs_u64_u64(const uint64_t *kin, uint64_t *kout, const uint64_t *vin, uint64_t *vout,
+                             size_t n, hipStream_t s);
+bool prim_sort_pairs_i32_u64(const int *kin, int *kout, const uint64_t *vin, uint64_t *vout, size_t n,
+                             int end_bit, hipStream_t s);
 bool prim_exclusive_scan_i64(const int64_t *in, int64_t *out, size_t n, hipStream_t s);
 bool prim_exclusive_scan_i32_to_i64(const int *in, int64_t *out, size_t n, hipStream_t s);
 

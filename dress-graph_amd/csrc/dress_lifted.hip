@@ -1,0 +1,1306 @@
+// dress_lifted.hip -- host driver of delta^k / nabla^k on the GPU and the
+// C ABI entry points dress_{delta,nabla}_fit_hip*, plus the multi-graph batch
+// API dress_hip_lifted_batch.
+//
+// Enumeration, striding and sampling follow the reference drivers
+// (libdress/src/delta_dress_impl.c:233-380, nabla_dress_impl.c:142-311):
+// subgraph s is the s-th k-combination (delta) or ordered k-tuple (nabla) in
+// lexicographic order; s is processed when s % stride == offset and, when
+// sampling, when its rank within the stride is one of the sorted, deduplicated
+// picks drawn with the reference's LCG.  The histogram is exact in the value:
+// all live values are radix sorted on the device and run-length encoded, and
+// returned ascending as [value bits, count] like dress_hist_flatten
+// (dress_histogram.c:124-155).
+#include "dress_batch.h"
+#include "dress_capi_internal.h"
+#include "dress/hip/dress_hip.h"
+
+#include <algorithm>
+#include <chrono>
+#include <cmath>
+#include <cstdio>
+#include <cstdlib>
+#include <cstring>
+#include <mutex>
+#include <unordered_map>
+#include <vector>
+
+namespace dh {
+
+namespace {
+
+constexpr int TPB = 256;
+
+inline int grid_for(int64_t threads, int cap)
+{
+    int64_t b = (threads + TPB - 1) / TPB;
+    if (b < 1) b = 1;
+    if (b > cap) b = cap;
+    return (int)b;
+}
+
+// ------------------------------------------------------------ host parents
+
+// Structure of one small parent graph, built on the host: the variant CSR
+// with rows ordered by (neighbour, edge), variant weights, and per edge the
+// multiset intersection of its two rows as pairs of edge ids.  Same rules as
+// the device build (dress_build.hip) and the reference (dress.c:103-363).
+struct HostParent {
+    int N = 0, E = 0, S = 0, T = 0;
+    int directed = 0;
+    std::vector<int> row_off, adj_x, adj_e, ioff;
+    std::vector<int2> uv;
+    std::vector<double> ew;
+    std::vector<uint32_t> pairs;
+};
+
+struct HalfEdge {
+    int x, e;
+    double w;
+};
+
+bool host_parent_build(int N, int E, const int *U, const int *V, const double *W, int variant,
+                       HostParent &P)
+{
+    P.N = N;
+    P.E = E;
+    P.directed = (variant == DRESS_VARIANT_FORWARD || variant == DRESS_VARIANT_BACKWARD);
+    auto by_xe = [](const HalfEdge &a, const HalfEdge &b) { return a.x != b.x ? a.x < b.x : a.e < b.e; };
+    // raw out-adjacency for reciprocal lookups (DIRECTED)
+    std::vector<int> roff;
+    std::vector<HalfEdge> raw;
+    if (variant == DRESS_VARIANT_DIRECTED) {
+        roff.assign(N + 1, 0);
+        for (int e = 0; e < E; e++) roff[U[e] + 1]++;
+        for (int u = 0; u < N; u++) roff[u + 1] += roff[u];
+        raw.resize(E);
+        std::vector<int> fill(roff.begin(), roff.end() - 1);
+        for (int e = 0; e < E; e++) raw[fill[U[e]]++] = HalfEdge{V[e], e, W ? W[e] : 1.0};
+        for (int u = 0; u < N; u++) std::sort(raw.begin() + roff[u], raw.begin() + roff[u + 1], by_xe);
+    }
+    auto raw_find = [&](int u, int v) {  // probe sequence of dress.c:70-87
+        int lo = roff[u], hi = roff[u + 1] - 1;
+        while (lo <= hi) {
+            int mid = lo + (hi - lo) / 2;
+            int x = raw[mid].x;
+            if (x == v) return mid;
+            if (x < v) lo = mid + 1; else hi = mid - 1;
+        }
+        return -1;
+    };
+    std::vector<std::pair<int, HalfEdge>> half;
+    half.reserve(2 * (size_t)E);
+    P.ew.assign(E, 0.0);
+    P.uv.resize(E);
+    for (int e = 0; e < E; e++) {
+        int u = U[e], v = V[e];
+        double w = W ? W[e] : 1.0;
+        P.uv[e] = make_int2(u, v);
+        switch (variant) {
+        case DRESS_VARIANT_UNDIRECTED:
+            half.push_back({u, HalfEdge{v, e, 2.0 * w}});
+            half.push_back({v, HalfEdge{u, e, 2.0 * w}});
+            P.ew[e] = 2.0 * w;
+            break;
+        case DRESS_VARIANT_DIRECTED: {
+            int r = raw_find(v, u);
+            double wb = r >= 0 ? w + raw[r].w : w;
+            half.push_back({u, HalfEdge{v, e, wb}});
+            if (r < 0) half.push_back({v, HalfEdge{u, e, w}});
+            P.ew[e] = wb;
+            break;
+        }
+        case DRESS_VARIANT_FORWARD:
+            half.push_back({u, HalfEdge{v, e, w}});
+            P.ew[e] = w;
+            break;
+        default:
+            half.push_back({v, HalfEdge{u, e, w}});
+            P.ew[e] = w;
+            break;
+        }
+    }
+    P.S = (int)half.size();
+    P.row_off.assign(N + 1, 0);
+    for (auto &h : half) P.row_off[h.first + 1]++;
+    for (int u = 0; u < N; u++) P.row_off[u + 1] += P.row_off[u];
+    std::vector<HalfEdge> rows(P.S);
+    {
+        std::vector<int> fill(P.row_off.begin(), P.row_off.end() - 1);
+        for (auto &h : half) rows[fill[h.first]++] = h.second;
+    }
+    for (int u = 0; u < N; u++) std::sort(rows.begin() + P.row_off[u], rows.begin() + P.row_off[u + 1], by_xe);
+    P.adj_x.resize(P.S);
+    P.adj_e.resize(P.S);
+    for (int i = 0; i < P.S; i++) { P.adj_x[i] = rows[i].x; P.adj_e[i] = rows[i].e; }
+    // intersections (dress.c:307-363 pairing of repeated neighbours)
+    P.ioff.assign(E + 1, 0);
+    P.pairs.clear();
+    for (int e = 0; e < E; e++) {
+        int a = P.row_off[U[e]], ae = P.row_off[U[e] + 1];
+        int b = P.row_off[V[e]], be = P.row_off[V[e] + 1];
+        while (a < ae && b < be) {
+            int x = P.adj_x[a], y = P.adj_x[b];
+            if (x == y) {
+                P.pairs.push_back((uint32_t)P.adj_e[a] | ((uint32_t)P.adj_e[b] << 16));
+                a++;
+                b++;
+            } else if (x < y) a++;
+            else b++;
+        }
+        P.ioff[e + 1] = (int)P.pairs.size();
+    }
+    P.T = (int)P.pairs.size();
+    return true;
+}
+
+// ------------------------------------------------------------ enumeration
+
+int host_rand(unsigned int *s)  // delta_dress_impl.h:21-25
+{
+    *s = *s * 1103515245u + 12345u;
+    return (int)((*s >> 16) & 0x7fff);
+}
+
+// delta_dress_impl.c:215-219, 296-316: sorted, deduplicated sample picks
+std::vector<int64_t> sample_picks(int64_t per, int64_t n, unsigned int seed, int offset)
+{
+    std::vector<int64_t> p((size_t)n);
+    unsigned int s = seed + (unsigned int)offset;
+    for (int64_t i = 0; i < n; i++) {
+        int64_t hi = (int64_t)host_rand(&s);
+        int64_t lo = (int64_t)host_rand(&s);
+        p[(size_t)i] = (((hi << 31) | lo) & 0x7FFFFFFFFFFFFFFFLL) % per;
+    }
+    std::sort(p.begin(), p.end());
+    p.erase(std::unique(p.begin(), p.end()), p.end());
+    return p;
+}
+
+bool next_combo(std::vector<int> &c, int N)
+{
+    int k = (int)c.size(), i = k - 1;
+    while (i >= 0 && c[i] == N - k + i) i--;
+    if (i < 0) return false;
+    c[i]++;
+    for (int j = i + 1; j < k; j++) c[j] = c[j - 1] + 1;
+    return true;
+}
+
+bool next_tuple(std::vector<int> &t, int N, std::vector<char> &used)
+{
+    int k = (int)t.size();
+    std::fill(used.begin(), used.end(), 0);
+    for (int v : t) used[v] = 1;
+    for (int i = k - 1; i >= 0; i--) {
+        used[t[i]] = 0;
+        int x = t[i] + 1;
+        while (x < N && used[x]) x++;
+        if (x < N) {
+            t[i] = x;
+            used[x] = 1;
+            int y = 0;
+            for (int j = i + 1; j < k; j++) {
+                while (used[y]) y++;
+                t[j] = y;
+                used[y] = 1;
+            }
+            return true;
+        }
+    }
+    return false;
+}
+
+// selected subgraph indices and their vertex tuples
+void enumerate_items(int kind, int N, int k, int n_samples, unsigned int seed, int offset, int stride,
+                     int64_t total, std::vector<int64_t> &sel, std::vector<int> &verts)
+{
+    int64_t per = (total + stride - 1 - offset) / stride;
+    if (per < 0) per = 0;
+    int64_t limit = (n_samples == 0) ? per : std::min<int64_t>(n_samples, per);
+    bool sampling = n_samples > 0 && limit < per;
+    std::vector<int64_t> picks;
+    if (sampling) picks = sample_picks(per, limit, seed, offset);
+    std::vector<int> cur(k);
+    for (int i = 0; i < k; i++) cur[i] = i;
+    std::vector<char> used(N + 1, 0);
+    int64_t s = 0, my = 0;
+    size_t pi = 0;
+    if (total <= 0) return;
+    do {
+        if (s % stride == offset) {
+            bool take = !sampling || (pi < picks.size() && my == picks[pi]);
+            if (take && sampling) pi++;
+            if (take) {
+                sel.push_back(s);
+                verts.insert(verts.end(), cur.begin(), cur.end());
+            }
+            my++;
+            if (sampling && pi >= picks.size()) break;
+        }
+        s++;
+    } while (kind == LIFT_NABLA ? next_tuple(cur, N, used) : next_combo(cur, N));
+}
+
+// ------------------------------------------------------------ device pools
+
+struct DevPools {
+    LiftParent *parents = nullptr;
+    int *parent_graph = nullptr;
+    int *row_off = nullptr, *adj_e = nullptr, *ioff = nullptr;
+    int2 *uv = nullptr;
+    double *ew = nullptr;
+    uint32_t *pairs = nullptr;
+    void release()
+    {
+        dfree(parents); dfree(parent_graph); dfree(row_off); dfree(adj_e); dfree(ioff);
+        dfree(uv); dfree(ew); dfree(pairs);
+        *this = DevPools();
+    }
+};
+
+template <typename T>
+bool upload(T **dst, const std::vector<T> &src, hipStream_t s, const char *what)
+{
+    if (!dalloc(dst, src.size() + 1, what)) return false;
+    if (!src.empty()) DH_CHECK(hipMemcpyAsync(*dst, src.data(), src.size() * sizeof(T), hipMemcpyHostToDevice, s));
+    return true;
+}
+
+bool upload_parents(const std::vector<const HostParent *> &ps, const std::vector<int> &graph_of,
+                    std::vector<LiftParent> &meta, DevPools &D, hipStream_t s)
+{
+    std::vector<int> row_off, adj_e, ioff;
+    std::vector<int2> uv;
+    std::vector<double> ew;
+    std::vector<uint32_t> pairs;
+    meta.clear();
+    for (const HostParent *P : ps) {
+        LiftParent m;
+        m.N = P->N; m.E = P->E; m.S = P->S; m.T = P->T;
+        m.row_ofs = (int64_t)row_off.size();
+        m.adj_ofs = (int64_t)adj_e.size();
+        m.edge_ofs = (int64_t)uv.size();
+        m.ioff_ofs = (int64_t)ioff.size();
+        m.pair_ofs = (int64_t)pairs.size();
+        m.directed = P->directed;
+        m.pad = 0;
+        row_off.insert(row_off.end(), P->row_off.begin(), P->row_off.end());
+        adj_e.insert(adj_e.end(), P->adj_e.begin(), P->adj_e.end());
+        uv.insert(uv.end(), P->uv.begin(), P->uv.end());
+        ew.insert(ew.end(), P->ew.begin(), P->ew.end());
+        ioff.insert(ioff.end(), P->ioff.begin(), P->ioff.end());
+        pairs.insert(pairs.end(), P->pairs.begin(), P->pairs.end());
+        meta.push_back(m);
+    }
+    return upload(&D.parents, meta, s, "parents") && upload(&D.parent_graph, graph_of, s, "parent graphs") &&
+           upload(&D.row_off, row_off, s, "row offsets") && upload(&D.adj_e, adj_e, s, "adjacency") &&
+           upload(&D.uv, uv, s, "edge ends") && upload(&D.ew, ew, s, "edge weights") &&
+           upload(&D.ioff, ioff, s, "isect offsets") && upload(&D.pairs, pairs, s, "isect pairs");
+}
+
+// ------------------------------------------------------------ histogram
+
+__global__ void k_zero_int(int *p, int n)
+{
+    for (int i = threadIdx.x; i < n; i += blockDim.x) p[i] = 0;
+}
+
+__global__ void k_sort_keys(const double *v, int64_t n, uint64_t *key)
+{
+    for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) {
+        uint64_t b = (uint64_t)__double_as_longlong(v[i]);
+        key[i] = (b >> 63) ? ~b : (b | 0x8000000000000000ull);
+    }
+}
+
+// run starts: flag where the key or the segment changes
+__global__ void k_run_flags(const uint64_t *key, const int *seg, int64_t n, int *flag)
+{
+    for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x)
+        flag[i] = (i == 0 || key[i] != key[i - 1] || (seg && seg[i] != seg[i - 1])) ? 1 : 0;
+}
+
+__global__ void k_run_emit(const uint64_t *key, const int *seg, const int *flag, const int64_t *pos, int64_t n,
+                           double *uval, int64_t *ustart, int *useg)
+{
+    for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) {
+        if (!flag[i]) continue;
+        uint64_t k = key[i];
+        uint64_t b = (k >> 63) ? (k & 0x7fffffffffffffffull) : ~k;
+        int64_t j = pos[i];
+        uval[j] = __longlong_as_double((long long)b);
+        ustart[j] = i;
+        if (useg) useg[j] = seg[i];
+    }
+}
+
+__global__ void k_fill_seg(const int64_t *seg_ofs, int nseg, int *seg)
+{
+    for (int g = blockIdx.x; g < nseg; g += gridDim.x)
+        for (int64_t i = seg_ofs[g] + threadIdx.x; i < seg_ofs[g + 1]; i += blockDim.x) seg[i] = g;
+}
+
+// Exact (value, count) histogram of vals[0..n) per segment (segments are
+// contiguous ranges, seg_ofs has nseg+1 entries): radix sort by value, then a
+// stable radix sort by segment, then run-length encoding.  Output ascending by
+// (segment, value); useg[j] is the segment of run j.
+bool exact_histogram(const double *vals, int64_t n, const std::vector<int64_t> &seg_ofs, hipStream_t s,
+                     std::vector<double> &uvals, std::vector<int64_t> &counts, std::vector<int> &useg)
+{
+    uvals.clear();
+    counts.clear();
+    useg.clear();
+    if (n <= 0) return true;
+    const int nseg = (int)seg_ofs.size() - 1;
+    uint64_t *k0 = nullptr, *k1 = nullptr;
+    int *g0 = nullptr, *g1 = nullptr, *flag = nullptr, *usg = nullptr;
+    int64_t *pos = nullptr, *ustart = nullptr, *dofs = nullptr;
+    double *uval = nullptr;
+    bool ok = dalloc(&k0, (size_t)n, "histogram keys") && dalloc(&k1, (size_t)n, "histogram keys") &&
+              dalloc(&g0, (size_t)n, "segments") && dalloc(&g1, (size_t)n, "segments") &&
+              dalloc(&flag, (size_t)n + 1, "run flags") && dalloc(&pos, (size_t)n + 1, "run positions") &&
+              dalloc(&dofs, (size_t)nseg + 1, "segment offsets");
+    int grid = grid_for(n, num_cus() * 8);
+    int64_t runs = 0;
+    if (ok) {
+        hipLaunchKernelGGL(k_sort_keys, dim3(grid), dim3(TPB), 0, s, vals, n, k0);
+        ok = hipMemcpyAsync(dofs, seg_ofs.data(), seg_ofs.size() * sizeof(int64_t), hipMemcpyHostToDevice, s) ==
+             hipSuccess;
+        hipLaunchKernelGGL(k_fill_seg, dim3(std::max(1, std::min(nseg, 4096))), dim3(TPB), 0, s, dofs, nseg, g0);
+        // by value (segment ids travel along) ...
+        ok = ok && prim_sort_pairs_u64_i32(k0, k1, g0, g1, (size_t)n, 64, s);
+        // ... then stably by segment
+        int bits = 1;
+        while ((1 << bits) <= nseg) bits++;
+        if (ok && nseg > 1) ok = prim_sort_pairs_i32_u64(g1, g0, k1, k0, (size_t)n, bits, s);
+        else if (ok) {
+            std::swap(k0, k1);
+            std::swap(g0, g1);
+        }
+        // now k0 / g0 hold the sorted (segment, value) keys
+        if (ok) {
+            hipLaunchKernelGGL(k_run_flags, dim3(grid), dim3(TPB), 0, s, k0, g0, n, flag);
+            hipLaunchKernelGGL(k_zero_int, dim3(1), dim3(64), 0, s, flag + n, 1);
+            ok = prim_exclusive_scan_i32_to_i64(flag, pos, (size_t)n + 1, s);
+        }
+        if (ok) {
+            ok = hipMemcpyAsync(&runs, pos + n, sizeof(int64_t), hipMemcpyDeviceToHost, s) == hipSuccess &&
+                 hipStreamSynchronize(s) == hipSuccess;
+        }
+        if (ok)
+            ok = dalloc(&uval, (size_t)runs + 1, "histogram values") &&
+                 dalloc(&ustart, (size_t)runs + 1, "histogram runs") && dalloc(&usg, (size_t)runs + 1, "histogram segs");
+        if (ok) {
+            hipLaunchKernelGGL(k_run_emit, dim3(grid), dim3(TPB), 0, s, k0, g0, flag, pos, n, uval, ustart, usg);
+            uvals.resize((size_t)runs);
+            counts.resize((size_t)runs + 1);
+            useg.resize((size_t)runs);
+            ok = hipMemcpyAsync(uvals.data(), uval, (size_t)runs * sizeof(double), hipMemcpyDeviceToHost, s) ==
+                     hipSuccess &&
+                 hipMemcpyAsync(counts.data(), ustart, (size_t)runs * sizeof(int64_t), hipMemcpyDeviceToHost, s) ==
+                     hipSuccess &&
+                 hipMemcpyAsync(useg.data(), usg, (size_t)runs * sizeof(int), hipMemcpyDeviceToHost, s) ==
+                     hipSuccess &&
+                 hipStreamSynchronize(s) == hipSuccess;
+            if (ok) {
+                counts[(size_t)runs] = n;
+                for (int64_t j = 0; j < runs; j++) counts[(size_t)j] = counts[(size_t)j + 1] - counts[(size_t)j];
+                counts.resize((size_t)runs);
+            }
+        }
+        if (!ok && !has_error()) set_error("histogram assembly failed");
+    }
+    dfree(k0); dfree(k1); dfree(g0); dfree(g1); dfree(flag); dfree(pos); dfree(ustart); dfree(dofs);
+    dfree(uval); dfree(usg);
+    return ok;
+}
+
+// ------------------------------------------------------------ quantised table
+
+__global__ void k_qtable_clear(unsigned long long *keys, unsigned long long *counts, int64_t cap)
+{
+    for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < cap; i += (int64_t)gridDim.x * blockDim.x) {
+        keys[i] = QEMPTY;
+        counts[i] = 0ull;
+    }
+}
+
+__device__ void qinsert(unsigned long long *keys, unsigned long long *counts, int64_t cap, unsigned long long key)
+{
+    unsigned long long h = key * 0x9E3779B97F4A7C15ull;
+    int64_t slot = (int64_t)(h >> 20) & (cap - 1);
+    for (int64_t probe = 0; probe < cap; probe++) {
+        unsigned long long prev = atomicCAS(&keys[slot], QEMPTY, key);
+        if (prev == QEMPTY || prev == key) {
+            atomicAdd(&counts[slot], 1ull);
+            return;
+        }
+        slot = (slot + 1) & (cap - 1);
+    }
+}
+
+// one block per item: count the item's final values into the table
+__global__ void k_requant(const double *vals, const int64_t *val_ofs, const int *item_graph, int n, double eps,
+                          double tol, unsigned long long *keys, unsigned long long *counts, int64_t cap)
+{
+    for (int it = blockIdx.x; it < n; it += gridDim.x) {
+        const unsigned long long gid = (unsigned long long)item_graph[it];
+        for (int64_t i = val_ofs[it] + threadIdx.x; i < val_ofs[it + 1]; i += blockDim.x) {
+            double v = vals[i];
+            double r = v / eps;
+            double q = floor(r);
+            bool bnd = (r - q) * eps < tol || (q + 1.0 - r) * eps < tol;
+            unsigned long long qq = (q < 0.0) ? 0ull : (q >= 1099511627775.0 ? 1099511627775ull : (unsigned long long)q);
+            qinsert(keys, counts, cap, (gid << 41) | ((unsigned long long)bnd << 40) | qq);
+        }
+    }
+}
+
+bool requantise(const double *d_vals, const std::vector<int64_t> &val_ofs, const std::vector<int> &item_parent,
+                const std::vector<int> &graph_of, double eps, double tol, unsigned long long *keys,
+                unsigned long long *counts, int64_t cap, hipStream_t s)
+{
+    const int n = (int)item_parent.size();
+    std::vector<int> item_graph((size_t)n);
+    for (int i = 0; i < n; i++) item_graph[(size_t)i] = graph_of[(size_t)item_parent[(size_t)i]];
+    int64_t *d_ofs = nullptr;
+    int *d_ig = nullptr;
+    bool ok = upload(&d_ofs, val_ofs, s, "value offsets") && upload(&d_ig, item_graph, s, "item graphs");
+    if (ok) {
+        hipLaunchKernelGGL(k_qtable_clear, dim3(grid_for(cap, 4096)), dim3(TPB), 0, s, keys, counts, cap);
+        hipLaunchKernelGGL(k_requant, dim3(std::max(1, std::min(n, 65536))), dim3(TPB), 0, s, d_vals, d_ofs, d_ig, n,
+                           eps, tol, keys, counts, cap);
+        ok = hipGetLastError() == hipSuccess && hipStreamSynchronize(s) == hipSuccess;
+        if (!ok) set_error("quantised histogram recount failed");
+    }
+    dfree(d_ofs);
+    dfree(d_ig);
+    return ok;
+}
+
+__global__ void k_q_flags(const unsigned long long *keys, int64_t cap, int *flag)
+{
+    for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < cap; i += (int64_t)gridDim.x * blockDim.x)
+        flag[i] = keys[i] != QEMPTY ? 1 : 0;
+}
+
+__global__ void k_q_compact(const unsigned long long *keys, const unsigned long long *counts, const int *flag,
+                            const int64_t *pos, int64_t cap, uint64_t *ok, uint64_t *oc)
+{
+    for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < cap; i += (int64_t)gridDim.x * blockDim.x)
+        if (flag[i]) {
+            ok[pos[i]] = keys[i];
+            oc[pos[i]] = counts[i];
+        }
+}
+
+// table -> (key, count) ascending by key
+bool qtable_download(const unsigned long long *keys, const unsigned long long *counts, int64_t cap, hipStream_t s,
+                     std::vector<uint64_t> &hk, std::vector<uint64_t> &hc)
+{
+    int *flag = nullptr;
+    int64_t *pos = nullptr;
+    uint64_t *k0 = nullptr, *c0 = nullptr, *k1 = nullptr, *c1 = nullptr;
+    int64_t n = 0;
+    bool ok = dalloc(&flag, (size_t)cap + 1, "table flags") && dalloc(&pos, (size_t)cap + 1, "table positions");
+    if (ok) {
+        hipLaunchKernelGGL(k_q_flags, dim3(grid_for(cap, 4096)), dim3(TPB), 0, s, keys, cap, flag);
+        hipLaunchKernelGGL(k_zero_int, dim3(1), dim3(64), 0, s, flag + cap, 1);
+        ok = prim_exclusive_scan_i32_to_i64(flag, pos, (size_t)cap + 1, s) &&
+             hipMemcpyAsync(&n, pos + cap, sizeof(int64_t), hipMemcpyDeviceToHost, s) == hipSuccess &&
+             hipStreamSynchronize(s) == hipSuccess;
+    }
+    ok = ok && dalloc(&k0, (size_t)n + 1, "table keys") && dalloc(&c0, (size_t)n + 1, "table counts") &&
+         dalloc(&k1, (size_t)n + 1, "table keys") && dalloc(&c1, (size_t)n + 1, "table counts");
+    if (ok) {
+        hipLaunchKernelGGL(k_q_compact, dim3(grid_for(cap, 4096)), dim3(TPB), 0, s, keys, counts, flag, pos, cap, k0, c0);
+        ok = prim_sort_pairs_u64_u64(k0, k1, c0, c1, (size_t)n, s);
+        hk.resize((size_t)n);
+        hc.resize((size_t)n);
+        ok = ok && (n == 0 || (hipMemcpyAsync(hk.data(), k1, (size_t)n * 8, hipMemcpyDeviceToHost, s) == hipSuccess &&
+                               hipMemcpyAsync(hc.data(), c1, (size_t)n * 8, hipMemcpyDeviceToHost, s) == hipSuccess)) &&
+             hipStreamSynchronize(s) == hipSuccess;
+    }
+    if (!ok && !has_error()) set_error("quantised histogram download failed");
+    dfree(flag); dfree(pos); dfree(k0); dfree(c0); dfree(k1); dfree(c1);
+    return ok;
+}
+
+// ------------------------------------------------------------ runner
+
+struct LiftJob {
+    int kind = LIFT_DELTA;
+    int k = 1;
+    int max_it = 100;
+    double eps = 1e-6;
+    bool mixed = false;
+    double marks[LIFT_KMAX];
+};
+
+struct LiftOut {
+    std::vector<int64_t> val_ofs;        // per item + 1: packed live values
+    double *d_vals = nullptr;            // device, val_ofs.back() entries
+    int reruns = 0;
+    double device_ms = 0.0;
+    int block = 0;
+};
+
+int choose_block(int E)
+{
+    const char *b = getenv("DRESS_HIP_LIFT_BLOCK");
+    if (b) {
+        int v = atoi(b);
+        if (v == 64 || v == 128 || v == 256 || v == 512) return v;
+    }
+    return E <= 512 ? 64 : (E <= 2048 ? 128 : 256);
+}
+
+// Runs every item (parent index, vertex tuple) in one persistent launch (plus
+// an FP64 relaunch of items whose mixed-precision stop was ambiguous).
+bool run_items(const std::vector<const HostParent *> &ps, const std::vector<int> &graph_of,
+               const std::vector<int> &item_parent, const std::vector<int> &item_verts,
+               const std::vector<int64_t> &item_live, const LiftJob &J, double *d_ms, int64_t ms_stride,
+               const std::vector<int64_t> &ms_rows, hipStream_t s, LiftOut &out, unsigned long long *qkeys,
+               unsigned long long *qcounts, int64_t qcap, double qtol)
+{
+    const int n = (int)item_parent.size();
+    out.val_ofs.assign((size_t)n + 1, 0);
+    for (int i = 0; i < n; i++) out.val_ofs[(size_t)i + 1] = out.val_ofs[(size_t)i] + item_live[(size_t)i];
+    if (n == 0) return true;
+    int ecap = 0, ncap = 0, emax = 0;
+    for (const HostParent *P : ps) {
+        ecap = std::max(ecap, P->E);
+        ncap = std::max(ncap, P->N);
+        emax = std::max(emax, P->E);
+    }
+    ecap = (ecap + 1) & ~1;
+    ncap = (ncap + 1) & ~1;
+    std::vector<LiftParent> meta;
+    DevPools D;
+    std::vector<LiftItem> items((size_t)n);
+    for (int i = 0; i < n; i++) {
+        LiftItem &I = items[(size_t)i];
+        I.parent = item_parent[(size_t)i];
+        I.kind = J.kind;
+        I.k = J.k;
+        I.mixed = J.mixed ? 1 : 0;
+        I.vofs = (int64_t)i * J.k;
+        I.val_ofs = out.val_ofs[(size_t)i];
+        I.ms_row = ms_rows.empty() ? 0 : ms_rows[(size_t)i];
+    }
+    // queue order: larger parents first
+    std::vector<int> order((size_t)n);
+    for (int i = 0; i < n; i++) order[(size_t)i] = i;
+    std::stable_sort(order.begin(), order.end(), [&](int a, int b) {
+        return ps[(size_t)item_parent[(size_t)a]]->E > ps[(size_t)item_parent[(size_t)b]]->E;
+    });
+    LiftItem *d_items = nullptr;
+    int *d_order = nullptr, *d_verts = nullptr, *d_queue = nullptr, *d_flags = nullptr;
+    double *d_marks = nullptr;
+    std::vector<double> marks(J.marks, J.marks + LIFT_KMAX);
+    bool ok = upload_parents(ps, graph_of, meta, D, s) && upload(&d_items, items, s, "items") &&
+              upload(&d_order, order, s, "item order") && upload(&d_verts, item_verts, s, "item vertices") &&
+              upload(&d_marks, marks, s, "marker weights") && dalloc(&d_queue, 1, "work queue") &&
+              dalloc(&d_flags, (size_t)n, "item flags") &&
+              dalloc(&out.d_vals, (size_t)out.val_ofs.back() + 1, "subgraph values");
+    hipEvent_t e0 = nullptr, e1 = nullptr;
+    if (ok) ok = hipEventCreate(&e0) == hipSuccess && hipEventCreate(&e1) == hipSuccess;
+    if (ok) {
+        LiftArgs a;
+        memset(&a, 0, sizeof(a));
+        a.parents = D.parents;
+        a.items = d_items;
+        a.order = d_order;
+        a.n_items = n;
+        a.item_verts = d_verts;
+        a.marks = d_marks;
+        a.row_off = D.row_off;
+        a.adj_e = D.adj_e;
+        a.uv = D.uv;
+        a.ew = D.ew;
+        a.ioff = D.ioff;
+        a.pairs = D.pairs;
+        a.queue = d_queue;
+        a.ecap = ecap;
+        a.ncap = ncap;
+        a.max_it = J.max_it;
+        a.eps = J.eps;
+        a.vals = out.d_vals;
+        a.ms = d_ms;
+        a.ms_stride = ms_stride;
+        a.flags = d_flags;
+        a.qkeys = qkeys;
+        a.qcounts = qcounts;
+        a.qcap = qcap;
+        a.qtol = qtol;
+        a.parent_graph = D.parent_graph;
+        int block = choose_block(emax);
+        out.block = block;
+        size_t lds = lifted_lds_bytes(ecap, ncap);
+        int per_cu_lds = (int)std::max<size_t>(1, (160 * 1024) / std::max<size_t>(lds + 1024, 1));
+        int per_cu_waves = std::max(1, 32 / (block / 64));
+        int per_cu = std::max(1, std::min(per_cu_lds, per_cu_waves));
+        int grid = std::max(1, std::min(n, num_cus() * per_cu));
+        ok = hipMemsetAsync(d_queue, 0, sizeof(int), s) == hipSuccess &&
+             hipMemsetAsync(d_flags, 0, (size_t)n * sizeof(int), s) == hipSuccess &&
+             hipEventRecord(e0, s) == hipSuccess && lifted_launch(a, block, grid, s) &&
+             hipEventRecord(e1, s) == hipSuccess && hipEventSynchronize(e1) == hipSuccess;
+        if (ok) {
+            float ms = 0.f;
+            (void)hipEventElapsedTime(&ms, e0, e1);
+            out.device_ms += ms;
+        }
+        // FP64 redo of ambiguous mixed-precision items
+        if (ok && J.mixed) {
+            std::vector<int> flags((size_t)n);
+            ok = hipMemcpyAsync(flags.data(), d_flags, (size_t)n * sizeof(int), hipMemcpyDeviceToHost, s) ==
+                     hipSuccess && hipStreamSynchronize(s) == hipSuccess;
+            std::vector<int> redo;
+            for (int i = 0; i < n && ok; i++)
+                if (flags[(size_t)i]) redo.push_back(i);
+            if (ok && !redo.empty()) {
+                out.reruns = (int)redo.size();
+                std::vector<LiftItem> items2 = items;
+                for (int i : redo) items2[(size_t)i].mixed = 0;
+                int *d_redo = nullptr;
+                ok = hipMemcpyAsync(d_items, items2.data(), items2.size() * sizeof(LiftItem), hipMemcpyHostToDevice,
+                                    s) == hipSuccess && upload(&d_redo, redo, s, "redo list");
+                if (ok) {
+                    a.order = d_redo;
+                    a.n_items = (int)redo.size();
+                    a.qkeys = nullptr;   // counted once, below
+                    ok = hipMemsetAsync(d_queue, 0, sizeof(int), s) == hipSuccess &&
+                         hipEventRecord(e0, s) == hipSuccess &&
+                         lifted_launch(a, block, std::max(1, std::min((int)redo.size(), grid)), s) &&
+                         hipEventRecord(e1, s) == hipSuccess && hipEventSynchronize(e1) == hipSuccess;
+                    if (ok) {
+                        float ms = 0.f;
+                        (void)hipEventElapsedTime(&ms, e0, e1);
+                        out.device_ms += ms;
+                    }
+                }
+                dfree(d_redo);
+                if (ok && qkeys) {
+                    // the quantised counts of redone items were taken from
+                    // their mixed values; recount everything from the final
+                    // values so each value is counted once
+                    ok = requantise(out.d_vals, out.val_ofs, item_parent, graph_of, J.eps, qtol, qkeys, qcounts,
+                                    qcap, s);
+                }
+            }
+        }
+    }
+    if (e0) (void)hipEventDestroy(e0);
+    if (e1) (void)hipEventDestroy(e1);
+    dfree(d_items); dfree(d_order); dfree(d_verts); dfree(d_marks); dfree(d_queue); dfree(d_flags);
+    D.release();
+    if (!ok) {
+        if (!has_error()) set_error("lifted launch failed");
+        dfree(out.d_vals);
+        out.d_vals = nullptr;
+    }
+    return ok;
+}
+
+}  // namespace
+
+}  // namespace dh
+
+namespace dh {
+
+namespace {
+
+std::mutex g_lift_mu;
+int g_lift_mode = 0;   // 0 batched when the parent fits LDS, 1 per-subgraph fits
+
+bool lds_eligible(const HostParent &P)
+{
+    return P.E <= 65535 && lifted_lds_bytes(P.E + 2, P.N + 2) <= 150 * 1024;
+}
+
+// live edge count of each delta item by inclusion-exclusion: E minus the
+// edge records touching a deleted vertex, plus those joining two deleted
+// vertices (counted twice above)
+std::vector<int64_t> delta_live_counts(const p_dress_graph_t g, const std::vector<int> &verts, int k, size_t n)
+{
+    std::vector<int64_t> inc(g->N + 1, 0);
+    std::unordered_map<uint64_t, int64_t> mult;
+    for (int e = 0; e < g->E; e++) {
+        int u = g->U[e], v = g->V[e];
+        inc[u]++;
+        if (v != u) inc[v]++;
+        if (k >= 2 && u != v) {
+            uint64_t key = ((uint64_t)std::min(u, v) << 32) | (uint32_t)std::max(u, v);
+            mult[key]++;
+        }
+    }
+    std::vector<int64_t> live(n);
+    for (size_t i = 0; i < n; i++) {
+        const int *d = &verts[i * (size_t)k];
+        int64_t x = g->E;
+        for (int a = 0; a < k; a++) x -= inc[d[a]];
+        for (int a = 0; a < k; a++)
+            for (int b = a + 1; b < k; b++) {
+                uint64_t key = ((uint64_t)std::min(d[a], d[b]) << 32) | (uint32_t)std::max(d[a], d[b]);
+                auto it = mult.find(key);
+                if (it != mult.end()) x += it->second;
+            }
+        live[i] = x;
+    }
+    return live;
+}
+
+__global__ void k_fill_nan(double *p, int64_t n)
+{
+    for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x)
+        p[i] = __longlong_as_double(0x7ff8000000000000ll);
+}
+
+// nabla_dress_impl.c:33-46
+void marker_weights(double *out, int k)
+{
+    static const int primes[16] = {2, 3, 5, 7, 11, 13, 17, 19, 23, 29, 31, 37, 41, 43, 47, 53};
+    for (int i = 0; i < k && i < LIFT_KMAX; i++)
+        out[i] = (i < 16) ? std::sqrt((double)primes[i]) : std::sqrt((double)(2 * i + 3));
+}
+
+// One subgraph by a separate device fit, as the reference's GPU drivers do
+// (delta_dress_impl.c:95-169 rebuild, then fit_fn): used for parents too
+// large for the batched kernel, and on request for comparison.
+bool plain_item(p_dress_graph_t g, int kind, const int *items, int k, const double *marks, int max_it,
+                double eps, int precision, std::vector<double> &vals, double *ms_row)
+{
+    const int N = g->N, E = g->E;
+    std::vector<int> sU, sV, emap(E, -1);
+    std::vector<double> sW, nw;
+    int sN = N;
+    if (kind == LIFT_DELTA) {
+        std::vector<int> map(N, 0);
+        for (int i = 0; i < k; i++) map[items[i]] = -1;
+        int id = 0;
+        for (int v = 0; v < N; v++)
+            if (map[v] >= 0) map[v] = id++;
+        sN = N - k;
+        for (int e = 0; e < E; e++) {
+            int a = map[g->U[e]], b = map[g->V[e]];
+            if (a >= 0 && b >= 0) {
+                emap[e] = (int)sU.size();
+                sU.push_back(a);
+                sV.push_back(b);
+                if (g->W) sW.push_back(g->W[e]);
+            }
+        }
+        if (sU.empty()) return true;   // delta_dress_impl.c:124-129: no edge, no values
+    } else {
+        sU.assign(g->U, g->U + E);
+        sV.assign(g->V, g->V + E);
+        if (g->W) sW.assign(g->W, g->W + E);
+        nw.assign(N, 1.0);
+        for (int i = 0; i < k; i++) nw[items[i]] = marks[i];
+        for (int e = 0; e < E; e++) emap[e] = e;
+    }
+    DevGraph *G = new DevGraph();
+    std::vector<double> d(sU.size(), 1.0);
+    FitResult r;
+    bool ok = build_graph(G, sN, (int)sU.size(), sU.data(), sV.data(), g->W ? sW.data() : nullptr,
+                          kind == LIFT_NABLA ? nw.data() : nullptr, (int)g->variant) &&
+              fit_upload_values(G, d.data()) && fit_run(G, max_it, eps, precision, &r) &&
+              fit_download(G, d.data(), nullptr);
+    free_graph(G);
+    if (!ok) return false;
+    vals.insert(vals.end(), d.begin(), d.end());
+    if (ms_row)
+        for (int e = 0; e < E; e++) ms_row[e] = emap[e] >= 0 ? d[(size_t)emap[e]] : NAN;
+    return true;
+}
+
+int64_t *flat_from(const std::vector<double> &uv, const std::vector<int64_t> &cnt, int *hist_size)
+{
+    if (uv.empty()) {
+        if (hist_size) *hist_size = 0;
+        return nullptr;
+    }
+    int64_t *flat = (int64_t *)malloc(uv.size() * 2 * sizeof(int64_t));
+    if (!flat) {
+        set_error("host allocation of the histogram failed");
+        return nullptr;
+    }
+    for (size_t i = 0; i < uv.size(); i++) {
+        memcpy(&flat[2 * i], &uv[i], sizeof(double));
+        flat[2 * i + 1] = cnt[i];
+    }
+    if (hist_size) *hist_size = (int)(2 * uv.size());
+    return flat;
+}
+
+bool host_values_histogram(const std::vector<double> &vals, hipStream_t s, std::vector<double> &uv,
+                           std::vector<int64_t> &cnt)
+{
+    double *dv = nullptr;
+    std::vector<int> useg;
+    bool ok = upload(&dv, vals, s, "values") &&
+              exact_histogram(dv, (int64_t)vals.size(), {0, (int64_t)vals.size()}, s, uv, cnt, useg);
+    dfree(dv);
+    return ok;
+}
+
+// delta_dress_impl.c:233-380 / nabla_dress_impl.c:142-311 on the GPU
+int64_t *lifted_flat(int kind, p_dress_graph_t g, int k, int iterations, double epsilon, int n_samples,
+                     unsigned int seed, int *hist_size, int keep_multisets, double **multisets,
+                     int64_t *num_items, int offset, int stride)
+{
+    clear_error();
+    if (hist_size) *hist_size = 0;
+    if (!g) { set_error("NULL graph"); return nullptr; }
+    if (stride <= 0 || offset < 0) { set_error("stride must be positive and offset non-negative"); return nullptr; }
+    if (k > LIFT_KMAX && k < g->N) { set_error("k larger than supported"); return nullptr; }
+    const int N = g->N, E = g->E;
+    const bool do_hist = hist_size != nullptr;
+    const int64_t total = (k == 0) ? 1 : (kind == LIFT_DELTA ? dress_delta_binom(N, k) : dress_nabla_perm_count(N, k));
+    if (num_items) *num_items = total;
+    double *host_ms = nullptr;
+    if (keep_multisets && multisets) {
+        size_t len = (size_t)std::max<int64_t>(total, 0) * (size_t)E;
+        host_ms = (double *)malloc((len ? len : 1) * sizeof(double));
+        if (host_ms) {
+            for (size_t i = 0; i < len; i++) host_ms[i] = NAN;
+            *multisets = host_ms;
+        }
+    }
+    int precision = default_precision();
+    std::vector<double> uvals;
+    std::vector<int64_t> counts;
+    hipStream_t s = nullptr;
+    DevGraph *G = device_graph(g);
+    if (!G) return nullptr;
+    s = G->stream;
+    if (k == 0) {
+        // delta_dress_impl.c:268-286: fit g itself in place
+        if (offset != 0) return nullptr;
+        int it = 0;
+        double dl = 0;
+        if (dress_fit_hip_ex(g, iterations, epsilon, (dress_hip_precision_t)precision, &it, &dl) != 0) return nullptr;
+        std::vector<double> v(g->edge_dress, g->edge_dress + E);
+        if (host_ms) memcpy(host_ms, g->edge_dress, (size_t)E * sizeof(double));
+        if (!do_hist) return nullptr;
+        if (!host_values_histogram(v, s, uvals, counts)) return nullptr;
+        return flat_from(uvals, counts, hist_size);
+    }
+    if (k >= N) return nullptr;     // delta_dress_impl.c:289-293, nabla_dress_impl.c:197-201
+
+    std::vector<int64_t> sel;
+    std::vector<int> verts;
+    enumerate_items(kind, N, k, n_samples, seed, offset, stride, total, sel, verts);
+    LiftJob J;
+    J.kind = kind;
+    J.k = k;
+    J.max_it = iterations;
+    J.eps = epsilon;
+    J.mixed = precision == PREC_FP32_FP64;
+    memset(J.marks, 0, sizeof(J.marks));
+    if (kind == LIFT_NABLA) marker_weights(J.marks, k);
+
+    HostParent P;
+    host_parent_build(N, E, g->U, g->V, g->W, (int)g->variant, P);
+    int mode;
+    {
+        std::lock_guard<std::mutex> lk(g_lift_mu);
+        mode = g_lift_mode;
+    }
+    if (mode == 0 && lds_eligible(P) && iterations > 0) {
+        std::vector<int64_t> live = kind == LIFT_DELTA ? delta_live_counts(g, verts, k, sel.size())
+                                                       : std::vector<int64_t>(sel.size(), (int64_t)E);
+        std::vector<int> item_parent(sel.size(), 0);
+        double *d_ms = nullptr;
+        bool ok = true;
+        if (host_ms && total > 0) {
+            ok = dalloc(&d_ms, (size_t)total * (size_t)E + 1, "multisets");
+            if (ok)
+                hipLaunchKernelGGL(k_fill_nan, dim3(grid_for((int64_t)total * E, 8192)), dim3(TPB), 0, s, d_ms,
+                                   (int64_t)total * E);
+        }
+        LiftOut out;
+        std::vector<const HostParent *> ps{&P};
+        std::vector<int> gof{0};
+        ok = ok && run_items(ps, gof, item_parent, verts, live, J, d_ms, E, sel, s, out, nullptr, nullptr, 0, 0.0);
+        if (ok && host_ms && total > 0)
+            ok = hipMemcpyAsync(host_ms, d_ms, (size_t)total * (size_t)E * sizeof(double), hipMemcpyDeviceToHost,
+                                s) == hipSuccess && hipStreamSynchronize(s) == hipSuccess;
+        std::vector<int> useg;
+        if (ok && do_hist)
+            ok = exact_histogram(out.d_vals, out.val_ofs.back(), {0, out.val_ofs.back()}, s, uvals, counts, useg);
+        dfree(out.d_vals);
+        dfree(d_ms);
+        if (!ok) return nullptr;
+        record_lifted(out.reruns, out.device_ms, (int64_t)sel.size(), out.val_ofs.back());
+    } else {
+        std::vector<double> vals;
+        for (size_t i = 0; i < sel.size(); i++) {
+            double *row = host_ms ? host_ms + (size_t)sel[i] * (size_t)E : nullptr;
+            if (!plain_item(g, kind, &verts[i * (size_t)k], k, J.marks, iterations, epsilon, precision, vals, row))
+                return nullptr;
+        }
+        if (do_hist && !host_values_histogram(vals, s, uvals, counts)) return nullptr;
+        record_lifted(0, 0.0, (int64_t)sel.size(), (int64_t)vals.size());
+    }
+    if (!do_hist) return nullptr;
+    return flat_from(uvals, counts, hist_size);
+}
+
+dress_hist_pair_t *to_pairs(int64_t *flat, int flat_len, int *hist_size)
+{
+    // dress_histogram.c:184-200
+    if (!flat || flat_len <= 0) {
+        free(flat);
+        if (hist_size) *hist_size = 0;
+        return nullptr;
+    }
+    int n = flat_len / 2;
+    dress_hist_pair_t *p = (dress_hist_pair_t *)malloc((size_t)n * sizeof(dress_hist_pair_t));
+    if (!p) {
+        set_error("host allocation of histogram pairs failed");
+        free(flat);
+        if (hist_size) *hist_size = 0;
+        return nullptr;
+    }
+    for (int i = 0; i < n; i++) {
+        memcpy(&p[i].value, &flat[2 * i], sizeof(double));
+        p[i].count = flat[2 * i + 1];
+    }
+    free(flat);
+    if (hist_size) *hist_size = n;
+    return p;
+}
+
+}  // namespace
+
+}  // namespace dh
+
+using namespace dh;
+
+extern "C" {
+
+int64_t *dress_delta_fit_hip_strided_flat(p_dress_graph_t g, int k, int iterations, double epsilon, int n_samples,
+                                          unsigned int seed, int *hist_size, int keep_multisets,
+                                          double **multisets, int64_t *num_subgraphs, int offset, int stride)
+{
+    return lifted_flat(LIFT_DELTA, g, k, iterations, epsilon, n_samples, seed, hist_size, keep_multisets,
+                       multisets, num_subgraphs, offset, stride);
+}
+
+int64_t *dress_delta_fit_hip_flat(p_dress_graph_t g, int k, int iterations, double epsilon, int n_samples,
+                                  unsigned int seed, int *hist_size, int keep_multisets, double **multisets,
+                                  int64_t *num_subgraphs)
+{
+    return lifted_flat(LIFT_DELTA, g, k, iterations, epsilon, n_samples, seed, hist_size, keep_multisets,
+                       multisets, num_subgraphs, 0, 1);
+}
+
+dress_hist_pair_t *dress_delta_fit_hip_strided(p_dress_graph_t g, int k, int iterations, double epsilon,
+                                               int n_samples, unsigned int seed, int *hist_size,
+                                               int keep_multisets, double **multisets, int64_t *num_subgraphs,
+                                               int offset, int stride)
+{
+    int flat_len = 0;
+    int64_t *flat = lifted_flat(LIFT_DELTA, g, k, iterations, epsilon, n_samples, seed,
+                                hist_size ? &flat_len : nullptr, keep_multisets, multisets, num_subgraphs, offset,
+                                stride);
+    return to_pairs(flat, flat_len, hist_size);
+}
+
+dress_hist_pair_t *dress_delta_fit_hip(p_dress_graph_t g, int k, int iterations, double epsilon, int n_samples,
+                                       unsigned int seed, int *hist_size, int keep_multisets, double **multisets,
+                                       int64_t *num_subgraphs)
+{
+    return dress_delta_fit_hip_strided(g, k, iterations, epsilon, n_samples, seed, hist_size, keep_multisets,
+                                       multisets, num_subgraphs, 0, 1);
+}
+
+int64_t *dress_nabla_fit_hip_strided_flat(p_dress_graph_t g, int k, int iterations, double epsilon, int n_samples,
+                                          unsigned int seed, int *hist_size, int keep_multisets,
+                                          double **multisets, int64_t *num_tuples, int offset, int stride)
+{
+    return lifted_flat(LIFT_NABLA, g, k, iterations, epsilon, n_samples, seed, hist_size, keep_multisets,
+                       multisets, num_tuples, offset, stride);
+}
+
+int64_t *dress_nabla_fit_hip_flat(p_dress_graph_t g, int k, int iterations, double epsilon, int n_samples,
+                                  unsigned int seed, int *hist_size, int keep_multisets, double **multisets,
+                                  int64_t *num_tuples)
+{
+    return lifted_flat(LIFT_NABLA, g, k, iterations, epsilon, n_samples, seed, hist_size, keep_multisets,
+                       multisets, num_tuples, 0, 1);
+}
+
+// nabla_dress.c:56-69: the pair form always builds the histogram
+dress_hist_pair_t *dress_nabla_fit_hip_strided(p_dress_graph_t g, int k, int iterations, double epsilon,
+                                               int n_samples, unsigned int seed, int *hist_size,
+                                               int keep_multisets, double **multisets, int64_t *num_tuples,
+                                               int offset, int stride)
+{
+    int flat_len = 0;
+    int64_t *flat = lifted_flat(LIFT_NABLA, g, k, iterations, epsilon, n_samples, seed, &flat_len, keep_multisets,
+                                multisets, num_tuples, offset, stride);
+    return to_pairs(flat, flat_len, hist_size);
+}
+
+dress_hist_pair_t *dress_nabla_fit_hip(p_dress_graph_t g, int k, int iterations, double epsilon, int n_samples,
+                                       unsigned int seed, int *hist_size, int keep_multisets, double **multisets,
+                                       int64_t *num_tuples)
+{
+    return dress_nabla_fit_hip_strided(g, k, iterations, epsilon, n_samples, seed, hist_size, keep_multisets,
+                                       multisets, num_tuples, 0, 1);
+}
+
+// In libdress_hip.so the plain names run on the GPU as well.
+dress_hist_pair_t *dress_delta_fit(p_dress_graph_t g, int k, int iterations, double epsilon, int n_samples,
+                                   unsigned int seed, int *hist_size, int keep_multisets, double **multisets,
+                                   int64_t *num_subgraphs)
+{
+    return dress_delta_fit_hip(g, k, iterations, epsilon, n_samples, seed, hist_size, keep_multisets, multisets,
+                               num_subgraphs);
+}
+
+int64_t *dress_delta_fit_flat(p_dress_graph_t g, int k, int iterations, double epsilon, int n_samples,
+                              unsigned int seed, int *hist_size, int keep_multisets, double **multisets,
+                              int64_t *num_subgraphs)
+{
+    return dress_delta_fit_hip_flat(g, k, iterations, epsilon, n_samples, seed, hist_size, keep_multisets,
+                                    multisets, num_subgraphs);
+}
+
+dress_hist_pair_t *dress_delta_fit_strided(p_dress_graph_t g, int k, int iterations, double epsilon,
+                                           int n_samples, unsigned int seed, int *hist_size, int keep_multisets,
+                                           double **multisets, int64_t *num_subgraphs, int offset, int stride)
+{
+    return dress_delta_fit_hip_strided(g, k, iterations, epsilon, n_samples, seed, hist_size, keep_multisets,
+                                       multisets, num_subgraphs, offset, stride);
+}
+
+int64_t *dress_delta_fit_strided_flat(p_dress_graph_t g, int k, int iterations, double epsilon, int n_samples,
+                                      unsigned int seed, int *hist_size, int keep_multisets, double **multisets,
+                                      int64_t *num_subgraphs, int offset, int stride)
+{
+    return dress_delta_fit_hip_strided_flat(g, k, iterations, epsilon, n_samples, seed, hist_size, keep_multisets,
+                                            multisets, num_subgraphs, offset, stride);
+}
+
+dress_hist_pair_t *dress_nabla_fit(p_dress_graph_t g, int k, int iterations, double epsilon, int n_samples,
+                                   unsigned int seed, int *hist_size, int keep_multisets, double **multisets,
+                                   int64_t *num_tuples)
+{
+    return dress_nabla_fit_hip(g, k, iterations, epsilon, n_samples, seed, hist_size, keep_multisets, multisets,
+                               num_tuples);
+}
+
+int64_t *dress_nabla_fit_flat(p_dress_graph_t g, int k, int iterations, double epsilon, int n_samples,
+                              unsigned int seed, int *hist_size, int keep_multisets, double **multisets,
+                              int64_t *num_tuples)
+{
+    return dress_nabla_fit_hip_flat(g, k, iterations, epsilon, n_samples, seed, hist_size, keep_multisets,
+                                    multisets, num_tuples);
+}
+
+int64_t *dress_nabla_fit_strided_flat(p_dress_graph_t g, int k, int iterations, double epsilon, int n_samples,
+                                      unsigned int seed, int *hist_size, int keep_multisets, double **multisets,
+                                      int64_t *num_tuples, int offset, int stride)
+{
+    return dress_nabla_fit_hip_strided_flat(g, k, iterations, epsilon, n_samples, seed, hist_size, keep_multisets,
+                                            multisets, num_tuples, offset, stride);
+}
+
+void dress_hip_set_lifted_mode(int mode)
+{
+    std::lock_guard<std::mutex> lk(g_lift_mu);
+    g_lift_mode = mode == 1 ? 1 : 0;
+}
+
+}  // extern "C"
+
+namespace dh {
+
+static thread_local dress_hip_lifted_stats_t t_lstats;
+
+void record_lifted(int reruns, double device_ms, int64_t subgraphs, int64_t values)
+{
+    t_lstats.subgraphs = subgraphs;
+    t_lstats.values = values;
+    t_lstats.fp64_reruns = reruns;
+    t_lstats.device_ms = device_ms;
+}
+
+}  // namespace dh
+
+extern "C" {
+
+void dress_hip_last_lifted_stats(dress_hip_lifted_stats_t *out)
+{
+    if (out) *out = dh::t_lstats;
+}
+
+void dress_hip_lifted_batch_free(dress_hip_lifted_batch_t *r)
+{
+    if (!r) return;
+    free(r->hist_offset); free(r->hist_value); free(r->hist_count);
+    free(r->qhist_offset); free(r->qhist_key); free(r->qhist_count);
+    free(r->qbound_offset); free(r->qbound_key); free(r->qbound_count);
+    memset(r, 0, sizeof(*r));
+}
+
+// Many graphs at once: every subgraph of every graph in one persistent launch;
+// per graph the exact (value, count) histogram and the quantised one.
+int dress_hip_lifted_batch(int kind, int k, int n_graphs, const int *N, const int64_t *edge_offset, const int *U,
+                           const int *V, const double *W, dress_variant_t variant, int max_iterations,
+                           double epsilon, dress_hip_precision_t precision, double qtol,
+                           dress_hip_lifted_batch_t *out)
+{
+    using namespace dh;
+    clear_error();
+    if (!out) { set_error("NULL result"); return -1; }
+    memset(out, 0, sizeof(*out));
+    if (n_graphs < 0 || k < 0 || k > LIFT_KMAX || (kind != LIFT_DELTA && kind != LIFT_NABLA)) {
+        set_error("bad batch arguments");
+        return -1;
+    }
+    auto t0 = std::chrono::steady_clock::now();
+    hipStream_t s = nullptr;
+    if (hipStreamCreateWithFlags(&s, hipStreamNonBlocking) != hipSuccess) {
+        set_error("stream creation failed");
+        return -1;
+    }
+    // parents and items
+    std::vector<HostParent> parents((size_t)n_graphs);
+    std::vector<int> graph_of((size_t)n_graphs), item_parent, item_verts;
+    std::vector<int64_t> item_live, ms_rows;
+    for (int gi = 0; gi < n_graphs; gi++) {
+        int64_t eb = edge_offset[gi], ee = edge_offset[gi + 1];
+        int Eg = (int)(ee - eb);
+        host_parent_build(N[gi], Eg, U + eb, V + eb, W ? W + eb : nullptr, (int)variant, parents[(size_t)gi]);
+        graph_of[(size_t)gi] = gi;
+        if (!lds_eligible(parents[(size_t)gi])) {
+            set_error("graph " + std::to_string(gi) + " is too large for the batched kernel");
+            (void)hipStreamDestroy(s);
+            return -1;
+        }
+        if (k == 0 || k >= N[gi]) {
+            if (k == 0) {                    // the graph itself
+                item_parent.push_back(gi);
+                item_live.push_back(Eg);
+            }
+            continue;
+        }
+        int64_t total = kind == LIFT_DELTA ? dress_delta_binom(N[gi], k) : dress_nabla_perm_count(N[gi], k);
+        std::vector<int64_t> sel;
+        std::vector<int> verts;
+        enumerate_items(kind, N[gi], k, 0, 0, 0, 1, total, sel, verts);
+        dress_graph_t tmp;
+        memset(&tmp, 0, sizeof(tmp));
+        tmp.N = N[gi];
+        tmp.E = Eg;
+        tmp.U = const_cast<int *>(U + eb);
+        tmp.V = const_cast<int *>(V + eb);
+        std::vector<int64_t> live = kind == LIFT_DELTA ? delta_live_counts(&tmp, verts, k, sel.size())
+                                                       : std::vector<int64_t>(sel.size(), (int64_t)Eg);
+        for (size_t i = 0; i < sel.size(); i++) {
+            item_parent.push_back(gi);
+            item_live.push_back(live[i]);
+        }
+        item_verts.insert(item_verts.end(), verts.begin(), verts.end());
+    }
+    const int kk = std::max(k, 0);
+    if (k == 0) item_verts.assign(item_parent.size() * (size_t)kk, 0);
+    LiftJob J;
+    J.kind = kind;
+    J.k = (k == 0) ? 0 : k;
+    J.max_it = max_iterations;
+    J.eps = epsilon;
+    J.mixed = precision == DRESS_HIP_FP32_FP64;
+    memset(J.marks, 0, sizeof(J.marks));
+    if (kind == LIFT_NABLA) marker_weights(J.marks, k);
+    std::vector<const HostParent *> ps;
+    for (auto &P : parents) ps.push_back(&P);
+    int64_t total_vals = 0;
+    for (int64_t x : item_live) total_vals += x;
+    int64_t qcap = 1024;
+    while (qcap < 2 * total_vals) qcap <<= 1;
+    unsigned long long *qk = nullptr, *qc = nullptr;
+    LiftOut lo;
+    bool ok = dalloc(&qk, (size_t)qcap, "quantised table") && dalloc(&qc, (size_t)qcap, "quantised table");
+    if (ok) {
+        hipLaunchKernelGGL(k_qtable_clear, dim3(grid_for(qcap, 4096)), dim3(TPB), 0, s, qk, qc, qcap);
+        ok = run_items(ps, graph_of, item_parent, item_verts, item_live, J, nullptr, 0, ms_rows, s, lo, qk, qc, qcap,
+                       qtol);
+    }
+    // exact histogram per graph: items of a graph are contiguous
+    std::vector<int64_t> seg_ofs((size_t)n_graphs + 1, 0);
+    if (ok) {
+        size_t it = 0;
+        for (int gi = 0; gi < n_graphs; gi++) {
+            while (it < item_parent.size() && item_parent[it] == gi) it++;
+            seg_ofs[(size_t)gi + 1] = lo.val_ofs[it];
+        }
+    }
+    std::vector<double> uv;
+    std::vector<int64_t> cnt;
+    std::vector<int> useg;
+    if (ok && n_graphs > 0) ok = exact_histogram(lo.d_vals, lo.val_ofs.back(), seg_ofs, s, uv, cnt, useg);
+    std::vector<uint64_t> hk, hc;
+    if (ok) ok = qtable_download(qk, qc, qcap, s, hk, hc);
+    dfree(qk); dfree(qc); dfree(lo.d_vals);
+    (void)hipStreamDestroy(s);
+    if (!ok) return -1;
+    // assemble
+    out->n_graphs = n_graphs;
+    out->hist_offset = (int64_t *)calloc((size_t)n_graphs + 1, sizeof(int64_t));
+    out->hist_value = (double *)malloc((uv.size() + 1) * sizeof(double));
+    out->hist_count = (int64_t *)malloc((uv.size() + 1) * sizeof(int64_t));
+    out->qhist_offset = (int64_t *)calloc((size_t)n_graphs + 1, sizeof(int64_t));
+    out->qbound_offset = (int64_t *)calloc((size_t)n_graphs + 1, sizeof(int64_t));
+    out->qhist_key = (int64_t *)malloc((hk.size() + 1) * sizeof(int64_t));
+    out->qhist_count = (int64_t *)malloc((hk.size() + 1) * sizeof(int64_t));
+    out->qbound_key = (int64_t *)malloc((hk.size() + 1) * sizeof(int64_t));
+    out->qbound_count = (int64_t *)malloc((hk.size() + 1) * sizeof(int64_t));
+    if (!out->hist_offset || !out->hist_value || !out->hist_count || !out->qhist_offset || !out->qbound_offset ||
+        !out->qhist_key || !out->qhist_count || !out->qbound_key || !out->qbound_count) {
+        dress_hip_lifted_batch_free(out);
+        set_error("host allocation of batch results failed");
+        return -1;
+    }
+    for (size_t i = 0; i < uv.size(); i++) {
+        out->hist_value[i] = uv[i];
+        out->hist_count[i] = cnt[i];
+        out->hist_offset[useg[i] + 1]++;
+    }
+    for (int gi = 0; gi < n_graphs; gi++) out->hist_offset[gi + 1] += out->hist_offset[gi];
+    // table keys ascend by (graph, boundary flag, bin)
+    size_t nq = 0, nb = 0;
+    for (size_t i = 0; i < hk.size(); i++) {
+        int gi = (int)(hk[i] >> 41);
+        bool bnd = (hk[i] >> 40) & 1;
+        int64_t q = (int64_t)(hk[i] & 0xffffffffffull);
+        if (gi < 0 || gi >= n_graphs) continue;
+        if (bnd) {
+            out->qbound_key[nb] = q;
+            out->qbound_count[nb++] = (int64_t)hc[i];
+            out->qbound_offset[gi + 1]++;
+        } else {
+            out->qhist_key[nq] = q;
+            out->qhist_count[nq++] = (int64_t)hc[i];
+            out->qhist_offset[gi + 1]++;
+        }
+    }
+    for (int gi = 0; gi < n_graphs; gi++) {
+        out->qhist_offset[gi + 1] += out->qhist_offset[gi];
+        out->qbound_offset[gi + 1] += out->qbound_offset[gi];
+    }
+    out->subgraphs = (int64_t)item_parent.size();
+    out->values = lo.val_ofs.back();
+    out->fp64_reruns = lo.reruns;
+    out->device_ms = lo.device_ms;
+    out->total_ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
+    record_lifted(lo.reruns, lo.device_ms, out->subgraphs, out->values);
+    return 0;
+}
+
+}  // extern "C"

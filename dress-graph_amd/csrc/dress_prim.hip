@@ -23,6 +23,34 @@ bool prim_sort_pairs_u64_i32(const uint64_t *kin, uint64_t *kout, const int *vin
     return true;
 }
 
+bool prim_sort_pairs_i32_u64(const int *kin, int *kout, const uint64_t *vin, uint64_t *vout, size_t n,
+                             int end_bit, hipStream_t s)
+{
+    if (n == 0) return true;
+    size_t tmp = 0;
+    DH_CHECK(rocprim::radix_sort_pairs(nullptr, tmp, kin, kout, vin, vout, n, 0, end_bit, s));
+    void *buf = nullptr;
+    if (!dmalloc(&buf, tmp + 16, "radix sort scratch")) return false;
+    hipError_t e = rocprim::radix_sort_pairs(buf, tmp, kin, kout, vin, vout, n, 0, end_bit, s);
+    dfree(buf);
+    DH_CHECK(e);
+    return true;
+}
+
+bool prim_sort_pairs_u64_u64(const uint64_t *kin, uint64_t *kout, const uint64_t *vin, uint64_t *vout,
+                             size_t n, hipStream_t s)
+{
+    if (n == 0) return true;
+    size_t tmp = 0;
+    DH_CHECK(rocprim::radix_sort_pairs(nullptr, tmp, kin, kout, vin, vout, n, 0, 64, s));
+    void *buf = nullptr;
+    if (!dmalloc(&buf, tmp + 16, "radix sort scratch")) return false;
+    hipError_t e = rocprim::radix_sort_pairs(buf, tmp, kin, kout, vin, vout, n, 0, 64, s);
+    dfree(buf);
+    DH_CHECK(e);
+    return true;
+}
+
 bool prim_sort_u64(const uint64_t *kin, uint64_t *kout, size_t n, hipStream_t s)
 {
     if (n == 0) return true;

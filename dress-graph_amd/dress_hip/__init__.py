@@ -89,6 +89,74 @@ def last_fit_stats():
                     s.intersections, s.half_edges)
 
 
+@dataclass
+class LiftedBatchResult:
+    """Per-graph outputs of :func:`lifted_batch`."""
+    histograms: List[Tuple[np.ndarray, np.ndarray]]     # exact (values, counts)
+    quantised: List[Tuple[np.ndarray, np.ndarray]]      # floor(d/eps) keys, counts
+    boundary: List[Tuple[np.ndarray, np.ndarray]]       # keys, counts of values near a bin edge
+    subgraphs: int
+    values: int
+    fp64_reruns: int
+    device_ms: float
+    total_ms: float
+
+
+def set_lifted_mode(mode):
+    """0: batched persistent kernel (default); 1: one device fit per subgraph."""
+    _lib.lib().dress_hip_set_lifted_mode(int(mode))
+
+
+def last_lifted_stats():
+    s = _lib.LiftedStats()
+    _lib.lib().dress_hip_last_lifted_stats(ctypes.byref(s))
+    return {"subgraphs": s.subgraphs, "values": s.values, "fp64_reruns": s.fp64_reruns,
+            "device_ms": s.device_ms}
+
+
+def lifted_batch(graphs, k=1, kind="delta", variant=UNDIRECTED, max_iterations=100, epsilon=1e-6,
+                 precision=FP64, qtol=1e-9, weights=None):
+    """delta^k / nabla^k of many graphs in one batched launch.
+
+    ``graphs`` is a list of ``(n, U, V)``; ``weights`` an optional list of
+    per-graph weight arrays.  Returns a :class:`LiftedBatchResult`."""
+    L = _lib.lib()
+    n = len(graphs)
+    Ns = np.array([g[0] for g in graphs], dtype=np.int32)
+    offs = np.zeros(n + 1, dtype=np.int64)
+    for i, g in enumerate(graphs):
+        offs[i + 1] = offs[i] + len(g[1])
+    U = np.ascontiguousarray(np.concatenate([np.asarray(g[1], dtype=np.int32) for g in graphs])
+                             if n else np.zeros(0, np.int32))
+    V = np.ascontiguousarray(np.concatenate([np.asarray(g[2], dtype=np.int32) for g in graphs])
+                             if n else np.zeros(0, np.int32))
+    W = None
+    if weights is not None:
+        W = np.ascontiguousarray(np.concatenate([np.asarray(w, dtype=np.float64) for w in weights]))
+    r = _lib.LiftedBatch()
+    rc = L.dress_hip_lifted_batch(0 if kind == "delta" else 1, int(k), n,
+                                  Ns.ctypes.data_as(c_int_p), offs.ctypes.data_as(_lib.c_i64_p),
+                                  U.ctypes.data_as(c_int_p), V.ctypes.data_as(c_int_p),
+                                  W.ctypes.data_as(c_dbl_p) if W is not None else c_dbl_p(),
+                                  int(variant), int(max_iterations), float(epsilon), int(precision),
+                                  float(qtol), ctypes.byref(r))
+    _lib.check(rc == 0, "dress_hip_lifted_batch")
+    try:
+        def split(off_p, a_p, b_p, dt_a):
+            off = np.ctypeslib.as_array(off_p, shape=(n + 1,)).copy()
+            tot = int(off[-1])
+            a = np.ctypeslib.as_array(a_p, shape=(max(tot, 1),))[:tot].astype(dt_a, copy=True)
+            b = np.ctypeslib.as_array(b_p, shape=(max(tot, 1),))[:tot].astype(np.int64, copy=True)
+            return [(a[off[i]:off[i + 1]], b[off[i]:off[i + 1]]) for i in range(n)]
+        hists = split(r.hist_offset, r.hist_value, r.hist_count, np.float64)
+        quant = split(r.qhist_offset, r.qhist_key, r.qhist_count, np.int64)
+        bound = split(r.qbound_offset, r.qbound_key, r.qbound_count, np.int64)
+        return LiftedBatchResult(hists, quant, bound, r.subgraphs, r.values, r.fp64_reruns,
+                                 r.device_ms, r.total_ms)
+    finally:
+        L.dress_hip_lifted_batch_free(ctypes.byref(r))
+
+
 class Graph:
     """Owns one dress_graph_t built by libdress_hip.so (device-resident)."""
 

@@ -45,6 +45,27 @@ class FitStats(ctypes.Structure):
                 ("intersections", ctypes.c_int64), ("half_edges", ctypes.c_int64)]
 
 
+class LiftedStats(ctypes.Structure):
+    _fields_ = [("subgraphs", ctypes.c_int64), ("values", ctypes.c_int64),
+                ("fp64_reruns", ctypes.c_int), ("device_ms", ctypes.c_double)]
+
+
+class LiftedBatch(ctypes.Structure):
+    _fields_ = [("n_graphs", ctypes.c_int),
+                ("hist_offset", ctypes.POINTER(ctypes.c_int64)),
+                ("hist_value", ctypes.POINTER(ctypes.c_double)),
+                ("hist_count", ctypes.POINTER(ctypes.c_int64)),
+                ("qhist_offset", ctypes.POINTER(ctypes.c_int64)),
+                ("qhist_key", ctypes.POINTER(ctypes.c_int64)),
+                ("qhist_count", ctypes.POINTER(ctypes.c_int64)),
+                ("qbound_offset", ctypes.POINTER(ctypes.c_int64)),
+                ("qbound_key", ctypes.POINTER(ctypes.c_int64)),
+                ("qbound_count", ctypes.POINTER(ctypes.c_int64)),
+                ("subgraphs", ctypes.c_int64), ("values", ctypes.c_int64),
+                ("fp64_reruns", ctypes.c_int), ("device_ms", ctypes.c_double),
+                ("total_ms", ctypes.c_double)]
+
+
 PG = ctypes.POINTER(DressGraph)
 
 _lib = None
@@ -90,6 +111,14 @@ def _bind(L):
                                          c_int_p, c_dbl_p]
     L.dress_hip_resident_download.restype = ctypes.c_int
     L.dress_hip_resident_download.argtypes = [PG]
+    L.dress_hip_set_lifted_mode.argtypes = [ctypes.c_int]
+    L.dress_hip_last_lifted_stats.argtypes = [ctypes.POINTER(LiftedStats)]
+    L.dress_hip_lifted_batch.restype = ctypes.c_int
+    L.dress_hip_lifted_batch.argtypes = [ctypes.c_int, ctypes.c_int, ctypes.c_int, c_int_p, c_i64_p,
+                                         c_int_p, c_int_p, c_dbl_p, ctypes.c_int, ctypes.c_int,
+                                         ctypes.c_double, ctypes.c_int, ctypes.c_double,
+                                         ctypes.POINTER(LiftedBatch)]
+    L.dress_hip_lifted_batch_free.argtypes = [ctypes.POINTER(LiftedBatch)]
     L.dress_delta_binom.restype = ctypes.c_int64
     L.dress_delta_binom.argtypes = [ctypes.c_int, ctypes.c_int]
     L.dress_nabla_perm_count.restype = ctypes.c_int64

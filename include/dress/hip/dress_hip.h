@@ -159,6 +159,58 @@ int dress_hip_fit_resident(p_dress_graph_t g, int max_iterations, double epsilon
                            int *iterations, double *delta);
 int dress_hip_resident_download(p_dress_graph_t g);
 
+/* How dress_{delta,nabla}_fit_hip* run their subgraphs:
+ *   0 (default) one batched persistent launch when the graph fits the
+ *     kernel's LDS state (up to 65535 edges and ~150 KB of state), otherwise
+ *     one device fit per subgraph;
+ *   1 always one device fit per subgraph (the structure of the CUDA backend,
+ *     libdress/src/cuda/delta_dress_cuda.c:24-51), kept for comparison. */
+void dress_hip_set_lifted_mode(int mode);
+
+/* Counters of the calling thread's last lifted call. */
+typedef struct {
+    int64_t subgraphs;            /* subgraphs fitted                         */
+    int64_t values;               /* live edge values produced                */
+    int     fp64_reruns;          /* mixed-precision subgraphs redone in FP64 */
+    double  device_ms;            /* time of the batched launch(es)           */
+} dress_hip_lifted_stats_t;
+void dress_hip_last_lifted_stats(dress_hip_lifted_stats_t *out);
+
+/* delta^k (kind 0) or nabla^k (kind 1) of many graphs in one batched
+ * launch.  Graph g has N[g] vertices and the edge records
+ * U/V/W[edge_offset[g] .. edge_offset[g+1]) (W may be NULL); all graphs use
+ * `variant`.  Every subgraph of every graph is fitted (no sampling or
+ * striding).  Per graph the result holds
+ *   - the exact histogram: distinct values ascending with their counts;
+ *   - the quantised histogram: counts of floor(d / epsilon), accumulated on
+ *     the device in integer atomics, for values farther than qtol from a bin
+ *     edge; values within qtol of an edge are counted separately in
+ *     qbound (their bin depends on the last bits of d).
+ * Arrays are malloc'd; release them with dress_hip_lifted_batch_free.
+ * Returns 0, or -1 (see dress_hip_last_error). */
+typedef struct {
+    int      n_graphs;
+    int64_t *hist_offset;         /* [n_graphs+1] into hist_value/hist_count  */
+    double  *hist_value;
+    int64_t *hist_count;
+    int64_t *qhist_offset;        /* [n_graphs+1] into qhist_key/qhist_count  */
+    int64_t *qhist_key;
+    int64_t *qhist_count;
+    int64_t *qbound_offset;       /* [n_graphs+1] into qbound_key/qbound_count */
+    int64_t *qbound_key;
+    int64_t *qbound_count;
+    int64_t  subgraphs;
+    int64_t  values;
+    int      fp64_reruns;
+    double   device_ms;           /* batched launch(es)                       */
+    double   total_ms;            /* whole call, wall clock                   */
+} dress_hip_lifted_batch_t;
+int dress_hip_lifted_batch(int kind, int k, int n_graphs, const int *N, const int64_t *edge_offset,
+                           const int *U, const int *V, const double *W, dress_variant_t variant,
+                           int max_iterations, double epsilon, dress_hip_precision_t precision,
+                           double qtol, dress_hip_lifted_batch_t *out);
+void dress_hip_lifted_batch_free(dress_hip_lifted_batch_t *r);
+
 #ifdef __cplusplus
 }
 #endif
