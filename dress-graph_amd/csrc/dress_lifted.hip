@@ -1304,3 +1304,30 @@ int dress_hip_lifted_batch(int kind, int k, int n_graphs, const int *N, const in
 }
 
 }  // extern "C"
+
+extern "C" {
+
+// Host-only: the subgraph indices (and vertex tuples) a lifted call with
+// these arguments processes, in processing order.  Returns the count, or -1.
+int64_t dress_hip_enumerate_subgraphs(int kind, int N, int k, int n_samples, unsigned int seed, int offset,
+                                      int stride, int64_t *sel, int *verts, int64_t cap)
+{
+    using namespace dh;
+    clear_error();
+    if (N < 0 || k < 1 || k >= N || stride <= 0 || offset < 0 || (kind != LIFT_DELTA && kind != LIFT_NABLA)) {
+        set_error("bad enumeration arguments");
+        return -1;
+    }
+    int64_t total = kind == LIFT_DELTA ? dress_delta_binom(N, k) : dress_nabla_perm_count(N, k);
+    std::vector<int64_t> s;
+    std::vector<int> v;
+    enumerate_items(kind, N, k, n_samples, seed, offset, stride, total, s, v);
+    int64_t n = (int64_t)s.size();
+    for (int64_t i = 0; i < n && i < cap; i++) {
+        if (sel) sel[i] = s[(size_t)i];
+        if (verts) memcpy(verts + i * k, &v[(size_t)i * k], (size_t)k * sizeof(int));
+    }
+    return n;
+}
+
+}  // extern "C"

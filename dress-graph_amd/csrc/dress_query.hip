@@ -1,0 +1,177 @@
+// dress_query.hip -- dress_get on the device, one thread per query
+// (libdress/src/dress.c:651-739).
+//
+// A query (u, v) returns the value of an existing edge found by the same
+// binary search as the reference, or, for a non-edge, the 1-D fixed point of
+// the virtual edge against the frozen state: A = sum over common neighbours of
+// w-bar*d(u,x) + w-bar*d(v,x) plus the self term, then
+//   d <- (A + cross * cw * d) / (sqrt(|u|^2 + cw*d) * sqrt(|v|^2 + cw*d)).
+// The state is the graph's host values (edge_dress, vertex_dress), uploaded
+// before the queries, so results match what the reference computes on g.
+#include "dress_capi_internal.h"
+#include "dress/hip/dress_hip.h"
+
+#include <cmath>
+#include <cstring>
+#include <vector>
+
+namespace dh {
+
+namespace {
+
+constexpr int TPB = 256;
+
+__device__ __forceinline__ void two_sum_add(double &s, double &c, double x)
+{
+    double t = s + x;
+    double z = t - s;
+    double e = (s - (t - z)) + (x - z);
+    s = t;
+    c += e;
+}
+
+__global__ void k_get(int n, const int *__restrict__ qu, const int *__restrict__ qv, const double *__restrict__ qw,
+                      int max_it, double eps, const int *__restrict__ row_off, const int *__restrict__ adj_x,
+                      const int *__restrict__ adj_e, const int *__restrict__ inv, const double *__restrict__ d,
+                      const double *__restrict__ wh, const double *__restrict__ vdn, const double *__restrict__ nw,
+                      int variant, double *out)
+{
+    for (int q = blockIdx.x * blockDim.x + threadIdx.x; q < n; q += gridDim.x * blockDim.x) {
+        const int u = qu[q], v = qv[q];
+        // dress.c:656-667: existing edge
+        int lo = row_off[u], hi = row_off[u + 1] - 1;
+        double res = 0.0;
+        bool found = false;
+        while (lo <= hi) {
+            int mid = lo + (hi - lo) / 2;
+            int nb = adj_x[mid];
+            if (nb == v) {
+                res = d[inv[adj_e[mid]]];
+                found = true;
+                break;
+            }
+            if (nb < v) lo = mid + 1; else hi = mid - 1;
+        }
+        if (!found) {
+            // dress.c:671-737: virtual edge
+            int a = row_off[u], ae = row_off[u + 1];
+            int b = row_off[v], be = row_off[v + 1];
+            double s = 0.0, c = 0.0;
+            while (a < ae && b < be) {
+                int x = adj_x[a], y = adj_x[b];
+                if (x == y) {
+                    two_sum_add(s, c, wh[a] + wh[b]);
+                    a++;
+                    b++;
+                } else if (x < y) a++;
+                else b++;
+            }
+            const bool dir = (variant == DRESS_VARIANT_FORWARD || variant == DRESS_VARIANT_BACKWARD);
+            two_sum_add(s, c, dir ? 4.0 * nw[u] : 4.0 * nw[u] + 4.0 * nw[v]);
+            const double A = s + c;
+            const double cross = dir ? 1.0 : 2.0;
+            const double cw = (variant == DRESS_VARIANT_UNDIRECTED) ? 2.0 * qw[q] : qw[q];
+            const double du2 = vdn[u] * vdn[u];
+            const double dv2 = vdn[v] * vdn[v];
+            if (du2 <= 0.0 || dv2 <= 0.0) {
+                res = 0.0;
+            } else {
+                double x = 1.0;
+                res = x;
+                for (int it = 0; it < max_it; it++) {
+                    double wd = cw * x;
+                    double den = sqrt(du2 + wd) * sqrt(dv2 + wd);
+                    if (den <= 0.0) { x = 0.0; break; }
+                    double nx = (A + cross * wd) / den;
+                    if (fabs(nx - x) <= eps) { x = nx; break; }
+                    x = nx;
+                }
+                res = x;
+            }
+        }
+        out[q] = res;
+    }
+}
+
+__global__ void k_wh_from_d(int E, const double *ew, const int2 *pos, const double *d, double *wh)
+{
+    for (int e = blockIdx.x * blockDim.x + threadIdx.x; e < E; e += gridDim.x * blockDim.x) {
+        double x = ew[e] * d[e];
+        int2 q = pos[e];
+        wh[q.x] = x;
+        if (q.y >= 0) wh[q.y] = x;
+    }
+}
+
+}  // namespace
+
+// queries against g's host state
+bool get_batch(p_dress_graph_t g, int n, const int *u, const int *v, const double *w, int max_it, double eps,
+               double *out)
+{
+    for (int i = 0; i < n; i++) {
+        if (u[i] < 0 || u[i] >= g->N || v[i] < 0 || v[i] >= g->N) {
+            set_error("dress_get: vertex out of range");
+            return false;
+        }
+    }
+    DevGraph *G = device_graph(g);
+    if (!G) return false;
+    if (!refresh_self_terms(G, g->NW) || !fit_upload_values(G, g->edge_dress)) return false;
+    hipStream_t s = G->stream;
+    int *du = nullptr, *dv = nullptr;
+    double *dw = nullptr, *dout = nullptr, *dvd = nullptr, *wh = nullptr;
+    bool ok = dalloc(&du, (size_t)n + 1, "queries") && dalloc(&dv, (size_t)n + 1, "queries") &&
+              dalloc(&dw, (size_t)n + 1, "queries") && dalloc(&dout, (size_t)n + 1, "answers") &&
+              dalloc(&dvd, (size_t)g->N + 1, "norms") && dalloc(&wh, (size_t)G->S + 1, "half-edge values");
+    if (ok) {
+        ok = hipMemcpyAsync(du, u, (size_t)n * sizeof(int), hipMemcpyHostToDevice, s) == hipSuccess &&
+             hipMemcpyAsync(dv, v, (size_t)n * sizeof(int), hipMemcpyHostToDevice, s) == hipSuccess &&
+             hipMemcpyAsync(dw, w, (size_t)n * sizeof(double), hipMemcpyHostToDevice, s) == hipSuccess &&
+             (g->N == 0 || hipMemcpyAsync(dvd, g->vertex_dress, (size_t)g->N * sizeof(double),
+                                          hipMemcpyHostToDevice, s) == hipSuccess);
+        if (ok && G->E > 0)
+            hipLaunchKernelGGL(k_wh_from_d, dim3(std::min(8192, (G->E + TPB - 1) / TPB)), dim3(TPB), 0, s, G->E,
+                               G->ew, G->pos, G->d[G->cur], wh);
+        if (ok && n > 0)
+            hipLaunchKernelGGL(k_get, dim3(std::min(8192, (n + TPB - 1) / TPB)), dim3(TPB), 0, s, n, du, dv, dw,
+                               max_it, eps, G->row_off, G->adj_x, G->adj_e, G->inv, G->d[G->cur], wh, dvd, G->nw,
+                               G->variant, dout);
+        ok = ok && hipGetLastError() == hipSuccess &&
+             hipMemcpyAsync(out, dout, (size_t)n * sizeof(double), hipMemcpyDeviceToHost, s) == hipSuccess &&
+             hipStreamSynchronize(s) == hipSuccess;
+        if (!ok) set_error("dress_get kernel failed");
+    }
+    dfree(du); dfree(dv); dfree(dw); dfree(dout); dfree(dvd); dfree(wh);
+    return ok;
+}
+
+}  // namespace dh
+
+extern "C" {
+
+// dress.h:105-107.  Returns NaN on error (dress_hip_last_error()).
+double dress_get(const p_dress_graph_t g, int u, int v, int max_iterations, double epsilon, double edge_weight)
+{
+    dh::clear_error();
+    double r = NAN;
+    if (!g) {
+        dh::set_error("dress_get: NULL graph");
+        return r;
+    }
+    if (!dh::get_batch(g, 1, &u, &v, &edge_weight, max_iterations, epsilon, &r)) return NAN;
+    return r;
+}
+
+int dress_hip_get_batch(p_dress_graph_t g, int n, const int *u, const int *v, const double *edge_weight,
+                        int max_iterations, double epsilon, double *out)
+{
+    dh::clear_error();
+    if (!g || n < 0 || (n > 0 && (!u || !v || !edge_weight || !out))) {
+        dh::set_error("dress_hip_get_batch: bad arguments");
+        return -1;
+    }
+    return dh::get_batch(g, n, u, v, edge_weight, max_iterations, epsilon, out) ? 0 : -1;
+}
+
+}  // extern "C"

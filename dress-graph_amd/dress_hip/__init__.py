@@ -107,6 +107,20 @@ def set_lifted_mode(mode):
     _lib.lib().dress_hip_set_lifted_mode(int(mode))
 
 
+def enumerate_subgraphs(kind, n, k, n_samples=0, seed=0, offset=0, stride=1):
+    """Host-side: (indices, vertex tuples) a lifted call would process."""
+    L = _lib.lib()
+    kd = 0 if kind == "delta" else 1
+    cnt = L.dress_hip_enumerate_subgraphs(kd, n, k, n_samples, ctypes.c_uint(seed), offset, stride,
+                                          None, None, 0)
+    _lib.check(cnt >= 0, "dress_hip_enumerate_subgraphs")
+    sel = np.zeros(max(cnt, 1), dtype=np.int64)
+    verts = np.zeros(max(cnt, 1) * k, dtype=np.int32)
+    L.dress_hip_enumerate_subgraphs(kd, n, k, n_samples, ctypes.c_uint(seed), offset, stride,
+                                    sel.ctypes.data_as(_lib.c_i64_p), verts.ctypes.data_as(c_int_p), cnt)
+    return sel[:cnt], verts[:cnt * k].reshape(cnt, k)
+
+
 def last_lifted_stats():
     s = _lib.LiftedStats()
     _lib.lib().dress_hip_last_lifted_stats(ctypes.byref(s))
@@ -278,6 +292,19 @@ class Graph:
         if err:
             raise HipError(f"dress_get failed: {err}")
         return val
+
+    def get_batch(self, u, v, max_iterations=100, epsilon=1e-6, edge_weight=1.0):
+        """dress_get for arrays of pairs in one device launch."""
+        u = np.ascontiguousarray(u, dtype=np.int32)
+        v = np.ascontiguousarray(v, dtype=np.int32)
+        w = np.ascontiguousarray(np.broadcast_to(np.asarray(edge_weight, dtype=np.float64), u.shape))
+        out = np.empty(u.size, dtype=np.float64)
+        r = _lib.lib().dress_hip_get_batch(self.g, int(u.size), u.ctypes.data_as(c_int_p),
+                                           v.ctypes.data_as(c_int_p), w.ctypes.data_as(c_dbl_p),
+                                           int(max_iterations), float(epsilon),
+                                           out.ctypes.data_as(c_dbl_p))
+        _lib.check(r == 0, "dress_hip_get_batch")
+        return out
 
     def lifted(self, kind, k, max_iterations=100, epsilon=1e-6, n_samples=0, seed=0,
                keep_multisets=False, compute_histogram=True, offset=0, stride=1):

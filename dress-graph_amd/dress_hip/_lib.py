@@ -112,6 +112,10 @@ def _bind(L):
     L.dress_hip_resident_download.restype = ctypes.c_int
     L.dress_hip_resident_download.argtypes = [PG]
     L.dress_hip_set_lifted_mode.argtypes = [ctypes.c_int]
+    L.dress_hip_enumerate_subgraphs.restype = ctypes.c_int64
+    L.dress_hip_enumerate_subgraphs.argtypes = [ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_int,
+                                                ctypes.c_uint, ctypes.c_int, ctypes.c_int, c_i64_p,
+                                                c_int_p, ctypes.c_int64]
     L.dress_hip_last_lifted_stats.argtypes = [ctypes.POINTER(LiftedStats)]
     L.dress_hip_lifted_batch.restype = ctypes.c_int
     L.dress_hip_lifted_batch.argtypes = [ctypes.c_int, ctypes.c_int, ctypes.c_int, c_int_p, c_i64_p,
@@ -123,10 +127,12 @@ def _bind(L):
     L.dress_delta_binom.argtypes = [ctypes.c_int, ctypes.c_int]
     L.dress_nabla_perm_count.restype = ctypes.c_int64
     L.dress_nabla_perm_count.argtypes = [ctypes.c_int, ctypes.c_int]
-    if hasattr(L, "dress_get"):
-        L.dress_get.restype = ctypes.c_double
-        L.dress_get.argtypes = [PG, ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_double,
-                                ctypes.c_double]
+    L.dress_get.restype = ctypes.c_double
+    L.dress_get.argtypes = [PG, ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_double,
+                            ctypes.c_double]
+    L.dress_hip_get_batch.restype = ctypes.c_int
+    L.dress_hip_get_batch.argtypes = [PG, ctypes.c_int, c_int_p, c_int_p, c_dbl_p, ctypes.c_int,
+                                      ctypes.c_double, c_dbl_p]
     strided = _lifted_args() + [ctypes.c_int, ctypes.c_int]
     for name, args, res in (
             ("dress_delta_fit_hip", _lifted_args(), ctypes.POINTER(HistPair)),
@@ -144,10 +150,9 @@ def _bind(L):
             ("dress_nabla_fit", _lifted_args(), ctypes.POINTER(HistPair)),
             ("dress_nabla_fit_flat", _lifted_args(), c_i64_p),
             ("dress_nabla_fit_strided_flat", strided, c_i64_p)):
-        if hasattr(L, name):
-            f = getattr(L, name)
-            f.restype = res
-            f.argtypes = args
+        f = getattr(L, name)
+        f.restype = res
+        f.argtypes = args
 
 
 def lib():

@@ -159,6 +159,14 @@ int dress_hip_fit_resident(p_dress_graph_t g, int max_iterations, double epsilon
                            int *iterations, double *delta);
 int dress_hip_resident_download(p_dress_graph_t g);
 
+/* dress_get for many pairs at once (libdress/src/dress.c:651-739 per
+ * query), against g's current edge_dress / vertex_dress.  out[i] receives
+ * the value of (u[i], v[i]) with hypothetical weight edge_weight[i].
+ * Returns 0, or -1 (see dress_hip_last_error). */
+int dress_hip_get_batch(p_dress_graph_t g, int n, const int *u, const int *v,
+                        const double *edge_weight, int max_iterations, double epsilon,
+                        double *out);
+
 /* How dress_{delta,nabla}_fit_hip* run their subgraphs:
  *   0 (default) one batched persistent launch when the graph fits the
  *     kernel's LDS state (up to 65535 edges and ~150 KB of state), otherwise
@@ -166,6 +174,14 @@ int dress_hip_resident_download(p_dress_graph_t g);
  *   1 always one device fit per subgraph (the structure of the CUDA backend,
  *     libdress/src/cuda/delta_dress_cuda.c:24-51), kept for comparison. */
 void dress_hip_set_lifted_mode(int mode);
+
+/* Host-only helper: the subgraph indices a lifted call with these arguments
+ * processes (kind 0 delta / 1 nabla), in order, with their k vertices each
+ * (combinations and ordered tuples in lexicographic order, stride/offset and
+ * sampling as in libdress/src/delta_dress_impl.c:295-371).  Writes up to
+ * `cap` entries; returns the full count, or -1 on bad arguments. */
+int64_t dress_hip_enumerate_subgraphs(int kind, int N, int k, int n_samples, unsigned int seed,
+                                      int offset, int stride, int64_t *sel, int *verts, int64_t cap);
 
 /* Counters of the calling thread's last lifted call. */
 typedef struct {

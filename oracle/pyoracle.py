@@ -125,6 +125,10 @@ class Lib:
             self.delta_omp.restype = c_i64_p
         for f in (self.delta, self.nabla):
             f.restype = c_i64_p
+        if which == "ref":
+            for nm in ("dress_delta_binom", "dress_nabla_perm_count"):
+                getattr(L, nm).restype = ctypes.c_int64
+                getattr(L, nm).argtypes = [ctypes.c_int, ctypes.c_int]
 
     # -- construction -------------------------------------------------
     def graph(self, N, U, V, W=None, NW=None, variant=0, precompute=0):

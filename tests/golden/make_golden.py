@@ -1,0 +1,123 @@
+"""Writes the golden fixtures under tests/golden/ from the compiled reference.
+
+Run in the development container (the reference sources must be present so
+oracle/_ref/libdress_ref.so can be built):
+
+    make -C oracle ref && python tests/golden/make_golden.py
+
+Every fixture is data: input graphs (or the generator parameters that make
+them) and the outputs libdress v0.8.3 produces for them.
+"""
+import hashlib
+import json
+import os
+import sys
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+ROOT = os.path.dirname(os.path.dirname(HERE))
+sys.path.insert(0, os.path.join(ROOT, "oracle"))
+sys.path.insert(0, os.path.join(ROOT, "dress-graph_amd"))
+
+import pyoracle as po                       # noqa: E402
+from dress_hip import generators as gen     # noqa: E402
+from graphs import named_graphs             # noqa: E402
+
+
+def hist_digest(vals, cnts):
+    h = hashlib.sha256()
+    h.update(np.ascontiguousarray(vals, dtype=np.float64).tobytes())
+    h.update(np.ascontiguousarray(cnts, dtype=np.int64).tobytes())
+    return h.hexdigest()
+
+
+def hexs(a):
+    return [float(x).hex() for x in np.asarray(a, dtype=np.float64)]
+
+
+def main():
+    ref = po.Lib("ref")
+    out = {}
+
+    # configuration 1: ER G(2000, 8/1999), seed 0, both directions
+    N, U, V = gen.er_both_directions()
+    r = ref.fit(N, U, V, max_iterations=100, epsilon=1e-6)
+    np.savez_compressed(os.path.join(HERE, "config1.npz"), U=U, V=V,
+                        edge_dress=r["edge_dress"], vertex_dress=r["vertex_dress"],
+                        iterations=r["iterations"], delta=r["delta"])
+
+    # small multigraphs in every variant, weighted, with vertex weights
+    rng = np.random.default_rng(20260101)
+    cases = {}
+    for var in range(4):
+        n, m = 40, 160
+        Uv = rng.integers(0, n, m).astype(np.int32)
+        Vv = rng.integers(0, n, m).astype(np.int32)
+        W = rng.random(m) + 0.1
+        NW = rng.random(n) + 0.5
+        for pre in (0, 1):
+            rr = ref.fit(n, Uv, Vv, W, NW, variant=var, precompute=pre, max_iterations=100, epsilon=1e-8)
+            g = ref.graph(n, Uv, Vv, W, NW, var, 1)
+            csr = ref.csr(g)
+            ref.free_graph(g)
+            key = f"v{var}_p{pre}"
+            cases[key + "_U"] = Uv
+            cases[key + "_V"] = Vv
+            cases[key + "_W"] = W
+            cases[key + "_NW"] = NW
+            cases[key + "_edge_dress"] = rr["edge_dress"]
+            cases[key + "_vertex_dress"] = rr["vertex_dress"]
+            cases[key + "_it"] = np.array([rr["iterations"]])
+            cases[key + "_delta"] = np.array([rr["delta"]])
+            for f in ("adj_offset", "adj_target", "adj_edge_idx", "edge_weight", "intercept_offset",
+                      "intercept_edge_ux", "intercept_edge_vx"):
+                cases[key + "_" + f] = csr[f]
+    np.savez_compressed(os.path.join(HERE, "variants.npz"), **cases)
+
+    # named graphs: delta^0 / delta^1 / nabla^1 exact histograms
+    named = {}
+    for name, (n, Un, Vn) in named_graphs().items():
+        rec = {"n": n, "U": [int(x) for x in Un], "V": [int(x) for x in Vn]}
+        for kind, k in (("delta", 0), ("delta", 1), ("nabla", 1)):
+            vals, cnts, _, num = ref.lifted(kind, n, Un, Vn, k=k)
+            rec[f"{kind}{k}"] = {"values": hexs(vals), "counts": [int(c) for c in cnts], "items": int(num)}
+        named[name] = rec
+    with open(os.path.join(HERE, "named_graphs.json"), "w") as f:
+        json.dump(named, f, indent=1)
+
+    # delta^1 batch: first 16 graphs of the 512 (n ~ U{16..128}, 6-regular)
+    batch = []
+    for i, (n, Ub, Vb) in enumerate(gen.delta1_batch_graphs(16)):
+        vals, cnts, _, num = ref.lifted("delta", n, Ub, Vb, k=1)
+        batch.append({"seed": i, "n": n, "m": int(Ub.size), "items": int(num),
+                      "distinct": int(vals.size), "total": int(cnts.sum()),
+                      "sha256": hist_digest(vals, cnts),
+                      "edges_sha256": hashlib.sha256(np.stack([Ub, Vb]).tobytes()).hexdigest()})
+    with open(os.path.join(HERE, "delta1_batch16.json"), "w") as f:
+        json.dump(batch, f, indent=1)
+
+    # configuration 4 graph: 12-regular, n = 256, switch chain seed 7
+    n4, U4, V4 = gen.regular_switch()
+    vals, cnts, _, num = ref.lifted("nabla", n4, U4, V4, k=1)
+    np.savez_compressed(os.path.join(HERE, "config4.npz"), U=U4, V=V4)
+    with open(os.path.join(HERE, "config4_nabla1.json"), "w") as f:
+        json.dump({"items": int(num), "distinct": int(vals.size), "total": int(cnts.sum()),
+                   "sha256": hist_digest(vals, cnts)}, f, indent=1)
+
+    # dress_get: existing and virtual edges of a fitted small graph
+    n5, U5, V5 = gen.er_single(80, 0.08, 11)
+    g = ref.graph(n5, U5, V5, None, None, 0, 0)
+    ref.fit_graph(g, 100, 1e-6)
+    q = np.random.default_rng(3)
+    qu = q.integers(0, n5, 200).astype(np.int32)
+    qv = q.integers(0, n5, 200).astype(np.int32)
+    qw = q.random(200) + 0.5
+    ans = np.array([ref.get(g, int(a), int(b), 100, 1e-6, float(w)) for a, b, w in zip(qu, qv, qw)])
+    ref.free_graph(g)
+    np.savez_compressed(os.path.join(HERE, "get_queries.npz"), U=U5, V=V5, qu=qu, qv=qv, qw=qw, ans=ans)
+    print("golden fixtures written to", HERE)
+
+
+if __name__ == "__main__":
+    main()

@@ -1,0 +1,157 @@
+"""delta^k / nabla^k on the GPU against the oracle and the golden fixtures.
+
+FP64: exact (value, count) histograms and multisets bit for bit.
+FP32->FP64: values within 1e-9, counts equal, and the quantised histogram
+floor(d/eps) bit-exact once values within the tolerance of a bin edge are
+set apart on both sides.
+"""
+import hashlib
+import json
+import math
+import os
+
+import numpy as np
+import pytest
+
+from conftest import GOLDEN
+
+pytestmark = pytest.mark.gpu
+
+
+def _lifted(gpu, kind, n, U, V, W=None, variant=0, **kw):
+    with gpu.Graph(n, U, V, W, None, variant) as g:
+        return g.lifted(kind, **kw)
+
+
+def _digest(vals, cnts):
+    h = hashlib.sha256()
+    h.update(np.ascontiguousarray(vals, dtype=np.float64).tobytes())
+    h.update(np.ascontiguousarray(cnts, dtype=np.int64).tobytes())
+    return h.hexdigest()
+
+
+def test_named_graphs_golden(gpu):
+    with open(os.path.join(GOLDEN, "named_graphs.json")) as f:
+        named = json.load(f)
+    for name, rec in named.items():
+        U = np.array(rec["U"], np.int32)
+        V = np.array(rec["V"], np.int32)
+        for key in ("delta0", "delta1", "nabla1"):
+            kind, k = key[:-1], int(key[-1])
+            vals, cnts, _, num = _lifted(gpu, kind, rec["n"], U, V, k=k)
+            assert num == rec[key]["items"]
+            assert [float(x).hex() for x in vals] == rec[key]["values"], (name, key)
+            assert cnts.tolist() == rec[key]["counts"], (name, key)
+
+
+@pytest.mark.parametrize("kind", ["delta", "nabla"])
+@pytest.mark.parametrize("variant", [0, 1, 2, 3])
+def test_random_multigraphs_bitwise(gpu, ora, kind, variant):
+    rng = np.random.default_rng(300 + variant)
+    n, m = 10, 26
+    U = rng.integers(0, n, m).astype(np.int32)
+    V = rng.integers(0, n, m).astype(np.int32)
+    W = rng.random(m) + 0.1
+    for k in (0, 1, 2, 3, 10):
+        for ns, seed in ((0, 0), (6, 9)):
+            for off, st in ((0, 1), (1, 2)):
+                a = _lifted(gpu, kind, n, U, V, W, variant, k=k, n_samples=ns, seed=seed, keep_multisets=True,
+                            offset=off, stride=st)
+                b = ora.lifted(kind, n, U, V, W, variant=variant, k=k, n_samples=ns, seed=seed,
+                               keep_multisets=True, offset=off, stride=st)
+                assert a[3] == b[3]
+                assert np.array_equal(a[0], b[0]) and np.array_equal(a[1], b[1]), (k, ns, off)
+                assert (a[2] is None) == (b[2] is None)
+                if a[2] is not None:
+                    assert np.array_equal(a[2], b[2], equal_nan=True)
+
+
+def test_batched_equals_per_subgraph_path(gpu, gen):
+    n, U, V = gen.random_regular(40, 6, 3)
+    a = _lifted(gpu, "delta", n, U, V, k=2, n_samples=60, seed=4)
+    gpu.set_lifted_mode(1)
+    try:
+        b = _lifted(gpu, "delta", n, U, V, k=2, n_samples=60, seed=4)
+    finally:
+        gpu.set_lifted_mode(0)
+    assert np.array_equal(a[0], b[0]) and np.array_equal(a[1], b[1])
+
+
+def test_delta1_batch_golden(gpu, gen):
+    with open(os.path.join(GOLDEN, "delta1_batch16.json")) as f:
+        gold = json.load(f)
+    graphs = gen.delta1_batch_graphs(16)
+    r = gpu.lifted_batch(graphs, k=1, precision=gpu.FP64)
+    assert r.subgraphs == sum(g["items"] for g in gold)
+    for (vals, cnts), rec, (n, U, V) in zip(r.histograms, gold, graphs):
+        assert n == rec["n"] and U.size == rec["m"]
+        assert hashlib.sha256(np.stack([U, V]).tobytes()).hexdigest() == rec["edges_sha256"]
+        assert int(cnts.sum()) == rec["total"] and vals.size == rec["distinct"]
+        assert _digest(vals, cnts) == rec["sha256"]
+
+
+def _quantise(vals, cnts, eps, tol, flag_vals):
+    """oracle bins of values, with boundary flags decided on flag_vals"""
+    r = flag_vals / eps
+    q = np.floor(r)
+    bnd = ((r - q) * eps < tol) | ((q + 1 - r) * eps < tol)
+    qo = np.floor(vals / eps).astype(np.int64)
+    keys, inv = np.unique(qo[~bnd], return_inverse=True)
+    return keys, np.bincount(inv, weights=cnts[~bnd]).astype(np.int64) if keys.size else np.zeros(0, np.int64)
+
+
+@pytest.mark.parametrize("precision", [0, 1])
+def test_batch_quantised_histograms(gpu, ora, gen, precision):
+    graphs = gen.delta1_batch_graphs(12)
+    eps, tol = 1e-6, 1e-9
+    r = gpu.lifted_batch(graphs, k=1, precision=precision, epsilon=eps, qtol=tol)
+    for (hv, hc), (qk, qc), (n, U, V) in zip(r.histograms, r.quantised, graphs):
+        ov, oc, _, _ = ora.lifted("delta", n, U, V, k=1, epsilon=eps)
+        assert np.array_equal(hc, oc) and hv.size == ov.size
+        if precision == 0:
+            assert np.array_equal(hv, ov)
+        else:
+            assert np.max(np.abs(hv - ov)) <= 1e-9
+        # every value counted once: table counts + boundary counts = total
+        keys, cnt = _quantise(ov, oc, eps, tol, hv)
+        assert np.array_equal(qk, keys) and np.array_equal(qc, cnt)
+    tot = sum(int(c.sum()) for _, c in r.quantised) + sum(int(c.sum()) for _, c in r.boundary)
+    assert tot == r.values
+
+
+def test_config4_nabla1_golden(gpu):
+    z = np.load(os.path.join(GOLDEN, "config4.npz"))
+    with open(os.path.join(GOLDEN, "config4_nabla1.json")) as f:
+        gold = json.load(f)
+    vals, cnts, _, num = _lifted(gpu, "nabla", 256, z["U"], z["V"], k=1)
+    assert num == gold["items"] == 256
+    assert int(cnts.sum()) == gold["total"] and vals.size == gold["distinct"]
+    assert _digest(vals, cnts) == gold["sha256"]
+
+
+def test_config4_delta2(gpu, ora):
+    z = np.load(os.path.join(GOLDEN, "config4.npz"))
+    U, V = z["U"], z["V"]
+    # all 32,640 subgraphs: histogram total = live edges of every subgraph
+    vals, cnts, _, num = _lifted(gpu, "delta", 256, U, V, k=2)
+    assert num == math.comb(256, 2)
+    assert int(cnts.sum()) == num * 1536 - 255 * 2 * 1536 + 1536
+    # a sampled slice bit for bit against the oracle
+    a = _lifted(gpu, "delta", 256, U, V, k=2, n_samples=150, seed=21, keep_multisets=True)
+    b = ora.lifted("delta", 256, U, V, k=2, n_samples=150, seed=21, keep_multisets=True)
+    assert np.array_equal(a[0], b[0]) and np.array_equal(a[1], b[1])
+    assert np.array_equal(a[2], b[2], equal_nan=True)
+
+
+def test_mixed_precision_lifted(gpu, ora, gen):
+    n, U, V = gen.random_regular(64, 6, 17)
+    gpu.set_precision(gpu.FP32_FP64)
+    try:
+        a = _lifted(gpu, "delta", n, U, V, k=1)
+        st = gpu.last_lifted_stats()
+    finally:
+        gpu.set_precision(gpu.FP64)
+    b = ora.lifted("delta", n, U, V, k=1)
+    assert np.array_equal(a[1], b[1])
+    assert np.max(np.abs(a[0] - b[0])) <= 1e-9
+    assert st["subgraphs"] == n
