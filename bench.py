@@ -1,0 +1,230 @@
+#!/usr/bin/env python3
+"""DRESS benchmark on one MI355X: prints one JSON line.
+
+Headline workload: plain FP64 dress_fit on the 4,000,000-vertex Chung-Lu
+power-law graph (exponent 2.1, expected degrees capped at 20,000, seed 1,
+28,000,000 undirected edges, epsilon 1e-6, at most 100 iterations).  One step
+is one complete fit (every iteration to convergence) of the device-resident
+graph from d = 1; value = edge-updates/s = E * iterations / time.
+
+The same line also carries
+  * the FP32->FP64 fit of the same graph,
+  * delta^1 of the 512 random 6-regular graphs (n ~ U{16..128}): subgraph
+    solves/s of the batched kernel (FP64 and FP32->FP64) beside the plain
+    FP64 GPU path (one device fit per subgraph, the CUDA backend's structure),
+  * the roofline of the dominant kernel (HIP events around its launches),
+  * the CPU baseline: libdress compiled unchanged (OpenMP) on this host.
+
+    python bench.py [--gpus 1] [--steps 5] [--warmup 1] [--no-cpu-baseline]
+"""
+import argparse
+import json
+import math
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, os.path.join(ROOT, "dress-graph_amd"))
+
+METRIC = "Δ¹ subgraph solves/s + edge-updates/s vs plain FP64 GPU; p50/p99 latency"
+HBM_PEAK_GBS = 8000.0
+# algorithmic bytes of the edge-phase kernel: per intersection entry the
+# packed index word (4 B) and the two gathered FP64 values (16 B); per edge
+# its offsets, row starts, ends, weight, self term, value, two norms, the new
+# value and its two half-edge copies (12 x 8 B)
+BYTES_PER_ENTRY = 20
+BYTES_PER_EDGE = 96
+
+
+def log(*a):
+    print(*a, file=sys.stderr, flush=True)
+
+
+def fit_steps(g, steps, precision, eps=1e-6, max_it=100):
+    import dress_hip as dh
+    iters, t_total, dev_ms, fp32 = 0, 0.0, 0.0, 0
+    last = None
+    for _ in range(steps):
+        g.resident_fill(1.0)
+        t0 = time.perf_counter()
+        last = g.fit_resident(max_it, eps, precision)
+        t_total += time.perf_counter() - t0
+        st = dh.last_fit_stats()
+        iters += st.iterations_executed
+        dev_ms += st.device_ms
+        fp32 += st.fp32_iterations
+    return {"seconds": t_total, "iterations_executed": iters, "device_ms": dev_ms, "fp32_iterations": fp32,
+            "result": last}
+
+
+def cpu_baseline_fit(N, U, V, csr, iterations=2):
+    """libdress (compiled unchanged, OpenMP) on this host: `iterations` of the
+    same fit with its intersection index prebuilt; returns edge-updates/s."""
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    import pyoracle
+    if not pyoracle.available("ref"):
+        return None, "oracle"
+    lib = pyoracle.Lib("ref")
+    g = pyoracle.ref_graph_from_csr(lib, N, U, V, csr)
+    try:
+        t0 = time.perf_counter()
+        lib.fit_graph(g, iterations, 1e-6, parallel=True)
+        dt = time.perf_counter() - t0
+    finally:
+        lib.free_graph(g)
+    return U.size * iterations / dt, dt
+
+
+def cpu_baseline_delta1(graphs):
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    import pyoracle
+    lib = pyoracle.Lib("ref") if pyoracle.available("ref") else pyoracle.Lib("oracle")
+    t0 = time.perf_counter()
+    solves = 0
+    for n, U, V in graphs:
+        _, _, _, num = lib.lifted("delta", n, U, V, k=1, use_omp=True)
+        solves += num
+    return solves / (time.perf_counter() - t0), lib.which
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=5)
+    ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--no-delta", action="store_true")
+    args = ap.parse_args()
+    if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
+        print("bench.py runs on one GPU: the DRESS engine keeps the whole edge set on a single "
+              "MI355X and has no multi-GPU path; use --gpus 1", file=sys.stderr)
+        sys.exit(2)
+
+    import dress_hip as dh
+    from dress_hip import generators as gen
+    if not dh.is_available():
+        raise SystemExit("no GPU visible to libdress_hip.so: " + dh._lib.last_error())
+
+    t = time.perf_counter()
+    N, U, V = gen.chung_lu()
+    log(f"graph generated in {time.perf_counter() - t:.1f} s: N={N} E={U.size}")
+    t = time.perf_counter()
+    g = dh.Graph(N, U, V)
+    g.resident_reset()
+    info = g.info()
+    log(f"device graph + intersection index built in {time.perf_counter() - t:.2f} s: {info}")
+
+    fit_steps(g, args.warmup, dh.FP64)
+    r64 = fit_steps(g, args.steps, dh.FP64)
+    log(f"FP64: {r64['iterations_executed'] / args.steps:.0f} iterations, "
+        f"{r64['seconds'] / args.steps * 1e3:.1f} ms/fit")
+    fit_steps(g, 1, dh.FP32_FP64)
+    r32 = fit_steps(g, args.steps, dh.FP32_FP64)
+    log(f"FP32->FP64: {r32['seconds'] / args.steps * 1e3:.1f} ms/fit")
+
+    E = U.size
+    value = E * r64["iterations_executed"] / r64["seconds"]
+    value32 = E * r32["iterations_executed"] / r32["seconds"]
+    ms_per_step = r64["seconds"] / args.steps * 1e3
+
+    # dominant kernel: the edge-phase bin with the most time
+    vms, ems = g.time_phases(reps=5)
+    b = int(np.argmax(ems))
+    launch_bytes = info["bin_terms"][b] * BYTES_PER_ENTRY + info["edge_bins"][b] * BYTES_PER_EDGE
+    achieved = launch_bytes / (ems[b] * 1e-3) / 1e9
+    group = [1, 4, 16, 64, 256][b]
+    traffic = None
+    tfile = os.path.join(ROOT, "profiles", "fit_edge_bin3_traffic.json")
+    if os.path.isfile(tfile):
+        with open(tfile) as f:
+            traffic = json.load(f).get("hbm_bytes_per_launch")
+    iters_per_fit = r64["iterations_executed"] / args.steps
+    # whole-iteration algorithmic bytes (FP64), every kernel of an iteration
+    S, T, Nn = info["S"], info["T"], info["N"]
+    b_iter = BYTES_PER_ENTRY * T + BYTES_PER_EDGE * E + 12 * S + 28 * Nn
+    iter_gbs = b_iter / (ms_per_step * 1e-3 / iters_per_fit) / 1e9
+
+    out = {
+        "metric": METRIC,
+        "value": value,
+        "unit": "edge-updates/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": ms_per_step,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64",
+        "data": "synthetic",
+        "config": {"workload": "plain FP64 dress_fit, Chung-Lu power law N=4,000,000 E=28,000,000 "
+                               "gamma=2.1 cap 20,000 seed 1, eps 1e-6, max 100 iterations",
+                   "N": N, "E": int(E), "S": S, "T": T, "iterations": iters_per_fit,
+                   "parallelism": "single GPU"},
+        "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": achieved / HBM_PEAK_GBS, "traffic": traffic,
+                     "kernel": f"k_edge<{group},packed,double> (edge bin {b})",
+                     "launch_ms": float(ems[b]), "algorithmic_bytes": int(launch_bytes),
+                     "iteration_algorithmic_GBs": iter_gbs},
+        "fp32_fp64": {"value": value32, "unit": "edge-updates/s",
+                      "ms_per_step": r32["seconds"] / args.steps * 1e3,
+                      "fp32_iterations_per_fit": r32["fp32_iterations"] / args.steps,
+                      "speedup_vs_fp64": value32 / value},
+        "phase_ms": {"vertex": [float(x) for x in vms], "edge": [float(x) for x in ems]},
+    }
+
+    if not args.no_delta:
+        graphs = gen.delta1_batch_graphs(512)
+        res = {}
+        for name, prec in (("fp64", dh.FP64), ("fp32_fp64", dh.FP32_FP64)):
+            dh.lifted_batch(graphs[:8], k=1, precision=prec)            # warm-up
+            best = None
+            for _ in range(3):
+                t0 = time.perf_counter()
+                r = dh.lifted_batch(graphs, k=1, precision=prec)
+                wall = time.perf_counter() - t0
+                if best is None or wall < best[0]:
+                    best = (wall, r)
+            wall, r = best
+            res[name] = {"solves_per_s": r.subgraphs / wall, "device_solves_per_s": r.subgraphs / (r.device_ms * 1e-3),
+                         "subgraphs": int(r.subgraphs), "values": int(r.values), "fp64_reruns": int(r.fp64_reruns),
+                         "wall_ms": wall * 1e3, "device_ms": r.device_ms}
+        # plain FP64 GPU path: one device fit per subgraph (sample of graphs)
+        dh.set_lifted_mode(1)
+        t0 = time.perf_counter()
+        ns = 0
+        for n, Ug, Vg in graphs[:6]:
+            with dh.Graph(n, Ug, Vg) as pg:
+                pg.lifted("delta", 1)
+            ns += n
+        plain = ns / (time.perf_counter() - t0)
+        dh.set_lifted_mode(0)
+        res["plain_fp64_gpu"] = {"solves_per_s": plain, "sample": f"{ns} subgraphs of the first 6 graphs"}
+        res["speedup_fp32_fp64_vs_plain_fp64_gpu"] = res["fp32_fp64"]["solves_per_s"] / plain
+        res["speedup_fp64_batched_vs_plain_fp64_gpu"] = res["fp64"]["solves_per_s"] / plain
+        res["workload"] = "delta^1 of 512 random 6-regular graphs, n ~ U{16..128}, seeds 0..511, eps 1e-6"
+        out["delta1"] = res
+
+    if not args.no_cpu_baseline:
+        cores = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
+        with dh.Graph(N, U, V, precompute_intercepts=True) as gi:
+            csr = gi.csr()
+        cpu_v, dt = cpu_baseline_fit(N, U, V, csr, iterations=2)
+        del csr
+        if cpu_v is not None:
+            out["cpu_baseline"] = {"value": cpu_v, "unit": "edge-updates/s", "cores": cores, "kind": "reference",
+                                   "sample": f"2 of the {iters_per_fit:.0f} iterations of the same fit, libdress "
+                                             f"dress_fit_omp with its intercept index prebuilt ({dt:.1f} s)"}
+        if "delta1" in out:
+            v, kind = cpu_baseline_delta1(graphs[:64])
+            out["delta1"]["cpu_baseline"] = {"solves_per_s": v, "cores": cores,
+                                            "kind": "reference" if kind == "ref" else "port",
+                                            "sample": "delta^1 of the first 64 graphs, dress_delta_fit_omp"}
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()

@@ -355,6 +355,24 @@ int dress_hip_fit_resident(p_dress_graph_t g, int max_iterations, double epsilon
     return 0;
 }
 
+int dress_hip_resident_fill(p_dress_graph_t g, double value)
+{
+    clear_error();
+    DevGraph *G = g ? device_graph(g) : nullptr;
+    if (!G) return -1;
+    if (!refresh_self_terms(G, g->NW)) return -1;
+    return fit_fill_values(G, value) ? 0 : -1;
+}
+
+int dress_hip_time_phases(p_dress_graph_t g, int reps, double *ms)
+{
+    clear_error();
+    DevGraph *G = g ? lookup(g) : nullptr;
+    if (!G || !G->d[0]) { set_error("dress_hip_time_phases: call dress_hip_resident_reset first"); return -1; }
+    if (!ms || reps < 1) { set_error("dress_hip_time_phases: bad arguments"); return -1; }
+    return fit_time_phases(G, reps, ms) ? 0 : -1;
+}
+
 int dress_hip_resident_download(p_dress_graph_t g)
 {
     clear_error();

@@ -39,6 +39,7 @@ struct FitState {
     int iterations;          // reference convention result
     int cur;                 // buffer index holding the current values
     double delta;            // max change of the last finished iteration
+    double prev_delta;       // ... and of the one before
     unsigned long long maxd; // running max |change| (bit pattern, >= 0)
     int fp32;                // 1 while values live in the FP32 buffers,
                              // 2 during the iteration that converts them
@@ -314,6 +315,7 @@ __global__ void k_end(FitState *st, double eps, int max_it)
     if (threadIdx.x != 0 || blockIdx.x != 0 || st->done) return;
     double md = __longlong_as_double((long long)st->maxd);
     st->maxd = 0ull;
+    st->prev_delta = st->delta;
     st->delta = md;
     st->cur ^= 1;
     int it = st->iter;
@@ -486,7 +488,15 @@ bool fit_run_once(DevGraph *G, int max_iterations, double epsilon, bool mixed, F
         DH_CHECK(hipMemcpyAsync(&h, st, sizeof(h), hipMemcpyDeviceToHost, s));
         DH_CHECK(hipStreamSynchronize(s));
         if (h.done) break;
-        chunk = std::min(chunk * 2, 16);
+        // next chunk: the iterations a geometric decay of the change needs to
+        // reach epsilon, so launches past convergence stay rare
+        chunk = 16;
+        if (h.prev_delta > 0.0 && h.delta > 0.0 && h.delta < h.prev_delta && epsilon > 0.0) {
+            double r = h.delta / h.prev_delta;
+            double need = std::ceil(std::log(epsilon / h.delta) / std::log(r));
+            if (need >= 1.0 && need < 16.0) chunk = (int)need;
+            else if (need < 1.0) chunk = 1;
+        }
     }
     DH_CHECK(hipEventRecord(ev1, s));
     DH_CHECK(hipEventSynchronize(ev1));
@@ -548,6 +558,101 @@ bool fit_run(DevGraph *G, int max_iterations, double epsilon, int precision, Fit
     }
     dfree(start);
     return ok;
+}
+
+namespace {
+__global__ void k_fill_vals(double *d, int n, double v)
+{
+    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) d[i] = v;
+}
+}  // namespace
+
+bool fit_fill_values(DevGraph *G, double value)
+{
+    if (!fit_alloc_state(G, false)) return false;
+    G->cur = 0;
+    if (G->E > 0)
+        hipLaunchKernelGGL(k_fill_vals, dim3(grid_for(G->E, num_cus() * 8)), dim3(TPB), 0, G->stream, G->d[0], G->E,
+                           value);
+    DH_CHECK(hipGetLastError());
+    DH_CHECK(hipStreamSynchronize(G->stream));
+    return true;
+}
+
+// Average device time of each phase kernel of one FP64 iteration over reps
+// iterations, bracketed by events on the graph's stream.  ms[0..4] vertex
+// bins, ms[5..9] edge bins (0 where the bin is empty).
+bool fit_time_phases(DevGraph *G, int reps, double *ms)
+{
+    if (!fit_alloc_state(G, false)) return false;
+    hipStream_t s = G->stream;
+    FitState h;
+    memset(&h, 0, sizeof(h));
+    h.cur = G->cur;
+    DH_CHECK(hipMemcpyAsync(G->state, &h, sizeof(h), hipMemcpyHostToDevice, s));
+    if (G->E > 0)
+        hipLaunchKernelGGL((k_init_wh<double>), dim3(grid_for(G->E, num_cus() * 8)), dim3(TPB), 0, s, G->E, G->ew,
+                           G->pos, G->d[0], G->d[1], (double *)nullptr, (double *)nullptr, G->wh[0], G->wh[1],
+                           (const FitState *)G->state);
+    std::vector<hipEvent_t> ev(22);
+    for (auto &e : ev) DH_CHECK(hipEventCreate(&e));
+    for (int i = 0; i < 10; i++) ms[i] = 0.0;
+    int cap = num_cus() * 32;
+    for (int r = 0; r < reps; r++) {
+        for (int b = 0; b < Bins::NB; b++) {
+            DH_CHECK(hipEventRecord(ev[2 * b], s));
+            if (G->packed) {
+                switch (b) {
+                case 0: launch_vertex_t<1, double>(G, 0, cap, G->wh[0], G->wh[1]); break;
+                case 1: launch_vertex_t<4, double>(G, 1, cap, G->wh[0], G->wh[1]); break;
+                case 2: launch_vertex_t<16, double>(G, 2, cap, G->wh[0], G->wh[1]); break;
+                case 3: launch_vertex_t<64, double>(G, 3, cap, G->wh[0], G->wh[1]); break;
+                default: launch_vertex_t<TPB, double>(G, 4, cap, G->wh[0], G->wh[1]); break;
+                }
+            } else {
+                switch (b) {
+                case 0: launch_vertex_t<1, double>(G, 0, cap, G->wh[0], G->wh[1]); break;
+                case 1: launch_vertex_t<4, double>(G, 1, cap, G->wh[0], G->wh[1]); break;
+                case 2: launch_vertex_t<16, double>(G, 2, cap, G->wh[0], G->wh[1]); break;
+                case 3: launch_vertex_t<64, double>(G, 3, cap, G->wh[0], G->wh[1]); break;
+                default: launch_vertex_t<TPB, double>(G, 4, cap, G->wh[0], G->wh[1]); break;
+                }
+            }
+            DH_CHECK(hipEventRecord(ev[2 * b + 1], s));
+        }
+        for (int b = 0; b < Bins::NB; b++) {
+            DH_CHECK(hipEventRecord(ev[10 + 2 * b], s));
+            if (G->packed) {
+                switch (b) {
+                case 0: launch_edge_t<1, true, double>(G, 0, cap, G->d[0], G->d[1], G->wh[0], G->wh[1]); break;
+                case 1: launch_edge_t<4, true, double>(G, 1, cap, G->d[0], G->d[1], G->wh[0], G->wh[1]); break;
+                case 2: launch_edge_t<16, true, double>(G, 2, cap, G->d[0], G->d[1], G->wh[0], G->wh[1]); break;
+                case 3: launch_edge_t<64, true, double>(G, 3, cap, G->d[0], G->d[1], G->wh[0], G->wh[1]); break;
+                default: launch_edge_t<TPB, true, double>(G, 4, cap, G->d[0], G->d[1], G->wh[0], G->wh[1]); break;
+                }
+            } else {
+                switch (b) {
+                case 0: launch_edge_t<1, false, double>(G, 0, cap, G->d[0], G->d[1], G->wh[0], G->wh[1]); break;
+                case 1: launch_edge_t<4, false, double>(G, 1, cap, G->d[0], G->d[1], G->wh[0], G->wh[1]); break;
+                case 2: launch_edge_t<16, false, double>(G, 2, cap, G->d[0], G->d[1], G->wh[0], G->wh[1]); break;
+                case 3: launch_edge_t<64, false, double>(G, 3, cap, G->d[0], G->d[1], G->wh[0], G->wh[1]); break;
+                default: launch_edge_t<TPB, false, double>(G, 4, cap, G->d[0], G->d[1], G->wh[0], G->wh[1]); break;
+                }
+            }
+            DH_CHECK(hipEventRecord(ev[11 + 2 * b], s));
+        }
+        DH_CHECK(hipEventSynchronize(ev[19]));
+        for (int i = 0; i < 10; i++) {
+            float t = 0.f;
+            DH_CHECK(hipEventElapsedTime(&t, ev[2 * i], ev[2 * i + 1]));
+            ms[i] += t;
+        }
+    }
+    for (auto &e : ev) (void)hipEventDestroy(e);
+    for (int i = 0; i < 10; i++) ms[i] /= std::max(reps, 1);
+    // the timing iterations only wrote the other buffer: the current values
+    // (buffer G->cur) are unchanged
+    return true;
 }
 
 bool fit_download(DevGraph *G, double *edge_dress, double *vertex_dress)

@@ -140,6 +140,8 @@ bool fit_upload_values(DevGraph *G, const double *edge_dress);
 bool fit_run(DevGraph *G, int max_iterations, double epsilon, int precision,
              FitResult *res);
 bool fit_download(DevGraph *G, double *edge_dress, double *vertex_dress);
+bool fit_fill_values(DevGraph *G, double value);
+bool fit_time_phases(DevGraph *G, int reps, double *ms);
 
 // dress_prim.hip (rocPRIM wrappers, stream-ordered, scratch allocated inside)
 bool prim_sort_pairs_u64_i32(const uint64_t *kin, uint64_t *kout, const int *vin, int *vout,

@@ -109,6 +109,10 @@ def _bind(L):
     L.dress_hip_fit_resident.restype = ctypes.c_int
     L.dress_hip_fit_resident.argtypes = [PG, ctypes.c_int, ctypes.c_double, ctypes.c_int,
                                          c_int_p, c_dbl_p]
+    L.dress_hip_resident_fill.restype = ctypes.c_int
+    L.dress_hip_resident_fill.argtypes = [PG, ctypes.c_double]
+    L.dress_hip_time_phases.restype = ctypes.c_int
+    L.dress_hip_time_phases.argtypes = [PG, ctypes.c_int, c_dbl_p]
     L.dress_hip_resident_download.restype = ctypes.c_int
     L.dress_hip_resident_download.argtypes = [PG]
     L.dress_hip_set_lifted_mode.argtypes = [ctypes.c_int]

@@ -159,6 +159,16 @@ int dress_hip_fit_resident(p_dress_graph_t g, int max_iterations, double epsilon
                            int *iterations, double *delta);
 int dress_hip_resident_download(p_dress_graph_t g);
 
+/* Sets every resident edge value to `value` on the device (a fresh start is
+ * value = 1.0, the state dress_init_graph creates).  Returns 0 or -1. */
+int dress_hip_resident_fill(p_dress_graph_t g, double value);
+
+/* Device time of each kernel of one FP64 iteration, averaged over `reps`
+ * iterations run on the resident state: ms[0..4] the vertex-phase kernels,
+ * ms[5..9] the edge-phase kernels of the five work bins (0 for an empty
+ * bin).  Leaves the resident values as they were.  Returns 0 or -1. */
+int dress_hip_time_phases(p_dress_graph_t g, int reps, double *ms);
+
 /* dress_get for many pairs at once (libdress/src/dress.c:651-739 per
  * query), against g's current edge_dress / vertex_dress.  out[i] receives
  * the value of (u[i], v[i]) with hypothetical weight edge_weight[i].

@@ -227,3 +227,37 @@ class Lib:
         n = self.lib.ora_sample_picks(int(count), int(n_samples), ctypes.c_uint(seed), int(offset),
                                       out.ctypes.data_as(c_i64_p))
         return out[:n]
+
+
+def ref_graph_from_csr(lib, N, U, V, csr, variant=0):
+    """A dress_graph_t for the compiled reference filled from construction
+    arrays built elsewhere (CSR, variant weights, intercept index), exactly the
+    fields libdress's dress_init_graph(..., precompute=1) sets
+    (dress.c:378-427); every array is malloc'd so the reference's
+    dress_free_graph releases it."""
+    E = int(np.asarray(U).size)
+    p = _libc.malloc(ctypes.sizeof(DressGraph))
+    if not p:
+        raise MemoryError
+    g = ctypes.cast(p, PG)
+    c = g.contents
+    c.variant = int(variant)
+    c.N = int(N)
+    c.E = E
+    c.U = malloc_copy(U, ctypes.c_int)
+    c.V = malloc_copy(V, ctypes.c_int)
+    c.adj_offset = malloc_copy(csr["adj_offset"], ctypes.c_int)
+    c.adj_target = malloc_copy(csr["adj_target"], ctypes.c_int)
+    c.adj_edge_idx = malloc_copy(csr["adj_edge_idx"], ctypes.c_int)
+    c.max_degree = int(csr["max_degree"])
+    c.W = c_dbl_p()
+    c.edge_weight = malloc_copy(csr["edge_weight"], ctypes.c_double)
+    c.edge_dress = malloc_copy(np.ones(E), ctypes.c_double)
+    c.edge_dress_next = malloc_copy(np.ones(E), ctypes.c_double)
+    c.NW = malloc_copy(np.ones(N), ctypes.c_double)
+    c.vertex_dress = malloc_copy(np.zeros(N), ctypes.c_double)
+    c.precompute_intercepts = 1
+    c.intercept_offset = malloc_copy(csr["intercept_offset"], ctypes.c_int)
+    c.intercept_edge_ux = malloc_copy(csr["intercept_edge_ux"], ctypes.c_int)
+    c.intercept_edge_vx = malloc_copy(csr["intercept_edge_vx"], ctypes.c_int)
+    return g
