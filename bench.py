@@ -19,7 +19,6 @@ The same line also carries
 """
 import argparse
 import json
-import math
 import os
 import sys
 import time
@@ -31,12 +30,19 @@ sys.path.insert(0, os.path.join(ROOT, "dress-graph_amd"))
 
 METRIC = "Δ¹ subgraph solves/s + edge-updates/s vs plain FP64 GPU; p50/p99 latency"
 HBM_PEAK_GBS = 8000.0
-# algorithmic bytes of the edge-phase kernel: per intersection entry the
-# packed index word (4 B) and the two gathered FP64 values (16 B); per edge
-# its offsets, row starts, ends, weight, self term, value, two norms, the new
-# value and its two half-edge copies (12 x 8 B)
-BYTES_PER_ENTRY = 20
-BYTES_PER_EDGE = 96
+# algorithmic bytes of one FP64 iteration, per unit of work: an intersection
+# entry is two int32 half-edge ids and two 8 B gathers (24 B); a half-edge an
+# int32 id and one 8 B value (12 B); an edge its offset, endpoints, own value,
+# new value and two norm gathers (44 B); a vertex its offset, weight and norm
+# (20 B).  The edge-phase kernel of one work bin moves 24 B per entry and
+# 44 B per edge of the bin.
+B_ENTRY, B_HALF, B_EDGE, B_VERTEX = 24, 12, 44, 20
+# HBM bytes per launch of the dominant kernel from hardware counters
+# (FETCH_SIZE and WRITE_SIZE, one rocprofv3 --pmc pass each, over
+# `python bench.py --traffic-run`), each scaled by the bytes a plain copy
+# kernel of known size moves per counted KiB.  Valid for the kernel and
+# graph it was measured on; None when not measured.
+TRAFFIC = None
 
 
 def log(*a):
@@ -44,20 +50,20 @@ def log(*a):
 
 
 def fit_steps(g, steps, precision, eps=1e-6, max_it=100):
+    """`steps` complete fits from d = 1 of the resident graph, timed as one
+    block (each fit ends with a device synchronisation)."""
     import dress_hip as dh
-    iters, t_total, dev_ms, fp32 = 0, 0.0, 0.0, 0
-    last = None
+    iters, dev_ms, fp32 = 0, 0.0, 0
+    t0 = time.perf_counter()
     for _ in range(steps):
         g.resident_fill(1.0)
-        t0 = time.perf_counter()
-        last = g.fit_resident(max_it, eps, precision)
-        t_total += time.perf_counter() - t0
+        g.fit_resident(max_it, eps, precision)
         st = dh.last_fit_stats()
         iters += st.iterations_executed
         dev_ms += st.device_ms
         fp32 += st.fp32_iterations
-    return {"seconds": t_total, "iterations_executed": iters, "device_ms": dev_ms, "fp32_iterations": fp32,
-            "result": last}
+    return {"seconds": time.perf_counter() - t0, "iterations_executed": iters, "device_ms": dev_ms,
+            "fp32_iterations": fp32}
 
 
 def cpu_baseline_fit(N, U, V, csr, iterations=2):
@@ -97,6 +103,8 @@ def main():
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-delta", action="store_true")
+    ap.add_argument("--traffic-run", action="store_true",
+                    help="one FP64 fit and 1 GiB copy-kernel passes only, for hardware-counter runs")
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
         print("bench.py runs on one GPU: the DRESS engine keeps the whole edge set on a single "
@@ -116,6 +124,11 @@ def main():
     g.resident_reset()
     info = g.info()
     log(f"device graph + intersection index built in {time.perf_counter() - t:.2f} s: {info}")
+    if args.traffic_run:
+        r = fit_steps(g, 1, dh.FP64)
+        ms = dh.copy_probe(1 << 30, 3)
+        log(f"traffic run: {r['iterations_executed']} iterations; 1 GiB copy {ms:.3f} ms")
+        return
 
     fit_steps(g, args.warmup, dh.FP64)
     r64 = fit_steps(g, args.steps, dh.FP64)
@@ -133,19 +146,15 @@ def main():
     # dominant kernel: the edge-phase bin with the most time
     vms, ems = g.time_phases(reps=5)
     b = int(np.argmax(ems))
-    launch_bytes = info["bin_terms"][b] * BYTES_PER_ENTRY + info["edge_bins"][b] * BYTES_PER_EDGE
+    launch_bytes = info["bin_terms"][b] * B_ENTRY + info["edge_bins"][b] * B_EDGE
     achieved = launch_bytes / (ems[b] * 1e-3) / 1e9
     group = [1, 4, 16, 64, 256][b]
-    traffic = None
-    tfile = os.path.join(ROOT, "profiles", "fit_edge_bin3_traffic.json")
-    if os.path.isfile(tfile):
-        with open(tfile) as f:
-            traffic = json.load(f).get("hbm_bytes_per_launch")
     iters_per_fit = r64["iterations_executed"] / args.steps
-    # whole-iteration algorithmic bytes (FP64), every kernel of an iteration
     S, T, Nn = info["S"], info["T"], info["N"]
-    b_iter = BYTES_PER_ENTRY * T + BYTES_PER_EDGE * E + 12 * S + 28 * Nn
+    b_iter = B_ENTRY * T + B_HALF * S + B_EDGE * E + B_VERTEX * Nn
     iter_gbs = b_iter / (ms_per_step * 1e-3 / iters_per_fit) / 1e9
+    copy_ms = dh.copy_probe(1 << 30, 5)
+    copy_gbs = 2 * (1 << 30) / (copy_ms * 1e-3) / 1e9
 
     out = {
         "metric": METRIC,
@@ -165,10 +174,12 @@ def main():
                    "N": N, "E": int(E), "S": S, "T": T, "iterations": iters_per_fit,
                    "parallelism": "single GPU"},
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": achieved / HBM_PEAK_GBS, "traffic": traffic,
+                     "frac": achieved / HBM_PEAK_GBS,
+                     "traffic": TRAFFIC["bytes_per_launch"] if TRAFFIC else None,
                      "kernel": f"k_edge<{group},packed,double> (edge bin {b})",
                      "launch_ms": float(ems[b]), "algorithmic_bytes": int(launch_bytes),
-                     "iteration_algorithmic_GBs": iter_gbs},
+                     "iteration_algorithmic_GBs": iter_gbs, "iteration_frac": iter_gbs / HBM_PEAK_GBS,
+                     "copy_kernel_GBs": copy_gbs},
         "fp32_fp64": {"value": value32, "unit": "edge-updates/s",
                       "ms_per_step": r32["seconds"] / args.steps * 1e3,
                       "fp32_iterations_per_fit": r32["fp32_iterations"] / args.steps,

@@ -242,11 +242,22 @@ __global__ void k_order_positions(int2 *pos_in, int E)
     }
 }
 
-// internal order key: (bin, first half-edge)
-__global__ void k_order_keys(const int64_t *cnt_in, const int2 *pos_in, int E, uint64_t *key, int *val)
+// internal order key: (bin, half-edge of the edge in its owner's row); the
+// owner is the endpoint of larger degree (lower id on ties), so the edges of
+// one hub are adjacent within a bin and share the hub's row of values
+__global__ void k_order_keys(const int64_t *cnt_in, const int2 *pos_in, const int *U, const int *V,
+                             const int *row_off, int E, uint64_t *key, int *val)
 {
     for (int e = blockIdx.x * blockDim.x + threadIdx.x; e < E; e += gridDim.x * blockDim.x) {
-        key[e] = ((uint64_t)term_bin(cnt_in[e] + 1) << 32) | (uint32_t)pos_in[e].x;
+        int2 p = pos_in[e];
+        int own = p.x;
+        if (p.y >= 0) {
+            int u = U[e], v = V[e];
+            int du = row_off[u + 1] - row_off[u], dv = row_off[v + 1] - row_off[v];
+            int o = (du > dv || (du == dv && u < v)) ? u : v;
+            own = (p.x >= row_off[o] && p.x < row_off[o + 1]) ? p.x : p.y;
+        }
+        key[e] = ((uint64_t)term_bin(cnt_in[e] + 1) << 32) | (uint32_t)own;
         val[e] = e;
     }
 }
@@ -506,7 +517,8 @@ bool build_index(DevGraph *G)
         bool ok2 = dalloc(&k0, (size_t)E + 1, "order keys") && dalloc(&k1, (size_t)E + 1, "order keys") &&
                    dalloc(&v0, (size_t)E + 1, "order vals") && dalloc(&cd, Bins::NB, "bin counts");
         if (ok2) {
-            hipLaunchKernelGGL(k_order_keys, dim3(grid_for(E, 8192)), dim3(TPB), 0, s, G->cnt_in, G->pos_in, E, k0, v0);
+            hipLaunchKernelGGL(k_order_keys, dim3(grid_for(E, 8192)), dim3(TPB), 0, s, G->cnt_in, G->pos_in, G->U, G->V,
+                               G->row_off, E, k0, v0);
             ok2 = prim_sort_pairs_u64_i32(k0, k1, v0, G->ord, (size_t)E, 64, s);
         }
         ok2 = ok2 && hipMemsetAsync(cd, 0, Bins::NB * sizeof(int), s) == hipSuccess;

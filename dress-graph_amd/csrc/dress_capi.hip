@@ -69,6 +69,22 @@ int num_cus()
     return cus[d];
 }
 
+int num_xcds()
+{
+    static int xcc[64] = {0};
+    int d = current_device();
+    if (d < 0 || d >= 64) d = 0;
+    if (!xcc[d]) {
+        int x = 0;
+        if (hipDeviceGetAttribute(&x, hipDeviceAttributeNumberOfXccs, d) != hipSuccess || x < 1) {
+            hipGetLastError();
+            x = 1;
+        }
+        xcc[d] = x;
+    }
+    return xcc[d];
+}
+
 // ---------------------------------------------------------------- side table
 static std::mutex g_mu;
 static std::unordered_map<const dress_graph_t *, DevGraph *> g_table;
@@ -172,6 +188,14 @@ int fit_impl(p_dress_graph_t g, int max_iterations, double epsilon, int precisio
     if (iterations) *iterations = r.iterations;
     if (delta) *delta = r.delta;
     return 0;
+}
+
+// plain streaming copy: the calibration workload for HBM byte counters
+__global__ void k_stream_copy(const double2 *__restrict__ a, double2 *__restrict__ b, int64_t n)
+{
+    for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n;
+         i += (int64_t)gridDim.x * blockDim.x)
+        b[i] = a[i];
 }
 
 }  // namespace
@@ -371,6 +395,44 @@ int dress_hip_time_phases(p_dress_graph_t g, int reps, double *ms)
     if (!G || !G->d[0]) { set_error("dress_hip_time_phases: call dress_hip_resident_reset first"); return -1; }
     if (!ms || reps < 1) { set_error("dress_hip_time_phases: bad arguments"); return -1; }
     return fit_time_phases(G, reps, ms) ? 0 : -1;
+}
+
+int dress_hip_copy_probe(int64_t n_bytes, int reps, double *ms)
+{
+    clear_error();
+    if (n_bytes < 16 || (n_bytes & 15) || reps < 1 || !ms) {
+        set_error("dress_hip_copy_probe: n_bytes must be a positive multiple of 16, reps >= 1");
+        return -1;
+    }
+    void *a = nullptr, *b = nullptr;
+    if (!dmalloc(&a, (size_t)n_bytes, "copy probe source") || !dmalloc(&b, (size_t)n_bytes, "copy probe target")) {
+        dfree(a);
+        return -1;
+    }
+    hipMemset(a, 0, (size_t)n_bytes);
+    const int64_t n = n_bytes / 16;
+    const int grid = num_cus() * 16;
+    hipEvent_t e0, e1;
+    hipEventCreate(&e0);
+    hipEventCreate(&e1);
+    hipLaunchKernelGGL(k_stream_copy, dim3(grid), dim3(256), 0, 0, (const double2 *)a, (double2 *)b, n);
+    hipEventRecord(e0, 0);
+    for (int r = 0; r < reps; r++)
+        hipLaunchKernelGGL(k_stream_copy, dim3(grid), dim3(256), 0, 0, (const double2 *)a, (double2 *)b, n);
+    hipEventRecord(e1, 0);
+    hipError_t err = hipEventSynchronize(e1);
+    float t = 0.f;
+    hipEventElapsedTime(&t, e0, e1);
+    hipEventDestroy(e0);
+    hipEventDestroy(e1);
+    dfree(a);
+    dfree(b);
+    if (err != hipSuccess) {
+        set_error(std::string("dress_hip_copy_probe: ") + hipGetErrorString(err));
+        return -1;
+    }
+    *ms = t / reps;
+    return 0;
 }
 
 int dress_hip_resident_download(p_dress_graph_t g)

@@ -179,10 +179,35 @@ __global__ void __launch_bounds__(TPB) k_vertex(const int *__restrict__ list, in
     }
 }
 
-// Edge phase over internal edges [lo, hi).
+// first edge of slice `j` of `nx` over [lo, hi): slices hold about the same
+// cost, counted as index entries + SLICE_EDGE_COST per edge
+constexpr int64_t SLICE_EDGE_COST = 8;
+__device__ __forceinline__ int slice_start(const int64_t *ioff, int lo, int hi, int j, int nx)
+{
+    if (j <= 0) return lo;
+    if (j >= nx) return hi;
+    const int64_t base = ioff[lo];
+    const int64_t total = (ioff[hi] - base) + SLICE_EDGE_COST * (hi - lo);
+    const int64_t target = total / nx * j + (total % nx) * j / nx;
+    int a = lo, b = hi;                  // first e with cost(e) >= target
+    while (a < b) {
+        int m = a + (b - a) / 2;
+        int64_t c = (ioff[m] - base) + SLICE_EDGE_COST * (m - lo);
+        if (c < target) a = m + 1;
+        else b = m;
+    }
+    return a;
+}
+
+// Edge phase over internal edges [lo, hi).  The range is cut into `nx`
+// slices of equal cost and block b works in slice b % nx, which is the XCD
+// the hardware dispatches it to (workgroups go round-robin over the XCDs):
+// edges of one owner vertex are adjacent in a bin, so the owner's row stays
+// in that XCD's L2 while its edges are processed.  The grid is what fits on
+// the device at once, so the blocks of a slice sweep it together.
 template <int G, bool PK, typename VT>
-__global__ void __launch_bounds__(TPB) k_edge(int lo, int hi, EdgeArgs a, VT *d0, VT *d1, VT *w0, VT *w1,
-                                              FitState *st)
+__global__ void __launch_bounds__(TPB) k_edge(int lo, int hi, int nx, EdgeArgs a, VT *d0, VT *d1, VT *w0,
+                                              VT *w1, FitState *st)
 {
     const bool f32 = sizeof(VT) == 4;
     if (st->done || (st->fp32 != 0) != f32) return;
@@ -192,9 +217,12 @@ __global__ void __launch_bounds__(TPB) k_edge(int lo, int hi, EdgeArgs a, VT *d0
     VT *__restrict__ dn = cur ? d0 : d1;
     VT *__restrict__ wn = cur ? w0 : w1;
     const int lane = (G == TPB) ? threadIdx.x : (threadIdx.x % G);
-    const int64_t groups = (int64_t)gridDim.x * (TPB / G);
+    const int xs = blockIdx.x % nx;
+    const int s_lo = slice_start(a.ioff, lo, hi, xs, nx);
+    const int s_hi = slice_start(a.ioff, lo, hi, xs + 1, nx);
+    const int64_t groups = (int64_t)(gridDim.x / nx) * (TPB / G);
     double mymax = 0.0;
-    for (int64_t g = lo + (int64_t)blockIdx.x * (TPB / G) + threadIdx.x / G; g < hi; g += groups) {
+    for (int64_t g = s_lo + (int64_t)(blockIdx.x / nx) * (TPB / G) + threadIdx.x / G; g < s_hi; g += groups) {
         const int e = (int)g;
         int64_t kb = ntl(a.ioff + e), ke = ntl(a.ioff + e + 1);
         int2 r = PK ? ntl2(a.rs + e) : make_int2(0, 0);
@@ -354,17 +382,36 @@ void launch_vertex_t(DevGraph *Gr, int bin, int cap, VT *w0, VT *w1)
                        Gr->row_off, w0, w1, Gr->nw, Gr->vd, (const FitState *)Gr->state);
 }
 
+// workgroups of k_edge<G, PK, VT> resident on the whole device at once
+template <int G, bool PK, typename VT>
+int edge_capacity()
+{
+    static const int cap = [] {
+        int nb = 0;
+        if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, k_edge<G, PK, VT>, TPB, 0) != hipSuccess || nb < 1) {
+            hipGetLastError();
+            nb = 4;
+        }
+        return nb * num_cus();
+    }();
+    return cap;
+}
+
 template <int G, bool PK, typename VT>
 void launch_edge_t(DevGraph *Gr, int bin, int cap, VT *d0, VT *d1, VT *w0, VT *w1)
 {
+    (void)cap;
     int lo = Gr->ebin[bin], hi = Gr->ebin[bin + 1];
     if (hi <= lo) return;
-    int grid = grid_for((int64_t)(hi - lo) * G, cap);
+    int grid = grid_for((int64_t)(hi - lo) * G, edge_capacity<G, PK, VT>());
+    int nx = num_xcds();
+    if (grid >= 4 * nx) grid -= grid % nx;
+    else nx = 1;
     EdgeArgs a;
     a.uv = Gr->uv; a.rs = Gr->rs; a.pos = Gr->pos; a.ew = Gr->ew; a.c4 = Gr->c4; a.ioff = Gr->ioff;
     a.ipk = Gr->ipk; a.ipair = Gr->ipair; a.vd = Gr->vd;
     a.directed = (Gr->variant == DRESS_VARIANT_FORWARD || Gr->variant == DRESS_VARIANT_BACKWARD);
-    hipLaunchKernelGGL((k_edge<G, PK, VT>), dim3(grid), dim3(TPB), 0, Gr->stream, lo, hi, a, d0, d1, w0, w1,
+    hipLaunchKernelGGL((k_edge<G, PK, VT>), dim3(grid), dim3(TPB), 0, Gr->stream, lo, hi, nx, a, d0, d1, w0, w1,
                        (FitState *)Gr->state);
 }
 

@@ -157,5 +157,6 @@ bool prim_exclusive_scan_i32_to_i64(const int *in, int64_t *out, size_t n, hipSt
 // misc
 int current_device();
 int num_cus();
+int num_xcds();   // XCDs (accelerator complex dies) of the current device
 
 }  // namespace dh

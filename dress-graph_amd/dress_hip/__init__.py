@@ -102,6 +102,14 @@ class LiftedBatchResult:
     total_ms: float
 
 
+def copy_probe(n_bytes=1 << 30, reps=5):
+    """Mean ms of a plain device-to-device copy kernel over n_bytes."""
+    ms = ctypes.c_double()
+    _lib.check(_lib.lib().dress_hip_copy_probe(int(n_bytes), int(reps), ctypes.byref(ms)) == 0,
+               "dress_hip_copy_probe")
+    return ms.value
+
+
 def set_lifted_mode(mode):
     """0: batched persistent kernel (default); 1: one device fit per subgraph."""
     _lib.lib().dress_hip_set_lifted_mode(int(mode))

@@ -113,6 +113,8 @@ def _bind(L):
     L.dress_hip_resident_fill.argtypes = [PG, ctypes.c_double]
     L.dress_hip_time_phases.restype = ctypes.c_int
     L.dress_hip_time_phases.argtypes = [PG, ctypes.c_int, c_dbl_p]
+    L.dress_hip_copy_probe.restype = ctypes.c_int
+    L.dress_hip_copy_probe.argtypes = [ctypes.c_int64, ctypes.c_int, c_dbl_p]
     L.dress_hip_resident_download.restype = ctypes.c_int
     L.dress_hip_resident_download.argtypes = [PG]
     L.dress_hip_set_lifted_mode.argtypes = [ctypes.c_int]

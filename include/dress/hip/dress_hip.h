@@ -169,6 +169,13 @@ int dress_hip_resident_fill(p_dress_graph_t g, double value);
  * bin).  Leaves the resident values as they were.  Returns 0 or -1. */
 int dress_hip_time_phases(p_dress_graph_t g, int reps, double *ms);
 
+/* Streams n_bytes (a multiple of 16) from one device buffer to another with
+ * a plain copy kernel, `reps` times after one warm-up, and stores the mean
+ * device time of one copy in *ms.  Measurement aid: the achievable HBM
+ * bandwidth, and the reference reading for hardware byte counters.  No
+ * counterpart in libdress.  Returns 0 or -1. */
+int dress_hip_copy_probe(int64_t n_bytes, int reps, double *ms);
+
 /* dress_get for many pairs at once (libdress/src/dress.c:651-739 per
  * query), against g's current edge_dress / vertex_dress.  out[i] receives
  * the value of (u[i], v[i]) with hypothetical weight edge_weight[i].
