@@ -354,29 +354,158 @@ bool vertex_bins(hipStream_t s, int n, const int *row_off, int *list, int counts
     return ok;
 }
 
-// reference-layout intercept arrays (input edge ids, input order offsets)
-__global__ void k_index_to_reference(const int *ord, int E, const int64_t *ioff, const int2 *rs,
-                                     const uint32_t *ipk, const int2 *ipair, const int *adj_e,
-                                     const int64_t *ioff_in, int *ux, int *vx)
+// ------------------------------------------------------ internal labels
+
+// rank key of vertex u: decreasing degree, then increasing id
+__global__ void k_rank_keys(const int *row_off, int N, uint64_t *key)
 {
-    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < E; i += gridDim.x * blockDim.x) {
-        int e = ord[i];
-        int64_t src = ioff[i], n = ioff[i + 1] - ioff[i], dst = ioff_in[e];
-        int2 r = rs[i];
-        for (int64_t k = 0; k < n; k++) {
-            int a, b;
-            if (ipk) {
-                uint32_t q = ipk[src + k];
-                a = r.x + (int)(q & 0xffffu);
-                b = r.y + (int)(q >> 16);
-            } else {
-                a = ipair[src + k].x;
-                b = ipair[src + k].y;
-            }
-            ux[dst + k] = adj_e[a];
-            vx[dst + k] = adj_e[b];
+    for (int u = blockIdx.x * blockDim.x + threadIdx.x; u < N; u += gridDim.x * blockDim.x) {
+        uint32_t deg = (uint32_t)(row_off[u + 1] - row_off[u]);
+        key[u] = ((uint64_t)(0xffffffffu - deg) << 32) | (uint32_t)u;
+    }
+}
+
+__global__ void k_rank_scatter(const uint64_t *sorted, int N, int *rank)
+{
+    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < N; i += gridDim.x * blockDim.x)
+        rank[(int)(uint32_t)(sorted[i] & 0xffffffffu)] = i;
+}
+
+__global__ void k_relabel_edges(const int *U, const int *V, const int *rank, int E, int *U2, int *V2)
+{
+    for (int e = blockIdx.x * blockDim.x + threadIdx.x; e < E; e += gridDim.x * blockDim.x) {
+        U2[e] = rank[U[e]];
+        V2[e] = rank[V[e]];
+    }
+}
+
+// to[rank[v]] = from[v]
+__global__ void k_to_internal(const double *from, const int *rank, int N, double *to)
+{
+    for (int v = blockIdx.x * blockDim.x + threadIdx.x; v < N; v += gridDim.x * blockDim.x)
+        to[rank[v]] = from[v];
+}
+
+// to[v] = from[rank[v]]
+__global__ void k_to_input(const double *from, const int *rank, int N, double *to)
+{
+    for (int v = blockIdx.x * blockDim.x + threadIdx.x; v < N; v += gridDim.x * blockDim.x)
+        to[v] = from[rank[v]];
+}
+
+// internal degree of every internal row
+__global__ void k_internal_degrees(const int *row_off, const int *rank, int N, int *deg)
+{
+    for (int u = blockIdx.x * blockDim.x + threadIdx.x; u < N; u += gridDim.x * blockDim.x)
+        deg[rank[u]] = row_off[u + 1] - row_off[u];
+}
+
+// half-edge p of reference row u -> key (rank[u], rank[x]); the stable sort
+// keeps equal neighbours in edge order, as in the reference row
+__global__ void k_internal_half_keys(const int *row_off, const int *adj_x, const int *adj_e, const int *rank,
+                                     int N, uint64_t *key, int *val)
+{
+    for (int u = blockIdx.x * blockDim.x + threadIdx.x; u < N; u += gridDim.x * blockDim.x) {
+        const uint64_t hi = (uint64_t)(uint32_t)rank[u] << 32;
+        for (int p = row_off[u]; p < row_off[u + 1]; p++) {
+            key[p] = hi | (uint32_t)rank[adj_x[p]];
+            val[p] = adj_e[p];
         }
     }
+}
+
+__global__ void k_unpack_internal(const uint64_t *key, const int *val, int64_t S, int *adj_x, int *adj_e)
+{
+    for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < S;
+         i += (int64_t)gridDim.x * blockDim.x) {
+        adj_x[i] = (int)(uint32_t)(key[i] & 0xffffffffu);
+        adj_e[i] = val[i];
+    }
+}
+
+__global__ void k_iota(int *p, int n)
+{
+    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) p[i] = i;
+}
+
+// absolute half-edge pairs -> edge ids through the reference adjacency
+__global__ void k_pairs_to_edges(const int2 *pairs, int64_t T, const int *adj_e, int *ux, int *vx)
+{
+    for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < T;
+         i += (int64_t)gridDim.x * blockDim.x) {
+        int2 p = pairs[i];
+        ux[i] = adj_e[p.x];
+        vx[i] = adj_e[p.y];
+    }
+}
+
+// Internal labels: moves the reference-layout CSR just built (input
+// labels) to G->ref_* and builds the CSR the fit runs on, over vertices
+// relabelled by decreasing degree with rows ordered by neighbour rank.
+// Every sum of the fit is exactly rounded, so results do not depend on
+// this order; it only decides which values share cache lines.
+bool relabel(DevGraph *G)
+{
+    hipStream_t s = G->stream;
+    const int N = G->N, E = G->E;
+    const int64_t S = G->S;
+    G->ref_U = G->U; G->ref_V = G->V;
+    G->ref_row_off = G->row_off; G->ref_adj_x = G->adj_x; G->ref_adj_e = G->adj_e;
+    G->U = G->V = G->row_off = G->adj_x = G->adj_e = nullptr;
+    double *nw_in = G->nw;
+    G->nw = nullptr;
+    const size_t nk = (size_t)std::max<int64_t>(N, S) + 1;
+    uint64_t *k0 = nullptr, *k1 = nullptr;
+    int *v0 = nullptr, *v1 = nullptr, *deg = nullptr;
+    int64_t *deg64 = nullptr;
+    bool ok = dalloc(&G->rank, (size_t)N + 1, "vertex ranks") && dalloc(&G->U, (size_t)E + 1, "U") &&
+              dalloc(&G->V, (size_t)E + 1, "V") && dalloc(&G->row_off, (size_t)N + 1, "row offsets") &&
+              dalloc(&G->adj_x, (size_t)S + 1, "adj_target") && dalloc(&G->adj_e, (size_t)S + 1, "adj_edge_idx") &&
+              dalloc(&G->nw, (size_t)N + 1, "NW") && dalloc(&k0, nk, "relabel keys") &&
+              dalloc(&k1, nk, "relabel keys") && dalloc(&v0, (size_t)S + 1, "relabel vals") &&
+              dalloc(&v1, (size_t)S + 1, "relabel vals") && dalloc(&deg, (size_t)N + 1, "degrees") &&
+              dalloc(&deg64, (size_t)N + 1, "degrees");
+    int bits = 1;
+    while (bits < 31 && (1 << bits) < N) bits++;
+    if (ok && N > 0) {
+        hipLaunchKernelGGL(k_rank_keys, dim3(grid_for(N, 4096)), dim3(TPB), 0, s, G->ref_row_off, N, k0);
+        ok = prim_sort_u64(k0, k1, (size_t)N, s);
+        if (ok) hipLaunchKernelGGL(k_rank_scatter, dim3(grid_for(N, 4096)), dim3(TPB), 0, s, k1, N, G->rank);
+    }
+    if (ok) {
+        if (E > 0)
+            hipLaunchKernelGGL(k_relabel_edges, dim3(grid_for(E, 8192)), dim3(TPB), 0, s, G->ref_U, G->ref_V,
+                               G->rank, E, G->U, G->V);
+        if (N > 0)
+            hipLaunchKernelGGL(k_to_internal, dim3(grid_for(N, 4096)), dim3(TPB), 0, s, nw_in, G->rank, N, G->nw);
+        hipLaunchKernelGGL(k_fill_i32, dim3(grid_for(N + 1, 4096)), dim3(TPB), 0, s, deg, (int64_t)N + 1, 0);
+        if (N > 0)
+            hipLaunchKernelGGL(k_internal_degrees, dim3(grid_for(N, 4096)), dim3(TPB), 0, s, G->ref_row_off,
+                               G->rank, N, deg);
+        ok = prim_exclusive_scan_i32_to_i64(deg, deg64, (size_t)N + 1, s);
+    }
+    if (ok) {
+        hipLaunchKernelGGL(k_i64_to_i32, dim3(grid_for(N + 1, 4096)), dim3(TPB), 0, s, deg64, G->row_off,
+                           (int64_t)N + 1);
+        if (S > 0) {
+            hipLaunchKernelGGL(k_internal_half_keys, dim3(grid_for(N, 4096)), dim3(TPB), 0, s, G->ref_row_off,
+                               G->ref_adj_x, G->ref_adj_e, G->rank, N, k0, v0);
+            ok = prim_sort_pairs_u64_i32(k0, k1, v0, v1, (size_t)S, 32 + bits, s);
+            if (ok)
+                hipLaunchKernelGGL(k_unpack_internal, dim3(grid_for(S, 8192)), dim3(TPB), 0, s, k1, v1, S,
+                                   G->adj_x, G->adj_e);
+        }
+    }
+    if (ok) {
+        hipError_t e1 = hipGetLastError();
+        hipError_t e2 = hipStreamSynchronize(s);
+        if (e1 != hipSuccess || e2 != hipSuccess) {
+            set_error(std::string("relabelling failed: ") + hipGetErrorString(e1 != hipSuccess ? e1 : e2));
+            ok = false;
+        }
+    }
+    dfree(k0); dfree(k1); dfree(v0); dfree(v1); dfree(deg); dfree(deg64); dfree(nw_in);
+    return ok;
 }
 
 }  // namespace
@@ -477,7 +606,7 @@ bool build_graph(DevGraph *G, int N, int E, const int *U, const int *V, const do
     }
     dfree(hk0); dfree(hk1); dfree(hv0); dfree(hv1);
     dfree(rkey); dfree(rval); dfree(roff); dfree(deg); dfree(deg64); dfree(dW);
-    return ok;
+    return ok && relabel(G);
 }
 
 bool build_index(DevGraph *G)
@@ -578,8 +707,12 @@ bool build_bins(DevGraph *G)
 bool refresh_self_terms(DevGraph *G, const double *NW_host)
 {
     hipStream_t s = G->stream;
-    if (NW_host && G->N > 0)
-        DH_CHECK(hipMemcpyAsync(G->nw, NW_host, (size_t)G->N * sizeof(double), hipMemcpyHostToDevice, s));
+    if (NW_host && G->N > 0) {
+        if (!G->vstage && !dalloc(&G->vstage, (size_t)G->N + 1, "vertex staging")) return false;
+        DH_CHECK(hipMemcpyAsync(G->vstage, NW_host, (size_t)G->N * sizeof(double), hipMemcpyHostToDevice, s));
+        hipLaunchKernelGGL(k_to_internal, dim3(grid_for(G->N, 4096)), dim3(TPB), 0, s, (const double *)G->vstage,
+                           (const int *)G->rank, G->N, G->nw);
+    }
     if (G->E > 0) {
         hipLaunchKernelGGL(k_edge_consts, dim3(grid_for(G->E, 8192)), dim3(TPB), 0, s, G->U, G->V, G->nw,
                            G->E, G->variant, G->c4_in);
@@ -595,10 +728,11 @@ bool download_csr(DevGraph *G, int *adj_offset, int *adj_target, int *adj_edge_i
                   double *edge_weight)
 {
     hipStream_t s = G->stream;
-    DH_CHECK(hipMemcpyAsync(adj_offset, G->row_off, ((size_t)G->N + 1) * sizeof(int), hipMemcpyDeviceToHost, s));
+    if (!G->ref_row_off) { set_error("reference adjacency is no longer on the device"); return false; }
+    DH_CHECK(hipMemcpyAsync(adj_offset, G->ref_row_off, ((size_t)G->N + 1) * sizeof(int), hipMemcpyDeviceToHost, s));
     if (G->S > 0) {
-        DH_CHECK(hipMemcpyAsync(adj_target, G->adj_x, (size_t)G->S * sizeof(int), hipMemcpyDeviceToHost, s));
-        DH_CHECK(hipMemcpyAsync(adj_edge_idx, G->adj_e, (size_t)G->S * sizeof(int), hipMemcpyDeviceToHost, s));
+        DH_CHECK(hipMemcpyAsync(adj_target, G->ref_adj_x, (size_t)G->S * sizeof(int), hipMemcpyDeviceToHost, s));
+        DH_CHECK(hipMemcpyAsync(adj_edge_idx, G->ref_adj_e, (size_t)G->S * sizeof(int), hipMemcpyDeviceToHost, s));
     }
     if (G->E > 0)
         DH_CHECK(hipMemcpyAsync(edge_weight, G->ew_in, (size_t)G->E * sizeof(double), hipMemcpyDeviceToHost, s));
@@ -607,23 +741,29 @@ bool download_csr(DevGraph *G, int *adj_offset, int *adj_target, int *adj_edge_i
 }
 
 // Intercept arrays in the reference's form (input edge ids, int offsets in
-// input order), malloc'd.
+// input order), malloc'd: the multiset intersections are recomputed on the
+// reference-layout rows, so every edge lists its common neighbours in
+// increasing input label, as dress.c:307-363 does.
 bool download_index(DevGraph *G, int **ioff_out, int **ux_out, int **vx_out)
 {
     if (!build_index(G)) return false;
+    if (!G->ref_row_off) { set_error("reference adjacency is no longer on the device"); return false; }
     if (G->T >= (int64_t)INT32_MAX) { set_error("intersection index exceeds int32 offsets of dress_graph_t"); return false; }
     hipStream_t s = G->stream;
     const int E = G->E;
     const int64_t T = G->T;
+    const int GS = 16;
     int *io = (int *)malloc(((size_t)E + 1) * sizeof(int));
     int *ux = (int *)malloc(((size_t)T + 1) * sizeof(int));
     int *vx = (int *)malloc(((size_t)T + 1) * sizeof(int));
     int64_t *io64 = (int64_t *)malloc(((size_t)E + 1) * sizeof(int64_t));
-    int *dux = nullptr, *dvx = nullptr;
+    int *dux = nullptr, *dvx = nullptr, *ident = nullptr;
     int64_t *dio = nullptr, *dcnt = nullptr;
+    int2 *pairs = nullptr;
     if (!io || !ux || !vx || !io64) set_error("host allocation failed for intercept arrays");
     bool ok = io && ux && vx && io64 && dalloc(&dux, (size_t)T + 1, "ux") && dalloc(&dvx, (size_t)T + 1, "vx") &&
-              dalloc(&dio, (size_t)E + 1, "input offsets") && dalloc(&dcnt, (size_t)E + 1, "input counts");
+              dalloc(&dio, (size_t)E + 1, "input offsets") && dalloc(&dcnt, (size_t)E + 1, "input counts") &&
+              dalloc(&ident, (size_t)E + 1, "identity order") && dalloc(&pairs, (size_t)T + 1, "half-edge pairs");
     if (ok) {
         hipError_t e0 = hipMemcpyAsync(dcnt, G->cnt_in, (size_t)E * sizeof(int64_t), hipMemcpyDeviceToDevice, s);
         if (e0 != hipSuccess) { set_error("intercept copy failed"); ok = false; }
@@ -631,9 +771,15 @@ bool download_index(DevGraph *G, int **ioff_out, int **ux_out, int **vx_out)
         ok = ok && prim_exclusive_scan_i64(dcnt, dio, (size_t)E + 1, s);
     }
     if (ok) {
-        if (E > 0)
-            hipLaunchKernelGGL(k_index_to_reference, dim3(grid_for(E, 8192)), dim3(TPB), 0, s, G->ord, E, G->ioff,
-                               G->rs, G->ipk, G->ipair, G->adj_e, dio, dux, dvx);
+        if (E > 0 && T > 0) {
+            hipLaunchKernelGGL(k_iota, dim3(grid_for(E, 8192)), dim3(TPB), 0, s, ident, E);
+            hipLaunchKernelGGL((k_isect<GS, true>), dim3(grid_for((int64_t)E * GS, num_cus() * 32)), dim3(TPB), 0,
+                               s, (const int *)G->ref_U, (const int *)G->ref_V, E, (const int *)G->ref_row_off,
+                               (const int *)G->ref_adj_x, (const int *)ident, (int64_t *)nullptr,
+                               (const int64_t *)dio, (uint32_t *)nullptr, pairs);
+            hipLaunchKernelGGL(k_pairs_to_edges, dim3(grid_for(T, 8192)), dim3(TPB), 0, s, (const int2 *)pairs, T,
+                               (const int *)G->ref_adj_e, dux, dvx);
+        }
         hipError_t e1 = hipMemcpyAsync(io64, dio, ((size_t)E + 1) * sizeof(int64_t), hipMemcpyDeviceToHost, s);
         hipError_t e2 = T > 0 ? hipMemcpyAsync(ux, dux, (size_t)T * sizeof(int), hipMemcpyDeviceToHost, s) : hipSuccess;
         hipError_t e3 = T > 0 ? hipMemcpyAsync(vx, dvx, (size_t)T * sizeof(int), hipMemcpyDeviceToHost, s) : hipSuccess;
@@ -645,9 +791,51 @@ bool download_index(DevGraph *G, int **ioff_out, int **ux_out, int **vx_out)
     }
     if (ok) for (int e = 0; e <= E; e++) io[e] = (int)io64[e];
     free(io64);
-    dfree(dux); dfree(dvx); dfree(dio); dfree(dcnt);
+    dfree(dux); dfree(dvx); dfree(dio); dfree(dcnt); dfree(ident); dfree(pairs);
     if (!ok) { free(io); free(ux); free(vx); return false; }
     *ioff_out = io; *ux_out = ux; *vx_out = vx;
+    return true;
+}
+
+// per-vertex device values in internal labels -> host array in input labels
+bool download_vertex_values(DevGraph *G, const double *internal, double *host)
+{
+    if (G->N <= 0) return true;
+    hipStream_t s = G->stream;
+    if (!G->vstage && !dalloc(&G->vstage, (size_t)G->N + 1, "vertex staging")) return false;
+    hipLaunchKernelGGL(k_to_input, dim3(grid_for(G->N, 4096)), dim3(TPB), 0, s, internal, (const int *)G->rank, G->N,
+                       G->vstage);
+    DH_CHECK(hipMemcpyAsync(host, G->vstage, (size_t)G->N * sizeof(double), hipMemcpyDeviceToHost, s));
+    DH_CHECK(hipStreamSynchronize(s));
+    return true;
+}
+
+void release_reference_csr(DevGraph *G)
+{
+    if (G->stream) hipStreamSynchronize(G->stream);
+    dfree(G->ref_U); dfree(G->ref_V); dfree(G->ref_row_off); dfree(G->ref_adj_x); dfree(G->ref_adj_e);
+    G->ref_U = G->ref_V = G->ref_row_off = G->ref_adj_x = G->ref_adj_e = nullptr;
+}
+
+// the reference-layout adjacency of a dress_graph_t (input labels), for
+// queries that must probe rows in the reference's order
+bool upload_reference_csr(DevGraph *G, const int *adj_offset, const int *adj_target, const int *adj_edge_idx)
+{
+    if (G->ref_row_off && G->ref_adj_x && G->ref_adj_e) return true;
+    if (!adj_offset || (G->S > 0 && (!adj_target || !adj_edge_idx))) {
+        set_error("graph has no adjacency arrays");
+        return false;
+    }
+    hipStream_t s = G->stream;
+    if (!G->ref_row_off && !dalloc(&G->ref_row_off, (size_t)G->N + 1, "reference row offsets")) return false;
+    if (!G->ref_adj_x && !dalloc(&G->ref_adj_x, (size_t)G->S + 1, "reference adjacency")) return false;
+    if (!G->ref_adj_e && !dalloc(&G->ref_adj_e, (size_t)G->S + 1, "reference adjacency")) return false;
+    DH_CHECK(hipMemcpyAsync(G->ref_row_off, adj_offset, ((size_t)G->N + 1) * sizeof(int), hipMemcpyHostToDevice, s));
+    if (G->S > 0) {
+        DH_CHECK(hipMemcpyAsync(G->ref_adj_x, adj_target, (size_t)G->S * sizeof(int), hipMemcpyHostToDevice, s));
+        DH_CHECK(hipMemcpyAsync(G->ref_adj_e, adj_edge_idx, (size_t)G->S * sizeof(int), hipMemcpyHostToDevice, s));
+    }
+    DH_CHECK(hipStreamSynchronize(s));
     return true;
 }
 
@@ -657,6 +845,8 @@ void free_graph(DevGraph *G)
     if (G->stream) hipStreamSynchronize(G->stream);
     dfree(G->U); dfree(G->V); dfree(G->row_off); dfree(G->adj_x); dfree(G->adj_e);
     dfree(G->ew_in); dfree(G->c4_in); dfree(G->nw); dfree(G->pos_in);
+    dfree(G->rank); dfree(G->vstage);
+    dfree(G->ref_U); dfree(G->ref_V); dfree(G->ref_row_off); dfree(G->ref_adj_x); dfree(G->ref_adj_e);
     dfree(G->ord); dfree(G->inv); dfree(G->uv); dfree(G->rs); dfree(G->pos); dfree(G->ew); dfree(G->c4);
     dfree(G->ioff); dfree(G->ipk); dfree(G->ipair); dfree(G->cnt_in);
     dfree(G->vbins.list);

@@ -134,6 +134,7 @@ DevGraph *device_graph(p_dress_graph_t g)
         free_graph(G);
         return nullptr;
     }
+    release_reference_csr(G);
     table_put(g, G);
     return G;
 }
@@ -270,6 +271,7 @@ p_dress_graph_t dress_init_graph(int N, int E, int *U, int *V, double *W, double
         if (precompute_intercepts)
             ok = download_index(G, &g->intercept_offset, &g->intercept_edge_ux, &g->intercept_edge_vx);
     }
+    if (ok) release_reference_csr(G);
     if (!ok) {
         free_graph(G);
         free(g->adj_offset); free(g->adj_target); free(g->adj_edge_idx); free(g->edge_weight);

@@ -710,9 +710,8 @@ bool fit_download(DevGraph *G, double *edge_dress, double *vertex_dress)
                            G->d[G->cur], G->stage);
         DH_CHECK(hipMemcpyAsync(edge_dress, G->stage, (size_t)G->E * sizeof(double), hipMemcpyDeviceToHost, s));
     }
-    if (vertex_dress && G->N > 0)
-        DH_CHECK(hipMemcpyAsync(vertex_dress, G->vd, (size_t)G->N * sizeof(double), hipMemcpyDeviceToHost, s));
     DH_CHECK(hipStreamSynchronize(s));
+    if (vertex_dress && G->N > 0 && !download_vertex_values(G, G->vd, vertex_dress)) return false;
     return true;
 }
 

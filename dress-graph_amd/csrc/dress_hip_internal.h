@@ -84,9 +84,20 @@ struct DevGraph {
     bool weighted = false;
     hipStream_t stream = nullptr;
 
-    int *U = nullptr, *V = nullptr;        // input order
-    int *row_off = nullptr, *adj_x = nullptr, *adj_e = nullptr;
-    double *ew_in = nullptr, *c4_in = nullptr, *nw = nullptr;   // input order
+    // Vertices carry internal labels: rank[v] is v's position in order of
+    // decreasing degree (ties by id).  Rows list neighbours by rank, so a
+    // row starts with its high-degree neighbours, where most common
+    // neighbours of an edge lie, and the rows of the hubs are adjacent.
+    int *U = nullptr, *V = nullptr;        // input order, internal labels
+    int *row_off = nullptr, *adj_x = nullptr, *adj_e = nullptr;   // internal CSR
+    double *ew_in = nullptr, *c4_in = nullptr;   // input order
+    double *nw = nullptr;                  // internal labels
+    int *rank = nullptr;                   // input label -> internal label
+    // reference-layout CSR in input labels (libdress's adjacency), kept
+    // until its download; rebuilt from the host struct for dress_get
+    int *ref_U = nullptr, *ref_V = nullptr;
+    int *ref_row_off = nullptr, *ref_adj_x = nullptr, *ref_adj_e = nullptr;
+    double *vstage = nullptr;              // per-vertex staging (input labels)
     int2 *pos_in = nullptr;                // half-edges of input edge e
 
     // internal edge order (valid once have_index)
@@ -124,6 +135,9 @@ bool download_csr(DevGraph *G, int *adj_offset, int *adj_target, int *adj_edge_i
                   double *edge_weight);
 bool download_index(DevGraph *G, int **ioff_out, int **ux_out, int **vx_out);
 bool refresh_self_terms(DevGraph *G, const double *NW_host);
+void release_reference_csr(DevGraph *G);
+bool download_vertex_values(DevGraph *G, const double *internal, double *host);
+bool upload_reference_csr(DevGraph *G, const int *adj_offset, const int *adj_target, const int *adj_edge_idx);
 void free_graph(DevGraph *G);
 
 // dress_fit.hip

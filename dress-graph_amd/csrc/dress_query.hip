@@ -6,8 +6,9 @@
 // the virtual edge against the frozen state: A = sum over common neighbours of
 // w-bar*d(u,x) + w-bar*d(v,x) plus the self term, then
 //   d <- (A + cross * cw * d) / (sqrt(|u|^2 + cw*d) * sqrt(|v|^2 + cw*d)).
-// The state is the graph's host values (edge_dress, vertex_dress), uploaded
-// before the queries, so results match what the reference computes on g.
+// The state is the graph's host values (edge_dress, vertex_dress, NW),
+// uploaded before the queries, so results match what the reference computes
+// on g.
 #include "dress_capi_internal.h"
 #include "dress/hip/dress_hip.h"
 
@@ -30,10 +31,14 @@ __device__ __forceinline__ void two_sum_add(double &s, double &c, double x)
     c += e;
 }
 
+// Rows are the reference's adjacency (input labels, neighbours ascending),
+// so the binary search probes the same entries as dress.c:656-667 and
+// returns the same copy of a repeated edge.  Half-edge values are
+// edge_weight * edge_dress of the edge the half-edge belongs to.
 __global__ void k_get(int n, const int *__restrict__ qu, const int *__restrict__ qv, const double *__restrict__ qw,
                       int max_it, double eps, const int *__restrict__ row_off, const int *__restrict__ adj_x,
                       const int *__restrict__ adj_e, const int *__restrict__ inv, const double *__restrict__ d,
-                      const double *__restrict__ wh, const double *__restrict__ vdn, const double *__restrict__ nw,
+                      const double *__restrict__ ew, const double *__restrict__ vdn, const double *__restrict__ nw,
                       int variant, double *out)
 {
     for (int q = blockIdx.x * blockDim.x + threadIdx.x; q < n; q += gridDim.x * blockDim.x) {
@@ -60,7 +65,8 @@ __global__ void k_get(int n, const int *__restrict__ qu, const int *__restrict__
             while (a < ae && b < be) {
                 int x = adj_x[a], y = adj_x[b];
                 if (x == y) {
-                    two_sum_add(s, c, wh[a] + wh[b]);
+                    const int ea = adj_e[a], eb = adj_e[b];
+                    two_sum_add(s, c, ew[ea] * d[inv[ea]] + ew[eb] * d[inv[eb]]);
                     a++;
                     b++;
                 } else if (x < y) a++;
@@ -93,16 +99,6 @@ __global__ void k_get(int n, const int *__restrict__ qu, const int *__restrict__
     }
 }
 
-__global__ void k_wh_from_d(int E, const double *ew, const int2 *pos, const double *d, double *wh)
-{
-    for (int e = blockIdx.x * blockDim.x + threadIdx.x; e < E; e += gridDim.x * blockDim.x) {
-        double x = ew[e] * d[e];
-        int2 q = pos[e];
-        wh[q.x] = x;
-        if (q.y >= 0) wh[q.y] = x;
-    }
-}
-
 }  // namespace
 
 // queries against g's host state
@@ -117,32 +113,33 @@ bool get_batch(p_dress_graph_t g, int n, const int *u, const int *v, const doubl
     }
     DevGraph *G = device_graph(g);
     if (!G) return false;
-    if (!refresh_self_terms(G, g->NW) || !fit_upload_values(G, g->edge_dress)) return false;
+    if (!build_index(G) || !fit_upload_values(G, g->edge_dress) ||
+        !upload_reference_csr(G, g->adj_offset, g->adj_target, g->adj_edge_idx))
+        return false;
     hipStream_t s = G->stream;
     int *du = nullptr, *dv = nullptr;
-    double *dw = nullptr, *dout = nullptr, *dvd = nullptr, *wh = nullptr;
+    double *dw = nullptr, *dout = nullptr, *dvd = nullptr, *dnw = nullptr;
     bool ok = dalloc(&du, (size_t)n + 1, "queries") && dalloc(&dv, (size_t)n + 1, "queries") &&
               dalloc(&dw, (size_t)n + 1, "queries") && dalloc(&dout, (size_t)n + 1, "answers") &&
-              dalloc(&dvd, (size_t)g->N + 1, "norms") && dalloc(&wh, (size_t)G->S + 1, "half-edge values");
+              dalloc(&dvd, (size_t)g->N + 1, "norms") && dalloc(&dnw, (size_t)g->N + 1, "vertex weights");
     if (ok) {
         ok = hipMemcpyAsync(du, u, (size_t)n * sizeof(int), hipMemcpyHostToDevice, s) == hipSuccess &&
              hipMemcpyAsync(dv, v, (size_t)n * sizeof(int), hipMemcpyHostToDevice, s) == hipSuccess &&
              hipMemcpyAsync(dw, w, (size_t)n * sizeof(double), hipMemcpyHostToDevice, s) == hipSuccess &&
-             (g->N == 0 || hipMemcpyAsync(dvd, g->vertex_dress, (size_t)g->N * sizeof(double),
-                                          hipMemcpyHostToDevice, s) == hipSuccess);
-        if (ok && G->E > 0)
-            hipLaunchKernelGGL(k_wh_from_d, dim3(std::min(8192, (G->E + TPB - 1) / TPB)), dim3(TPB), 0, s, G->E,
-                               G->ew, G->pos, G->d[G->cur], wh);
+             (g->N == 0 || (hipMemcpyAsync(dvd, g->vertex_dress, (size_t)g->N * sizeof(double),
+                                           hipMemcpyHostToDevice, s) == hipSuccess &&
+                            hipMemcpyAsync(dnw, g->NW, (size_t)g->N * sizeof(double), hipMemcpyHostToDevice,
+                                           s) == hipSuccess));
         if (ok && n > 0)
             hipLaunchKernelGGL(k_get, dim3(std::min(8192, (n + TPB - 1) / TPB)), dim3(TPB), 0, s, n, du, dv, dw,
-                               max_it, eps, G->row_off, G->adj_x, G->adj_e, G->inv, G->d[G->cur], wh, dvd, G->nw,
-                               G->variant, dout);
+                               max_it, eps, G->ref_row_off, G->ref_adj_x, G->ref_adj_e, G->inv, G->d[G->cur],
+                               G->ew_in, dvd, dnw, G->variant, dout);
         ok = ok && hipGetLastError() == hipSuccess &&
              hipMemcpyAsync(out, dout, (size_t)n * sizeof(double), hipMemcpyDeviceToHost, s) == hipSuccess &&
              hipStreamSynchronize(s) == hipSuccess;
         if (!ok) set_error("dress_get kernel failed");
     }
-    dfree(du); dfree(dv); dfree(dw); dfree(dout); dfree(dvd); dfree(wh);
+    dfree(du); dfree(dv); dfree(dw); dfree(dout); dfree(dvd); dfree(dnw);
     return ok;
 }
 
