@@ -444,6 +444,19 @@ __global__ void k_pairs_to_edges(const int2 *pairs, int64_t T, const int *adj_e,
 // relabelled by decreasing degree with rows ordered by neighbour rank.
 // Every sum of the fit is exactly rounded, so results do not depend on
 // this order; it only decides which values share cache lines.
+// Graphs below this many half-edges keep their input labels: their values
+// fit in one XCD's L2, and the extra sort would only add build latency.
+// DRESS_HIP_RELABEL=always|never overrides.
+constexpr int64_t RELABEL_MIN_HALF_EDGES = 1 << 18;
+
+bool want_relabel(int64_t S)
+{
+    const char *e = getenv("DRESS_HIP_RELABEL");
+    if (e && (!strcmp(e, "always") || !strcmp(e, "1"))) return true;
+    if (e && (!strcmp(e, "never") || !strcmp(e, "0"))) return false;
+    return S >= RELABEL_MIN_HALF_EDGES;
+}
+
 bool relabel(DevGraph *G)
 {
     hipStream_t s = G->stream;
@@ -451,6 +464,13 @@ bool relabel(DevGraph *G)
     const int64_t S = G->S;
     G->ref_U = G->U; G->ref_V = G->V;
     G->ref_row_off = G->row_off; G->ref_adj_x = G->adj_x; G->ref_adj_e = G->adj_e;
+    if (!want_relabel(S)) {
+        // identity labels: the reference arrays are the fit's arrays
+        if (!dalloc(&G->rank, (size_t)N + 1, "vertex ranks")) return false;
+        if (N > 0) hipLaunchKernelGGL(k_iota, dim3(grid_for(N, 4096)), dim3(TPB), 0, s, G->rank, N);
+        DH_CHECK(hipGetLastError());
+        return true;
+    }
     G->U = G->V = G->row_off = G->adj_x = G->adj_e = nullptr;
     double *nw_in = G->nw;
     G->nw = nullptr;
@@ -797,6 +817,17 @@ bool download_index(DevGraph *G, int **ioff_out, int **ux_out, int **vx_out)
     return true;
 }
 
+// frees the reference-layout arrays that are not also the fit's arrays
+void free_reference_arrays(DevGraph *G)
+{
+    if (G->ref_U != G->U) dfree(G->ref_U);
+    if (G->ref_V != G->V) dfree(G->ref_V);
+    if (G->ref_row_off != G->row_off) dfree(G->ref_row_off);
+    if (G->ref_adj_x != G->adj_x) dfree(G->ref_adj_x);
+    if (G->ref_adj_e != G->adj_e) dfree(G->ref_adj_e);
+    G->ref_U = G->ref_V = G->ref_row_off = G->ref_adj_x = G->ref_adj_e = nullptr;
+}
+
 // per-vertex device values in internal labels -> host array in input labels
 bool download_vertex_values(DevGraph *G, const double *internal, double *host)
 {
@@ -813,8 +844,7 @@ bool download_vertex_values(DevGraph *G, const double *internal, double *host)
 void release_reference_csr(DevGraph *G)
 {
     if (G->stream) hipStreamSynchronize(G->stream);
-    dfree(G->ref_U); dfree(G->ref_V); dfree(G->ref_row_off); dfree(G->ref_adj_x); dfree(G->ref_adj_e);
-    G->ref_U = G->ref_V = G->ref_row_off = G->ref_adj_x = G->ref_adj_e = nullptr;
+    free_reference_arrays(G);
 }
 
 // the reference-layout adjacency of a dress_graph_t (input labels), for
@@ -846,7 +876,7 @@ void free_graph(DevGraph *G)
     dfree(G->U); dfree(G->V); dfree(G->row_off); dfree(G->adj_x); dfree(G->adj_e);
     dfree(G->ew_in); dfree(G->c4_in); dfree(G->nw); dfree(G->pos_in);
     dfree(G->rank); dfree(G->vstage);
-    dfree(G->ref_U); dfree(G->ref_V); dfree(G->ref_row_off); dfree(G->ref_adj_x); dfree(G->ref_adj_e);
+    free_reference_arrays(G);
     dfree(G->ord); dfree(G->inv); dfree(G->uv); dfree(G->rs); dfree(G->pos); dfree(G->ew); dfree(G->c4);
     dfree(G->ioff); dfree(G->ipk); dfree(G->ipair); dfree(G->cnt_in);
     dfree(G->vbins.list);
