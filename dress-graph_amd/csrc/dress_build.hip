@@ -537,7 +537,15 @@ bool build_graph(DevGraph *G, int N, int E, const int *U, const int *V, const do
     if ((int64_t)E * 2 >= (int64_t)INT32_MAX) { set_error("too many edges for int32 CSR"); return false; }
     if (variant < 0 || variant > 3) { set_error("unknown variant"); return false; }
     G->N = N; G->E = E; G->variant = variant; G->weighted = W != nullptr;
-    if (!G->stream) DH_CHECK(hipStreamCreateWithFlags(&G->stream, hipStreamNonBlocking));
+    if (!G->stream) {
+        if (hipStream_t cs = ctx_stream()) {
+            G->stream = cs;
+            G->own_stream = false;
+        } else {
+            DH_CHECK(hipStreamCreateWithFlags(&G->stream, hipStreamNonBlocking));
+            G->own_stream = true;
+        }
+    }
     hipStream_t s = G->stream;
 
     if (!dalloc(&G->U, E, "U") || !dalloc(&G->V, E, "V") || !dalloc(&G->ew_in, E, "edge_weight") ||
@@ -882,7 +890,7 @@ void free_graph(DevGraph *G)
     dfree(G->vbins.list);
     for (int i = 0; i < 2; i++) { dfree(G->d[i]); dfree(G->wh[i]); dfree(G->d32[i]); dfree(G->wh32[i]); }
     dfree(G->vd); dfree(G->stage); dfree(G->state);
-    if (G->stream) hipStreamDestroy(G->stream);
+    if (G->stream && G->own_stream) hipStreamDestroy(G->stream);
     delete G;
 }
 

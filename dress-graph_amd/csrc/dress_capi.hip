@@ -13,6 +13,7 @@
 #include <cstring>
 #include <mutex>
 #include <unordered_map>
+#include <vector>
 
 #include "dress/hip/dress_hip.h"
 
@@ -30,10 +31,41 @@ void set_error(const std::string &msg)
 void clear_error() { t_err.clear(); }
 bool has_error() { return !t_err.empty(); }
 
+// ---------------------------------------------------------------- allocation
+namespace {
+struct AllocCtx {
+    bool active = false;
+    hipStream_t stream = nullptr;
+    std::unordered_map<void *, size_t> size_of;          // every block this context handed out
+    std::unordered_map<size_t, std::vector<void *>> free_blocks;
+    int64_t bytes = 0;
+};
+thread_local AllocCtx t_ctx;
+
+// size class: at most 25 % above the request, 256 B granules
+size_t size_class(size_t bytes)
+{
+    size_t b = bytes < 256 ? 256 : bytes;
+    size_t top = (size_t)1 << (63 - __builtin_clzll((unsigned long long)b));
+    size_t step = std::max<size_t>(256, top / 4);
+    return (b + step - 1) / step * step;
+}
+}  // namespace
+
 bool dmalloc(void **p, size_t bytes, const char *what)
 {
     *p = nullptr;
-    hipError_t e = hipMalloc(p, bytes ? bytes : 16);
+    size_t want = bytes ? bytes : 16;
+    if (t_ctx.active) {
+        want = size_class(want);
+        auto it = t_ctx.free_blocks.find(want);
+        if (it != t_ctx.free_blocks.end() && !it->second.empty()) {
+            *p = it->second.back();
+            it->second.pop_back();
+            return true;
+        }
+    }
+    hipError_t e = hipMalloc(p, want);
     if (e != hipSuccess) {
         *p = nullptr;
         hipGetLastError();
@@ -41,13 +73,52 @@ bool dmalloc(void **p, size_t bytes, const char *what)
                   " failed: " + hipGetErrorString(e));
         return false;
     }
+    if (t_ctx.active) {
+        t_ctx.size_of[*p] = want;
+        t_ctx.bytes += (int64_t)want;
+    }
     return true;
 }
 
 void dfree(void *p)
 {
-    if (p) hipFree(p);
+    if (!p) return;
+    if (t_ctx.active) {
+        auto it = t_ctx.size_of.find(p);
+        if (it != t_ctx.size_of.end()) {
+            t_ctx.free_blocks[it->second].push_back(p);
+            return;
+        }
+    }
+    hipFree(p);
 }
+
+void ctx_begin(hipStream_t stream)
+{
+    ctx_end();
+    t_ctx.active = true;
+    t_ctx.stream = stream;
+}
+
+void ctx_end()
+{
+    if (!t_ctx.active) return;
+    if (t_ctx.stream) hipStreamSynchronize(t_ctx.stream);
+    for (auto &kv : t_ctx.free_blocks)
+        for (void *q : kv.second) {
+            hipFree(q);
+            t_ctx.size_of.erase(q);
+        }
+    // blocks still in use are freed by their owner's later dfree()
+    t_ctx.free_blocks.clear();
+    t_ctx.size_of.clear();
+    t_ctx.bytes = 0;
+    t_ctx.active = false;
+    t_ctx.stream = nullptr;
+}
+
+hipStream_t ctx_stream() { return t_ctx.active ? t_ctx.stream : nullptr; }
+int64_t ctx_cached_bytes() { return t_ctx.active ? t_ctx.bytes : 0; }
 
 int current_device()
 {
@@ -90,8 +161,18 @@ static std::mutex g_mu;
 static std::unordered_map<const dress_graph_t *, DevGraph *> g_table;
 static int g_precision = -1;
 
+static thread_local int t_precision = -1;
+
+void set_thread_precision(int precision) { t_precision = precision; }
+
+static thread_local const std::atomic<bool> *t_cancel = nullptr;
+
+void set_thread_cancel(const std::atomic<bool> *flag) { t_cancel = flag; }
+bool thread_cancelled() { return t_cancel && t_cancel->load(std::memory_order_relaxed); }
+
 int default_precision()
 {
+    if (t_precision >= 0) return t_precision;
     std::lock_guard<std::mutex> lk(g_mu);
     if (g_precision < 0) {
         const char *e = getenv("DRESS_HIP_PRECISION");

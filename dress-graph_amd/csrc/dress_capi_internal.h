@@ -2,6 +2,7 @@
 #pragma once
 
 #include <algorithm>
+#include <atomic>
 #include <cmath>
 
 #include "dress_hip_internal.h"
@@ -12,7 +13,15 @@ namespace dh {
 DevGraph *device_graph(p_dress_graph_t g);
 DevGraph *lookup(const dress_graph_t *g);
 int default_precision();
+// precision used by this thread's calls that take none (lifted fits);
+// -1 restores the process-wide setting
+void set_thread_precision(int precision);
+// cancellation flag watched by this thread's long host loops (the
+// per-subgraph lifted path); nullptr: none
+void set_thread_cancel(const std::atomic<bool> *flag);
+bool thread_cancelled();
 void record_stats(DevGraph *G, const FitResult &r);
 void record_lifted(int reruns, double device_ms, int64_t subgraphs, int64_t values);
+bool lds_eligible_size(int N, int E);
 
 }  // namespace dh

@@ -8,6 +8,7 @@
 #include <stdint.h>
 #include <stdlib.h>
 #include <string.h>
+#include <math.h>
 
 static uint64_t gen_next(uint64_t *s)
 {
@@ -111,4 +112,53 @@ int dress_gen_regular_switch(int n, int d, uint64_t seed, int swaps_per_edge,
             if (adj[(size_t)a * n + b]) { U[k] = a; V[k] = b; k++; }
     free(eu); free(ev); free(adj);
     return 0;
+}
+
+/* uniform double in [0, 1) */
+static double gen_unit(uint64_t *s) { return (double)(gen_next(s) >> 11) * (1.0 / 9007199254740992.0); }
+
+/*
+ * G(n, p): every pair {w, v}, w < v, is an edge independently with
+ * probability p, drawn by geometric skipping over the pairs in (v, w)
+ * order (Batagelj and Brandes, Phys. Rev. E 71, 036113, 2005), so the cost
+ * is O(n + m).  Edges are written as U = w < V = v, ordered by (v, w).
+ * W, if not NULL, receives float32 weights uniform in (0.1, 1] widened to
+ * double.  Returns the edge count, or -1 when more than cap edges were
+ * drawn (nothing beyond cap is written).
+ */
+int64_t dress_gen_gnp(int n, double p, uint64_t seed, int64_t cap, int *U, int *V, double *W)
+{
+    if (n < 0 || !(p >= 0.0)) return -1;
+    uint64_t s = seed;
+    int64_t m = 0;
+    if (n < 2 || p <= 0.0) return 0;
+    if (p >= 1.0) {
+        for (int v = 1; v < n; v++)
+            for (int w = 0; w < v; w++) {
+                if (m >= cap) return -1;
+                U[m] = w;
+                V[m] = v;
+                if (W) W[m] = (double)(float)(1.0 - 0.9 * gen_unit(&s));
+                m++;
+            }
+        return m;
+    }
+    const double lq = log(1.0 - p);
+    int64_t v = 1, w = -1;
+    while (v < n) {
+        double r = gen_unit(&s);
+        w += 1 + (int64_t)floor(log(1.0 - r) / lq);
+        while (w >= v && v < n) {
+            w -= v;
+            v++;
+        }
+        if (v < n) {
+            if (m >= cap) return -1;
+            U[m] = (int)w;
+            V[m] = (int)v;
+            if (W) W[m] = (double)(float)(1.0 - 0.9 * gen_unit(&s));
+            m++;
+        }
+    }
+    return m;
 }

@@ -61,6 +61,17 @@ inline bool dalloc(T **p, size_t n, const char *what) {
 }
 void dfree(void *p);
 
+// Execution context of the calling host thread (servers): while active,
+// graphs built on this thread run on `stream`, and device blocks freed by
+// dfree() are kept in a per-thread cache and handed out again by dmalloc()
+// instead of being returned with hipFree (which waits for the whole device
+// and would serialise concurrent requests).  Blocks are reused only by the
+// thread that freed them, i.e. in the order of its stream.
+void ctx_begin(hipStream_t stream);
+void ctx_end();                    // returns the cached blocks to the device
+hipStream_t ctx_stream();          // nullptr when no context is active
+int64_t ctx_cached_bytes();        // bytes held by the cache (free + in use)
+
 // ---------------------------------------------------------------- graph
 enum Precision { PREC_FP64 = 0, PREC_FP32_FP64 = 1 };
 
@@ -83,6 +94,7 @@ struct DevGraph {
     int max_degree = 0;
     bool weighted = false;
     hipStream_t stream = nullptr;
+    bool own_stream = true;        // false when the thread context lent it
 
     // Vertices carry internal labels: rank[v] is v's position in order of
     // decreasing degree (ties by id).  Rows list neighbours by rank, so a

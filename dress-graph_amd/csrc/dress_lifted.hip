@@ -714,10 +714,7 @@ namespace {
 std::mutex g_lift_mu;
 int g_lift_mode = 0;   // 0 batched when the parent fits LDS, 1 per-subgraph fits
 
-bool lds_eligible(const HostParent &P)
-{
-    return P.E <= 65535 && lifted_lds_bytes(P.E + 2, P.N + 2) <= 150 * 1024;
-}
+bool lds_eligible(const HostParent &P) { return lds_eligible_size(P.N, P.E); }
 
 // live edge count of each delta item by inclusion-exclusion: E minus the
 // edge records touching a deleted vertex, plus those joining two deleted
@@ -937,6 +934,10 @@ int64_t *lifted_flat(int kind, p_dress_graph_t g, int k, int iterations, double 
     } else {
         std::vector<double> vals;
         for (size_t i = 0; i < sel.size(); i++) {
+            if (thread_cancelled()) {
+                set_error("cancelled");
+                return nullptr;
+            }
             double *row = host_ms ? host_ms + (size_t)sel[i] * (size_t)E : nullptr;
             if (!plain_item(g, kind, &verts[i * (size_t)k], k, J.marks, iterations, epsilon, precision, vals, row))
                 return nullptr;
@@ -974,6 +975,12 @@ dress_hist_pair_t *to_pairs(int64_t *flat, int flat_len, int *hist_size)
 }
 
 }  // namespace
+
+// a graph of N vertices and E edge records runs in the LDS-resident kernel
+bool lds_eligible_size(int N, int E)
+{
+    return E <= 65535 && lifted_lds_bytes(E + 2, N + 2) <= 150 * 1024;
+}
 
 }  // namespace dh
 
@@ -1164,8 +1171,9 @@ int dress_hip_lifted_batch(int kind, int k, int n_graphs, const int *N, const in
         return -1;
     }
     auto t0 = std::chrono::steady_clock::now();
-    hipStream_t s = nullptr;
-    if (hipStreamCreateWithFlags(&s, hipStreamNonBlocking) != hipSuccess) {
+    hipStream_t s = ctx_stream();
+    const bool own_stream = s == nullptr;
+    if (own_stream && hipStreamCreateWithFlags(&s, hipStreamNonBlocking) != hipSuccess) {
         set_error("stream creation failed");
         return -1;
     }
@@ -1180,7 +1188,7 @@ int dress_hip_lifted_batch(int kind, int k, int n_graphs, const int *N, const in
         graph_of[(size_t)gi] = gi;
         if (!lds_eligible(parents[(size_t)gi])) {
             set_error("graph " + std::to_string(gi) + " is too large for the batched kernel");
-            (void)hipStreamDestroy(s);
+            if (own_stream) (void)hipStreamDestroy(s);
             return -1;
         }
         if (k == 0 || k >= N[gi]) {
@@ -1248,7 +1256,8 @@ int dress_hip_lifted_batch(int kind, int k, int n_graphs, const int *N, const in
     std::vector<uint64_t> hk, hc;
     if (ok) ok = qtable_download(qk, qc, qcap, s, hk, hc);
     dfree(qk); dfree(qc); dfree(lo.d_vals);
-    (void)hipStreamDestroy(s);
+    if (own_stream) (void)hipStreamDestroy(s);
+    else (void)hipStreamSynchronize(s);
     if (!ok) return -1;
     // assemble
     out->n_graphs = n_graphs;

@@ -1,0 +1,476 @@
+// dress_serve.hip -- the request server (continuous batching) in front of
+// the engine.
+//
+// Requests (fit, delta^k, nabla^k) are copied into a queue.  Each worker
+// thread owns a stream and a device-block cache (ctx_begin), so the work
+// of different requests overlaps on the device and no hipFree stalls it.
+// A worker takes the oldest request whose estimated footprint fits in the
+// memory budget beside the running ones (half of the workers look for a
+// fit first); a delta/nabla request that fits the LDS-resident kernel takes
+// every other waiting request of the same kind and parameters along, up to
+// max_batch and the budget, and they run as one batched launch
+// (dress_hip_lifted_batch).  Results are those of the single-graph entry
+// points, bit for bit.
+#include "dress_capi_internal.h"
+#include "dress_hip_internal.h"
+
+#include <atomic>
+#include <chrono>
+#include <condition_variable>
+#include <cstdio>
+#include <cstdlib>
+#include <cstring>
+#include <deque>
+#include <mutex>
+#include <thread>
+#include <vector>
+
+#include "dress/hip/dress_hip.h"
+
+struct dress_hip_server {
+    std::mutex mu;
+    std::condition_variable cv_work, cv_done;
+    std::deque<struct ServeReq *> queue;
+    std::deque<dress_hip_response_t> done;
+    std::vector<std::thread> workers;
+    int device = 0;
+    int fit_workers = 1;
+    int max_batch = 64;
+    bool stop = false;
+    std::atomic<bool> cancel{false};    // running requests give up at the next subgraph
+    int64_t next_id = 0;
+    dress_hip_server_stats_t st{};
+};
+
+struct ServeReq {
+    int64_t id;
+    dress_hip_request_t r;          // pointer fields unused; see U/V/W
+    std::vector<int> U, V;
+    std::vector<double> W;
+    bool weighted;
+    bool lds;                       // fits the batched kernel
+    int64_t bytes;                  // estimated device footprint
+    double arrival;
+};
+
+namespace {
+
+using namespace dh;
+
+double now_s()
+{
+    return std::chrono::duration<double>(std::chrono::steady_clock::now().time_since_epoch()).count();
+}
+
+int64_t count_items(int kind, int N, int k)
+{
+    if (kind == DRESS_HIP_REQ_FIT || k == 0) return 1;
+    if (k >= N) return 0;
+    return kind == DRESS_HIP_REQ_DELTA ? dress_delta_binom(N, k) : dress_nabla_perm_count(N, k);
+}
+
+// Device bytes a request may hold at its peak: graph build (half-edge keys,
+// both CSRs, intersection index of at most T_bound entries) and fit state;
+// lifted requests add their subgraph values and the histogram sort.
+int64_t footprint(const ServeReq &q, int64_t t_bound)
+{
+    const int64_t N = q.r.N, E = q.r.E;
+    int64_t fit = 400 * E + 64 * N + 8 * t_bound + (1 << 20);
+    if (q.r.kind == DRESS_HIP_REQ_FIT) return fit;
+    int64_t items = count_items(q.r.kind, q.r.N, q.r.k);
+    double vals = (double)items * (double)E;
+    double extra = q.lds ? 80.0 * vals : 48.0 * vals;
+    double b = (double)fit + extra;
+    return b > 9.0e18 ? (int64_t)9.0e18 : (int64_t)b;
+}
+
+// sum over edges of min(deg u, deg v): an upper bound of the index size
+int64_t intersection_bound(int N, int E, const int *U, const int *V)
+{
+    std::vector<int> deg((size_t)N + 1, 0);
+    for (int e = 0; e < E; e++) {
+        if (U[e] >= 0 && U[e] < N) deg[(size_t)U[e]]++;
+        if (V[e] >= 0 && V[e] < N && V[e] != U[e]) deg[(size_t)V[e]]++;
+    }
+    int64_t t = 0;
+    for (int e = 0; e < E; e++) {
+        int u = U[e], v = V[e];
+        if (u < 0 || u >= N || v < 0 || v >= N) continue;
+        t += std::min(deg[(size_t)u], deg[(size_t)v]);
+    }
+    return t;
+}
+
+bool same_batch(const ServeReq &a, const ServeReq &b)
+{
+    return a.r.kind == b.r.kind && a.r.k == b.r.k && a.r.variant == b.r.variant &&
+           a.r.max_iterations == b.r.max_iterations && a.r.epsilon == b.r.epsilon &&
+           a.r.precision == b.r.precision;
+}
+
+void fail(dress_hip_response_t &out, const std::string &msg)
+{
+    out.status = -1;
+    snprintf(out.error, sizeof(out.error), "%s", msg.c_str());
+}
+
+dress_hip_response_t blank_response(const ServeReq &q)
+{
+    dress_hip_response_t out;
+    memset(&out, 0, sizeof(out));
+    out.id = q.id;
+    out.kind = q.r.kind;
+    out.N = q.r.N;
+    out.E = q.r.E;
+    out.footprint = q.bytes;
+    out.arrival = q.arrival;
+    out.delta = NAN;
+    return out;
+}
+
+// dress_init_graph takes ownership of malloc'd arrays
+p_dress_graph_t make_graph(const ServeReq &q)
+{
+    size_t E = (size_t)q.r.E;
+    int *U = (int *)malloc((E + 1) * sizeof(int));
+    int *V = (int *)malloc((E + 1) * sizeof(int));
+    double *W = q.weighted ? (double *)malloc((E + 1) * sizeof(double)) : nullptr;
+    if (!U || !V || (q.weighted && !W)) {
+        free(U); free(V); free(W);
+        return nullptr;
+    }
+    if (E) {
+        memcpy(U, q.U.data(), E * sizeof(int));
+        memcpy(V, q.V.data(), E * sizeof(int));
+        if (W) memcpy(W, q.W.data(), E * sizeof(double));
+    }
+    p_dress_graph_t g = dress_init_graph(q.r.N, q.r.E, U, V, W, nullptr, q.r.variant, 0);
+    if (!g) { free(U); free(V); free(W); }
+    return g;
+}
+
+void run_fit(const ServeReq &q, dress_hip_response_t &out)
+{
+    p_dress_graph_t g = make_graph(q);
+    if (!g) return fail(out, std::string("graph build failed: ") + dress_hip_last_error());
+    int it = 0;
+    double dl = 0.0;
+    if (dress_fit_hip_ex(g, q.r.max_iterations, q.r.epsilon, q.r.precision, &it, &dl) != 0) {
+        fail(out, std::string("fit failed: ") + dress_hip_last_error());
+        dress_free_graph(g);
+        return;
+    }
+    out.iterations = it;
+    out.delta = dl;
+    out.edge_dress = (double *)malloc(((size_t)q.r.E + 1) * sizeof(double));
+    out.vertex_dress = (double *)malloc(((size_t)q.r.N + 1) * sizeof(double));
+    if (!out.edge_dress || !out.vertex_dress) {
+        fail(out, "host allocation of results failed");
+    } else {
+        if (q.r.E) memcpy(out.edge_dress, g->edge_dress, (size_t)q.r.E * sizeof(double));
+        if (q.r.N) memcpy(out.vertex_dress, g->vertex_dress, (size_t)q.r.N * sizeof(double));
+    }
+    dress_free_graph(g);
+}
+
+void run_lifted_one(const ServeReq &q, dress_hip_response_t &out)
+{
+    p_dress_graph_t g = make_graph(q);
+    if (!g) return fail(out, std::string("graph build failed: ") + dress_hip_last_error());
+    set_thread_precision(q.r.precision == DRESS_HIP_FP32_FP64 ? PREC_FP32_FP64 : PREC_FP64);
+    int hs = 0;
+    int64_t num = 0;
+    dress_hist_pair_t *h =
+        q.r.kind == DRESS_HIP_REQ_DELTA
+            ? dress_delta_fit_hip(g, q.r.k, q.r.max_iterations, q.r.epsilon, 0, 0, &hs, 0, nullptr, &num)
+            : dress_nabla_fit_hip(g, q.r.k, q.r.max_iterations, q.r.epsilon, 0, 0, &hs, 0, nullptr, &num);
+    set_thread_precision(-1);
+    const char *err = dress_hip_last_error();
+    if (!h && err && *err) fail(out, std::string("lifted fit failed: ") + err);
+    out.hist = h;
+    out.hist_size = hs;
+    out.subgraphs = num;
+    dress_free_graph(g);
+}
+
+void run_lifted_batch(const std::vector<ServeReq *> &qs, std::vector<dress_hip_response_t> &outs)
+{
+    const int n = (int)qs.size();
+    std::vector<int> N((size_t)n);
+    std::vector<int64_t> off((size_t)n + 1, 0);
+    bool any_w = false;
+    for (int i = 0; i < n; i++) {
+        N[(size_t)i] = qs[(size_t)i]->r.N;
+        off[(size_t)i + 1] = off[(size_t)i] + qs[(size_t)i]->r.E;
+        any_w = any_w || qs[(size_t)i]->weighted;
+    }
+    std::vector<int> U((size_t)off[(size_t)n] + 1), V((size_t)off[(size_t)n] + 1);
+    std::vector<double> W(any_w ? (size_t)off[(size_t)n] + 1 : 0);
+    for (int i = 0; i < n; i++) {
+        const ServeReq &q = *qs[(size_t)i];
+        std::copy(q.U.begin(), q.U.end(), U.begin() + off[(size_t)i]);
+        std::copy(q.V.begin(), q.V.end(), V.begin() + off[(size_t)i]);
+        if (any_w) {
+            if (q.weighted) std::copy(q.W.begin(), q.W.end(), W.begin() + off[(size_t)i]);
+            else std::fill(W.begin() + off[(size_t)i], W.begin() + off[(size_t)i + 1], 1.0);
+        }
+    }
+    const ServeReq &q0 = *qs[0];
+    dress_hip_lifted_batch_t r;
+    int kind = q0.r.kind == DRESS_HIP_REQ_DELTA ? 0 : 1;
+    int rc = dress_hip_lifted_batch(kind, q0.r.k, n, N.data(), off.data(), U.data(), V.data(),
+                                    any_w ? W.data() : nullptr, q0.r.variant, q0.r.max_iterations, q0.r.epsilon,
+                                    q0.r.precision, 0.0, &r);
+    if (rc != 0) {
+        std::string msg = std::string("batched lifted fit failed: ") + dress_hip_last_error();
+        for (auto &o : outs) fail(o, msg);
+        return;
+    }
+    for (int i = 0; i < n; i++) {
+        dress_hip_response_t &o = outs[(size_t)i];
+        int64_t b = r.hist_offset[i], e = r.hist_offset[i + 1];
+        o.subgraphs = count_items(qs[(size_t)i]->r.kind, N[(size_t)i], q0.r.k);
+        o.batch_size = n;
+        o.hist_size = (int)(e - b);
+        if (e > b) {
+            o.hist = (dress_hist_pair_t *)malloc((size_t)(e - b) * sizeof(dress_hist_pair_t));
+            if (!o.hist) { fail(o, "host allocation of results failed"); continue; }
+            for (int64_t j = b; j < e; j++) {
+                o.hist[j - b].value = r.hist_value[j];
+                o.hist[j - b].count = r.hist_count[j];
+            }
+        }
+    }
+    dress_hip_lifted_batch_free(&r);
+}
+
+// Under the lock: takes the next request(s) for a worker, or returns false.
+bool take(dress_hip_server *S, bool prefer_fit, std::vector<ServeReq *> &out, int64_t &bytes)
+{
+    if (S->queue.empty()) return false;
+    const int64_t room = S->st.budget_bytes - S->st.reserved_bytes;
+    const bool idle = S->st.running == 0;
+    auto admissible = [&](const ServeReq *q) { return q->bytes <= room || idle; };
+    auto pick = S->queue.end();
+    if (prefer_fit)
+        for (auto it = S->queue.begin(); it != S->queue.end(); ++it)
+            if ((*it)->r.kind == DRESS_HIP_REQ_FIT && admissible(*it)) { pick = it; break; }
+    if (pick == S->queue.end())
+        for (auto it = S->queue.begin(); it != S->queue.end(); ++it)
+            if (admissible(*it)) { pick = it; break; }
+    if (pick == S->queue.end()) return false;
+    ServeReq *first = *pick;
+    S->queue.erase(pick);
+    out.push_back(first);
+    bytes = first->bytes;
+    if (first->r.kind != DRESS_HIP_REQ_FIT && first->lds) {
+        for (auto it = S->queue.begin(); it != S->queue.end() && (int)out.size() < S->max_batch;) {
+            ServeReq *q = *it;
+            if (q->lds && same_batch(*q, *first) && bytes + q->bytes <= std::max(room, first->bytes)) {
+                out.push_back(q);
+                bytes += q->bytes;
+                it = S->queue.erase(it);
+            } else {
+                ++it;
+            }
+        }
+    }
+    S->st.reserved_bytes += bytes;
+    S->st.peak_reserved_bytes = std::max(S->st.peak_reserved_bytes, S->st.reserved_bytes);
+    S->st.running += (int64_t)out.size();
+    S->st.queued = (int64_t)S->queue.size();
+    return true;
+}
+
+void worker_main(dress_hip_server *S, int index)
+{
+    hipSetDevice(S->device);
+    hipStream_t stream = nullptr;
+    if (hipStreamCreateWithFlags(&stream, hipStreamNonBlocking) != hipSuccess) stream = nullptr;
+    if (stream) ctx_begin(stream);
+    set_thread_cancel(&S->cancel);
+    const bool prefer_fit = index < S->fit_workers;
+    for (;;) {
+        std::vector<ServeReq *> batch;
+        int64_t bytes = 0;
+        double admitted = 0.0;
+        {
+            std::unique_lock<std::mutex> lk(S->mu);
+            while (!S->stop && !take(S, prefer_fit, batch, bytes)) S->cv_work.wait(lk);
+            if (batch.empty()) break;                   // stopping
+            admitted = now_s();
+        }
+        std::vector<dress_hip_response_t> outs;
+        for (ServeReq *q : batch) {
+            outs.push_back(blank_response(*q));
+            outs.back().admitted = admitted;
+            outs.back().batch_size = (int)batch.size();
+        }
+        if (!stream) {
+            for (auto &o : outs) fail(o, "stream creation failed");
+        } else if (batch[0]->r.kind == DRESS_HIP_REQ_FIT) {
+            run_fit(*batch[0], outs[0]);
+        } else if (batch.size() == 1) {
+            run_lifted_one(*batch[0], outs[0]);
+        } else {
+            run_lifted_batch(batch, outs);
+        }
+        const double fin = now_s();
+        {
+            std::lock_guard<std::mutex> lk(S->mu);
+            S->st.reserved_bytes -= bytes;
+            S->st.running -= (int64_t)batch.size();
+            if (batch[0]->r.kind != DRESS_HIP_REQ_FIT) {
+                S->st.lifted_launches++;
+                S->st.lifted_requests += (int64_t)batch.size();
+            }
+            for (auto &o : outs) {
+                o.finished = fin;
+                if (o.status == 0) S->st.completed++;
+                else S->st.failed++;
+                S->done.push_back(o);
+            }
+        }
+        for (ServeReq *q : batch) delete q;
+        S->cv_done.notify_all();
+        S->cv_work.notify_all();
+    }
+    set_thread_cancel(nullptr);
+    if (stream) {
+        ctx_end();
+        hipStreamDestroy(stream);
+    }
+}
+
+}  // namespace
+
+extern "C" {
+
+double dress_hip_clock(void) { return now_s(); }
+
+dress_hip_server_t *dress_hip_server_create(int workers, int64_t budget_bytes, int max_batch)
+{
+    clear_error();
+    if (workers < 1 || workers > 64) {
+        set_error("dress_hip_server_create: workers must be in 1..64");
+        return nullptr;
+    }
+    int ndev = 0;
+    if (hipGetDeviceCount(&ndev) != hipSuccess || ndev < 1) {
+        hipGetLastError();
+        set_error("dress_hip_server_create: no GPU");
+        return nullptr;
+    }
+    auto *S = new dress_hip_server();
+    hipGetDevice(&S->device);
+    if (budget_bytes <= 0) {
+        size_t fr = 0, tot = 0;
+        if (hipMemGetInfo(&fr, &tot) != hipSuccess) {
+            hipGetLastError();
+            fr = (size_t)8 << 30;
+        }
+        budget_bytes = (int64_t)(fr / 2);
+    }
+    S->st.budget_bytes = budget_bytes;
+    S->max_batch = max_batch < 1 ? 1 : max_batch;
+    S->fit_workers = std::max(1, workers / 2);
+    for (int i = 0; i < workers; i++) S->workers.emplace_back(worker_main, S, i);
+    return S;
+}
+
+int64_t dress_hip_server_submit(dress_hip_server_t *S, const dress_hip_request_t *req)
+{
+    clear_error();
+    if (!S || !req) { set_error("dress_hip_server_submit: NULL argument"); return -1; }
+    if (req->N < 0 || req->E < 0 || (req->E > 0 && (!req->U || !req->V)) ||
+        (req->kind != DRESS_HIP_REQ_FIT && req->kind != DRESS_HIP_REQ_DELTA && req->kind != DRESS_HIP_REQ_NABLA) ||
+        (req->kind != DRESS_HIP_REQ_FIT && req->k < 0) || (int)req->variant < 0 || (int)req->variant > 3) {
+        set_error("dress_hip_server_submit: bad request");
+        return -1;
+    }
+    for (int e = 0; e < req->E; e++)
+        if (req->U[e] < 0 || req->U[e] >= req->N || req->V[e] < 0 || req->V[e] >= req->N) {
+            set_error("dress_hip_server_submit: vertex id out of range");
+            return -1;
+        }
+    auto *q = new ServeReq();
+    q->r = *req;
+    q->r.U = q->r.V = nullptr;
+    q->r.W = nullptr;
+    q->U.assign(req->U, req->U + req->E);
+    q->V.assign(req->V, req->V + req->E);
+    q->weighted = req->W != nullptr;
+    if (q->weighted) q->W.assign(req->W, req->W + req->E);
+    q->arrival = req->arrival >= 0 ? req->arrival : now_s();
+    q->lds = req->kind != DRESS_HIP_REQ_FIT && lds_eligible_size(req->N, req->E);
+    q->bytes = footprint(*q, intersection_bound(req->N, req->E, req->U, req->V));
+    std::lock_guard<std::mutex> lk(S->mu);
+    q->id = S->next_id++;
+    S->st.submitted++;
+    if (q->bytes > S->st.budget_bytes) {
+        dress_hip_response_t o = blank_response(*q);
+        o.admitted = o.finished = now_s();
+        fail(o, "request footprint " + std::to_string(q->bytes) + " B exceeds the server budget of " +
+                    std::to_string(S->st.budget_bytes) + " B");
+        S->st.rejected++;
+        S->done.push_back(o);
+        int64_t id = q->id;
+        delete q;
+        S->cv_done.notify_all();
+        return id;
+    }
+    S->queue.push_back(q);
+    S->st.queued = (int64_t)S->queue.size();
+    S->st.max_queued = std::max(S->st.max_queued, S->st.queued);
+    S->cv_work.notify_one();
+    return q->id;
+}
+
+int dress_hip_server_next(dress_hip_server_t *S, double timeout_s, dress_hip_response_t *out)
+{
+    if (!S || !out) return 0;
+    std::unique_lock<std::mutex> lk(S->mu);
+    auto until = std::chrono::steady_clock::now() +
+                 std::chrono::microseconds((int64_t)(std::max(timeout_s, 0.0) * 1e6));
+    while (S->done.empty())
+        if (S->cv_done.wait_until(lk, until) == std::cv_status::timeout && S->done.empty()) return 0;
+    *out = S->done.front();
+    S->done.pop_front();
+    return 1;
+}
+
+void dress_hip_response_free(dress_hip_response_t *r)
+{
+    if (!r) return;
+    free(r->edge_dress);
+    free(r->vertex_dress);
+    free(r->hist);
+    r->edge_dress = r->vertex_dress = nullptr;
+    r->hist = nullptr;
+}
+
+void dress_hip_server_stats(dress_hip_server_t *S, dress_hip_server_stats_t *out)
+{
+    if (!S || !out) return;
+    std::lock_guard<std::mutex> lk(S->mu);
+    *out = S->st;
+}
+
+void dress_hip_server_destroy(dress_hip_server_t *S)
+{
+    if (!S) return;
+    {
+        std::lock_guard<std::mutex> lk(S->mu);
+        S->stop = true;
+        S->cancel = true;
+        for (ServeReq *q : S->queue) delete q;
+        S->queue.clear();
+        S->st.queued = 0;
+    }
+    S->cv_work.notify_all();
+    for (auto &t : S->workers) t.join();
+    for (auto &o : S->done) dress_hip_response_free(&o);
+    delete S;
+}
+
+}  // extern "C"

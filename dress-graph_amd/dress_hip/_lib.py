@@ -66,6 +66,30 @@ class LiftedBatch(ctypes.Structure):
                 ("total_ms", ctypes.c_double)]
 
 
+class ServeRequest(ctypes.Structure):
+    _fields_ = [("kind", ctypes.c_int), ("k", ctypes.c_int), ("N", ctypes.c_int), ("E", ctypes.c_int),
+                ("U", ctypes.POINTER(ctypes.c_int)), ("V", ctypes.POINTER(ctypes.c_int)),
+                ("W", ctypes.POINTER(ctypes.c_double)), ("variant", ctypes.c_int),
+                ("max_iterations", ctypes.c_int), ("epsilon", ctypes.c_double),
+                ("precision", ctypes.c_int), ("arrival", ctypes.c_double)]
+
+
+class ServeResponse(ctypes.Structure):
+    _fields_ = [("id", ctypes.c_int64), ("status", ctypes.c_int), ("kind", ctypes.c_int),
+                ("N", ctypes.c_int), ("E", ctypes.c_int), ("iterations", ctypes.c_int),
+                ("delta", ctypes.c_double), ("edge_dress", ctypes.POINTER(ctypes.c_double)),
+                ("vertex_dress", ctypes.POINTER(ctypes.c_double)), ("hist", ctypes.POINTER(HistPair)),
+                ("hist_size", ctypes.c_int), ("subgraphs", ctypes.c_int64), ("batch_size", ctypes.c_int),
+                ("footprint", ctypes.c_int64), ("arrival", ctypes.c_double), ("admitted", ctypes.c_double),
+                ("finished", ctypes.c_double), ("error", ctypes.c_char * 192)]
+
+
+class ServeStats(ctypes.Structure):
+    _fields_ = [(n, ctypes.c_int64) for n in (
+        "submitted", "completed", "failed", "rejected", "queued", "running", "max_queued",
+        "budget_bytes", "reserved_bytes", "peak_reserved_bytes", "lifted_launches", "lifted_requests")]
+
+
 PG = ctypes.POINTER(DressGraph)
 
 _lib = None
@@ -139,6 +163,20 @@ def _bind(L):
     L.dress_hip_get_batch.restype = ctypes.c_int
     L.dress_hip_get_batch.argtypes = [PG, ctypes.c_int, c_int_p, c_int_p, c_dbl_p, ctypes.c_int,
                                       ctypes.c_double, c_dbl_p]
+    L.dress_hip_clock.restype = ctypes.c_double
+    L.dress_hip_clock.argtypes = []
+    L.dress_hip_server_create.restype = ctypes.c_void_p
+    L.dress_hip_server_create.argtypes = [ctypes.c_int, ctypes.c_int64, ctypes.c_int]
+    L.dress_hip_server_submit.restype = ctypes.c_int64
+    L.dress_hip_server_submit.argtypes = [ctypes.c_void_p, ctypes.POINTER(ServeRequest)]
+    L.dress_hip_server_next.restype = ctypes.c_int
+    L.dress_hip_server_next.argtypes = [ctypes.c_void_p, ctypes.c_double, ctypes.POINTER(ServeResponse)]
+    L.dress_hip_response_free.restype = None
+    L.dress_hip_response_free.argtypes = [ctypes.POINTER(ServeResponse)]
+    L.dress_hip_server_stats.restype = None
+    L.dress_hip_server_stats.argtypes = [ctypes.c_void_p, ctypes.POINTER(ServeStats)]
+    L.dress_hip_server_destroy.restype = None
+    L.dress_hip_server_destroy.argtypes = [ctypes.c_void_p]
     strided = _lifted_args() + [ctypes.c_int, ctypes.c_int]
     for name, args, res in (
             ("dress_delta_fit_hip", _lifted_args(), ctypes.POINTER(HistPair)),

@@ -19,6 +19,7 @@ Workloads (sizes from the DRESS benchmark configurations):
 * ``gnp``                 G(n, p) requests of the serving stream.
 """
 import ctypes
+import math
 import os
 
 import numpy as np
@@ -41,6 +42,9 @@ def _genlib():
         lib.dress_gen_regular_switch.restype = ctypes.c_int
         lib.dress_gen_regular_switch.argtypes = [ctypes.c_int, ctypes.c_int, ctypes.c_uint64,
                                                  ctypes.c_int, i32p, i32p]
+        lib.dress_gen_gnp.restype = ctypes.c_int64
+        lib.dress_gen_gnp.argtypes = [ctypes.c_int, ctypes.c_double, ctypes.c_uint64, ctypes.c_int64,
+                                      i32p, i32p, ctypes.POINTER(ctypes.c_double)]
         _gen = lib
     return _gen
 
@@ -142,41 +146,20 @@ def delta1_batch_graphs(count=512, d=6, n_lo=16, n_hi=128):
 
 
 def gnp(n, mean_deg, seed, weighted=False):
-    """G(n, p) with p = mean_deg / (n - 1): the edge count is binomial and the
-    edge set a uniform sample of that size.  Weighted requests carry float32
-    weights uniform in (0.1, 1] (passed on as float64)."""
-    rng = np.random.default_rng(seed)
-    pairs = n * (n - 1) // 2
+    """G(n, p) with p = mean_deg / (n - 1), drawn in C by geometric skipping
+    over the vertex pairs (dress_gen_gnp).  Weighted requests carry float32
+    weights uniform in (0.1, 1], passed on as float64."""
     p = min(1.0, mean_deg / max(n - 1, 1))
-    m = int(rng.binomial(pairs, p))
-    if pairs <= 4 * m or pairs < 200_000:
-        idx = np.sort(rng.choice(pairs, m, replace=False))
-    else:
-        idx = np.empty(0, dtype=np.int64)
-        while idx.size < m:
-            idx = np.unique(np.concatenate([idx, rng.integers(0, pairs, int((m - idx.size) * 1.1) + 16)]))
-        if idx.size > m:
-            idx = np.sort(rng.choice(idx, m, replace=False))
-    # unrank idx -> (i, j), i < j, in triu order
-    i = (n - 2 - np.floor(np.sqrt(-8.0 * idx + 4.0 * n * (n - 1) - 7) / 2.0 - 0.5)).astype(np.int64)
-    start = i * (2 * n - i - 1) // 2
-    # guard against rounding at row boundaries
-    fix_lo = idx < start
-    while fix_lo.any():
-        i[fix_lo] -= 1
-        start = i * (2 * n - i - 1) // 2
-        fix_lo = idx < start
-    nxt = (i + 1) * (2 * n - i - 2) // 2
-    fix_hi = idx >= nxt
-    while fix_hi.any():
-        i[fix_hi] += 1
-        start = i * (2 * n - i - 1) // 2
-        nxt = (i + 1) * (2 * n - i - 2) // 2
-        fix_hi = idx >= nxt
-    j = idx - start + i + 1
-    U = i.astype(np.int32)
-    V = j.astype(np.int32)
-    W = None
-    if weighted:
-        W = (1.0 - 0.9 * rng.random(m, dtype=np.float32)).astype(np.float64)
-    return n, U, V, W
+    pairs = n * (n - 1) // 2
+    mu = pairs * p
+    cap = int(mu + 12.0 * math.sqrt(mu + 1.0)) + 64
+    L = _genlib()
+    while True:
+        U = np.empty(cap, dtype=np.int32)
+        V = np.empty(cap, dtype=np.int32)
+        W = np.empty(cap, dtype=np.float64) if weighted else None
+        m = L.dress_gen_gnp(int(n), float(p), int(seed) & ((1 << 64) - 1), cap, _i32p(U), _i32p(V),
+                            W.ctypes.data_as(ctypes.POINTER(ctypes.c_double)) if weighted else None)
+        if m >= 0:
+            return n, U[:m].copy(), V[:m].copy(), (W[:m].copy() if weighted else None)
+        cap *= 2

@@ -244,6 +244,86 @@ int dress_hip_lifted_batch(int kind, int k, int n_graphs, const int *N, const in
                            double qtol, dress_hip_lifted_batch_t *out);
 void dress_hip_lifted_batch_free(dress_hip_lifted_batch_t *r);
 
+/* ---- serving (no counterpart in libdress) ---------------------------
+ * A request server for streams of fit / delta^k / nabla^k requests of mixed
+ * sizes.  Worker threads, each with its own stream, take requests as soon
+ * as their estimated device-memory footprint fits in the budget next to
+ * the requests already running, so work of many requests overlaps on the
+ * device; delta/nabla requests small enough for the LDS-resident kernel are
+ * packed with the others of their kind waiting at that moment into one
+ * batched launch.  Half of the workers take fits first, so short requests
+ * keep flowing behind long ones.  A request whose footprint exceeds the
+ * whole budget is answered at once with an error.  Results are those of
+ * the corresponding single-graph calls (dress_fit_hip_ex,
+ * dress_delta_fit_hip, dress_nabla_fit_hip), bit for bit. */
+typedef struct dress_hip_server dress_hip_server_t;
+
+typedef enum {
+    DRESS_HIP_REQ_FIT   = 0,
+    DRESS_HIP_REQ_DELTA = 1,
+    DRESS_HIP_REQ_NABLA = 2
+} dress_hip_request_kind_t;
+
+typedef struct {
+    int kind;                      /* dress_hip_request_kind_t             */
+    int k;                         /* delta/nabla order                    */
+    int N, E;
+    const int *U, *V;              /* copied by dress_hip_server_submit    */
+    const double *W;               /* NULL: unweighted                     */
+    dress_variant_t variant;
+    int max_iterations;
+    double epsilon;
+    dress_hip_precision_t precision;
+    double arrival;                /* dress_hip_clock() seconds; < 0: now   */
+} dress_hip_request_t;
+
+typedef struct {
+    int64_t id;
+    int status;                    /* 0 done, -1 failed or rejected        */
+    int kind, N, E;
+    int iterations;                /* fits                                 */
+    double delta;
+    double *edge_dress;            /* fits: E values, malloc'd             */
+    double *vertex_dress;          /* fits: N values, malloc'd             */
+    dress_hist_pair_t *hist;       /* delta/nabla: malloc'd                */
+    int hist_size;
+    int64_t subgraphs;
+    int batch_size;                /* requests sharing the launch          */
+    int64_t footprint;             /* estimated device bytes               */
+    double arrival, admitted, finished;   /* dress_hip_clock() seconds     */
+    char error[192];
+} dress_hip_response_t;
+
+typedef struct {
+    int64_t submitted, completed, failed, rejected;
+    int64_t queued, running, max_queued;
+    int64_t budget_bytes, reserved_bytes, peak_reserved_bytes;
+    int64_t lifted_launches, lifted_requests;  /* batched launches, requests in them */
+} dress_hip_server_stats_t;
+
+/* Monotonic clock in seconds, the time base of arrival/admitted/finished. */
+double dress_hip_clock(void);
+
+/* workers >= 1 threads on the current device; budget_bytes <= 0 takes half
+ * of the device's free memory; max_batch bounds the requests packed into
+ * one launch.  NULL on error. */
+dress_hip_server_t *dress_hip_server_create(int workers, int64_t budget_bytes, int max_batch);
+
+/* Copies the request and queues it.  Returns its id (>= 0), or -1. */
+int64_t dress_hip_server_submit(dress_hip_server_t *s, const dress_hip_request_t *req);
+
+/* Waits up to timeout_s for a finished request (any, in completion order).
+ * Returns 1 and fills *out (free with dress_hip_response_free), or 0 on
+ * timeout. */
+int dress_hip_server_next(dress_hip_server_t *s, double timeout_s, dress_hip_response_t *out);
+void dress_hip_response_free(dress_hip_response_t *r);
+void dress_hip_server_stats(dress_hip_server_t *s, dress_hip_server_stats_t *out);
+
+/* Drops queued requests, stops running delta/nabla requests at their next
+ * subgraph (they finish with status -1), waits for the rest, frees the
+ * server. */
+void dress_hip_server_destroy(dress_hip_server_t *s);
+
 #ifdef __cplusplus
 }
 #endif

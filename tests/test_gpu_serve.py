@@ -1,0 +1,112 @@
+"""The request server: results in a concurrent, batched mix equal the same
+requests run alone (through the server one at a time, and through the
+single-graph entry points), bit for bit; admission by footprint; a short
+synthetic stream end to end."""
+import numpy as np
+import pytest
+
+pytestmark = pytest.mark.gpu
+
+
+def _requests(gen):
+    """fits (FP64 and FP32->FP64, weighted or not, two variants), delta^1
+    small enough to be batched, one too large for the batched kernel, and a
+    nabla^1."""
+    out = []
+    rng = np.random.default_rng(11)
+    for i in range(14):
+        n = int(rng.integers(20, 3000))
+        _, U, V, W = gen.gnp(n, float(rng.uniform(3, 20)), 500 + i, weighted=bool(i % 2))
+        out.append(dict(kind=0, n=n, U=U, V=V, W=W, variant=int(i % 3 == 2), precision=int(i % 4 == 3)))
+    for i in range(9):
+        n = int(rng.integers(16, 160))
+        _, U, V, W = gen.gnp(n, float(rng.uniform(3, 10)), 900 + i, weighted=bool(i % 3 == 0))
+        out.append(dict(kind=1, n=n, U=U, V=V, W=W, variant=0, precision=0))
+    _, U, V, _ = gen.gnp(150, 130.0, 77)                      # ~9.7k edges: per-subgraph path
+    out.append(dict(kind=1, n=150, U=U, V=V, W=None, variant=0, precision=0))
+    _, U, V, _ = gen.gnp(40, 6.0, 78)
+    out.append(dict(kind=2, n=40, U=U, V=V, W=None, variant=0, precision=0))
+    return out
+
+
+def _submit(srv, q):
+    return srv.submit(q["n"], q["U"], q["V"], q["W"], kind=q["kind"], k=1, variant=q["variant"],
+                      precision=q["precision"])
+
+
+def _collect(srv, count, timeout=120.0):
+    import time
+    got = {}
+    t_end = time.time() + timeout
+    while len(got) < count and time.time() < t_end:
+        r = srv.next(1.0)
+        if r is not None:
+            got[r.id] = r
+    assert len(got) == count, f"{len(got)} of {count} requests finished"
+    return got
+
+
+def _same(a, b):
+    assert a.status == 0 and b.status == 0, (a.error, b.error)
+    if a.kind == 0:
+        assert a.iterations == b.iterations and a.delta == b.delta
+        assert np.array_equal(a.edge_dress, b.edge_dress) and np.array_equal(a.vertex_dress, b.vertex_dress)
+    else:
+        assert a.subgraphs == b.subgraphs
+        assert np.array_equal(a.hist_values, b.hist_values) and np.array_equal(a.hist_counts, b.hist_counts)
+
+
+def test_mix_equals_alone(gpu, gen):
+    from dress_hip import serve
+    reqs = _requests(gen)
+    with serve.Server(workers=4, budget_bytes=0, max_batch=8) as srv:
+        ids = [_submit(srv, q) for q in reqs]
+        mix = _collect(srv, len(ids))
+        st = srv.stats()
+    assert st["completed"] == len(reqs) and st["failed"] == 0
+    assert st["lifted_requests"] > st["lifted_launches"]          # some launches were shared
+    assert max(mix[i].batch_size for i in ids) > 1
+    with serve.Server(workers=1, budget_bytes=0, max_batch=1) as srv:
+        for q, i in zip(reqs, ids):
+            j = _submit(srv, q)
+            alone = _collect(srv, 1)[j]
+            _same(mix[i], alone)
+            assert alone.batch_size == 1
+    # and the single-graph entry points
+    for q, i in zip(reqs, ids):
+        r = mix[i]
+        with gpu.Graph(q["n"], q["U"], q["V"], q["W"], None, q["variant"]) as g:
+            if q["kind"] == 0:
+                it, dl = g.fit(100, 1e-6, q["precision"])
+                assert (it, dl) == (r.iterations, r.delta)
+                assert np.array_equal(g.edge_dress, r.edge_dress)
+                assert np.array_equal(g.vertex_dress, r.vertex_dress)
+            else:
+                vals, cnts, _, num = g.lifted("delta" if q["kind"] == 1 else "nabla", 1)
+                assert num == r.subgraphs
+                assert np.array_equal(vals, r.hist_values) and np.array_equal(cnts, r.hist_counts)
+
+
+def test_admission_by_footprint(gpu, gen):
+    from dress_hip import serve
+    _, U, V, _ = gen.gnp(2000, 10.0, 5)
+    with serve.Server(workers=2, budget_bytes=4 << 20, max_batch=4) as srv:
+        big = srv.submit(2000, U, V, kind=0)                     # ~8 MB estimated: over budget
+        _, u, v, _ = gen.gnp(30, 4.0, 6)
+        small = srv.submit(30, u, v, kind=0)
+        got = _collect(srv, 2)
+        st = srv.stats()
+    assert got[big].status == -1 and "budget" in got[big].error
+    assert got[small].status == 0
+    assert st["rejected"] == 1 and st["peak_reserved_bytes"] <= 4 << 20
+
+
+def test_short_stream(gpu):
+    from dress_hip import serve
+    rep = serve.run_stream(duration=2.0, rate=150.0, seed=1, workers=4, max_batch=16, drain=20.0,
+                           generators_threads=2)
+    assert rep["offered"]["requests"] > 200
+    assert rep["completed"]["fit"] > 0 and rep["completed"]["delta"] > 0
+    assert rep["failed"] == 0
+    lat = rep["latency"]["all"]
+    assert lat["n"] > 0 and 0 < lat["p50_ms"] <= lat["p99_ms"]
