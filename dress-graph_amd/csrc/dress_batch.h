@@ -25,6 +25,7 @@ struct LiftItem {
     int64_t vofs;                  // item vertices at item_verts[vofs .. vofs+k)
     int64_t val_ofs;               // live values at vals[val_ofs ...]
     int64_t ms_row;                // multiset row
+    int64_t vn_ofs;                // vertex norms at vnorm[vn_ofs ...] (whole-graph fits)
 };
 
 struct LiftArgs {
@@ -47,6 +48,7 @@ struct LiftArgs {
     double *vals;                  // packed live values (or null)
     double *ms;                    // multiset rows (or null)
     int64_t ms_stride;
+    double *vnorm;                 // per item: the norms of the last vertex phase (or null)
     int *iters;                    // per item (or null)
     double *deltas;                // per item (or null)
     int *flags;                    // per item: 1 = redo in FP64 (or null)
@@ -62,6 +64,8 @@ struct LiftArgs {
 constexpr unsigned long long QEMPTY = ~0ull;
 
 size_t lifted_lds_bytes(int ecap, int ncap);
+size_t lifted_lds_limit();
+size_t lifted_lds_per_cu();
 bool lifted_launch(const LiftArgs &a, int block, int grid, hipStream_t s);
 
 }  // namespace dh

@@ -288,6 +288,11 @@ __global__ void __launch_bounds__(B) k_lifted(LiftArgs a)
                 }
             }
         }
+        if (a.vnorm) {
+            // vertex_dress of dress.c:555-574: norms of the last vertex phase
+            for (int u = threadIdx.x; u < N; u += B)
+                a.vnorm[I.vn_ofs + u] = alive[u] ? vd[u] : __longlong_as_double(0x7ff8000000000000ll);
+        }
         if (threadIdx.x == 0) {
             if (a.iters) a.iters[it] = iters;
             if (a.deltas) a.deltas[it] = last;
@@ -304,9 +309,63 @@ size_t lifted_lds_bytes(int ecap, int ncap)
     return (size_t)(2 * ecap + ncap) * sizeof(double) + (size_t)ncap * sizeof(int) + 16;
 }
 
+// Largest dynamic LDS one k_lifted launch may take on the current device:
+// the per-workgroup maximum less the kernel's own static LDS (and a margin),
+// at most 150 KiB.  Launches above it are refused: the workgroups would
+// address LDS beyond their allocation.
+size_t lifted_lds_limit()
+{
+    static size_t cache[64] = {0};
+    int d = current_device();
+    if (d < 0 || d >= 64) d = 0;
+    if (!cache[d]) {
+        int maxblk = 0;
+        if (hipDeviceGetAttribute(&maxblk, hipDeviceAttributeMaxSharedMemoryPerBlock, d) != hipSuccess || maxblk <= 0) {
+            hipGetLastError();
+            maxblk = 64 * 1024;
+        }
+        size_t stat = 0;
+        const void *fns[4] = {reinterpret_cast<const void *>(&k_lifted<64>), reinterpret_cast<const void *>(&k_lifted<128>),
+                              reinterpret_cast<const void *>(&k_lifted<256>), reinterpret_cast<const void *>(&k_lifted<512>)};
+        for (const void *f : fns) {
+            hipFuncAttributes fa;
+            if (hipFuncGetAttributes(&fa, f) == hipSuccess) stat = std::max(stat, (size_t)fa.sharedSizeBytes);
+            else {
+                hipGetLastError();
+                stat = std::max<size_t>(stat, 4096);
+            }
+        }
+        size_t lim = (size_t)maxblk > stat + 512 ? (size_t)maxblk - stat - 512 : 0;
+        cache[d] = std::min<size_t>(lim, 150 * 1024);
+    }
+    return cache[d];
+}
+
+// LDS of one compute unit (bounds how many launch groups share a CU)
+size_t lifted_lds_per_cu()
+{
+    static size_t cache[64] = {0};
+    int d = current_device();
+    if (d < 0 || d >= 64) d = 0;
+    if (!cache[d]) {
+        int v = 0;
+        if (hipDeviceGetAttribute(&v, hipDeviceAttributeMaxSharedMemoryPerMultiprocessor, d) != hipSuccess || v <= 0) {
+            hipGetLastError();
+            v = 64 * 1024;
+        }
+        cache[d] = (size_t)v;
+    }
+    return cache[d];
+}
+
 bool lifted_launch(const LiftArgs &a, int block, int grid, hipStream_t s)
 {
     size_t lds = lifted_lds_bytes(a.ecap, a.ncap);
+    if (lds > lifted_lds_limit()) {
+        set_error("batched launch needs " + std::to_string(lds) + " B of LDS, above the device limit of " +
+                  std::to_string(lifted_lds_limit()) + " B");
+        return false;
+    }
     if (block == 64)
         hipLaunchKernelGGL(k_lifted<64>, dim3(grid), dim3(64), lds, s, a);
     else if (block == 128)

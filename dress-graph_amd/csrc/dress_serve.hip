@@ -4,13 +4,15 @@
 // Requests (fit, delta^k, nabla^k) are copied into a queue.  Each worker
 // thread owns a stream and a device-block cache (ctx_begin), so the work
 // of different requests overlaps on the device and no hipFree stalls it.
-// A worker takes the oldest request whose estimated footprint fits in the
-// memory budget beside the running ones (half of the workers look for a
-// fit first); a delta/nabla request that fits the LDS-resident kernel takes
+// A worker takes the oldest request of its lane (large fits; requests small
+// enough for the LDS-resident kernel; large delta/nabla) whose estimated
+// footprint fits in the memory budget beside the running ones; a request
+// that fits the LDS-resident kernel takes
 // every other waiting request of the same kind and parameters along, up to
 // max_batch and the budget, and they run as one batched launch
-// (dress_hip_lifted_batch).  Results are those of the single-graph entry
-// points, bit for bit.
+// (dress_hip_fit_batch, dress_hip_lifted_batch).  A result does not depend
+// on what ran beside it; FP64 results equal the single-graph entry points'
+// bit for bit.
 #include "dress_capi_internal.h"
 #include "dress_hip_internal.h"
 
@@ -34,7 +36,9 @@ struct dress_hip_server {
     std::deque<dress_hip_response_t> done;
     std::vector<std::thread> workers;
     int device = 0;
-    int fit_workers = 1;
+    int fit_workers = 1;            // lanes: [0, fit) fits, then batched, then large
+    int batched_workers = 1;
+    int n_workers = 1;
     int max_batch = 64;
     bool stop = false;
     std::atomic<bool> cancel{false};    // running requests give up at the next subgraph
@@ -103,7 +107,7 @@ int64_t intersection_bound(int N, int E, const int *U, const int *V)
 
 bool same_batch(const ServeReq &a, const ServeReq &b)
 {
-    return a.r.kind == b.r.kind && a.r.k == b.r.k && a.r.variant == b.r.variant &&
+    return a.r.kind == b.r.kind && (a.r.kind == DRESS_HIP_REQ_FIT || a.r.k == b.r.k) && a.r.variant == b.r.variant &&
            a.r.max_iterations == b.r.max_iterations && a.r.epsilon == b.r.epsilon &&
            a.r.precision == b.r.precision;
 }
@@ -244,26 +248,84 @@ void run_lifted_batch(const std::vector<ServeReq *> &qs, std::vector<dress_hip_r
     dress_hip_lifted_batch_free(&r);
 }
 
+// fits small enough for the LDS-resident kernel, many per launch
+void run_fit_batch(const std::vector<ServeReq *> &qs, std::vector<dress_hip_response_t> &outs)
+{
+    const int n = (int)qs.size();
+    std::vector<int> N((size_t)n);
+    std::vector<int64_t> off((size_t)n + 1, 0);
+    bool any_w = false;
+    for (int i = 0; i < n; i++) {
+        N[(size_t)i] = qs[(size_t)i]->r.N;
+        off[(size_t)i + 1] = off[(size_t)i] + qs[(size_t)i]->r.E;
+        any_w = any_w || qs[(size_t)i]->weighted;
+    }
+    std::vector<int> U((size_t)off[(size_t)n] + 1), V((size_t)off[(size_t)n] + 1);
+    std::vector<double> W(any_w ? (size_t)off[(size_t)n] + 1 : 0);
+    for (int i = 0; i < n; i++) {
+        const ServeReq &q = *qs[(size_t)i];
+        std::copy(q.U.begin(), q.U.end(), U.begin() + off[(size_t)i]);
+        std::copy(q.V.begin(), q.V.end(), V.begin() + off[(size_t)i]);
+        if (any_w) {
+            if (q.weighted) std::copy(q.W.begin(), q.W.end(), W.begin() + off[(size_t)i]);
+            else std::fill(W.begin() + off[(size_t)i], W.begin() + off[(size_t)i + 1], 1.0);
+        }
+    }
+    const ServeReq &q0 = *qs[0];
+    dress_hip_fit_batch_t r;
+    if (dress_hip_fit_batch(n, N.data(), off.data(), U.data(), V.data(), any_w ? W.data() : nullptr, q0.r.variant,
+                            q0.r.max_iterations, q0.r.epsilon, q0.r.precision, &r) != 0) {
+        std::string msg = std::string("batched fit failed: ") + dress_hip_last_error();
+        for (auto &o : outs) fail(o, msg);
+        return;
+    }
+    for (int i = 0; i < n; i++) {
+        dress_hip_response_t &o = outs[(size_t)i];
+        const int Eg = qs[(size_t)i]->r.E, Ng = N[(size_t)i];
+        o.iterations = r.iterations[i];
+        o.delta = r.delta[i];
+        o.edge_dress = (double *)malloc(((size_t)Eg + 1) * sizeof(double));
+        o.vertex_dress = (double *)malloc(((size_t)Ng + 1) * sizeof(double));
+        if (!o.edge_dress || !o.vertex_dress) { fail(o, "host allocation of results failed"); continue; }
+        if (Eg) memcpy(o.edge_dress, r.edge_dress + off[(size_t)i], (size_t)Eg * sizeof(double));
+        if (Ng) memcpy(o.vertex_dress, r.vertex_dress + r.vertex_offset[i], (size_t)Ng * sizeof(double));
+    }
+    dress_hip_fit_batch_free(&r);
+}
+
+// Worker lanes: fits too large for the batched kernel; requests (fits,
+// delta, nabla) for the batched kernel; delta/nabla requests too large for
+// it (a subgraph at a time: long).  A worker serves its
+// own lane first; when that is empty it helps with fits and batched
+// requests, but only the large lane takes large requests, so they never
+// hold every worker.
+enum Lane { LANE_FIT = 0, LANE_BATCHED = 1, LANE_LARGE = 2 };
+
+int lane_of(const ServeReq *q)
+{
+    if (q->lds) return LANE_BATCHED;
+    return q->r.kind == DRESS_HIP_REQ_FIT ? LANE_FIT : LANE_LARGE;
+}
+
 // Under the lock: takes the next request(s) for a worker, or returns false.
-bool take(dress_hip_server *S, bool prefer_fit, std::vector<ServeReq *> &out, int64_t &bytes)
+bool take(dress_hip_server *S, int lane, bool may_take_large, std::vector<ServeReq *> &out, int64_t &bytes)
 {
     if (S->queue.empty()) return false;
     const int64_t room = S->st.budget_bytes - S->st.reserved_bytes;
     const bool idle = S->st.running == 0;
     auto admissible = [&](const ServeReq *q) { return q->bytes <= room || idle; };
     auto pick = S->queue.end();
-    if (prefer_fit)
-        for (auto it = S->queue.begin(); it != S->queue.end(); ++it)
-            if ((*it)->r.kind == DRESS_HIP_REQ_FIT && admissible(*it)) { pick = it; break; }
+    for (auto it = S->queue.begin(); it != S->queue.end(); ++it)
+        if (lane_of(*it) == lane && admissible(*it)) { pick = it; break; }
     if (pick == S->queue.end())
         for (auto it = S->queue.begin(); it != S->queue.end(); ++it)
-            if (admissible(*it)) { pick = it; break; }
+            if ((may_take_large || lane_of(*it) != LANE_LARGE) && admissible(*it)) { pick = it; break; }
     if (pick == S->queue.end()) return false;
     ServeReq *first = *pick;
     S->queue.erase(pick);
     out.push_back(first);
     bytes = first->bytes;
-    if (first->r.kind != DRESS_HIP_REQ_FIT && first->lds) {
+    if (first->lds) {
         for (auto it = S->queue.begin(); it != S->queue.end() && (int)out.size() < S->max_batch;) {
             ServeReq *q = *it;
             if (q->lds && same_batch(*q, *first) && bytes + q->bytes <= std::max(room, first->bytes)) {
@@ -289,14 +351,18 @@ void worker_main(dress_hip_server *S, int index)
     if (hipStreamCreateWithFlags(&stream, hipStreamNonBlocking) != hipSuccess) stream = nullptr;
     if (stream) ctx_begin(stream);
     set_thread_cancel(&S->cancel);
-    const bool prefer_fit = index < S->fit_workers;
+    const int lane = index < S->fit_workers ? LANE_FIT
+                     : (index < S->fit_workers + S->batched_workers ? LANE_BATCHED : LANE_LARGE);
+    // without a large lane the last worker takes large requests as well
+    const bool may_take_large =
+        lane == LANE_LARGE || (S->fit_workers + S->batched_workers >= S->n_workers && index == S->n_workers - 1);
     for (;;) {
         std::vector<ServeReq *> batch;
         int64_t bytes = 0;
         double admitted = 0.0;
         {
             std::unique_lock<std::mutex> lk(S->mu);
-            while (!S->stop && !take(S, prefer_fit, batch, bytes)) S->cv_work.wait(lk);
+            while (!S->stop && !take(S, lane, may_take_large, batch, bytes)) S->cv_work.wait(lk);
             if (batch.empty()) break;                   // stopping
             admitted = now_s();
         }
@@ -309,7 +375,8 @@ void worker_main(dress_hip_server *S, int index)
         if (!stream) {
             for (auto &o : outs) fail(o, "stream creation failed");
         } else if (batch[0]->r.kind == DRESS_HIP_REQ_FIT) {
-            run_fit(*batch[0], outs[0]);
+            if (batch[0]->lds) run_fit_batch(batch, outs);
+            else run_fit(*batch[0], outs[0]);
         } else if (batch.size() == 1) {
             run_lifted_one(*batch[0], outs[0]);
         } else {
@@ -320,7 +387,7 @@ void worker_main(dress_hip_server *S, int index)
             std::lock_guard<std::mutex> lk(S->mu);
             S->st.reserved_bytes -= bytes;
             S->st.running -= (int64_t)batch.size();
-            if (batch[0]->r.kind != DRESS_HIP_REQ_FIT) {
+            if (batch[0]->lds) {
                 S->st.lifted_launches++;
                 S->st.lifted_requests += (int64_t)batch.size();
             }
@@ -373,7 +440,13 @@ dress_hip_server_t *dress_hip_server_create(int workers, int64_t budget_bytes, i
     }
     S->st.budget_bytes = budget_bytes;
     S->max_batch = max_batch < 1 ? 1 : max_batch;
+    // lanes: half fits, a quarter batched delta/nabla, the rest large ones
+    // (with fewer than three workers the large lane shares the last one)
     S->fit_workers = std::max(1, workers / 2);
+    S->batched_workers = std::max(workers >= 3 ? 1 : 0, workers / 4);
+    if (S->fit_workers + S->batched_workers >= workers && workers >= 2) S->fit_workers = workers - 1 - S->batched_workers;
+    if (S->fit_workers < 1) S->fit_workers = 1;
+    S->n_workers = workers;
     for (int i = 0; i < workers; i++) S->workers.emplace_back(worker_main, S, i);
     return S;
 }
@@ -402,7 +475,7 @@ int64_t dress_hip_server_submit(dress_hip_server_t *S, const dress_hip_request_t
     q->weighted = req->W != nullptr;
     if (q->weighted) q->W.assign(req->W, req->W + req->E);
     q->arrival = req->arrival >= 0 ? req->arrival : now_s();
-    q->lds = req->kind != DRESS_HIP_REQ_FIT && lds_eligible_size(req->N, req->E);
+    q->lds = lds_eligible_size(req->N, req->E) && (req->kind != DRESS_HIP_REQ_FIT || req->max_iterations > 0);
     q->bytes = footprint(*q, intersection_bound(req->N, req->E, req->U, req->V));
     std::lock_guard<std::mutex> lk(S->mu);
     q->id = S->next_id++;

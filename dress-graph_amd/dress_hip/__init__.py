@@ -179,6 +179,57 @@ def lifted_batch(graphs, k=1, kind="delta", variant=UNDIRECTED, max_iterations=1
         L.dress_hip_lifted_batch_free(ctypes.byref(r))
 
 
+@dataclass
+class FitBatchResult:
+    iterations: np.ndarray
+    delta: np.ndarray
+    edge_dress: List[np.ndarray]
+    vertex_dress: List[np.ndarray]
+    fp64_reruns: int
+    device_ms: float
+    total_ms: float
+
+
+def fit_batch_eligible(n_vertices, n_edges):
+    """True when a graph of this size runs in the batched (LDS) kernel."""
+    return bool(_lib.lib().dress_hip_fit_batch_eligible(int(n_vertices), int(n_edges)))
+
+
+def fit_batch(graphs, variant=UNDIRECTED, max_iterations=100, epsilon=1e-6, precision=FP64, weights=None):
+    """Whole-graph fits of many small graphs in one launch; ``graphs`` is a
+    list of ``(n, U, V)``.  Per graph the results equal a single fit."""
+    L = _lib.lib()
+    n = len(graphs)
+    Ns = np.array([g[0] for g in graphs], dtype=np.int32)
+    offs = np.zeros(n + 1, dtype=np.int64)
+    for i, g in enumerate(graphs):
+        offs[i + 1] = offs[i] + len(g[1])
+    U = np.ascontiguousarray(np.concatenate([np.asarray(g[1], dtype=np.int32) for g in graphs])
+                             if n else np.zeros(0, np.int32))
+    V = np.ascontiguousarray(np.concatenate([np.asarray(g[2], dtype=np.int32) for g in graphs])
+                             if n else np.zeros(0, np.int32))
+    W = None
+    if weights is not None:
+        W = np.ascontiguousarray(np.concatenate([np.asarray(w, dtype=np.float64) for w in weights]))
+    r = _lib.FitBatch()
+    rc = L.dress_hip_fit_batch(n, Ns.ctypes.data_as(c_int_p), offs.ctypes.data_as(_lib.c_i64_p),
+                               U.ctypes.data_as(c_int_p), V.ctypes.data_as(c_int_p),
+                               W.ctypes.data_as(c_dbl_p) if W is not None else c_dbl_p(),
+                               int(variant), int(max_iterations), float(epsilon), int(precision), ctypes.byref(r))
+    _lib.check(rc == 0, "dress_hip_fit_batch")
+    try:
+        totE, totN = int(offs[-1]), int(Ns.sum())
+        it = np.ctypeslib.as_array(r.iterations, shape=(max(n, 1),))[:n].copy()
+        dl = np.ctypeslib.as_array(r.delta, shape=(max(n, 1),))[:n].copy()
+        ed = np.ctypeslib.as_array(r.edge_dress, shape=(max(totE, 1),))[:totE].copy()
+        vo = np.ctypeslib.as_array(r.vertex_offset, shape=(n + 1,)).copy()
+        vd = np.ctypeslib.as_array(r.vertex_dress, shape=(max(totN, 1),))[:totN].copy()
+        return FitBatchResult(it, dl, [ed[offs[i]:offs[i + 1]] for i in range(n)],
+                              [vd[vo[i]:vo[i + 1]] for i in range(n)], r.fp64_reruns, r.device_ms, r.total_ms)
+    finally:
+        L.dress_hip_fit_batch_free(ctypes.byref(r))
+
+
 class Graph:
     """Owns one dress_graph_t built by libdress_hip.so (device-resident)."""
 

@@ -66,6 +66,14 @@ class LiftedBatch(ctypes.Structure):
                 ("total_ms", ctypes.c_double)]
 
 
+class FitBatch(ctypes.Structure):
+    _fields_ = [("n_graphs", ctypes.c_int), ("iterations", ctypes.POINTER(ctypes.c_int)),
+                ("delta", ctypes.POINTER(ctypes.c_double)), ("edge_dress", ctypes.POINTER(ctypes.c_double)),
+                ("vertex_offset", ctypes.POINTER(ctypes.c_int64)),
+                ("vertex_dress", ctypes.POINTER(ctypes.c_double)), ("fp64_reruns", ctypes.c_int),
+                ("device_ms", ctypes.c_double), ("total_ms", ctypes.c_double)]
+
+
 class ServeRequest(ctypes.Structure):
     _fields_ = [("kind", ctypes.c_int), ("k", ctypes.c_int), ("N", ctypes.c_int), ("E", ctypes.c_int),
                 ("U", ctypes.POINTER(ctypes.c_int)), ("V", ctypes.POINTER(ctypes.c_int)),
@@ -163,6 +171,13 @@ def _bind(L):
     L.dress_hip_get_batch.restype = ctypes.c_int
     L.dress_hip_get_batch.argtypes = [PG, ctypes.c_int, c_int_p, c_int_p, c_dbl_p, ctypes.c_int,
                                       ctypes.c_double, c_dbl_p]
+    L.dress_hip_fit_batch_eligible.restype = ctypes.c_int
+    L.dress_hip_fit_batch_eligible.argtypes = [ctypes.c_int, ctypes.c_int]
+    L.dress_hip_fit_batch.restype = ctypes.c_int
+    L.dress_hip_fit_batch.argtypes = [ctypes.c_int, c_int_p, c_i64_p, c_int_p, c_int_p, c_dbl_p, ctypes.c_int,
+                                      ctypes.c_int, ctypes.c_double, ctypes.c_int, ctypes.POINTER(FitBatch)]
+    L.dress_hip_fit_batch_free.restype = None
+    L.dress_hip_fit_batch_free.argtypes = [ctypes.POINTER(FitBatch)]
     L.dress_hip_clock.restype = ctypes.c_double
     L.dress_hip_clock.argtypes = []
     L.dress_hip_server_create.restype = ctypes.c_void_p

@@ -244,18 +244,52 @@ int dress_hip_lifted_batch(int kind, int k, int n_graphs, const int *N, const in
                            double qtol, dress_hip_lifted_batch_t *out);
 void dress_hip_lifted_batch_free(dress_hip_lifted_batch_t *r);
 
+/* Whole-graph fits of many small graphs in one persistent launch (no
+ * counterpart in libdress): graph g is vertices [0, N[g]) and edge records
+ * [edge_offset[g], edge_offset[g+1]) of U/V/W (W may be NULL), with vertex
+ * weights 1.0.  Each graph runs from d = 1 in one workgroup with its state
+ * in LDS, so it must satisfy dress_hip_fit_batch_eligible().  Per graph the
+ * FP64 results equal those of dress_init_graph + dress_fit_hip_ex on it, bit
+ * for bit (FP32->FP64: the same stopping step and every value within 1e-9
+ * of FP64, deterministic but rounded on its own in the FP32 phase):
+ * iterations and delta in the reference's convention, edge values in
+ * edge order (concatenated like the input), vertex_dress (the norms of the
+ * last vertex phase) concatenated at vertex_offset.  Arrays are malloc'd;
+ * release them with dress_hip_fit_batch_free.  Returns 0, or -1. */
+typedef struct {
+    int      n_graphs;
+    int     *iterations;          /* [n_graphs]                               */
+    double  *delta;               /* [n_graphs]                               */
+    double  *edge_dress;          /* [edge_offset[n_graphs]]                  */
+    int64_t *vertex_offset;       /* [n_graphs+1] into vertex_dress           */
+    double  *vertex_dress;
+    int      fp64_reruns;
+    double   device_ms;
+    double   total_ms;
+} dress_hip_fit_batch_t;
+int dress_hip_fit_batch_eligible(int N, int E);
+int dress_hip_fit_batch(int n_graphs, const int *N, const int64_t *edge_offset, const int *U, const int *V,
+                        const double *W, dress_variant_t variant, int max_iterations, double epsilon,
+                        dress_hip_precision_t precision, dress_hip_fit_batch_t *out);
+void dress_hip_fit_batch_free(dress_hip_fit_batch_t *r);
+
 /* ---- serving (no counterpart in libdress) ---------------------------
  * A request server for streams of fit / delta^k / nabla^k requests of mixed
  * sizes.  Worker threads, each with its own stream, take requests as soon
  * as their estimated device-memory footprint fits in the budget next to
  * the requests already running, so work of many requests overlaps on the
- * device; delta/nabla requests small enough for the LDS-resident kernel are
- * packed with the others of their kind waiting at that moment into one
- * batched launch.  Half of the workers take fits first, so short requests
- * keep flowing behind long ones.  A request whose footprint exceeds the
- * whole budget is answered at once with an error.  Results are those of
- * the corresponding single-graph calls (dress_fit_hip_ex,
- * dress_delta_fit_hip, dress_nabla_fit_hip), bit for bit. */
+ * device; requests whose graph fits the LDS-resident kernel (fits as whole
+ * graphs, delta/nabla as subgraphs) are packed with the others of their
+ * kind and parameters waiting at that moment into one batched launch
+ * (dress_hip_fit_batch, dress_hip_lifted_batch).  Workers are split into
+ * lanes -- half take larger fits first, a quarter batched requests, the
+ * rest the delta/nabla requests too large for the batched kernel, which no
+ * other lane takes -- so short requests keep flowing behind long ones.  A
+ * request whose footprint exceeds the whole budget is answered at once with
+ * an error.  A request's result does not depend on what runs beside it;
+ * FP64 results are those of the single-graph calls (dress_fit_hip_ex,
+ * dress_delta_fit_hip, dress_nabla_fit_hip) bit for bit, FP32->FP64 ones
+ * within 1e-9 of them. */
 typedef struct dress_hip_server dress_hip_server_t;
 
 typedef enum {
