@@ -13,9 +13,12 @@ The same line also carries
     solves/s of the batched kernel (FP64 and FP32->FP64) beside the plain
     FP64 GPU path (one device fit per subgraph, the CUDA backend's structure),
   * the roofline of the dominant kernel (HIP events around its launches),
+  * the serving stream: Poisson arrivals at 2000 requests/s (70 % fits, 30 %
+    delta^1, G(n,p) with n log-uniform in [20, 20000], mean degree U[3, 30],
+    half FP32-weighted) against the request server, with p50/p99 latency,
   * the CPU baseline: libdress compiled unchanged (OpenMP) on this host.
 
-    python bench.py [--gpus 1] [--steps 5] [--warmup 1] [--no-cpu-baseline]
+    python bench.py [--gpus 1] [--steps 5] [--warmup 1] [--no-cpu-baseline] [--serve-seconds 10]
 """
 import argparse
 import json
@@ -103,6 +106,8 @@ def main():
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-delta", action="store_true")
+    ap.add_argument("--serve-seconds", type=float, default=10.0,
+                    help="length of the serving stream (Poisson 2000 req/s); 0 skips it")
     ap.add_argument("--traffic-run", action="store_true",
                     help="one FP64 fit and 1 GiB copy-kernel passes only, for hardware-counter runs")
     args = ap.parse_args()
@@ -218,6 +223,14 @@ def main():
         res["speedup_fp64_batched_vs_plain_fp64_gpu"] = res["fp64"]["solves_per_s"] / plain
         res["workload"] = "delta^1 of 512 random 6-regular graphs, n ~ U{16..128}, seeds 0..511, eps 1e-6"
         out["delta1"] = res
+
+    if args.serve_seconds > 0:
+        from dress_hip import serve
+        rep = serve.run_stream(duration=args.serve_seconds, rate=2000.0, seed=0, workers=8, max_batch=64,
+                               drain=args.serve_seconds)
+        out["serving"] = rep
+        log(f"serving: {rep['completed']} of {rep['offered']['requests']} offered; "
+            f"p50/p99 {rep['latency']['all']['p50_ms']:.0f}/{rep['latency']['all']['p99_ms']:.0f} ms")
 
     if not args.no_cpu_baseline:
         cores = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))

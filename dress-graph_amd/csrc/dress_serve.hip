@@ -4,12 +4,12 @@
 // Requests (fit, delta^k, nabla^k) are copied into a queue.  Each worker
 // thread owns a stream and a device-block cache (ctx_begin), so the work
 // of different requests overlaps on the device and no hipFree stalls it.
-// A worker takes the oldest request of its lane (large fits; requests small
-// enough for the LDS-resident kernel; large delta/nabla) whose estimated
-// footprint fits in the memory budget beside the running ones; a request
-// that fits the LDS-resident kernel takes
+// A worker takes the oldest request of its lane (batched fits, device-wide
+// fits, batched delta/nabla, large delta/nabla) whose estimated footprint
+// fits in the memory budget beside the running ones; a request that fits
+// the LDS-resident kernel takes
 // every other waiting request of the same kind and parameters along, up to
-// max_batch and the budget, and they run as one batched launch
+// max_batch, a work cap and the budget, and they run as one batched launch
 // (dress_hip_fit_batch, dress_hip_lifted_batch).  A result does not depend
 // on what ran beside it; FP64 results equal the single-graph entry points'
 // bit for bit.
@@ -36,8 +36,6 @@ struct dress_hip_server {
     std::deque<dress_hip_response_t> done;
     std::vector<std::thread> workers;
     int device = 0;
-    int fit_workers = 1;            // lanes: [0, fit) fits, then batched, then large
-    int batched_workers = 1;
     int n_workers = 1;
     int max_batch = 64;
     bool stop = false;
@@ -54,6 +52,7 @@ struct ServeReq {
     bool weighted;
     bool lds;                       // fits the batched kernel
     int64_t bytes;                  // estimated device footprint
+    int64_t work;                   // edges (fit) or subgraph values (delta/nabla)
     double arrival;
 };
 
@@ -153,28 +152,37 @@ p_dress_graph_t make_graph(const ServeReq &q)
     return g;
 }
 
+// Device-wide fit of one request: the graph is built on the worker's
+// stream and fitted from d = 1 with vertex weights 1.0 -- the arithmetic of
+// dress_init_graph + dress_fit_hip_ex -- without filling a host
+// dress_graph_t (no CSR download, no uploads of initial values).
 void run_fit(const ServeReq &q, dress_hip_response_t &out)
 {
-    p_dress_graph_t g = make_graph(q);
-    if (!g) return fail(out, std::string("graph build failed: ") + dress_hip_last_error());
-    int it = 0;
-    double dl = 0.0;
-    if (dress_fit_hip_ex(g, q.r.max_iterations, q.r.epsilon, q.r.precision, &it, &dl) != 0) {
-        fail(out, std::string("fit failed: ") + dress_hip_last_error());
-        dress_free_graph(g);
-        return;
-    }
-    out.iterations = it;
-    out.delta = dl;
     out.edge_dress = (double *)malloc(((size_t)q.r.E + 1) * sizeof(double));
     out.vertex_dress = (double *)malloc(((size_t)q.r.N + 1) * sizeof(double));
-    if (!out.edge_dress || !out.vertex_dress) {
-        fail(out, "host allocation of results failed");
-    } else {
-        if (q.r.E) memcpy(out.edge_dress, g->edge_dress, (size_t)q.r.E * sizeof(double));
-        if (q.r.N) memcpy(out.vertex_dress, g->vertex_dress, (size_t)q.r.N * sizeof(double));
+    if (!out.edge_dress || !out.vertex_dress) return fail(out, "host allocation of results failed");
+    if (q.r.max_iterations <= 0) {
+        // dress.c:546-608 runs no iteration
+        for (int e = 0; e < q.r.E; e++) out.edge_dress[e] = 1.0;
+        for (int v = 0; v < q.r.N; v++) out.vertex_dress[v] = 0.0;
+        out.iterations = q.r.max_iterations;
+        out.delta = 0.0;
+        return;
     }
-    dress_free_graph(g);
+    clear_error();
+    DevGraph *G = new DevGraph();
+    FitResult r;
+    bool ok = build_graph(G, q.r.N, q.r.E, q.U.data(), q.V.data(), q.weighted ? q.W.data() : nullptr, nullptr,
+                          (int)q.r.variant);
+    if (ok) release_reference_csr(G);
+    ok = ok && build_bins(G) && fit_fill_values(G, 1.0) &&
+         fit_run(G, q.r.max_iterations, q.r.epsilon, q.r.precision == DRESS_HIP_FP32_FP64 ? PREC_FP32_FP64 : PREC_FP64,
+                 &r) &&
+         fit_download(G, out.edge_dress, out.vertex_dress);
+    free_graph(G);
+    if (!ok) return fail(out, std::string("fit failed: ") + dress_hip_last_error());
+    out.iterations = r.iterations;
+    out.delta = r.delta;
 }
 
 void run_lifted_one(const ServeReq &q, dress_hip_response_t &out)
@@ -293,19 +301,33 @@ void run_fit_batch(const std::vector<ServeReq *> &qs, std::vector<dress_hip_resp
     dress_hip_fit_batch_free(&r);
 }
 
-// Worker lanes: fits too large for the batched kernel; requests (fits,
-// delta, nabla) for the batched kernel; delta/nabla requests too large for
-// it (a subgraph at a time: long).  A worker serves its
-// own lane first; when that is empty it helps with fits and batched
-// requests, but only the large lane takes large requests, so they never
-// hold every worker.
-enum Lane { LANE_FIT = 0, LANE_BATCHED = 1, LANE_LARGE = 2 };
+// Worker lanes.  Batched fits, device-wide fits, batched delta/nabla and
+// the delta/nabla requests too large for the batched kernel (a subgraph at
+// a time: long) each have workers of their own, so short requests never
+// queue behind long launches of another kind.  A worker serves its own lane
+// first and, when that is empty, helps with any lane but the large one;
+// only large-lane workers take large requests.
+enum Lane { LANE_FIT_BATCHED = 0, LANE_FIT = 1, LANE_LIFT_BATCHED = 2, LANE_LIFT_LARGE = 3 };
 
 int lane_of(const ServeReq *q)
 {
-    if (q->lds) return LANE_BATCHED;
-    return q->r.kind == DRESS_HIP_REQ_FIT ? LANE_FIT : LANE_LARGE;
+    if (q->r.kind == DRESS_HIP_REQ_FIT) return q->lds ? LANE_FIT_BATCHED : LANE_FIT;
+    return q->lds ? LANE_LIFT_BATCHED : LANE_LIFT_LARGE;
 }
+
+// lane of worker i: one per lane first, then device-wide fits twice as
+// often as the other lanes
+int worker_lane(int i)
+{
+    static const int first[4] = {LANE_FIT_BATCHED, LANE_FIT, LANE_LIFT_BATCHED, LANE_LIFT_LARGE};
+    static const int cycle[4] = {LANE_FIT, LANE_FIT, LANE_LIFT_BATCHED, LANE_LIFT_LARGE};
+    return i < 4 ? first[i] : cycle[(i - 4) % 4];
+}
+
+// most work one batched launch takes on: edges of its graphs for fits,
+// subgraph values for delta/nabla (keeps each launch to tens of ms)
+constexpr int64_t FIT_BATCH_EDGES = 4 << 20;
+constexpr int64_t LIFT_BATCH_VALUES = 48 << 20;
 
 // Under the lock: takes the next request(s) for a worker, or returns false.
 bool take(dress_hip_server *S, int lane, bool may_take_large, std::vector<ServeReq *> &out, int64_t &bytes)
@@ -319,18 +341,22 @@ bool take(dress_hip_server *S, int lane, bool may_take_large, std::vector<ServeR
         if (lane_of(*it) == lane && admissible(*it)) { pick = it; break; }
     if (pick == S->queue.end())
         for (auto it = S->queue.begin(); it != S->queue.end(); ++it)
-            if ((may_take_large || lane_of(*it) != LANE_LARGE) && admissible(*it)) { pick = it; break; }
+            if ((may_take_large || lane_of(*it) != LANE_LIFT_LARGE) && admissible(*it)) { pick = it; break; }
     if (pick == S->queue.end()) return false;
     ServeReq *first = *pick;
     S->queue.erase(pick);
     out.push_back(first);
     bytes = first->bytes;
     if (first->lds) {
+        const int64_t cap = first->r.kind == DRESS_HIP_REQ_FIT ? FIT_BATCH_EDGES : LIFT_BATCH_VALUES;
+        int64_t work = first->work;
         for (auto it = S->queue.begin(); it != S->queue.end() && (int)out.size() < S->max_batch;) {
             ServeReq *q = *it;
-            if (q->lds && same_batch(*q, *first) && bytes + q->bytes <= std::max(room, first->bytes)) {
+            if (q->lds && same_batch(*q, *first) && bytes + q->bytes <= std::max(room, first->bytes) &&
+                work + q->work <= cap) {
                 out.push_back(q);
                 bytes += q->bytes;
+                work += q->work;
                 it = S->queue.erase(it);
             } else {
                 ++it;
@@ -351,11 +377,10 @@ void worker_main(dress_hip_server *S, int index)
     if (hipStreamCreateWithFlags(&stream, hipStreamNonBlocking) != hipSuccess) stream = nullptr;
     if (stream) ctx_begin(stream);
     set_thread_cancel(&S->cancel);
-    const int lane = index < S->fit_workers ? LANE_FIT
-                     : (index < S->fit_workers + S->batched_workers ? LANE_BATCHED : LANE_LARGE);
-    // without a large lane the last worker takes large requests as well
-    const bool may_take_large =
-        lane == LANE_LARGE || (S->fit_workers + S->batched_workers >= S->n_workers && index == S->n_workers - 1);
+    const int lane = worker_lane(index);
+    // without a large lane (fewer than four workers) the last worker takes
+    // large requests as well
+    const bool may_take_large = lane == LANE_LIFT_LARGE || (S->n_workers < 4 && index == S->n_workers - 1);
     for (;;) {
         std::vector<ServeReq *> batch;
         int64_t bytes = 0;
@@ -440,12 +465,6 @@ dress_hip_server_t *dress_hip_server_create(int workers, int64_t budget_bytes, i
     }
     S->st.budget_bytes = budget_bytes;
     S->max_batch = max_batch < 1 ? 1 : max_batch;
-    // lanes: half fits, a quarter batched delta/nabla, the rest large ones
-    // (with fewer than three workers the large lane shares the last one)
-    S->fit_workers = std::max(1, workers / 2);
-    S->batched_workers = std::max(workers >= 3 ? 1 : 0, workers / 4);
-    if (S->fit_workers + S->batched_workers >= workers && workers >= 2) S->fit_workers = workers - 1 - S->batched_workers;
-    if (S->fit_workers < 1) S->fit_workers = 1;
     S->n_workers = workers;
     for (int i = 0; i < workers; i++) S->workers.emplace_back(worker_main, S, i);
     return S;
@@ -477,6 +496,8 @@ int64_t dress_hip_server_submit(dress_hip_server_t *S, const dress_hip_request_t
     q->arrival = req->arrival >= 0 ? req->arrival : now_s();
     q->lds = lds_eligible_size(req->N, req->E) && (req->kind != DRESS_HIP_REQ_FIT || req->max_iterations > 0);
     q->bytes = footprint(*q, intersection_bound(req->N, req->E, req->U, req->V));
+    q->work = req->kind == DRESS_HIP_REQ_FIT ? (int64_t)req->E
+                                             : count_items(req->kind, req->N, req->k) * (int64_t)req->E;
     std::lock_guard<std::mutex> lk(S->mu);
     q->id = S->next_id++;
     S->st.submitted++;

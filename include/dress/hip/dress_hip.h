@@ -282,9 +282,9 @@ void dress_hip_fit_batch_free(dress_hip_fit_batch_t *r);
  * graphs, delta/nabla as subgraphs) are packed with the others of their
  * kind and parameters waiting at that moment into one batched launch
  * (dress_hip_fit_batch, dress_hip_lifted_batch).  Workers are split into
- * lanes -- half take larger fits first, a quarter batched requests, the
- * rest the delta/nabla requests too large for the batched kernel, which no
- * other lane takes -- so short requests keep flowing behind long ones.  A
+ * lanes -- batched fits, device-wide fits, batched delta/nabla, and the
+ * delta/nabla requests too large for the batched kernel, which no other
+ * lane takes -- so short requests keep flowing behind long ones.  A
  * request whose footprint exceeds the whole budget is answered at once with
  * an error.  A request's result does not depend on what runs beside it;
  * FP64 results are those of the single-graph calls (dress_fit_hip_ex,
