@@ -42,10 +42,14 @@ HBM_PEAK_GBS = 8000.0
 B_ENTRY, B_HALF, B_EDGE, B_VERTEX = 24, 12, 44, 20
 # HBM bytes per launch of the dominant kernel from hardware counters
 # (FETCH_SIZE and WRITE_SIZE, one rocprofv3 --pmc pass each, over
-# `python bench.py --traffic-run`), each scaled by the bytes a plain copy
-# kernel of known size moves per counted KiB.  Valid for the kernel and
-# graph it was measured on; None when not measured.
-TRAFFIC = None
+# `python bench.py --traffic-run`; profiles/fit_power_law_4M_fp64.md), each
+# scaled by the bytes a plain 1 GiB copy kernel moves per counted KiB.  Valid
+# for the kernel and graph it was measured on: reported only when that
+# kernel is the dominant one.
+TRAFFIC = {"kernel": "k_edge<64,packed,double> (edge bin 3)", "fetch_kib": 6242718, "write_kib": 78604,
+           "fetch_scale": 2.0, "write_scale": 1.0}
+TRAFFIC["bytes_per_launch"] = int((TRAFFIC["fetch_kib"] * TRAFFIC["fetch_scale"] +
+                                   TRAFFIC["write_kib"] * TRAFFIC["write_scale"]) * 1024)
 
 
 def log(*a):
@@ -180,7 +184,9 @@ def main():
                    "parallelism": "single GPU"},
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBS,
-                     "traffic": TRAFFIC["bytes_per_launch"] if TRAFFIC else None,
+                     "traffic": (TRAFFIC["bytes_per_launch"]
+                                 if TRAFFIC and TRAFFIC["kernel"] == f"k_edge<{group},packed,double> (edge bin {b})"
+                                 else None),
                      "kernel": f"k_edge<{group},packed,double> (edge bin {b})",
                      "launch_ms": float(ems[b]), "algorithmic_bytes": int(launch_bytes),
                      "iteration_algorithmic_GBs": iter_gbs, "iteration_frac": iter_gbs / HBM_PEAK_GBS,
