@@ -12,6 +12,7 @@ The same line also carries
   * delta^1 of the 512 random 6-regular graphs (n ~ U{16..128}): subgraph
     solves/s of the batched kernel (FP64 and FP32->FP64) beside the plain
     FP64 GPU path (one device fit per subgraph, the CUDA backend's structure),
+  * delta^2 and nabla^1 of a random 12-regular graph on 256 vertices,
   * the roofline of the dominant kernel (HIP events around its launches),
   * the serving stream: Poisson arrivals at 2000 requests/s (70 % fits, 30 %
     delta^1, G(n,p) with n log-uniform in [20, 20000], mean degree U[3, 30],
@@ -230,6 +231,30 @@ def main():
         res["workload"] = "delta^1 of 512 random 6-regular graphs, n ~ U{16..128}, seeds 0..511, eps 1e-6"
         out["delta1"] = res
 
+    if not args.no_delta:
+        # delta^2 and nabla^1 of the random 12-regular graph on 256 vertices
+        n4, U4, V4 = gen.regular_switch(256, 12, 7)
+        c4 = {"workload": "delta^2 (32,640 subgraphs of 254 vertices) and nabla^1 (256 marked graphs) of a "
+                          "random 12-regular graph, n=256 (switch chain, seed 7), eps 1e-6"}
+        with dh.Graph(n4, U4, V4) as g4:
+            g4.lifted("delta", 1)                                       # warm-up
+            for name, prec in (("delta2_fp64", dh.FP64), ("delta2_fp32_fp64", dh.FP32_FP64)):
+                dh.set_precision(prec)
+                try:
+                    t0 = time.perf_counter()
+                    _, cnts, _, num = g4.lifted("delta", 2)
+                    wall = time.perf_counter() - t0
+                    st = dh.last_lifted_stats()
+                finally:
+                    dh.set_precision(dh.FP64)
+                c4[name] = {"solves_per_s": num / wall, "subgraphs": int(num), "values": int(cnts.sum()),
+                            "wall_ms": wall * 1e3, "device_ms": st["device_ms"], "fp64_reruns": st["fp64_reruns"]}
+            t0 = time.perf_counter()
+            _, _, _, num = g4.lifted("nabla", 1)
+            wall = time.perf_counter() - t0
+            c4["nabla1_fp64"] = {"solves_per_s": num / wall, "subgraphs": int(num), "wall_ms": wall * 1e3}
+        out["config4"] = c4
+
     if args.serve_seconds > 0:
         from dress_hip import serve
         rep = serve.run_stream(duration=args.serve_seconds, rate=2000.0, seed=0, workers=8, max_batch=64,
@@ -248,6 +273,16 @@ def main():
             out["cpu_baseline"] = {"value": cpu_v, "unit": "edge-updates/s", "cores": cores, "kind": "reference",
                                    "sample": f"2 of the {iters_per_fit:.0f} iterations of the same fit, libdress "
                                              f"dress_fit_omp with its intercept index prebuilt ({dt:.1f} s)"}
+        if "config4" in out:
+            sys.path.insert(0, os.path.join(ROOT, "oracle"))
+            import pyoracle
+            lib4 = pyoracle.Lib("ref") if pyoracle.available("ref") else pyoracle.Lib("oracle")
+            t0 = time.perf_counter()
+            _, _, _, num = lib4.lifted("delta", n4, U4, V4, k=2, n_samples=1500, seed=1, use_omp=True)
+            dt = time.perf_counter() - t0
+            out["config4"]["cpu_baseline"] = {"solves_per_s": 1500 / dt, "cores": cores,
+                                              "kind": "reference" if lib4.which == "ref" else "port",
+                                              "sample": "1,500 sampled delta^2 subgraphs, dress_delta_fit_omp"}
         if "delta1" in out:
             v, kind = cpu_baseline_delta1(graphs[:64])
             out["delta1"]["cpu_baseline"] = {"solves_per_s": v, "cores": cores,

@@ -68,6 +68,7 @@ void dfree(void *p);
 // and would serialise concurrent requests).  Blocks are reused only by the
 // thread that freed them, i.e. in the order of its stream.
 void ctx_begin(hipStream_t stream);
+void *host_alloc(size_t bytes);    // malloc-compatible (free() releases it)
 void ctx_end();                    // returns the cached blocks to the device
 hipStream_t ctx_stream();          // nullptr when no context is active
 int64_t ctx_cached_bytes();        // bytes held by the cache (free + in use)

@@ -341,26 +341,58 @@ __global__ void k_fill_seg(const int64_t *seg_ofs, int nseg, int *seg)
         for (int64_t i = seg_ofs[g] + threadIdx.x; i < seg_ofs[g + 1]; i += blockDim.x) seg[i] = g;
 }
 
+// run lengths from run starts (ustart ascending, n values in all)
+__global__ void k_run_counts(const int64_t *ustart, int64_t runs, int64_t n, int64_t *cnt)
+{
+    for (int64_t j = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; j < runs; j += (int64_t)gridDim.x * blockDim.x)
+        cnt[j] = (j + 1 < runs ? ustart[j + 1] : n) - ustart[j];
+}
+
+// first run of every segment: ofs[g] for g in [0, nseg], runs ordered by
+// segment (segments without runs get the next run's index)
+__global__ void k_seg_run_starts(const int *useg, int64_t runs, int nseg, int64_t *ofs)
+{
+    for (int64_t j = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; j < runs; j += (int64_t)gridDim.x * blockDim.x) {
+        int prev = j ? useg[j - 1] : -1;
+        for (int g = prev + 1; g <= useg[j]; g++) ofs[g] = j;
+        if (j == runs - 1)
+            for (int g = useg[j] + 1; g <= nseg; g++) ofs[g] = runs;
+    }
+}
+
+// (value bits, count) pairs, the layout of dress_hist_pair_t and of the
+// flat histogram arrays
+__global__ void k_interleave(const double *v, const int64_t *c, int64_t runs, int64_t *flat)
+{
+    for (int64_t j = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; j < runs; j += (int64_t)gridDim.x * blockDim.x) {
+        flat[2 * j] = __double_as_longlong(v[j]);
+        flat[2 * j + 1] = c[j];
+    }
+}
+
 // Exact (value, count) histogram of vals[0..n) per segment (segments are
 // contiguous ranges, seg_ofs has nseg+1 entries): radix sort by value, then a
-// stable radix sort by segment, then run-length encoding.  Output ascending by
-// (segment, value); useg[j] is the segment of run j.
-bool exact_histogram(const double *vals, int64_t n, const std::vector<int64_t> &seg_ofs, hipStream_t s,
-                     std::vector<double> &uvals, std::vector<int64_t> &counts, std::vector<int> &useg)
+// stable radix sort by segment, then run-length encoding, all on the device.
+// Leaves d_val / d_cnt (runs entries, ascending by (segment, value)) on the
+// device for the caller to download and dfree; run_ofs[g] is the first run
+// of segment g (nseg+1 entries).
+bool exact_histogram_dev(const double *vals, int64_t n, const std::vector<int64_t> &seg_ofs, hipStream_t s,
+                         double **d_val, int64_t **d_cnt, int64_t *runs_out, std::vector<int64_t> &run_ofs)
 {
-    uvals.clear();
-    counts.clear();
-    useg.clear();
-    if (n <= 0) return true;
+    *d_val = nullptr;
+    *d_cnt = nullptr;
+    *runs_out = 0;
     const int nseg = (int)seg_ofs.size() - 1;
+    run_ofs.assign((size_t)nseg + 1, 0);
+    if (n <= 0) return true;
     uint64_t *k0 = nullptr, *k1 = nullptr;
     int *g0 = nullptr, *g1 = nullptr, *flag = nullptr, *usg = nullptr;
-    int64_t *pos = nullptr, *ustart = nullptr, *dofs = nullptr;
+    int64_t *pos = nullptr, *ustart = nullptr, *dofs = nullptr, *cnt = nullptr, *rofs = nullptr;
     double *uval = nullptr;
     bool ok = dalloc(&k0, (size_t)n, "histogram keys") && dalloc(&k1, (size_t)n, "histogram keys") &&
               dalloc(&g0, (size_t)n, "segments") && dalloc(&g1, (size_t)n, "segments") &&
               dalloc(&flag, (size_t)n + 1, "run flags") && dalloc(&pos, (size_t)n + 1, "run positions") &&
-              dalloc(&dofs, (size_t)nseg + 1, "segment offsets");
+              dalloc(&dofs, (size_t)nseg + 1, "segment offsets") && dalloc(&rofs, (size_t)nseg + 1, "run offsets");
     int grid = grid_for(n, num_cus() * 8);
     int64_t runs = 0;
     if (ok) {
@@ -390,29 +422,91 @@ bool exact_histogram(const double *vals, int64_t n, const std::vector<int64_t> &
         }
         if (ok)
             ok = dalloc(&uval, (size_t)runs + 1, "histogram values") &&
-                 dalloc(&ustart, (size_t)runs + 1, "histogram runs") && dalloc(&usg, (size_t)runs + 1, "histogram segs");
+                 dalloc(&ustart, (size_t)runs + 1, "histogram runs") && dalloc(&usg, (size_t)runs + 1, "histogram segs") &&
+                 dalloc(&cnt, (size_t)runs + 1, "histogram counts");
         if (ok) {
+            int rgrid = grid_for(runs, num_cus() * 8);
             hipLaunchKernelGGL(k_run_emit, dim3(grid), dim3(TPB), 0, s, k0, g0, flag, pos, n, uval, ustart, usg);
-            uvals.resize((size_t)runs);
-            counts.resize((size_t)runs + 1);
-            useg.resize((size_t)runs);
-            ok = hipMemcpyAsync(uvals.data(), uval, (size_t)runs * sizeof(double), hipMemcpyDeviceToHost, s) ==
-                     hipSuccess &&
-                 hipMemcpyAsync(counts.data(), ustart, (size_t)runs * sizeof(int64_t), hipMemcpyDeviceToHost, s) ==
-                     hipSuccess &&
-                 hipMemcpyAsync(useg.data(), usg, (size_t)runs * sizeof(int), hipMemcpyDeviceToHost, s) ==
-                     hipSuccess &&
+            hipLaunchKernelGGL(k_run_counts, dim3(rgrid), dim3(TPB), 0, s, ustart, runs, n, cnt);
+            hipLaunchKernelGGL(k_seg_run_starts, dim3(rgrid), dim3(TPB), 0, s, usg, runs, nseg, rofs);
+            ok = hipGetLastError() == hipSuccess &&
+                 hipMemcpyAsync(run_ofs.data(), rofs, ((size_t)nseg + 1) * sizeof(int64_t), hipMemcpyDeviceToHost,
+                                s) == hipSuccess &&
                  hipStreamSynchronize(s) == hipSuccess;
-            if (ok) {
-                counts[(size_t)runs] = n;
-                for (int64_t j = 0; j < runs; j++) counts[(size_t)j] = counts[(size_t)j + 1] - counts[(size_t)j];
-                counts.resize((size_t)runs);
-            }
         }
         if (!ok && !has_error()) set_error("histogram assembly failed");
     }
     dfree(k0); dfree(k1); dfree(g0); dfree(g1); dfree(flag); dfree(pos); dfree(ustart); dfree(dofs);
-    dfree(uval); dfree(usg);
+    dfree(usg); dfree(rofs);
+    if (!ok) {
+        dfree(uval);
+        dfree(cnt);
+        return false;
+    }
+    *d_val = uval;
+    *d_cnt = cnt;
+    *runs_out = runs;
+    return true;
+}
+
+// the same, downloaded into vectors (runs of all segments; useg unused here)
+bool exact_histogram(const double *vals, int64_t n, const std::vector<int64_t> &seg_ofs, hipStream_t s,
+                     std::vector<double> &uvals, std::vector<int64_t> &counts, std::vector<int> &useg)
+{
+    uvals.clear();
+    counts.clear();
+    useg.clear();
+    double *dv = nullptr;
+    int64_t *dc = nullptr, runs = 0;
+    std::vector<int64_t> rofs;
+    if (!exact_histogram_dev(vals, n, seg_ofs, s, &dv, &dc, &runs, rofs)) return false;
+    uvals.resize((size_t)runs);
+    counts.resize((size_t)runs);
+    bool ok = runs == 0 ||
+              (hipMemcpyAsync(uvals.data(), dv, (size_t)runs * sizeof(double), hipMemcpyDeviceToHost, s) ==
+                   hipSuccess &&
+               hipMemcpyAsync(counts.data(), dc, (size_t)runs * sizeof(int64_t), hipMemcpyDeviceToHost, s) ==
+                   hipSuccess &&
+               hipStreamSynchronize(s) == hipSuccess);
+    dfree(dv);
+    dfree(dc);
+    if (!ok) set_error("histogram download failed");
+    return ok;
+}
+
+// Flat histogram (dress_histogram.c:184-200 layout: value bits, count, ...)
+// of vals[0..n), malloc'd; NULL with *hist_size = 0 when empty.
+bool histogram_flat(const double *vals, int64_t n, hipStream_t s, int64_t **flat_out, int *hist_size)
+{
+    *flat_out = nullptr;
+    if (hist_size) *hist_size = 0;
+    double *dv = nullptr;
+    int64_t *dc = nullptr, runs = 0;
+    std::vector<int64_t> rofs;
+    if (!exact_histogram_dev(vals, n, {0, n}, s, &dv, &dc, &runs, rofs)) return false;
+    bool ok = true;
+    if (runs > 0) {
+        int64_t *dflat = nullptr;
+        int64_t *flat = (int64_t *)host_alloc((size_t)runs * 2 * sizeof(int64_t));
+        if (!flat) set_error("host allocation of the histogram failed");
+        ok = flat && dalloc(&dflat, (size_t)runs * 2, "flat histogram");
+        if (ok) {
+            hipLaunchKernelGGL(k_interleave, dim3(grid_for(runs, num_cus() * 8)), dim3(TPB), 0, s, dv, dc, runs, dflat);
+            ok = hipMemcpyAsync(flat, dflat, (size_t)runs * 2 * sizeof(int64_t), hipMemcpyDeviceToHost, s) ==
+                     hipSuccess &&
+                 hipStreamSynchronize(s) == hipSuccess;
+            if (!ok) set_error("histogram download failed");
+        }
+        dfree(dflat);
+        if (ok) {
+            *flat_out = flat;
+            if (hist_size) *hist_size = (int)(2 * runs);
+        } else {
+            free(flat);
+        }
+    }
+    dfree(dv);
+    dfree(dc);
     return ok;
 }
 
@@ -495,14 +589,44 @@ __global__ void k_q_compact(const unsigned long long *keys, const unsigned long 
         }
 }
 
-// table -> (key, count) ascending by key
-bool qtable_download(const unsigned long long *keys, const unsigned long long *counts, int64_t cap, hipStream_t s,
-                     std::vector<uint64_t> &hk, std::vector<uint64_t> &hc)
+__global__ void k_bnd_flags(const uint64_t *key, int64_t n, int *flag)
 {
-    int *flag = nullptr;
-    int64_t *pos = nullptr;
+    for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x)
+        flag[i] = (int)((key[i] >> 40) & 1);
+}
+
+// sorted (graph << 41 | boundary << 40 | bin) keys -> the per-graph bin
+// table and the per-graph boundary table, with their graph offsets
+__global__ void k_qsplit(const uint64_t *key, const uint64_t *cnt, const int64_t *bpos, int64_t n, int ngraph,
+                         int64_t *qk, int64_t *qc, int64_t *bk, int64_t *bc, int64_t *qofs, int64_t *bofs)
+{
+    for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) {
+        const uint64_t k = key[i];
+        const int g = (int)(k >> 41);
+        const int64_t b = bpos[i];
+        const int64_t q = (int64_t)(k & 0xffffffffffull);
+        if ((k >> 40) & 1) { bk[b] = q; bc[b] = (int64_t)cnt[i]; }
+        else { qk[i - b] = q; qc[i - b] = (int64_t)cnt[i]; }
+        const int prev = i ? (int)(key[i - 1] >> 41) : -1;
+        for (int h = prev + 1; h <= g && h <= ngraph; h++) { qofs[h] = i - b; bofs[h] = b; }
+        if (i == n - 1)
+            for (int h = g + 1; h <= ngraph; h++) { qofs[h] = n - bpos[n]; bofs[h] = bpos[n]; }
+    }
+}
+
+// Table -> per-graph (bin, count) tables ascending by bin, values near a
+// bin edge in a separate per-graph table; all left on the device (the caller
+// downloads and dfrees them).  qofs / bofs hold n_graphs + 1 offsets.
+bool qtable_split(const unsigned long long *keys, const unsigned long long *counts, int64_t cap, int n_graphs,
+                  hipStream_t s, int64_t **qk, int64_t **qc, int64_t **bk, int64_t **bc, int64_t **qofs,
+                  int64_t **bofs, int64_t *nq_out, int64_t *nb_out)
+{
+    *qk = *qc = *bk = *bc = *qofs = *bofs = nullptr;
+    *nq_out = *nb_out = 0;
+    int *flag = nullptr, *bflag = nullptr;
+    int64_t *pos = nullptr, *bpos = nullptr;
     uint64_t *k0 = nullptr, *c0 = nullptr, *k1 = nullptr, *c1 = nullptr;
-    int64_t n = 0;
+    int64_t n = 0, nb = 0;
     bool ok = dalloc(&flag, (size_t)cap + 1, "table flags") && dalloc(&pos, (size_t)cap + 1, "table positions");
     if (ok) {
         hipLaunchKernelGGL(k_q_flags, dim3(grid_for(cap, 4096)), dim3(TPB), 0, s, keys, cap, flag);
@@ -512,19 +636,45 @@ bool qtable_download(const unsigned long long *keys, const unsigned long long *c
              hipStreamSynchronize(s) == hipSuccess;
     }
     ok = ok && dalloc(&k0, (size_t)n + 1, "table keys") && dalloc(&c0, (size_t)n + 1, "table counts") &&
-         dalloc(&k1, (size_t)n + 1, "table keys") && dalloc(&c1, (size_t)n + 1, "table counts");
+         dalloc(&k1, (size_t)n + 1, "table keys") && dalloc(&c1, (size_t)n + 1, "table counts") &&
+         dalloc(&bflag, (size_t)n + 1, "boundary flags") && dalloc(&bpos, (size_t)n + 1, "boundary positions");
     if (ok) {
         hipLaunchKernelGGL(k_q_compact, dim3(grid_for(cap, 4096)), dim3(TPB), 0, s, keys, counts, flag, pos, cap, k0, c0);
         ok = prim_sort_pairs_u64_u64(k0, k1, c0, c1, (size_t)n, s);
-        hk.resize((size_t)n);
-        hc.resize((size_t)n);
-        ok = ok && (n == 0 || (hipMemcpyAsync(hk.data(), k1, (size_t)n * 8, hipMemcpyDeviceToHost, s) == hipSuccess &&
-                               hipMemcpyAsync(hc.data(), c1, (size_t)n * 8, hipMemcpyDeviceToHost, s) == hipSuccess)) &&
-             hipStreamSynchronize(s) == hipSuccess;
+        if (ok) {
+            if (n > 0)
+                hipLaunchKernelGGL(k_bnd_flags, dim3(grid_for(n, 4096)), dim3(TPB), 0, s, (const uint64_t *)k1, n, bflag);
+            hipLaunchKernelGGL(k_zero_int, dim3(1), dim3(64), 0, s, bflag + n, 1);
+            ok = prim_exclusive_scan_i32_to_i64(bflag, bpos, (size_t)n + 1, s) &&
+                 hipMemcpyAsync(&nb, bpos + n, sizeof(int64_t), hipMemcpyDeviceToHost, s) == hipSuccess &&
+                 hipStreamSynchronize(s) == hipSuccess;
+        }
     }
-    if (!ok && !has_error()) set_error("quantised histogram download failed");
-    dfree(flag); dfree(pos); dfree(k0); dfree(c0); dfree(k1); dfree(c1);
-    return ok;
+    const int64_t nq = n - nb;
+    ok = ok && dalloc(qk, (size_t)nq + 1, "bin keys") && dalloc(qc, (size_t)nq + 1, "bin counts") &&
+         dalloc(bk, (size_t)nb + 1, "boundary keys") && dalloc(bc, (size_t)nb + 1, "boundary counts") &&
+         dalloc(qofs, (size_t)n_graphs + 1, "bin offsets") && dalloc(bofs, (size_t)n_graphs + 1, "boundary offsets");
+    if (ok) {
+        if (n > 0) {
+            hipLaunchKernelGGL(k_qsplit, dim3(grid_for(n, 4096)), dim3(TPB), 0, s, (const uint64_t *)k1,
+                               (const uint64_t *)c1, (const int64_t *)bpos, n, n_graphs, *qk, *qc, *bk, *bc, *qofs,
+                               *bofs);
+        } else {
+            ok = hipMemsetAsync(*qofs, 0, ((size_t)n_graphs + 1) * sizeof(int64_t), s) == hipSuccess &&
+                 hipMemsetAsync(*bofs, 0, ((size_t)n_graphs + 1) * sizeof(int64_t), s) == hipSuccess;
+        }
+        ok = ok && hipGetLastError() == hipSuccess;
+    }
+    if (!ok && !has_error()) set_error("quantised histogram assembly failed");
+    dfree(flag); dfree(pos); dfree(k0); dfree(c0); dfree(k1); dfree(c1); dfree(bflag); dfree(bpos);
+    if (!ok) {
+        dfree(*qk); dfree(*qc); dfree(*bk); dfree(*bc); dfree(*qofs); dfree(*bofs);
+        *qk = *qc = *bk = *bc = *qofs = *bofs = nullptr;
+        return false;
+    }
+    *nq_out = nq;
+    *nb_out = nb;
+    return true;
 }
 
 // ------------------------------------------------------------ runner
@@ -552,6 +702,23 @@ struct FitOutputs {
     std::vector<int64_t> vn_ofs;
     std::vector<int> *iters;
     std::vector<double> *deltas;
+};
+
+// Phase timings printed to stderr when DRESS_HIP_TIMING is set.
+struct PhaseTimer {
+    const char *what;
+    bool on;
+    std::chrono::steady_clock::time_point last;
+    explicit PhaseTimer(const char *w) : what(w), on(getenv("DRESS_HIP_TIMING") != nullptr),
+                                         last(std::chrono::steady_clock::now()) {}
+    void mark(const char *phase)
+    {
+        if (!on) return;
+        auto now = std::chrono::steady_clock::now();
+        fprintf(stderr, "%s: %-18s %8.3f ms\n", what, phase,
+                std::chrono::duration<double, std::milli>(now - last).count());
+        last = now;
+    }
 };
 
 int choose_block(int E)
@@ -851,34 +1018,15 @@ bool plain_item(p_dress_graph_t g, int kind, const int *items, int k, const doub
     return true;
 }
 
-int64_t *flat_from(const std::vector<double> &uv, const std::vector<int64_t> &cnt, int *hist_size)
-{
-    if (uv.empty()) {
-        if (hist_size) *hist_size = 0;
-        return nullptr;
-    }
-    int64_t *flat = (int64_t *)malloc(uv.size() * 2 * sizeof(int64_t));
-    if (!flat) {
-        set_error("host allocation of the histogram failed");
-        return nullptr;
-    }
-    for (size_t i = 0; i < uv.size(); i++) {
-        memcpy(&flat[2 * i], &uv[i], sizeof(double));
-        flat[2 * i + 1] = cnt[i];
-    }
-    if (hist_size) *hist_size = (int)(2 * uv.size());
-    return flat;
-}
-
-bool host_values_histogram(const std::vector<double> &vals, hipStream_t s, std::vector<double> &uv,
-                           std::vector<int64_t> &cnt)
+// flat histogram of host values (uploaded, then histogram_flat); NULL on
+// error or when empty
+int64_t *host_values_flat(const std::vector<double> &vals, hipStream_t s, int *hist_size)
 {
     double *dv = nullptr;
-    std::vector<int> useg;
-    bool ok = upload(&dv, vals, s, "values") &&
-              exact_histogram(dv, (int64_t)vals.size(), {0, (int64_t)vals.size()}, s, uv, cnt, useg);
+    int64_t *flat = nullptr;
+    if (upload(&dv, vals, s, "values")) histogram_flat(dv, (int64_t)vals.size(), s, &flat, hist_size);
     dfree(dv);
-    return ok;
+    return flat;
 }
 
 // delta_dress_impl.c:233-380 / nabla_dress_impl.c:142-311 on the GPU
@@ -905,8 +1053,6 @@ int64_t *lifted_flat(int kind, p_dress_graph_t g, int k, int iterations, double 
         }
     }
     int precision = default_precision();
-    std::vector<double> uvals;
-    std::vector<int64_t> counts;
     hipStream_t s = nullptr;
     DevGraph *G = device_graph(g);
     if (!G) return nullptr;
@@ -920,8 +1066,7 @@ int64_t *lifted_flat(int kind, p_dress_graph_t g, int k, int iterations, double 
         std::vector<double> v(g->edge_dress, g->edge_dress + E);
         if (host_ms) memcpy(host_ms, g->edge_dress, (size_t)E * sizeof(double));
         if (!do_hist) return nullptr;
-        if (!host_values_histogram(v, s, uvals, counts)) return nullptr;
-        return flat_from(uvals, counts, hist_size);
+        return host_values_flat(v, s, hist_size);
     }
     if (k >= N) return nullptr;     // delta_dress_impl.c:289-293, nabla_dress_impl.c:197-201
 
@@ -963,13 +1108,13 @@ int64_t *lifted_flat(int kind, p_dress_graph_t g, int k, int iterations, double 
         if (ok && host_ms && total > 0)
             ok = hipMemcpyAsync(host_ms, d_ms, (size_t)total * (size_t)E * sizeof(double), hipMemcpyDeviceToHost,
                                 s) == hipSuccess && hipStreamSynchronize(s) == hipSuccess;
-        std::vector<int> useg;
-        if (ok && do_hist)
-            ok = exact_histogram(out.d_vals, out.val_ofs.back(), {0, out.val_ofs.back()}, s, uvals, counts, useg);
+        int64_t *flat = nullptr;
+        if (ok && do_hist) ok = histogram_flat(out.d_vals, out.val_ofs.back(), s, &flat, hist_size);
         dfree(out.d_vals);
         dfree(d_ms);
         if (!ok) return nullptr;
         record_lifted(out.reruns, out.device_ms, (int64_t)sel.size(), out.val_ofs.back());
+        return flat;
     } else {
         std::vector<double> vals;
         for (size_t i = 0; i < sel.size(); i++) {
@@ -981,36 +1126,25 @@ int64_t *lifted_flat(int kind, p_dress_graph_t g, int k, int iterations, double 
             if (!plain_item(g, kind, &verts[i * (size_t)k], k, J.marks, iterations, epsilon, precision, vals, row))
                 return nullptr;
         }
-        if (do_hist && !host_values_histogram(vals, s, uvals, counts)) return nullptr;
         record_lifted(0, 0.0, (int64_t)sel.size(), (int64_t)vals.size());
+        if (!do_hist) return nullptr;
+        return host_values_flat(vals, s, hist_size);
     }
-    if (!do_hist) return nullptr;
-    return flat_from(uvals, counts, hist_size);
 }
 
 dress_hist_pair_t *to_pairs(int64_t *flat, int flat_len, int *hist_size)
 {
-    // dress_histogram.c:184-200
+    // dress_histogram.c:184-200: the flat array (value bits, count, ...) has
+    // the byte layout of dress_hist_pair_t {double; int64_t}, so the same
+    // malloc'd block is handed over (the caller frees it with free())
+    static_assert(sizeof(dress_hist_pair_t) == 2 * sizeof(int64_t), "pair layout");
     if (!flat || flat_len <= 0) {
         free(flat);
         if (hist_size) *hist_size = 0;
         return nullptr;
     }
-    int n = flat_len / 2;
-    dress_hist_pair_t *p = (dress_hist_pair_t *)malloc((size_t)n * sizeof(dress_hist_pair_t));
-    if (!p) {
-        set_error("host allocation of histogram pairs failed");
-        free(flat);
-        if (hist_size) *hist_size = 0;
-        return nullptr;
-    }
-    for (int i = 0; i < n; i++) {
-        memcpy(&p[i].value, &flat[2 * i], sizeof(double));
-        p[i].count = flat[2 * i + 1];
-    }
-    free(flat);
-    if (hist_size) *hist_size = n;
-    return p;
+    if (hist_size) *hist_size = flat_len / 2;
+    return reinterpret_cast<dress_hist_pair_t *>(flat);
 }
 
 }  // namespace
@@ -1327,6 +1461,7 @@ int dress_hip_lifted_batch(int kind, int k, int n_graphs, const int *N, const in
         return -1;
     }
     auto t0 = std::chrono::steady_clock::now();
+    PhaseTimer pt("dress_hip_lifted_batch");
     hipStream_t s = ctx_stream();
     const bool own_stream = s == nullptr;
     if (own_stream && hipStreamCreateWithFlags(&s, hipStreamNonBlocking) != hipSuccess) {
@@ -1372,6 +1507,7 @@ int dress_hip_lifted_batch(int kind, int k, int n_graphs, const int *N, const in
         }
         item_verts.insert(item_verts.end(), verts.begin(), verts.end());
     }
+    pt.mark("parents+items");
     const int kk = std::max(k, 0);
     if (k == 0) item_verts.assign(item_parent.size() * (size_t)kk, 0);
     LiftJob J;
@@ -1396,6 +1532,7 @@ int dress_hip_lifted_batch(int kind, int k, int n_graphs, const int *N, const in
         ok = run_items(ps, graph_of, item_parent, item_verts, item_live, J, nullptr, 0, ms_rows, s, lo, qk, qc, qcap,
                        qtol);
     }
+    pt.mark("run_items");
     // exact histogram per graph: items of a graph are contiguous
     std::vector<int64_t> seg_ofs((size_t)n_graphs + 1, 0);
     if (ok) {
@@ -1405,64 +1542,60 @@ int dress_hip_lifted_batch(int kind, int k, int n_graphs, const int *N, const in
             seg_ofs[(size_t)gi + 1] = lo.val_ofs[it];
         }
     }
-    std::vector<double> uv;
-    std::vector<int64_t> cnt;
-    std::vector<int> useg;
-    if (ok && n_graphs > 0) ok = exact_histogram(lo.d_vals, lo.val_ofs.back(), seg_ofs, s, uv, cnt, useg);
-    std::vector<uint64_t> hk, hc;
-    if (ok) ok = qtable_download(qk, qc, qcap, s, hk, hc);
-    dfree(qk); dfree(qc); dfree(lo.d_vals);
-    if (own_stream) (void)hipStreamDestroy(s);
-    else (void)hipStreamSynchronize(s);
-    if (!ok) return -1;
-    // assemble
+    double *hv = nullptr;
+    int64_t *hc = nullptr, runs = 0;
+    std::vector<int64_t> run_ofs;
+    if (ok && n_graphs > 0) ok = exact_histogram_dev(lo.d_vals, lo.val_ofs.back(), seg_ofs, s, &hv, &hc, &runs, run_ofs);
+    pt.mark("exact_histogram");
+    int64_t nq = 0, nb = 0;
+    int64_t *qk2 = nullptr, *qc2 = nullptr, *bk2 = nullptr, *bc2 = nullptr, *qofs = nullptr, *bofs = nullptr;
+    if (ok) ok = qtable_split(qk, qc, qcap, n_graphs, s, &qk2, &qc2, &bk2, &bc2, &qofs, &bofs, &nq, &nb);
+    pt.mark("qtable_split");
     out->n_graphs = n_graphs;
-    out->hist_offset = (int64_t *)calloc((size_t)n_graphs + 1, sizeof(int64_t));
-    out->hist_value = (double *)malloc((uv.size() + 1) * sizeof(double));
-    out->hist_count = (int64_t *)malloc((uv.size() + 1) * sizeof(int64_t));
-    out->qhist_offset = (int64_t *)calloc((size_t)n_graphs + 1, sizeof(int64_t));
-    out->qbound_offset = (int64_t *)calloc((size_t)n_graphs + 1, sizeof(int64_t));
-    out->qhist_key = (int64_t *)malloc((hk.size() + 1) * sizeof(int64_t));
-    out->qhist_count = (int64_t *)malloc((hk.size() + 1) * sizeof(int64_t));
-    out->qbound_key = (int64_t *)malloc((hk.size() + 1) * sizeof(int64_t));
-    out->qbound_count = (int64_t *)malloc((hk.size() + 1) * sizeof(int64_t));
-    if (!out->hist_offset || !out->hist_value || !out->hist_count || !out->qhist_offset || !out->qbound_offset ||
-        !out->qhist_key || !out->qhist_count || !out->qbound_key || !out->qbound_count) {
-        dress_hip_lifted_batch_free(out);
-        set_error("host allocation of batch results failed");
-        return -1;
-    }
-    for (size_t i = 0; i < uv.size(); i++) {
-        out->hist_value[i] = uv[i];
-        out->hist_count[i] = cnt[i];
-        out->hist_offset[useg[i] + 1]++;
-    }
-    for (int gi = 0; gi < n_graphs; gi++) out->hist_offset[gi + 1] += out->hist_offset[gi];
-    // table keys ascend by (graph, boundary flag, bin)
-    size_t nq = 0, nb = 0;
-    for (size_t i = 0; i < hk.size(); i++) {
-        int gi = (int)(hk[i] >> 41);
-        bool bnd = (hk[i] >> 40) & 1;
-        int64_t q = (int64_t)(hk[i] & 0xffffffffffull);
-        if (gi < 0 || gi >= n_graphs) continue;
-        if (bnd) {
-            out->qbound_key[nb] = q;
-            out->qbound_count[nb++] = (int64_t)hc[i];
-            out->qbound_offset[gi + 1]++;
-        } else {
-            out->qhist_key[nq] = q;
-            out->qhist_count[nq++] = (int64_t)hc[i];
-            out->qhist_offset[gi + 1]++;
+    if (ok) {
+        out->hist_offset = (int64_t *)malloc(((size_t)n_graphs + 1) * sizeof(int64_t));
+        out->hist_value = (double *)host_alloc(((size_t)runs + 1) * sizeof(double));
+        out->hist_count = (int64_t *)host_alloc(((size_t)runs + 1) * sizeof(int64_t));
+        out->qhist_offset = (int64_t *)malloc(((size_t)n_graphs + 1) * sizeof(int64_t));
+        out->qbound_offset = (int64_t *)malloc(((size_t)n_graphs + 1) * sizeof(int64_t));
+        out->qhist_key = (int64_t *)host_alloc(((size_t)nq + 1) * sizeof(int64_t));
+        out->qhist_count = (int64_t *)host_alloc(((size_t)nq + 1) * sizeof(int64_t));
+        out->qbound_key = (int64_t *)host_alloc(((size_t)nb + 1) * sizeof(int64_t));
+        out->qbound_count = (int64_t *)host_alloc(((size_t)nb + 1) * sizeof(int64_t));
+        if (!out->hist_offset || !out->hist_value || !out->hist_count || !out->qhist_offset || !out->qbound_offset ||
+            !out->qhist_key || !out->qhist_count || !out->qbound_key || !out->qbound_count) {
+            set_error("host allocation of batch results failed");
+            ok = false;
         }
     }
-    for (int gi = 0; gi < n_graphs; gi++) {
-        out->qhist_offset[gi + 1] += out->qhist_offset[gi];
-        out->qbound_offset[gi + 1] += out->qbound_offset[gi];
+    if (ok) {
+        for (int gi = 0; gi <= n_graphs; gi++) out->hist_offset[gi] = n_graphs > 0 ? run_ofs[(size_t)gi] : 0;
+        auto d2h = [&](void *dst, const void *src, size_t bytes) {
+            return bytes == 0 || hipMemcpyAsync(dst, src, bytes, hipMemcpyDeviceToHost, s) == hipSuccess;
+        };
+        ok = d2h(out->hist_value, hv, (size_t)runs * sizeof(double)) &&
+             d2h(out->hist_count, hc, (size_t)runs * sizeof(int64_t)) &&
+             d2h(out->qhist_key, qk2, (size_t)nq * sizeof(int64_t)) &&
+             d2h(out->qhist_count, qc2, (size_t)nq * sizeof(int64_t)) &&
+             d2h(out->qbound_key, bk2, (size_t)nb * sizeof(int64_t)) &&
+             d2h(out->qbound_count, bc2, (size_t)nb * sizeof(int64_t)) &&
+             d2h(out->qhist_offset, qofs, ((size_t)n_graphs + 1) * sizeof(int64_t)) &&
+             d2h(out->qbound_offset, bofs, ((size_t)n_graphs + 1) * sizeof(int64_t)) &&
+             hipStreamSynchronize(s) == hipSuccess;
+        if (!ok && !has_error()) set_error("batch result download failed");
+    }
+    dfree(hv); dfree(hc); dfree(qk2); dfree(qc2); dfree(bk2); dfree(bc2); dfree(qofs); dfree(bofs);
+    dfree(qk); dfree(qc); dfree(lo.d_vals);
+    if (own_stream) (void)hipStreamDestroy(s);
+    if (!ok) {
+        dress_hip_lifted_batch_free(out);
+        return -1;
     }
     out->subgraphs = (int64_t)item_parent.size();
     out->values = lo.val_ofs.back();
     out->fp64_reruns = lo.reruns;
     out->device_ms = lo.device_ms;
+    pt.mark("assemble");
     out->total_ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
     record_lifted(lo.reruns, lo.device_ms, out->subgraphs, out->values);
     return 0;

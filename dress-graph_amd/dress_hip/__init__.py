@@ -163,20 +163,45 @@ def lifted_batch(graphs, k=1, kind="delta", variant=UNDIRECTED, max_iterations=1
                                   int(variant), int(max_iterations), float(epsilon), int(precision),
                                   float(qtol), ctypes.byref(r))
     _lib.check(rc == 0, "dress_hip_lifted_batch")
-    try:
-        def split(off_p, a_p, b_p, dt_a):
-            off = np.ctypeslib.as_array(off_p, shape=(n + 1,)).copy()
-            tot = int(off[-1])
-            a = np.ctypeslib.as_array(a_p, shape=(max(tot, 1),))[:tot].astype(dt_a, copy=True)
-            b = np.ctypeslib.as_array(b_p, shape=(max(tot, 1),))[:tot].astype(np.int64, copy=True)
-            return [(a[off[i]:off[i + 1]], b[off[i]:off[i + 1]]) for i in range(n)]
-        hists = split(r.hist_offset, r.hist_value, r.hist_count, np.float64)
-        quant = split(r.qhist_offset, r.qhist_key, r.qhist_count, np.int64)
-        bound = split(r.qbound_offset, r.qbound_key, r.qbound_count, np.int64)
-        return LiftedBatchResult(hists, quant, bound, r.subgraphs, r.values, r.fp64_reruns,
-                                 r.device_ms, r.total_ms)
-    finally:
-        L.dress_hip_lifted_batch_free(ctypes.byref(r))
+    # the result arrays stay where the library wrote them: numpy views own
+    # the block and release it (dress_hip_lifted_batch_free) with the last view
+    owner = _BatchOwner(r)
+
+    def split(off_p, a_p, b_p, typestr_a):
+        off = np.ctypeslib.as_array(off_p, shape=(n + 1,)).copy()
+        tot = int(off[-1])
+        a = _owned_view(a_p, tot, typestr_a, owner)
+        b = _owned_view(b_p, tot, "<i8", owner)
+        return [(a[off[i]:off[i + 1]], b[off[i]:off[i + 1]]) for i in range(n)]
+    hists = split(r.hist_offset, r.hist_value, r.hist_count, "<f8")
+    quant = split(r.qhist_offset, r.qhist_key, r.qhist_count, "<i8")
+    bound = split(r.qbound_offset, r.qbound_key, r.qbound_count, "<i8")
+    return LiftedBatchResult(hists, quant, bound, r.subgraphs, r.values, r.fp64_reruns, r.device_ms, r.total_ms)
+
+
+class _BatchOwner:
+    """Frees a dress_hip_lifted_batch_t when the last array viewing it goes."""
+
+    def __init__(self, r):
+        self.r = r
+
+    def __del__(self):
+        try:
+            _lib.lib().dress_hip_lifted_batch_free(ctypes.byref(self.r))
+        except Exception:
+            pass
+
+
+class _OwnedBuffer:
+    def __init__(self, addr, count, typestr, owner):
+        self.__array_interface__ = {"data": (addr, False), "shape": (count,), "typestr": typestr, "version": 3}
+        self._owner = owner
+
+
+def _owned_view(ptr, count, typestr, owner):
+    if count == 0 or not ptr:
+        return np.zeros(0, dtype=np.dtype(typestr))
+    return np.asarray(_OwnedBuffer(ctypes.cast(ptr, ctypes.c_void_p).value, count, typestr, owner))
 
 
 @dataclass
