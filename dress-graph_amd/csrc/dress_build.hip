@@ -890,6 +890,8 @@ void free_graph(DevGraph *G)
     dfree(G->vbins.list);
     for (int i = 0; i < 2; i++) { dfree(G->d[i]); dfree(G->wh[i]); dfree(G->d32[i]); dfree(G->wh32[i]); }
     dfree(G->vd); dfree(G->stage); dfree(G->state);
+    for (int i = 0; i < 2; i++)
+        if (G->iter_graph[i]) hipGraphExecDestroy(G->iter_graph[i]);
     if (G->stream && G->own_stream) hipStreamDestroy(G->stream);
     delete G;
 }

@@ -431,6 +431,29 @@ void launch_phase_t(DevGraph *Gr, VT *d0, VT *d1, VT *w0, VT *w1)
     launch_edge_t<TPB, PK, VT>(Gr, 4, cap, d0, d1, w0, w1);
 }
 
+// resolves the occupancy of every edge kernel the iteration launches (a
+// one-time query that must not happen inside a stream capture)
+void launch_phase_prepare(DevGraph *Gr)
+{
+    (void)num_xcds();
+    (void)num_cus();
+    if (Gr->packed) {
+        (void)edge_capacity<1, true, double>(); (void)edge_capacity<4, true, double>();
+        (void)edge_capacity<16, true, double>(); (void)edge_capacity<64, true, double>();
+        (void)edge_capacity<TPB, true, double>();
+        (void)edge_capacity<1, true, float>(); (void)edge_capacity<4, true, float>();
+        (void)edge_capacity<16, true, float>(); (void)edge_capacity<64, true, float>();
+        (void)edge_capacity<TPB, true, float>();
+    } else {
+        (void)edge_capacity<1, false, double>(); (void)edge_capacity<4, false, double>();
+        (void)edge_capacity<16, false, double>(); (void)edge_capacity<64, false, double>();
+        (void)edge_capacity<TPB, false, double>();
+        (void)edge_capacity<1, false, float>(); (void)edge_capacity<4, false, float>();
+        (void)edge_capacity<16, false, float>(); (void)edge_capacity<64, false, float>();
+        (void)edge_capacity<TPB, false, float>();
+    }
+}
+
 void launch_phase(DevGraph *Gr, bool fp32)
 {
     if (fp32) {
@@ -484,6 +507,63 @@ bool fit_upload_values(DevGraph *G, const double *edge_dress)
 
 namespace {
 
+// the kernels of one iteration on stream s (state-driven: the same launches
+// serve every iteration)
+void launch_iteration(DevGraph *G, bool mixed, double epsilon, int max_iterations, int egrid, hipStream_t s)
+{
+    FitState *st = (FitState *)G->state;
+    if (mixed) launch_phase(G, true);
+    launch_phase(G, false);
+    hipLaunchKernelGGL(k_end, dim3(1), dim3(64), 0, s, st, epsilon, max_iterations);
+    if (mixed) {
+        hipLaunchKernelGGL(k_to64, dim3(egrid), dim3(TPB), 0, s, G->E, G->ew, G->pos, G->d[0], G->d[1], G->wh[0],
+                           G->wh[1], G->d32[0], G->d32[1], st);
+        hipLaunchKernelGGL(k_switch_done, dim3(1), dim3(64), 0, s, st);
+    }
+}
+
+// With DRESS_HIP_GRAPH=1, one iteration is captured as a HIP graph (kernel
+// nodes only) and kept with the device graph, so each iteration is one
+// graph launch instead of one launch per kernel.  The captured arguments are
+// the graph's buffers, epsilon and max_iterations; a fit with other values
+// recaptures.  Off by default: on MI355X the replay saves nothing
+// measurable for fits of 10^4..10^5 edges (the time is the kernels' device
+// dispatch, not the host launch), and the capture and instantiation cost
+// of a one-shot fit is a loss under many concurrent requests.
+hipGraphExec_t iteration_graph(DevGraph *G, bool mixed, double epsilon, int max_iterations, int egrid)
+{
+    const char *env = getenv("DRESS_HIP_GRAPH");
+    if (!(env && !strcmp(env, "1"))) return nullptr;
+    const int m = mixed ? 1 : 0;
+    if (G->iter_graph[m] && G->graph_eps[m] == epsilon && G->graph_maxit[m] == max_iterations)
+        return G->iter_graph[m];
+    if (G->iter_graph[m]) {
+        hipGraphExecDestroy(G->iter_graph[m]);
+        G->iter_graph[m] = nullptr;
+    }
+    // occupancy queries and device attributes are settled before capturing
+    launch_phase_prepare(G);
+    hipStream_t s = G->stream;
+    hipGraph_t graph = nullptr;
+    if (hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal) != hipSuccess) {
+        hipGetLastError();
+        return nullptr;
+    }
+    launch_iteration(G, mixed, epsilon, max_iterations, egrid, s);
+    hipError_t e = hipStreamEndCapture(s, &graph);
+    hipGraphExec_t exec = nullptr;
+    if (e == hipSuccess && graph) e = hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0);
+    if (graph) hipGraphDestroy(graph);
+    if (e != hipSuccess || !exec) {
+        hipGetLastError();
+        return nullptr;                     // fall back to eager launches
+    }
+    G->iter_graph[m] = exec;
+    G->graph_eps[m] = epsilon;
+    G->graph_maxit[m] = max_iterations;
+    return exec;
+}
+
 bool fit_run_once(DevGraph *G, int max_iterations, double epsilon, bool mixed, FitResult *res,
                   bool *ambiguous)
 {
@@ -516,19 +596,14 @@ bool fit_run_once(DevGraph *G, int max_iterations, double epsilon, bool mixed, F
             hipLaunchKernelGGL((k_init_wh<double>), dim3(egrid), dim3(TPB), 0, s, G->E, G->ew, G->pos, G->d[0],
                                G->d[1], (double *)nullptr, (double *)nullptr, G->wh[0], G->wh[1], st);
     }
+    hipGraphExec_t ig = iteration_graph(G, mixed, epsilon, max_iterations, egrid);
     int launched = 0;
     int chunk = 4;
     while (launched < max_iterations) {
         int n = std::min(chunk, max_iterations - launched);
         for (int j = 0; j < n; j++) {
-            if (mixed) launch_phase(G, true);
-            launch_phase(G, false);
-            hipLaunchKernelGGL(k_end, dim3(1), dim3(64), 0, s, st, epsilon, max_iterations);
-            if (mixed) {
-                hipLaunchKernelGGL(k_to64, dim3(egrid), dim3(TPB), 0, s, G->E, G->ew, G->pos, G->d[0],
-                                   G->d[1], G->wh[0], G->wh[1], G->d32[0], G->d32[1], st);
-                hipLaunchKernelGGL(k_switch_done, dim3(1), dim3(64), 0, s, st);
-            }
+            if (ig) DH_CHECK(hipGraphLaunch(ig, s));
+            else launch_iteration(G, mixed, epsilon, max_iterations, egrid, s);
         }
         launched += n;
         DH_CHECK(hipGetLastError());

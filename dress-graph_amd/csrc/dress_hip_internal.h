@@ -136,6 +136,9 @@ struct DevGraph {
     float *wh32[2] = {nullptr, nullptr};
     double *stage = nullptr;       // input-order staging for uploads/downloads
     void *state = nullptr;         // FitState (device)
+    hipGraphExec_t iter_graph[2] = {nullptr, nullptr};   // one iteration, [mixed]
+    double graph_eps[2] = {0.0, 0.0};                      // ... captured with these
+    int graph_maxit[2] = {0, 0};
     int cur = 0;                   // buffer holding the current values
 };
 

@@ -21,6 +21,7 @@
 #include <cstdio>
 #include <cstdlib>
 #include <cstring>
+#include <map>
 #include <mutex>
 #include <unordered_map>
 #include <vector>
@@ -721,14 +722,59 @@ struct PhaseTimer {
     }
 };
 
-int choose_block(int E)
+int forced_block()
 {
     const char *b = getenv("DRESS_HIP_LIFT_BLOCK");
     if (b) {
         int v = atoi(b);
         if (v == 64 || v == 128 || v == 256 || v == 512) return v;
     }
+    return 0;
+}
+
+int choose_block(int E)
+{
+    if (int f = forced_block()) return f;
     return E <= 512 ? 64 : (E <= 2048 ? 128 : 256);
+}
+
+// Block-size autotuner of the batched kernel.  A size class is (bit length
+// of the largest item's edge count, precision, kind).  The first launch
+// group of an untuned class splits its leading items over the candidate
+// block sizes -- every candidate computes real items, results do not
+// depend on the block size -- and the size with the least device time per
+// item is kept for the rest of the process.  DRESS_HIP_LIFT_BLOCK forces a
+// size; DRESS_HIP_LIFT_TUNE=0 keeps the static choice.
+struct BlockTuner {
+    std::mutex mu;
+    std::map<int, int> best;           // class -> block
+    static int size_class(int E, bool mixed, int kind)
+    {
+        int bits = 0;
+        while ((1 << bits) < std::max(E, 1)) bits++;
+        return bits * 4 + (mixed ? 2 : 0) + (kind ? 1 : 0);
+    }
+    int lookup(int cls)
+    {
+        std::lock_guard<std::mutex> lk(mu);
+        auto it = best.find(cls);
+        return it == best.end() ? 0 : it->second;
+    }
+    void store(int cls, int block)
+    {
+        std::lock_guard<std::mutex> lk(mu);
+        best[cls] = block;
+    }
+};
+BlockTuner g_tuner;
+
+bool tuning_enabled()
+{
+    static const bool on = [] {
+        const char *e = getenv("DRESS_HIP_LIFT_TUNE");
+        return !(e && !strcmp(e, "0"));
+    }();
+    return on;
 }
 
 // Runs every item (parent index, vertex tuple) in one persistent launch (plus
@@ -834,24 +880,61 @@ bool run_items(const std::vector<const HostParent *> &ps, const std::vector<int>
                     gN = nN;
                     g1++;
                 }
-                a.order = d_ids + g0;
-                a.n_items = (int)(g1 - g0);
                 a.ecap = (gE + 1) & ~1;
                 a.ncap = (gN + 1) & ~1;
-                const int block = choose_block(gE);
-                out.block = std::max(out.block, block);
                 const size_t lds = lifted_lds_bytes(a.ecap, a.ncap);
-                int per_cu_lds = (int)std::max<size_t>(1, per_cu_bytes / (lds + 1024));
-                int per_cu_waves = std::max(1, 32 / (block / 64));
-                int per_cu = std::max(1, std::min(per_cu_lds, per_cu_waves));
-                int grid = std::max(1, std::min(a.n_items, num_cus() * per_cu));
-                if (!(hipMemsetAsync(d_queue, 0, sizeof(int), s) == hipSuccess && hipEventRecord(e0, s) == hipSuccess &&
-                      lifted_launch(a, block, grid, s) && hipEventRecord(e1, s) == hipSuccess &&
-                      hipEventSynchronize(e1) == hipSuccess))
-                    return false;
-                float ms = 0.f;
-                (void)hipEventElapsedTime(&ms, e0, e1);
-                out.device_ms += ms;
+                // workgroups of `block` threads one CU holds at once
+                auto per_cu_of = [&](int block) {
+                    int per_cu_lds = (int)std::max<size_t>(1, per_cu_bytes / (lds + 1024));
+                    int per_cu_waves = std::max(1, 32 / (block / 64));
+                    return std::max(1, std::min(per_cu_lds, per_cu_waves));
+                };
+                // one launch of items [b0, b1) of the list with `block` threads
+                auto launch_range = [&](size_t b0, size_t b1, int block, float *ms_out) -> bool {
+                    a.order = d_ids + b0;
+                    a.n_items = (int)(b1 - b0);
+                    int grid = std::max(1, std::min(a.n_items, num_cus() * per_cu_of(block)));
+                    if (!(hipMemsetAsync(d_queue, 0, sizeof(int), s) == hipSuccess &&
+                          hipEventRecord(e0, s) == hipSuccess && lifted_launch(a, block, grid, s) &&
+                          hipEventRecord(e1, s) == hipSuccess && hipEventSynchronize(e1) == hipSuccess))
+                        return false;
+                    float ms = 0.f;
+                    (void)hipEventElapsedTime(&ms, e0, e1);
+                    out.device_ms += ms;
+                    if (ms_out) *ms_out = ms;
+                    return true;
+                };
+                int block = forced_block();
+                const int cls = BlockTuner::size_class(gE, J.mixed, J.kind);
+                if (!block && tuning_enabled()) block = g_tuner.lookup(cls);
+                size_t b0 = g0;
+                if (!block && tuning_enabled()) {
+                    // calibrate on the group's leading items, each candidate
+                    // on two full waves of its own resident workgroups, when
+                    // the group holds at least twice what that takes
+                    static const int cand[4] = {64, 128, 256, 512};
+                    size_t chunk[4], need = 0;
+                    for (int i = 0; i < 4; i++) {
+                        chunk[i] = 2 * (size_t)num_cus() * (size_t)per_cu_of(cand[i]);
+                        need += chunk[i];
+                    }
+                    if (g1 - g0 >= 2 * need) {
+                        double best_per_item = 0.0;
+                        int best = 0;
+                        for (int i = 0; i < 4; i++) {
+                            float ms = 0.f;
+                            if (!launch_range(b0, b0 + chunk[i], cand[i], &ms)) return false;
+                            double per_item = ms / (double)chunk[i];
+                            if (!best || per_item < best_per_item) { best = cand[i]; best_per_item = per_item; }
+                            b0 += chunk[i];
+                        }
+                        g_tuner.store(cls, best);
+                        block = best;
+                    }
+                }
+                if (!block) block = choose_block(gE);
+                out.block = std::max(out.block, block);
+                if (b0 < g1 && !launch_range(b0, g1, block, nullptr)) return false;
                 g0 = g1;
             }
             return true;

@@ -184,3 +184,31 @@ def test_precision_setting(gpu):
         assert gpu.get_precision() == gpu.FP32_FP64
     finally:
         gpu.set_precision(gpu.FP64)
+
+
+def test_graph_captured_iterations(gpu, ora, gen, monkeypatch):
+    """DRESS_HIP_GRAPH=1 replays one captured iteration per step: repeated
+    fits (eager work and synchronisations between the replays) stay bitwise
+    equal to the oracle, in FP64 and through a precision change."""
+    monkeypatch.setenv("DRESS_HIP_GRAPH", "1")
+    z = np.load(os.path.join(GOLDEN, "config1.npz"))
+    with gpu.Graph(2000, z["U"], z["V"]) as g:
+        for _ in range(3):
+            g.set_edge_dress(np.ones(z["U"].size))
+            it, dl = g.fit(100, 1e-6)
+            assert it == int(z["iterations"]) and dl == float(z["delta"])
+            assert np.array_equal(g.edge_dress, z["edge_dress"])
+        g.set_edge_dress(np.ones(z["U"].size))
+        g.fit(100, 1e-6, gpu.FP32_FP64)
+        assert np.max(np.abs(g.edge_dress - z["edge_dress"])) <= TOL_MIXED
+        g.set_edge_dress(np.ones(z["U"].size))
+        g.fit(100, 1e-6)
+        assert np.array_equal(g.edge_dress, z["edge_dress"])
+    n, U, V = gen.er_single(400, 0.02, 12)
+    with gpu.Graph(n, U, V) as g:
+        for eps in (1e-6, 1e-8):
+            g.set_edge_dress(np.ones(U.size))
+            it, dl = g.fit(100, eps)
+            b = ora.fit(n, U, V, max_iterations=100, epsilon=eps)
+            assert (it, dl) == (b["iterations"], b["delta"])
+            assert np.array_equal(g.edge_dress, b["edge_dress"])

@@ -155,3 +155,22 @@ def test_mixed_precision_lifted(gpu, ora, gen):
     assert np.array_equal(a[1], b[1])
     assert np.max(np.abs(a[0] - b[0])) <= 1e-9
     assert st["subgraphs"] == n
+
+
+def test_block_size_does_not_change_results(gpu, gen, monkeypatch):
+    """The autotuner and forced block sizes only move work between threads:
+    every sum is exactly rounded by one thread, so histograms are equal."""
+    graphs = gen.delta1_batch_graphs(64)        # ~4.7k subgraphs: enough to calibrate
+    ref = None
+    for blk in ("64", "512", None):
+        if blk:
+            monkeypatch.setenv("DRESS_HIP_LIFT_BLOCK", blk)
+        else:
+            monkeypatch.delenv("DRESS_HIP_LIFT_BLOCK", raising=False)
+        r = gpu.lifted_batch(graphs, k=1)
+        cur = [(np.array(v), np.array(c)) for v, c in r.histograms]
+        if ref is None:
+            ref = cur
+        else:
+            for (v0, c0), (v1, c1) in zip(ref, cur):
+                assert np.array_equal(v0, v1) and np.array_equal(c0, c1)
