@@ -26,6 +26,7 @@ struct LiftItem {
     int64_t val_ofs;               // live values at vals[val_ofs ...]
     int64_t ms_row;                // multiset row
     int64_t vn_ofs;                // vertex norms at vnorm[vn_ofs ...] (whole-graph fits)
+    int64_t ew_ofs;                // variant weights at ew[ew_ofs ...], or -1: the parent's
 };
 
 struct LiftArgs {

@@ -95,7 +95,7 @@ __global__ void __launch_bounds__(B) k_lifted(LiftArgs a)
         const int *__restrict__ row_off = a.row_off + P.row_ofs;
         const int *__restrict__ adj_e = a.adj_e + P.adj_ofs;
         const int2 *__restrict__ uv = a.uv + P.edge_ofs;
-        const double *__restrict__ ew = a.ew + P.edge_ofs;
+        const double *__restrict__ ew = a.ew + (I.ew_ofs >= 0 ? I.ew_ofs : P.edge_ofs);
         const int *__restrict__ ioff = a.ioff + P.ioff_ofs;
         const uint32_t *__restrict__ pairs = a.pairs + P.pair_ofs;
         const int directed = P.directed;

@@ -4,6 +4,7 @@
 #include <algorithm>
 #include <atomic>
 #include <cmath>
+#include <vector>
 
 #include "dress_hip_internal.h"
 
@@ -23,5 +24,11 @@ bool thread_cancelled();
 void record_stats(DevGraph *G, const FitResult &r);
 void record_lifted(int reruns, double device_ms, int64_t subgraphs, int64_t values);
 bool lds_eligible_size(int N, int E);
+// dress_masked.hip: delta/nabla items as masked fits of the parent's device graph
+bool masked_items(p_dress_graph_t g, DevGraph *G, int kind, int k, const std::vector<int> &verts,
+                  const std::vector<int64_t> &live, const std::vector<int64_t> &sel, const double *marks, int max_it,
+                  double eps, int precision, double *d_vals, int64_t cap, int64_t *n_vals, double *ms_rows,
+                  const std::vector<int64_t> &patch_off, const std::vector<int> &patch_e,
+                  const std::vector<double> &patch_w, const std::vector<double> &patch_parent_w);
 
 }  // namespace dh

@@ -249,23 +249,27 @@ __global__ void __launch_bounds__(TPB) k_edge(int lo, int hi, int nx, EdgeArgs a
             entry<PK>(a, k, r, x, y);
             two_sum_add(s, c, (double)wc[x] + (double)wc[y]);
         }
-        double de = 0.0, w = 0.0;
+        double de = 0.0, w = 0.0, c4 = 0.0;
         if (lane == 0) {
             de = (double)dc[e];
             w = ntl(a.ew + e);
             double own = w * de;
-            double c4 = ntl(a.c4 + e);
+            c4 = ntl(a.c4 + e);
             double t0 = a.directed ? (c4 + own) : (c4 + 2.0 * own);
             two_sum_add(s, c, t0);
         }
         if (G == TPB) block_reduce(s, c);
         else group_reduce<G>(s, c);
         if (lane == 0) {
+            // a NaN self term marks an edge removed from a masked subgraph:
+            // its value and half-edge copies stay 0 (exact zeros in every
+            // other sum) and it takes no part in the change
+            const bool dead = !(c4 == c4);
             double num = s + c;
             int2 p = ntl2(a.uv + e);
             double den = a.vd[p.x] * a.vd[p.y];
-            double x = den > 0.0 ? num / den : 0.0;
-            mymax = fmax(mymax, fabs(de - x));
+            double x = dead ? 0.0 : (den > 0.0 ? num / den : 0.0);
+            if (!dead) mymax = fmax(mymax, fabs(de - x));
             dn[e] = (VT)x;
             VT wx = (VT)(w * x);
             int2 q = ntl2(a.pos + e);

@@ -53,6 +53,13 @@ struct HostParent {
     std::vector<int2> uv;
     std::vector<double> ew;
     std::vector<uint32_t> pairs;
+    // DIRECTED: edges whose reciprocal lookup meets repeated reciprocal
+    // edges of different weights (amb, with their own weights amb_w), and
+    // the raw out-rows the lookup searches (kept only when amb is not empty)
+    std::vector<int> amb;
+    std::vector<double> amb_w;
+    std::vector<int> raw_off, raw_x;
+    std::vector<double> raw_w;
 };
 
 struct HalfEdge {
@@ -109,6 +116,17 @@ bool host_parent_build(int N, int E, const int *U, const int *V, const double *W
             half.push_back({u, HalfEdge{v, e, wb}});
             if (r < 0) half.push_back({v, HalfEdge{u, e, w}});
             P.ew[e] = wb;
+            if (r >= 0) {
+                // the run of v -> u edges around r: a subgraph's lookup can
+                // stop at another of them (dress.c:70-87 on a shorter row)
+                bool differ = false;
+                for (int j = r - 1; j >= roff[v] && raw[j].x == u; j--) differ = differ || raw[j].w != raw[r].w;
+                for (int j = r + 1; j < roff[v + 1] && raw[j].x == u; j++) differ = differ || raw[j].w != raw[r].w;
+                if (differ) {
+                    P.amb.push_back(e);
+                    P.amb_w.push_back(w);
+                }
+            }
             break;
         }
         case DRESS_VARIANT_FORWARD:
@@ -120,6 +138,12 @@ bool host_parent_build(int N, int E, const int *U, const int *V, const double *W
             P.ew[e] = w;
             break;
         }
+    }
+    if (!P.amb.empty()) {
+        P.raw_off = roff;
+        P.raw_x.resize(raw.size());
+        P.raw_w.resize(raw.size());
+        for (size_t i = 0; i < raw.size(); i++) { P.raw_x[i] = raw[i].x; P.raw_w[i] = raw[i].w; }
     }
     P.S = (int)half.size();
     P.row_off.assign(N + 1, 0);
@@ -153,6 +177,67 @@ bool host_parent_build(int N, int E, const int *U, const int *V, const double *W
     }
     P.T = (int)P.pairs.size();
     return true;
+}
+
+// DIRECTED delta^k: the variant weights that differ between the subgraph
+// without the k vertices in del and the parent.  The subgraph's weight of
+// u -> v is w + the weight of the v -> u edge its own lookup finds
+// (dress.c:70-87, dress.c:239-246) in v's raw row without the deleted
+// vertices; only edges in P.amb can change.  Appends (edge, weight).
+void directed_patches(const HostParent &P, const int *del, int k, std::vector<int> &pe, std::vector<double> &pw)
+{
+    auto gone = [&](int x) {
+        for (int j = 0; j < k; j++)
+            if (del[j] == x) return true;
+        return false;
+    };
+    std::vector<int> row;
+    for (size_t i = 0; i < P.amb.size(); i++) {
+        const int e = P.amb[i], u = P.uv[(size_t)e].x, v = P.uv[(size_t)e].y;
+        if (gone(u) || gone(v)) continue;
+        const int b = P.raw_off[(size_t)v], en = P.raw_off[(size_t)v + 1];
+        row.clear();
+        bool shorter = false;
+        for (int j = b; j < en; j++) {
+            if (gone(P.raw_x[(size_t)j])) shorter = true;
+            else row.push_back(j);
+        }
+        if (!shorter) continue;
+        int lo = 0, hi = (int)row.size() - 1, r = -1;
+        while (lo <= hi) {
+            int mid = lo + (hi - lo) / 2;
+            int x = P.raw_x[(size_t)row[(size_t)mid]];
+            if (x == u) { r = row[(size_t)mid]; break; }
+            if (x < u) lo = mid + 1; else hi = mid - 1;
+        }
+        if (r < 0) continue;                 // not reached: u's edges stay
+        const double wb = P.amb_w[i] + P.raw_w[(size_t)r];
+        if (wb != P.ew[(size_t)e]) {
+            pe.push_back(e);
+            pw.push_back(wb);
+        }
+    }
+}
+
+// per-item weight patches (CSR over items); empty when no item has any
+struct EwPatches {
+    std::vector<int64_t> off;
+    std::vector<int> e;
+    std::vector<double> w;
+    bool any() const { return !e.empty(); }
+};
+
+EwPatches delta_patches(const HostParent &P, const std::vector<int> &verts, int k, size_t n)
+{
+    EwPatches R;
+    if (P.amb.empty() || k <= 0) return R;
+    R.off.assign(n + 1, 0);
+    for (size_t i = 0; i < n; i++) {
+        directed_patches(P, &verts[i * (size_t)k], k, R.e, R.w);
+        R.off[i + 1] = (int64_t)R.e.size();
+    }
+    if (R.e.empty()) R.off.clear();
+    return R;
 }
 
 // ------------------------------------------------------------ enumeration
@@ -269,7 +354,8 @@ bool upload(T **dst, const std::vector<T> &src, hipStream_t s, const char *what)
 }
 
 bool upload_parents(const std::vector<const HostParent *> &ps, const std::vector<int> &graph_of,
-                    std::vector<LiftParent> &meta, DevPools &D, hipStream_t s)
+                    std::vector<LiftParent> &meta, DevPools &D, hipStream_t s,
+                    const std::vector<double> *extra_ew = nullptr)
 {
     std::vector<int> row_off, adj_e, ioff;
     std::vector<int2> uv;
@@ -294,6 +380,7 @@ bool upload_parents(const std::vector<const HostParent *> &ps, const std::vector
         pairs.insert(pairs.end(), P->pairs.begin(), P->pairs.end());
         meta.push_back(m);
     }
+    if (extra_ew) ew.insert(ew.end(), extra_ew->begin(), extra_ew->end());
     return upload(&D.parents, meta, s, "parents") && upload(&D.parent_graph, graph_of, s, "parent graphs") &&
            upload(&D.row_off, row_off, s, "row offsets") && upload(&D.adj_e, adj_e, s, "adjacency") &&
            upload(&D.uv, uv, s, "edge ends") && upload(&D.ew, ew, s, "edge weights") &&
@@ -802,6 +889,39 @@ bool run_items(const std::vector<const HostParent *> &ps, const std::vector<int>
         I.val_ofs = out.val_ofs[(size_t)i];
         I.ms_row = ms_rows.empty() ? 0 : ms_rows[(size_t)i];
         I.vn_ofs = fo ? fo->vn_ofs[(size_t)i] : 0;
+        I.ew_ofs = -1;
+    }
+    // DIRECTED delta items whose own reciprocal lookups give other weights
+    // than the parent's get a patched copy of the weights (one per distinct
+    // patch set), placed after the parents' weights
+    std::vector<double> extra_ew;
+    if (J.kind == LIFT_DELTA && J.k > 0) {
+        int64_t base = 0;
+        for (const HostParent *P : ps) base += P->E;
+        std::map<std::pair<int, std::vector<std::pair<int, double>>>, int64_t> seen;
+        std::vector<int> pe;
+        std::vector<double> pw;
+        for (int i = 0; i < n; i++) {
+            const int pi = item_parent[(size_t)i];
+            const HostParent &P = *ps[(size_t)pi];
+            if (P.amb.empty()) continue;
+            pe.clear();
+            pw.clear();
+            directed_patches(P, &item_verts[(size_t)i * J.k], J.k, pe, pw);
+            if (pe.empty()) continue;
+            std::vector<std::pair<int, double>> key;
+            for (size_t j = 0; j < pe.size(); j++) key.push_back({pe[j], pw[j]});
+            auto it = seen.find({pi, key});
+            if (it != seen.end()) {
+                items[(size_t)i].ew_ofs = it->second;
+                continue;
+            }
+            int64_t at = base + (int64_t)extra_ew.size();
+            extra_ew.insert(extra_ew.end(), P.ew.begin(), P.ew.end());
+            for (size_t j = 0; j < pe.size(); j++) extra_ew[(size_t)(at - base) + (size_t)pe[j]] = pw[j];
+            seen.emplace(std::make_pair(pi, std::move(key)), at);
+            items[(size_t)i].ew_ofs = at;
+        }
     }
     // queue order: larger parents first
     std::vector<int> order((size_t)n);
@@ -814,7 +934,7 @@ bool run_items(const std::vector<const HostParent *> &ps, const std::vector<int>
     double *d_deltas = nullptr;
     double *d_marks = nullptr;
     std::vector<double> marks(J.marks, J.marks + LIFT_KMAX);
-    bool ok = upload_parents(ps, graph_of, meta, D, s) && upload(&d_items, items, s, "items") &&
+    bool ok = upload_parents(ps, graph_of, meta, D, s, &extra_ew) && upload(&d_items, items, s, "items") &&
               upload(&d_order, order, s, "item order") && upload(&d_verts, item_verts, s, "item vertices") &&
               upload(&d_marks, marks, s, "marker weights") && dalloc(&d_queue, 1, "work queue") &&
               dalloc(&d_flags, (size_t)n, "item flags") &&
@@ -1001,7 +1121,7 @@ namespace dh {
 namespace {
 
 std::mutex g_lift_mu;
-int g_lift_mode = 0;   // 0 batched when the parent fits LDS, 1 per-subgraph fits
+int g_lift_mode = 0;   // 0 batched when the parent fits LDS else masked, 1 per-subgraph fits, 2 masked
 
 bool lds_eligible(const HostParent &P) { return lds_eligible_size(P.N, P.E); }
 
@@ -1198,6 +1318,31 @@ int64_t *lifted_flat(int kind, p_dress_graph_t g, int k, int iterations, double 
         if (!ok) return nullptr;
         record_lifted(out.reruns, out.device_ms, (int64_t)sel.size(), out.val_ofs.back());
         return flat;
+    } else if (mode == 0 || mode == 2) {
+        // parent beyond LDS (or forced): masked fits of the resident device graph
+        std::vector<int64_t> live = kind == LIFT_DELTA ? delta_live_counts(g, verts, k, sel.size())
+                                                       : std::vector<int64_t>(sel.size(), (int64_t)E);
+        int64_t total_live = 0;
+        for (int64_t x : live) total_live += x;
+        double *d_vals = nullptr;
+        int64_t nv = 0;
+        EwPatches ep;
+        if (kind == LIFT_DELTA) ep = delta_patches(P, verts, k, sel.size());
+        std::vector<double> parent_w(ep.e.size());
+        for (size_t j = 0; j < ep.e.size(); j++) parent_w[j] = P.ew[(size_t)ep.e[j]];
+        bool ok = dalloc(&d_vals, (size_t)total_live + 1, "subgraph values") &&
+                  masked_items(g, G, kind, k, verts, live, sel, kind == LIFT_NABLA ? J.marks : nullptr, iterations,
+                               epsilon, precision, d_vals, total_live, &nv, host_ms, ep.off, ep.e, ep.w, parent_w);
+        if (ok && nv != total_live) {
+            set_error("masked subgraph fits: live value count mismatch");
+            ok = false;
+        }
+        int64_t *flat = nullptr;
+        if (ok && do_hist) ok = histogram_flat(d_vals, nv, s, &flat, hist_size);
+        dfree(d_vals);
+        if (!ok) return nullptr;
+        record_lifted(0, 0.0, (int64_t)sel.size(), nv);
+        return flat;
     } else {
         std::vector<double> vals;
         for (size_t i = 0; i < sel.size(); i++) {
@@ -1376,7 +1521,7 @@ int64_t *dress_nabla_fit_strided_flat(p_dress_graph_t g, int k, int iterations, 
 void dress_hip_set_lifted_mode(int mode)
 {
     std::lock_guard<std::mutex> lk(g_lift_mu);
-    g_lift_mode = mode == 1 ? 1 : 0;
+    g_lift_mode = (mode == 1 || mode == 2) ? mode : 0;
 }
 
 }  // extern "C"

@@ -187,9 +187,14 @@ int dress_hip_get_batch(p_dress_graph_t g, int n, const int *u, const int *v,
 /* How dress_{delta,nabla}_fit_hip* run their subgraphs:
  *   0 (default) one batched persistent launch when the graph fits the
  *     kernel's LDS state (up to 65535 edges and ~150 KB of state), otherwise
- *     one device fit per subgraph;
- *   1 always one device fit per subgraph (the structure of the CUDA backend,
- *     libdress/src/cuda/delta_dress_cuda.c:24-51), kept for comparison. */
+ *     masked fits of the graph's own device copy: the parent's CSR and
+ *     intersection index serve every subgraph, delta^k marks the removed
+ *     edges dead, nabla^k re-weights the tuple's vertices;
+ *   1 always one device fit per rebuilt subgraph (the structure of the CUDA
+ *     backend, libdress/src/cuda/delta_dress_cuda.c:24-51), kept for
+ *     comparison;
+ *   2 always the masked fits (the large-graph path, forced for testing).
+ * All three give the same values bit for bit in FP64. */
 void dress_hip_set_lifted_mode(int mode);
 
 /* Host-only helper: the subgraph indices a lifted call with these arguments
