@@ -242,22 +242,28 @@ __global__ void k_order_positions(int2 *pos_in, int E)
     }
 }
 
-// internal order key: (bin, half-edge of the edge in its owner's row); the
-// owner is the endpoint of larger degree (lower id on ties), so the edges of
-// one hub are adjacent within a bin and share the hub's row of values
+// internal order key: (bin, half-edge of the edge in its owner's row).  In
+// the bins set in low_bins the owner is the endpoint of smaller degree: the
+// edges of one low-degree vertex are adjacent, its norm and row are read and
+// written in sequence, and the other endpoints' norms come from the short
+// prefix of high-degree labels that stays in L2.  Elsewhere the owner is the
+// endpoint of larger degree, so the edges of one hub share its row of values.
+// Ties go to the lower id.
 __global__ void k_order_keys(const int64_t *cnt_in, const int2 *pos_in, const int *U, const int *V,
-                             const int *row_off, int E, uint64_t *key, int *val)
+                             const int *row_off, int E, int low_bins, uint64_t *key, int *val)
 {
     for (int e = blockIdx.x * blockDim.x + threadIdx.x; e < E; e += gridDim.x * blockDim.x) {
         int2 p = pos_in[e];
         int own = p.x;
+        const int bin = term_bin(cnt_in[e] + 1);
         if (p.y >= 0) {
             int u = U[e], v = V[e];
             int du = row_off[u + 1] - row_off[u], dv = row_off[v + 1] - row_off[v];
-            int o = (du > dv || (du == dv && u < v)) ? u : v;
+            bool u_high = du > dv || (du == dv && u < v);
+            int o = (((low_bins >> bin) & 1) ? !u_high : u_high) ? u : v;
             own = (p.x >= row_off[o] && p.x < row_off[o + 1]) ? p.x : p.y;
         }
-        key[e] = ((uint64_t)term_bin(cnt_in[e] + 1) << 32) | (uint32_t)own;
+        key[e] = ((uint64_t)bin << 32) | (uint32_t)own;
         val[e] = e;
     }
 }
@@ -448,6 +454,15 @@ __global__ void k_pairs_to_edges(const int2 *pairs, int64_t T, const int *adj_e,
 // fit in one XCD's L2, and the extra sort would only add build latency.
 // DRESS_HIP_RELABEL=always|never overrides.
 constexpr int64_t RELABEL_MIN_HALF_EDGES = 1 << 18;
+
+// bins whose edges are ordered by their low-degree endpoint (bit b = bin b);
+// DRESS_HIP_LOW_ORDER_BINS overrides the mask
+int low_order_bins()
+{
+    const char *e = getenv("DRESS_HIP_LOW_ORDER_BINS");
+    if (e && *e) return (int)strtol(e, nullptr, 0) & 0x1f;
+    return 0;
+}
 
 bool want_relabel(int64_t S)
 {
@@ -675,7 +690,7 @@ bool build_index(DevGraph *G)
                    dalloc(&v0, (size_t)E + 1, "order vals") && dalloc(&cd, Bins::NB, "bin counts");
         if (ok2) {
             hipLaunchKernelGGL(k_order_keys, dim3(grid_for(E, 8192)), dim3(TPB), 0, s, G->cnt_in, G->pos_in, G->U, G->V,
-                               G->row_off, E, k0, v0);
+                               G->row_off, E, low_order_bins(), k0, v0);
             ok2 = prim_sort_pairs_u64_i32(k0, k1, v0, G->ord, (size_t)E, 64, s);
         }
         ok2 = ok2 && hipMemsetAsync(cd, 0, Bins::NB * sizeof(int), s) == hipSuccess;

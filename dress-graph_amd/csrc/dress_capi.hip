@@ -11,6 +11,8 @@
 #include <cstdio>
 #include <cstdlib>
 #include <cstring>
+#include <thread>
+#include <chrono>
 #include <sys/mman.h>
 #include <mutex>
 #include <unordered_map>
@@ -131,6 +133,32 @@ void ctx_end()
     t_ctx.stream = nullptr;
 }
 
+// returns the context's cached free blocks larger than max_block to the
+// device once the cached free bytes exceed keep_bytes, so memory another
+// thread was admitted for is not held here (hipFree waits for the device,
+// so it is kept off the common path)
+void ctx_trim(size_t max_block, size_t keep_bytes)
+{
+    if (!t_ctx.active) return;
+    size_t cached = 0;
+    for (auto &kv : t_ctx.free_blocks) cached += kv.first * kv.second.size();
+    if (cached <= keep_bytes) return;
+    bool synced = false;
+    for (auto &kv : t_ctx.free_blocks) {
+        if (kv.first <= max_block || kv.second.empty()) continue;
+        if (!synced && t_ctx.stream) {
+            hipStreamSynchronize(t_ctx.stream);
+            synced = true;
+        }
+        for (void *q : kv.second) {
+            hipFree(q);
+            t_ctx.size_of.erase(q);
+            t_ctx.bytes -= (int64_t)kv.first;
+        }
+        kv.second.clear();
+    }
+}
+
 hipStream_t ctx_stream() { return t_ctx.active ? t_ctx.stream : nullptr; }
 int64_t ctx_cached_bytes() { return t_ctx.active ? t_ctx.bytes : 0; }
 
@@ -178,6 +206,24 @@ static int g_precision = -1;
 static thread_local int t_precision = -1;
 
 void set_thread_precision(int precision) { t_precision = precision; }
+
+// share of the device (percent of its resident workgroups) the calling
+// thread's multi-subgraph passes may fill
+static thread_local int t_grid_share = 100;
+void set_thread_grid_share(int percent) { t_grid_share = percent < 1 ? 1 : (percent > 100 ? 100 : percent); }
+int thread_grid_share() { return t_grid_share; }
+
+// requests of higher priority waiting or running (read by the calling
+// thread's long loops between passes)
+static thread_local const std::atomic<int> *t_pressure = nullptr;
+void set_thread_pressure(const std::atomic<int> *p) { t_pressure = p; }
+void yield_to_pressure(double max_seconds)
+{
+    if (!t_pressure) return;
+    const auto until = std::chrono::steady_clock::now() + std::chrono::microseconds((int64_t)(max_seconds * 1e6));
+    while (t_pressure->load(std::memory_order_relaxed) > 0 && std::chrono::steady_clock::now() < until)
+        std::this_thread::sleep_for(std::chrono::microseconds(200));
+}
 
 static thread_local const std::atomic<bool> *t_cancel = nullptr;
 

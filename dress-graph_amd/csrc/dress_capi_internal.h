@@ -24,6 +24,17 @@ bool thread_cancelled();
 void record_stats(DevGraph *G, const FitResult &r);
 void record_lifted(int reruns, double device_ms, int64_t subgraphs, int64_t values);
 bool lds_eligible_size(int N, int E);
+// dress_multi.hip: delta/nabla items fitted many at a time on the parent's device graph
+void set_thread_grid_share(int percent);
+void set_thread_pressure(const std::atomic<int> *p);
+void yield_to_pressure(double max_seconds);
+int thread_grid_share();
+int multi_max_k();
+int multi_slots(const DevGraph *G);
+size_t multi_pass_bytes(const DevGraph *G);
+bool multi_items(DevGraph *G, int kind, int k, const std::vector<int> &verts, const std::vector<int64_t> &live,
+                 const std::vector<int64_t> &sel, const double *marks, int max_it, double eps, double *d_vals,
+                 double *ms_rows);
 // dress_masked.hip: delta/nabla items as masked fits of the parent's device graph
 bool masked_items(p_dress_graph_t g, DevGraph *G, int kind, int k, const std::vector<int> &verts,
                   const std::vector<int64_t> &live, const std::vector<int64_t> &sel, const double *marks, int max_it,

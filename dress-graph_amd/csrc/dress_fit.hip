@@ -25,6 +25,7 @@
 // Work is degree binned: rows/edges with <= 4, 32, 256, 4096 terms use groups
 // of 1, 4, 16, 64 lanes; larger ones a 256-lane workgroup.  Edges of one bin
 // are a contiguous range of internal ids.
+#include "dress_device.h"
 #include "dress_hip_internal.h"
 
 #include <algorithm>
@@ -56,35 +57,6 @@ namespace {
 
 constexpr int TPB = 256;
 
-__device__ __forceinline__ void two_sum_add(double &s, double &c, double x)
-{
-    double t = s + x;
-    double z = t - s;
-    double e = (s - (t - z)) + (x - z);
-    s = t;
-    c += e;
-}
-
-__device__ __forceinline__ void two_sum_merge(double &s, double &c, double s2, double c2)
-{
-    double t = s + s2;
-    double z = t - s;
-    double e = (s - (t - z)) + (s2 - z);
-    s = t;
-    c = (c + c2) + e;
-}
-
-template <int G>
-__device__ __forceinline__ void group_reduce(double &s, double &c)
-{
-#pragma unroll
-    for (int o = G / 2; o > 0; o >>= 1) {
-        double s2 = __shfl_xor(s, o, G);
-        double c2 = __shfl_xor(c, o, G);
-        two_sum_merge(s, c, s2, c2);
-    }
-}
-
 // workgroup-wide merge of (s, c); result valid in thread 0
 __device__ __forceinline__ void block_reduce(double &s, double &c)
 {
@@ -104,19 +76,6 @@ __device__ __forceinline__ void wave_max_atomic(double m, unsigned long long *ou
     for (int o = 32; o > 0; o >>= 1) m = fmax(m, __shfl_xor(m, o));
     if ((threadIdx.x & 63) == 0 && m > 0.0)
         atomicMax(out, (unsigned long long)__double_as_longlong(m));
-}
-
-template <typename T>
-__device__ __forceinline__ T ntl(const T *p) { return __builtin_nontemporal_load(p); }
-
-typedef int v2i __attribute__((ext_vector_type(2)));
-__device__ __forceinline__ int2 ntl2(const int2 *p)
-{
-    v2i v = __builtin_nontemporal_load(reinterpret_cast<const v2i *>(p));
-    int2 r;
-    r.x = v.x;
-    r.y = v.y;
-    return r;
 }
 
 struct EdgeArgs {

@@ -1121,7 +1121,8 @@ namespace dh {
 namespace {
 
 std::mutex g_lift_mu;
-int g_lift_mode = 0;   // 0 batched when the parent fits LDS else masked, 1 per-subgraph fits, 2 masked
+int g_lift_mode = 0;   // 0 batched when the parent fits LDS else multi/masked, 1 per-subgraph fits,
+                       // 2 masked one at a time, 3 multi-subgraph passes
 
 bool lds_eligible(const HostParent &P) { return lds_eligible_size(P.N, P.E); }
 
@@ -1318,8 +1319,11 @@ int64_t *lifted_flat(int kind, p_dress_graph_t g, int k, int iterations, double 
         if (!ok) return nullptr;
         record_lifted(out.reruns, out.device_ms, (int64_t)sel.size(), out.val_ofs.back());
         return flat;
-    } else if (mode == 0 || mode == 2) {
-        // parent beyond LDS (or forced): masked fits of the resident device graph
+    } else if (mode == 0 || mode == 2 || mode == 3) {
+        // parent beyond LDS (or forced): the parent's resident device graph
+        // serves every subgraph -- many subgraphs per pass, or one masked
+        // fit at a time for tuples longer than a pass takes, DIRECTED
+        // reciprocal weight patches, or graphs whose pass state is too large
         std::vector<int64_t> live = kind == LIFT_DELTA ? delta_live_counts(g, verts, k, sel.size())
                                                        : std::vector<int64_t>(sel.size(), (int64_t)E);
         int64_t total_live = 0;
@@ -1328,11 +1332,18 @@ int64_t *lifted_flat(int kind, p_dress_graph_t g, int k, int iterations, double 
         int64_t nv = 0;
         EwPatches ep;
         if (kind == LIFT_DELTA) ep = delta_patches(P, verts, k, sel.size());
-        std::vector<double> parent_w(ep.e.size());
-        for (size_t j = 0; j < ep.e.size(); j++) parent_w[j] = P.ew[(size_t)ep.e[j]];
-        bool ok = dalloc(&d_vals, (size_t)total_live + 1, "subgraph values") &&
-                  masked_items(g, G, kind, k, verts, live, sel, kind == LIFT_NABLA ? J.marks : nullptr, iterations,
-                               epsilon, precision, d_vals, total_live, &nv, host_ms, ep.off, ep.e, ep.w, parent_w);
+        const bool multi = mode != 2 && !ep.any() && k <= multi_max_k() && build_bins(G) && multi_slots(G) > 0;
+        bool ok = dalloc(&d_vals, (size_t)total_live + 1, "subgraph values");
+        if (ok && multi) {
+            ok = multi_items(G, kind, k, verts, live, sel, kind == LIFT_NABLA ? J.marks : nullptr, iterations,
+                             epsilon, d_vals, host_ms);
+            nv = total_live;
+        } else if (ok) {
+            std::vector<double> parent_w(ep.e.size());
+            for (size_t j = 0; j < ep.e.size(); j++) parent_w[j] = P.ew[(size_t)ep.e[j]];
+            ok = masked_items(g, G, kind, k, verts, live, sel, kind == LIFT_NABLA ? J.marks : nullptr, iterations,
+                              epsilon, precision, d_vals, total_live, &nv, host_ms, ep.off, ep.e, ep.w, parent_w);
+        }
         if (ok && nv != total_live) {
             set_error("masked subgraph fits: live value count mismatch");
             ok = false;
@@ -1521,7 +1532,7 @@ int64_t *dress_nabla_fit_strided_flat(p_dress_graph_t g, int k, int iterations, 
 void dress_hip_set_lifted_mode(int mode)
 {
     std::lock_guard<std::mutex> lk(g_lift_mu);
-    g_lift_mode = (mode == 1 || mode == 2) ? mode : 0;
+    g_lift_mode = (mode >= 1 && mode <= 3) ? mode : 0;
 }
 
 }  // extern "C"

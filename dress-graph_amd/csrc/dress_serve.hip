@@ -41,6 +41,8 @@ struct dress_hip_server {
     bool stop = false;
     std::atomic<bool> cancel{false};    // running requests give up at the next subgraph
     int64_t next_id = 0;
+    int64_t reserved_large = 0;         // part of st.reserved_bytes held by the large lane
+    std::atomic<int> short_load{0};     // queued + running requests outside the large lane
     dress_hip_server_stats_t st{};
 };
 
@@ -74,7 +76,11 @@ int64_t count_items(int kind, int N, int k)
 
 // Device bytes a request may hold at its peak: graph build (half-edge keys,
 // both CSRs, intersection index of at most T_bound entries) and fit state;
-// lifted requests add their subgraph values and the histogram sort.
+// lifted requests add their subgraph values and the histogram assembly
+// (values, two key buffers, two segment buffers, run flags and positions,
+// unique values, run starts and counts: 72 B per value, 96 with the sort's
+// temporary storage) and, beyond the batched kernel, one multi-subgraph
+// pass (64 slots of values, half-edge copies and norms).
 int64_t footprint(const ServeReq &q, int64_t t_bound)
 {
     const int64_t N = q.r.N, E = q.r.E;
@@ -82,7 +88,8 @@ int64_t footprint(const ServeReq &q, int64_t t_bound)
     if (q.r.kind == DRESS_HIP_REQ_FIT) return fit;
     int64_t items = count_items(q.r.kind, q.r.N, q.r.k);
     double vals = (double)items * (double)E;
-    double extra = q.lds ? 80.0 * vals : 48.0 * vals;
+    double extra = 96.0 * vals;
+    if (!q.lds) extra += 512.0 * (8.0 * (double)E + (double)N + 8.0);
     double b = (double)fit + extra;
     return b > 9.0e18 ? (int64_t)9.0e18 : (int64_t)b;
 }
@@ -316,13 +323,17 @@ int lane_of(const ServeReq *q)
 }
 
 // lane of worker i: one per lane first, then device-wide fits twice as
-// often as the other lanes
+// often as the batched lanes; the large delta/nabla lane keeps one worker
 int worker_lane(int i)
 {
     static const int first[4] = {LANE_FIT_BATCHED, LANE_FIT, LANE_LIFT_BATCHED, LANE_LIFT_LARGE};
-    static const int cycle[4] = {LANE_FIT, LANE_FIT, LANE_LIFT_BATCHED, LANE_LIFT_LARGE};
+    static const int cycle[4] = {LANE_FIT, LANE_FIT, LANE_LIFT_BATCHED, LANE_FIT_BATCHED};
     return i < 4 ? first[i] : cycle[(i - 4) % 4];
 }
+
+// the large delta/nabla lane holds at most this share of the budget, so the
+// short requests of the other lanes always find room
+constexpr double LARGE_BUDGET_SHARE = 0.5;
 
 // most work one batched launch takes on: edges of its graphs for fits,
 // subgraph values for delta/nabla (keeps each launch to tens of ms)
@@ -334,8 +345,12 @@ bool take(dress_hip_server *S, int lane, bool may_take_large, std::vector<ServeR
 {
     if (S->queue.empty()) return false;
     const int64_t room = S->st.budget_bytes - S->st.reserved_bytes;
+    const int64_t large_room = (int64_t)(LARGE_BUDGET_SHARE * (double)S->st.budget_bytes) - S->reserved_large;
     const bool idle = S->st.running == 0;
-    auto admissible = [&](const ServeReq *q) { return q->bytes <= room || idle; };
+    auto admissible = [&](const ServeReq *q) {
+        if (idle) return true;
+        return q->bytes <= room && (lane_of(q) != LANE_LIFT_LARGE || q->bytes <= large_room);
+    };
     auto pick = S->queue.end();
     for (auto it = S->queue.begin(); it != S->queue.end(); ++it)
         if (lane_of(*it) == lane && admissible(*it)) { pick = it; break; }
@@ -364,6 +379,7 @@ bool take(dress_hip_server *S, int lane, bool may_take_large, std::vector<ServeR
         }
     }
     S->st.reserved_bytes += bytes;
+    if (lane_of(first) == LANE_LIFT_LARGE) S->reserved_large += bytes;
     S->st.peak_reserved_bytes = std::max(S->st.peak_reserved_bytes, S->st.reserved_bytes);
     S->st.running += (int64_t)out.size();
     S->st.queued = (int64_t)S->queue.size();
@@ -378,6 +394,12 @@ void worker_main(dress_hip_server *S, int index)
     if (stream) ctx_begin(stream);
     set_thread_cancel(&S->cancel);
     const int lane = worker_lane(index);
+    // the large delta/nabla lane fills a quarter of the device at most, so
+    // the short requests of the other lanes keep finding free compute units
+    if (lane == LANE_LIFT_LARGE) {
+        set_thread_grid_share(25);
+        set_thread_pressure(&S->short_load);      // and yields to them between passes
+    }
     // without a large lane (fewer than four workers) the last worker takes
     // large requests as well
     const bool may_take_large = lane == LANE_LIFT_LARGE || (S->n_workers < 4 && index == S->n_workers - 1);
@@ -407,15 +429,21 @@ void worker_main(dress_hip_server *S, int index)
         } else {
             run_lifted_batch(batch, outs);
         }
+        // large transient blocks go back to the device before the budget
+        // they were reserved under is released, once this worker holds
+        // more than 16 GiB of them
+        ctx_trim((size_t)256 << 20, (size_t)16 << 30);
         const double fin = now_s();
         {
             std::lock_guard<std::mutex> lk(S->mu);
             S->st.reserved_bytes -= bytes;
+            if (lane_of(batch[0]) == LANE_LIFT_LARGE) S->reserved_large -= bytes;
             S->st.running -= (int64_t)batch.size();
             if (batch[0]->lds) {
                 S->st.lifted_launches++;
                 S->st.lifted_requests += (int64_t)batch.size();
             }
+            if (lane_of(batch[0]) != LANE_LIFT_LARGE) S->short_load -= (int)batch.size();
             for (auto &o : outs) {
                 o.finished = fin;
                 if (o.status == 0) S->st.completed++;
@@ -428,6 +456,7 @@ void worker_main(dress_hip_server *S, int index)
         S->cv_work.notify_all();
     }
     set_thread_cancel(nullptr);
+    set_thread_pressure(nullptr);
     if (stream) {
         ctx_end();
         hipStreamDestroy(stream);
@@ -501,11 +530,14 @@ int64_t dress_hip_server_submit(dress_hip_server_t *S, const dress_hip_request_t
     std::lock_guard<std::mutex> lk(S->mu);
     q->id = S->next_id++;
     S->st.submitted++;
-    if (q->bytes > S->st.budget_bytes) {
+    const int64_t limit = lane_of(q) == LANE_LIFT_LARGE
+                              ? (int64_t)(LARGE_BUDGET_SHARE * (double)S->st.budget_bytes)
+                              : S->st.budget_bytes;
+    if (q->bytes > limit) {
         dress_hip_response_t o = blank_response(*q);
         o.admitted = o.finished = now_s();
         fail(o, "request footprint " + std::to_string(q->bytes) + " B exceeds the server budget of " +
-                    std::to_string(S->st.budget_bytes) + " B");
+                    std::to_string(limit) + " B for its kind");
         S->st.rejected++;
         S->done.push_back(o);
         int64_t id = q->id;
@@ -513,6 +545,7 @@ int64_t dress_hip_server_submit(dress_hip_server_t *S, const dress_hip_request_t
         S->cv_done.notify_all();
         return id;
     }
+    if (lane_of(q) != LANE_LIFT_LARGE) S->short_load++;
     S->queue.push_back(q);
     S->st.queued = (int64_t)S->queue.size();
     S->st.max_queued = std::max(S->st.max_queued, S->st.queued);

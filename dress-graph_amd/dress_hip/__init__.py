@@ -112,8 +112,9 @@ def copy_probe(n_bytes=1 << 30, reps=5):
 
 def set_lifted_mode(mode):
     """0: batched persistent kernel when the graph fits its LDS state, else
-    masked fits of the graph's device copy (default); 1: one device fit per
-    rebuilt subgraph; 2: always the masked fits."""
+    fits on the graph's device copy, many subgraphs per pass (default);
+    1: one device fit per rebuilt subgraph; 2: always one-at-a-time fits on
+    the device copy; 3: always multi-subgraph passes where they apply."""
     _lib.lib().dress_hip_set_lifted_mode(int(mode))
 
 

@@ -187,14 +187,17 @@ int dress_hip_get_batch(p_dress_graph_t g, int n, const int *u, const int *v,
 /* How dress_{delta,nabla}_fit_hip* run their subgraphs:
  *   0 (default) one batched persistent launch when the graph fits the
  *     kernel's LDS state (up to 65535 edges and ~150 KB of state), otherwise
- *     masked fits of the graph's own device copy: the parent's CSR and
- *     intersection index serve every subgraph, delta^k marks the removed
- *     edges dead, nabla^k re-weights the tuple's vertices;
+ *     fits on the graph's own device copy: its CSR and intersection index
+ *     serve every subgraph, delta^k marks the removed edges dead, nabla^k
+ *     re-weights the tuple's vertices; up to 64 subgraphs share one pass
+ *     (k <= 8), one at a time otherwise;
  *   1 always one device fit per rebuilt subgraph (the structure of the CUDA
  *     backend, libdress/src/cuda/delta_dress_cuda.c:24-51), kept for
  *     comparison;
- *   2 always the masked fits (the large-graph path, forced for testing).
- * All three give the same values bit for bit in FP64. */
+ *   2 always the one-at-a-time fits on the graph's device copy;
+ *   3 always the multi-subgraph passes where they apply (2 otherwise).
+ * All give the same values bit for bit in FP64; the multi-subgraph passes
+ * run in FP64 whatever the precision setting. */
 void dress_hip_set_lifted_mode(int mode);
 
 /* Host-only helper: the subgraph indices a lifted call with these arguments
@@ -289,9 +292,11 @@ void dress_hip_fit_batch_free(dress_hip_fit_batch_t *r);
  * (dress_hip_fit_batch, dress_hip_lifted_batch).  Workers are split into
  * lanes -- batched fits, device-wide fits, batched delta/nabla, and the
  * delta/nabla requests too large for the batched kernel, which no other
- * lane takes -- so short requests keep flowing behind long ones.  A
- * request whose footprint exceeds the whole budget is answered at once with
- * an error.  A request's result does not depend on what runs beside it;
+ * lane takes -- so short requests keep flowing behind long ones.  That
+ * lane has one worker, holds at most half of the budget and fills at most a
+ * quarter of the device's compute units.  A request whose footprint exceeds
+ * the budget of its kind (the whole budget, half of it for the large lane)
+ * is answered at once with an error.  A request's result does not depend on what runs beside it;
  * FP64 results are those of the single-graph calls (dress_fit_hip_ex,
  * dress_delta_fit_hip, dress_nabla_fit_hip) bit for bit, FP32->FP64 ones
  * within 1e-9 of them. */

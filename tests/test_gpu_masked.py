@@ -1,8 +1,9 @@
-"""delta^k / nabla^k as masked fits of the parent's device graph (the path
-for graphs beyond the batched kernel's LDS state): histograms and multisets
-equal the oracle, and the per-rebuilt-subgraph device path, bit for bit in
-FP64, for every variant, weighted or not, with parent vertex weights, and
-the parent's own fit is unchanged afterwards."""
+"""delta^k / nabla^k on the parent's own device graph (the paths for graphs
+beyond the batched kernel's LDS state): one masked fit at a time (mode 2)
+and up to 64 subgraphs per pass (mode 3, the default for such graphs).
+Histograms and multisets equal the oracle, and the per-rebuilt-subgraph
+device path, bit for bit in FP64, for every variant, weighted or not, with
+parent vertex weights, and the parent's own fit is unchanged afterwards."""
 import numpy as np
 import pytest
 
@@ -26,9 +27,10 @@ def _same(a, b):
         assert np.array_equal(a[2], b[2], equal_nan=True)
 
 
+@pytest.mark.parametrize("mode", [2, 3])
 @pytest.mark.parametrize("kind", ["delta", "nabla"])
 @pytest.mark.parametrize("variant", [0, 1, 2, 3])
-def test_masked_multigraphs_match_oracle(gpu, ora, kind, variant):
+def test_masked_multigraphs_match_oracle(gpu, ora, kind, variant, mode):
     """Duplicate edges, self loops, edge and vertex weights, strides and
     sampling: the forced masked path against the oracle."""
     rng = np.random.default_rng(700 + variant)
@@ -39,37 +41,54 @@ def test_masked_multigraphs_match_oracle(gpu, ora, kind, variant):
     NW = rng.random(n) + 0.5
     for k in (1, 2, 3):
         for ns, seed, off, st in ((0, 0, 0, 1), (7, 5, 0, 1), (0, 0, 1, 3)):
-            a = _run(gpu, 2, kind, n, U, V, W, NW, variant, k=k, n_samples=ns, seed=seed,
+            a = _run(gpu, mode, kind, n, U, V, W, NW, variant, k=k, n_samples=ns, seed=seed,
                      keep_multisets=True, offset=off, stride=st)
             b = ora.lifted(kind, n, U, V, W, NW, variant=variant, k=k, n_samples=ns, seed=seed,
                            keep_multisets=True, offset=off, stride=st)
             _same(a, b)
 
 
-def test_masked_iteration_limits(gpu, ora):
+@pytest.mark.parametrize("mode", [2, 3])
+def test_masked_iteration_limits(gpu, ora, mode):
     rng = np.random.default_rng(9)
     n, m = 12, 28
     U = rng.integers(0, n, m).astype(np.int32)
     V = rng.integers(0, n, m).astype(np.int32)
     for max_it, eps in ((0, 1e-6), (1, 1e-6), (4, 0.0)):
         for kind in ("delta", "nabla"):
-            a = _run(gpu, 2, kind, n, U, V, k=1, max_iterations=max_it, epsilon=eps, keep_multisets=True)
+            a = _run(gpu, mode, kind, n, U, V, k=1, max_iterations=max_it, epsilon=eps, keep_multisets=True)
             b = ora.lifted(kind, n, U, V, k=1, max_iterations=max_it, epsilon=eps, keep_multisets=True)
             _same(a, b)
 
 
 @pytest.mark.parametrize("variant,weighted", [(0, False), (0, True), (1, True), (2, False), (3, True)])
 def test_large_parent_masked_equals_rebuilt(gpu, gen, variant, weighted):
-    """A parent beyond the LDS state (the default mode takes the masked
-    path) against one device fit per rebuilt subgraph."""
+    """A parent beyond the LDS state: the default (multi-subgraph passes)
+    and the one-at-a-time masked fits against one device fit per rebuilt
+    subgraph."""
     n, U, V, W = gen.gnp(150, 130.0, 77, weighted=weighted)
     assert not gpu.fit_batch_eligible(n, U.size)
     NW = np.linspace(0.5, 2.0, n)
-    for kind, k, ns in (("delta", 1, 0), ("delta", 2, 40), ("nabla", 1, 30)):
-        a = _run(gpu, 0, kind, n, U, V, W, NW, variant, k=k, n_samples=ns, seed=3, keep_multisets=True)
+    for kind, k, ns in (("delta", 1, 0), ("delta", 2, 40), ("nabla", 1, 30), ("nabla", 2, 70)):
         b = _run(gpu, 1, kind, n, U, V, W, NW, variant, k=k, n_samples=ns, seed=3, keep_multisets=True)
+        for mode in (0, 2):
+            a = _run(gpu, mode, kind, n, U, V, W, NW, variant, k=k, n_samples=ns, seed=3, keep_multisets=True)
+            _same(a, b)
+        assert b[3] > 0 and b[1].sum() > 0
+
+
+def test_multi_passes_partial_and_long_tuples(gpu, ora):
+    """Item counts that leave the last pass partly filled, delta items
+    without edges (skipped), and tuples longer than a pass takes (k = 9,
+    one at a time)."""
+    rng = np.random.default_rng(21)
+    n = 13
+    U = rng.integers(0, 6, 20).astype(np.int32)        # vertices 6..12 isolated
+    V = rng.integers(0, 6, 20).astype(np.int32)
+    for kind, k, ns in (("delta", 1, 0), ("delta", 2, 0), ("nabla", 1, 0), ("delta", 9, 30), ("nabla", 2, 100)):
+        a = _run(gpu, 3, kind, n, U, V, k=k, n_samples=ns, seed=4, keep_multisets=True)
+        b = ora.lifted(kind, n, U, V, k=k, n_samples=ns, seed=4, keep_multisets=True)
         _same(a, b)
-        assert a[3] > 0 and a[1].sum() > 0
 
 
 def test_parent_unchanged_after_masked_fits(gpu, gen):
