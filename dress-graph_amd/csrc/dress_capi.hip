@@ -109,6 +109,77 @@ void *host_alloc(size_t bytes)
     return p;
 }
 
+// Pinned host blocks for the batch entry points' result arrays (they have
+// their own free functions).  A device-to-host copy into pinned memory runs
+// at the link's rate; into pageable memory it is staged and several times
+// slower.  Freed blocks are kept by size class for the next call, up to
+// 8 GiB; small requests, and threads inside an allocation context, stay
+// with pageable memory.
+namespace {
+struct PinnedPool {
+    std::mutex mu;
+    std::unordered_map<void *, size_t> size_of;                 // pinned blocks handed out
+    std::unordered_map<size_t, std::vector<void *>> free_blocks;
+    size_t cached = 0;
+};
+PinnedPool &pinned_pool()
+{
+    static PinnedPool *p = new PinnedPool();   // lives to process end
+    return *p;
+}
+constexpr size_t PINNED_MIN = (size_t)1 << 20;
+constexpr size_t PINNED_CACHE = (size_t)8 << 30;
+}  // namespace
+
+void *pinned_alloc(size_t bytes)
+{
+    // a thread with an allocation context (a server worker) stays with
+    // pageable memory: pinning new blocks waits for the whole device
+    if (bytes < PINNED_MIN || ctx_stream()) return host_alloc(bytes);
+    const size_t want = size_class(bytes);
+    PinnedPool &P = pinned_pool();
+    {
+        std::lock_guard<std::mutex> lk(P.mu);
+        auto it = P.free_blocks.find(want);
+        if (it != P.free_blocks.end() && !it->second.empty()) {
+            void *q = it->second.back();
+            it->second.pop_back();
+            P.cached -= want;
+            return q;
+        }
+    }
+    void *q = nullptr;
+    if (hipHostMalloc(&q, want, hipHostMallocDefault) != hipSuccess || !q) {
+        hipGetLastError();
+        return host_alloc(bytes);
+    }
+    std::lock_guard<std::mutex> lk(P.mu);
+    P.size_of[q] = want;
+    return q;
+}
+
+void pinned_free(void *p)
+{
+    if (!p) return;
+    PinnedPool &P = pinned_pool();
+    std::unique_lock<std::mutex> lk(P.mu);
+    auto it = P.size_of.find(p);
+    if (it == P.size_of.end()) {
+        lk.unlock();
+        free(p);
+        return;
+    }
+    const size_t sz = it->second;
+    if (P.cached + sz <= PINNED_CACHE) {
+        P.free_blocks[sz].push_back(p);
+        P.cached += sz;
+        return;
+    }
+    P.size_of.erase(it);
+    lk.unlock();
+    hipHostFree(p);
+}
+
 void ctx_begin(hipStream_t stream)
 {
     ctx_end();

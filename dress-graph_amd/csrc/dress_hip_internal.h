@@ -70,6 +70,9 @@ void dfree(void *p);
 void ctx_begin(hipStream_t stream);
 void *host_alloc(size_t bytes);    // malloc-compatible (free() releases it)
 void ctx_end();                    // returns the cached blocks to the device
+// pinned host blocks for batch results; pinned_free also takes malloc'd ones
+void *pinned_alloc(size_t bytes);
+void pinned_free(void *p);
 void ctx_trim(size_t max_block, size_t keep_bytes);
 hipStream_t ctx_stream();          // nullptr when no context is active
 int64_t ctx_cached_bytes();        // bytes held by the cache (free + in use)

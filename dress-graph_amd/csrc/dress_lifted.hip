@@ -25,6 +25,8 @@
 #include <mutex>
 #include <unordered_map>
 #include <vector>
+#include <atomic>
+#include <thread>
 
 namespace dh {
 
@@ -1561,9 +1563,9 @@ void dress_hip_last_lifted_stats(dress_hip_lifted_stats_t *out)
 void dress_hip_lifted_batch_free(dress_hip_lifted_batch_t *r)
 {
     if (!r) return;
-    free(r->hist_offset); free(r->hist_value); free(r->hist_count);
-    free(r->qhist_offset); free(r->qhist_key); free(r->qhist_count);
-    free(r->qbound_offset); free(r->qbound_key); free(r->qbound_count);
+    pinned_free(r->hist_offset); pinned_free(r->hist_value); pinned_free(r->hist_count);
+    pinned_free(r->qhist_offset); pinned_free(r->qhist_key); pinned_free(r->qhist_count);
+    pinned_free(r->qbound_offset); pinned_free(r->qbound_key); pinned_free(r->qbound_count);
     memset(r, 0, sizeof(*r));
 }
 
@@ -1572,7 +1574,8 @@ int dress_hip_fit_batch_eligible(int N, int E) { return (N >= 0 && E >= 0 && lds
 void dress_hip_fit_batch_free(dress_hip_fit_batch_t *r)
 {
     if (!r) return;
-    free(r->iterations); free(r->delta); free(r->edge_dress); free(r->vertex_offset); free(r->vertex_dress);
+    free(r->iterations); free(r->delta); pinned_free(r->edge_dress); free(r->vertex_offset);
+    pinned_free(r->vertex_dress);
     memset(r, 0, sizeof(*r));
 }
 
@@ -1603,9 +1606,9 @@ int dress_hip_fit_batch(int n_graphs, const int *N, const int64_t *edge_offset, 
     out->n_graphs = n_graphs;
     out->iterations = (int *)malloc(((size_t)n_graphs + 1) * sizeof(int));
     out->delta = (double *)malloc(((size_t)n_graphs + 1) * sizeof(double));
-    out->edge_dress = (double *)malloc(((size_t)totE + 1) * sizeof(double));
+    out->edge_dress = (double *)pinned_alloc(((size_t)totE + 1) * sizeof(double));
     out->vertex_offset = (int64_t *)malloc(((size_t)n_graphs + 1) * sizeof(int64_t));
-    out->vertex_dress = (double *)malloc(((size_t)totN + 1) * sizeof(double));
+    out->vertex_dress = (double *)pinned_alloc(((size_t)totN + 1) * sizeof(double));
     if (!out->iterations || !out->delta || !out->edge_dress || !out->vertex_offset || !out->vertex_dress) {
         dress_hip_fit_batch_free(out);
         set_error("host allocation of batch results failed");
@@ -1707,44 +1710,57 @@ int dress_hip_lifted_batch(int kind, int k, int n_graphs, const int *N, const in
         set_error("stream creation failed");
         return -1;
     }
-    // parents and items
+    // parents and items: each graph on its own (host worker threads), then
+    // concatenated in graph order
     std::vector<HostParent> parents((size_t)n_graphs);
-    std::vector<int> graph_of((size_t)n_graphs), item_parent, item_verts;
-    std::vector<int64_t> item_live, ms_rows;
-    for (int gi = 0; gi < n_graphs; gi++) {
+    std::vector<std::vector<int>> g_verts((size_t)n_graphs);
+    std::vector<std::vector<int64_t>> g_live((size_t)n_graphs);
+    auto build_one = [&](int gi) {
         int64_t eb = edge_offset[gi], ee = edge_offset[gi + 1];
         int Eg = (int)(ee - eb);
         host_parent_build(N[gi], Eg, U + eb, V + eb, W ? W + eb : nullptr, (int)variant, parents[(size_t)gi]);
-        graph_of[(size_t)gi] = gi;
-        if (!lds_eligible(parents[(size_t)gi])) {
-            set_error("graph " + std::to_string(gi) + " is too large for the batched kernel");
-            if (own_stream) (void)hipStreamDestroy(s);
-            return -1;
+        if (k == 0) {                        // the graph itself
+            g_live[(size_t)gi].assign(1, Eg);
+            return;
         }
-        if (k == 0 || k >= N[gi]) {
-            if (k == 0) {                    // the graph itself
-                item_parent.push_back(gi);
-                item_live.push_back(Eg);
-            }
-            continue;
-        }
+        if (k >= N[gi]) return;
         int64_t total = kind == LIFT_DELTA ? dress_delta_binom(N[gi], k) : dress_nabla_perm_count(N[gi], k);
         std::vector<int64_t> sel;
-        std::vector<int> verts;
-        enumerate_items(kind, N[gi], k, 0, 0, 0, 1, total, sel, verts);
+        enumerate_items(kind, N[gi], k, 0, 0, 0, 1, total, sel, g_verts[(size_t)gi]);
         dress_graph_t tmp;
         memset(&tmp, 0, sizeof(tmp));
         tmp.N = N[gi];
         tmp.E = Eg;
         tmp.U = const_cast<int *>(U + eb);
         tmp.V = const_cast<int *>(V + eb);
-        std::vector<int64_t> live = kind == LIFT_DELTA ? delta_live_counts(&tmp, verts, k, sel.size())
-                                                       : std::vector<int64_t>(sel.size(), (int64_t)Eg);
-        for (size_t i = 0; i < sel.size(); i++) {
-            item_parent.push_back(gi);
-            item_live.push_back(live[i]);
+        g_live[(size_t)gi] = kind == LIFT_DELTA ? delta_live_counts(&tmp, g_verts[(size_t)gi], k, sel.size())
+                                                : std::vector<int64_t>(sel.size(), (int64_t)Eg);
+    };
+    {
+        const int nt = std::max(1, std::min({n_graphs / 32, (int)std::thread::hardware_concurrency(), 16}));
+        std::atomic<int> next{0};
+        auto worker = [&] {
+            for (int gi = next++; gi < n_graphs; gi = next++) build_one(gi);
+        };
+        std::vector<std::thread> pool;
+        for (int t = 1; t < nt; t++) pool.emplace_back(worker);
+        worker();
+        for (auto &th : pool) th.join();
+    }
+    std::vector<int> graph_of((size_t)n_graphs), item_parent, item_verts;
+    std::vector<int64_t> item_live, ms_rows;
+    for (int gi = 0; gi < n_graphs; gi++) {
+        graph_of[(size_t)gi] = gi;
+        if (!lds_eligible(parents[(size_t)gi])) {
+            set_error("graph " + std::to_string(gi) + " is too large for the batched kernel");
+            if (own_stream) (void)hipStreamDestroy(s);
+            return -1;
         }
-        item_verts.insert(item_verts.end(), verts.begin(), verts.end());
+        for (int64_t x : g_live[(size_t)gi]) {
+            item_parent.push_back(gi);
+            item_live.push_back(x);
+        }
+        item_verts.insert(item_verts.end(), g_verts[(size_t)gi].begin(), g_verts[(size_t)gi].end());
     }
     pt.mark("parents+items");
     const int kk = std::max(k, 0);
@@ -1786,6 +1802,12 @@ int dress_hip_lifted_batch(int kind, int k, int n_graphs, const int *N, const in
     std::vector<int64_t> run_ofs;
     if (ok && n_graphs > 0) ok = exact_histogram_dev(lo.d_vals, lo.val_ofs.back(), seg_ofs, s, &hv, &hc, &runs, run_ofs);
     pt.mark("exact_histogram");
+    if (ok) {
+        out->hist_value = (double *)pinned_alloc(((size_t)runs + 1) * sizeof(double));
+        out->hist_count = (int64_t *)pinned_alloc(((size_t)runs + 1) * sizeof(int64_t));
+        ok = out->hist_value && out->hist_count;
+        if (!ok) set_error("host allocation of batch results failed");
+    }
     int64_t nq = 0, nb = 0;
     int64_t *qk2 = nullptr, *qc2 = nullptr, *bk2 = nullptr, *bc2 = nullptr, *qofs = nullptr, *bofs = nullptr;
     if (ok) ok = qtable_split(qk, qc, qcap, n_graphs, s, &qk2, &qc2, &bk2, &bc2, &qofs, &bofs, &nq, &nb);
@@ -1793,15 +1815,13 @@ int dress_hip_lifted_batch(int kind, int k, int n_graphs, const int *N, const in
     out->n_graphs = n_graphs;
     if (ok) {
         out->hist_offset = (int64_t *)malloc(((size_t)n_graphs + 1) * sizeof(int64_t));
-        out->hist_value = (double *)host_alloc(((size_t)runs + 1) * sizeof(double));
-        out->hist_count = (int64_t *)host_alloc(((size_t)runs + 1) * sizeof(int64_t));
         out->qhist_offset = (int64_t *)malloc(((size_t)n_graphs + 1) * sizeof(int64_t));
         out->qbound_offset = (int64_t *)malloc(((size_t)n_graphs + 1) * sizeof(int64_t));
-        out->qhist_key = (int64_t *)host_alloc(((size_t)nq + 1) * sizeof(int64_t));
-        out->qhist_count = (int64_t *)host_alloc(((size_t)nq + 1) * sizeof(int64_t));
-        out->qbound_key = (int64_t *)host_alloc(((size_t)nb + 1) * sizeof(int64_t));
-        out->qbound_count = (int64_t *)host_alloc(((size_t)nb + 1) * sizeof(int64_t));
-        if (!out->hist_offset || !out->hist_value || !out->hist_count || !out->qhist_offset || !out->qbound_offset ||
+        out->qhist_key = (int64_t *)pinned_alloc(((size_t)nq + 1) * sizeof(int64_t));
+        out->qhist_count = (int64_t *)pinned_alloc(((size_t)nq + 1) * sizeof(int64_t));
+        out->qbound_key = (int64_t *)pinned_alloc(((size_t)nb + 1) * sizeof(int64_t));
+        out->qbound_count = (int64_t *)pinned_alloc(((size_t)nb + 1) * sizeof(int64_t));
+        if (!out->hist_offset || !out->qhist_offset || !out->qbound_offset ||
             !out->qhist_key || !out->qhist_count || !out->qbound_key || !out->qbound_count) {
             set_error("host allocation of batch results failed");
             ok = false;

@@ -227,7 +227,8 @@ void dress_hip_last_lifted_stats(dress_hip_lifted_stats_t *out);
  *     the device in integer atomics, for values farther than qtol from a bin
  *     edge; values within qtol of an edge are counted separately in
  *     qbound (their bin depends on the last bits of d).
- * Arrays are malloc'd; release them with dress_hip_lifted_batch_free.
+ * Large arrays are pinned host memory kept in a pool for later calls;
+ * release every array only with dress_hip_lifted_batch_free.
  * Returns 0, or -1 (see dress_hip_last_error). */
 typedef struct {
     int      n_graphs;
@@ -262,8 +263,9 @@ void dress_hip_lifted_batch_free(dress_hip_lifted_batch_t *r);
  * of FP64, deterministic but rounded on its own in the FP32 phase):
  * iterations and delta in the reference's convention, edge values in
  * edge order (concatenated like the input), vertex_dress (the norms of the
- * last vertex phase) concatenated at vertex_offset.  Arrays are malloc'd;
- * release them with dress_hip_fit_batch_free.  Returns 0, or -1. */
+ * last vertex phase) concatenated at vertex_offset.  Large arrays are
+ * pinned host memory kept in a pool for later calls; release every array
+ * only with dress_hip_fit_batch_free.  Returns 0, or -1. */
 typedef struct {
     int      n_graphs;
     int     *iterations;          /* [n_graphs]                               */
