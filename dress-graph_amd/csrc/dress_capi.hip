@@ -11,6 +11,8 @@
 #include <cstdio>
 #include <cstdlib>
 #include <cstring>
+#include <algorithm>
+#include <atomic>
 #include <thread>
 #include <chrono>
 #include <sys/mman.h>
@@ -178,6 +180,62 @@ void pinned_free(void *p)
     P.size_of.erase(it);
     lk.unlock();
     hipHostFree(p);
+}
+
+// Device-to-host copy of a large buffer into pageable memory (results the
+// caller frees with free()).  T host threads each take a contiguous slice
+// and move it in 8 MiB chunks through two pinned staging blocks of their
+// own on a stream of their own: the copy of chunk i+1 runs while chunk i
+// is copied into place, and the first-touch faults of the destination are
+// taken by T threads at once.  src must be complete (its stream idle).
+// Threads inside an allocation context copy directly.
+bool d2h_parallel(void *dst, const void *src, size_t bytes)
+{
+    constexpr size_t CH = (size_t)8 << 20;
+    if (bytes == 0) return true;
+    if (bytes < 8 * CH || ctx_stream())
+        return hipMemcpy(dst, src, bytes, hipMemcpyDeviceToHost) == hipSuccess;
+    const int T = (int)std::max<size_t>(1, std::min<size_t>({(size_t)8, bytes / (4 * CH),
+                                                              (size_t)std::max(1u, std::thread::hardware_concurrency())}));
+    int dev = 0;
+    hipGetDevice(&dev);
+    std::atomic<bool> ok{true};
+    auto slice = [&](int w) {
+        hipSetDevice(dev);
+        const size_t b = bytes * (size_t)w / (size_t)T, e = bytes * (size_t)(w + 1) / (size_t)T;
+        hipStream_t st = nullptr;
+        char *stage[2] = {(char *)pinned_alloc(CH), (char *)pinned_alloc(CH)};
+        hipEvent_t ev[2] = {nullptr, nullptr};
+        bool good = stage[0] && stage[1] && hipStreamCreateWithFlags(&st, hipStreamNonBlocking) == hipSuccess &&
+                    hipEventCreateWithFlags(&ev[0], hipEventDisableTiming) == hipSuccess &&
+                    hipEventCreateWithFlags(&ev[1], hipEventDisableTiming) == hipSuccess;
+        auto issue = [&](size_t at, int j) {
+            const size_t n = std::min(CH, e - at);
+            return hipMemcpyAsync(stage[j], (const char *)src + at, n, hipMemcpyDeviceToHost, st) == hipSuccess &&
+                   hipEventRecord(ev[j], st) == hipSuccess;
+        };
+        if (good && b < e) good = issue(b, 0);
+        int j = 0;
+        for (size_t at = b; good && at < e; at += CH, j ^= 1) {
+            if (at + CH < e) good = issue(at + CH, j ^ 1);
+            good = good && hipEventSynchronize(ev[j]) == hipSuccess;
+            if (good) memcpy((char *)dst + at, stage[j], std::min(CH, e - at));
+        }
+        if (st) {
+            hipStreamSynchronize(st);
+            hipStreamDestroy(st);
+        }
+        for (hipEvent_t x : ev)
+            if (x) hipEventDestroy(x);
+        pinned_free(stage[0]);
+        pinned_free(stage[1]);
+        if (!good) ok = false;
+    };
+    std::vector<std::thread> pool;
+    for (int w = 1; w < T; w++) pool.emplace_back(slice, w);
+    slice(0);
+    for (auto &th : pool) th.join();
+    return ok.load();
 }
 
 void ctx_begin(hipStream_t stream)

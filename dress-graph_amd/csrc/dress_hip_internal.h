@@ -73,6 +73,8 @@ void ctx_end();                    // returns the cached blocks to the device
 // pinned host blocks for batch results; pinned_free also takes malloc'd ones
 void *pinned_alloc(size_t bytes);
 void pinned_free(void *p);
+// large device-to-host copy into pageable memory, host-parallel
+bool d2h_parallel(void *dst, const void *src, size_t bytes);
 void ctx_trim(size_t max_block, size_t keep_bytes);
 hipStream_t ctx_stream();          // nullptr when no context is active
 int64_t ctx_cached_bytes();        // bytes held by the cache (free + in use)

@@ -582,9 +582,8 @@ bool histogram_flat(const double *vals, int64_t n, hipStream_t s, int64_t **flat
         ok = flat && dalloc(&dflat, (size_t)runs * 2, "flat histogram");
         if (ok) {
             hipLaunchKernelGGL(k_interleave, dim3(grid_for(runs, num_cus() * 8)), dim3(TPB), 0, s, dv, dc, runs, dflat);
-            ok = hipMemcpyAsync(flat, dflat, (size_t)runs * 2 * sizeof(int64_t), hipMemcpyDeviceToHost, s) ==
-                     hipSuccess &&
-                 hipStreamSynchronize(s) == hipSuccess;
+            ok = hipStreamSynchronize(s) == hipSuccess &&
+                 d2h_parallel(flat, dflat, (size_t)runs * 2 * sizeof(int64_t));
             if (!ok) set_error("histogram download failed");
         }
         dfree(dflat);
@@ -1276,9 +1275,11 @@ int64_t *lifted_flat(int kind, p_dress_graph_t g, int k, int iterations, double 
     }
     if (k >= N) return nullptr;     // delta_dress_impl.c:289-293, nabla_dress_impl.c:197-201
 
+    PhaseTimer pt("lifted_flat");
     std::vector<int64_t> sel;
     std::vector<int> verts;
     enumerate_items(kind, N, k, n_samples, seed, offset, stride, total, sel, verts);
+    pt.mark("enumerate");
     LiftJob J;
     J.kind = kind;
     J.k = k;
@@ -1290,6 +1291,7 @@ int64_t *lifted_flat(int kind, p_dress_graph_t g, int k, int iterations, double 
 
     HostParent P;
     host_parent_build(N, E, g->U, g->V, g->W, (int)g->variant, P);
+    pt.mark("parent");
     int mode;
     {
         std::lock_guard<std::mutex> lk(g_lift_mu);
@@ -1299,6 +1301,7 @@ int64_t *lifted_flat(int kind, p_dress_graph_t g, int k, int iterations, double 
         std::vector<int64_t> live = kind == LIFT_DELTA ? delta_live_counts(g, verts, k, sel.size())
                                                        : std::vector<int64_t>(sel.size(), (int64_t)E);
         std::vector<int> item_parent(sel.size(), 0);
+        pt.mark("live counts");
         double *d_ms = nullptr;
         bool ok = true;
         if (host_ms && total > 0) {
@@ -1311,11 +1314,14 @@ int64_t *lifted_flat(int kind, p_dress_graph_t g, int k, int iterations, double 
         std::vector<const HostParent *> ps{&P};
         std::vector<int> gof{0};
         ok = ok && run_items(ps, gof, item_parent, verts, live, J, d_ms, E, sel, s, out, nullptr, nullptr, 0, 0.0);
+        pt.mark("run_items");
         if (ok && host_ms && total > 0)
-            ok = hipMemcpyAsync(host_ms, d_ms, (size_t)total * (size_t)E * sizeof(double), hipMemcpyDeviceToHost,
-                                s) == hipSuccess && hipStreamSynchronize(s) == hipSuccess;
+            ok = hipStreamSynchronize(s) == hipSuccess &&
+                 d2h_parallel(host_ms, d_ms, (size_t)total * (size_t)E * sizeof(double));
+        pt.mark("multisets");
         int64_t *flat = nullptr;
         if (ok && do_hist) ok = histogram_flat(out.d_vals, out.val_ofs.back(), s, &flat, hist_size);
+        pt.mark("histogram");
         dfree(out.d_vals);
         dfree(d_ms);
         if (!ok) return nullptr;

@@ -195,6 +195,19 @@ class _BatchOwner:
             pass
 
 
+class _CFree:
+    """Frees one malloc'd block with libc free() when the last view goes."""
+
+    def __init__(self, ptr):
+        self.addr = ctypes.cast(ptr, ctypes.c_void_p).value
+
+    def __del__(self):
+        try:
+            _lib.c_free(ctypes.c_void_p(self.addr))
+        except Exception:
+            pass
+
+
 class _OwnedBuffer:
     def __init__(self, addr, count, typestr, owner):
         self.__array_interface__ = {"data": (addr, False), "shape": (count,), "typestr": typestr, "version": 3}
@@ -425,13 +438,21 @@ class Graph:
         vals = np.empty(0)
         cnts = np.empty(0, dtype=np.int64)
         if flat and n > 0:
-            arr = np.ctypeslib.as_array(flat, shape=(n,)).copy()
-            vals = arr[0::2].view(np.float64).copy()
-            cnts = arr[1::2].copy()
-        _lib.c_free(flat)
+            # zero-copy views of the malloc'd (value bits, count) pairs; the
+            # block is freed when the last view goes
+            arr = _owned_view(flat, n, "<i8", _CFree(flat))
+            vals = arr.view(np.float64)[0::2]
+            cnts = arr[1::2]
+        else:
+            _lib.c_free(flat)
         mset = None
         if keep_multisets and ms:
-            mset = np.ctypeslib.as_array(ms, shape=(ns.value * self.E,)).copy().reshape(ns.value, self.E)
+            if ns.value * self.E > 0:
+                mset = _owned_view(ms, ns.value * self.E, "<f8", _CFree(ms)).reshape(ns.value, self.E)
+            else:
+                _lib.c_free(ms)
+                mset = np.zeros((ns.value, self.E))
+        elif ms:
             _lib.c_free(ms)
         return vals, cnts, mset, ns.value
 
