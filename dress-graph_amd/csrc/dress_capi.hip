@@ -40,6 +40,7 @@ bool has_error() { return !t_err.empty(); }
 namespace {
 struct AllocCtx {
     bool active = false;
+    bool serving = false;              // a server worker's context (ctx_begin)
     hipStream_t stream = nullptr;
     std::unordered_map<void *, size_t> size_of;          // every block this context handed out
     std::unordered_map<size_t, std::vector<void *>> free_blocks;
@@ -137,7 +138,7 @@ void *pinned_alloc(size_t bytes)
 {
     // a thread with an allocation context (a server worker) stays with
     // pageable memory: pinning new blocks waits for the whole device
-    if (bytes < PINNED_MIN || ctx_stream()) return host_alloc(bytes);
+    if (bytes < PINNED_MIN || ctx_serving()) return host_alloc(bytes);
     const size_t want = size_class(bytes);
     PinnedPool &P = pinned_pool();
     {
@@ -193,7 +194,7 @@ bool d2h_parallel(void *dst, const void *src, size_t bytes)
 {
     constexpr size_t CH = (size_t)8 << 20;
     if (bytes == 0) return true;
-    if (bytes < 8 * CH || ctx_stream())
+    if (bytes < 8 * CH || ctx_serving())
         return hipMemcpy(dst, src, bytes, hipMemcpyDeviceToHost) == hipSuccess;
     const int T = (int)std::max<size_t>(1, std::min<size_t>({(size_t)8, bytes / (4 * CH),
                                                               (size_t)std::max(1u, std::thread::hardware_concurrency())}));
@@ -242,7 +243,53 @@ void ctx_begin(hipStream_t stream)
 {
     ctx_end();
     t_ctx.active = true;
+    t_ctx.serving = true;
     t_ctx.stream = stream;
+}
+
+bool ctx_serving() { return t_ctx.active && t_ctx.serving; }
+
+// A thread's block cache for the batch entry points, kept between calls:
+// between two calls it holds only free blocks whose last use finished (the
+// leaving call synchronises its stream), so the next call may reuse them on
+// any stream.  It keeps at most 8 GiB of free blocks (a batch call of
+// ~10^7 subgraph values works in ~2 GiB); the thread's exit returns them.
+namespace {
+struct BatchCache {
+    AllocCtx ctx;
+    hipStream_t stream = nullptr;
+    ~BatchCache()
+    {
+        for (auto &kv : ctx.free_blocks)
+            for (void *q : kv.second) hipFree(q);
+        if (stream) hipStreamDestroy(stream);
+    }
+};
+thread_local BatchCache t_batch;
+}  // namespace
+
+hipStream_t batch_ctx_enter()
+{
+    if (t_ctx.active) return nullptr;       // inside a server context already
+    if (!t_batch.stream && hipStreamCreateWithFlags(&t_batch.stream, hipStreamNonBlocking) != hipSuccess) {
+        t_batch.stream = nullptr;
+        hipGetLastError();
+        return nullptr;
+    }
+    std::swap(t_ctx, t_batch.ctx);
+    t_ctx.active = true;
+    t_ctx.serving = false;
+    t_ctx.stream = t_batch.stream;
+    return t_batch.stream;
+}
+
+void batch_ctx_leave()
+{
+    if (!t_ctx.active || t_ctx.serving) return;
+    hipStreamSynchronize(t_ctx.stream);
+    ctx_trim(0, (size_t)8 << 30);
+    t_ctx.active = false;
+    std::swap(t_ctx, t_batch.ctx);
 }
 
 void ctx_end()

@@ -76,6 +76,11 @@ void pinned_free(void *p);
 // large device-to-host copy into pageable memory, host-parallel
 bool d2h_parallel(void *dst, const void *src, size_t bytes);
 void ctx_trim(size_t max_block, size_t keep_bytes);
+bool ctx_serving();                // the calling thread is a server worker
+// the calling thread's block cache for batch calls (kept between calls);
+// enter returns its stream, or nullptr when a context is already active
+hipStream_t batch_ctx_enter();
+void batch_ctx_leave();
 hipStream_t ctx_stream();          // nullptr when no context is active
 int64_t ctx_cached_bytes();        // bytes held by the cache (free + in use)
 

@@ -48,6 +48,36 @@ inline int grid_for(int64_t threads, int cap)
 // with rows ordered by (neighbour, edge), variant weights, and per edge the
 // multiset intersection of its two rows as pairs of edge ids.  Same rules as
 // the device build (dress_build.hip) and the reference (dress.c:103-363).
+// The calling thread's batch block cache for the length of one batch call
+// (a no-op inside a server worker, which has its own).
+struct BatchScope {
+    bool on;
+    BatchScope() : on(batch_ctx_enter() != nullptr) {}
+    ~BatchScope()
+    {
+        if (on) batch_ctx_leave();
+    }
+    BatchScope(const BatchScope &) = delete;
+    BatchScope &operator=(const BatchScope &) = delete;
+};
+
+// Phase timings printed to stderr when DRESS_HIP_TIMING is set.
+struct PhaseTimer {
+    const char *what;
+    bool on;
+    std::chrono::steady_clock::time_point last;
+    explicit PhaseTimer(const char *w) : what(w), on(getenv("DRESS_HIP_TIMING") != nullptr),
+                                         last(std::chrono::steady_clock::now()) {}
+    void mark(const char *phase)
+    {
+        if (!on) return;
+        auto now = std::chrono::steady_clock::now();
+        fprintf(stderr, "%s: %-18s %8.3f ms\n", what, phase,
+                std::chrono::duration<double, std::milli>(now - last).count());
+        last = now;
+    }
+};
+
 struct HostParent {
     int N = 0, E = 0, S = 0, T = 0;
     int directed = 0;
@@ -469,6 +499,7 @@ __global__ void k_interleave(const double *v, const int64_t *c, int64_t runs, in
 bool exact_histogram_dev(const double *vals, int64_t n, const std::vector<int64_t> &seg_ofs, hipStream_t s,
                          double **d_val, int64_t **d_cnt, int64_t *runs_out, std::vector<int64_t> &run_ofs)
 {
+    PhaseTimer pt("  exact_histogram_dev");
     *d_val = nullptr;
     *d_cnt = nullptr;
     *runs_out = 0;
@@ -483,6 +514,7 @@ bool exact_histogram_dev(const double *vals, int64_t n, const std::vector<int64_
               dalloc(&g0, (size_t)n, "segments") && dalloc(&g1, (size_t)n, "segments") &&
               dalloc(&flag, (size_t)n + 1, "run flags") && dalloc(&pos, (size_t)n + 1, "run positions") &&
               dalloc(&dofs, (size_t)nseg + 1, "segment offsets") && dalloc(&rofs, (size_t)nseg + 1, "run offsets");
+    pt.mark("alloc");
     int grid = grid_for(n, num_cus() * 8);
     int64_t runs = 0;
     if (ok) {
@@ -501,6 +533,7 @@ bool exact_histogram_dev(const double *vals, int64_t n, const std::vector<int64_
             std::swap(g0, g1);
         }
         // now k0 / g0 hold the sorted (segment, value) keys
+        if (pt.on) { hipStreamSynchronize(s); pt.mark("sorts"); }
         if (ok) {
             hipLaunchKernelGGL(k_run_flags, dim3(grid), dim3(TPB), 0, s, k0, g0, n, flag);
             hipLaunchKernelGGL(k_zero_int, dim3(1), dim3(64), 0, s, flag + n, 1);
@@ -510,6 +543,7 @@ bool exact_histogram_dev(const double *vals, int64_t n, const std::vector<int64_
             ok = hipMemcpyAsync(&runs, pos + n, sizeof(int64_t), hipMemcpyDeviceToHost, s) == hipSuccess &&
                  hipStreamSynchronize(s) == hipSuccess;
         }
+        pt.mark("flags+scan");
         if (ok)
             ok = dalloc(&uval, (size_t)runs + 1, "histogram values") &&
                  dalloc(&ustart, (size_t)runs + 1, "histogram runs") && dalloc(&usg, (size_t)runs + 1, "histogram segs") &&
@@ -526,8 +560,10 @@ bool exact_histogram_dev(const double *vals, int64_t n, const std::vector<int64_
         }
         if (!ok && !has_error()) set_error("histogram assembly failed");
     }
+    pt.mark("runs");
     dfree(k0); dfree(k1); dfree(g0); dfree(g1); dfree(flag); dfree(pos); dfree(ustart); dfree(dofs);
     dfree(usg); dfree(rofs);
+    pt.mark("free");
     if (!ok) {
         dfree(uval);
         dfree(cnt);
@@ -793,23 +829,6 @@ struct FitOutputs {
     std::vector<double> *deltas;
 };
 
-// Phase timings printed to stderr when DRESS_HIP_TIMING is set.
-struct PhaseTimer {
-    const char *what;
-    bool on;
-    std::chrono::steady_clock::time_point last;
-    explicit PhaseTimer(const char *w) : what(w), on(getenv("DRESS_HIP_TIMING") != nullptr),
-                                         last(std::chrono::steady_clock::now()) {}
-    void mark(const char *phase)
-    {
-        if (!on) return;
-        auto now = std::chrono::steady_clock::now();
-        fprintf(stderr, "%s: %-18s %8.3f ms\n", what, phase,
-                std::chrono::duration<double, std::milli>(now - last).count());
-        last = now;
-    }
-};
-
 int forced_block()
 {
     const char *b = getenv("DRESS_HIP_LIFT_BLOCK");
@@ -935,6 +954,7 @@ bool run_items(const std::vector<const HostParent *> &ps, const std::vector<int>
     double *d_deltas = nullptr;
     double *d_marks = nullptr;
     std::vector<double> marks(J.marks, J.marks + LIFT_KMAX);
+    PhaseTimer pt("  run_items");
     bool ok = upload_parents(ps, graph_of, meta, D, s, &extra_ew) && upload(&d_items, items, s, "items") &&
               upload(&d_order, order, s, "item order") && upload(&d_verts, item_verts, s, "item vertices") &&
               upload(&d_marks, marks, s, "marker weights") && dalloc(&d_queue, 1, "work queue") &&
@@ -943,6 +963,7 @@ bool run_items(const std::vector<const HostParent *> &ps, const std::vector<int>
     if (ok && fo) ok = dalloc(&d_iters, (size_t)n, "item iterations") && dalloc(&d_deltas, (size_t)n, "item deltas");
     hipEvent_t e0 = nullptr, e1 = nullptr;
     if (ok) ok = hipEventCreate(&e0) == hipSuccess && hipEventCreate(&e1) == hipSuccess;
+    if (pt.on) { hipStreamSynchronize(s); pt.mark("uploads+allocs"); }
     if (ok) {
         LiftArgs a;
         memset(&a, 0, sizeof(a));
@@ -1061,6 +1082,7 @@ bool run_items(const std::vector<const HostParent *> &ps, const std::vector<int>
             return true;
         };
         ok = hipMemsetAsync(d_flags, 0, (size_t)n * sizeof(int), s) == hipSuccess && launch_ids(d_order, order);
+        pt.mark("launches");
         // FP64 redo of ambiguous mixed-precision items
         if (ok && J.mixed) {
             std::vector<int> flags((size_t)n);
@@ -1102,9 +1124,11 @@ bool run_items(const std::vector<const HostParent *> &ps, const std::vector<int>
     }
     if (e0) (void)hipEventDestroy(e0);
     if (e1) (void)hipEventDestroy(e1);
+    pt.mark("reruns");
     dfree(d_items); dfree(d_order); dfree(d_verts); dfree(d_marks); dfree(d_queue); dfree(d_flags);
     dfree(d_iters); dfree(d_deltas);
     D.release();
+    pt.mark("free");
     if (!ok) {
         if (!has_error()) set_error("lifted launch failed");
         dfree(out.d_vals);
@@ -1262,6 +1286,9 @@ int64_t *lifted_flat(int kind, p_dress_graph_t g, int k, int iterations, double 
     DevGraph *G = device_graph(g);
     if (!G) return nullptr;
     s = G->stream;
+    // work buffers of this call from the thread's batch block cache (the
+    // device graph itself was built above, outside it)
+    BatchScope scope;
     if (k == 0) {
         // delta_dress_impl.c:268-286: fit g itself in place
         if (offset != 0) return nullptr;
@@ -1629,6 +1656,7 @@ int dress_hip_fit_batch(int n_graphs, const int *N, const int64_t *edge_offset, 
         for (int64_t v = 0; v < totN; v++) out->vertex_dress[v] = 0.0;
         return 0;
     }
+    BatchScope scope;
     hipStream_t s = ctx_stream();
     const bool own_stream = s == nullptr;
     if (own_stream && hipStreamCreateWithFlags(&s, hipStreamNonBlocking) != hipSuccess) {
@@ -1710,6 +1738,7 @@ int dress_hip_lifted_batch(int kind, int k, int n_graphs, const int *N, const in
     }
     auto t0 = std::chrono::steady_clock::now();
     PhaseTimer pt("dress_hip_lifted_batch");
+    BatchScope scope;
     hipStream_t s = ctx_stream();
     const bool own_stream = s == nullptr;
     if (own_stream && hipStreamCreateWithFlags(&s, hipStreamNonBlocking) != hipSuccess) {

@@ -14,6 +14,7 @@ libdress_hip.so's C ABI; there is no CPU path.  ``is_available()`` probes
 for the library and a GPU.
 """
 import ctypes
+from collections.abc import Sequence
 from dataclasses import dataclass, field
 from typing import List, Optional, Tuple
 
@@ -92,9 +93,9 @@ def last_fit_stats():
 @dataclass
 class LiftedBatchResult:
     """Per-graph outputs of :func:`lifted_batch`."""
-    histograms: List[Tuple[np.ndarray, np.ndarray]]     # exact (values, counts)
-    quantised: List[Tuple[np.ndarray, np.ndarray]]      # floor(d/eps) keys, counts
-    boundary: List[Tuple[np.ndarray, np.ndarray]]       # keys, counts of values near a bin edge
+    histograms: Sequence        # per graph: exact (values, counts)
+    quantised: Sequence         # per graph: floor(d/eps) keys, counts
+    boundary: Sequence          # per graph: keys, counts of values near a bin edge
     subgraphs: int
     values: int
     fp64_reruns: int
@@ -173,13 +174,33 @@ def lifted_batch(graphs, k=1, kind="delta", variant=UNDIRECTED, max_iterations=1
     def split(off_p, a_p, b_p, typestr_a):
         off = np.ctypeslib.as_array(off_p, shape=(n + 1,)).copy()
         tot = int(off[-1])
-        a = _owned_view(a_p, tot, typestr_a, owner)
-        b = _owned_view(b_p, tot, "<i8", owner)
-        return [(a[off[i]:off[i + 1]], b[off[i]:off[i + 1]]) for i in range(n)]
+        return _PerGraph(off, _owned_view(a_p, tot, typestr_a, owner), _owned_view(b_p, tot, "<i8", owner))
     hists = split(r.hist_offset, r.hist_value, r.hist_count, "<f8")
     quant = split(r.qhist_offset, r.qhist_key, r.qhist_count, "<i8")
     bound = split(r.qbound_offset, r.qbound_key, r.qbound_count, "<i8")
     return LiftedBatchResult(hists, quant, bound, r.subgraphs, r.values, r.fp64_reruns, r.device_ms, r.total_ms)
+
+
+class _PerGraph(Sequence):
+    """Per-graph (a, b) views of two concatenated result arrays, made on
+    access: item i is (a[off[i]:off[i+1]], b[off[i]:off[i+1]])."""
+
+    def __init__(self, off, a, b):
+        self._off, self._a, self._b = off, a, b
+
+    def __len__(self):
+        return len(self._off) - 1
+
+    def __getitem__(self, i):
+        if isinstance(i, slice):
+            return [self[j] for j in range(*i.indices(len(self)))]
+        n = len(self)
+        if i < 0:
+            i += n
+        if not 0 <= i < n:
+            raise IndexError(i)
+        lo, hi = int(self._off[i]), int(self._off[i + 1])
+        return self._a[lo:hi], self._b[lo:hi]
 
 
 class _BatchOwner:
