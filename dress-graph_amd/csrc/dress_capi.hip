@@ -387,12 +387,24 @@ void set_thread_precision(int precision) { t_precision = precision; }
 // thread's multi-subgraph passes may fill
 static thread_local int t_grid_share = 100;
 void set_thread_grid_share(int percent) { t_grid_share = percent < 1 ? 1 : (percent > 100 ? 100 : percent); }
-int thread_grid_share() { return t_grid_share; }
+static thread_local const std::atomic<int> *t_pressure_for_share = nullptr;
+// the share applies while requests of higher priority are waiting or
+// running (see set_thread_pressure); otherwise the whole device
+int thread_grid_share()
+{
+    if (t_grid_share < 100 && t_pressure_for_share && t_pressure_for_share->load(std::memory_order_relaxed) == 0)
+        return 100;
+    return t_grid_share;
+}
 
 // requests of higher priority waiting or running (read by the calling
 // thread's long loops between passes)
 static thread_local const std::atomic<int> *t_pressure = nullptr;
-void set_thread_pressure(const std::atomic<int> *p) { t_pressure = p; }
+void set_thread_pressure(const std::atomic<int> *p)
+{
+    t_pressure = p;
+    t_pressure_for_share = p;
+}
 void yield_to_pressure(double max_seconds)
 {
     if (!t_pressure) return;

@@ -406,7 +406,7 @@ bool run_passes(DevGraph *G, int kind, int k, const std::vector<int> &verts, con
     a.tup = d_tup;
     a.marks = d_marks;
     a.st = d_st;
-    const int cap = std::max(1, num_cus() * 8 * thread_grid_share() / 100);
+    int cap = std::max(1, num_cus() * 8 * thread_grid_share() / 100);
     std::vector<int> slot_item, tin;
     std::vector<int64_t> sofs, slim;
     std::vector<SlotState> hst(MB);
@@ -450,6 +450,7 @@ bool run_passes(DevGraph *G, int kind, int k, const std::vector<int> &verts, con
         int launched = 0, remaining = n, chunk = 4;
         while (max_it > 0 && remaining > 0 && launched < max_it) {
             if (launched > 0) yield_to_pressure(0.05);
+            cap = std::max(1, num_cus() * 8 * thread_grid_share() / 100);
             const int m = std::min(chunk, max_it - launched);
             for (int j = 0; j < m; j++) {
                 if (G->packed) Pass<MB>::template iteration<true>(G, a, cap);
