@@ -12,6 +12,9 @@ The same line also carries
   * delta^1 of the 512 random 6-regular graphs (n ~ U{16..128}): subgraph
     solves/s of the batched kernel (FP64 and FP32->FP64) beside the plain
     FP64 GPU path (one device fit per subgraph, the CUDA backend's structure),
+  * delta^1 of a 2,000-vertex G(n, p) graph too large for the batched
+    kernel: 64-subgraph passes on its device copy beside one device fit per
+    rebuilt subgraph,
   * delta^2 and nabla^1 of a random 12-regular graph on 256 vertices,
   * the roofline of the dominant kernel (HIP events around its launches),
   * the serving stream: Poisson arrivals at 2000 requests/s (70 % fits, 30 %
@@ -104,6 +107,32 @@ def cpu_baseline_delta1(graphs):
     return solves / (time.perf_counter() - t0), lib.which
 
 
+def delta1_large(dh, gen):
+    """delta^1 of a graph beyond the batched kernel's LDS state: passes of 64
+    subgraphs on the graph's own device copy, against one device fit per
+    rebuilt subgraph (a sample)."""
+    nl, Ul, Vl, Wl = gen.gnp(2000, 10.0, 5, weighted=True)
+    big = {"workload": f"delta^1 of a weighted G(n, p) graph, n=2,000, mean degree 10 (E={Ul.size}), seed 5, "
+                       "eps 1e-6"}
+    with dh.Graph(nl, Ul, Vl, Wl) as gl:
+        gl.lifted("delta", 1, n_samples=64, seed=1)                 # warm-up
+        t0 = time.perf_counter()
+        _, cnts, _, num = gl.lifted("delta", 1)
+        wall = time.perf_counter() - t0
+        big["multi_pass_fp64"] = {"solves_per_s": num / wall, "subgraphs": int(num), "values": int(cnts.sum()),
+                                  "wall_ms": wall * 1e3}
+        dh.set_lifted_mode(1)
+        try:
+            t0 = time.perf_counter()
+            gl.lifted("delta", 1, n_samples=200, seed=2)
+            wall = time.perf_counter() - t0
+        finally:
+            dh.set_lifted_mode(0)
+        big["plain_fp64_gpu"] = {"solves_per_s": 200 / wall, "sample": "200 sampled subgraphs"}
+    big["speedup_vs_plain_fp64_gpu"] = big["multi_pass_fp64"]["solves_per_s"] / big["plain_fp64_gpu"]["solves_per_s"]
+    return big
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -115,6 +144,8 @@ def main():
                     help="length of the serving stream (Poisson 2000 req/s); 0 skips it")
     ap.add_argument("--traffic-run", action="store_true",
                     help="one FP64 fit and 1 GiB copy-kernel passes only, for hardware-counter runs")
+    ap.add_argument("--large-delta-only", action="store_true",
+                    help="only the delta^1 section of the 2,000-vertex graph (for kernel traces)")
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
         print("bench.py runs on one GPU: the DRESS engine keeps the whole edge set on a single "
@@ -125,6 +156,9 @@ def main():
     from dress_hip import generators as gen
     if not dh.is_available():
         raise SystemExit("no GPU visible to libdress_hip.so: " + dh._lib.last_error())
+    if args.large_delta_only:
+        print(json.dumps({"delta1_large": delta1_large(dh, gen)}), flush=True)
+        return
 
     t = time.perf_counter()
     N, U, V = gen.chung_lu()
@@ -232,6 +266,8 @@ def main():
         out["delta1"] = res
 
     if not args.no_delta:
+        out["delta1_large"] = delta1_large(dh, gen)
+
         # delta^2 and nabla^1 of the random 12-regular graph on 256 vertices
         n4, U4, V4 = gen.regular_switch(256, 12, 7)
         c4 = {"workload": "delta^2 (32,640 subgraphs of 254 vertices) and nabla^1 (256 marked graphs) of a "

@@ -71,11 +71,15 @@ __device__ __forceinline__ void block_reduce(double &s, double &c)
     __syncthreads();
 }
 
+// max over the wave into *out (bit pattern of a value >= 0); the atomic is
+// skipped where the wave's max cannot raise the running one
 __device__ __forceinline__ void wave_max_atomic(double m, unsigned long long *out)
 {
     for (int o = 32; o > 0; o >>= 1) m = fmax(m, __shfl_xor(m, o));
-    if ((threadIdx.x & 63) == 0 && m > 0.0)
-        atomicMax(out, (unsigned long long)__double_as_longlong(m));
+    if ((threadIdx.x & 63) == 0 && m > 0.0) {
+        const unsigned long long bits = (unsigned long long)__double_as_longlong(m);
+        if (bits > __atomic_load_n(out, __ATOMIC_RELAXED)) atomicMax(out, bits);
+    }
 }
 
 struct EdgeArgs {

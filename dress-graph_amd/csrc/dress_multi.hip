@@ -226,13 +226,24 @@ __global__ void __launch_bounds__(TPB) k_medge(int lo, int hi, MultiArgs a)
             if (q.y >= 0) wn[(int64_t)q.y * MB + i] = wx;
         }
     }
-    // per-slot max over the wave, then one atomic per slot and wave
+    // per-slot max over the wave, then over the workgroup's waves (LDS),
+    // then one atomic per slot and workgroup, only where it can raise the
+    // slot's running max
     if (MB < 64) {
 #pragma unroll
         for (int o = MB; o < 64; o <<= 1) mymax = fmax(mymax, __shfl_xor(mymax, o));
     }
-    if ((threadIdx.x & 63) < MB && mymax > 0.0)
-        atomicMax(&a.st[i].maxd, (unsigned long long)__double_as_longlong(mymax));
+    __shared__ double smax[TPB / 64][MB];
+    const int w = threadIdx.x >> 6, l = threadIdx.x & 63;
+    if (l < MB) smax[w][l] = mymax;
+    __syncthreads();
+    if (w == 0 && l < MB) {
+        double m = smax[0][l];
+        for (int v = 1; v < TPB / 64; v++) m = fmax(m, smax[v][l]);
+        unsigned long long *dst = &a.st[l].maxd;
+        const unsigned long long bits = (unsigned long long)__double_as_longlong(m);
+        if (m > 0.0 && bits > __atomic_load_n(dst, __ATOMIC_RELAXED)) atomicMax(dst, bits);
+    }
 }
 
 // convergence bookkeeping of every slot for the iteration just finished
