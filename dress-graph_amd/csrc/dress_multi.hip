@@ -315,27 +315,43 @@ __global__ void k_minit(int E, MultiArgs a)
 }
 
 // live values of every slot appended at vals[ofs[i] .. lim[i]) (order does
-// not matter to the histogram); with rows, slot i's multiset row in input edge
-// order (NaN for removed edges) at rows[i * E ..)
+// not matter to the histogram); with rows, slot i's multiset row in input
+// edge order (NaN for removed edges) at rows[i * E ..).  Thread t takes
+// slot t / Ep and input edge t % Ep (Ep = E rounded up to a wave), so a
+// wave holds one slot: it compacts its live values with a ballot and takes
+// their places with one atomic.
 template <int MB>
 __global__ void k_mcollect(int E, MultiArgs a, const int *inv, const int64_t *ofs, const int64_t *lim, double *vals,
                            double *rows)
 {
     __shared__ SlotCache<MB> sc;
     load_slots<MB>(a, sc);
-    const int64_t total = (int64_t)E * MB;
-    for (int64_t t = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; t < total; t += (int64_t)gridDim.x * blockDim.x) {
-        const int ein = (int)(t / MB), i = (int)(t % MB);
-        if (i >= a.n) continue;
-        const int e = inv[ein];
-        const int2 p = a.uv[e];
-        const bool dead = a.kind == LIFT_DELTA && (slot_has<MB>(a, sc, i, p.x) || slot_has<MB>(a, sc, i, p.y));
-        const double v = (a.st[i].cur ? a.d1 : a.d0)[(int64_t)e * MB + i];
-        if (!dead) {
-            const int64_t at = ofs[i] + (int64_t)atomicAdd(&a.st[i].count, 1ull);
+    const int64_t Ep = ((int64_t)E + 63) & ~(int64_t)63;
+    const int64_t total = Ep * a.n;
+    const int lane = threadIdx.x & 63;
+    for (int64_t t = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; t - lane < total;
+         t += (int64_t)gridDim.x * blockDim.x) {
+        const int i = (int)(t / Ep);
+        const int ein = (int)(t % Ep);
+        bool live = false;
+        double v = 0.0;
+        if (ein < E) {
+            const int e = inv[ein];
+            const int2 p = a.uv[e];
+            const bool dead =
+                a.kind == LIFT_DELTA && (slot_has<MB>(a, sc, i, p.x) || slot_has<MB>(a, sc, i, p.y));
+            v = (sc.cur[i] ? a.d1 : a.d0)[(int64_t)e * MB + i];
+            live = !dead;
+            if (rows) rows[(int64_t)i * E + ein] = dead ? __longlong_as_double(0x7ff8000000000000ll) : v;
+        }
+        const unsigned long long b = __ballot(live);
+        unsigned long long base = 0;
+        if (lane == 0 && b) base = atomicAdd(&a.st[i].count, (unsigned long long)__popcll(b));
+        base = __shfl(base, 0);
+        if (live) {
+            const int64_t at = ofs[i] + (int64_t)base + __popcll(b & ((1ull << lane) - 1ull));
             if (at < lim[i]) vals[at] = v;          // a miscount is reported by the host
         }
-        if (rows) rows[(int64_t)i * E + ein] = dead ? __longlong_as_double(0x7ff8000000000000ll) : v;
     }
 }
 
@@ -482,8 +498,8 @@ bool run_passes(DevGraph *G, int kind, int k, const std::vector<int> &verts, con
             break;
         }
         if (E > 0)
-            hipLaunchKernelGGL(k_mcollect<MB>, dim3(grid_for((int64_t)E * MB, cap)), dim3(TPB), 0, s, E, a, G->inv,
-                               d_ofs, d_lim, d_vals, d_rows);
+            hipLaunchKernelGGL(k_mcollect<MB>, dim3(grid_for((((int64_t)E + 63) & ~(int64_t)63) * n, cap)),
+                               dim3(TPB), 0, s, E, a, G->inv, d_ofs, d_lim, d_vals, d_rows);
         ok = hipGetLastError() == hipSuccess &&
              hipMemcpyAsync(hst.data(), d_st, (size_t)MB * sizeof(SlotState), hipMemcpyDeviceToHost, s) == hipSuccess;
         if (ok && ms_rows)
