@@ -1035,7 +1035,8 @@ bool run_items(const std::vector<const HostParent *> &ps, const std::vector<int>
                 auto launch_range = [&](size_t b0, size_t b1, int block, float *ms_out) -> bool {
                     a.order = d_ids + b0;
                     a.n_items = (int)(b1 - b0);
-                    int grid = std::max(1, std::min(a.n_items, num_cus() * per_cu_of(block)));
+                    const int resident = std::max(1, num_cus() * per_cu_of(block) * thread_grid_share() / 100);
+                    int grid = std::max(1, std::min(a.n_items, resident));
                     if (!(hipMemsetAsync(d_queue, 0, sizeof(int), s) == hipSuccess &&
                           hipEventRecord(e0, s) == hipSuccess && lifted_launch(a, block, grid, s) &&
                           hipEventRecord(e1, s) == hipSuccess && hipEventSynchronize(e1) == hipSuccess))

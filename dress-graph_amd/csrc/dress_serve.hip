@@ -32,7 +32,11 @@
 struct dress_hip_server {
     std::mutex mu;
     std::condition_variable cv_work, cv_done;
-    std::deque<struct ServeReq *> queue;
+    std::deque<struct ServeReq *> lanes[4];   // waiting requests, one queue per lane
+    int64_t queued() const
+    {
+        return (int64_t)(lanes[0].size() + lanes[1].size() + lanes[2].size() + lanes[3].size());
+    }
     std::deque<dress_hip_response_t> done;
     std::vector<std::thread> workers;
     int device = 0;
@@ -43,6 +47,7 @@ struct dress_hip_server {
     int64_t next_id = 0;
     int64_t reserved_large = 0;         // part of st.reserved_bytes held by the large lane
     std::atomic<int> short_load{0};     // queued + running requests outside the large lane
+    std::atomic<int> fit_load{0};       // queued + running fit requests
     dress_hip_server_stats_t st{};
 };
 
@@ -338,12 +343,19 @@ constexpr double LARGE_BUDGET_SHARE = 0.5;
 // most work one batched launch takes on: edges of its graphs for fits,
 // subgraph values for delta/nabla (keeps each launch to tens of ms)
 constexpr int64_t FIT_BATCH_EDGES = 4 << 20;
-constexpr int64_t LIFT_BATCH_VALUES = 48 << 20;
+constexpr int64_t LIFT_BATCH_VALUES = 16 << 20;
 
 // Under the lock: takes the next request(s) for a worker, or returns false.
+// The worker's own lane first: its oldest request that fits the budget,
+// looking at most SCAN_LIMIT deep; otherwise the oldest such request at the
+// head of another lane (not the large lane, unless the worker may take
+// it).  A batchable request takes waiting requests of its lane with the
+// same parameters along, looking at most 4 x max_batch deep.
+constexpr size_t SCAN_LIMIT = 256;
+
 bool take(dress_hip_server *S, int lane, bool may_take_large, std::vector<ServeReq *> &out, int64_t &bytes)
 {
-    if (S->queue.empty()) return false;
+    if (S->queued() == 0) return false;
     const int64_t room = S->st.budget_bytes - S->st.reserved_bytes;
     const int64_t large_room = (int64_t)(LARGE_BUDGET_SHARE * (double)S->st.budget_bytes) - S->reserved_large;
     const bool idle = S->st.running == 0;
@@ -351,28 +363,46 @@ bool take(dress_hip_server *S, int lane, bool may_take_large, std::vector<ServeR
         if (idle) return true;
         return q->bytes <= room && (lane_of(q) != LANE_LIFT_LARGE || q->bytes <= large_room);
     };
-    auto pick = S->queue.end();
-    for (auto it = S->queue.begin(); it != S->queue.end(); ++it)
-        if (lane_of(*it) == lane && admissible(*it)) { pick = it; break; }
-    if (pick == S->queue.end())
-        for (auto it = S->queue.begin(); it != S->queue.end(); ++it)
-            if ((may_take_large || lane_of(*it) != LANE_LIFT_LARGE) && admissible(*it)) { pick = it; break; }
-    if (pick == S->queue.end()) return false;
+    auto first_admissible = [&](int l) {
+        auto &dq = S->lanes[l];
+        size_t seen = 0;
+        for (auto it = dq.begin(); it != dq.end() && seen < SCAN_LIMIT; ++it, ++seen)
+            if (admissible(*it)) return it;
+        return dq.end();
+    };
+    int from = -1;
+    auto pick = S->lanes[lane].end();
+    {
+        auto it = first_admissible(lane);
+        if (it != S->lanes[lane].end()) { from = lane; pick = it; }
+    }
+    if (from < 0) {
+        for (int l = 0; l < 4; l++) {
+            if (l == lane || (l == LANE_LIFT_LARGE && !may_take_large)) continue;
+            auto it = first_admissible(l);
+            if (it == S->lanes[l].end()) continue;
+            if (from < 0 || (*it)->arrival < (*pick)->arrival) { from = l; pick = it; }
+        }
+    }
+    if (from < 0) return false;
+    auto &dq = S->lanes[from];
     ServeReq *first = *pick;
-    S->queue.erase(pick);
+    dq.erase(pick);
     out.push_back(first);
     bytes = first->bytes;
     if (first->lds) {
         const int64_t cap = first->r.kind == DRESS_HIP_REQ_FIT ? FIT_BATCH_EDGES : LIFT_BATCH_VALUES;
         int64_t work = first->work;
-        for (auto it = S->queue.begin(); it != S->queue.end() && (int)out.size() < S->max_batch;) {
+        size_t seen = 0;
+        const size_t depth = 4 * (size_t)std::max(S->max_batch, 1);
+        for (auto it = dq.begin(); it != dq.end() && (int)out.size() < S->max_batch && seen < depth; ++seen) {
             ServeReq *q = *it;
             if (q->lds && same_batch(*q, *first) && bytes + q->bytes <= std::max(room, first->bytes) &&
                 work + q->work <= cap) {
                 out.push_back(q);
                 bytes += q->bytes;
                 work += q->work;
-                it = S->queue.erase(it);
+                it = dq.erase(it);
             } else {
                 ++it;
             }
@@ -382,7 +412,7 @@ bool take(dress_hip_server *S, int lane, bool may_take_large, std::vector<ServeR
     if (lane_of(first) == LANE_LIFT_LARGE) S->reserved_large += bytes;
     S->st.peak_reserved_bytes = std::max(S->st.peak_reserved_bytes, S->st.reserved_bytes);
     S->st.running += (int64_t)out.size();
-    S->st.queued = (int64_t)S->queue.size();
+    S->st.queued = S->queued();
     return true;
 }
 
@@ -399,6 +429,10 @@ void worker_main(dress_hip_server *S, int index)
     if (lane == LANE_LIFT_LARGE) {
         set_thread_grid_share(25);
         set_thread_pressure(&S->short_load);      // and yields to them between passes
+    } else if (lane == LANE_LIFT_BATCHED) {
+        // batched delta/nabla launches fill half the device while fits wait
+        set_thread_grid_share(50);
+        set_thread_pressure(&S->fit_load);
     }
     // without a large lane (fewer than four workers) the last worker takes
     // large requests as well
@@ -444,6 +478,7 @@ void worker_main(dress_hip_server *S, int index)
                 S->st.lifted_requests += (int64_t)batch.size();
             }
             if (lane_of(batch[0]) != LANE_LIFT_LARGE) S->short_load -= (int)batch.size();
+            if (batch[0]->r.kind == DRESS_HIP_REQ_FIT) S->fit_load -= (int)batch.size();
             for (auto &o : outs) {
                 o.finished = fin;
                 if (o.status == 0) S->st.completed++;
@@ -546,8 +581,9 @@ int64_t dress_hip_server_submit(dress_hip_server_t *S, const dress_hip_request_t
         return id;
     }
     if (lane_of(q) != LANE_LIFT_LARGE) S->short_load++;
-    S->queue.push_back(q);
-    S->st.queued = (int64_t)S->queue.size();
+    if (q->r.kind == DRESS_HIP_REQ_FIT) S->fit_load++;
+    S->lanes[lane_of(q)].push_back(q);
+    S->st.queued = S->queued();
     S->st.max_queued = std::max(S->st.max_queued, S->st.queued);
     S->cv_work.notify_one();
     return q->id;
@@ -590,8 +626,10 @@ void dress_hip_server_destroy(dress_hip_server_t *S)
         std::lock_guard<std::mutex> lk(S->mu);
         S->stop = true;
         S->cancel = true;
-        for (ServeReq *q : S->queue) delete q;
-        S->queue.clear();
+        for (auto &dq : S->lanes) {
+            for (ServeReq *q : dq) delete q;
+            dq.clear();
+        }
         S->st.queued = 0;
     }
     S->cv_work.notify_all();
