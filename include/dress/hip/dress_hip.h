@@ -295,8 +295,10 @@ void dress_hip_fit_batch_free(dress_hip_fit_batch_t *r);
  * lanes -- batched fits, device-wide fits, batched delta/nabla, and the
  * delta/nabla requests too large for the batched kernel, which no other
  * lane takes -- so short requests keep flowing behind long ones.  That
- * lane has one worker, holds at most half of the budget and fills at most a
- * quarter of the device's compute units.  A request whose footprint exceeds
+ * lane has one worker, holds at most half of the budget, fills at most a
+ * quarter of the device while other requests wait or run, and pauses
+ * between its passes for them; batched delta/nabla launches fill at most
+ * half the device while fits wait or run.  A request whose footprint exceeds
  * the budget of its kind (the whole budget, half of it for the large lane)
  * is answered at once with an error.  A request's result does not depend on what runs beside it;
  * FP64 results are those of the single-graph calls (dress_fit_hip_ex,
