@@ -239,6 +239,59 @@ bool d2h_parallel(void *dst, const void *src, size_t bytes)
     return ok.load();
 }
 
+// Host-to-device copy of a large pageable buffer, the mirror of
+// d2h_parallel: T host threads each move a contiguous slice through two
+// pinned 8 MiB staging blocks on a stream of their own.  dst is complete
+// when the call returns.
+bool h2d_parallel(void *dst, const void *src, size_t bytes)
+{
+    constexpr size_t CH = (size_t)8 << 20;
+    if (bytes == 0) return true;
+    if (bytes < 8 * CH || ctx_serving())
+        return hipMemcpy(dst, src, bytes, hipMemcpyHostToDevice) == hipSuccess;
+    const int T = (int)std::max<size_t>(1, std::min<size_t>({(size_t)8, bytes / (4 * CH),
+                                                              (size_t)std::max(1u, std::thread::hardware_concurrency())}));
+    int dev = 0;
+    hipGetDevice(&dev);
+    std::atomic<bool> ok{true};
+    auto slice = [&](int w) {
+        hipSetDevice(dev);
+        const size_t b = bytes * (size_t)w / (size_t)T, e = bytes * (size_t)(w + 1) / (size_t)T;
+        hipStream_t st = nullptr;
+        char *stage[2] = {(char *)pinned_alloc(CH), (char *)pinned_alloc(CH)};
+        hipEvent_t ev[2] = {nullptr, nullptr};
+        bool good = stage[0] && stage[1] && hipStreamCreateWithFlags(&st, hipStreamNonBlocking) == hipSuccess &&
+                    hipEventCreateWithFlags(&ev[0], hipEventDisableTiming) == hipSuccess &&
+                    hipEventCreateWithFlags(&ev[1], hipEventDisableTiming) == hipSuccess;
+        int j = 0;
+        bool used[2] = {false, false};
+        for (size_t at = b; good && at < e; at += CH, j ^= 1) {
+            const size_t n = std::min(CH, e - at);
+            // the staging block's previous copy must be done before refilling it
+            if (used[j]) good = hipEventSynchronize(ev[j]) == hipSuccess;
+            if (!good) break;
+            memcpy(stage[j], (const char *)src + at, n);
+            good = hipMemcpyAsync((char *)dst + at, stage[j], n, hipMemcpyHostToDevice, st) == hipSuccess &&
+                   hipEventRecord(ev[j], st) == hipSuccess;
+            used[j] = true;
+        }
+        if (st) {
+            if (hipStreamSynchronize(st) != hipSuccess) good = false;
+            hipStreamDestroy(st);
+        }
+        for (hipEvent_t x : ev)
+            if (x) hipEventDestroy(x);
+        pinned_free(stage[0]);
+        pinned_free(stage[1]);
+        if (!good) ok = false;
+    };
+    std::vector<std::thread> pool;
+    for (int w = 1; w < T; w++) pool.emplace_back(slice, w);
+    slice(0);
+    for (auto &th : pool) th.join();
+    return ok.load();
+}
+
 void ctx_begin(hipStream_t stream)
 {
     ctx_end();

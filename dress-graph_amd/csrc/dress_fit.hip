@@ -460,7 +460,12 @@ bool fit_upload_values(DevGraph *G, const double *edge_dress)
     G->cur = 0;
     if (G->E == 0) return true;
     hipStream_t s = G->stream;
-    DH_CHECK(hipMemcpyAsync(G->stage, edge_dress, (size_t)G->E * sizeof(double), hipMemcpyHostToDevice, s));
+    // the stage may still be read by earlier work on s
+    DH_CHECK(hipStreamSynchronize(s));
+    if (!h2d_parallel(G->stage, edge_dress, (size_t)G->E * sizeof(double))) {
+        set_error("edge value upload failed");
+        return false;
+    }
     FitState h;
     memset(&h, 0, sizeof(h));
     h.cur = 0;
@@ -750,7 +755,11 @@ bool fit_download(DevGraph *G, double *edge_dress, double *vertex_dress)
     if (edge_dress && G->E > 0) {
         hipLaunchKernelGGL(k_scatter_out, dim3(grid_for(G->E, num_cus() * 8)), dim3(TPB), 0, s, G->E, G->ord,
                            G->d[G->cur], G->stage);
-        DH_CHECK(hipMemcpyAsync(edge_dress, G->stage, (size_t)G->E * sizeof(double), hipMemcpyDeviceToHost, s));
+        DH_CHECK(hipStreamSynchronize(s));
+        if (!d2h_parallel(edge_dress, G->stage, (size_t)G->E * sizeof(double))) {
+            set_error("edge value download failed");
+            return false;
+        }
     }
     DH_CHECK(hipStreamSynchronize(s));
     if (vertex_dress && G->N > 0 && !download_vertex_values(G, G->vd, vertex_dress)) return false;

@@ -73,8 +73,9 @@ void ctx_end();                    // returns the cached blocks to the device
 // pinned host blocks for batch results; pinned_free also takes malloc'd ones
 void *pinned_alloc(size_t bytes);
 void pinned_free(void *p);
-// large device-to-host copy into pageable memory, host-parallel
+// large copies between pageable host memory and the device, host-parallel
 bool d2h_parallel(void *dst, const void *src, size_t bytes);
+bool h2d_parallel(void *dst, const void *src, size_t bytes);
 void ctx_trim(size_t max_block, size_t keep_bytes);
 bool ctx_serving();                // the calling thread is a server worker
 // the calling thread's block cache for batch calls (kept between calls);
