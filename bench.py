@@ -13,7 +13,7 @@ The same line also carries
     solves/s of the batched kernel (FP64 and FP32->FP64) beside the plain
     FP64 GPU path (one device fit per subgraph, the CUDA backend's structure),
   * delta^1 of a 2,000-vertex G(n, p) graph too large for the batched
-    kernel: 64-subgraph passes on its device copy beside one device fit per
+    kernel: multi-subgraph passes on its device copy beside one device fit per
     rebuilt subgraph,
   * delta^2 and nabla^1 of a random 12-regular graph on 256 vertices,
   * the roofline of the dominant kernel (HIP events around its launches),
@@ -108,7 +108,7 @@ def cpu_baseline_delta1(graphs):
 
 
 def delta1_large(dh, gen):
-    """delta^1 of a graph beyond the batched kernel's LDS state: passes of 64
+    """delta^1 of a graph beyond the batched kernel's LDS state: passes of up to 256
     subgraphs on the graph's own device copy, against one device fit per
     rebuilt subgraph (a sample)."""
     nl, Ul, Vl, Wl = gen.gnp(2000, 10.0, 5, weighted=True)

@@ -110,11 +110,15 @@ __device__ __forceinline__ double slot_nw(const MultiArgs &a, const SlotCache<MB
 }
 
 // merge the (s, c) of the GS lanes of one slot inside a group of L = MB * GS
-// lanes (the slot's lanes differ in the bits from MB up); with L = TPB the
-// group is the workgroup and the result is valid in its first wave
+// lanes (slot = lane index % MB, sub = lane index / MB).  Subs of one slot
+// in the same wave merge by shuffles; with L = TPB the group is the
+// workgroup and subs in different waves merge through LDS, the result valid
+// in the waves of sub 0 (the first MB threads).  MB = TPB has one lane per
+// slot and nothing to merge.
 template <int MB, int GS>
-__device__ __forceinline__ void slot_reduce(double &s, double &c, double (*ls)[MB], double (*lc)[MB])
+__device__ __forceinline__ void slot_reduce(double &s, double &c, double (*ls)[64], double (*lc)[64])
 {
+    if (GS == 1) return;
     constexpr int L = MB * GS;
     constexpr int HI = L < 64 ? L : 64;
 #pragma unroll
@@ -123,12 +127,15 @@ __device__ __forceinline__ void slot_reduce(double &s, double &c, double (*ls)[M
         double c2 = __shfl_xor(c, o);
         two_sum_merge(s, c, s2, c2);
     }
-    if (L == TPB) {
+    if (L == TPB && MB < TPB) {
+        // waves w and w + k * (MB / 64) hold the same slots (MB >= 64), or
+        // every wave holds all MB slots (MB < 64)
+        constexpr int WS = MB >= 64 ? MB / 64 : 1;     // waves of one sub
         const int w = threadIdx.x >> 6, l = threadIdx.x & 63;
-        if (l < MB) { ls[w][l] = s; lc[w][l] = c; }
+        if (l < (MB < 64 ? MB : 64)) { ls[w][l] = s; lc[w][l] = c; }
         __syncthreads();
-        if (w == 0 && l < MB)
-            for (int v = 1; v < TPB / 64; v++) two_sum_merge(s, c, ls[v][l], lc[v][l]);
+        if (w < WS && l < (MB < 64 ? MB : 64))
+            for (int v = w + WS; v < TPB / 64; v += WS) two_sum_merge(s, c, ls[v][l], lc[v][l]);
         __syncthreads();
     }
 }
@@ -139,7 +146,7 @@ __global__ void __launch_bounds__(TPB) k_mvertex(const int *__restrict__ list, i
 {
     constexpr int L = MB * GS;
     __shared__ SlotCache<MB> sc;
-    __shared__ double ls[TPB / 64][MB], lc[TPB / 64][MB];
+    __shared__ double ls[TPB / 64][64], lc[TPB / 64][64];
     load_slots<MB>(a, sc);
     const int i = threadIdx.x % MB;
     const int sub = (threadIdx.x / MB) % GS;
@@ -155,7 +162,7 @@ __global__ void __launch_bounds__(TPB) k_mvertex(const int *__restrict__ list, i
             for (int p = b + sub; p < e; p += GS) two_sum_add(s, c, wh[(int64_t)p * MB + i]);
         }
         slot_reduce<MB, GS>(s, c, ls, lc);
-        if (act && sub == 0 && (L < TPB || threadIdx.x < 64)) a.vd[(int64_t)u * MB + i] = sqrt(s + c);
+        if (act && sub == 0) a.vd[(int64_t)u * MB + i] = sqrt(s + c);
     }
 }
 
@@ -179,7 +186,7 @@ __global__ void __launch_bounds__(TPB) k_medge(int lo, int hi, MultiArgs a)
 {
     constexpr int L = MB * GS;
     __shared__ SlotCache<MB> sc;
-    __shared__ double ls[TPB / 64][MB], lc[TPB / 64][MB];
+    __shared__ double ls[TPB / 64][64], lc[TPB / 64][64];
     load_slots<MB>(a, sc);
     const int i = threadIdx.x % MB;
     const int sub = (threadIdx.x / MB) % GS;
@@ -215,7 +222,7 @@ __global__ void __launch_bounds__(TPB) k_medge(int lo, int hi, MultiArgs a)
             }
         }
         slot_reduce<MB, GS>(s, c, ls, lc);
-        if (live && sub == 0 && (L < TPB || threadIdx.x < 64)) {
+        if (live && sub == 0) {
             const double den = a.vd[(int64_t)p.x * MB + i] * a.vd[(int64_t)p.y * MB + i];
             const double x = den > 0.0 ? (s + c) / den : 0.0;
             mymax = fmax(mymax, fabs(de - x));
@@ -233,14 +240,16 @@ __global__ void __launch_bounds__(TPB) k_medge(int lo, int hi, MultiArgs a)
 #pragma unroll
         for (int o = MB; o < 64; o <<= 1) mymax = fmax(mymax, __shfl_xor(mymax, o));
     }
-    __shared__ double smax[TPB / 64][MB];
-    const int w = threadIdx.x >> 6, l = threadIdx.x & 63;
-    if (l < MB) smax[w][l] = mymax;
+    __shared__ double smax[TPB];
+    smax[threadIdx.x] = mymax;
     __syncthreads();
-    if (w == 0 && l < MB) {
-        double m = smax[0][l];
-        for (int v = 1; v < TPB / 64; v++) m = fmax(m, smax[v][l]);
-        unsigned long long *dst = &a.st[l].maxd;
+    if ((int)threadIdx.x < MB) {
+        // slot t's lanes: t, t + MB, t + 2 MB, ... (and, for MB < 64, the
+        // wave-reduced copies at t + 64 k)
+        constexpr int STEP = MB < 64 ? 64 : MB;
+        double m = smax[threadIdx.x];
+        for (int v = threadIdx.x + STEP; v < TPB; v += STEP) m = fmax(m, smax[v]);
+        unsigned long long *dst = &a.st[threadIdx.x].maxd;
         const unsigned long long bits = (unsigned long long)__double_as_longlong(m);
         if (m > 0.0 && bits > __atomic_load_n(dst, __ATOMIC_RELAXED)) atomicMax(dst, bits);
     }
@@ -251,8 +260,10 @@ __global__ void __launch_bounds__(TPB) k_medge(int lo, int hi, MultiArgs a)
 // max_iterations); *remaining = slots still active
 __global__ void k_mend(SlotState *st, int n, double eps, int max_it, int *remaining)
 {
+    __shared__ int cnt;
+    if (threadIdx.x == 0) cnt = 0;
+    __syncthreads();
     const int i = threadIdx.x;
-    int act = 0;
     if (i < n && st[i].active) {
         const double md = __longlong_as_double((long long)st[i].maxd);
         st[i].maxd = 0ull;
@@ -260,10 +271,10 @@ __global__ void k_mend(SlotState *st, int n, double eps, int max_it, int *remain
         const int it = st[i].iter;
         st[i].iter = it + 1;
         if (md < eps || it + 1 >= max_it) st[i].active = 0;
-        act = st[i].active;
+        if (st[i].active) atomicAdd(&cnt, 1);
     }
-    const unsigned long long b = __ballot(act);
-    if (threadIdx.x == 0) *remaining = __popcll(b);
+    __syncthreads();
+    if (threadIdx.x == 0) *remaining = cnt;
 }
 
 __global__ void k_mslots(SlotState *st, int n, int mb)
@@ -380,7 +391,7 @@ struct Pass {
     static void iteration(DevGraph *G, const MultiArgs &a, int cap)
     {
         // bins of <= 4, 32, 256, 4096, more terms
-        constexpr int G1 = MB >= 32 ? 1 : 2, G2 = MB >= 64 ? 1 : 64 / MB, G3 = TPB / MB;
+        constexpr int G1 = MB >= 32 ? 1 : 2, G2 = MB >= 64 ? 1 : 64 / MB, G3 = MB >= TPB ? 1 : TPB / MB;
         vertex<1>(G, 0, a, cap);
         vertex<G1>(G, 1, a, cap);
         vertex<G2>(G, 2, a, cap);
@@ -472,7 +483,7 @@ bool run_passes(DevGraph *G, int kind, int k, const std::vector<int> &verts, con
         if (!tin.empty())
             hipLaunchKernelGGL(k_mrank, dim3(grid_for((int64_t)tin.size(), 64)), dim3(TPB), 0, s, d_in,
                                (int)tin.size(), G->rank, d_tup);
-        hipLaunchKernelGGL(k_mslots, dim3(1), dim3(64), 0, s, d_st, n, MB);
+        hipLaunchKernelGGL(k_mslots, dim3(1), dim3(TPB), 0, s, d_st, n, MB);
         if (E > 0) hipLaunchKernelGGL(k_minit<MB>, dim3(grid_for((int64_t)E * MB, cap)), dim3(TPB), 0, s, E, a);
         int launched = 0, remaining = n, chunk = 4;
         while (max_it > 0 && remaining > 0 && launched < max_it) {
@@ -482,7 +493,7 @@ bool run_passes(DevGraph *G, int kind, int k, const std::vector<int> &verts, con
             for (int j = 0; j < m; j++) {
                 if (G->packed) Pass<MB>::template iteration<true>(G, a, cap);
                 else Pass<MB>::template iteration<false>(G, a, cap);
-                hipLaunchKernelGGL(k_mend, dim3(1), dim3(64), 0, s, d_st, n, eps, max_it, d_rem);
+                hipLaunchKernelGGL(k_mend, dim3(1), dim3(TPB), 0, s, d_st, n, eps, max_it, d_rem);
             }
             launched += m;
             ok = hipGetLastError() == hipSuccess &&
@@ -524,10 +535,13 @@ bool run_passes(DevGraph *G, int kind, int k, const std::vector<int> &verts, con
 
 int multi_max_k() { return MKMAX; }
 
-// slots per pass for G: 64 while a pass's state stays within 2 GiB, else 16
-// while it stays within 4 GiB; 0 when even that is exceeded
+// slots per pass for G: 256 for graphs without edges of more than 256 terms
+// (one lane per slot then carries a whole sum) whose pass state stays within
+// 2 GiB; else 64 within 2 GiB, else 16 within 4 GiB; 0 beyond
 int multi_slots(const DevGraph *G)
 {
+    const bool light = G->ebin[5] == G->ebin[3];          // edge bins 3 and 4 empty
+    if (light && pass_bytes(G, 256) <= ((size_t)2 << 30)) return 256;
     if (pass_bytes(G, 64) <= ((size_t)2 << 30)) return 64;
     if (pass_bytes(G, 16) <= ((size_t)4 << 30)) return 16;
     return 0;
@@ -555,6 +569,7 @@ bool multi_items(DevGraph *G, int kind, int k, const std::vector<int> &verts, co
         return false;
     }
     switch (multi_slots(G)) {
+    case 256: return run_passes<256>(G, kind, k, verts, live, sel, marks, max_it, eps, d_vals, ms_rows);
     case 64: return run_passes<64>(G, kind, k, verts, live, sel, marks, max_it, eps, d_vals, ms_rows);
     case 16: return run_passes<16>(G, kind, k, verts, live, sel, marks, max_it, eps, d_vals, ms_rows);
     default: set_error("graph too large for multi-subgraph passes"); return false;

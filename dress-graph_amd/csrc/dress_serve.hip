@@ -85,7 +85,8 @@ int64_t count_items(int kind, int N, int k)
 // (values, two key buffers, two segment buffers, run flags and positions,
 // unique values, run starts and counts: 72 B per value, 96 with the sort's
 // temporary storage) and, beyond the batched kernel, one multi-subgraph
-// pass (64 slots of values, half-edge copies and norms).
+// pass (up to 256 slots of values, half-edge copies and norms; at most
+// 4 GiB).
 int64_t footprint(const ServeReq &q, int64_t t_bound)
 {
     const int64_t N = q.r.N, E = q.r.E;
@@ -94,7 +95,7 @@ int64_t footprint(const ServeReq &q, int64_t t_bound)
     int64_t items = count_items(q.r.kind, q.r.N, q.r.k);
     double vals = (double)items * (double)E;
     double extra = 96.0 * vals;
-    if (!q.lds) extra += 512.0 * (8.0 * (double)E + (double)N + 8.0);
+    if (!q.lds) extra += std::min(2048.0 * (8.0 * (double)E + (double)N + 8.0), 4294967296.0);
     double b = (double)fit + extra;
     return b > 9.0e18 ? (int64_t)9.0e18 : (int64_t)b;
 }

@@ -189,7 +189,7 @@ int dress_hip_get_batch(p_dress_graph_t g, int n, const int *u, const int *v,
  *     kernel's LDS state (up to 65535 edges and ~150 KB of state), otherwise
  *     fits on the graph's own device copy: its CSR and intersection index
  *     serve every subgraph, delta^k marks the removed edges dead, nabla^k
- *     re-weights the tuple's vertices; up to 64 subgraphs share one pass
+ *     re-weights the tuple's vertices; up to 256 subgraphs share one pass
  *     (k <= 8), one at a time otherwise;
  *   1 always one device fit per rebuilt subgraph (the structure of the CUDA
  *     backend, libdress/src/cuda/delta_dress_cuda.c:24-51), kept for
