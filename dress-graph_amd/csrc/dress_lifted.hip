@@ -455,10 +455,19 @@ __global__ void k_run_emit(const uint64_t *key, const int *seg, const int *flag,
     }
 }
 
-__global__ void k_fill_seg(const int64_t *seg_ofs, int nseg, int *seg)
+// seg[i] = the segment holding value i (the last g with seg_ofs[g] <= i),
+// one thread per value
+__global__ void k_fill_seg(const int64_t *seg_ofs, int nseg, int64_t n, int *seg)
 {
-    for (int g = blockIdx.x; g < nseg; g += gridDim.x)
-        for (int64_t i = seg_ofs[g] + threadIdx.x; i < seg_ofs[g + 1]; i += blockDim.x) seg[i] = g;
+    for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) {
+        int lo = 0, hi = nseg - 1;
+        while (lo < hi) {
+            const int m = (lo + hi + 1) >> 1;
+            if (seg_ofs[m] <= i) lo = m;
+            else hi = m - 1;
+        }
+        seg[i] = lo;
+    }
 }
 
 // run lengths from run starts (ustart ascending, n values in all)
@@ -521,16 +530,18 @@ bool exact_histogram_dev(const double *vals, int64_t n, const std::vector<int64_
         hipLaunchKernelGGL(k_sort_keys, dim3(grid), dim3(TPB), 0, s, vals, n, k0);
         ok = hipMemcpyAsync(dofs, seg_ofs.data(), seg_ofs.size() * sizeof(int64_t), hipMemcpyHostToDevice, s) ==
              hipSuccess;
-        hipLaunchKernelGGL(k_fill_seg, dim3(std::max(1, std::min(nseg, 4096))), dim3(TPB), 0, s, dofs, nseg, g0);
-        // by value (segment ids travel along) ...
-        ok = ok && prim_sort_pairs_u64_i32(k0, k1, g0, g1, (size_t)n, 64, s);
-        // ... then stably by segment
-        int bits = 1;
-        while ((1 << bits) <= nseg) bits++;
-        if (ok && nseg > 1) ok = prim_sort_pairs_i32_u64(g1, g0, k1, k0, (size_t)n, bits, s);
-        else if (ok) {
-            std::swap(k0, k1);
-            std::swap(g0, g1);
+        hipLaunchKernelGGL(k_fill_seg, dim3(grid), dim3(TPB), 0, s, dofs, nseg, n, g0);
+        if (nseg > 1) {
+            // by value (segment ids travel along) ...
+            ok = ok && prim_sort_pairs_u64_i32(k0, k1, g0, g1, (size_t)n, 64, s);
+            // ... then stably by segment
+            int bits = 1;
+            while ((1 << bits) <= nseg) bits++;
+            ok = ok && prim_sort_pairs_i32_u64(g1, g0, k1, k0, (size_t)n, bits, s);
+        } else {
+            // one segment: the keys alone (g0 holds its id, 0, throughout)
+            ok = ok && prim_sort_u64(k0, k1, (size_t)n, s);
+            if (ok) std::swap(k0, k1);
         }
         // now k0 / g0 hold the sorted (segment, value) keys
         if (pt.on) { hipStreamSynchronize(s); pt.mark("sorts"); }
