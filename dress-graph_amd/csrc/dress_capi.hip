@@ -183,11 +183,25 @@ void pinned_free(void *p)
     hipHostFree(p);
 }
 
+// threads of the staged copies: 16 (first-touch faults and copies into
+// the caller's fresh pageable memory are the limit: 790 MB took 20.5 ms with
+// 8 threads and 16.7 ms with 16, against 14 ms for the link);
+// DRESS_HIP_COPY_THREADS overrides
+int copy_threads()
+{
+    static const int t = [] {
+        const char *e = getenv("DRESS_HIP_COPY_THREADS");
+        int v = e ? atoi(e) : 16;
+        return v < 1 ? 1 : (v > 32 ? 32 : v);
+    }();
+    return t;
+}
+
 // Device-to-host copy of a large buffer into pageable memory (results the
-// caller frees with free()).  T host threads each take a contiguous slice
-// and move it in 8 MiB chunks through two pinned staging blocks of their
-// own on a stream of their own: the copy of chunk i+1 runs while chunk i
-// is copied into place, and the first-touch faults of the destination are
+// caller frees with free()).  T host threads (copy_threads()) each take a
+// contiguous slice and move it in 8 MiB chunks through two pinned staging
+// blocks of their own on a stream of their own: the copy of chunk i+1 runs
+// while chunk i is copied into place, and the first-touch faults of the destination are
 // taken by T threads at once.  src must be complete (its stream idle).
 // Threads inside an allocation context copy directly.
 bool d2h_parallel(void *dst, const void *src, size_t bytes)
@@ -196,7 +210,7 @@ bool d2h_parallel(void *dst, const void *src, size_t bytes)
     if (bytes == 0) return true;
     if (bytes < 8 * CH || ctx_serving())
         return hipMemcpy(dst, src, bytes, hipMemcpyDeviceToHost) == hipSuccess;
-    const int T = (int)std::max<size_t>(1, std::min<size_t>({(size_t)8, bytes / (4 * CH),
+    const int T = (int)std::max<size_t>(1, std::min<size_t>({(size_t)copy_threads(), bytes / (4 * CH),
                                                               (size_t)std::max(1u, std::thread::hardware_concurrency())}));
     int dev = 0;
     hipGetDevice(&dev);
@@ -249,7 +263,7 @@ bool h2d_parallel(void *dst, const void *src, size_t bytes)
     if (bytes == 0) return true;
     if (bytes < 8 * CH || ctx_serving())
         return hipMemcpy(dst, src, bytes, hipMemcpyHostToDevice) == hipSuccess;
-    const int T = (int)std::max<size_t>(1, std::min<size_t>({(size_t)8, bytes / (4 * CH),
+    const int T = (int)std::max<size_t>(1, std::min<size_t>({(size_t)copy_threads(), bytes / (4 * CH),
                                                               (size_t)std::max(1u, std::thread::hardware_concurrency())}));
     int dev = 0;
     hipGetDevice(&dev);

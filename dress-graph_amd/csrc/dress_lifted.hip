@@ -629,8 +629,11 @@ bool histogram_flat(const double *vals, int64_t n, hipStream_t s, int64_t **flat
         ok = flat && dalloc(&dflat, (size_t)runs * 2, "flat histogram");
         if (ok) {
             hipLaunchKernelGGL(k_interleave, dim3(grid_for(runs, num_cus() * 8)), dim3(TPB), 0, s, dv, dc, runs, dflat);
-            ok = hipStreamSynchronize(s) == hipSuccess &&
-                 d2h_parallel(flat, dflat, (size_t)runs * 2 * sizeof(int64_t));
+            PhaseTimer pt("  histogram_flat");
+            ok = hipStreamSynchronize(s) == hipSuccess;
+            pt.mark("interleave");
+            ok = ok && d2h_parallel(flat, dflat, (size_t)runs * 2 * sizeof(int64_t));
+            pt.mark("download");
             if (!ok) set_error("histogram download failed");
         }
         dfree(dflat);
@@ -1879,6 +1882,7 @@ int dress_hip_lifted_batch(int kind, int k, int n_graphs, const int *N, const in
         auto d2h = [&](void *dst, const void *src, size_t bytes) {
             return bytes == 0 || hipMemcpyAsync(dst, src, bytes, hipMemcpyDeviceToHost, s) == hipSuccess;
         };
+        PhaseTimer pt2("  assemble");
         ok = d2h(out->hist_value, hv, (size_t)runs * sizeof(double)) &&
              d2h(out->hist_count, hc, (size_t)runs * sizeof(int64_t)) &&
              d2h(out->qhist_key, qk2, (size_t)nq * sizeof(int64_t)) &&
@@ -1888,6 +1892,7 @@ int dress_hip_lifted_batch(int kind, int k, int n_graphs, const int *N, const in
              d2h(out->qhist_offset, qofs, ((size_t)n_graphs + 1) * sizeof(int64_t)) &&
              d2h(out->qbound_offset, bofs, ((size_t)n_graphs + 1) * sizeof(int64_t)) &&
              hipStreamSynchronize(s) == hipSuccess;
+        pt2.mark("download");
         if (!ok && !has_error()) set_error("batch result download failed");
     }
     dfree(hv); dfree(hc); dfree(qk2); dfree(qc2); dfree(bk2); dfree(bc2); dfree(qofs); dfree(bofs);
