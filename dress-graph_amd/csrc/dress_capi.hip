@@ -183,15 +183,16 @@ void pinned_free(void *p)
     hipHostFree(p);
 }
 
-// threads of the staged copies: 16 (first-touch faults and copies into
-// the caller's fresh pageable memory are the limit: 790 MB took 20.5 ms with
-// 8 threads and 16.7 ms with 16, against 14 ms for the link);
+// threads of the staged copies: 8 (first-touch faults and copies into the
+// caller's fresh pageable memory are the limit: 790 MB took 20.5 ms with 8
+// threads and 16.7 ms with 16 once warm, against 14 ms for the link, but 16
+// threads' staging costs more to pin on a first call);
 // DRESS_HIP_COPY_THREADS overrides
 int copy_threads()
 {
     static const int t = [] {
         const char *e = getenv("DRESS_HIP_COPY_THREADS");
-        int v = e ? atoi(e) : 16;
+        int v = e ? atoi(e) : 8;
         return v < 1 ? 1 : (v > 32 ? 32 : v);
     }();
     return t;
@@ -200,7 +201,7 @@ int copy_threads()
 // Device-to-host copy of a large buffer into pageable memory (results the
 // caller frees with free()).  T host threads (copy_threads()) each take a
 // contiguous slice and move it in 8 MiB chunks through two pinned staging
-// blocks of their own on a stream of their own: the copy of chunk i+1 runs
+// chunks of their own (one pooled block for all) on a stream of their own: the copy of chunk i+1 runs
 // while chunk i is copied into place, and the first-touch faults of the destination are
 // taken by T threads at once.  src must be complete (its stream idle).
 // Threads inside an allocation context copy directly.
@@ -214,12 +215,15 @@ bool d2h_parallel(void *dst, const void *src, size_t bytes)
                                                               (size_t)std::max(1u, std::thread::hardware_concurrency())}));
     int dev = 0;
     hipGetDevice(&dev);
+    // one pinned block holds every thread's two staging chunks
+    char *staging = (char *)pinned_alloc((size_t)T * 2 * CH);
+    if (!staging) return false;
     std::atomic<bool> ok{true};
     auto slice = [&](int w) {
         hipSetDevice(dev);
         const size_t b = bytes * (size_t)w / (size_t)T, e = bytes * (size_t)(w + 1) / (size_t)T;
         hipStream_t st = nullptr;
-        char *stage[2] = {(char *)pinned_alloc(CH), (char *)pinned_alloc(CH)};
+        char *stage[2] = {staging + (size_t)w * 2 * CH, staging + (size_t)w * 2 * CH + CH};
         hipEvent_t ev[2] = {nullptr, nullptr};
         bool good = stage[0] && stage[1] && hipStreamCreateWithFlags(&st, hipStreamNonBlocking) == hipSuccess &&
                     hipEventCreateWithFlags(&ev[0], hipEventDisableTiming) == hipSuccess &&
@@ -242,14 +246,13 @@ bool d2h_parallel(void *dst, const void *src, size_t bytes)
         }
         for (hipEvent_t x : ev)
             if (x) hipEventDestroy(x);
-        pinned_free(stage[0]);
-        pinned_free(stage[1]);
         if (!good) ok = false;
     };
     std::vector<std::thread> pool;
     for (int w = 1; w < T; w++) pool.emplace_back(slice, w);
     slice(0);
     for (auto &th : pool) th.join();
+    pinned_free(staging);
     return ok.load();
 }
 
@@ -267,12 +270,15 @@ bool h2d_parallel(void *dst, const void *src, size_t bytes)
                                                               (size_t)std::max(1u, std::thread::hardware_concurrency())}));
     int dev = 0;
     hipGetDevice(&dev);
+    // one pinned block holds every thread's two staging chunks
+    char *staging = (char *)pinned_alloc((size_t)T * 2 * CH);
+    if (!staging) return false;
     std::atomic<bool> ok{true};
     auto slice = [&](int w) {
         hipSetDevice(dev);
         const size_t b = bytes * (size_t)w / (size_t)T, e = bytes * (size_t)(w + 1) / (size_t)T;
         hipStream_t st = nullptr;
-        char *stage[2] = {(char *)pinned_alloc(CH), (char *)pinned_alloc(CH)};
+        char *stage[2] = {staging + (size_t)w * 2 * CH, staging + (size_t)w * 2 * CH + CH};
         hipEvent_t ev[2] = {nullptr, nullptr};
         bool good = stage[0] && stage[1] && hipStreamCreateWithFlags(&st, hipStreamNonBlocking) == hipSuccess &&
                     hipEventCreateWithFlags(&ev[0], hipEventDisableTiming) == hipSuccess &&
@@ -295,14 +301,13 @@ bool h2d_parallel(void *dst, const void *src, size_t bytes)
         }
         for (hipEvent_t x : ev)
             if (x) hipEventDestroy(x);
-        pinned_free(stage[0]);
-        pinned_free(stage[1]);
         if (!good) ok = false;
     };
     std::vector<std::thread> pool;
     for (int w = 1; w < T; w++) pool.emplace_back(slice, w);
     slice(0);
     for (auto &th : pool) th.join();
+    pinned_free(staging);
     return ok.load();
 }
 
@@ -455,6 +460,10 @@ void set_thread_precision(int precision) { t_precision = precision; }
 static thread_local int t_grid_share = 100;
 void set_thread_grid_share(int percent) { t_grid_share = percent < 1 ? 1 : (percent > 100 ? 100 : percent); }
 static thread_local const std::atomic<int> *t_pressure_for_share = nullptr;
+// widest multi-subgraph pass the calling thread may run
+static thread_local int t_max_slots = 256;
+void set_thread_max_slots(int slots) { t_max_slots = slots < 16 ? 16 : slots; }
+int thread_max_slots() { return t_max_slots; }
 // the share applies while requests of higher priority are waiting or
 // running (see set_thread_pressure); otherwise the whole device
 int thread_grid_share()

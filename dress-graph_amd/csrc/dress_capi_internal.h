@@ -26,6 +26,8 @@ void record_lifted(int reruns, double device_ms, int64_t subgraphs, int64_t valu
 bool lds_eligible_size(int N, int E);
 // dress_multi.hip: delta/nabla items fitted many at a time on the parent's device graph
 void set_thread_grid_share(int percent);
+void set_thread_max_slots(int slots);
+int thread_max_slots();
 void set_thread_pressure(const std::atomic<int> *p);
 void yield_to_pressure(double max_seconds);
 int thread_grid_share();

@@ -537,12 +537,14 @@ int multi_max_k() { return MKMAX; }
 
 // slots per pass for G: 256 for graphs without edges of more than 256 terms
 // (one lane per slot then carries a whole sum) whose pass state stays within
-// 2 GiB; else 64 within 2 GiB, else 16 within 4 GiB; 0 beyond
+// 2 GiB; else 64 within 2 GiB, else 16 within 4 GiB; 0 beyond.  A thread's
+// slot cap (set_thread_max_slots) keeps its passes narrower.
 int multi_slots(const DevGraph *G)
 {
     const bool light = G->ebin[5] == G->ebin[3];          // edge bins 3 and 4 empty
-    if (light && pass_bytes(G, 256) <= ((size_t)2 << 30)) return 256;
-    if (pass_bytes(G, 64) <= ((size_t)2 << 30)) return 64;
+    const int cap = thread_max_slots();
+    if (cap >= 256 && light && pass_bytes(G, 256) <= ((size_t)2 << 30)) return 256;
+    if (cap >= 64 && pass_bytes(G, 64) <= ((size_t)2 << 30)) return 64;
     if (pass_bytes(G, 16) <= ((size_t)4 << 30)) return 16;
     return 0;
 }

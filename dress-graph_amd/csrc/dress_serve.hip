@@ -430,6 +430,7 @@ void worker_main(dress_hip_server *S, int index)
     if (lane == LANE_LIFT_LARGE) {
         set_thread_grid_share(25);
         set_thread_pressure(&S->short_load);      // and yields to them between passes
+        set_thread_max_slots(64);                 // short launches to yield between
     } else if (lane == LANE_LIFT_BATCHED) {
         // batched delta/nabla launches fill half the device while fits wait
         set_thread_grid_share(50);
