@@ -21,6 +21,7 @@
 #include <cstdio>
 #include <cstdlib>
 #include <cstring>
+#include <functional>
 #include <map>
 #include <mutex>
 #include <unordered_map>
@@ -957,12 +958,24 @@ bool run_items(const std::vector<const HostParent *> &ps, const std::vector<int>
             items[(size_t)i].ew_ofs = at;
         }
     }
-    // queue order: larger parents first
+    // queue order: larger parents first, items of equal size in index order
+    // (a counting sort over the parents' distinct edge counts)
     std::vector<int> order((size_t)n);
-    for (int i = 0; i < n; i++) order[(size_t)i] = i;
-    std::stable_sort(order.begin(), order.end(), [&](int a, int b) {
-        return ps[(size_t)item_parent[(size_t)a]]->E > ps[(size_t)item_parent[(size_t)b]]->E;
-    });
+    {
+        std::vector<int> sizes;
+        sizes.reserve(ps.size());
+        for (const HostParent *P : ps) sizes.push_back(P->E);
+        std::sort(sizes.begin(), sizes.end(), std::greater<int>());
+        sizes.erase(std::unique(sizes.begin(), sizes.end()), sizes.end());
+        std::vector<int> prank(ps.size());
+        for (size_t p = 0; p < ps.size(); p++)
+            prank[p] = (int)(std::lower_bound(sizes.begin(), sizes.end(), ps[p]->E, std::greater<int>()) -
+                             sizes.begin());
+        std::vector<int> start(sizes.size() + 1, 0);
+        for (int i = 0; i < n; i++) start[(size_t)prank[(size_t)item_parent[(size_t)i]] + 1]++;
+        for (size_t r = 0; r < sizes.size(); r++) start[r + 1] += start[r];
+        for (int i = 0; i < n; i++) order[(size_t)start[(size_t)prank[(size_t)item_parent[(size_t)i]]]++] = i;
+    }
     LiftItem *d_items = nullptr;
     int *d_order = nullptr, *d_verts = nullptr, *d_queue = nullptr, *d_flags = nullptr, *d_iters = nullptr;
     double *d_deltas = nullptr;
