@@ -289,7 +289,7 @@ def main():
             _, _, _, num = g4.lifted("nabla", 1)
             wall = time.perf_counter() - t0
             c4["nabla1_fp64"] = {"solves_per_s": num / wall, "subgraphs": int(num), "wall_ms": wall * 1e3}
-        out["config4"] = c4
+        out["regular12_n256"] = c4
 
     if args.serve_seconds > 0:
         from dress_hip import serve
@@ -309,14 +309,14 @@ def main():
             out["cpu_baseline"] = {"value": cpu_v, "unit": "edge-updates/s", "cores": cores, "kind": "reference",
                                    "sample": f"2 of the {iters_per_fit:.0f} iterations of the same fit, libdress "
                                              f"dress_fit_omp with its intercept index prebuilt ({dt:.1f} s)"}
-        if "config4" in out:
+        if "regular12_n256" in out:
             sys.path.insert(0, os.path.join(ROOT, "oracle"))
             import pyoracle
             lib4 = pyoracle.Lib("ref") if pyoracle.available("ref") else pyoracle.Lib("oracle")
             t0 = time.perf_counter()
             _, _, _, num = lib4.lifted("delta", n4, U4, V4, k=2, n_samples=1500, seed=1, use_omp=True)
             dt = time.perf_counter() - t0
-            out["config4"]["cpu_baseline"] = {"solves_per_s": 1500 / dt, "cores": cores,
+            out["regular12_n256"]["cpu_baseline"] = {"solves_per_s": 1500 / dt, "cores": cores,
                                               "kind": "reference" if lib4.which == "ref" else "port",
                                               "sample": "1,500 sampled delta^2 subgraphs, dress_delta_fit_omp"}
         if "delta1" in out:

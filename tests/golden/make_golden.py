@@ -43,7 +43,7 @@ def main():
     # configuration 1: ER G(2000, 8/1999), seed 0, both directions
     N, U, V = gen.er_both_directions()
     r = ref.fit(N, U, V, max_iterations=100, epsilon=1e-6)
-    np.savez_compressed(os.path.join(HERE, "config1.npz"), U=U, V=V,
+    np.savez_compressed(os.path.join(HERE, "er2000_both.npz"), U=U, V=V,
                         edge_dress=r["edge_dress"], vertex_dress=r["vertex_dress"],
                         iterations=r["iterations"], delta=r["delta"])
 
@@ -100,8 +100,8 @@ def main():
     # configuration 4 graph: 12-regular, n = 256, switch chain seed 7
     n4, U4, V4 = gen.regular_switch()
     vals, cnts, _, num = ref.lifted("nabla", n4, U4, V4, k=1)
-    np.savez_compressed(os.path.join(HERE, "config4.npz"), U=U4, V=V4)
-    with open(os.path.join(HERE, "config4_nabla1.json"), "w") as f:
+    np.savez_compressed(os.path.join(HERE, "regular12_n256.npz"), U=U4, V=V4)
+    with open(os.path.join(HERE, "regular12_n256_nabla1.json"), "w") as f:
         json.dump({"items": int(num), "distinct": int(vals.size), "total": int(cnts.sum()),
                    "sha256": hist_digest(vals, cnts)}, f, indent=1)
 

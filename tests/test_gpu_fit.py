@@ -25,8 +25,8 @@ def _fit(gpu, N, U, V, W=None, NW=None, variant=0, pre=0, max_it=100, eps=1e-6, 
                 "csr": g.csr(), "stats": gpu.last_fit_stats()}
 
 
-def test_config1_bitwise_golden(gpu):
-    z = np.load(os.path.join(GOLDEN, "config1.npz"))
+def test_er2000_bitwise_golden(gpu):
+    z = np.load(os.path.join(GOLDEN, "er2000_both.npz"))
     r = _fit(gpu, 2000, z["U"], z["V"])
     assert r["iterations"] == int(z["iterations"]) == 27
     assert r["delta"] == float(z["delta"])
@@ -34,8 +34,8 @@ def test_config1_bitwise_golden(gpu):
     assert np.array_equal(r["vertex_dress"], z["vertex_dress"])
 
 
-def test_config1_mixed_precision(gpu):
-    z = np.load(os.path.join(GOLDEN, "config1.npz"))
+def test_er2000_mixed_precision(gpu):
+    z = np.load(os.path.join(GOLDEN, "er2000_both.npz"))
     r = _fit(gpu, 2000, z["U"], z["V"], precision=gpu.FP32_FP64)
     assert r["iterations"] == int(z["iterations"])
     err = np.max(np.abs(np.sort(r["edge_dress"]) - np.sort(z["edge_dress"])))
@@ -153,7 +153,7 @@ def _edge_residual(U, V, csr, d, sample):
     return np.array(out)
 
 
-def test_config2_full_size_properties(gpu, gen):
+def test_power_law_4m_full_size_properties(gpu, gen):
     """4M-vertex power-law graph: FP64 and FP32->FP64 agree to 1e-9 with the
     same stopping step; values lie in (0, 2]; sampled edges are fixed points
     to within the stopping tolerance."""
@@ -191,7 +191,7 @@ def test_graph_captured_iterations(gpu, ora, gen, monkeypatch):
     fits (eager work and synchronisations between the replays) stay bitwise
     equal to the oracle, in FP64 and through a precision change."""
     monkeypatch.setenv("DRESS_HIP_GRAPH", "1")
-    z = np.load(os.path.join(GOLDEN, "config1.npz"))
+    z = np.load(os.path.join(GOLDEN, "er2000_both.npz"))
     with gpu.Graph(2000, z["U"], z["V"]) as g:
         for _ in range(3):
             g.set_edge_dress(np.ones(z["U"].size))

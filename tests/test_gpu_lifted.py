@@ -119,9 +119,9 @@ def test_batch_quantised_histograms(gpu, ora, gen, precision):
     assert tot == r.values
 
 
-def test_config4_nabla1_golden(gpu):
-    z = np.load(os.path.join(GOLDEN, "config4.npz"))
-    with open(os.path.join(GOLDEN, "config4_nabla1.json")) as f:
+def test_regular12_n256_nabla1_golden(gpu):
+    z = np.load(os.path.join(GOLDEN, "regular12_n256.npz"))
+    with open(os.path.join(GOLDEN, "regular12_n256_nabla1.json")) as f:
         gold = json.load(f)
     vals, cnts, _, num = _lifted(gpu, "nabla", 256, z["U"], z["V"], k=1)
     assert num == gold["items"] == 256
@@ -129,8 +129,8 @@ def test_config4_nabla1_golden(gpu):
     assert _digest(vals, cnts) == gold["sha256"]
 
 
-def test_config4_delta2(gpu, ora):
-    z = np.load(os.path.join(GOLDEN, "config4.npz"))
+def test_regular12_n256_delta2(gpu, ora):
+    z = np.load(os.path.join(GOLDEN, "regular12_n256.npz"))
     U, V = z["U"], z["V"]
     # all 32,640 subgraphs: histogram total = live edges of every subgraph
     vals, cnts, _, num = _lifted(gpu, "delta", 256, U, V, k=2)

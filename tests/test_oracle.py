@@ -63,7 +63,7 @@ def test_construction_bitwise_vs_reference(ora, ref, variant):
         assert np.array_equal(ca[k], cb[k]), k
 
 
-def test_fit_config1_vs_reference(ora, ref, gen):
+def test_fit_er2000_vs_reference(ora, ref, gen):
     N, U, V = gen.er_both_directions()
     a = ora.fit(N, U, V, parallel=True)
     b = ref.fit(N, U, V, parallel=True)
@@ -109,8 +109,8 @@ def test_lifted_omp_vs_reference(ora, ref, gen):
 
 # ---------------------------------------------------------------- vs golden
 
-def test_golden_config1(ora):
-    z = np.load(os.path.join(GOLDEN, "config1.npz"))
+def test_golden_er2000(ora):
+    z = np.load(os.path.join(GOLDEN, "er2000_both.npz"))
     r = ora.fit(2000, z["U"], z["V"], max_iterations=100, epsilon=1e-6)
     assert r["iterations"] == int(z["iterations"]) == 27
     assert r["delta"] == float(z["delta"])
@@ -118,8 +118,8 @@ def test_golden_config1(ora):
     assert np.array_equal(r["vertex_dress"], z["vertex_dress"])
 
 
-def test_golden_generator_config1(gen):
-    z = np.load(os.path.join(GOLDEN, "config1.npz"))
+def test_golden_generator_er2000(gen):
+    z = np.load(os.path.join(GOLDEN, "er2000_both.npz"))
     N, U, V = gen.er_both_directions()
     assert N == 2000 and U.size == 16040
     assert np.array_equal(U, z["U"]) and np.array_equal(V, z["V"])
