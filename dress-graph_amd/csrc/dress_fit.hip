@@ -22,7 +22,7 @@
 // under relabelling like the reference's.  All FP64 code is compiled with
 // -ffp-contract=off: the reference computes w*d + w*d with two roundings.
 //
-// Work is degree binned: rows/edges with <= 4, 32, 256, 4096 terms use groups
+// Work is degree binned: rows/edges with <= 4, 64, 512, 8192 terms use groups
 // of 1, 4, 16, 64 lanes; larger ones a 256-lane workgroup.  Edges of one bin
 // are a contiguous range of internal ids.
 #include "dress_device.h"

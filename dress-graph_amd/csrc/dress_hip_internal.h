@@ -95,10 +95,12 @@ struct Bins {                      // degree-binned vertex work list
     int start[NB] = {0, 0, 0, 0, 0};
 };
 
-// number of summed terms -> bin (group of 1, 4, 16, 64 lanes or a workgroup)
+// number of summed terms -> bin (group of 1, 4, 16, 64 lanes or a workgroup);
+// the bounds were picked by timing the 4M-vertex power-law fit (5.60 ms per
+// iteration with 4 / 32 / 256 / 4096, 5.48 ms with 4 / 64 / 512 / 8192)
 __host__ __device__ inline int term_bin(int64_t terms)
 {
-    return terms <= 4 ? 0 : terms <= 32 ? 1 : terms <= 256 ? 2 : terms <= 4096 ? 3 : 4;
+    return terms <= 4 ? 0 : terms <= 64 ? 1 : terms <= 512 ? 2 : terms <= 8192 ? 3 : 4;
 }
 
 struct DevGraph {

@@ -390,7 +390,7 @@ struct Pass {
     template <bool PK>
     static void iteration(DevGraph *G, const MultiArgs &a, int cap)
     {
-        // bins of <= 4, 32, 256, 4096, more terms
+        // bins of <= 4, 64, 512, 8192, more terms
         constexpr int G1 = MB >= 32 ? 1 : 2, G2 = MB >= 64 ? 1 : 64 / MB, G3 = MB >= TPB ? 1 : TPB / MB;
         vertex<1>(G, 0, a, cap);
         vertex<G1>(G, 1, a, cap);
@@ -535,7 +535,7 @@ bool run_passes(DevGraph *G, int kind, int k, const std::vector<int> &verts, con
 
 int multi_max_k() { return MKMAX; }
 
-// slots per pass for G: 256 for graphs without edges of more than 256 terms
+// slots per pass for G: 256 for graphs without edges of more than 512 terms
 // (one lane per slot then carries a whole sum) whose pass state stays within
 // 2 GiB; else 64 within 2 GiB, else 16 within 4 GiB; 0 beyond.  A thread's
 // slot cap (set_thread_max_slots) keeps its passes narrower.

@@ -145,7 +145,7 @@ int dress_fit_hip_ex(p_dress_graph_t g, int max_iterations, double epsilon,
  * (intersection entries), max degree, 1 if the index is packed (16-bit row
  * positions), edges per work bin (5), vertices per work bin (5), intersection
  * entries per edge bin (5).  Bins group items whose sums
- * have <= 4, 32, 256, 4096, more terms.  Returns 0, or -1 on error. */
+ * have <= 4, 64, 512, 8192, more terms.  Returns 0, or -1 on error. */
 int dress_hip_graph_info(p_dress_graph_t g, int64_t *info, int n);
 
 /* Device-resident fits for callers that keep the graph on the GPU between
