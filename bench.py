@@ -50,7 +50,7 @@ B_ENTRY, B_HALF, B_EDGE, B_VERTEX = 24, 12, 44, 20
 # scaled by the bytes a plain 1 GiB copy kernel moves per counted KiB.  Valid
 # for the kernel and graph it was measured on: reported only when that
 # kernel is the dominant one.
-TRAFFIC = {"kernel": "k_edge<64,packed,double> (edge bin 3)", "fetch_kib": 6242718, "write_kib": 78604,
+TRAFFIC = {"kernel": "k_edge<16,packed,double> (edge bin 2)", "fetch_kib": 5418725, "write_kib": 117693,
            "fetch_scale": 2.0, "write_scale": 1.0}
 TRAFFIC["bytes_per_launch"] = int((TRAFFIC["fetch_kib"] * TRAFFIC["fetch_scale"] +
                                    TRAFFIC["write_kib"] * TRAFFIC["write_scale"]) * 1024)
