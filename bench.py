@@ -115,12 +115,17 @@ def delta1_large(dh, gen):
     big = {"workload": f"delta^1 of a weighted G(n, p) graph, n=2,000, mean degree 10 (E={Ul.size}), seed 5, "
                        "eps 1e-6"}
     with dh.Graph(nl, Ul, Vl, Wl) as gl:
-        gl.lifted("delta", 1, n_samples=64, seed=1)                 # warm-up
+        # warm-up: the same call (its first large download pins the
+        # library's host staging block, a one-time cost per process)
+        t0 = time.perf_counter()
+        gl.lifted("delta", 1)
+        first = time.perf_counter() - t0
         t0 = time.perf_counter()
         _, cnts, _, num = gl.lifted("delta", 1)
         wall = time.perf_counter() - t0
         big["multi_pass_fp64"] = {"solves_per_s": num / wall, "subgraphs": int(num), "values": int(cnts.sum()),
-                                  "wall_ms": wall * 1e3}
+                                  "histogram_pairs": int(cnts.size), "wall_ms": wall * 1e3,
+                                  "first_call_ms": first * 1e3}
         dh.set_lifted_mode(1)
         try:
             t0 = time.perf_counter()

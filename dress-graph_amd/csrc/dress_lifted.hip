@@ -1344,15 +1344,20 @@ int64_t *lifted_flat(int kind, p_dress_graph_t g, int k, int iterations, double 
     memset(J.marks, 0, sizeof(J.marks));
     if (kind == LIFT_NABLA) marker_weights(J.marks, k);
 
-    HostParent P;
-    host_parent_build(N, E, g->U, g->V, g->W, (int)g->variant, P);
-    pt.mark("parent");
     int mode;
     {
         std::lock_guard<std::mutex> lk(g_lift_mu);
         mode = g_lift_mode;
     }
-    if (mode == 0 && lds_eligible(P) && iterations > 0) {
+    // the host parent (CSR, intersection pairs, DIRECTED reciprocal
+    // lookups) serves the LDS-resident kernel and the DIRECTED delta weight
+    // patches; the paths on the device graph need nothing else of it
+    const bool lds = mode == 0 && lds_eligible_size(N, E) && iterations > 0;
+    HostParent P;
+    if (lds || (kind == LIFT_DELTA && g->variant == DRESS_VARIANT_DIRECTED && (mode == 0 || mode == 2 || mode == 3)))
+        host_parent_build(N, E, g->U, g->V, g->W, (int)g->variant, P);
+    pt.mark("parent");
+    if (lds) {
         std::vector<int64_t> live = kind == LIFT_DELTA ? delta_live_counts(g, verts, k, sel.size())
                                                        : std::vector<int64_t>(sel.size(), (int64_t)E);
         std::vector<int> item_parent(sel.size(), 0);
@@ -1391,16 +1396,19 @@ int64_t *lifted_flat(int kind, p_dress_graph_t g, int k, int iterations, double 
                                                        : std::vector<int64_t>(sel.size(), (int64_t)E);
         int64_t total_live = 0;
         for (int64_t x : live) total_live += x;
+        pt.mark("live counts");
         double *d_vals = nullptr;
         int64_t nv = 0;
         EwPatches ep;
         if (kind == LIFT_DELTA) ep = delta_patches(P, verts, k, sel.size());
         const bool multi = mode != 2 && !ep.any() && k <= multi_max_k() && build_bins(G) && multi_slots(G) > 0;
+        pt.mark("patches, bins");
         bool ok = dalloc(&d_vals, (size_t)total_live + 1, "subgraph values");
         if (ok && multi) {
             ok = multi_items(G, kind, k, verts, live, sel, kind == LIFT_NABLA ? J.marks : nullptr, iterations,
                              epsilon, d_vals, host_ms);
             nv = total_live;
+            pt.mark("multi_items");
         } else if (ok) {
             std::vector<double> parent_w(ep.e.size());
             for (size_t j = 0; j < ep.e.size(); j++) parent_w[j] = P.ew[(size_t)ep.e[j]];
@@ -1413,6 +1421,7 @@ int64_t *lifted_flat(int kind, p_dress_graph_t g, int k, int iterations, double 
         }
         int64_t *flat = nullptr;
         if (ok && do_hist) ok = histogram_flat(d_vals, nv, s, &flat, hist_size);
+        pt.mark("histogram");
         dfree(d_vals);
         if (!ok) return nullptr;
         record_lifted(0, 0.0, (int64_t)sel.size(), nv);

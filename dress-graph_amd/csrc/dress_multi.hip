@@ -61,6 +61,9 @@ struct MultiArgs {
     double *d0, *d1, *w0, *w1;     // item-minor values and half-edge copies
     double *vd;                    // item-minor norms
     SlotState *st;
+    double eps;                    // stopping rule of every slot
+    int max_it;
+    int *remaining;                // slots still active after the iteration
 };
 
 inline int grid_for(int64_t threads, int cap)
@@ -180,6 +183,33 @@ __device__ __forceinline__ void mentry(const MultiArgs &a, int64_t k, int2 r, in
     }
 }
 
+// convergence bookkeeping of every slot for the iteration just finished
+// (dress.c:590-600: stop once the max change is < epsilon, or after
+// max_iterations); *remaining = slots still active.  One workgroup.  (A
+// last-workgroup-done form inside the edge launch measured 3.4x slower per
+// pass: the device-scope fence each workgroup needs writes back and
+// invalidates its XCD's L2, which holds the slots' state.)
+__global__ void k_mend(MultiArgs a)
+{
+    SlotState *st = a.st;
+    const int n = a.n;
+    __shared__ int cnt;
+    if (threadIdx.x == 0) cnt = 0;
+    __syncthreads();
+    for (int i = threadIdx.x; i < n; i += blockDim.x) {
+        if (!st[i].active) continue;
+        const double md = __longlong_as_double((long long)st[i].maxd);
+        st[i].maxd = 0ull;
+        st[i].cur ^= 1;
+        const int it = st[i].iter;
+        st[i].iter = it + 1;
+        if (md < a.eps || it + 1 >= a.max_it) st[i].active = 0;
+        if (st[i].active) atomicAdd(&cnt, 1);
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) *a.remaining = cnt;
+}
+
 // edge phase of every active slot over internal edges [lo, hi)
 template <int MB, int GS, bool PK>
 __global__ void __launch_bounds__(TPB) k_medge(int lo, int hi, MultiArgs a)
@@ -255,28 +285,7 @@ __global__ void __launch_bounds__(TPB) k_medge(int lo, int hi, MultiArgs a)
     }
 }
 
-// convergence bookkeeping of every slot for the iteration just finished
-// (dress.c:590-600: stop once the max change is < epsilon, or after
-// max_iterations); *remaining = slots still active
-__global__ void k_mend(SlotState *st, int n, double eps, int max_it, int *remaining)
-{
-    __shared__ int cnt;
-    if (threadIdx.x == 0) cnt = 0;
-    __syncthreads();
-    const int i = threadIdx.x;
-    if (i < n && st[i].active) {
-        const double md = __longlong_as_double((long long)st[i].maxd);
-        st[i].maxd = 0ull;
-        st[i].cur ^= 1;
-        const int it = st[i].iter;
-        st[i].iter = it + 1;
-        if (md < eps || it + 1 >= max_it) st[i].active = 0;
-        if (st[i].active) atomicAdd(&cnt, 1);
-    }
-    __syncthreads();
-    if (threadIdx.x == 0) *remaining = cnt;
-}
-
+// fresh slot states for a pass of n items
 __global__ void k_mslots(SlotState *st, int n, int mb)
 {
     const int i = threadIdx.x;
@@ -368,40 +377,67 @@ __global__ void k_mcollect(int E, MultiArgs a, const int *inv, const int64_t *of
 
 template <int MB>
 struct Pass {
-    // lanes per row / edge group by term bin: MB slots x GS lanes each
-    template <int GS, bool PK>
-    static void edge(DevGraph *G, int bin, const MultiArgs &a, int cap)
+    // lanes per slot of vertex / edge bins 0..4 (<= 4, 64, 512, 8192, more
+    // terms): a slot's sum over one row or edge is split over GS lanes
+    static constexpr int G1 = MB >= 32 ? 1 : 2, G2 = MB >= 64 ? 1 : 64 / MB, G3 = MB >= TPB ? 1 : TPB / MB;
+    static constexpr int gs(int bin) { return bin == 0 ? 1 : bin == 1 ? G1 : bin == 2 ? G2 : G3; }
+
+    template <int GS>
+    static void vertex(DevGraph *G, int b0, int b1, const MultiArgs &a, int cap)
     {
-        const int lo = G->ebin[bin], hi = G->ebin[bin + 1];
-        if (hi <= lo) return;
-        constexpr int L = MB * GS;
-        hipLaunchKernelGGL((k_medge<MB, GS, PK>), dim3(grid_for((int64_t)(hi - lo) * L, cap)), dim3(TPB), 0,
+        const Bins &B = G->vbins;
+        const int lo = B.start[b0], n = B.start[b1 - 1] + B.count[b1 - 1] - lo;
+        if (n <= 0) return;
+        hipLaunchKernelGGL((k_mvertex<MB, GS>), dim3(grid_for((int64_t)n * MB * GS, cap)), dim3(TPB), 0, G->stream,
+                           B.list + lo, n, a);
+    }
+    template <int GS, bool PK>
+    static void edge(DevGraph *G, int lo, int hi, const MultiArgs &a, int cap)
+    {
+        hipLaunchKernelGGL((k_medge<MB, GS, PK>), dim3(grid_for((int64_t)(hi - lo) * MB * GS, cap)), dim3(TPB), 0,
                            G->stream, lo, hi, a);
     }
     template <int GS>
-    static void vertex(DevGraph *G, int bin, const MultiArgs &a, int cap)
+    static void vertex_gs(int gsv, DevGraph *G, int b0, int b1, const MultiArgs &a, int cap)
     {
-        const int n = G->vbins.count[bin];
-        if (!n) return;
-        constexpr int L = MB * GS;
-        hipLaunchKernelGGL((k_mvertex<MB, GS>), dim3(grid_for((int64_t)n * L, cap)), dim3(TPB), 0, G->stream,
-                           G->vbins.list + G->vbins.start[bin], n, a);
+        if constexpr (MB * GS <= TPB) {
+            if (gsv == GS) vertex<GS>(G, b0, b1, a, cap);
+        }
     }
+    template <int GS, bool PK>
+    static void edge_gs(int gsv, DevGraph *G, int lo, int hi, const MultiArgs &a, int cap)
+    {
+        if constexpr (MB * GS <= TPB) {
+            if (gsv == GS) edge<GS, PK>(G, lo, hi, a, cap);
+        }
+    }
+    // one iteration: adjacent bins with the same lanes per slot share a
+    // launch; k_mend closes it
     template <bool PK>
     static void iteration(DevGraph *G, const MultiArgs &a, int cap)
     {
-        // bins of <= 4, 64, 512, 8192, more terms
-        constexpr int G1 = MB >= 32 ? 1 : 2, G2 = MB >= 64 ? 1 : 64 / MB, G3 = MB >= TPB ? 1 : TPB / MB;
-        vertex<1>(G, 0, a, cap);
-        vertex<G1>(G, 1, a, cap);
-        vertex<G2>(G, 2, a, cap);
-        vertex<G3>(G, 3, a, cap);
-        vertex<G3>(G, 4, a, cap);
-        edge<1, PK>(G, 0, a, cap);
-        edge<G1, PK>(G, 1, a, cap);
-        edge<G2, PK>(G, 2, a, cap);
-        edge<G3, PK>(G, 3, a, cap);
-        edge<G3, PK>(G, 4, a, cap);
+        int runs[Bins::NB][2], nr = 0;
+        for (int b = 0; b < Bins::NB; b++) {
+            if (nr > 0 && gs(runs[nr - 1][0]) == gs(b)) runs[nr - 1][1] = b + 1;
+            else { runs[nr][0] = b; runs[nr][1] = b + 1; nr++; }
+        }
+        for (int r = 0; r < nr; r++) {
+            const int g = gs(runs[r][0]);
+            vertex_gs<1>(g, G, runs[r][0], runs[r][1], a, cap);
+            vertex_gs<2>(g, G, runs[r][0], runs[r][1], a, cap);
+            vertex_gs<4>(g, G, runs[r][0], runs[r][1], a, cap);
+            vertex_gs<16>(g, G, runs[r][0], runs[r][1], a, cap);
+        }
+        for (int r = 0; r < nr; r++) {
+            const int lo = G->ebin[runs[r][0]], hi = G->ebin[runs[r][1]];
+            if (hi <= lo) continue;
+            const int g = gs(runs[r][0]);
+            edge_gs<1, PK>(g, G, lo, hi, a, cap);
+            edge_gs<2, PK>(g, G, lo, hi, a, cap);
+            edge_gs<4, PK>(g, G, lo, hi, a, cap);
+            edge_gs<16, PK>(g, G, lo, hi, a, cap);
+        }
+        hipLaunchKernelGGL(k_mend, dim3(1), dim3(TPB), 0, G->stream, a);
     }
 };
 
@@ -444,6 +480,9 @@ bool run_passes(DevGraph *G, int kind, int k, const std::vector<int> &verts, con
     a.tup = d_tup;
     a.marks = d_marks;
     a.st = d_st;
+    a.eps = eps;
+    a.max_it = max_it;
+    a.remaining = d_rem;
     int cap = std::max(1, num_cus() * 8 * thread_grid_share() / 100);
     std::vector<int> slot_item, tin;
     std::vector<int64_t> sofs, slim;
@@ -493,7 +532,6 @@ bool run_passes(DevGraph *G, int kind, int k, const std::vector<int> &verts, con
             for (int j = 0; j < m; j++) {
                 if (G->packed) Pass<MB>::template iteration<true>(G, a, cap);
                 else Pass<MB>::template iteration<false>(G, a, cap);
-                hipLaunchKernelGGL(k_mend, dim3(1), dim3(TPB), 0, s, d_st, n, eps, max_it, d_rem);
             }
             launched += m;
             ok = hipGetLastError() == hipSuccess &&
