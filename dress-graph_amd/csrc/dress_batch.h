@@ -64,6 +64,24 @@ struct LiftArgs {
 
 constexpr unsigned long long QEMPTY = ~0ull;
 
+// quantised-table key of value v of graph gid: floor(v / eps) (clamped to
+// 40 bits), with bit 40 set for values within tol of a bin edge
+__device__ __forceinline__ unsigned long long quant_key(double v, unsigned long long gid, double eps, double tol)
+{
+    const double r = v / eps;
+    const double q = floor(r);
+    const bool bnd = (r - q) * eps < tol || (q + 1.0 - r) * eps < tol;
+    const unsigned long long qq =
+        (q < 0.0) ? 0ull : (q >= 1099511627775.0 ? 1099511627775ull : (unsigned long long)q);
+    return (gid << 41) | ((unsigned long long)bnd << 40) | qq;
+}
+
+// first probe slot of a key in a table of cap (a power of two) slots
+__device__ __forceinline__ int64_t quant_slot(unsigned long long key, int64_t cap)
+{
+    return (int64_t)((key * 0x9E3779B97F4A7C15ull) >> 20) & (cap - 1);
+}
+
 size_t lifted_lds_bytes(int ecap, int ncap);
 size_t lifted_lds_limit();
 size_t lifted_lds_per_cu();

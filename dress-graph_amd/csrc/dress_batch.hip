@@ -269,15 +269,8 @@ __global__ void __launch_bounds__(B) k_lifted(LiftArgs a)
             for (int e = threadIdx.x; e < E; e += B) {
                 int2 p = uv[e];
                 if (!(alive[p.x] && alive[p.y])) continue;
-                double v = dc[e];
-                double r = v / a.eps;
-                double q = floor(r);
-                bool bnd = (r - q) * a.eps < a.qtol || (q + 1.0 - r) * a.eps < a.qtol;
-                unsigned long long qq = (q < 0.0) ? 0ull : (q >= 1099511627775.0 ? 1099511627775ull
-                                                                                  : (unsigned long long)q);
-                unsigned long long key = (gid << 41) | ((unsigned long long)bnd << 40) | qq;
-                unsigned long long h = key * 0x9E3779B97F4A7C15ull;
-                int64_t slot = (int64_t)(h >> 20) & (a.qcap - 1);
+                const unsigned long long key = quant_key((double)dc[e], gid, a.eps, a.qtol);
+                int64_t slot = quant_slot(key, a.qcap);
                 for (int64_t probe = 0; probe < a.qcap; probe++) {
                     unsigned long long prev = atomicCAS(&a.qkeys[slot], QEMPTY, key);
                     if (prev == QEMPTY || prev == key) {

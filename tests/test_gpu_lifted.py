@@ -119,6 +119,28 @@ def test_batch_quantised_histograms(gpu, ora, gen, precision):
     assert tot == r.values
 
 
+def test_mixed_batch_quantised_after_fp64_redo(gpu, gen):
+    """Items whose mixed-precision stop was ambiguous are fitted again in
+    FP64; their mixed values' counts leave the quantised tables first, so the
+    tables of every graph are those of its final (returned) values."""
+    graphs = gen.delta1_batch_graphs(160)
+    eps, tol = 1e-6, 1e-9
+    r = gpu.lifted_batch(graphs, k=1, precision=1, epsilon=eps, qtol=tol)
+    assert r.fp64_reruns > 0
+    for (hv, hc), (qk, qc), (bk, bc) in zip(r.histograms, r.quantised, r.boundary):
+        keys, cnt = _quantise(hv, hc, eps, tol, hv)
+        assert np.array_equal(qk, keys) and np.array_equal(qc, cnt)
+        q = hv / eps
+        f = np.floor(q)
+        near = ((q - f) * eps < tol) | ((f + 1 - q) * eps < tol)
+        bkeys, binv = np.unique(f[near].astype(np.int64), return_inverse=True)
+        assert np.array_equal(bk, bkeys)
+        assert np.array_equal(bc, np.bincount(binv, weights=hc[near]).astype(np.int64) if bkeys.size else bc)
+        assert np.all(qc > 0) and np.all(bc > 0)
+    tot = sum(int(c.sum()) for _, c in r.quantised) + sum(int(c.sum()) for _, c in r.boundary)
+    assert tot == r.values
+
+
 def test_regular12_n256_nabla1_golden(gpu):
     z = np.load(os.path.join(GOLDEN, "regular12_n256.npz"))
     with open(os.path.join(GOLDEN, "regular12_n256_nabla1.json")) as f:
