@@ -260,6 +260,31 @@ bool d2h_parallel(void *dst, const void *src, size_t bytes)
     return ok.load();
 }
 
+void fill_nan_parallel(double *p, size_t n)
+{
+    const double nan = std::nan("");
+    const size_t T = n * sizeof(double) < ((size_t)64 << 20)
+                         ? 1
+                         : std::max<size_t>(1, std::min<size_t>((size_t)copy_threads(),
+                                                                std::max(1u, std::thread::hardware_concurrency())));
+    auto fill = [&](size_t w) {
+        const size_t b = n * w / T, e = n * (w + 1) / T;
+        for (size_t i = b; i < e; i++) p[i] = nan;
+    };
+    std::vector<std::thread> pool;
+    std::vector<size_t> left;                 // slices no thread could take
+    for (size_t w = 1; w < T; w++) {
+        try {
+            pool.emplace_back(fill, w);
+        } catch (...) {
+            left.push_back(w);
+        }
+    }
+    fill(0);
+    for (size_t w : left) fill(w);
+    for (auto &th : pool) th.join();
+}
+
 // Host-to-device copy of a large pageable buffer, the mirror of
 // d2h_parallel: T host threads each move a contiguous slice through two
 // pinned 8 MiB staging blocks on a stream of their own.  dst is complete

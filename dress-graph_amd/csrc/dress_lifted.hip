@@ -671,7 +671,10 @@ bool exact_histogram(const double *vals, int64_t n, const std::vector<int64_t> &
 }
 
 // Flat histogram (dress_histogram.c:184-200 layout: value bits, count, ...)
-// of vals[0..n), malloc'd; NULL with *hist_size = 0 when empty.
+// of vals[0..n), malloc'd; NULL with *hist_size = 0 when empty.  (Faulting
+// the result block in from host threads while the device worked made calls
+// slower, 790 MB results 32-36 -> 71-92 ms, most likely because the faults
+// contend with the runtime's own mappings for the address space.)
 bool histogram_flat(const double *vals, int64_t n, hipStream_t s, int64_t **flat_out, int *hist_size)
 {
     *flat_out = nullptr;
@@ -1360,9 +1363,9 @@ int64_t *lifted_flat(int kind, p_dress_graph_t g, int k, int iterations, double 
     double *host_ms = nullptr;
     if (keep_multisets && multisets) {
         size_t len = (size_t)std::max<int64_t>(total, 0) * (size_t)E;
-        host_ms = (double *)malloc((len ? len : 1) * sizeof(double));
+        host_ms = (double *)host_alloc((len ? len : 1) * sizeof(double));
         if (host_ms) {
-            for (size_t i = 0; i < len; i++) host_ms[i] = NAN;
+            fill_nan_parallel(host_ms, len);
             *multisets = host_ms;
         }
     }
