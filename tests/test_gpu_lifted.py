@@ -119,6 +119,26 @@ def test_batch_quantised_histograms(gpu, ora, gen, precision):
     assert tot == r.values
 
 
+@pytest.mark.parametrize("kind,k", [("delta", 1), ("delta", 2), ("nabla", 1)])
+def test_batch_parents_without_common_neighbours(gpu, ora, gen, kind, k):
+    """Parents whose intersection index is empty (a path, a star, K3,3, an odd
+    cycle) batched with one that has triangles: each graph's histogram equals
+    the oracle's."""
+    i32 = np.int32
+    graphs = [
+        (6, np.arange(5, dtype=i32), np.arange(1, 6, dtype=i32)),
+        (6, np.zeros(5, i32), np.arange(1, 6, dtype=i32)),
+        (6, np.repeat(np.arange(3, dtype=i32), 3), np.tile(np.arange(3, 6, dtype=i32), 3)),
+        (7, np.arange(7, dtype=i32), ((np.arange(7) + 1) % 7).astype(i32)),
+        gen.random_regular(12, 4, 5),
+    ]
+    r = gpu.lifted_batch(graphs, k=k, kind=kind)
+    assert len(r.histograms) == len(graphs)
+    for (hv, hc), (n, U, V) in zip(r.histograms, graphs):
+        ov, oc, _, _ = ora.lifted(kind, n, U, V, k=k)
+        assert np.array_equal(hv, ov) and np.array_equal(hc, oc), (kind, k, n, U.size)
+
+
 def test_mixed_batch_quantised_after_fp64_redo(gpu, gen):
     """Items whose mixed-precision stop was ambiguous are fitted again in
     FP64; their mixed values' counts leave the quantised tables first, so the
