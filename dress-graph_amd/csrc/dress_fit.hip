@@ -53,6 +53,20 @@ struct FitState {
                              // pure FP64 run, so the host reruns in FP64
 };
 
+// two pinned FitState slots per host thread for the fit loop's state
+// snapshots, allocated on the thread's first fit and kept for its lifetime
+FitState *fit_snapshots()
+{
+    static thread_local FitState *slots = nullptr;
+    if (!slots && hipHostMalloc((void **)&slots, 2 * sizeof(FitState), hipHostMallocDefault) != hipSuccess) {
+        (void)hipGetLastError();
+        slots = nullptr;
+        set_error("pinned allocation of the fit state snapshots failed");
+    }
+    return slots;
+}
+
+
 namespace {
 
 constexpr int TPB = 256;
@@ -569,35 +583,64 @@ bool fit_run_once(DevGraph *G, int max_iterations, double epsilon, bool mixed, F
                                G->d[1], (double *)nullptr, (double *)nullptr, G->wh[0], G->wh[1], st);
     }
     hipGraphExec_t ig = iteration_graph(G, mixed, epsilon, max_iterations, egrid);
+    // Iterations go in chunks, each followed by a copy of the state into a
+    // pinned snapshot slot.  The next chunk is launched before the last
+    // snapshot is read, so the device does not wait for the host between
+    // chunks; iterations launched past convergence return at once.
+    FitState *snap = fit_snapshots();
+    if (!snap) return false;
+    hipEvent_t sev[2];
+    DH_CHECK(hipEventCreateWithFlags(&sev[0], hipEventDisableTiming));
+    DH_CHECK(hipEventCreateWithFlags(&sev[1], hipEventDisableTiming));
     int launched = 0;
-    int chunk = 4;
-    while (launched < max_iterations) {
-        int n = std::min(chunk, max_iterations - launched);
+    auto issue = [&](int n, int slot) {
         for (int j = 0; j < n; j++) {
-            if (ig) DH_CHECK(hipGraphLaunch(ig, s));
+            if (ig) (void)hipGraphLaunch(ig, s);
             else launch_iteration(G, mixed, epsilon, max_iterations, egrid, s);
         }
         launched += n;
-        DH_CHECK(hipGetLastError());
-        DH_CHECK(hipMemcpyAsync(&h, st, sizeof(h), hipMemcpyDeviceToHost, s));
-        DH_CHECK(hipStreamSynchronize(s));
-        if (h.done) break;
-        // next chunk: the iterations a geometric decay of the change needs to
-        // reach epsilon, so launches past convergence stay rare
+        return hipGetLastError() == hipSuccess &&
+               hipMemcpyAsync(&snap[slot], st, sizeof(FitState), hipMemcpyDeviceToHost, s) == hipSuccess &&
+               hipEventRecord(sev[slot], s) == hipSuccess;
+    };
+    bool ok = max_iterations <= 0 || issue(std::min(4, max_iterations), 0);
+    int pend = 0, chunk = 4;                 // the second chunk goes in blind
+    while (ok && launched > 0) {
+        int nxt = -1, n = 0;
+        if (launched < max_iterations) {
+            n = std::min(chunk, max_iterations - launched);
+            ok = issue(n, pend ^ 1);
+            nxt = pend ^ 1;
+        }
+        ok = ok && hipEventSynchronize(sev[pend]) == hipSuccess;
+        if (!ok) break;
+        h = snap[pend];
+        if (h.done || nxt < 0) break;
+        // the chunk after the one in flight: what a geometric decay of the
+        // change still needs to reach epsilon, so launches past convergence
+        // stay few
         chunk = 16;
         if (h.prev_delta > 0.0 && h.delta > 0.0 && h.delta < h.prev_delta && epsilon > 0.0) {
             double r = h.delta / h.prev_delta;
-            double need = std::ceil(std::log(epsilon / h.delta) / std::log(r));
+            double need = std::ceil(std::log(epsilon / h.delta) / std::log(r)) - n;
             if (need >= 1.0 && need < 16.0) chunk = (int)need;
             else if (need < 1.0) chunk = 1;
         }
+        pend = nxt;
     }
     DH_CHECK(hipEventRecord(ev1, s));
     DH_CHECK(hipEventSynchronize(ev1));
+    hipEventDestroy(sev[0]);
+    hipEventDestroy(sev[1]);
     float ms = 0.f;
     hipEventElapsedTime(&ms, ev0, ev1);
     hipEventDestroy(ev0);
     hipEventDestroy(ev1);
+    if (!ok) { set_error("fit iteration launch failed"); return false; }
+    // the state after everything launched (the chunk in flight at the stop
+    // changed nothing)
+    DH_CHECK(hipMemcpyAsync(&h, st, sizeof(h), hipMemcpyDeviceToHost, s));
+    DH_CHECK(hipStreamSynchronize(s));
     if (!h.done) { set_error("fit did not finish within the launched iterations"); return false; }
     G->cur = h.cur;
     res->iterations = h.iterations;
