@@ -54,7 +54,8 @@ struct FitState {
 };
 
 // two pinned FitState slots per host thread for the fit loop's state
-// snapshots, allocated on the thread's first fit and kept for its lifetime
+// snapshots, allocated on the thread's first fit and never freed (a few
+// hundred bytes per thread that ever fits)
 FitState *fit_snapshots()
 {
     static thread_local FitState *slots = nullptr;
