@@ -161,9 +161,22 @@ __device__ __forceinline__ int lower_bound_i(const int *a, int lo, int hi, int x
 // edge i (input edge ord[i]) write the half-edge pair (a in row U, b in row V)
 // of each common neighbour at ioff[i], ascending in the neighbour, packed as
 // row-relative 16-bit positions (ipk) or absolute (ipair).
+// Hub rows as bitmaps: hub h (slot[v] = h, or -1) has bits[h * W + i], bit
+// j set when label 64 i + j is in v's row, and pre[h * W + i] = the set bits
+// of words 0 .. i-1, so the position of a member x in the row is
+// pre[x / 64] + popcount(word below x).  Only rows without repeated
+// neighbours get a slot.
+struct HubBits {
+    const int *slot = nullptr;
+    const unsigned long long *bits = nullptr;
+    const unsigned *pre = nullptr;
+    int64_t W = 0;
+};
+
 template <int GS, bool FILL>
 __global__ void k_isect(const int *U, const int *V, int E, const int *row_off, const int *adj_x,
-                        const int *ord, int64_t *cnt, const int64_t *ioff, uint32_t *ipk, int2 *ipair)
+                        const int *ord, int64_t *cnt, const int64_t *ioff, uint32_t *ipk, int2 *ipair,
+                        HubBits hub)
 {
     const int lane = threadIdx.x % GS;
     const int wlane = threadIdx.x & 63;
@@ -177,6 +190,10 @@ __global__ void k_isect(const int *U, const int *V, int E, const int *row_off, c
         bool a_is_u = (ue - ub) <= (ve - vb);
         int ab = a_is_u ? ub : vb, ae = a_is_u ? ue : ve;
         int bb = a_is_u ? vb : ub, be = a_is_u ? ve : ue;
+        // the longer row as a hub bitmap, when it has one
+        const int hs = hub.slot ? hub.slot[a_is_u ? v : u] : -1;
+        const unsigned long long *hbits = hs >= 0 ? hub.bits + (int64_t)hs * hub.W : nullptr;
+        const unsigned *hpre = hs >= 0 ? hub.pre + (int64_t)hs * hub.W : nullptr;
         int64_t out = FILL ? ioff[g] : 0;
         int64_t found = 0;
         for (int base = ab; base < ae; base += GS) {
@@ -185,12 +202,22 @@ __global__ void k_isect(const int *U, const int *V, int E, const int *row_off, c
             int q = 0;
             if (p < ae) {
                 int x = adj_x[p];
-                int first = (p > ab && adj_x[p - 1] == x) ? lower_bound_i(adj_x, ab, p, x) : p;
-                int r = p - first;
-                int lo = lower_bound_i(adj_x, bb, be, x);
-                int have = 0;
-                while (lo + have < be && adj_x[lo + have] == x && have <= r) have++;
-                if (r < have) { hit = true; q = lo + r; }
+                const bool again = p > ab && adj_x[p - 1] == x;     // not x's first copy in row a
+                if (hbits) {
+                    // row b holds x at most once: only x's first copy pairs
+                    const unsigned long long w = hbits[x >> 6], m = 1ull << (x & 63);
+                    if (!again && (w & m)) {
+                        hit = true;
+                        q = bb + (int)hpre[x >> 6] + __popcll(w & (m - 1));
+                    }
+                } else {
+                    int first = again ? lower_bound_i(adj_x, ab, p, x) : p;
+                    int r = p - first;
+                    int lo = lower_bound_i(adj_x, bb, be, x);
+                    int have = 0;
+                    while (lo + have < be && adj_x[lo + have] == x && have <= r) have++;
+                    if (r < have) { hit = true; q = lo + r; }
+                }
             }
             if (FILL) {
                 unsigned long long m = __ballot(hit);
@@ -216,6 +243,71 @@ __global__ void k_isect(const int *U, const int *V, int E, const int *row_off, c
             for (int o = GS / 2; o > 0; o >>= 1) found += __shfl_xor(found, o, GS);
             if (lane == 0) cnt[e] = found;
         }
+    }
+}
+
+// vertices with at least `deg_min` neighbours: a slot each (in no order)
+__global__ void k_hub_assign(const int *row_off, int N, int deg_min, int *slot, int *hub_vertex, int *count)
+{
+    for (int v = blockIdx.x * blockDim.x + threadIdx.x; v < N; v += gridDim.x * blockDim.x) {
+        int h = -1;
+        if (row_off[v + 1] - row_off[v] >= deg_min) {
+            h = atomicAdd(count, 1);
+            hub_vertex[h] = v;
+        }
+        slot[v] = h;
+    }
+}
+
+// bits of hub rows (one workgroup per hub; bits start at 0); a row with a
+// repeated neighbour is marked in bad[]
+__global__ void k_hub_bits(const int *row_off, const int *adj_x, const int *hub_vertex, int nh, int64_t W,
+                           unsigned long long *bits, int *bad)
+{
+    for (int h = blockIdx.x; h < nh; h += gridDim.x) {
+        const int v = hub_vertex[h];
+        const int b = row_off[v], e = row_off[v + 1];
+        for (int p = b + threadIdx.x; p < e; p += blockDim.x) {
+            const int x = adj_x[p];
+            atomicOr(&bits[(int64_t)h * W + (x >> 6)], 1ull << (x & 63));
+            if (p > b && adj_x[p - 1] == x) bad[h] = 1;
+        }
+    }
+}
+
+// pre[h * W + i] = set bits of words 0 .. i-1 of hub h (one workgroup per
+// hub: each thread sums a contiguous run of words, then a scan over the
+// threads); hubs marked bad lose their slot
+__global__ void k_hub_prefix(const unsigned long long *bits, int nh, int64_t W, unsigned *pre, const int *bad,
+                             const int *hub_vertex, int *slot)
+{
+    __shared__ unsigned part[256];
+    const int T = blockDim.x;                      // 256
+    for (int h = blockIdx.x; h < nh; h += gridDim.x) {
+        const unsigned long long *wb = bits + (int64_t)h * W;
+        unsigned *pb = pre + (int64_t)h * W;
+        const int64_t per = (W + T - 1) / T;
+        const int64_t i0 = threadIdx.x * per, i1 = i0 + per < W ? i0 + per : W;
+        unsigned sum = 0;
+        for (int64_t i = i0; i < i1; i++) sum += (unsigned)__popcll(wb[i]);
+        part[threadIdx.x] = sum;
+        __syncthreads();
+        if (threadIdx.x == 0) {
+            unsigned run = 0;
+            for (int t = 0; t < T; t++) {
+                const unsigned c = part[t];
+                part[t] = run;
+                run += c;
+            }
+        }
+        __syncthreads();
+        unsigned run = part[threadIdx.x];
+        for (int64_t i = i0; i < i1; i++) {
+            pb[i] = run;
+            run += (unsigned)__popcll(wb[i]);
+        }
+        if (threadIdx.x == 0 && bad[h]) slot[hub_vertex[h]] = -1;
+        __syncthreads();
     }
 }
 
@@ -655,6 +747,90 @@ bool build_graph(DevGraph *G, int N, int E, const int *U, const int *V, const do
     return ok && relabel(G);
 }
 
+// least row length that gets a hub bitmap (DRESS_HIP_HUB_DEGREE overrides,
+// read at every index build; 0 switches bitmaps off)
+int hub_degree()
+{
+    const char *e = getenv("DRESS_HIP_HUB_DEGREE");
+    return e ? std::max(0, atoi(e)) : 1024;
+}
+
+// device buffers of the hub bitmaps of one index build
+struct HubBuffers {
+    int *slot = nullptr, *vertex = nullptr, *count = nullptr, *bad = nullptr;
+    unsigned long long *bits = nullptr;
+    unsigned *pre = nullptr;
+    void release()
+    {
+        dfree(slot); dfree(vertex); dfree(count); dfree(bad); dfree(bits); dfree(pre);
+        slot = vertex = count = bad = nullptr;
+        bits = nullptr;
+        pre = nullptr;
+    }
+    HubBuffers() = default;
+    HubBuffers(const HubBuffers &) = delete;
+    HubBuffers &operator=(const HubBuffers &) = delete;
+    ~HubBuffers() { release(); }
+};
+
+// Bitmaps of the rows with at least hub_degree() neighbours, the least
+// degree doubled until they take at most 2 GiB (ceil(N / 64) words of 12
+// bytes per hub).  Returns an empty view when there are none.
+HubBits build_hubs(DevGraph *G, HubBuffers &B)
+{
+    HubBits view;
+    const int N = G->N;
+    int deg_min = hub_degree();
+    if (deg_min <= 0 || N <= 0 || G->max_degree < deg_min) return view;
+    hipStream_t s = G->stream;
+    const int64_t W = ((int64_t)N + 63) / 64;
+    const int64_t cap = (int64_t)2 << 30;
+    if (!dalloc(&B.slot, (size_t)N + 1, "hub slots") || !dalloc(&B.vertex, (size_t)N + 1, "hub vertices") ||
+        !dalloc(&B.count, 1, "hub count")) {
+        hipGetLastError();
+        clear_error();
+        B.release();
+        return view;
+    }
+    int nh = 0;
+    for (;;) {
+        hipLaunchKernelGGL(k_fill_i32, dim3(1), dim3(64), 0, s, B.count, (int64_t)1, 0);
+        hipLaunchKernelGGL(k_hub_assign, dim3(grid_for(N, 4096)), dim3(TPB), 0, s, G->row_off, N, deg_min, B.slot,
+                           B.vertex, B.count);
+        if (hipMemcpyAsync(&nh, B.count, sizeof(int), hipMemcpyDeviceToHost, s) != hipSuccess ||
+            hipStreamSynchronize(s) != hipSuccess) {
+            nh = 0;
+            break;
+        }
+        if ((int64_t)nh * W * 12 <= cap || deg_min > G->max_degree) break;
+        deg_min *= 2;
+    }
+    if (nh <= 0 || (int64_t)nh * W * 12 > cap) {
+        B.release();
+        return view;
+    }
+    if (!dalloc(&B.bits, (size_t)nh * (size_t)W, "hub bitmaps") || !dalloc(&B.pre, (size_t)nh * (size_t)W, "hub counts") ||
+        !dalloc(&B.bad, (size_t)nh, "hub flags")) {
+        // no room: the index is built without bitmaps
+        hipGetLastError();
+        clear_error();
+        B.release();
+        return view;
+    }
+    hipLaunchKernelGGL(k_fill_i32, dim3(grid_for((int64_t)nh * W * 2, 8192)), dim3(TPB), 0, s, (int *)B.bits,
+                       (int64_t)nh * W * 2, 0);
+    hipLaunchKernelGGL(k_fill_i32, dim3(grid_for(nh, 4096)), dim3(TPB), 0, s, B.bad, (int64_t)nh, 0);
+    const int hgrid = std::min(nh, 65535);
+    hipLaunchKernelGGL(k_hub_bits, dim3(hgrid), dim3(TPB), 0, s, G->row_off, G->adj_x, B.vertex, nh, W, B.bits,
+                       B.bad);
+    hipLaunchKernelGGL(k_hub_prefix, dim3(hgrid), dim3(256), 0, s, B.bits, nh, W, B.pre, B.bad, B.vertex, B.slot);
+    view.slot = B.slot;
+    view.bits = B.bits;
+    view.pre = B.pre;
+    view.W = W;
+    return view;
+}
+
 bool build_index(DevGraph *G)
 {
     if (G->have_index) return true;
@@ -673,6 +849,8 @@ bool build_index(DevGraph *G)
         return false;
     }
     // 1. half-edges of every edge, |N(u) & N(v)| per input edge
+    HubBuffers hubs;
+    HubBits hub;
     if (E > 0) {
         hipLaunchKernelGGL(k_fill_i32, dim3(grid_for(2 * (int64_t)E, 8192)), dim3(TPB), 0, s, (int *)G->pos_in,
                            2 * (int64_t)E, -1);
@@ -680,9 +858,10 @@ bool build_index(DevGraph *G)
             hipLaunchKernelGGL(k_edge_positions, dim3(grid_for(G->S, 8192)), dim3(TPB), 0, s, G->adj_e, G->S,
                                G->pos_in);
         hipLaunchKernelGGL(k_order_positions, dim3(grid_for(E, 8192)), dim3(TPB), 0, s, G->pos_in, E);
+        hub = build_hubs(G, hubs);
         hipLaunchKernelGGL((k_isect<GS, false>), dim3(grid), dim3(TPB), 0, s, G->U, G->V, E, G->row_off,
                            G->adj_x, (const int *)nullptr, G->cnt_in, (const int64_t *)nullptr,
-                           (uint32_t *)nullptr, (int2 *)nullptr);
+                           (uint32_t *)nullptr, (int2 *)nullptr, hub);
     }
     // 2. internal order: by bin, then by first half-edge (CSR order)
     int counts[Bins::NB] = {0, 0, 0, 0, 0};
@@ -730,7 +909,7 @@ bool build_index(DevGraph *G)
     }
     if (E > 0 && T > 0)
         hipLaunchKernelGGL((k_isect<GS, true>), dim3(grid), dim3(TPB), 0, s, G->U, G->V, E, G->row_off,
-                           G->adj_x, G->ord, (int64_t *)nullptr, (const int64_t *)G->ioff, G->ipk, G->ipair);
+                           G->adj_x, G->ord, (int64_t *)nullptr, (const int64_t *)G->ioff, G->ipk, G->ipair, hub);
     DH_CHECK(hipGetLastError());
     DH_CHECK(hipStreamSynchronize(s));
     G->have_index = true;
@@ -822,7 +1001,7 @@ bool download_index(DevGraph *G, int **ioff_out, int **ux_out, int **vx_out)
             hipLaunchKernelGGL((k_isect<GS, true>), dim3(grid_for((int64_t)E * GS, num_cus() * 32)), dim3(TPB), 0,
                                s, (const int *)G->ref_U, (const int *)G->ref_V, E, (const int *)G->ref_row_off,
                                (const int *)G->ref_adj_x, (const int *)ident, (int64_t *)nullptr,
-                               (const int64_t *)dio, (uint32_t *)nullptr, pairs);
+                               (const int64_t *)dio, (uint32_t *)nullptr, pairs, HubBits());
             hipLaunchKernelGGL(k_pairs_to_edges, dim3(grid_for(T, 8192)), dim3(TPB), 0, s, (const int2 *)pairs, T,
                                (const int *)G->ref_adj_e, dux, dvx);
         }

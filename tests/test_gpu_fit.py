@@ -76,6 +76,44 @@ def test_random_multigraphs_bitwise(gpu, ora, variant):
         assert np.array_equal(a["vertex_dress"], b["vertex_dress"])
 
 
+@pytest.mark.parametrize("variant", [0, 1, 2, 3])
+def test_hub_bitmaps_bitwise(gpu, ora, variant, monkeypatch):
+    """Every row of two or more neighbours indexed through its hub bitmap
+    (rows with a repeated neighbour keep the binary search): fits equal the
+    oracle's bit for bit, on simple graphs and on multigraphs."""
+    monkeypatch.setenv("DRESS_HIP_HUB_DEGREE", "2")
+    rng = np.random.default_rng(70 + variant)
+    for t in range(4):
+        n, m = int(rng.integers(5, 150)), int(rng.integers(1, 900))
+        U = rng.integers(0, n, m).astype(np.int32)
+        V = rng.integers(0, n, m).astype(np.int32)
+        if t < 2:
+            # simple graph: no loops, each unordered pair once
+            keep = U != V
+            U, V = U[keep], V[keep]
+            _, first = np.unique(np.stack([np.minimum(U, V), np.maximum(U, V)]), axis=1, return_index=True)
+            first = np.sort(first)
+            U, V = U[first], V[first]
+        W = rng.random(U.size) + 0.05 if t % 2 else None
+        a = _fit(gpu, n, U, V, W, None, variant)
+        b = ora.fit(n, U, V, W, None, variant, 0, 100, 1e-6)
+        assert a["iterations"] == b["iterations"] and a["delta"] == b["delta"], (variant, t)
+        assert np.array_equal(a["edge_dress"], b["edge_dress"]), (variant, t)
+        assert np.array_equal(a["vertex_dress"], b["vertex_dress"]), (variant, t)
+
+
+def test_hub_bitmaps_power_law(gpu, ora, gen, monkeypatch):
+    """Rows of 64 or more neighbours of a 60,000-vertex power-law graph as hub
+    bitmaps: the fit equals the oracle's bit for bit."""
+    monkeypatch.setenv("DRESS_HIP_HUB_DEGREE", "64")
+    n, U, V = gen.chung_lu(n=60_000, n_edges=420_000, seed=5)
+    a = _fit(gpu, n, U, V)
+    b = ora.fit(n, U, V, parallel=True)
+    assert a["iterations"] == b["iterations"] and a["delta"] == b["delta"]
+    assert np.array_equal(a["edge_dress"], b["edge_dress"])
+    assert np.array_equal(a["vertex_dress"], b["vertex_dress"])
+
+
 def test_iteration_limits(gpu, ora, gen):
     n, U, V = gen.er_single(300, 0.03, 2)
     for max_it, eps in ((1, 1e-6), (2, 1e-6), (5, 0.0), (0, 1e-6)):
