@@ -954,15 +954,15 @@ bool download_csr(DevGraph *G, int *adj_offset, int *adj_target, int *adj_edge_i
 {
     hipStream_t s = G->stream;
     if (!G->ref_row_off) { set_error("reference adjacency is no longer on the device"); return false; }
-    DH_CHECK(hipMemcpyAsync(adj_offset, G->ref_row_off, ((size_t)G->N + 1) * sizeof(int), hipMemcpyDeviceToHost, s));
-    if (G->S > 0) {
-        DH_CHECK(hipMemcpyAsync(adj_target, G->ref_adj_x, (size_t)G->S * sizeof(int), hipMemcpyDeviceToHost, s));
-        DH_CHECK(hipMemcpyAsync(adj_edge_idx, G->ref_adj_e, (size_t)G->S * sizeof(int), hipMemcpyDeviceToHost, s));
-    }
-    if (G->E > 0)
-        DH_CHECK(hipMemcpyAsync(edge_weight, G->ew_in, (size_t)G->E * sizeof(double), hipMemcpyDeviceToHost, s));
+    // the caller's pageable arrays: large ones through the host-parallel
+    // staged copy (its source must be complete)
     DH_CHECK(hipStreamSynchronize(s));
-    return true;
+    bool ok = d2h_parallel(adj_offset, G->ref_row_off, ((size_t)G->N + 1) * sizeof(int)) &&
+              (G->S == 0 || (d2h_parallel(adj_target, G->ref_adj_x, (size_t)G->S * sizeof(int)) &&
+                             d2h_parallel(adj_edge_idx, G->ref_adj_e, (size_t)G->S * sizeof(int)))) &&
+              (G->E == 0 || d2h_parallel(edge_weight, G->ew_in, (size_t)G->E * sizeof(double)));
+    if (!ok) set_error("download of the host adjacency failed");
+    return ok;
 }
 
 // Intercept arrays in the reference's form (input edge ids, int offsets in

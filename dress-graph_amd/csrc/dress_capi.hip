@@ -260,16 +260,17 @@ bool d2h_parallel(void *dst, const void *src, size_t bytes)
     return ok.load();
 }
 
-void fill_nan_parallel(double *p, size_t n)
+void fill_nan_parallel(double *p, size_t n) { fill_parallel(p, n, std::nan("")); }
+
+void fill_parallel(double *p, size_t n, double value)
 {
-    const double nan = std::nan("");
     const size_t T = n * sizeof(double) < ((size_t)64 << 20)
                          ? 1
                          : std::max<size_t>(1, std::min<size_t>((size_t)copy_threads(),
                                                                 std::max(1u, std::thread::hardware_concurrency())));
     auto fill = [&](size_t w) {
         const size_t b = n * w / T, e = n * (w + 1) / T;
-        for (size_t i = b; i < e; i++) p[i] = nan;
+        for (size_t i = b; i < e; i++) p[i] = value;
     };
     std::vector<std::thread> pool;
     std::vector<size_t> left;                 // slices no thread could take
@@ -677,31 +678,34 @@ p_dress_graph_t dress_init_graph(int N, int E, int *U, int *V, double *W, double
     g->variant = variant;
     g->N = N; g->E = E; g->U = U; g->V = V; g->W = W;
     g->precompute_intercepts = precompute_intercepts;
+    // the host arrays of dress_graph_t (free() releases every one): large
+    // ones on huge pages, filled by several host threads
     if (NW) g->NW = NW;
     else {
-        g->NW = (double *)malloc(((size_t)N + 1) * sizeof(double));
-        if (g->NW) for (int i = 0; i < N; i++) g->NW[i] = 1.0;
+        g->NW = (double *)host_alloc(((size_t)N + 1) * sizeof(double));
+        if (g->NW) fill_parallel(g->NW, (size_t)N, 1.0);
     }
     g->vertex_dress = (double *)calloc((size_t)N + 1, sizeof(double));
-    g->edge_dress = (double *)malloc(((size_t)E + 1) * sizeof(double));
-    g->edge_dress_next = (double *)malloc(((size_t)E + 1) * sizeof(double));
-    g->adj_offset = (int *)malloc(((size_t)N + 1) * sizeof(int));
-    g->edge_weight = (double *)malloc(((size_t)E + 1) * sizeof(double));
+    g->edge_dress = (double *)host_alloc(((size_t)E + 1) * sizeof(double));
+    g->edge_dress_next = (double *)host_alloc(((size_t)E + 1) * sizeof(double));
+    g->adj_offset = (int *)host_alloc(((size_t)N + 1) * sizeof(int));
+    g->edge_weight = (double *)host_alloc(((size_t)E + 1) * sizeof(double));
     DevGraph *G = new DevGraph();
     bool ok = g->NW && g->vertex_dress && g->edge_dress && g->edge_dress_next && g->adj_offset &&
               g->edge_weight;
     if (!ok) set_error("host allocation failed in dress_init_graph");
     ok = ok && build_graph(G, N, E, U, V, W, g->NW, (int)variant);
     if (ok) {
-        g->adj_target = (int *)malloc(((size_t)G->S + 1) * sizeof(int));
-        g->adj_edge_idx = (int *)malloc(((size_t)G->S + 1) * sizeof(int));
+        g->adj_target = (int *)host_alloc(((size_t)G->S + 1) * sizeof(int));
+        g->adj_edge_idx = (int *)host_alloc(((size_t)G->S + 1) * sizeof(int));
         ok = g->adj_target && g->adj_edge_idx;
         if (!ok) set_error("host allocation failed in dress_init_graph");
     }
     ok = ok && download_csr(G, g->adj_offset, g->adj_target, g->adj_edge_idx, g->edge_weight);
     if (ok) {
         g->max_degree = G->max_degree;
-        for (int e = 0; e < E; e++) g->edge_dress[e] = g->edge_dress_next[e] = 1.0;
+        fill_parallel(g->edge_dress, (size_t)E, 1.0);
+        fill_parallel(g->edge_dress_next, (size_t)E, 1.0);
         if (precompute_intercepts)
             ok = download_index(G, &g->intercept_offset, &g->intercept_edge_ux, &g->intercept_edge_vx);
     }

@@ -76,7 +76,8 @@ void pinned_free(void *p);
 // large copies between pageable host memory and the device, host-parallel
 bool d2h_parallel(void *dst, const void *src, size_t bytes);
 bool h2d_parallel(void *dst, const void *src, size_t bytes);
-// n doubles set to NaN, by several host threads when n is large
+// n doubles set to a value (NaN), by several host threads when n is large
+void fill_parallel(double *p, size_t n, double value);
 void fill_nan_parallel(double *p, size_t n);
 
 void ctx_trim(size_t max_block, size_t keep_bytes);
