@@ -773,10 +773,12 @@ struct HubBuffers {
     ~HubBuffers() { release(); }
 };
 
-// Bitmaps of the rows with at least hub_degree() neighbours, the least
-// degree doubled until they take at most 2 GiB (ceil(N / 64) words of 12
-// bytes per hub).  Returns an empty view when there are none.
-HubBits build_hubs(DevGraph *G, HubBuffers &B)
+// Bitmaps of the rows (row_off / adj_x: the internal CSR, or the
+// reference-layout one, both sorted by label) with at least hub_degree()
+// neighbours, the least degree doubled until they take at most 2 GiB
+// (ceil(N / 64) words of 12 bytes per hub).  Returns an empty view when
+// there are none.
+HubBits build_hubs(DevGraph *G, HubBuffers &B, const int *row_off, const int *adj_x)
 {
     HubBits view;
     const int N = G->N;
@@ -795,7 +797,7 @@ HubBits build_hubs(DevGraph *G, HubBuffers &B)
     int nh = 0;
     for (;;) {
         hipLaunchKernelGGL(k_fill_i32, dim3(1), dim3(64), 0, s, B.count, (int64_t)1, 0);
-        hipLaunchKernelGGL(k_hub_assign, dim3(grid_for(N, 4096)), dim3(TPB), 0, s, G->row_off, N, deg_min, B.slot,
+        hipLaunchKernelGGL(k_hub_assign, dim3(grid_for(N, 4096)), dim3(TPB), 0, s, row_off, N, deg_min, B.slot,
                            B.vertex, B.count);
         if (hipMemcpyAsync(&nh, B.count, sizeof(int), hipMemcpyDeviceToHost, s) != hipSuccess ||
             hipStreamSynchronize(s) != hipSuccess) {
@@ -821,8 +823,7 @@ HubBits build_hubs(DevGraph *G, HubBuffers &B)
                        (int64_t)nh * W * 2, 0);
     hipLaunchKernelGGL(k_fill_i32, dim3(grid_for(nh, 4096)), dim3(TPB), 0, s, B.bad, (int64_t)nh, 0);
     const int hgrid = std::min(nh, 65535);
-    hipLaunchKernelGGL(k_hub_bits, dim3(hgrid), dim3(TPB), 0, s, G->row_off, G->adj_x, B.vertex, nh, W, B.bits,
-                       B.bad);
+    hipLaunchKernelGGL(k_hub_bits, dim3(hgrid), dim3(TPB), 0, s, row_off, adj_x, B.vertex, nh, W, B.bits, B.bad);
     hipLaunchKernelGGL(k_hub_prefix, dim3(hgrid), dim3(256), 0, s, B.bits, nh, W, B.pre, B.bad, B.vertex, B.slot);
     view.slot = B.slot;
     view.bits = B.bits;
@@ -858,7 +859,7 @@ bool build_index(DevGraph *G)
             hipLaunchKernelGGL(k_edge_positions, dim3(grid_for(G->S, 8192)), dim3(TPB), 0, s, G->adj_e, G->S,
                                G->pos_in);
         hipLaunchKernelGGL(k_order_positions, dim3(grid_for(E, 8192)), dim3(TPB), 0, s, G->pos_in, E);
-        hub = build_hubs(G, hubs);
+        hub = build_hubs(G, hubs, G->row_off, G->adj_x);
         hipLaunchKernelGGL((k_isect<GS, false>), dim3(grid), dim3(TPB), 0, s, G->U, G->V, E, G->row_off,
                            G->adj_x, (const int *)nullptr, G->cnt_in, (const int64_t *)nullptr,
                            (uint32_t *)nullptr, (int2 *)nullptr, hub);
@@ -978,16 +979,17 @@ bool download_index(DevGraph *G, int **ioff_out, int **ux_out, int **vx_out)
     const int E = G->E;
     const int64_t T = G->T;
     const int GS = 16;
-    int *io = (int *)malloc(((size_t)E + 1) * sizeof(int));
-    int *ux = (int *)malloc(((size_t)T + 1) * sizeof(int));
-    int *vx = (int *)malloc(((size_t)T + 1) * sizeof(int));
-    int64_t *io64 = (int64_t *)malloc(((size_t)E + 1) * sizeof(int64_t));
-    int *dux = nullptr, *dvx = nullptr, *ident = nullptr;
+    // the caller frees these with free(): large ones on huge pages
+    int *io = (int *)host_alloc(((size_t)E + 1) * sizeof(int));
+    int *ux = (int *)host_alloc(((size_t)T + 1) * sizeof(int));
+    int *vx = (int *)host_alloc(((size_t)T + 1) * sizeof(int));
+    int *dux = nullptr, *dvx = nullptr, *ident = nullptr, *dio32 = nullptr;
     int64_t *dio = nullptr, *dcnt = nullptr;
     int2 *pairs = nullptr;
-    if (!io || !ux || !vx || !io64) set_error("host allocation failed for intercept arrays");
-    bool ok = io && ux && vx && io64 && dalloc(&dux, (size_t)T + 1, "ux") && dalloc(&dvx, (size_t)T + 1, "vx") &&
-              dalloc(&dio, (size_t)E + 1, "input offsets") && dalloc(&dcnt, (size_t)E + 1, "input counts") &&
+    if (!io || !ux || !vx) set_error("host allocation failed for intercept arrays");
+    bool ok = io && ux && vx && dalloc(&dux, (size_t)T + 1, "ux") && dalloc(&dvx, (size_t)T + 1, "vx") &&
+              dalloc(&dio, (size_t)E + 1, "input offsets") && dalloc(&dio32, (size_t)E + 1, "input offsets") &&
+              dalloc(&dcnt, (size_t)E + 1, "input counts") &&
               dalloc(&ident, (size_t)E + 1, "identity order") && dalloc(&pairs, (size_t)T + 1, "half-edge pairs");
     if (ok) {
         hipError_t e0 = hipMemcpyAsync(dcnt, G->cnt_in, (size_t)E * sizeof(int64_t), hipMemcpyDeviceToDevice, s);
@@ -995,28 +997,29 @@ bool download_index(DevGraph *G, int **ioff_out, int **ux_out, int **vx_out)
         hipLaunchKernelGGL(k_fill_i32, dim3(1), dim3(64), 0, s, (int *)(dcnt + E), (int64_t)2, 0);
         ok = ok && prim_exclusive_scan_i64(dcnt, dio, (size_t)E + 1, s);
     }
+    HubBuffers ref_hubs;
     if (ok) {
         if (E > 0 && T > 0) {
+            const HubBits ref_hub = build_hubs(G, ref_hubs, G->ref_row_off, G->ref_adj_x);
             hipLaunchKernelGGL(k_iota, dim3(grid_for(E, 8192)), dim3(TPB), 0, s, ident, E);
             hipLaunchKernelGGL((k_isect<GS, true>), dim3(grid_for((int64_t)E * GS, num_cus() * 32)), dim3(TPB), 0,
                                s, (const int *)G->ref_U, (const int *)G->ref_V, E, (const int *)G->ref_row_off,
                                (const int *)G->ref_adj_x, (const int *)ident, (int64_t *)nullptr,
-                               (const int64_t *)dio, (uint32_t *)nullptr, pairs, HubBits());
+                               (const int64_t *)dio, (uint32_t *)nullptr, pairs, ref_hub);
             hipLaunchKernelGGL(k_pairs_to_edges, dim3(grid_for(T, 8192)), dim3(TPB), 0, s, (const int2 *)pairs, T,
                                (const int *)G->ref_adj_e, dux, dvx);
         }
-        hipError_t e1 = hipMemcpyAsync(io64, dio, ((size_t)E + 1) * sizeof(int64_t), hipMemcpyDeviceToHost, s);
-        hipError_t e2 = T > 0 ? hipMemcpyAsync(ux, dux, (size_t)T * sizeof(int), hipMemcpyDeviceToHost, s) : hipSuccess;
-        hipError_t e3 = T > 0 ? hipMemcpyAsync(vx, dvx, (size_t)T * sizeof(int), hipMemcpyDeviceToHost, s) : hipSuccess;
-        hipError_t e4 = hipStreamSynchronize(s);
-        if (e1 != hipSuccess || e2 != hipSuccess || e3 != hipSuccess || e4 != hipSuccess) {
-            set_error("intercept download failed");
-            ok = false;
-        }
+        // offsets to int on the device (T < 2^31 was checked), then the
+        // host-parallel staged copies (their sources complete)
+        hipLaunchKernelGGL(k_i64_to_i32, dim3(grid_for((int64_t)E + 1, 8192)), dim3(TPB), 0, s, dio, dio32,
+                           (int64_t)E + 1);
+        ok = hipGetLastError() == hipSuccess && hipStreamSynchronize(s) == hipSuccess &&
+             d2h_parallel(io, dio32, ((size_t)E + 1) * sizeof(int)) &&
+             (T == 0 || (d2h_parallel(ux, dux, (size_t)T * sizeof(int)) &&
+                         d2h_parallel(vx, dvx, (size_t)T * sizeof(int))));
+        if (!ok) set_error("intercept download failed");
     }
-    if (ok) for (int e = 0; e <= E; e++) io[e] = (int)io64[e];
-    free(io64);
-    dfree(dux); dfree(dvx); dfree(dio); dfree(dcnt); dfree(ident); dfree(pairs);
+    dfree(dux); dfree(dvx); dfree(dio); dfree(dio32); dfree(dcnt); dfree(ident); dfree(pairs);
     if (!ok) { free(io); free(ux); free(vx); return false; }
     *ioff_out = io; *ux_out = ux; *vx_out = vx;
     return true;

@@ -102,6 +102,30 @@ def test_hub_bitmaps_bitwise(gpu, ora, variant, monkeypatch):
         assert np.array_equal(a["vertex_dress"], b["vertex_dress"]), (variant, t)
 
 
+@pytest.mark.parametrize("variant", [0, 1, 2, 3])
+def test_hub_bitmaps_intercepts(gpu, ora, variant, monkeypatch):
+    """The reference-layout intercept arrays (precompute_intercepts) built
+    through hub bitmaps of the reference rows equal the oracle's."""
+    monkeypatch.setenv("DRESS_HIP_HUB_DEGREE", "2")
+    rng = np.random.default_rng(90 + variant)
+    for t in range(3):
+        n, m = int(rng.integers(5, 120)), int(rng.integers(1, 700))
+        U = rng.integers(0, n, m).astype(np.int32)
+        V = rng.integers(0, n, m).astype(np.int32)
+        if t == 0:
+            keep = U != V
+            U, V = U[keep], V[keep]
+        with gpu.Graph(n, U, V, None, None, variant, 1) as g:
+            a = g.csr()
+        og = ora.graph(n, U, V, None, None, variant, 1)
+        try:
+            b = ora.csr(og)
+        finally:
+            ora.free_graph(og)
+        for f in ("intercept_offset", "intercept_edge_ux", "intercept_edge_vx"):
+            assert np.array_equal(a[f], b[f]), (variant, t, f)
+
+
 def test_hub_bitmaps_power_law(gpu, ora, gen, monkeypatch):
     """Rows of 64 or more neighbours of a 60,000-vertex power-law graph as hub
     bitmaps: the fit equals the oracle's bit for bit."""
