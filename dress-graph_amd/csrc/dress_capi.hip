@@ -198,6 +198,75 @@ int copy_threads()
     return t;
 }
 
+// Streams and events of the staged copies' host threads, kept for the
+// process: creating a stream took 8 ms here, and every large copy needs one
+// per thread.  Lane w belongs to copy thread w and is created by it on first
+// use.  One copy at a time holds the lanes (for one device); a copy that
+// finds them held creates and destroys its own.
+namespace {
+struct CopyLane {
+    hipStream_t st = nullptr;
+    hipEvent_t ev[2] = {nullptr, nullptr};
+};
+struct CopyLanes {
+    std::mutex mu;
+    int dev = -1;
+    CopyLane lane[32];
+};
+CopyLanes &copy_lanes()
+{
+    static CopyLanes *p = new CopyLanes();   // lives to process end
+    return *p;
+}
+bool make_lane(CopyLane &c)
+{
+    if (hipStreamCreateWithFlags(&c.st, hipStreamNonBlocking) == hipSuccess &&
+        hipEventCreateWithFlags(&c.ev[0], hipEventDisableTiming) == hipSuccess &&
+        hipEventCreateWithFlags(&c.ev[1], hipEventDisableTiming) == hipSuccess)
+        return true;
+    (void)hipGetLastError();
+    if (c.st) (void)hipStreamDestroy(c.st);
+    for (hipEvent_t &x : c.ev)
+        if (x) (void)hipEventDestroy(x);
+    c = CopyLane();
+    return false;
+}
+void drop_lane(CopyLane &c)
+{
+    if (c.st) (void)hipStreamDestroy(c.st);
+    for (hipEvent_t &x : c.ev)
+        if (x) (void)hipEventDestroy(x);
+    c = CopyLane();
+}
+// the lanes of one staged copy of T threads on device dev: the pooled ones
+// when free, else each thread's own
+struct LaneSet {
+    std::unique_lock<std::mutex> lk;
+    bool pooled = false;
+    LaneSet(int dev, int T) : lk(copy_lanes().mu, std::try_to_lock)
+    {
+        CopyLanes &P = copy_lanes();
+        pooled = lk.owns_lock() && T <= 32 && (P.dev < 0 || P.dev == dev);
+        if (pooled) P.dev = dev;
+    }
+    // lane of thread w (created on first use); *own when it is the thread's
+    // own, to be dropped after the copy
+    bool get(int w, CopyLane &out, bool *own)
+    {
+        *own = false;
+        if (pooled) {
+            CopyLane &c = copy_lanes().lane[w];
+            if (c.st || make_lane(c)) {
+                out = c;
+                return true;
+            }
+        }
+        *own = true;
+        return make_lane(out);
+    }
+};
+}  // namespace
+
 // Device-to-host copy of a large buffer (64 MiB and up) into pageable
 // memory (results the caller frees with free()).  T host threads
 // (copy_threads(), one per 32 MiB at most) each take a contiguous slice and
@@ -223,15 +292,16 @@ bool d2h_parallel(void *dst, const void *src, size_t bytes)
     char *staging = (char *)pinned_alloc((size_t)T * 2 * CH);
     if (!staging) return false;
     std::atomic<bool> ok{true};
+    LaneSet lanes(dev, T);
     auto slice = [&](int w) {
         hipSetDevice(dev);
         const size_t b = bytes * (size_t)w / (size_t)T, e = bytes * (size_t)(w + 1) / (size_t)T;
-        hipStream_t st = nullptr;
         char *stage[2] = {staging + (size_t)w * 2 * CH, staging + (size_t)w * 2 * CH + CH};
-        hipEvent_t ev[2] = {nullptr, nullptr};
-        bool good = stage[0] && stage[1] && hipStreamCreateWithFlags(&st, hipStreamNonBlocking) == hipSuccess &&
-                    hipEventCreateWithFlags(&ev[0], hipEventDisableTiming) == hipSuccess &&
-                    hipEventCreateWithFlags(&ev[1], hipEventDisableTiming) == hipSuccess;
+        CopyLane lane;
+        bool own = false;
+        bool good = lanes.get(w, lane, &own);
+        hipStream_t st = lane.st;
+        hipEvent_t *ev = lane.ev;
         auto issue = [&](size_t at, int j) {
             const size_t n = std::min(CH, e - at);
             return hipMemcpyAsync(stage[j], (const char *)src + at, n, hipMemcpyDeviceToHost, st) == hipSuccess &&
@@ -244,12 +314,8 @@ bool d2h_parallel(void *dst, const void *src, size_t bytes)
             good = good && hipEventSynchronize(ev[j]) == hipSuccess;
             if (good) memcpy((char *)dst + at, stage[j], std::min(CH, e - at));
         }
-        if (st) {
-            hipStreamSynchronize(st);
-            hipStreamDestroy(st);
-        }
-        for (hipEvent_t x : ev)
-            if (x) hipEventDestroy(x);
+        if (st && hipStreamSynchronize(st) != hipSuccess) good = false;
+        if (own) drop_lane(lane);
         if (!good) ok = false;
     };
     std::vector<std::thread> pool;
@@ -304,15 +370,16 @@ bool h2d_parallel(void *dst, const void *src, size_t bytes)
     char *staging = (char *)pinned_alloc((size_t)T * 2 * CH);
     if (!staging) return false;
     std::atomic<bool> ok{true};
+    LaneSet lanes(dev, T);
     auto slice = [&](int w) {
         hipSetDevice(dev);
         const size_t b = bytes * (size_t)w / (size_t)T, e = bytes * (size_t)(w + 1) / (size_t)T;
-        hipStream_t st = nullptr;
         char *stage[2] = {staging + (size_t)w * 2 * CH, staging + (size_t)w * 2 * CH + CH};
-        hipEvent_t ev[2] = {nullptr, nullptr};
-        bool good = stage[0] && stage[1] && hipStreamCreateWithFlags(&st, hipStreamNonBlocking) == hipSuccess &&
-                    hipEventCreateWithFlags(&ev[0], hipEventDisableTiming) == hipSuccess &&
-                    hipEventCreateWithFlags(&ev[1], hipEventDisableTiming) == hipSuccess;
+        CopyLane lane;
+        bool own = false;
+        bool good = lanes.get(w, lane, &own);
+        hipStream_t st = lane.st;
+        hipEvent_t *ev = lane.ev;
         int j = 0;
         bool used[2] = {false, false};
         for (size_t at = b; good && at < e; at += CH, j ^= 1) {
@@ -325,12 +392,8 @@ bool h2d_parallel(void *dst, const void *src, size_t bytes)
                    hipEventRecord(ev[j], st) == hipSuccess;
             used[j] = true;
         }
-        if (st) {
-            if (hipStreamSynchronize(st) != hipSuccess) good = false;
-            hipStreamDestroy(st);
-        }
-        for (hipEvent_t x : ev)
-            if (x) hipEventDestroy(x);
+        if (st && hipStreamSynchronize(st) != hipSuccess) good = false;
+        if (own) drop_lane(lane);
         if (!good) ok = false;
     };
     std::vector<std::thread> pool;
