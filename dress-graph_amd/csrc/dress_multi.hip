@@ -148,6 +148,7 @@ template <int MB, int GS>
 __global__ void __launch_bounds__(TPB) k_mvertex(const int *__restrict__ list, int count, MultiArgs a)
 {
     constexpr int L = MB * GS;
+    if (*a.remaining == 0) return;             // every slot has stopped
     __shared__ SlotCache<MB> sc;
     __shared__ double ls[TPB / 64][64], lc[TPB / 64][64];
     load_slots<MB>(a, sc);
@@ -191,6 +192,7 @@ __device__ __forceinline__ void mentry(const MultiArgs &a, int64_t k, int2 r, in
 // invalidates its XCD's L2, which holds the slots' state.)
 __global__ void k_mend(MultiArgs a)
 {
+    if (*a.remaining == 0) return;
     SlotState *st = a.st;
     const int n = a.n;
     __shared__ int cnt;
@@ -215,6 +217,7 @@ template <int MB, int GS, bool PK>
 __global__ void __launch_bounds__(TPB) k_medge(int lo, int hi, MultiArgs a)
 {
     constexpr int L = MB * GS;
+    if (*a.remaining == 0) return;             // every slot has stopped
     __shared__ SlotCache<MB> sc;
     __shared__ double ls[TPB / 64][64], lc[TPB / 64][64];
     load_slots<MB>(a, sc);
@@ -285,10 +288,11 @@ __global__ void __launch_bounds__(TPB) k_medge(int lo, int hi, MultiArgs a)
     }
 }
 
-// fresh slot states for a pass of n items
-__global__ void k_mslots(SlotState *st, int n, int mb)
+// fresh slot states for a pass of n items, all of them active
+__global__ void k_mslots(SlotState *st, int n, int mb, int *remaining)
 {
     const int i = threadIdx.x;
+    if (i == 0) *remaining = n;
     if (i < mb) {
         SlotState z;
         z.active = i < n ? 1 : 0;
@@ -447,6 +451,19 @@ size_t pass_bytes(const DevGraph *G, int mb)
     return (size_t)mb * (4 * (size_t)G->E + 2 * (size_t)G->S + (size_t)G->N + 8) * sizeof(double);
 }
 
+// two pinned ints per host thread for the passes' active-slot snapshots,
+// allocated on the thread's first pass and never freed
+int *pass_snapshots()
+{
+    static thread_local int *slots = nullptr;
+    if (!slots && hipHostMalloc((void **)&slots, 2 * sizeof(int), hipHostMallocDefault) != hipSuccess) {
+        (void)hipGetLastError();
+        slots = nullptr;
+        set_error("pinned allocation of the pass snapshots failed");
+    }
+    return slots;
+}
+
 template <int MB>
 bool run_passes(DevGraph *G, int kind, int k, const std::vector<int> &verts, const std::vector<int64_t> &live,
                 const std::vector<int64_t> &sel, const double *marks, int max_it, double eps, double *d_vals,
@@ -484,6 +501,10 @@ bool run_passes(DevGraph *G, int kind, int k, const std::vector<int> &verts, con
     a.max_it = max_it;
     a.remaining = d_rem;
     int cap = std::max(1, num_cus() * 8 * thread_grid_share() / 100);
+    int *snap = pass_snapshots();
+    hipEvent_t sev[2] = {nullptr, nullptr};
+    ok = ok && snap && hipEventCreateWithFlags(&sev[0], hipEventDisableTiming) == hipSuccess &&
+         hipEventCreateWithFlags(&sev[1], hipEventDisableTiming) == hipSuccess;
     std::vector<int> slot_item, tin;
     std::vector<int64_t> sofs, slim;
     std::vector<SlotState> hst(MB);
@@ -522,24 +543,43 @@ bool run_passes(DevGraph *G, int kind, int k, const std::vector<int> &verts, con
         if (!tin.empty())
             hipLaunchKernelGGL(k_mrank, dim3(grid_for((int64_t)tin.size(), 64)), dim3(TPB), 0, s, d_in,
                                (int)tin.size(), G->rank, d_tup);
-        hipLaunchKernelGGL(k_mslots, dim3(1), dim3(TPB), 0, s, d_st, n, MB);
+        hipLaunchKernelGGL(k_mslots, dim3(1), dim3(TPB), 0, s, d_st, n, MB, d_rem);
         if (E > 0) hipLaunchKernelGGL(k_minit<MB>, dim3(grid_for((int64_t)E * MB, cap)), dim3(TPB), 0, s, E, a);
-        int launched = 0, remaining = n, chunk = 4;
-        while (max_it > 0 && remaining > 0 && launched < max_it) {
-            if (launched > 0) yield_to_pressure(0.05);
-            cap = std::max(1, num_cus() * 8 * thread_grid_share() / 100);
-            const int m = std::min(chunk, max_it - launched);
+        // chunks of 4 iterations, each followed by a copy of the active-slot
+        // count into a pinned snapshot slot; the next chunk is launched
+        // before the last snapshot is read, and iterations launched after
+        // every slot stopped return at once
+        int launched = 0, remaining = n;
+        auto issue = [&](int slot) {
+            const int m = std::min(4, max_it - launched);
             for (int j = 0; j < m; j++) {
                 if (G->packed) Pass<MB>::template iteration<true>(G, a, cap);
                 else Pass<MB>::template iteration<false>(G, a, cap);
             }
             launched += m;
-            ok = hipGetLastError() == hipSuccess &&
-                 hipMemcpyAsync(&remaining, d_rem, sizeof(int), hipMemcpyDeviceToHost, s) == hipSuccess &&
-                 hipStreamSynchronize(s) == hipSuccess;
+            return hipGetLastError() == hipSuccess &&
+                   hipMemcpyAsync(&snap[slot], d_rem, sizeof(int), hipMemcpyDeviceToHost, s) == hipSuccess &&
+                   hipEventRecord(sev[slot], s) == hipSuccess;
+        };
+        int pend = 0;
+        if (max_it > 0) ok = issue(0);
+        while (ok && max_it > 0) {
+            int nxt = -1;
+            if (launched < max_it) {
+                yield_to_pressure(0.05);
+                cap = std::max(1, num_cus() * 8 * thread_grid_share() / 100);
+                ok = issue(pend ^ 1);
+                nxt = pend ^ 1;
+            }
+            ok = ok && hipEventSynchronize(sev[pend]) == hipSuccess;
             if (!ok) break;
-            chunk = 4;
+            remaining = snap[pend];
+            if (remaining == 0 || nxt < 0) break;
+            pend = nxt;
         }
+        // the count after everything launched
+        ok = ok && hipMemcpyAsync(&remaining, d_rem, sizeof(int), hipMemcpyDeviceToHost, s) == hipSuccess &&
+             hipStreamSynchronize(s) == hipSuccess;
         if (!ok) break;
         if (max_it > 0 && remaining > 0) {
             set_error("subgraph fits did not stop within max_iterations");
@@ -563,6 +603,8 @@ bool run_passes(DevGraph *G, int kind, int k, const std::vector<int> &verts, con
                 ok = false;
             }
     }
+    for (hipEvent_t x : sev)
+        if (x) (void)hipEventDestroy(x);
     dfree(a.d0); dfree(a.d1); dfree(a.w0); dfree(a.w1); dfree(a.vd); dfree(d_st);
     dfree(d_in); dfree(d_tup); dfree(d_marks); dfree(d_rem); dfree(d_ofs); dfree(d_lim); dfree(d_rows);
     if (!ok && !has_error()) set_error("multi-subgraph fits failed");
