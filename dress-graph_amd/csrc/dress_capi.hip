@@ -198,6 +198,49 @@ int copy_threads()
     return t;
 }
 
+// Small pinned blocks for the state snapshots of running fits and passes,
+// taken per call and returned: the pool holds at most as many as calls ran
+// at once, kept for the process.
+namespace {
+struct SnapPool {
+    std::mutex mu;
+    std::vector<void *> free_blocks;
+};
+SnapPool &snap_pool()
+{
+    static SnapPool *p = new SnapPool();      // lives to process end
+    return *p;
+}
+}  // namespace
+
+void *snapshot_take()
+{
+    SnapPool &P = snap_pool();
+    {
+        std::lock_guard<std::mutex> lk(P.mu);
+        if (!P.free_blocks.empty()) {
+            void *q = P.free_blocks.back();
+            P.free_blocks.pop_back();
+            return q;
+        }
+    }
+    void *q = nullptr;
+    if (hipHostMalloc(&q, SNAPSHOT_BYTES, hipHostMallocDefault) != hipSuccess) {
+        (void)hipGetLastError();
+        set_error("pinned allocation of a state snapshot block failed");
+        return nullptr;
+    }
+    return q;
+}
+
+void snapshot_give(void *q)
+{
+    if (!q) return;
+    SnapPool &P = snap_pool();
+    std::lock_guard<std::mutex> lk(P.mu);
+    P.free_blocks.push_back(q);
+}
+
 // Streams and events of the staged copies' host threads, kept for the
 // process: creating a stream took 8 ms here, and every large copy needs one
 // per thread.  Lane w belongs to copy thread w and is created by it on first

@@ -53,20 +53,7 @@ struct FitState {
                              // pure FP64 run, so the host reruns in FP64
 };
 
-// two pinned FitState slots per host thread for the fit loop's state
-// snapshots, allocated on the thread's first fit and never freed (a few
-// hundred bytes per thread that ever fits)
-FitState *fit_snapshots()
-{
-    static thread_local FitState *slots = nullptr;
-    if (!slots && hipHostMalloc((void **)&slots, 2 * sizeof(FitState), hipHostMallocDefault) != hipSuccess) {
-        (void)hipGetLastError();
-        slots = nullptr;
-        set_error("pinned allocation of the fit state snapshots failed");
-    }
-    return slots;
-}
-
+static_assert(2 * sizeof(FitState) <= SNAPSHOT_BYTES, "two fit state snapshots per block");
 
 namespace {
 
@@ -588,8 +575,18 @@ bool fit_run_once(DevGraph *G, int max_iterations, double epsilon, bool mixed, F
     // pinned snapshot slot.  The next chunk is launched before the last
     // snapshot is read, so the device does not wait for the host between
     // chunks; iterations launched past convergence return at once.
-    FitState *snap = fit_snapshots();
+    FitState *snap = (FitState *)snapshot_take();
     if (!snap) return false;
+    // back to the pool once no copy into it can be in flight
+    struct Give {
+        void *p;
+        hipStream_t s;
+        ~Give()
+        {
+            if (p) (void)hipStreamSynchronize(s);
+            snapshot_give(p);
+        }
+    } give{snap, s};
     hipEvent_t sev[2];
     DH_CHECK(hipEventCreateWithFlags(&sev[0], hipEventDisableTiming));
     DH_CHECK(hipEventCreateWithFlags(&sev[1], hipEventDisableTiming));

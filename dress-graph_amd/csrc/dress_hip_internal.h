@@ -76,6 +76,12 @@ void pinned_free(void *p);
 // large copies between pageable host memory and the device, host-parallel
 bool d2h_parallel(void *dst, const void *src, size_t bytes);
 bool h2d_parallel(void *dst, const void *src, size_t bytes);
+// pinned blocks of SNAPSHOT_BYTES for the state snapshots of a running fit
+// or pass: take at its start, give back at its end (nullptr on failure,
+// with the error set)
+constexpr size_t SNAPSHOT_BYTES = 256;
+void *snapshot_take();
+void snapshot_give(void *p);
 // n doubles set to a value (NaN), by several host threads when n is large
 void fill_parallel(double *p, size_t n, double value);
 void fill_nan_parallel(double *p, size_t n);

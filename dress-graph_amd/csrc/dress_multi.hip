@@ -451,19 +451,6 @@ size_t pass_bytes(const DevGraph *G, int mb)
     return (size_t)mb * (4 * (size_t)G->E + 2 * (size_t)G->S + (size_t)G->N + 8) * sizeof(double);
 }
 
-// two pinned ints per host thread for the passes' active-slot snapshots,
-// allocated on the thread's first pass and never freed
-int *pass_snapshots()
-{
-    static thread_local int *slots = nullptr;
-    if (!slots && hipHostMalloc((void **)&slots, 2 * sizeof(int), hipHostMallocDefault) != hipSuccess) {
-        (void)hipGetLastError();
-        slots = nullptr;
-        set_error("pinned allocation of the pass snapshots failed");
-    }
-    return slots;
-}
-
 template <int MB>
 bool run_passes(DevGraph *G, int kind, int k, const std::vector<int> &verts, const std::vector<int64_t> &live,
                 const std::vector<int64_t> &sel, const double *marks, int max_it, double eps, double *d_vals,
@@ -501,7 +488,17 @@ bool run_passes(DevGraph *G, int kind, int k, const std::vector<int> &verts, con
     a.max_it = max_it;
     a.remaining = d_rem;
     int cap = std::max(1, num_cus() * 8 * thread_grid_share() / 100);
-    int *snap = pass_snapshots();
+    int *snap = (int *)snapshot_take();
+    // back to the pool once no copy into it can be in flight
+    struct Give {
+        void *p;
+        hipStream_t s;
+        ~Give()
+        {
+            if (p) (void)hipStreamSynchronize(s);
+            snapshot_give(p);
+        }
+    } give{snap, s};
     hipEvent_t sev[2] = {nullptr, nullptr};
     ok = ok && snap && hipEventCreateWithFlags(&sev[0], hipEventDisableTiming) == hipSuccess &&
          hipEventCreateWithFlags(&sev[1], hipEventDisableTiming) == hipSuccess;
