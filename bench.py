@@ -83,16 +83,23 @@ def cpu_baseline_fit(N, U, V, csr, iterations=2):
     sys.path.insert(0, os.path.join(ROOT, "oracle"))
     import pyoracle
     if not pyoracle.available("ref"):
-        return None, "oracle"
+        return None, "oracle", None
     lib = pyoracle.Lib("ref")
     g = pyoracle.ref_graph_from_csr(lib, N, U, V, csr)
     try:
         t0 = time.perf_counter()
         lib.fit_graph(g, iterations, 1e-6, parallel=True)
         dt = time.perf_counter() - t0
+        # dress_get of the same library on a sample of the bench's virtual-edge
+        # pairs (one host thread: the reference's dress_get is serial)
+        qu, qv, _, _ = get_pairs(N, U, V)
+        t0 = time.perf_counter()
+        for a, b in zip(qu[:GET_SAMPLE].tolist(), qv[:GET_SAMPLE].tolist()):
+            lib.get(g, a, b, 100, 1e-6, 1.0)
+        get_rate = GET_SAMPLE / (time.perf_counter() - t0)
     finally:
         lib.free_graph(g)
-    return U.size * iterations / dt, dt
+    return U.size * iterations / dt, dt, get_rate
 
 
 def cpu_baseline_delta1(graphs):
@@ -105,6 +112,35 @@ def cpu_baseline_delta1(graphs):
         _, _, _, num = lib.lifted("delta", n, U, V, k=1, use_omp=True)
         solves += num
     return solves / (time.perf_counter() - t0), lib.which
+
+
+GET_SAMPLE = 20_000          # reference dress_get queries timed on the host
+
+
+def get_pairs(N, U, V, n=1_000_000, seed=3):
+    """n random vertex pairs (almost all of them non-edges) and n edges."""
+    rng = np.random.default_rng(seed)
+    qu = rng.integers(0, N, n).astype(np.int32)
+    qv = rng.integers(0, N, n).astype(np.int32)
+    eidx = rng.integers(0, U.size, n)
+    return qu, qv, np.ascontiguousarray(U[eidx]), np.ascontiguousarray(V[eidx])
+
+
+def get_queries(g, N, U, V):
+    """dress_get batched on the fitted graph: one device launch per million
+    queries, host arrays in and out."""
+    qu, qv, eu, ev = get_pairs(N, U, V)
+    g.get_batch(qu[:1000], qv[:1000])                          # warm-up
+    t0 = time.perf_counter()
+    g.get_batch(qu, qv)
+    tv = time.perf_counter() - t0
+    t0 = time.perf_counter()
+    g.get_batch(eu, ev)
+    te = time.perf_counter() - t0
+    return {"workload": "1,000,000 random vertex pairs (virtual edges) and 1,000,000 existing edges of the "
+                        "fitted 4M-vertex graph, max 100 iterations, eps 1e-6",
+            "virtual_pairs_per_s": qu.size / tv, "edge_queries_per_s": eu.size / te,
+            "virtual_ms": tv * 1e3, "edge_ms": te * 1e3}
 
 
 def delta1_large(dh, gen):
@@ -183,6 +219,9 @@ def main():
     r64 = fit_steps(g, args.steps, dh.FP64)
     log(f"FP64: {r64['iterations_executed'] / args.steps:.0f} iterations, "
         f"{r64['seconds'] / args.steps * 1e3:.1f} ms/fit")
+    get_res = get_queries(g, N, U, V)
+    log(f"dress_get: {get_res['virtual_pairs_per_s']:.3g} virtual-edge queries/s, "
+        f"{get_res['edge_queries_per_s']:.3g} edge queries/s")
     fit_steps(g, 1, dh.FP32_FP64)
     r32 = fit_steps(g, args.steps, dh.FP32_FP64)
     log(f"FP32->FP64: {r32['seconds'] / args.steps * 1e3:.1f} ms/fit")
@@ -236,6 +275,7 @@ def main():
                       "fp32_iterations_per_fit": r32["fp32_iterations"] / args.steps,
                       "speedup_vs_fp64": value32 / value},
         "phase_ms": {"vertex": [float(x) for x in vms], "edge": [float(x) for x in ems]},
+        "dress_get": get_res,
     }
 
     if not args.no_delta:
@@ -311,12 +351,17 @@ def main():
         cores = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
         with dh.Graph(N, U, V, precompute_intercepts=True) as gi:
             csr = gi.csr()
-        cpu_v, dt = cpu_baseline_fit(N, U, V, csr, iterations=2)
+        cpu_v, dt, get_rate = cpu_baseline_fit(N, U, V, csr, iterations=2)
         del csr
         if cpu_v is not None:
             out["cpu_baseline"] = {"value": cpu_v, "unit": "edge-updates/s", "cores": cores, "kind": "reference",
                                    "sample": f"2 of the {iters_per_fit:.0f} iterations of the same fit, libdress "
-                                             f"dress_fit_omp with its intercept index prebuilt ({dt:.1f} s)"}
+                                             f"dress_fit_omp with its intercept index prebuilt; the 2 iterations "
+                                             f"took {dt:.1f} s"}
+        if get_rate is not None:
+            out["dress_get"]["cpu_baseline"] = {"virtual_pairs_per_s": get_rate, "cores": 1, "kind": "reference",
+                                                "sample": f"the first {GET_SAMPLE:,} virtual-edge pairs, libdress "
+                                                          "dress_get on the graph after the 2 iterations above"}
         if "regular12_n256" in out:
             sys.path.insert(0, os.path.join(ROOT, "oracle"))
             import pyoracle
