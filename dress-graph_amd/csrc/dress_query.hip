@@ -1,4 +1,4 @@
-// dress_query.hip -- dress_get on the device, one thread per query
+// dress_query.hip -- dress_get on the device, a group of lanes per query
 // (libdress/src/dress.c:651-739).
 //
 // A query (u, v) returns the value of an existing edge found by the same
@@ -10,6 +10,7 @@
 // uploaded before the queries, so results match what the reference computes
 // on g.
 #include "dress_capi_internal.h"
+#include "dress_device.h"
 #include "dress/hip/dress_hip.h"
 
 #include <cmath>
@@ -21,81 +22,102 @@ namespace dh {
 namespace {
 
 constexpr int TPB = 256;
+constexpr int GQ = 16;                 // lanes per query
 
-__device__ __forceinline__ void two_sum_add(double &s, double &c, double x)
+// first index in [lo, hi) whose value is >= x
+__device__ __forceinline__ int lower_bound_q(const int *a, int lo, int hi, int x)
 {
-    double t = s + x;
-    double z = t - s;
-    double e = (s - (t - z)) + (x - z);
-    s = t;
-    c += e;
+    while (lo < hi) {
+        int mid = (lo + hi) >> 1;
+        if (a[mid] < x) lo = mid + 1; else hi = mid;
+    }
+    return lo;
 }
 
 // Rows are the reference's adjacency (input labels, neighbours ascending),
-// so the binary search probes the same entries as dress.c:656-667 and
-// returns the same copy of a repeated edge.  Half-edge values are
-// edge_weight * edge_dress of the edge the half-edge belongs to.
-__global__ void k_get(int n, const int *__restrict__ qu, const int *__restrict__ qv, const double *__restrict__ qw,
-                      int max_it, double eps, const int *__restrict__ row_off, const int *__restrict__ adj_x,
-                      const int *__restrict__ adj_e, const int *__restrict__ inv, const double *__restrict__ d,
-                      const double *__restrict__ ew, const double *__restrict__ vdn, const double *__restrict__ nw,
-                      int variant, double *out)
+// so the binary search (lane 0 of the query's group) probes the same entries
+// as dress.c:656-667 and returns the same copy of a repeated edge.  For a
+// virtual edge the group walks the shorter row and looks each neighbour up
+// in the longer one; the i-th copy of a neighbour in one row pairs with its
+// i-th copy in the other, as the reference's merge does (dress.c:679-693).
+// Half-edge values are edge_weight * edge_dress of the edge the half-edge
+// belongs to; the group's compensated partial sums merge exactly, so the
+// sum is the serial merge's.
+__global__ void __launch_bounds__(TPB) k_get(int n, const int *__restrict__ qu, const int *__restrict__ qv,
+                                             const double *__restrict__ qw, int max_it, double eps,
+                                             const int *__restrict__ row_off, const int *__restrict__ adj_x,
+                                             const int *__restrict__ adj_e, const int *__restrict__ inv,
+                                             const double *__restrict__ d, const double *__restrict__ ew,
+                                             const double *__restrict__ vdn, const double *__restrict__ nw,
+                                             int variant, double *out)
 {
-    for (int q = blockIdx.x * blockDim.x + threadIdx.x; q < n; q += gridDim.x * blockDim.x) {
+    const int lane = threadIdx.x % GQ;
+    const int64_t groups = (int64_t)gridDim.x * (TPB / GQ);
+    for (int64_t q = (int64_t)blockIdx.x * (TPB / GQ) + threadIdx.x / GQ; q < n; q += groups) {
         const int u = qu[q], v = qv[q];
         // dress.c:656-667: existing edge
-        int lo = row_off[u], hi = row_off[u + 1] - 1;
         double res = 0.0;
-        bool found = false;
-        while (lo <= hi) {
-            int mid = lo + (hi - lo) / 2;
-            int nb = adj_x[mid];
-            if (nb == v) {
-                res = d[inv[adj_e[mid]]];
-                found = true;
-                break;
+        int found = 0;
+        if (lane == 0) {
+            int lo = row_off[u], hi = row_off[u + 1] - 1;
+            while (lo <= hi) {
+                int mid = lo + (hi - lo) / 2;
+                int nb = adj_x[mid];
+                if (nb == v) {
+                    res = d[inv[adj_e[mid]]];
+                    found = 1;
+                    break;
+                }
+                if (nb < v) lo = mid + 1; else hi = mid - 1;
             }
-            if (nb < v) lo = mid + 1; else hi = mid - 1;
         }
+        found = __shfl(found, 0, GQ);
         if (!found) {
             // dress.c:671-737: virtual edge
-            int a = row_off[u], ae = row_off[u + 1];
-            int b = row_off[v], be = row_off[v + 1];
+            const int ub = row_off[u], ue = row_off[u + 1];
+            const int vb = row_off[v], ve = row_off[v + 1];
+            const bool a_is_u = (ue - ub) <= (ve - vb);
+            const int ab = a_is_u ? ub : vb, ae = a_is_u ? ue : ve;
+            const int bb = a_is_u ? vb : ub, be = a_is_u ? ve : ue;
             double s = 0.0, c = 0.0;
-            while (a < ae && b < be) {
-                int x = adj_x[a], y = adj_x[b];
-                if (x == y) {
-                    const int ea = adj_e[a], eb = adj_e[b];
+            for (int p = ab + lane; p < ae; p += GQ) {
+                const int x = adj_x[p];
+                const int first = (p > ab && adj_x[p - 1] == x) ? lower_bound_q(adj_x, ab, p, x) : p;
+                const int r = p - first;
+                const int lo = lower_bound_q(adj_x, bb, be, x);
+                int have = 0;
+                while (lo + have < be && adj_x[lo + have] == x && have <= r) have++;
+                if (r < have) {
+                    const int ea = adj_e[a_is_u ? p : lo + r], eb = adj_e[a_is_u ? lo + r : p];
                     two_sum_add(s, c, ew[ea] * d[inv[ea]] + ew[eb] * d[inv[eb]]);
-                    a++;
-                    b++;
-                } else if (x < y) a++;
-                else b++;
-            }
-            const bool dir = (variant == DRESS_VARIANT_FORWARD || variant == DRESS_VARIANT_BACKWARD);
-            two_sum_add(s, c, dir ? 4.0 * nw[u] : 4.0 * nw[u] + 4.0 * nw[v]);
-            const double A = s + c;
-            const double cross = dir ? 1.0 : 2.0;
-            const double cw = (variant == DRESS_VARIANT_UNDIRECTED) ? 2.0 * qw[q] : qw[q];
-            const double du2 = vdn[u] * vdn[u];
-            const double dv2 = vdn[v] * vdn[v];
-            if (du2 <= 0.0 || dv2 <= 0.0) {
-                res = 0.0;
-            } else {
-                double x = 1.0;
-                res = x;
-                for (int it = 0; it < max_it; it++) {
-                    double wd = cw * x;
-                    double den = sqrt(du2 + wd) * sqrt(dv2 + wd);
-                    if (den <= 0.0) { x = 0.0; break; }
-                    double nx = (A + cross * wd) / den;
-                    if (fabs(nx - x) <= eps) { x = nx; break; }
-                    x = nx;
                 }
-                res = x;
+            }
+            group_reduce<GQ>(s, c);
+            if (lane == 0) {
+                const bool dir = (variant == DRESS_VARIANT_FORWARD || variant == DRESS_VARIANT_BACKWARD);
+                two_sum_add(s, c, dir ? 4.0 * nw[u] : 4.0 * nw[u] + 4.0 * nw[v]);
+                const double A = s + c;
+                const double cross = dir ? 1.0 : 2.0;
+                const double cw = (variant == DRESS_VARIANT_UNDIRECTED) ? 2.0 * qw[q] : qw[q];
+                const double du2 = vdn[u] * vdn[u];
+                const double dv2 = vdn[v] * vdn[v];
+                if (du2 <= 0.0 || dv2 <= 0.0) {
+                    res = 0.0;
+                } else {
+                    double x = 1.0;
+                    for (int it = 0; it < max_it; it++) {
+                        double wd = cw * x;
+                        double den = sqrt(du2 + wd) * sqrt(dv2 + wd);
+                        if (den <= 0.0) { x = 0.0; break; }
+                        double nx = (A + cross * wd) / den;
+                        if (fabs(nx - x) <= eps) { x = nx; break; }
+                        x = nx;
+                    }
+                    res = x;
+                }
             }
         }
-        out[q] = res;
+        if (lane == 0) out[q] = res;
     }
 }
 
@@ -131,7 +153,8 @@ bool get_batch(p_dress_graph_t g, int n, const int *u, const int *v, const doubl
                             hipMemcpyAsync(dnw, g->NW, (size_t)g->N * sizeof(double), hipMemcpyHostToDevice,
                                            s) == hipSuccess));
         if (ok && n > 0)
-            hipLaunchKernelGGL(k_get, dim3(std::min(8192, (n + TPB - 1) / TPB)), dim3(TPB), 0, s, n, du, dv, dw,
+            hipLaunchKernelGGL(k_get, dim3((int)std::min<int64_t>(8192, ((int64_t)n * GQ + TPB - 1) / TPB)), dim3(TPB),
+                               0, s, n, du, dv, dw,
                                max_it, eps, G->ref_row_off, G->ref_adj_x, G->ref_adj_e, G->inv, G->d[G->cur],
                                G->ew_in, dvd, dnw, G->variant, dout);
         ok = ok && hipGetLastError() == hipSuccess &&
