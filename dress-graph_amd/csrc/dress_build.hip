@@ -1097,13 +1097,14 @@ bool upload_reference_csr(DevGraph *G, const int *adj_offset, const int *adj_tar
     if (!G->ref_row_off && !dalloc(&G->ref_row_off, (size_t)G->N + 1, "reference row offsets")) return false;
     if (!G->ref_adj_x && !dalloc(&G->ref_adj_x, (size_t)G->S + 1, "reference adjacency")) return false;
     if (!G->ref_adj_e && !dalloc(&G->ref_adj_e, (size_t)G->S + 1, "reference adjacency")) return false;
-    DH_CHECK(hipMemcpyAsync(G->ref_row_off, adj_offset, ((size_t)G->N + 1) * sizeof(int), hipMemcpyHostToDevice, s));
-    if (G->S > 0) {
-        DH_CHECK(hipMemcpyAsync(G->ref_adj_x, adj_target, (size_t)G->S * sizeof(int), hipMemcpyHostToDevice, s));
-        DH_CHECK(hipMemcpyAsync(G->ref_adj_e, adj_edge_idx, (size_t)G->S * sizeof(int), hipMemcpyHostToDevice, s));
-    }
+    // the caller's pageable arrays, large ones through the host-parallel
+    // staged copy (the blocks may still be read by earlier work on s)
     DH_CHECK(hipStreamSynchronize(s));
-    return true;
+    const bool ok = h2d_parallel(G->ref_row_off, adj_offset, ((size_t)G->N + 1) * sizeof(int)) &&
+                    (G->S == 0 || (h2d_parallel(G->ref_adj_x, adj_target, (size_t)G->S * sizeof(int)) &&
+                                   h2d_parallel(G->ref_adj_e, adj_edge_idx, (size_t)G->S * sizeof(int))));
+    if (!ok) set_error("upload of the reference adjacency failed");
+    return ok;
 }
 
 void free_graph(DevGraph *G)
