@@ -870,14 +870,15 @@ bool build_index(DevGraph *G)
                                G->pos_in);
         hipLaunchKernelGGL(k_order_positions, dim3(grid_for(E, 8192)), dim3(TPB), 0, s, G->pos_in, E);
         hub = build_hubs(G, hubs, G->row_off, G->adj_x);
-        // the count walks the edges in the order of their first half-edge,
-        // so the edges of one row run together (71 -> 33 ms on the
-        // 4M-vertex graph against input order); without room for the
-        // sort it walks them in input order
+        // from 1M edges on, the count walks the edges in the order of their
+        // first half-edge, so the edges of one row run together (71 -> 33
+        // ms on the 4M-vertex graph against input order); smaller graphs,
+        // or a build without room for the sort, walk them in input order
         uint64_t *ck0 = nullptr, *ck1 = nullptr;
         int *cv0 = nullptr, *cord = nullptr;
-        bool sorted = dalloc(&ck0, (size_t)E + 1, "count order keys") && dalloc(&ck1, (size_t)E + 1, "count order keys") &&
-                      dalloc(&cv0, (size_t)E + 1, "count order") && dalloc(&cord, (size_t)E + 1, "count order");
+        bool sorted = E >= (1 << 20) && dalloc(&ck0, (size_t)E + 1, "count order keys") &&
+                      dalloc(&ck1, (size_t)E + 1, "count order keys") && dalloc(&cv0, (size_t)E + 1, "count order") &&
+                      dalloc(&cord, (size_t)E + 1, "count order");
         if (sorted) {
             int bits = 1;
             while (bits < 32 && ((int64_t)1 << bits) <= G->S) bits++;

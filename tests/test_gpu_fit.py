@@ -127,10 +127,12 @@ def test_hub_bitmaps_intercepts(gpu, ora, variant, monkeypatch):
 
 
 def test_hub_bitmaps_power_law(gpu, ora, gen, monkeypatch):
-    """Rows of 64 or more neighbours of a 60,000-vertex power-law graph as hub
-    bitmaps: the fit equals the oracle's bit for bit."""
+    """Rows of 64 or more neighbours of a 150,000-vertex power-law graph as hub
+    bitmaps, and the intersection count in row order (graphs of 1M edges and
+    more): the fit equals the oracle's bit for bit."""
     monkeypatch.setenv("DRESS_HIP_HUB_DEGREE", "64")
-    n, U, V = gen.chung_lu(n=60_000, n_edges=420_000, seed=5)
+    n, U, V = gen.chung_lu(n=150_000, n_edges=1_100_000, seed=5)
+    assert U.size >= 1 << 20
     a = _fit(gpu, n, U, V)
     b = ora.fit(n, U, V, parallel=True)
     assert a["iterations"] == b["iterations"] and a["delta"] == b["delta"]
