@@ -962,6 +962,7 @@ bool build_bins(DevGraph *G)
 
 bool refresh_self_terms(DevGraph *G, const double *NW_host)
 {
+    G->fit_state_valid = false;
     hipStream_t s = G->stream;
     if (NW_host && G->N > 0) {
         if (!G->vstage && !dalloc(&G->vstage, (size_t)G->N + 1, "vertex staging")) return false;

@@ -726,6 +726,7 @@ int fit_impl(p_dress_graph_t g, int max_iterations, double epsilon, int precisio
         !fit_run(G, max_iterations, epsilon, precision, &r) ||
         !fit_download(G, g->edge_dress, g->vertex_dress))
         return -1;
+    G->fit_state_valid = true;
     record_stats(G, r);
     if (iterations) *iterations = r.iterations;
     if (delta) *delta = r.delta;
@@ -919,6 +920,7 @@ int dress_hip_fit_resident(p_dress_graph_t g, int max_iterations, double epsilon
     if (!fit_run(G, max_iterations, epsilon,
                  precision == DRESS_HIP_FP32_FP64 ? PREC_FP32_FP64 : PREC_FP64, &r))
         return -1;
+    G->fit_state_valid = true;
     record_stats(G, r);
     if (iterations) *iterations = r.iterations;
     if (delta) *delta = r.delta;
@@ -940,6 +942,7 @@ int dress_hip_time_phases(p_dress_graph_t g, int reps, double *ms)
     DevGraph *G = g ? lookup(g) : nullptr;
     if (!G || !G->d[0]) { set_error("dress_hip_time_phases: call dress_hip_resident_reset first"); return -1; }
     if (!ms || reps < 1) { set_error("dress_hip_time_phases: bad arguments"); return -1; }
+    G->fit_state_valid = false;              // the timed phases rewrite norms and values
     return fit_time_phases(G, reps, ms) ? 0 : -1;
 }
 

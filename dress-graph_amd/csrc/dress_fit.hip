@@ -458,6 +458,7 @@ bool fit_alloc_state(DevGraph *G, bool fp32)
 // host edge values (input order) -> current device buffer (internal order)
 bool fit_upload_values(DevGraph *G, const double *edge_dress)
 {
+    G->fit_state_valid = false;
     if (!fit_alloc_state(G, false)) return false;
     G->cur = 0;
     if (G->E == 0) return true;
@@ -661,6 +662,7 @@ __global__ void k_copy_f64(const double *a, double *b, int64_t n)
 
 bool fit_run(DevGraph *G, int max_iterations, double epsilon, int precision, FitResult *res)
 {
+    G->fit_state_valid = false;
     bool mixed = (precision == PREC_FP32_FP64);
     res->ms = 0.f;
     res->reruns = 0;

@@ -151,6 +151,10 @@ struct DevGraph {
 
     Bins vbins;
     bool have_bins = false;
+    // d[cur], vd and nw hold the state a completed dress_fit (or resident
+    // fit) left on the device: set at its end, cleared by every other fit
+    // run and every upload of host values into d
+    bool fit_state_valid = false;
 
     // fit state
     double *d[2] = {nullptr, nullptr};     // internal order

@@ -49,8 +49,12 @@ __global__ void __launch_bounds__(TPB) k_get(int n, const int *__restrict__ qu, 
                                              const int *__restrict__ adj_e, const int *__restrict__ inv,
                                              const double *__restrict__ d, const double *__restrict__ ew,
                                              const double *__restrict__ vdn, const double *__restrict__ nw,
-                                             int variant, double *out)
+                                             const int *__restrict__ rank, int variant, double *out)
 {
+    // d in input edge order (inv null) or internal order; vdn / nw in input
+    // labels (rank null) or internal labels
+#define EDGE_D(e) d[inv ? inv[e] : (e)]
+#define VTX(a, x) a[rank ? rank[x] : (x)]
     const int lane = threadIdx.x % GQ;
     const int64_t groups = (int64_t)gridDim.x * (TPB / GQ);
     for (int64_t q = (int64_t)blockIdx.x * (TPB / GQ) + threadIdx.x / GQ; q < n; q += groups) {
@@ -64,7 +68,7 @@ __global__ void __launch_bounds__(TPB) k_get(int n, const int *__restrict__ qu, 
                 int mid = lo + (hi - lo) / 2;
                 int nb = adj_x[mid];
                 if (nb == v) {
-                    res = d[inv[adj_e[mid]]];
+                    res = EDGE_D(adj_e[mid]);
                     found = 1;
                     break;
                 }
@@ -89,18 +93,18 @@ __global__ void __launch_bounds__(TPB) k_get(int n, const int *__restrict__ qu, 
                 while (lo + have < be && adj_x[lo + have] == x && have <= r) have++;
                 if (r < have) {
                     const int ea = adj_e[a_is_u ? p : lo + r], eb = adj_e[a_is_u ? lo + r : p];
-                    two_sum_add(s, c, ew[ea] * d[inv[ea]] + ew[eb] * d[inv[eb]]);
+                    two_sum_add(s, c, ew[ea] * EDGE_D(ea) + ew[eb] * EDGE_D(eb));
                 }
             }
             group_reduce<GQ>(s, c);
             if (lane == 0) {
                 const bool dir = (variant == DRESS_VARIANT_FORWARD || variant == DRESS_VARIANT_BACKWARD);
-                two_sum_add(s, c, dir ? 4.0 * nw[u] : 4.0 * nw[u] + 4.0 * nw[v]);
+                two_sum_add(s, c, dir ? 4.0 * VTX(nw, u) : 4.0 * VTX(nw, u) + 4.0 * VTX(nw, v));
                 const double A = s + c;
                 const double cross = dir ? 1.0 : 2.0;
                 const double cw = (variant == DRESS_VARIANT_UNDIRECTED) ? 2.0 * qw[q] : qw[q];
-                const double du2 = vdn[u] * vdn[u];
-                const double dv2 = vdn[v] * vdn[v];
+                const double du2 = VTX(vdn, u) * VTX(vdn, u);
+                const double dv2 = VTX(vdn, v) * VTX(vdn, v);
                 if (du2 <= 0.0 || dv2 <= 0.0) {
                     res = 0.0;
                 } else {
@@ -119,13 +123,17 @@ __global__ void __launch_bounds__(TPB) k_get(int n, const int *__restrict__ qu, 
         }
         if (lane == 0) out[q] = res;
     }
+#undef EDGE_D
+#undef VTX
 }
 
 }  // namespace
 
-// queries against g's host state
+// Queries against g's host state (edge_dress, vertex_dress, NW, uploaded
+// for the call into buffers of its own), or, with resident, against the
+// state the last completed fit left on the device.
 bool get_batch(p_dress_graph_t g, int n, const int *u, const int *v, const double *w, int max_it, double eps,
-               double *out)
+               double *out, bool resident)
 {
     for (int i = 0; i < n; i++) {
         if (u[i] < 0 || u[i] >= g->N || v[i] < 0 || v[i] >= g->N) {
@@ -135,34 +143,48 @@ bool get_batch(p_dress_graph_t g, int n, const int *u, const int *v, const doubl
     }
     DevGraph *G = device_graph(g);
     if (!G) return false;
-    if (!build_index(G) || !fit_upload_values(G, g->edge_dress) ||
-        !upload_reference_csr(G, g->adj_offset, g->adj_target, g->adj_edge_idx))
+    if (resident && !G->fit_state_valid) {
+        set_error("dress_hip_get_batch_resident: no fitted state on the device (run dress_fit on the graph "
+                  "first; host values uploaded since, or other fits on its device copy, replace it)");
         return false;
+    }
+    if (!upload_reference_csr(G, g->adj_offset, g->adj_target, g->adj_edge_idx)) return false;
     hipStream_t s = G->stream;
     int *du = nullptr, *dv = nullptr;
-    double *dw = nullptr, *dout = nullptr, *dvd = nullptr, *dnw = nullptr;
+    double *dw = nullptr, *dout = nullptr, *dvd = nullptr, *dnw = nullptr, *dd = nullptr;
     bool ok = dalloc(&du, (size_t)n + 1, "queries") && dalloc(&dv, (size_t)n + 1, "queries") &&
-              dalloc(&dw, (size_t)n + 1, "queries") && dalloc(&dout, (size_t)n + 1, "answers") &&
-              dalloc(&dvd, (size_t)g->N + 1, "norms") && dalloc(&dnw, (size_t)g->N + 1, "vertex weights");
+              dalloc(&dw, (size_t)n + 1, "queries") && dalloc(&dout, (size_t)n + 1, "answers");
+    if (ok && !resident) {
+        // the host state in input order and input labels
+        ok = dalloc(&dd, (size_t)G->E + 1, "edge values") && dalloc(&dvd, (size_t)g->N + 1, "norms") &&
+             dalloc(&dnw, (size_t)g->N + 1, "vertex weights") &&
+             h2d_parallel(dd, g->edge_dress, (size_t)G->E * sizeof(double)) &&
+             (g->N == 0 || (h2d_parallel(dvd, g->vertex_dress, (size_t)g->N * sizeof(double)) &&
+                            h2d_parallel(dnw, g->NW, (size_t)g->N * sizeof(double))));
+    }
     if (ok) {
         ok = hipMemcpyAsync(du, u, (size_t)n * sizeof(int), hipMemcpyHostToDevice, s) == hipSuccess &&
              hipMemcpyAsync(dv, v, (size_t)n * sizeof(int), hipMemcpyHostToDevice, s) == hipSuccess &&
-             hipMemcpyAsync(dw, w, (size_t)n * sizeof(double), hipMemcpyHostToDevice, s) == hipSuccess &&
-             (g->N == 0 || (hipMemcpyAsync(dvd, g->vertex_dress, (size_t)g->N * sizeof(double),
-                                           hipMemcpyHostToDevice, s) == hipSuccess &&
-                            hipMemcpyAsync(dnw, g->NW, (size_t)g->N * sizeof(double), hipMemcpyHostToDevice,
-                                           s) == hipSuccess));
-        if (ok && n > 0)
-            hipLaunchKernelGGL(k_get, dim3((int)std::min<int64_t>(8192, ((int64_t)n * GQ + TPB - 1) / TPB)), dim3(TPB),
-                               0, s, n, du, dv, dw,
-                               max_it, eps, G->ref_row_off, G->ref_adj_x, G->ref_adj_e, G->inv, G->d[G->cur],
-                               G->ew_in, dvd, dnw, G->variant, dout);
+             hipMemcpyAsync(dw, w, (size_t)n * sizeof(double), hipMemcpyHostToDevice, s) == hipSuccess;
+        if (ok && n > 0) {
+            const int grid = (int)std::min<int64_t>(8192, ((int64_t)n * GQ + TPB - 1) / TPB);
+            if (resident)
+                hipLaunchKernelGGL(k_get, dim3(grid), dim3(TPB), 0, s, n, du, dv, dw, max_it, eps, G->ref_row_off,
+                                   G->ref_adj_x, G->ref_adj_e, (const int *)G->inv, (const double *)G->d[G->cur],
+                                   (const double *)G->ew_in, (const double *)G->vd, (const double *)G->nw,
+                                   (const int *)G->rank, G->variant, dout);
+            else
+                hipLaunchKernelGGL(k_get, dim3(grid), dim3(TPB), 0, s, n, du, dv, dw, max_it, eps, G->ref_row_off,
+                                   G->ref_adj_x, G->ref_adj_e, (const int *)nullptr, (const double *)dd,
+                                   (const double *)G->ew_in, (const double *)dvd, (const double *)dnw,
+                                   (const int *)nullptr, G->variant, dout);
+        }
         ok = ok && hipGetLastError() == hipSuccess &&
              hipMemcpyAsync(out, dout, (size_t)n * sizeof(double), hipMemcpyDeviceToHost, s) == hipSuccess &&
              hipStreamSynchronize(s) == hipSuccess;
-        if (!ok) set_error("dress_get kernel failed");
+        if (!ok && !has_error()) set_error("dress_get kernel failed");
     }
-    dfree(du); dfree(dv); dfree(dw); dfree(dout); dfree(dvd); dfree(dnw);
+    dfree(du); dfree(dv); dfree(dw); dfree(dout); dfree(dvd); dfree(dnw); dfree(dd);
     return ok;
 }
 
@@ -179,7 +201,7 @@ double dress_get(const p_dress_graph_t g, int u, int v, int max_iterations, doub
         dh::set_error("dress_get: NULL graph");
         return r;
     }
-    if (!dh::get_batch(g, 1, &u, &v, &edge_weight, max_iterations, epsilon, &r)) return NAN;
+    if (!dh::get_batch(g, 1, &u, &v, &edge_weight, max_iterations, epsilon, &r, false)) return NAN;
     return r;
 }
 
@@ -191,7 +213,18 @@ int dress_hip_get_batch(p_dress_graph_t g, int n, const int *u, const int *v, co
         dh::set_error("dress_hip_get_batch: bad arguments");
         return -1;
     }
-    return dh::get_batch(g, n, u, v, edge_weight, max_iterations, epsilon, out) ? 0 : -1;
+    return dh::get_batch(g, n, u, v, edge_weight, max_iterations, epsilon, out, false) ? 0 : -1;
+}
+
+int dress_hip_get_batch_resident(p_dress_graph_t g, int n, const int *u, const int *v,
+                                 const double *edge_weight, int max_iterations, double epsilon, double *out)
+{
+    dh::clear_error();
+    if (!g || n < 0 || (n > 0 && (!u || !v || !edge_weight || !out))) {
+        dh::set_error("dress_hip_get_batch_resident: bad arguments");
+        return -1;
+    }
+    return dh::get_batch(g, n, u, v, edge_weight, max_iterations, epsilon, out, true) ? 0 : -1;
 }
 
 }  // extern "C"

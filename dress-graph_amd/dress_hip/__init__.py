@@ -424,17 +424,19 @@ class Graph:
             raise HipError(f"dress_get failed: {err}")
         return val
 
-    def get_batch(self, u, v, max_iterations=100, epsilon=1e-6, edge_weight=1.0):
-        """dress_get for arrays of pairs in one device launch."""
+    def get_batch(self, u, v, max_iterations=100, epsilon=1e-6, edge_weight=1.0, resident=False):
+        """dress_get for arrays of pairs in one device launch.  With
+        ``resident``, against the state the last fit left on the device
+        (dress_hip_get_batch_resident) instead of the graph's host arrays."""
         u = np.ascontiguousarray(u, dtype=np.int32)
         v = np.ascontiguousarray(v, dtype=np.int32)
         w = np.ascontiguousarray(np.broadcast_to(np.asarray(edge_weight, dtype=np.float64), u.shape))
         out = np.empty(u.size, dtype=np.float64)
-        r = _lib.lib().dress_hip_get_batch(self.g, int(u.size), u.ctypes.data_as(c_int_p),
-                                           v.ctypes.data_as(c_int_p), w.ctypes.data_as(c_dbl_p),
-                                           int(max_iterations), float(epsilon),
-                                           out.ctypes.data_as(c_dbl_p))
-        _lib.check(r == 0, "dress_hip_get_batch")
+        L = _lib.lib()
+        f = L.dress_hip_get_batch_resident if resident else L.dress_hip_get_batch
+        r = f(self.g, int(u.size), u.ctypes.data_as(c_int_p), v.ctypes.data_as(c_int_p),
+              w.ctypes.data_as(c_dbl_p), int(max_iterations), float(epsilon), out.ctypes.data_as(c_dbl_p))
+        _lib.check(r == 0, "dress_hip_get_batch_resident" if resident else "dress_hip_get_batch")
         return out
 
     def lifted(self, kind, k, max_iterations=100, epsilon=1e-6, n_samples=0, seed=0,

@@ -184,6 +184,20 @@ int dress_hip_get_batch(p_dress_graph_t g, int n, const int *u, const int *v,
                         const double *edge_weight, int max_iterations, double epsilon,
                         double *out);
 
+/* dress_hip_get_batch against the state the last completed dress_fit (or
+ * dress_hip_fit_resident) left on the device -- the graph's values, vertex
+ * norms and vertex weights as that fit ended -- instead of uploading g's
+ * host arrays for the call: for link-prediction queries in batches after a
+ * fit.  No counterpart in libdress.  Answers equal dress_hip_get_batch's
+ * as long as g's host arrays still hold what the fit wrote.  Returns -1
+ * (see dress_hip_last_error) when no such state is there: before the first
+ * fit, or after host values were uploaded (a fit in progress, a reset) or
+ * another fit ran on the graph's device copy (delta / nabla subgraphs run
+ * one at a time). */
+int dress_hip_get_batch_resident(p_dress_graph_t g, int n, const int *u, const int *v,
+                                 const double *edge_weight, int max_iterations, double epsilon,
+                                 double *out);
+
 /* How dress_{delta,nabla}_fit_hip* run their subgraphs:
  *   0 (default) one batched persistent launch when the graph fits the
  *     kernel's LDS state (up to 65535 edges and ~150 KB of state), otherwise

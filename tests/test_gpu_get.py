@@ -46,3 +46,54 @@ def test_get_out_of_range_is_an_error(gpu):
         g.fit()
         with pytest.raises(gpu.HipError):
             g.get(0, 7)
+
+
+@pytest.mark.parametrize("relabel", ["0", "1"])
+@pytest.mark.parametrize("variant", [0, 1, 2, 3])
+def test_get_resident_vs_oracle(gpu, ora, variant, relabel, monkeypatch):
+    """Queries against the state the fit left on the device (internal edge
+    order and, with relabelling, internal vertex labels) equal the host-array
+    path and the oracle bit for bit."""
+    monkeypatch.setenv("DRESS_HIP_RELABEL", relabel)
+    rng = np.random.default_rng(170 + variant)
+    n, m = 60, 260
+    U = rng.integers(0, n, m).astype(np.int32)
+    V = rng.integers(0, n, m).astype(np.int32)
+    W = rng.random(m) + 0.2
+    NW = rng.random(n) + 0.5
+    qu = rng.integers(0, n, 300).astype(np.int32)
+    qv = rng.integers(0, n, 300).astype(np.int32)
+    with gpu.Graph(n, U, V, W, NW, variant) as g:
+        g.fit(100, 1e-7)
+        res = g.get_batch(qu, qv, 50, 1e-7, 0.7, resident=True)
+        host = g.get_batch(qu, qv, 50, 1e-7, 0.7)
+    og = ora.graph(n, U, V, W, NW, variant, 0)
+    ora.fit_graph(og, 100, 1e-7)
+    want = np.array([ora.get(og, int(a), int(b), 50, 1e-7, 0.7) for a, b in zip(qu, qv)])
+    ora.free_graph(og)
+    assert np.array_equal(res, host)
+    assert np.array_equal(res, want)
+
+
+def test_get_resident_state(gpu, gen):
+    """The resident state exists from a completed fit until other fits run on
+    the graph's device copy; host-array queries leave it alone."""
+    n, U, V = gen.random_regular(40, 6, 3)
+    qu, qv = np.arange(20, dtype=np.int32), (np.arange(20, dtype=np.int32) * 7 + 3) % 40
+    with gpu.Graph(n, U, V) as g:
+        with pytest.raises(gpu.HipError):
+            g.get_batch(qu, qv, resident=True)
+        g.fit(100, 1e-6)
+        a = g.get_batch(qu, qv, resident=True)
+        b = g.get_batch(qu, qv)
+        assert np.array_equal(a, b)
+        assert np.array_equal(g.get_batch(qu, qv, resident=True), a)
+        gpu.set_lifted_mode(2)                 # subgraph fits on this graph's device copy
+        try:
+            g.lifted("delta", 1, n_samples=3, seed=1)
+        finally:
+            gpu.set_lifted_mode(0)
+        with pytest.raises(gpu.HipError):
+            g.get_batch(qu, qv, resident=True)
+        g.fit(100, 1e-6)                       # continues from the first fit's values
+        assert np.array_equal(g.get_batch(qu, qv, resident=True), g.get_batch(qu, qv))
