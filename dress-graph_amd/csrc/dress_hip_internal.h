@@ -92,6 +92,20 @@ bool ctx_serving();                // the calling thread is a server worker
 // enter returns its stream, or nullptr when a context is already active
 hipStream_t batch_ctx_enter();
 void batch_ctx_leave();
+// The calling thread's batch block cache for the length of one batch call
+// (a no-op inside a server worker, which has its own).  Enter it after the
+// graph's own device arrays exist: blocks taken inside come back to the
+// cache when freed.
+struct BatchScope {
+    bool on;
+    BatchScope() : on(batch_ctx_enter() != nullptr) {}
+    ~BatchScope()
+    {
+        if (on) batch_ctx_leave();
+    }
+    BatchScope(const BatchScope &) = delete;
+    BatchScope &operator=(const BatchScope &) = delete;
+};
 hipStream_t ctx_stream();          // nullptr when no context is active
 int64_t ctx_cached_bytes();        // bytes held by the cache (free + in use)
 

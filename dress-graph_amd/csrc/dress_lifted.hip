@@ -49,19 +49,6 @@ inline int grid_for(int64_t threads, int cap)
 // with rows ordered by (neighbour, edge), variant weights, and per edge the
 // multiset intersection of its two rows as pairs of edge ids.  Same rules as
 // the device build (dress_build.hip) and the reference (dress.c:103-363).
-// The calling thread's batch block cache for the length of one batch call
-// (a no-op inside a server worker, which has its own).
-struct BatchScope {
-    bool on;
-    BatchScope() : on(batch_ctx_enter() != nullptr) {}
-    ~BatchScope()
-    {
-        if (on) batch_ctx_leave();
-    }
-    BatchScope(const BatchScope &) = delete;
-    BatchScope &operator=(const BatchScope &) = delete;
-};
-
 // Phase timings printed to stderr when DRESS_HIP_TIMING is set.
 struct PhaseTimer {
     const char *what;

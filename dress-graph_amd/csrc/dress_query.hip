@@ -149,6 +149,8 @@ bool get_batch(p_dress_graph_t g, int n, const int *u, const int *v, const doubl
         return false;
     }
     if (!upload_reference_csr(G, g->adj_offset, g->adj_target, g->adj_edge_idx)) return false;
+    // this call's buffers from the thread's block cache, kept between calls
+    BatchScope scope;
     hipStream_t s = G->stream;
     int *du = nullptr, *dv = nullptr;
     double *dw = nullptr, *dout = nullptr, *dvd = nullptr, *dnw = nullptr, *dd = nullptr;
@@ -184,6 +186,7 @@ bool get_batch(p_dress_graph_t g, int n, const int *u, const int *v, const doubl
              hipStreamSynchronize(s) == hipSuccess;
         if (!ok && !has_error()) set_error("dress_get kernel failed");
     }
+    (void)hipStreamSynchronize(s);          // the cached blocks are idle before they go back
     dfree(du); dfree(dv); dfree(dw); dfree(dout); dfree(dvd); dfree(dnw); dfree(dd);
     return ok;
 }
