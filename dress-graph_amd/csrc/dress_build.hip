@@ -1115,7 +1115,7 @@ void free_graph(DevGraph *G)
     if (G->stream) hipStreamSynchronize(G->stream);
     dfree(G->U); dfree(G->V); dfree(G->row_off); dfree(G->adj_x); dfree(G->adj_e);
     dfree(G->ew_in); dfree(G->c4_in); dfree(G->nw); dfree(G->pos_in);
-    dfree(G->rank); dfree(G->vstage);
+    dfree(G->rank); dfree(G->vstage); dfree(G->q_d); dfree(G->q_vd); dfree(G->q_nw);
     free_reference_arrays(G);
     dfree(G->ord); dfree(G->inv); dfree(G->uv); dfree(G->rs); dfree(G->pos); dfree(G->ew); dfree(G->c4);
     dfree(G->ioff); dfree(G->ipk); dfree(G->ipair); dfree(G->cnt_in);

@@ -149,6 +149,9 @@ struct DevGraph {
     int *ref_U = nullptr, *ref_V = nullptr;
     int *ref_row_off = nullptr, *ref_adj_x = nullptr, *ref_adj_e = nullptr;
     double *vstage = nullptr;              // per-vertex staging (input labels)
+    // dress_get's copy of the host state (input order and labels): edge
+    // values, norms, vertex weights, written per call where queries read
+    double *q_d = nullptr, *q_vd = nullptr, *q_nw = nullptr;
     int2 *pos_in = nullptr;                // half-edges of input edge e
 
     // internal edge order (valid once have_index)

@@ -127,6 +127,23 @@ __global__ void __launch_bounds__(TPB) k_get(int n, const int *__restrict__ qu, 
 #undef VTX
 }
 
+// the entries of g's host state the queries read: edge values at edge ids,
+// norms and vertex weights at vertices
+__global__ void k_get_scatter(int ne, const int *__restrict__ eid, const double *__restrict__ ev, int nv,
+                              const int *__restrict__ vid, const double *__restrict__ vv,
+                              const double *__restrict__ vw, double *d, double *vd, double *nw)
+{
+    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < ne + nv; i += gridDim.x * blockDim.x) {
+        if (i < ne) {
+            d[eid[i]] = ev[i];
+        } else {
+            const int j = i - ne;
+            vd[vid[j]] = vv[j];
+            nw[vid[j]] = vw[j];
+        }
+    }
+}
+
 }  // namespace
 
 // Queries against g's host state (edge_dress, vertex_dress, NW, uploaded
@@ -149,20 +166,74 @@ bool get_batch(p_dress_graph_t g, int n, const int *u, const int *v, const doubl
         return false;
     }
     if (!upload_reference_csr(G, g->adj_offset, g->adj_target, g->adj_edge_idx)) return false;
+    hipStream_t s = G->stream;
+    if (!resident) {
+        // the graph's copy of the host state (input order and labels), kept
+        // with the graph and rewritten per call where the queries read
+        const bool have = (G->q_d || dalloc(&G->q_d, (size_t)G->E + 1, "query edge values")) &&
+                          (G->q_vd || dalloc(&G->q_vd, (size_t)g->N + 1, "query norms")) &&
+                          (G->q_nw || dalloc(&G->q_nw, (size_t)g->N + 1, "query vertex weights"));
+        if (!have) return false;
+    }
     // this call's buffers from the thread's block cache, kept between calls
     BatchScope scope;
-    hipStream_t s = G->stream;
-    int *du = nullptr, *dv = nullptr;
-    double *dw = nullptr, *dout = nullptr, *dvd = nullptr, *dnw = nullptr, *dd = nullptr;
+    // host gathers of the state entries the queries read (alive until the
+    // stream is synchronised below)
+    std::vector<int> eid, vid;
+    std::vector<double> ev, vv, vw;
+    int *du = nullptr, *dv = nullptr, *deid = nullptr, *dvid = nullptr;
+    double *dw = nullptr, *dout = nullptr, *dev = nullptr, *dvv = nullptr, *dvw = nullptr;
     bool ok = dalloc(&du, (size_t)n + 1, "queries") && dalloc(&dv, (size_t)n + 1, "queries") &&
               dalloc(&dw, (size_t)n + 1, "queries") && dalloc(&dout, (size_t)n + 1, "answers");
     if (ok && !resident) {
-        // the host state in input order and input labels
-        ok = dalloc(&dd, (size_t)G->E + 1, "edge values") && dalloc(&dvd, (size_t)g->N + 1, "norms") &&
-             dalloc(&dnw, (size_t)g->N + 1, "vertex weights") &&
-             h2d_parallel(dd, g->edge_dress, (size_t)G->E * sizeof(double)) &&
-             (g->N == 0 || (h2d_parallel(dvd, g->vertex_dress, (size_t)g->N * sizeof(double)) &&
-                            h2d_parallel(dnw, g->NW, (size_t)g->N * sizeof(double))));
+        // A query reads the values of the edges in its two rows and the norms
+        // and weights of its two vertices.  When the queries' rows hold less
+        // than a quarter of the half-edges, only those entries go up (a
+        // single dress_get on a large graph moves a few KB); otherwise the
+        // whole host state does.
+        int64_t rows = 0;
+        for (int i = 0; i < n && rows * 4 < (int64_t)G->S; i++)
+            rows += (int64_t)(g->adj_offset[u[i] + 1] - g->adj_offset[u[i]]) +
+                    (g->adj_offset[v[i] + 1] - g->adj_offset[v[i]]);
+        if (rows * 4 < (int64_t)G->S) {
+            eid.reserve((size_t)rows);
+            ev.reserve((size_t)rows);
+            for (int i = 0; i < n; i++) {
+                for (int x : {u[i], v[i]}) {
+                    for (int p = g->adj_offset[x]; p < g->adj_offset[x + 1]; p++) {
+                        const int e = g->adj_edge_idx[p];
+                        eid.push_back(e);
+                        ev.push_back(g->edge_dress[e]);
+                    }
+                    vid.push_back(x);
+                    vv.push_back(g->vertex_dress[x]);
+                    vw.push_back(g->NW[x]);
+                }
+            }
+            const int ne = (int)eid.size(), nv = (int)vid.size();
+            ok = dalloc(&deid, (size_t)ne + 1, "query edge ids") && dalloc(&dev, (size_t)ne + 1, "query edge values") &&
+                 dalloc(&dvid, (size_t)nv + 1, "query vertices") && dalloc(&dvv, (size_t)nv + 1, "query norms") &&
+                 dalloc(&dvw, (size_t)nv + 1, "query vertex weights") &&
+                 (ne == 0 || (hipMemcpyAsync(deid, eid.data(), (size_t)ne * sizeof(int), hipMemcpyHostToDevice, s) ==
+                                  hipSuccess &&
+                              hipMemcpyAsync(dev, ev.data(), (size_t)ne * sizeof(double), hipMemcpyHostToDevice,
+                                             s) == hipSuccess)) &&
+                 (nv == 0 || (hipMemcpyAsync(dvid, vid.data(), (size_t)nv * sizeof(int), hipMemcpyHostToDevice, s) ==
+                                  hipSuccess &&
+                              hipMemcpyAsync(dvv, vv.data(), (size_t)nv * sizeof(double), hipMemcpyHostToDevice,
+                                             s) == hipSuccess &&
+                              hipMemcpyAsync(dvw, vw.data(), (size_t)nv * sizeof(double), hipMemcpyHostToDevice,
+                                             s) == hipSuccess));
+            if (ok && ne + nv > 0)
+                hipLaunchKernelGGL(k_get_scatter, dim3((int)std::min<int64_t>(4096, ((int64_t)ne + nv + TPB - 1) / TPB)),
+                                   dim3(TPB), 0, s, ne, deid, dev, nv, dvid, dvv, dvw, G->q_d, G->q_vd, G->q_nw);
+        } else {
+            // the caller's arrays may still be read by earlier work on s
+            ok = hipStreamSynchronize(s) == hipSuccess &&
+                 h2d_parallel(G->q_d, g->edge_dress, (size_t)G->E * sizeof(double)) &&
+                 (g->N == 0 || (h2d_parallel(G->q_vd, g->vertex_dress, (size_t)g->N * sizeof(double)) &&
+                                h2d_parallel(G->q_nw, g->NW, (size_t)g->N * sizeof(double))));
+        }
     }
     if (ok) {
         ok = hipMemcpyAsync(du, u, (size_t)n * sizeof(int), hipMemcpyHostToDevice, s) == hipSuccess &&
@@ -177,8 +248,8 @@ bool get_batch(p_dress_graph_t g, int n, const int *u, const int *v, const doubl
                                    (const int *)G->rank, G->variant, dout);
             else
                 hipLaunchKernelGGL(k_get, dim3(grid), dim3(TPB), 0, s, n, du, dv, dw, max_it, eps, G->ref_row_off,
-                                   G->ref_adj_x, G->ref_adj_e, (const int *)nullptr, (const double *)dd,
-                                   (const double *)G->ew_in, (const double *)dvd, (const double *)dnw,
+                                   G->ref_adj_x, G->ref_adj_e, (const int *)nullptr, (const double *)G->q_d,
+                                   (const double *)G->ew_in, (const double *)G->q_vd, (const double *)G->q_nw,
                                    (const int *)nullptr, G->variant, dout);
         }
         ok = ok && hipGetLastError() == hipSuccess &&
@@ -187,7 +258,7 @@ bool get_batch(p_dress_graph_t g, int n, const int *u, const int *v, const doubl
         if (!ok && !has_error()) set_error("dress_get kernel failed");
     }
     (void)hipStreamSynchronize(s);          // the cached blocks are idle before they go back
-    dfree(du); dfree(dv); dfree(dw); dfree(dout); dfree(dvd); dfree(dnw); dfree(dd);
+    dfree(du); dfree(dv); dfree(dw); dfree(dout); dfree(deid); dfree(dev); dfree(dvid); dfree(dvv); dfree(dvw);
     return ok;
 }
 

@@ -97,3 +97,32 @@ def test_get_resident_state(gpu, gen):
             g.get_batch(qu, qv, resident=True)
         g.fit(100, 1e-6)                       # continues from the first fit's values
         assert np.array_equal(g.get_batch(qu, qv, resident=True), g.get_batch(qu, qv))
+
+
+@pytest.mark.parametrize("variant", [0, 1, 2, 3])
+def test_get_few_queries_vs_oracle(gpu, ora, variant):
+    """Few queries on a larger graph upload only the state entries they read
+    (the rows of their vertices); single gets, a small batch and a batch that
+    uploads the whole state answer alike, bit for bit with the oracle."""
+    rng = np.random.default_rng(270 + variant)
+    n, m = 3000, 15000
+    U = rng.integers(0, n, m).astype(np.int32)
+    V = rng.integers(0, n, m).astype(np.int32)
+    W = rng.random(m) + 0.2
+    NW = rng.random(n) + 0.5
+    qu = rng.integers(0, n, 40).astype(np.int32)
+    qv = rng.integers(0, n, 40).astype(np.int32)
+    big_u = rng.integers(0, n, 4000).astype(np.int32)
+    big_v = rng.integers(0, n, 4000).astype(np.int32)
+    with gpu.Graph(n, U, V, W, NW, variant) as g:
+        g.fit(100, 1e-7)
+        single = np.array([g.get(int(a), int(b), 50, 1e-7, 0.7) for a, b in zip(qu[:10], qv[:10])])
+        few = g.get_batch(qu, qv, 50, 1e-7, 0.7)
+        whole = g.get_batch(np.concatenate([qu, big_u]), np.concatenate([qv, big_v]), 50, 1e-7, 0.7)[:qu.size]
+    og = ora.graph(n, U, V, W, NW, variant, 0)
+    ora.fit_graph(og, 100, 1e-7)
+    want = np.array([ora.get(og, int(a), int(b), 50, 1e-7, 0.7) for a, b in zip(qu, qv)])
+    ora.free_graph(og)
+    assert np.array_equal(few, want)
+    assert np.array_equal(whole, want)
+    assert np.array_equal(single, want[:10])
