@@ -104,13 +104,17 @@ __global__ void __launch_bounds__(B) k_lifted(LiftArgs a)
             s_tup[threadIdx.x] = a.item_verts[I.vofs + threadIdx.x];
             s_mark[threadIdx.x] = a.marks[threadIdx.x];
         }
-        for (int u = threadIdx.x; u < N; u += B) alive[u] = 1;
+        for (int u = threadIdx.x; u < N; u += B) {
+            alive[u] = 1;
+            vd[u] = 0.0;        // norms before any vertex phase (calloc'd in dress_init_graph)
+        }
         __syncthreads();
         if (I.kind == LIFT_DELTA && threadIdx.x < k) alive[s_tup[threadIdx.x]] = 0;
         __syncthreads();
         // d = 1.0 on live edges (delta_dress_impl.c:166 -> dress.c:418-421);
         // a dead edge keeps value 0 so it adds nothing to a row sum
-        bool fp32 = I.mixed != 0;
+        // (with no iteration to run, the values stay at their FP64 start)
+        bool fp32 = I.mixed != 0 && a.max_it > 0;
         float *fA = (float *)dA, *fB = (float *)dB;
         for (int e = threadIdx.x; e < E; e += B) {
             int2 p = uv[e];

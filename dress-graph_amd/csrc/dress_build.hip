@@ -656,6 +656,8 @@ bool build_graph(DevGraph *G, int N, int E, const int *U, const int *V, const do
     if (N < 0 || E < 0) { set_error("negative graph size"); return false; }
     if ((int64_t)E * 2 >= (int64_t)INT32_MAX) { set_error("too many edges for int32 CSR"); return false; }
     if (variant < 0 || variant > 3) { set_error("unknown variant"); return false; }
+    // ids index the degree counters and rows below: reject any outside [0, N)
+    if (!check_vertex_ids(N, E, U, V, "graph build")) return false;
     G->N = N; G->E = E; G->variant = variant; G->weighted = W != nullptr;
     if (!G->stream) {
         if (hipStream_t cs = ctx_stream()) {

@@ -371,6 +371,49 @@ bool d2h_parallel(void *dst, const void *src, size_t bytes)
 
 void fill_nan_parallel(double *p, size_t n) { fill_parallel(p, n, std::nan("")); }
 
+// Every vertex id of an edge list lies in [0, N); otherwise the error is set
+// and the call that builds from it returns before any device or host write
+// indexed by an id.  Long lists are scanned by several host threads.
+bool check_vertex_ids(int N, int64_t E, const int *U, const int *V, const char *who)
+{
+    if (E <= 0) return true;
+    if (!U || !V) {
+        set_error(std::string(who) + ": NULL edge list");
+        return false;
+    }
+    const uint32_t n = (uint32_t)std::max(N, 0);
+    const int64_t T = E < ((int64_t)1 << 22) ? 1
+                                            : std::max<int64_t>(1, std::min<int64_t>((int64_t)copy_threads(),
+                                                                   std::max(1u, std::thread::hardware_concurrency())));
+    std::vector<int64_t> first_bad((size_t)T, -1);
+    auto scan = [&](int64_t w) {
+        const int64_t b = E * w / T, e = E * (w + 1) / T;
+        unsigned bad = 0;
+        for (int64_t i = b; i < e; i++) bad |= ((uint32_t)U[i] >= n) | ((uint32_t)V[i] >= n);
+        if (!bad) return;
+        for (int64_t i = b; i < e; i++)
+            if ((uint32_t)U[i] >= n || (uint32_t)V[i] >= n) { first_bad[(size_t)w] = i; return; }
+    };
+    std::vector<std::thread> pool;
+    for (int64_t w = 1; w < T; w++) {
+        try {
+            pool.emplace_back(scan, w);
+        } catch (...) {
+            scan(w);
+        }
+    }
+    scan(0);
+    for (auto &th : pool) th.join();
+    for (int64_t i : first_bad)
+        if (i >= 0) {
+            set_error(std::string(who) + ": edge " + std::to_string(i) + " (" + std::to_string(U[i]) + ", " +
+                      std::to_string(V[i]) + ") has a vertex id outside [0, " + std::to_string(N) + ")");
+            return false;
+        }
+    return true;
+}
+
+
 void fill_parallel(double *p, size_t n, double value)
 {
     const size_t T = n * sizeof(double) < ((size_t)64 << 20)

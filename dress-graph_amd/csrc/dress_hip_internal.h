@@ -85,6 +85,8 @@ void snapshot_give(void *p);
 // n doubles set to a value (NaN), by several host threads when n is large
 void fill_parallel(double *p, size_t n, double value);
 void fill_nan_parallel(double *p, size_t n);
+// every id of the edge list in [0, N), else the error is set (host scan)
+bool check_vertex_ids(int N, int64_t E, const int *U, const int *V, const char *who);
 
 void ctx_trim(size_t max_block, size_t keep_bytes);
 bool ctx_serving();                // the calling thread is a server worker

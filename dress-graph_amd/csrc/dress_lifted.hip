@@ -1722,6 +1722,9 @@ int dress_hip_fit_batch(int n_graphs, const int *N, const int64_t *edge_offset, 
             set_error("graph " + std::to_string(gi) + " does not fit the batched kernel");
             return -1;
         }
+        if (!check_vertex_ids(N[gi], Eg, U ? U + edge_offset[gi] : nullptr, V ? V + edge_offset[gi] : nullptr,
+                              ("dress_hip_fit_batch graph " + std::to_string(gi)).c_str()))
+            return -1;
         totN += N[gi];
     }
     out->n_graphs = n_graphs;
@@ -1823,6 +1826,20 @@ int dress_hip_lifted_batch(int kind, int k, int n_graphs, const int *N, const in
     if (n_graphs < 0 || k < 0 || k > LIFT_KMAX || (kind != LIFT_DELTA && kind != LIFT_NABLA)) {
         set_error("bad batch arguments");
         return -1;
+    }
+    if (n_graphs > 0 && (!N || !edge_offset)) {
+        set_error("bad batch arguments");
+        return -1;
+    }
+    for (int gi = 0; gi < n_graphs; gi++) {
+        const int64_t Eg = edge_offset[gi + 1] - edge_offset[gi];
+        if (N[gi] < 0 || Eg < 0) {
+            set_error("graph " + std::to_string(gi) + ": negative size");
+            return -1;
+        }
+        if (!check_vertex_ids(N[gi], Eg, U ? U + edge_offset[gi] : nullptr, V ? V + edge_offset[gi] : nullptr,
+                              ("dress_hip_lifted_batch graph " + std::to_string(gi)).c_str()))
+            return -1;
     }
     auto t0 = std::chrono::steady_clock::now();
     PhaseTimer pt("dress_hip_lifted_batch");

@@ -165,6 +165,11 @@ bool masked_items(p_dress_graph_t g, DevGraph *G, int kind, int k, const std::ve
     const int egrid = grid_for(E, num_cus() * 8);
     if (ok && N > 0) hipLaunchKernelGGL(k_fill_f64, dim3(grid_for(N, 4096)), dim3(TPB), 0, s, G->nw, (int64_t)N, 1.0);
     if (ok && E > 0) hipLaunchKernelGGL(k_self_from_nw, dim3(egrid), dim3(TPB), 0, s, G->uv, E, G->nw, directed, G->c4);
+    // the item whose weight patch is on the device and not yet undone: a
+    // failed fit leaves the loop before the undo, and the parent's weights
+    // must come back on every exit (later fits and dress_get read them)
+    int64_t undo_p0 = -1;
+    int undo_np = 0;
     for (int it = 0; ok && it < n_items; it++) {
         if (thread_cancelled()) {
             set_error("cancelled");
@@ -185,9 +190,12 @@ bool masked_items(p_dress_graph_t g, DevGraph *G, int kind, int k, const std::ve
         }
         const int64_t p0 = patched ? patch_off[(size_t)it] : 0;
         const int np = patched ? (int)(patch_off[(size_t)it + 1] - p0) : 0;
-        if (np > 0)
+        if (np > 0) {
             hipLaunchKernelGGL(k_patch_ew, dim3(grid_for(np, 64)), dim3(TPB), 0, s, G->inv, d_pe + p0, d_pw + p0, np,
                                G->ew);
+            undo_p0 = p0;
+            undo_np = np;
+        }
         FitResult r;
         ok = hipGetLastError() == hipSuccess && fit_run(G, max_it, eps, precision, &r);
         if (!ok) break;
@@ -200,9 +208,11 @@ bool masked_items(p_dress_graph_t g, DevGraph *G, int kind, int k, const std::ve
                                 hipMemcpyDeviceToHost, s) == hipSuccess;
         }
         // back to the parent: self terms of the removed edges, or weights
-        if (np > 0)
+        if (np > 0) {
             hipLaunchKernelGGL(k_patch_ew, dim3(grid_for(np, 64)), dim3(TPB), 0, s, G->inv, d_pe + p0, d_ppw + p0, np,
                                G->ew);
+            undo_np = 0;
+        }
         if (kind == LIFT_DELTA) {
             if (k > 0)
                 hipLaunchKernelGGL(k_mask_edges, dim3(egrid), dim3(TPB), 0, s, G->uv, E, vi, k, G->rank, 0, live_c4,
@@ -212,6 +222,11 @@ bool masked_items(p_dress_graph_t g, DevGraph *G, int kind, int k, const std::ve
             hipLaunchKernelGGL(k_self_from_nw, dim3(egrid), dim3(TPB), 0, s, G->uv, E, G->nw, directed, G->c4);
         }
         ok = ok && hipGetLastError() == hipSuccess;
+    }
+    if (undo_np > 0) {
+        hipLaunchKernelGGL(k_patch_ew, dim3(grid_for(undo_np, 64)), dim3(TPB), 0, s, G->inv, d_pe + undo_p0,
+                           d_ppw + undo_p0, undo_np, G->ew);
+        (void)hipGetLastError();
     }
     unsigned long long cnt = 0;
     if (ok)

@@ -24,6 +24,9 @@
 extern "C" {
 #endif
 
+/* the library is built with hidden visibility; these names are its interface */
+#pragma GCC visibility push(default)
+
 /* One (value, count) entry of a sparse histogram (dress.h:12-15). */
 typedef struct __dress_hist_pair_t {
     double  value;
@@ -151,6 +154,8 @@ int64_t *dress_nabla_fit_strided_flat(p_dress_graph_t g, int k,
  * (delta_dress_impl.h:60, nabla_dress_impl.h:57). */
 int64_t dress_delta_binom(int n, int k);
 int64_t dress_nabla_perm_count(int n, int k);
+
+#pragma GCC visibility pop
 
 #ifdef __cplusplus
 }

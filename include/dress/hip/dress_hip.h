@@ -16,7 +16,15 @@
  *    (deleted vertices) or re-weighted (marked vertices) in LDS;
  *  - errors do not exit(): the call leaves its outputs unset (fits set
  *    *iterations = -1 and *delta = NaN, lifted fits return NULL with
- *    *hist_size = 0) and dress_hip_last_error() describes what failed.
+ *    *hist_size = 0) and dress_hip_last_error() describes what failed;
+ *  - edge lists are checked: an id outside [0, N) is an error, not a write
+ *    out of bounds.
+ *
+ * Threading: calls on distinct graphs may run concurrently from different
+ * host threads (each graph has its own stream).  One dress_graph_t must not
+ * be used by two calls at once: delta^k / nabla^k of a graph too large for
+ * the batched kernel reuse its device state (values, self terms, weights)
+ * and restore it before they return.
  *
  * Numerics: FP64 results equal the CPU library's bit for bit, except in the
  * rare case where an exact sum lies within ~n^2*2^-106 of a rounding midpoint
@@ -35,13 +43,16 @@
 extern "C" {
 #endif
 
+/* the library is built with hidden visibility; these names are its interface */
+#pragma GCC visibility push(default)
+
 /* ---- counterparts of libdress/include/dress/cuda/dress_cuda.h -------- */
 
-/* dress_cuda.h:71-72 */
+/* dress_cuda.h:38-39 */
 void dress_fit_hip(p_dress_graph_t g, int max_iterations, double epsilon,
                    int *iterations, double *delta);
 
-/* dress_cuda.h:80-85 */
+/* dress_cuda.h:47-52 */
 dress_hist_pair_t *dress_delta_fit_hip(p_dress_graph_t g, int k, int iterations,
                                        double epsilon,
                                        int n_samples, unsigned int seed,
@@ -49,7 +60,7 @@ dress_hist_pair_t *dress_delta_fit_hip(p_dress_graph_t g, int k, int iterations,
                                        int keep_multisets, double **multisets,
                                        int64_t *num_subgraphs);
 
-/* dress_cuda.h:93-99 */
+/* dress_cuda.h:60-66 */
 dress_hist_pair_t *dress_delta_fit_hip_strided(p_dress_graph_t g, int k,
                                                int iterations, double epsilon,
                                                int n_samples, unsigned int seed,
@@ -58,7 +69,7 @@ dress_hist_pair_t *dress_delta_fit_hip_strided(p_dress_graph_t g, int k,
                                                int64_t *num_subgraphs,
                                                int offset, int stride);
 
-/* dress_cuda.h:105-111 (and the unstrided flat form delta_dress_cuda.c:24-36) */
+/* dress_cuda.h:72-78 (and the unstrided flat form delta_dress_cuda.c:24-36) */
 int64_t *dress_delta_fit_hip_flat(p_dress_graph_t g, int k, int iterations,
                                   double epsilon,
                                   int n_samples, unsigned int seed,
@@ -73,7 +84,7 @@ int64_t *dress_delta_fit_hip_strided_flat(p_dress_graph_t g, int k,
                                           int64_t *num_subgraphs,
                                           int offset, int stride);
 
-/* dress_cuda.h:115-143 */
+/* dress_cuda.h:82-87, :89-95, :97-102, :104-110 */
 dress_hist_pair_t *dress_nabla_fit_hip(p_dress_graph_t g, int k, int iterations,
                                        double epsilon,
                                        int n_samples, unsigned int seed,
@@ -385,6 +396,8 @@ void dress_hip_server_stats(dress_hip_server_t *s, dress_hip_server_stats_t *out
  * subgraph (they finish with status -1), waits for the rest, frees the
  * server. */
 void dress_hip_server_destroy(dress_hip_server_t *s);
+
+#pragma GCC visibility pop
 
 #ifdef __cplusplus
 }

@@ -3,7 +3,14 @@
 Run in the development container (the reference sources must be present so
 oracle/_ref/libdress_ref.so can be built):
 
-    make -C oracle ref && python tests/golden/make_golden.py
+    make -C oracle ref && python tests/golden/make_golden.py          # small fixtures
+    make -C oracle ref && python tests/golden/make_golden.py --full   # full-size digests
+    (``--full fit``, ``--full batch``, ``--full delta2`` write one of them)
+
+``--full`` runs the reference at the benchmark sizes (about 10 minutes on 8
+cores) and writes digests only: the 4M-vertex power-law fit, all 512 graphs
+of the delta^1 batch and the delta^2 histogram of the 12-regular n = 256
+graph.
 
 Every fixture is data: input graphs (or the generator parameters that make
 them) and the outputs libdress v0.8.3 produces for them.
@@ -12,6 +19,7 @@ import hashlib
 import json
 import os
 import sys
+import time
 
 import numpy as np
 
@@ -119,5 +127,77 @@ def main():
     print("golden fixtures written to", HERE)
 
 
+def array_digest(a):
+    return hashlib.sha256(np.ascontiguousarray(a, dtype=np.float64).tobytes()).hexdigest()
+
+
+def main_full(parts):
+    ref = po.Lib("ref")
+    if "delta2" in parts:
+        full_delta2(ref)
+    if "batch" in parts:
+        full_batch(ref)
+    if "fit" in parts:
+        full_fit(ref)
+
+
+def full_delta2(ref):
+    # configuration 4: delta^2 of the 12-regular n = 256 graph, every subgraph
+    t0 = time.time()
+    n4, U4, V4 = gen.regular_switch()
+    vals, cnts, _, num = ref.lifted("delta", n4, U4, V4, k=2, use_omp=True)
+    with open(os.path.join(HERE, "regular12_n256_delta2.json"), "w") as f:
+        json.dump({"items": int(num), "distinct": int(vals.size), "total": int(cnts.sum()),
+                   "min": float(vals[0]).hex(), "max": float(vals[-1]).hex(),
+                   "sha256": hist_digest(vals, cnts)}, f, indent=1)
+    del vals, cnts
+    print(f"delta^2 digest ({time.time() - t0:.1f} s)", flush=True)
+
+
+def full_batch(ref):
+    # configuration 3: all 512 graphs of the delta^1 batch
+    t0 = time.time()
+    batch = []
+    for i, (n, Ub, Vb) in enumerate(gen.delta1_batch_graphs(512)):
+        vals, cnts, _, num = ref.lifted("delta", n, Ub, Vb, k=1, use_omp=True)
+        batch.append({"seed": i, "n": n, "m": int(Ub.size), "items": int(num),
+                      "distinct": int(vals.size), "total": int(cnts.sum()),
+                      "sha256": hist_digest(vals, cnts),
+                      "edges_sha256": hashlib.sha256(np.stack([Ub, Vb]).tobytes()).hexdigest()})
+    with open(os.path.join(HERE, "delta1_batch512.json"), "w") as f:
+        json.dump(batch, f, indent=0)
+    print(f"delta^1 batch digests ({time.time() - t0:.1f} s)", flush=True)
+
+
+def full_fit(ref):
+    # configuration 2: FP64 fit of the 4M-vertex power-law graph.  Digests of
+    # the values in input order and sorted, plus 4,096 values at fixed
+    # positions, so a mismatch can still be sized.
+    t0 = time.time()
+    N, U, V = gen.chung_lu()
+    r = ref.fit(N, U, V, precompute=1, max_iterations=100, epsilon=1e-6, parallel=True)
+    ed, vd = r["edge_dress"], r["vertex_dress"]
+    pick = np.random.default_rng(2).choice(ed.size, 4096, replace=False)
+    pick.sort()
+    vpick = np.random.default_rng(3).choice(vd.size, 4096, replace=False)
+    vpick.sort()
+    rec = {"N": int(N), "E": int(U.size),
+           "edges_sha256": hashlib.sha256(np.stack([U, V]).tobytes()).hexdigest(),
+           "iterations": int(r["iterations"]), "delta": float(r["delta"]).hex(),
+           "edge_dress_sha256": array_digest(ed), "edge_dress_sorted_sha256": array_digest(np.sort(ed)),
+           "vertex_dress_sha256": array_digest(vd), "vertex_dress_sorted_sha256": array_digest(np.sort(vd)),
+           "edge_sum": float(np.sum(ed)).hex(), "edge_min": float(ed.min()).hex(), "edge_max": float(ed.max()).hex(),
+           "edge_pick": [int(x) for x in pick], "edge_pick_values": hexs(ed[pick]),
+           "vertex_pick": [int(x) for x in vpick], "vertex_pick_values": hexs(vd[vpick])}
+    with open(os.path.join(HERE, "chung_lu_4m_fit.json"), "w") as f:
+        json.dump(rec, f, indent=0)
+    print(f"4M-vertex fit digests ({time.time() - t0:.1f} s): iterations {rec['iterations']}", flush=True)
+
+
 if __name__ == "__main__":
-    main()
+    if "--full" in sys.argv[1:]:
+        # optional selection: --full fit / --full batch delta2 (default: all three)
+        sel = [a for a in sys.argv[1:] if a in ("fit", "batch", "delta2")]
+        main_full(sel or ["delta2", "batch", "fit"])
+    else:
+        main()

@@ -1,7 +1,7 @@
 """The request server: results in a concurrent, batched mix equal the same
-requests run alone through the server, bit for bit, and the single-graph
-entry points (bit for bit in FP64, within 1e-9 for FP32->FP64); admission
-by footprint; a short synthetic stream end to end."""
+requests run alone through the server, bit for bit, the single-graph entry
+points and the CPU oracle (bit for bit in FP64, within 1e-9 for
+FP32->FP64); admission by footprint; a short synthetic stream end to end."""
 import numpy as np
 import pytest
 
@@ -56,7 +56,7 @@ def _same(a, b):
         assert np.array_equal(a.hist_values, b.hist_values) and np.array_equal(a.hist_counts, b.hist_counts)
 
 
-def test_mix_equals_alone(gpu, gen):
+def test_mix_equals_alone(gpu, ora, gen):
     from dress_hip import serve
     reqs = _requests(gen)
     with serve.Server(workers=4, budget_bytes=0, max_batch=8) as srv:
@@ -72,6 +72,23 @@ def test_mix_equals_alone(gpu, gen):
             alone = _collect(srv, 1)[j]
             _same(mix[i], alone)
             assert alone.batch_size == 1
+    # the oracle: bit for bit in FP64, within 1e-9 in FP32->FP64
+    for q, i in zip(reqs, ids):
+        r = mix[i]
+        if q["kind"] == 0:
+            o = ora.fit(q["n"], q["U"], q["V"], q["W"], variant=q["variant"], max_iterations=100, epsilon=1e-6)
+            assert r.iterations == o["iterations"]
+            if q["precision"] == 0:
+                assert r.delta == o["delta"]
+                assert np.array_equal(r.edge_dress, o["edge_dress"])
+                assert np.array_equal(r.vertex_dress, o["vertex_dress"])
+            else:
+                assert np.max(np.abs(r.edge_dress - o["edge_dress"]), initial=0.0) <= 1e-9
+        else:
+            vals, cnts, _, num = ora.lifted("delta" if q["kind"] == 1 else "nabla", q["n"], q["U"], q["V"], q["W"],
+                                            variant=q["variant"], k=1, use_omp=True)
+            assert num == r.subgraphs
+            assert np.array_equal(vals, r.hist_values) and np.array_equal(cnts, r.hist_counts)
     # and the single-graph entry points
     for q, i in zip(reqs, ids):
         r = mix[i]
@@ -116,3 +133,26 @@ def test_short_stream(gpu):
     assert rep["failed"] == 0
     lat = rep["latency"]["all"]
     assert lat["n"] > 0 and 0 < lat["p50_ms"] <= lat["p99_ms"]
+
+
+@pytest.mark.parametrize("precision", [0, 1])
+def test_zero_iterations_served_and_batched(gpu, ora, gen, precision):
+    """max_iterations = 0 runs no iteration (dress.c:546-608): every value
+    stays 1.0, in FP64 and FP32->FP64, through the server's batched lane and
+    through dress_hip_lifted_batch."""
+    from dress_hip import serve
+    graphs = [gen.random_regular(n, 6, 40 + n) for n in (16, 24, 30)]
+    r = gpu.lifted_batch(graphs, k=1, max_iterations=0, precision=precision)
+    for (hv, hc), (n, U, V) in zip(r.histograms, graphs):
+        ov, oc, _, _ = ora.lifted("delta", n, U, V, k=1, max_iterations=0)
+        assert np.array_equal(np.asarray(hv), ov) and np.array_equal(np.asarray(hc), oc)
+        assert np.array_equal(ov, [1.0])
+    with serve.Server(workers=1, budget_bytes=0, max_batch=8) as srv:
+        ids = [srv.submit(n, U, V, kind=serve.DELTA, k=1, max_iterations=0, precision=precision)
+               for n, U, V in graphs]
+        got = _collect(srv, len(ids))
+    for i, (n, U, V) in zip(ids, graphs):
+        ov, oc, _, num = ora.lifted("delta", n, U, V, k=1, max_iterations=0)
+        assert got[i].status == 0, got[i].error
+        assert got[i].subgraphs == num
+        assert np.array_equal(got[i].hist_values, ov) and np.array_equal(got[i].hist_counts, oc)

@@ -52,6 +52,19 @@ def test_library_exports_every_declared_symbol(dh):
     assert not missing, missing
 
 
+def test_library_exports_only_the_c_interface(dh):
+    """Internal C++ symbols (namespace dh, std template instances) stay
+    local, so bindings that load several DRESS backends cannot clash."""
+    import subprocess
+    out = subprocess.run(["nm", "-D", "--defined-only", dh._lib.LIB_PATH], capture_output=True, text=True,
+                         check=True).stdout
+    names = [ln.split()[-1] for ln in out.splitlines() if ln.strip()]
+    declared = set(declared_functions())
+    extra = [n for n in names if n not in declared]
+    assert not extra, extra[:20]
+    assert not [n for n in names if n.startswith("_Z")]
+
+
 def test_redirect_header_maps_to_hip_symbols():
     text = open(os.path.join(ROOT, "include", "dress", "hip", "dress.h")).read()
     pairs = dict(re.findall(r"#define\s+(\w+)\s+(\w+)", text))
@@ -90,6 +103,23 @@ def test_counts_match_reference_helpers(dh, ref):
         for k in range(-1, 10):
             assert L.dress_delta_binom(n, k) == ref.lib.dress_delta_binom(n, k)
             assert L.dress_nabla_perm_count(n, k) == ref.lib.dress_nabla_perm_count(n, k)
+
+
+@pytest.mark.parametrize("bad", [(3, 0), (0, -1), (2147483647, 1)])
+def test_vertex_ids_outside_the_graph_are_rejected(dh, bad):
+    """An edge list with an id outside [0, N) is refused by a host scan
+    before anything is written with it (on a GPU box too: no device work is
+    started, so this test is safe there)."""
+    n = 3
+    U = np.array([0, 1, bad[0]], np.int32)
+    V = np.array([1, 2, bad[1]], np.int32)
+    with pytest.raises(dh.HipError, match=r"outside \[0, 3\)"):
+        dh.Graph(n, U, V)
+    ok = (4, np.array([0, 1, 2], np.int32), np.array([1, 2, 3], np.int32))
+    with pytest.raises(dh.HipError, match=r"graph 1.*outside \[0, 3\)"):
+        dh.lifted_batch([ok, (n, U, V)], k=1)
+    with pytest.raises(dh.HipError, match=r"graph 1.*outside \[0, 3\)"):
+        dh.fit_batch([ok, (n, U, V)])
 
 
 def test_no_gpu_fails_loudly(dh):
