@@ -11,7 +11,7 @@ libdress v0.8.3's own outputs at full size:
 
 FP64 results must equal them bit for bit.  FP32->FP64 results are held to
 the north-star tolerance against the FP64 results just proven equal to the
-reference: histogram counts equal, every value (every edge of every
+reference: the same number of values, every value (every edge of every
 subgraph, through the multisets, where the output keeps edge identity)
 within 1e-9, and the floor(d/eps) histogram bit-exact once values within
 1e-9 of a bin edge are set apart.
@@ -111,15 +111,16 @@ def _expand(vals, cnts):
     return np.repeat(np.asarray(vals, dtype=np.float64), np.asarray(cnts, dtype=np.int64))
 
 
-def _quantised(vals, cnts, eps, tol, flags_from):
-    """floor(d / eps) histogram of vals (with counts), leaving out the
-    entries whose value in flags_from lies within tol of a bin edge"""
-    r = flags_from / eps
+def _quantised(v64, vm, eps, tol):
+    """floor(d / eps) histogram of the FP64 values v64 (one entry per value,
+    sorted), leaving out the entries whose FP32->FP64 value vm (sorted, same
+    length) lies within tol of a bin edge: what the device's table of vm
+    must hold"""
+    r = vm / eps
     q = np.floor(r)
     near = ((r - q) * eps < tol) | ((q + 1 - r) * eps < tol)
-    keys, inv = np.unique(np.floor(vals[~near] / eps).astype(np.int64), return_inverse=True)
-    counts = np.bincount(inv, weights=cnts[~near]).astype(np.int64) if keys.size else np.zeros(0, np.int64)
-    return keys, counts
+    keys, counts = np.unique(np.floor(v64[~near] / eps).astype(np.int64), return_counts=True)
+    return keys, counts.astype(np.int64), int(near.sum())
 
 
 @pytest.fixture(scope="module")
@@ -144,7 +145,7 @@ def test_delta1_batch512_fp64_equals_reference(gpu, batch512):
 
 
 def test_delta1_batch512_mixed_precision(gpu, batch512):
-    """FP32->FP64: per graph the same counts, values within 1e-9 of the FP64
+    """FP32->FP64: per graph the same number of values, within 1e-9 of the FP64
     (= reference) values, and the floor(d/eps) histogram bit-exact after the
     values within 1e-9 of a bin edge are set apart."""
     gold, graphs = batch512
@@ -155,13 +156,14 @@ def test_delta1_batch512_mixed_precision(gpu, batch512):
     for i, ((v64, c64), (vm, cm), (qk, qc), (bk, bc)) in enumerate(
             zip(ref.histograms, mix.histograms, mix.quantised, mix.boundary)):
         assert _hist_sha(v64, c64) == gold[i]["sha256"], i
-        v64, c64, vm, cm = (np.asarray(x) for x in (v64, c64, vm, cm))
-        assert np.array_equal(cm, c64), i
-        e = float(np.max(np.abs(vm - v64))) if vm.size else 0.0
+        e64, em = _expand(v64, c64), _expand(vm, cm)
+        assert e64.size == em.size == gold[i]["total"], i
+        e = float(np.max(np.abs(em - e64))) if em.size else 0.0
         worst = max(worst, e)
-        keys, counts = _quantised(v64, c64, eps, TOL, vm)
+        keys, counts, near = _quantised(e64, em, eps, TOL)
         assert np.array_equal(np.asarray(qk), keys) and np.array_equal(np.asarray(qc), counts), i
-        apart += int(np.asarray(bc).sum())
+        assert int(np.asarray(bc).sum()) == near, i
+        apart += near
     tot = sum(int(np.asarray(c).sum()) for _, c in mix.quantised) + apart
     assert tot == mix.values
     print(f"delta^1 batch of 512, FP32->FP64: max value error {worst:.3e}, {mix.fp64_reruns} FP64 reruns, "
@@ -217,6 +219,29 @@ def test_regular12_mixed_precision_per_edge(gpu, regular12, kind, k):
     print(f"{kind}^{k} of the 12-regular n=256 graph, FP32->FP64: max edge error {err:.3e} "
           f"over {int(live.sum())} values")
     assert err <= TOL
-    assert int(a[1].sum()) == int(b[1].sum())
-    assert np.array_equal(a[1], b[1])
-    assert np.max(np.abs(a[0] - b[0])) <= TOL
+    # exact histograms: the same number of values, within 1e-9 in sorted
+    # order (two FP64 values that coincide need not coincide in FP32->FP64,
+    # so the (value, count) pairs themselves may differ)
+    ea, eb = _expand(a[0], a[1]), _expand(b[0], b[1])
+    assert ea.size == eb.size == int(live.sum())
+    assert float(np.max(np.abs(ea - eb))) <= TOL
+
+
+def test_regular12_delta2_quantised_mixed(gpu, regular12):
+    """delta^2 through the batched API: the FP32->FP64 floor(d/eps) table
+    equals the FP64 (= reference) values' table once the values within 1e-9
+    of a bin edge are set apart, and counts add up to every value."""
+    n, U, V = regular12
+    eps = 1e-6
+    ref = gpu.lifted_batch([(n, U, V)], k=2, precision=gpu.FP64, epsilon=eps, qtol=TOL)
+    mix = gpu.lifted_batch([(n, U, V)], k=2, precision=gpu.FP32_FP64, epsilon=eps, qtol=TOL)
+    (v64, c64), (vm, cm) = ref.histograms[0], mix.histograms[0]
+    assert _hist_sha(v64, c64) == _load("regular12_n256_delta2.json")["sha256"]
+    e64, em = _expand(v64, c64), _expand(vm, cm)
+    assert e64.size == em.size
+    assert float(np.max(np.abs(e64 - em))) <= TOL
+    keys, counts, near = _quantised(e64, em, eps, TOL)
+    (qk, qc), (bk, bc) = mix.quantised[0], mix.boundary[0]
+    assert np.array_equal(np.asarray(qk), keys) and np.array_equal(np.asarray(qc), counts)
+    assert int(np.asarray(bc).sum()) == near
+    assert int(np.asarray(qc).sum()) + near == mix.values == e64.size
