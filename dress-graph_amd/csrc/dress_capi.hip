@@ -668,6 +668,27 @@ void yield_to_pressure(double max_seconds)
         std::this_thread::sleep_for(std::chrono::microseconds(200));
 }
 
+// most subgraph values one chunk of a delta/nabla call holds on the device
+// (the values and their histogram assembly): the calling thread's limit when
+// set (server workers size it from their reservation), else
+// DRESS_HIP_LIFT_CHUNK_VALUES, else what half of the free device memory holds
+static thread_local int64_t t_lift_values = 0;
+void set_thread_lifted_values(int64_t max_values) { t_lift_values = max_values > 0 ? max_values : 0; }
+int64_t lifted_chunk_values()
+{
+    if (t_lift_values > 0) return t_lift_values;
+    if (const char *e = getenv("DRESS_HIP_LIFT_CHUNK_VALUES")) {
+        long long v = atoll(e);
+        if (v > 0) return (int64_t)v;
+    }
+    size_t fr = 0, tot = 0;
+    if (hipMemGetInfo(&fr, &tot) != hipSuccess) {
+        (void)hipGetLastError();
+        fr = (size_t)8 << 30;
+    }
+    return std::max<int64_t>((int64_t)1 << 20, (int64_t)(fr / 2 / LIFT_BYTES_PER_VALUE));
+}
+
 static thread_local const std::atomic<bool> *t_cancel = nullptr;
 
 void set_thread_cancel(const std::atomic<bool> *flag) { t_cancel = flag; }

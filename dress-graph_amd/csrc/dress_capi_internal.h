@@ -21,6 +21,14 @@ void set_thread_precision(int precision);
 // per-subgraph lifted path); nullptr: none
 void set_thread_cancel(const std::atomic<bool> *flag);
 bool thread_cancelled();
+// device bytes per subgraph value while a delta/nabla call holds it: the
+// value, the histogram assembly's keys, segments, run flags and positions,
+// runs and the sort's temporary storage
+constexpr int64_t LIFT_BYTES_PER_VALUE = 112;
+// most values one chunk of a delta/nabla call keeps on the device; 0 clears
+// the calling thread's limit
+void set_thread_lifted_values(int64_t max_values);
+int64_t lifted_chunk_values();
 void record_stats(DevGraph *G, const FitResult &r);
 void record_lifted(int reruns, double device_ms, int64_t subgraphs, int64_t values);
 bool lds_eligible_size(int N, int E);

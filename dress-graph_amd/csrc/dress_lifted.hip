@@ -657,6 +657,16 @@ bool exact_histogram(const double *vals, int64_t n, const std::vector<int64_t> &
     return ok;
 }
 
+// The pairs of a histogram of `runs` entries are counted by an int in the
+// reference API (hist_size: 2 x pairs in the flat form)
+bool hist_size_fits(int64_t runs)
+{
+    if (runs <= (int64_t)(INT32_MAX / 2)) return true;
+    set_error("the histogram has " + std::to_string(runs) +
+              " distinct values, more than the int hist_size of the API can count");
+    return false;
+}
+
 // Flat histogram (dress_histogram.c:184-200 layout: value bits, count, ...)
 // of vals[0..n), malloc'd; NULL with *hist_size = 0 when empty.  (Faulting
 // the result block in from host threads while the device worked made calls
@@ -670,8 +680,8 @@ bool histogram_flat(const double *vals, int64_t n, hipStream_t s, int64_t **flat
     int64_t *dc = nullptr, runs = 0;
     std::vector<int64_t> rofs;
     if (!exact_histogram_dev(vals, n, {0, n}, s, &dv, &dc, &runs, rofs)) return false;
-    bool ok = true;
-    if (runs > 0) {
+    bool ok = hist_size_fits(runs);
+    if (ok && runs > 0) {
         int64_t *dflat = nullptr;
         int64_t *flat = (int64_t *)host_alloc((size_t)runs * 2 * sizeof(int64_t));
         if (!flat) set_error("host allocation of the histogram failed");
@@ -696,6 +706,94 @@ bool histogram_flat(const double *vals, int64_t n, hipStream_t s, int64_t **flat
     dfree(dv);
     dfree(dc);
     return ok;
+}
+
+// items [b, e) of a chunk and their live values
+struct ItemChunk {
+    size_t b, e;
+    int64_t values;
+};
+
+// consecutive items in chunks of at most cap live values (an item larger
+// than cap makes a chunk of its own); no chunk for no items
+std::vector<ItemChunk> item_chunks(const std::vector<int64_t> &live, int64_t cap)
+{
+    std::vector<ItemChunk> out;
+    size_t b = 0;
+    int64_t acc = 0;
+    for (size_t i = 0; i < live.size(); i++) {
+        if (i > b && acc + live[i] > cap) {
+            out.push_back({b, i, acc});
+            b = i;
+            acc = 0;
+        }
+        acc += live[i];
+    }
+    if (b < live.size()) out.push_back({b, live.size(), acc});
+    return out;
+}
+
+// order of the device histogram's sort keys (k_sort_keys): ascending value
+inline uint64_t hist_order(double x)
+{
+    uint64_t b;
+    memcpy(&b, &x, sizeof(b));
+    return (b >> 63) ? ~b : (b | 0x8000000000000000ull);
+}
+
+// (av, ac) += (v, c): both sorted ascending by value, exact keys (equal
+// bit patterns add their counts, as dress_hist_add compares with ==)
+void merge_histograms(std::vector<double> &av, std::vector<int64_t> &ac, const std::vector<double> &v,
+                      const std::vector<int64_t> &c)
+{
+    if (v.empty()) return;
+    if (av.empty()) {
+        av = v;
+        ac = c;
+        return;
+    }
+    std::vector<double> mv;
+    std::vector<int64_t> mc;
+    mv.reserve(av.size() + v.size());
+    mc.reserve(av.size() + v.size());
+    size_t i = 0, j = 0;
+    while (i < av.size() || j < v.size()) {
+        if (j == v.size() || (i < av.size() && hist_order(av[i]) < hist_order(v[j]))) {
+            mv.push_back(av[i]);
+            mc.push_back(ac[i++]);
+        } else if (i == av.size() || hist_order(v[j]) < hist_order(av[i])) {
+            mv.push_back(v[j]);
+            mc.push_back(c[j++]);
+        } else {
+            mv.push_back(av[i]);
+            mc.push_back(ac[i++] + c[j++]);
+        }
+    }
+    av.swap(mv);
+    ac.swap(mc);
+}
+
+// the flat (value bits, count, ...) form of a host histogram, malloc'd
+bool flat_from_vectors(const std::vector<double> &v, const std::vector<int64_t> &c, int64_t **flat_out,
+                       int *hist_size)
+{
+    *flat_out = nullptr;
+    if (hist_size) *hist_size = 0;
+    const int64_t runs = (int64_t)v.size();
+    if (runs == 0) return true;
+    if (!hist_size_fits(runs)) return false;
+    int64_t *flat = (int64_t *)host_alloc((size_t)runs * 2 * sizeof(int64_t));
+    if (!flat) {
+        set_error("host allocation of the histogram failed");
+        return false;
+    }
+    for (int64_t j = 0; j < runs; j++) {
+        memcpy(&flat[2 * j], &v[(size_t)j], sizeof(double));
+        flat[2 * j + 1] = c[(size_t)j];
+    }
+    *flat_out = flat;
+    if (hist_size) *hist_size = (int)(2 * runs);
+    return true;
 }
 
 // ------------------------------------------------------------ quantised table
@@ -1408,74 +1506,131 @@ int64_t *lifted_flat(int kind, p_dress_graph_t g, int k, int iterations, double 
     if (lds || (kind == LIFT_DELTA && g->variant == DRESS_VARIANT_DIRECTED && (mode == 0 || mode == 2 || mode == 3)))
         host_parent_build(N, E, g->U, g->V, g->W, (int)g->variant, P);
     pt.mark("parent");
-    if (lds) {
+    if (lds || mode == 0 || mode == 2 || mode == 3) {
         std::vector<int64_t> live = kind == LIFT_DELTA ? delta_live_counts(g, verts, k, sel.size())
                                                        : std::vector<int64_t>(sel.size(), (int64_t)E);
-        std::vector<int> item_parent(sel.size(), 0);
         pt.mark("live counts");
+        // Chunks of items whose live values (and their histogram assembly)
+        // fit the call's device budget; one chunk unless the request is
+        // larger than that.  Each chunk's exact histogram is merged on the
+        // host, so the result does not depend on the chunking.
+        const std::vector<ItemChunk> chunks = item_chunks(live, lifted_chunk_values());
         double *d_ms = nullptr;
         bool ok = true;
-        if (host_ms && total > 0) {
+        if (lds && host_ms && total > 0) {
             ok = dalloc(&d_ms, (size_t)total * (size_t)E + 1, "multisets");
             if (ok)
                 hipLaunchKernelGGL(k_fill_nan, dim3(grid_for((int64_t)total * E, 8192)), dim3(TPB), 0, s, d_ms,
                                    (int64_t)total * E);
         }
-        LiftOut out;
-        std::vector<const HostParent *> ps{&P};
-        std::vector<int> gof{0};
-        ok = ok && run_items(ps, gof, item_parent, verts, live, J, d_ms, E, sel, s, out, nullptr, nullptr, 0, 0.0);
-        pt.mark("run_items");
-        if (ok && host_ms && total > 0)
-            ok = hipStreamSynchronize(s) == hipSuccess &&
-                 d2h_parallel(host_ms, d_ms, (size_t)total * (size_t)E * sizeof(double));
-        pt.mark("multisets");
-        int64_t *flat = nullptr;
-        if (ok && do_hist) ok = histogram_flat(out.d_vals, out.val_ofs.back(), s, &flat, hist_size);
-        pt.mark("histogram");
-        dfree(out.d_vals);
-        dfree(d_ms);
-        if (!ok) return nullptr;
-        record_lifted(out.reruns, out.device_ms, (int64_t)sel.size(), out.val_ofs.back());
-        return flat;
-    } else if (mode == 0 || mode == 2 || mode == 3) {
-        // parent beyond LDS (or forced): the parent's resident device graph
-        // serves every subgraph -- many subgraphs per pass, or one masked
-        // fit at a time for tuples longer than a pass takes, DIRECTED
-        // reciprocal weight patches, or graphs whose pass state is too large
-        std::vector<int64_t> live = kind == LIFT_DELTA ? delta_live_counts(g, verts, k, sel.size())
-                                                       : std::vector<int64_t>(sel.size(), (int64_t)E);
-        int64_t total_live = 0;
-        for (int64_t x : live) total_live += x;
-        pt.mark("live counts");
-        double *d_vals = nullptr;
-        int64_t nv = 0;
         EwPatches ep;
-        if (kind == LIFT_DELTA) ep = delta_patches(P, verts, k, sel.size());
-        const bool multi = mode != 2 && !ep.any() && k <= multi_max_k() && build_bins(G) && multi_slots(G) > 0;
-        pt.mark("patches, bins");
-        bool ok = dalloc(&d_vals, (size_t)total_live + 1, "subgraph values");
-        if (ok && multi) {
-            ok = multi_items(G, kind, k, verts, live, sel, kind == LIFT_NABLA ? J.marks : nullptr, iterations,
-                             epsilon, d_vals, host_ms);
-            nv = total_live;
-            pt.mark("multi_items");
-        } else if (ok) {
-            std::vector<double> parent_w(ep.e.size());
-            for (size_t j = 0; j < ep.e.size(); j++) parent_w[j] = P.ew[(size_t)ep.e[j]];
-            ok = masked_items(g, G, kind, k, verts, live, sel, kind == LIFT_NABLA ? J.marks : nullptr, iterations,
-                              epsilon, precision, d_vals, total_live, &nv, host_ms, ep.off, ep.e, ep.w, parent_w);
+        bool multi = false;
+        if (!lds) {
+            // parent beyond LDS (or forced): the parent's resident device
+            // graph serves every subgraph -- many subgraphs per pass, or one
+            // masked fit at a time for tuples longer than a pass takes,
+            // DIRECTED reciprocal weight patches, or graphs whose pass state
+            // is too large
+            if (kind == LIFT_DELTA) ep = delta_patches(P, verts, k, sel.size());
+            multi = mode != 2 && !ep.any() && k <= multi_max_k() && build_bins(G) && multi_slots(G) > 0;
+            pt.mark("patches, bins");
         }
-        if (ok && nv != total_live) {
-            set_error("masked subgraph fits: live value count mismatch");
-            ok = false;
-        }
+        int reruns = 0;
+        double dev_ms = 0.0;
+        int64_t n_values = 0;
+        // fits items [c.b, c.e) into *d_vals (device, dfree'd by the caller)
+        auto run_chunk = [&](const ItemChunk &c, double **d_vals, int64_t *nv) -> bool {
+            *d_vals = nullptr;
+            *nv = 0;
+            const std::vector<int> cv(verts.begin() + (ptrdiff_t)(c.b * (size_t)k),
+                                      verts.begin() + (ptrdiff_t)(c.e * (size_t)k));
+            const std::vector<int64_t> cl(live.begin() + (ptrdiff_t)c.b, live.begin() + (ptrdiff_t)c.e);
+            const std::vector<int64_t> cs(sel.begin() + (ptrdiff_t)c.b, sel.begin() + (ptrdiff_t)c.e);
+            if (lds) {
+                LiftOut out;
+                std::vector<const HostParent *> ps{&P};
+                std::vector<int> gof{0};
+                std::vector<int> item_parent(cl.size(), 0);
+                if (!run_items(ps, gof, item_parent, cv, cl, J, d_ms, E, cs, s, out, nullptr, nullptr, 0, 0.0)) {
+                    dfree(out.d_vals);
+                    return false;
+                }
+                reruns += out.reruns;
+                dev_ms += out.device_ms;
+                *d_vals = out.d_vals;
+                *nv = out.val_ofs.back();
+                return true;
+            }
+            if (!dalloc(d_vals, (size_t)c.values + 1, "subgraph values")) return false;
+            if (multi) {
+                if (!multi_items(G, kind, k, cv, cl, cs, kind == LIFT_NABLA ? J.marks : nullptr, iterations, epsilon,
+                                 *d_vals, host_ms))
+                    return false;
+                *nv = c.values;
+                return true;
+            }
+            std::vector<int64_t> po;
+            std::vector<int> pe;
+            std::vector<double> pw, parent_w;
+            if (ep.any()) {
+                const int64_t p0 = ep.off[c.b], p1 = ep.off[c.e];
+                for (size_t i = c.b; i <= c.e; i++) po.push_back(ep.off[i] - p0);
+                pe.assign(ep.e.begin() + p0, ep.e.begin() + p1);
+                pw.assign(ep.w.begin() + p0, ep.w.begin() + p1);
+                for (int e : pe) parent_w.push_back(P.ew[(size_t)e]);
+            }
+            if (!masked_items(g, G, kind, k, cv, cl, cs, kind == LIFT_NABLA ? J.marks : nullptr, iterations, epsilon,
+                              precision, *d_vals, c.values, nv, host_ms, po, pe, pw, parent_w))
+                return false;
+            if (*nv != c.values) {
+                set_error("masked subgraph fits: live value count mismatch");
+                return false;
+            }
+            return true;
+        };
         int64_t *flat = nullptr;
-        if (ok && do_hist) ok = histogram_flat(d_vals, nv, s, &flat, hist_size);
-        pt.mark("histogram");
-        dfree(d_vals);
-        if (!ok) return nullptr;
-        record_lifted(0, 0.0, (int64_t)sel.size(), nv);
+        if (ok && chunks.size() <= 1) {
+            double *d_vals = nullptr;
+            ok = chunks.empty() || run_chunk(chunks[0], &d_vals, &n_values);
+            pt.mark("items");
+            if (ok && lds && host_ms && total > 0)
+                ok = hipStreamSynchronize(s) == hipSuccess &&
+                     d2h_parallel(host_ms, d_ms, (size_t)total * (size_t)E * sizeof(double));
+            if (ok && do_hist) ok = histogram_flat(d_vals, n_values, s, &flat, hist_size);
+            pt.mark("histogram");
+            dfree(d_vals);
+        } else if (ok) {
+            std::vector<double> acc_v, v;
+            std::vector<int64_t> acc_c, c;
+            std::vector<int> useg;
+            for (const ItemChunk &ch : chunks) {
+                if (thread_cancelled()) {
+                    set_error("cancelled");
+                    ok = false;
+                    break;
+                }
+                double *d_vals = nullptr;
+                int64_t nv = 0;
+                ok = run_chunk(ch, &d_vals, &nv);
+                if (ok && do_hist) ok = exact_histogram(d_vals, nv, {0, nv}, s, v, c, useg);
+                dfree(d_vals);
+                if (!ok) break;
+                n_values += nv;
+                if (do_hist) merge_histograms(acc_v, acc_c, v, c);
+            }
+            if (ok && lds && host_ms && total > 0)
+                ok = hipStreamSynchronize(s) == hipSuccess &&
+                     d2h_parallel(host_ms, d_ms, (size_t)total * (size_t)E * sizeof(double));
+            if (ok && do_hist) ok = flat_from_vectors(acc_v, acc_c, &flat, hist_size);
+            pt.mark("chunks");
+        }
+        dfree(d_ms);
+        if (!ok) {
+            free(flat);
+            if (hist_size) *hist_size = 0;
+            return nullptr;
+        }
+        record_lifted(reruns, dev_ms, (int64_t)sel.size(), n_values);
         return flat;
     } else {
         std::vector<double> vals;

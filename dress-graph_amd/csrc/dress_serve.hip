@@ -24,6 +24,7 @@
 #include <cstring>
 #include <deque>
 #include <mutex>
+#include <string>
 #include <thread>
 #include <vector>
 
@@ -48,6 +49,11 @@ struct dress_hip_server {
     int64_t reserved_large = 0;         // part of st.reserved_bytes held by the large lane
     std::atomic<int> short_load{0};     // queued + running requests outside the large lane
     std::atomic<int> fit_load{0};       // queued + running fit requests
+    std::atomic<int> inject{0};         // test hook: HIP error code for the next finished request
+    bool device_failed = false;         // a device error left the device unusable
+    std::string failure;                // ... and what it was
+    int64_t host_budget = 0;            // largest result one request may hold in host memory
+    int64_t chunk_values = 0;           // most subgraph values of a request on the device at once
     dress_hip_server_stats_t st{};
 };
 
@@ -87,17 +93,32 @@ int64_t count_items(int kind, int N, int k)
 // temporary storage) and, beyond the batched kernel, one multi-subgraph
 // pass (up to 256 slots of values, half-edge copies and norms; at most
 // 4 GiB).
-int64_t footprint(const ServeReq &q, int64_t t_bound)
+// Delta/nabla requests run in chunks of at most this many subgraph values
+// on the device (their histograms merged on the host), so a request's
+// device footprint is bounded whatever its size: 32M values take 3.6 GB
+// (DRESS_HIP_SERVE_CHUNK_VALUES overrides).
+constexpr int64_t SERVE_CHUNK_VALUES = (int64_t)32 << 20;
+
+int64_t footprint(const ServeReq &q, int64_t t_bound, int64_t chunk_values)
 {
     const int64_t N = q.r.N, E = q.r.E;
     int64_t fit = 400 * E + 64 * N + 8 * t_bound + (1 << 20);
     if (q.r.kind == DRESS_HIP_REQ_FIT) return fit;
     int64_t items = count_items(q.r.kind, q.r.N, q.r.k);
-    double vals = (double)items * (double)E;
-    double extra = 96.0 * vals;
+    double vals = std::min((double)items * (double)E, (double)chunk_values);
+    double extra = (double)LIFT_BYTES_PER_VALUE * vals;
     if (!q.lds) extra += std::min(2048.0 * (8.0 * (double)E + (double)N + 8.0), 4294967296.0);
     double b = (double)fit + extra;
     return b > 9.0e18 ? (int64_t)9.0e18 : (int64_t)b;
+}
+
+// Host bytes a request's result may take at its peak: a fit's values; a
+// delta/nabla histogram of up to one pair per subgraph value (16 B), twice
+// while chunk histograms are merged
+double host_result_bytes(const ServeReq &q)
+{
+    if (q.r.kind == DRESS_HIP_REQ_FIT) return 8.0 * ((double)q.r.E + (double)q.r.N);
+    return 32.0 * (double)count_items(q.r.kind, q.r.N, q.r.k) * (double)q.r.E;
 }
 
 // sum over edges of min(deg u, deg v): an upper bound of the index size
@@ -356,7 +377,7 @@ constexpr size_t SCAN_LIMIT = 256;
 
 bool take(dress_hip_server *S, int lane, bool may_take_large, std::vector<ServeReq *> &out, int64_t &bytes)
 {
-    if (S->queued() == 0) return false;
+    if (S->queued() == 0 || S->device_failed) return false;
     const int64_t room = S->st.budget_bytes - S->st.reserved_bytes;
     const int64_t large_room = (int64_t)(LARGE_BUDGET_SHARE * (double)S->st.budget_bytes) - S->reserved_large;
     const bool idle = S->st.running == 0;
@@ -417,6 +438,75 @@ bool take(dress_hip_server *S, int lane, bool may_take_large, std::vector<ServeR
     return true;
 }
 
+// HIP errors after which the device (its context) stays unusable: every
+// later call on it fails the same way
+bool sticky_error(hipError_t e)
+{
+    switch (e) {
+    case hipErrorIllegalAddress:
+    case hipErrorLaunchFailure:
+    case hipErrorAssert:
+    case hipErrorLaunchTimeOut:
+    case hipErrorECCNotCorrectable:
+    case hipErrorContextIsDestroyed:
+    case hipErrorRuntimeMemory:     // HSA queue errors (a queue abort, an
+    case hipErrorRuntimeOther:      // invalid allocation) end in these
+        return true;
+    default:
+        return false;
+    }
+}
+
+// After a failed request: is the device still usable?  Returns "" when it
+// is, else what is wrong.  `injected` (test hook) stands for the probe's
+// result; otherwise the worker's stream is synchronised and a small block
+// is allocated, written and freed.
+std::string device_error_check(hipStream_t stream, hipError_t injected)
+{
+    hipError_t e = injected;
+    if (e == hipSuccess) {
+        e = hipStreamSynchronize(stream);
+        if (e == hipSuccess) e = hipGetLastError();
+        void *p = nullptr;
+        if (e == hipSuccess) e = hipMalloc(&p, 256);
+        if (e == hipSuccess) e = hipMemsetAsync(p, 0, 256, stream);
+        if (e == hipSuccess) e = hipStreamSynchronize(stream);
+        if (p) (void)hipFree(p);
+        if (!sticky_error(e)) {
+            (void)hipGetLastError();
+            return std::string();
+        }
+    } else if (!sticky_error(e)) {
+        return std::string();
+    }
+    return std::string("device error ") + std::to_string((int)e) + " (" + hipGetErrorString(e) + ")";
+}
+
+// Under the lock: the device is unusable; every queued request is answered
+// with the cause, later submissions are refused, workers take nothing more.
+void stop_after_device_error(dress_hip_server *S, const std::string &why)
+{
+    S->device_failed = true;
+    S->failure = why;
+    S->st.device_failed = 1;
+    const double t = now_s();
+    for (auto &dq : S->lanes) {
+        for (ServeReq *q : dq) {
+            dress_hip_response_t o = blank_response(*q);
+            o.admitted = o.finished = t;
+            fail(o, "server stopped after " + why);
+            S->st.failed++;
+            if (lane_of(q) != LANE_LIFT_LARGE) S->short_load--;
+            if (q->r.kind == DRESS_HIP_REQ_FIT) S->fit_load--;
+            S->done.push_back(o);
+            delete q;
+        }
+        dq.clear();
+    }
+    S->st.queued = 0;
+    fprintf(stderr, "libdress_hip: request server stopped after %s\n", why.c_str());
+}
+
 void worker_main(dress_hip_server *S, int index)
 {
     hipSetDevice(S->device);
@@ -439,6 +529,7 @@ void worker_main(dress_hip_server *S, int index)
     // without a large lane (fewer than four workers) the last worker takes
     // large requests as well
     const bool may_take_large = lane == LANE_LIFT_LARGE || (S->n_workers < 4 && index == S->n_workers - 1);
+    set_thread_lifted_values(S->chunk_values);
     for (;;) {
         std::vector<ServeReq *> batch;
         int64_t bytes = 0;
@@ -455,20 +546,36 @@ void worker_main(dress_hip_server *S, int index)
             outs.back().admitted = admitted;
             outs.back().batch_size = (int)batch.size();
         }
+        bool chunked = false;
         if (!stream) {
             for (auto &o : outs) fail(o, "stream creation failed");
         } else if (batch[0]->r.kind == DRESS_HIP_REQ_FIT) {
             if (batch[0]->lds) run_fit_batch(batch, outs);
             else run_fit(*batch[0], outs[0]);
         } else if (batch.size() == 1) {
+            chunked = (double)count_items(batch[0]->r.kind, batch[0]->r.N, batch[0]->r.k) * (double)batch[0]->r.E >
+                      (double)S->chunk_values;
             run_lifted_one(*batch[0], outs[0]);
         } else {
             run_lifted_batch(batch, outs);
         }
+        // the test hook: this request fails as if the device reported `code`
+        const int injected = S->inject.exchange(0);
+        if (injected) {
+            const std::string msg = std::string("device error (injected): ") +
+                                    hipGetErrorString((hipError_t)injected);
+            for (auto &o : outs) fail(o, msg);
+        }
+        bool failed = false;
+        for (auto &o : outs) failed = failed || o.status != 0;
+        // a failure may come from a device error that leaves the device
+        // unusable (every later call fails): check, and stop the server
+        std::string dead;
+        if (failed) dead = device_error_check(stream, (hipError_t)injected);
         // large transient blocks go back to the device before the budget
         // they were reserved under is released, once this worker holds
         // more than 16 GiB of them
-        ctx_trim((size_t)256 << 20, (size_t)16 << 30);
+        if (dead.empty()) ctx_trim((size_t)256 << 20, (size_t)16 << 30);
         const double fin = now_s();
         {
             std::lock_guard<std::mutex> lk(S->mu);
@@ -487,6 +594,8 @@ void worker_main(dress_hip_server *S, int index)
                 else S->st.failed++;
                 S->done.push_back(o);
             }
+            if (chunked) S->st.chunked_requests++;
+            if (!dead.empty() && !S->device_failed) stop_after_device_error(S, dead);
         }
         for (ServeReq *q : batch) delete q;
         S->cv_done.notify_all();
@@ -494,6 +603,7 @@ void worker_main(dress_hip_server *S, int index)
     }
     set_thread_cancel(nullptr);
     set_thread_pressure(nullptr);
+    set_thread_lifted_values(0);
     if (stream) {
         ctx_end();
         hipStreamDestroy(stream);
@@ -530,6 +640,16 @@ dress_hip_server_t *dress_hip_server_create(int workers, int64_t budget_bytes, i
         budget_bytes = (int64_t)(fr / 2);
     }
     S->st.budget_bytes = budget_bytes;
+    S->host_budget = (int64_t)32 << 30;
+    S->chunk_values = SERVE_CHUNK_VALUES;
+    if (const char *e = getenv("DRESS_HIP_SERVE_CHUNK_VALUES")) {
+        long long v = atoll(e);
+        if (v > 0) S->chunk_values = (int64_t)v;
+    }
+    if (const char *e = getenv("DRESS_HIP_SERVE_HOST_BYTES")) {
+        long long v = atoll(e);
+        if (v > 0) S->host_budget = (int64_t)v;
+    }
     S->max_batch = max_batch < 1 ? 1 : max_batch;
     S->n_workers = workers;
     for (int i = 0; i < workers; i++) S->workers.emplace_back(worker_main, S, i);
@@ -551,6 +671,13 @@ int64_t dress_hip_server_submit(dress_hip_server_t *S, const dress_hip_request_t
             set_error("dress_hip_server_submit: vertex id out of range");
             return -1;
         }
+    {
+        std::lock_guard<std::mutex> lk(S->mu);
+        if (S->device_failed) {
+            set_error("dress_hip_server_submit: the server stopped after " + S->failure);
+            return -1;
+        }
+    }
     auto *q = new ServeReq();
     q->r = *req;
     q->r.U = q->r.V = nullptr;
@@ -561,7 +688,7 @@ int64_t dress_hip_server_submit(dress_hip_server_t *S, const dress_hip_request_t
     if (q->weighted) q->W.assign(req->W, req->W + req->E);
     q->arrival = req->arrival >= 0 ? req->arrival : now_s();
     q->lds = lds_eligible_size(req->N, req->E) && (req->kind != DRESS_HIP_REQ_FIT || req->max_iterations > 0);
-    q->bytes = footprint(*q, intersection_bound(req->N, req->E, req->U, req->V));
+    q->bytes = footprint(*q, intersection_bound(req->N, req->E, req->U, req->V), S->chunk_values);
     q->work = req->kind == DRESS_HIP_REQ_FIT ? (int64_t)req->E
                                              : count_items(req->kind, req->N, req->k) * (int64_t)req->E;
     std::lock_guard<std::mutex> lk(S->mu);
@@ -570,6 +697,26 @@ int64_t dress_hip_server_submit(dress_hip_server_t *S, const dress_hip_request_t
     const int64_t limit = lane_of(q) == LANE_LIFT_LARGE
                               ? (int64_t)(LARGE_BUDGET_SHARE * (double)S->st.budget_bytes)
                               : S->st.budget_bytes;
+    // a delta/nabla result counts its pairs in an int (hist_size = pairs);
+    // every subgraph value may be distinct
+    const double values = q->r.kind == DRESS_HIP_REQ_FIT
+                              ? 0.0
+                              : (double)count_items(q->r.kind, q->r.N, q->r.k) * (double)q->r.E;
+    const double hb = host_result_bytes(*q);
+    if (hb > (double)S->host_budget || values > (double)INT32_MAX) {
+        dress_hip_response_t o = blank_response(*q);
+        o.admitted = o.finished = now_s();
+        fail(o, "request result of up to " + std::to_string((int64_t)hb) +
+                    " B exceeds the server's host budget of " + std::to_string(S->host_budget) +
+                    " B per result (or the int hist_size of the API)");
+        S->st.rejected++;
+        S->st.rejected_host++;
+        S->done.push_back(o);
+        int64_t id = q->id;
+        delete q;
+        S->cv_done.notify_all();
+        return id;
+    }
     if (q->bytes > limit) {
         dress_hip_response_t o = blank_response(*q);
         o.admitted = o.finished = now_s();
@@ -619,6 +766,13 @@ void dress_hip_server_stats(dress_hip_server_t *S, dress_hip_server_stats_t *out
     if (!S || !out) return;
     std::lock_guard<std::mutex> lk(S->mu);
     *out = S->st;
+}
+
+int dress_hip_server_inject_error(dress_hip_server_t *S, int hip_error)
+{
+    if (!S) return -1;
+    S->inject = hip_error;
+    return 0;
 }
 
 void dress_hip_server_destroy(dress_hip_server_t *S)

@@ -95,7 +95,8 @@ class ServeResponse(ctypes.Structure):
 class ServeStats(ctypes.Structure):
     _fields_ = [(n, ctypes.c_int64) for n in (
         "submitted", "completed", "failed", "rejected", "queued", "running", "max_queued",
-        "budget_bytes", "reserved_bytes", "peak_reserved_bytes", "lifted_launches", "lifted_requests")]
+        "budget_bytes", "reserved_bytes", "peak_reserved_bytes", "lifted_launches", "lifted_requests",
+        "rejected_host", "chunked_requests", "device_failed")]
 
 
 PG = ctypes.POINTER(DressGraph)
@@ -193,6 +194,8 @@ def _bind(L):
     L.dress_hip_response_free.argtypes = [ctypes.POINTER(ServeResponse)]
     L.dress_hip_server_stats.restype = None
     L.dress_hip_server_stats.argtypes = [ctypes.c_void_p, ctypes.POINTER(ServeStats)]
+    L.dress_hip_server_inject_error.restype = ctypes.c_int
+    L.dress_hip_server_inject_error.argtypes = [ctypes.c_void_p, ctypes.c_int]
     L.dress_hip_server_destroy.restype = None
     L.dress_hip_server_destroy.argtypes = [ctypes.c_void_p]
     strided = _lifted_args() + [ctypes.c_int, ctypes.c_int]

@@ -124,6 +124,11 @@ class Server:
         _lib.lib().dress_hip_server_stats(self._h, ctypes.byref(s))
         return {name: getattr(s, name) for name, _ in ServeStats._fields_}
 
+    def inject_error(self, hip_error):
+        """Failure-path test hook: the next finished request fails as if
+        the device had reported HIP error code ``hip_error``."""
+        _lib.lib().dress_hip_server_inject_error(self._h, int(hip_error))
+
     def close(self):
         if self._h:
             _lib.lib().dress_hip_server_destroy(self._h)
@@ -243,6 +248,8 @@ def run_stream(duration=60.0, rate=2000.0, seed=0, workers=8, budget_bytes=0, ma
         "completed": {"fit": len(by["fit"]), "delta": len(by["delta"]),
                       "per_s": len(ok) / elapsed if elapsed > 0 else None},
         "failed": int(st["failed"]), "rejected": int(st["rejected"]),
+        "rejected_host": int(st["rejected_host"]), "chunked": int(st["chunked_requests"]),
+        "device_failed": int(st["device_failed"]),
         "unfinished": n_unfinished,
         "fit_edge_updates_per_s": sum(r.E * (r.iterations + 1) for r in by["fit"]) / elapsed if elapsed > 0 else None,
         "delta1_subgraph_solves_per_s": sum(r.subgraphs for r in by["delta"]) / elapsed if elapsed > 0 else None,

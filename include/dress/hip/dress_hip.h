@@ -323,9 +323,18 @@ void dress_hip_fit_batch_free(dress_hip_fit_batch_t *r);
  * lane has one worker, holds at most half of the budget, fills at most a
  * quarter of the device while other requests wait or run, and pauses
  * between its passes for them; batched delta/nabla launches fill at most
- * half the device while fits wait or run.  A request whose footprint exceeds
- * the budget of its kind (the whole budget, half of it for the large lane)
- * is answered at once with an error.  A request's result does not depend on what runs beside it;
+ * half the device while fits wait or run.  Delta/nabla requests run in
+ * chunks of subgraphs whose values fit a fixed device share (32M values),
+ * each chunk's histogram merged on the host, so their footprint is bounded
+ * whatever their size.  A request whose footprint still exceeds the budget
+ * of its kind (the whole budget, half of it for the large lane), or whose
+ * result would exceed the server's host-memory budget for one result
+ * (DRESS_HIP_SERVE_HOST_BYTES, default 32 GiB; also the int hist_size of
+ * the API), is answered at once with an error.  After a device error that
+ * leaves the device unusable (an illegal address, a failed launch, ...)
+ * the server stops: queued requests are answered with that error, submit
+ * returns -1, and dress_hip_server_stats reports device_failed = 1.
+ * A request's result does not depend on what runs beside it;
  * FP64 results are those of the single-graph calls (dress_fit_hip_ex,
  * dress_delta_fit_hip, dress_nabla_fit_hip) bit for bit, FP32->FP64 ones
  * within 1e-9 of them. */
@@ -372,6 +381,9 @@ typedef struct {
     int64_t queued, running, max_queued;
     int64_t budget_bytes, reserved_bytes, peak_reserved_bytes;
     int64_t lifted_launches, lifted_requests;  /* batched launches, requests in them */
+    int64_t rejected_host;         /* of rejected: result beyond the host budget  */
+    int64_t chunked_requests;      /* delta/nabla requests run in several chunks   */
+    int64_t device_failed;         /* 1 once a device error stopped the server    */
 } dress_hip_server_stats_t;
 
 /* Monotonic clock in seconds, the time base of arrival/admitted/finished. */
@@ -391,6 +403,12 @@ int64_t dress_hip_server_submit(dress_hip_server_t *s, const dress_hip_request_t
 int dress_hip_server_next(dress_hip_server_t *s, double timeout_s, dress_hip_response_t *out);
 void dress_hip_response_free(dress_hip_response_t *r);
 void dress_hip_server_stats(dress_hip_server_t *s, dress_hip_server_stats_t *out);
+
+/* Failure-path test hook: the next request a worker finishes is failed as if
+ * the device had reported the HIP error code `hip_error`, and the server's
+ * device-error check runs on that code (a code that leaves the device
+ * unusable stops the server).  No device work is involved.  Returns 0. */
+int dress_hip_server_inject_error(dress_hip_server_t *s, int hip_error);
 
 /* Drops queued requests, stops running delta/nabla requests at their next
  * subgraph (they finish with status -1), waits for the rest, frees the

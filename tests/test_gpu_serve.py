@@ -156,3 +156,55 @@ def test_zero_iterations_served_and_batched(gpu, ora, gen, precision):
         assert got[i].status == 0, got[i].error
         assert got[i].subgraphs == num
         assert np.array_equal(got[i].hist_values, ov) and np.array_equal(got[i].hist_counts, oc)
+
+
+def test_device_error_stops_the_server(gpu, gen):
+    """A request failing with an error that leaves the device unusable
+    (injected on the host as HIP error 700, illegal address: no device fault
+    is made) stops the server: queued requests are answered with the cause,
+    new submissions are refused, the stats say so.  A failure that leaves
+    the device usable (out of memory, 2) does not."""
+    from dress_hip import serve
+    from dress_hip._lib import HipError
+    _, U, V, _ = gen.gnp(60, 6.0, 31)
+    with serve.Server(workers=1, budget_bytes=0, max_batch=1) as srv:
+        srv.inject_error(2)
+        a = srv.submit(60, U, V, kind=serve.FIT)
+        ra = _collect(srv, 1)[a]
+        assert ra.status == -1 and "injected" in ra.error
+        b = srv.submit(60, U, V, kind=serve.FIT)
+        assert _collect(srv, 1)[b].status == 0
+        assert srv.stats()["device_failed"] == 0
+        srv.inject_error(700)
+        ids = [srv.submit(60, U, V, kind=serve.FIT) for _ in range(6)]
+        got = _collect(srv, len(ids))
+        st = srv.stats()
+        assert all(got[i].status == -1 for i in ids)
+        assert any("injected" in got[i].error for i in ids)
+        assert st["device_failed"] == 1 and st["queued"] == 0
+        with pytest.raises(HipError, match="stopped"):
+            srv.submit(60, U, V, kind=serve.FIT)
+
+
+def test_large_delta_runs_in_chunks(gpu, ora, gen, monkeypatch):
+    """A delta^1 request whose subgraph values exceed the server's chunk is
+    planned into chunks (not refused for its footprint) and its histogram
+    equals the oracle's; a request whose result exceeds the host budget is
+    refused and counted apart."""
+    from dress_hip import serve
+    monkeypatch.setenv("DRESS_HIP_SERVE_CHUNK_VALUES", "20000")
+    n, U, V, _ = gen.gnp(400, 12.0, 41)
+    monkeypatch.setenv("DRESS_HIP_SERVE_HOST_BYTES", str(64 << 20))
+    with serve.Server(workers=4, budget_bytes=2 << 30, max_batch=4) as srv:
+        a = srv.submit(n, U, V, kind=serve.DELTA, k=1)
+        n2, U2, V2, _ = gen.gnp(3000, 12.0, 42)                   # 3000 x 18k values: > 64 MB of pairs
+        b = srv.submit(n2, U2, V2, kind=serve.DELTA, k=1)
+        got = _collect(srv, 2)
+        st = srv.stats()
+    assert got[a].status == 0, got[a].error
+    vals, cnts, _, num = ora.lifted("delta", n, U, V, k=1, use_omp=True)
+    assert got[a].subgraphs == num
+    assert np.array_equal(got[a].hist_values, vals) and np.array_equal(got[a].hist_counts, cnts)
+    assert got[b].status == -1 and "host budget" in got[b].error
+    assert st["rejected_host"] == 1 and st["rejected"] == 1
+    assert st["chunked_requests"] == 1
