@@ -5,26 +5,32 @@ Headline workload: plain FP64 dress_fit on the 4,000,000-vertex Chung-Lu
 power-law graph (exponent 2.1, expected degrees capped at 20,000, seed 1,
 28,000,000 undirected edges, epsilon 1e-6, at most 100 iterations).  One step
 is one complete fit (every iteration to convergence) of the device-resident
-graph from d = 1; value = edge-updates/s = E * iterations / time.
+graph from d = 1; value = edge-updates/s = E * iterations / time.  The timed
+FP64 values are checked against the reference's digest of the same fit.
 
 The same line also carries
-  * the FP32->FP64 fit of the same graph,
+  * the same fit through the C ABI (dress_fit: host arrays in and out),
+  * the FP32->FP64 fit of the same graph, with its max per-edge error,
+  * the ER G(2000, 8/1999) fit (both directions) in FP64 and FP32->FP64,
   * delta^1 of the 512 random 6-regular graphs (n ~ U{16..128}): subgraph
-    solves/s of the batched kernel (FP64 and FP32->FP64) beside the plain
-    FP64 GPU path (one device fit per subgraph, the CUDA backend's structure),
+    solves/s of the batched kernel (FP64 and FP32->FP64, medians of 5) beside
+    the plain FP64 GPU path (one device fit per subgraph, the CUDA backend's
+    structure), the kernel's LDS roofline against a measured LDS peak,
   * delta^1 of a 2,000-vertex G(n, p) graph too large for the batched
     kernel: multi-subgraph passes on its device copy beside one device fit per
     rebuilt subgraph,
   * delta^2 and nabla^1 of a random 12-regular graph on 256 vertices,
   * the roofline of the dominant kernel (HIP events around its launches),
-  * the serving stream: Poisson arrivals at 2000 requests/s (70 % fits, 30 %
-    delta^1, G(n,p) with n log-uniform in [20, 20000], mean degree U[3, 30],
-    half FP32-weighted) against the request server, with p50/p99 latency,
-  * the CPU baseline: libdress compiled unchanged (OpenMP) on this host.
+  * the serving stream: Poisson arrivals at 2000 requests/s for 60 s (70 %
+    fits, 30 % delta^1, G(n,p) with n log-uniform in [20, 20000], mean degree
+    U[3, 30], half FP32-weighted) against the request server, p50/p99 latency,
+  * the CPU baseline: libdress compiled unchanged (OpenMP) on this host,
+    medians of 5, with the host's CPU described.
 
-    python bench.py [--gpus 1] [--steps 5] [--warmup 1] [--no-cpu-baseline] [--serve-seconds 10]
+    python bench.py [--gpus 1] [--steps 5] [--warmup 1] [--no-cpu-baseline] [--serve-seconds 60]
 """
 import argparse
+import hashlib
 import json
 import os
 import sys
@@ -44,20 +50,32 @@ HBM_PEAK_GBS = 8000.0
 # (20 B).  The edge-phase kernel of one work bin moves 24 B per entry and
 # 44 B per edge of the bin.
 B_ENTRY, B_HALF, B_EDGE, B_VERTEX = 24, 12, 44, 20
-# HBM bytes per launch of the dominant kernel from hardware counters
-# (FETCH_SIZE and WRITE_SIZE, one rocprofv3 --pmc pass each, over
-# `python bench.py --traffic-run`; profiles/fit_power_law_4M_fp64.md), each
-# scaled by the bytes a plain 1 GiB copy kernel moves per counted KiB.  Valid
-# for the kernel and graph it was measured on: reported only when that
-# kernel is the dominant one.
+# LDS bytes of one iteration of a subgraph in the batched kernel: per entry
+# two 8 B value reads, per half-edge one, per edge its value, new value, two
+# norms and two alive flags, per vertex its norm and alive flag
+L_ENTRY, L_HALF, L_EDGE, L_VERTEX = 16, 8, 40, 12
+# HBM bytes per launch of the dominant kernel from hardware counters, from
+# the run recorded in profiles/fit_power_law_4M_fp64.md (rocprofv3 --pmc
+# FETCH_SIZE and --pmc WRITE_SIZE over `python bench.py --traffic-run`), each
+# scaled by what the same counter reads for probes of known bytes: the
+# streaming copy (FETCH_SIZE x2: it counts half of a copy's reads) and random
+# 8-byte gathers (the pattern of the edge kernels' gathers).  Counters
+# cannot be read inside this process, so the figure is labelled with its
+# source and reported only when that kernel is the dominant one.
 TRAFFIC = {"kernel": "k_edge<16,packed,double> (edge bin 2)", "fetch_kib": 5418725, "write_kib": 117693,
-           "fetch_scale": 2.0, "write_scale": 1.0}
+           "fetch_scale": 2.0, "write_scale": 1.0,
+           "source": "profiles/fit_power_law_4M_fp64.md, counter runs of bench.py --traffic-run"}
 TRAFFIC["bytes_per_launch"] = int((TRAFFIC["fetch_kib"] * TRAFFIC["fetch_scale"] +
                                    TRAFFIC["write_kib"] * TRAFFIC["write_scale"]) * 1024)
+GOLDEN = os.path.join(ROOT, "tests", "golden")
 
 
 def log(*a):
     print(*a, file=sys.stderr, flush=True)
+
+
+def median(xs):
+    return float(np.median(np.asarray(xs, dtype=np.float64)))
 
 
 def fit_steps(g, steps, precision, eps=1e-6, max_it=100):
@@ -77,19 +95,67 @@ def fit_steps(g, steps, precision, eps=1e-6, max_it=100):
             "fp32_iterations": fp32}
 
 
-def cpu_baseline_fit(N, U, V, csr, iterations=2):
-    """libdress (compiled unchanged, OpenMP) on this host: `iterations` of the
-    same fit with its intersection index prebuilt; returns edge-updates/s."""
+def expand(vals, cnts):
+    return np.repeat(np.asarray(vals, dtype=np.float64), np.asarray(cnts, dtype=np.int64))
+
+
+def sorted_err(a_vals, a_cnts, b_vals, b_cnts):
+    """max |a - b| over two histograms of the same number of values, each
+    expanded and compared in sorted order"""
+    a, b = expand(a_vals, a_cnts), expand(b_vals, b_cnts)
+    if a.size != b.size:
+        return float("inf")
+    return float(np.max(np.abs(a - b))) if a.size else 0.0
+
+
+def host_cpu():
+    """CPU model, sockets, physical cores and logical CPUs of this host."""
+    model, sockets, cores, logical = None, set(), set(), 0
+    phys = None
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                k, _, v = line.partition(":")
+                k, v = k.strip(), v.strip()
+                if k == "processor":
+                    logical += 1
+                elif k == "model name" and model is None:
+                    model = v
+                elif k == "physical id":
+                    phys = v
+                    sockets.add(v)
+                elif k == "core id":
+                    cores.add((phys, v))
+    except OSError:
+        pass
+    return {"model": model, "sockets": len(sockets) or None, "physical_cores": len(cores) or None,
+            "logical_cpus": logical or os.cpu_count(), "affinity_cpus": len(os.sched_getaffinity(0)),
+            "OMP_NUM_THREADS": os.environ.get("OMP_NUM_THREADS")}
+
+
+def omp_threads(cpu):
+    v = os.environ.get("OMP_NUM_THREADS")
+    if v and v.isdigit():
+        return int(v)
+    return cpu["physical_cores"] or cpu["affinity_cpus"]
+
+
+def cpu_baseline_fit(N, U, V, csr, runs=5):
+    """libdress (compiled unchanged, OpenMP) on this host: one iteration of
+    the same fit with its intersection index prebuilt, `runs` times; returns
+    the median edge-updates/s."""
     sys.path.insert(0, os.path.join(ROOT, "oracle"))
     import pyoracle
     if not pyoracle.available("ref"):
-        return None, "oracle", None
+        return None, None, None
     lib = pyoracle.Lib("ref")
     g = pyoracle.ref_graph_from_csr(lib, N, U, V, csr)
     try:
-        t0 = time.perf_counter()
-        lib.fit_graph(g, iterations, 1e-6, parallel=True)
-        dt = time.perf_counter() - t0
+        times = []
+        for _ in range(runs):
+            t0 = time.perf_counter()
+            lib.fit_graph(g, 1, 1e-6, parallel=True)
+            times.append(time.perf_counter() - t0)
         # dress_get of the same library on a sample of the bench's virtual-edge
         # pairs (one host thread: the reference's dress_get is serial)
         qu, qv, _, _ = get_pairs(N, U, V)
@@ -99,19 +165,17 @@ def cpu_baseline_fit(N, U, V, csr, iterations=2):
         get_rate = GET_SAMPLE / (time.perf_counter() - t0)
     finally:
         lib.free_graph(g)
-    return U.size * iterations / dt, dt, get_rate
+    return U.size / median(times), times, get_rate
 
 
-def cpu_baseline_delta1(graphs):
-    sys.path.insert(0, os.path.join(ROOT, "oracle"))
-    import pyoracle
-    lib = pyoracle.Lib("ref") if pyoracle.available("ref") else pyoracle.Lib("oracle")
-    t0 = time.perf_counter()
-    solves = 0
-    for n, U, V in graphs:
-        _, _, _, num = lib.lifted("delta", n, U, V, k=1, use_omp=True)
-        solves += num
-    return solves / (time.perf_counter() - t0), lib.which
+def cpu_lifted_rate(lib, runs, fn):
+    """median solves/s of `runs` calls of fn(lib) -> solves"""
+    rates = []
+    for _ in range(runs):
+        t0 = time.perf_counter()
+        num = fn(lib)
+        rates.append(num / (time.perf_counter() - t0))
+    return median(rates), rates
 
 
 GET_SAMPLE = 20_000          # reference dress_get queries timed on the host
@@ -174,22 +238,55 @@ def delta1_large(dh, gen):
         t0 = time.perf_counter()
         gl.lifted("delta", 1)
         first = time.perf_counter() - t0
-        t0 = time.perf_counter()
-        _, cnts, _, num = gl.lifted("delta", 1)
-        wall = time.perf_counter() - t0
+        walls = []
+        for _ in range(3):
+            t0 = time.perf_counter()
+            _, cnts, _, num = gl.lifted("delta", 1)
+            walls.append(time.perf_counter() - t0)
+        wall = median(walls)
         big["multi_pass_fp64"] = {"solves_per_s": num / wall, "subgraphs": int(num), "values": int(cnts.sum()),
                                   "histogram_pairs": int(cnts.size), "wall_ms": wall * 1e3,
-                                  "first_call_ms": first * 1e3}
+                                  "wall_ms_runs": [w * 1e3 for w in walls], "first_call_ms": first * 1e3}
         dh.set_lifted_mode(1)
         try:
             t0 = time.perf_counter()
-            gl.lifted("delta", 1, n_samples=200, seed=2)
+            _, _, _, _ = gl.lifted("delta", 1, n_samples=200, seed=2)
             wall = time.perf_counter() - t0
         finally:
             dh.set_lifted_mode(0)
-        big["plain_fp64_gpu"] = {"solves_per_s": 200 / wall, "sample": "200 sampled subgraphs"}
+        picks = len(dh.enumerate_subgraphs("delta", nl, 1, 200, 2)[0])      # sorted, deduplicated picks
+        big["plain_fp64_gpu"] = {"solves_per_s": picks / wall,
+                                 "sample": f"200 sampled subgraphs ({picks} distinct picks)"}
     big["speedup_vs_plain_fp64_gpu"] = big["multi_pass_fp64"]["solves_per_s"] / big["plain_fp64_gpu"]["solves_per_s"]
     return big
+
+
+def sha256(a):
+    return hashlib.sha256(np.ascontiguousarray(a, dtype=np.float64).tobytes()).hexdigest()
+
+
+def er2000(dh, gen):
+    """configuration 1: ER G(2000, 8/1999), both directions, FP64 and
+    FP32->FP64 through the C ABI"""
+    N, U, V = gen.er_both_directions()
+    out = {"workload": "ER G(2000, 8/1999) seed 0, every edge in both directions (E=16,040), eps 1e-6"}
+    vals = {}
+    with dh.Graph(N, U, V) as g:
+        for name, prec in (("fp64", dh.FP64), ("fp32_fp64", dh.FP32_FP64)):
+            walls = []
+            for _ in range(5):
+                g.set_edge_dress(np.ones(U.size))
+                t0 = time.perf_counter()
+                it, dl = g.fit(100, 1e-6, prec)
+                walls.append(time.perf_counter() - t0)
+            executed = dh.last_fit_stats().iterations_executed
+            vals[name] = np.array(g.edge_dress)
+            out[name] = {"iterations": it, "wall_ms": median(walls) * 1e3,
+                         "edge_updates_per_s": U.size * executed / median(walls)}
+    z = np.load(os.path.join(GOLDEN, "er2000_both.npz"))
+    out["fp64"]["equals_reference"] = bool(np.array_equal(vals["fp64"], z["edge_dress"]))
+    out["fp32_fp64"]["max_edge_err"] = float(np.max(np.abs(vals["fp32_fp64"] - z["edge_dress"])))
+    return out
 
 
 def main():
@@ -199,10 +296,10 @@ def main():
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-delta", action="store_true")
-    ap.add_argument("--serve-seconds", type=float, default=10.0,
+    ap.add_argument("--serve-seconds", type=float, default=60.0,
                     help="length of the serving stream (Poisson 2000 req/s); 0 skips it")
     ap.add_argument("--traffic-run", action="store_true",
-                    help="one FP64 fit and 1 GiB copy-kernel passes only, for hardware-counter runs")
+                    help="one FP64 fit and the copy and gather probes only, for hardware-counter runs")
     ap.add_argument("--large-delta-only", action="store_true",
                     help="only the delta^1 section of the 2,000-vertex graph (for kernel traces)")
     args = ap.parse_args()
@@ -230,24 +327,51 @@ def main():
     if args.traffic_run:
         r = fit_steps(g, 1, dh.FP64)
         ms = dh.copy_probe(1 << 30, 3)
-        log(f"traffic run: {r['iterations_executed']} iterations; 1 GiB copy {ms:.3f} ms")
+        gms = dh.gather_probe(1 << 30, 1 << 26, 3)
+        log(f"traffic run: {r['iterations_executed']} iterations; 1 GiB copy {ms:.3f} ms; "
+            f"2^26 random 8-byte gathers from 1 GiB {gms:.3f} ms")
         return
 
     fit_steps(g, args.warmup, dh.FP64)
     r64 = fit_steps(g, args.steps, dh.FP64)
     log(f"FP64: {r64['iterations_executed'] / args.steps:.0f} iterations, "
         f"{r64['seconds'] / args.steps * 1e3:.1f} ms/fit")
+    # the timed result against the reference's digest of the same fit
+    g.resident_download()
+    d64 = np.array(g.edge_dress)
+    with open(os.path.join(GOLDEN, "chung_lu_4m_fit.json")) as f:
+        gold = json.load(f)
+    equals_ref = sha256(d64) == gold["edge_dress_sha256"]
     get_res = get_queries(g, N, U, V)
     log(f"dress_get: {get_res['virtual_pairs_per_s']:.3g} virtual-edge queries/s, "
         f"{get_res['edge_queries_per_s']:.3g} edge queries/s")
     fit_steps(g, 1, dh.FP32_FP64)
     r32 = fit_steps(g, args.steps, dh.FP32_FP64)
-    log(f"FP32->FP64: {r32['seconds'] / args.steps * 1e3:.1f} ms/fit")
+    g.resident_download()
+    err32 = float(np.max(np.abs(np.array(g.edge_dress) - d64)))
+    log(f"FP32->FP64: {r32['seconds'] / args.steps * 1e3:.1f} ms/fit, max edge error {err32:.2e}")
 
     E = U.size
     value = E * r64["iterations_executed"] / r64["seconds"]
     value32 = E * r32["iterations_executed"] / r32["seconds"]
     ms_per_step = r64["seconds"] / args.steps * 1e3
+    iters_per_fit = r64["iterations_executed"] / args.steps
+
+    # the reference's entry point: dress_fit with the graph's host arrays in
+    # and out (values uploaded, results downloaded into g->edge_dress and
+    # g->vertex_dress), median of 3
+    walls = []
+    for _ in range(3):
+        g.set_edge_dress(np.ones(E))
+        t0 = time.perf_counter()
+        g.fit(100, 1e-6)
+        walls.append(time.perf_counter() - t0)
+    executed_c = dh.last_fit_stats().iterations_executed
+    fit_c_abi = {"workload": "dress_fit (C ABI, host arrays in and out) of the same graph",
+                 "edge_updates_per_s": E * executed_c / median(walls), "ms_per_fit": median(walls) * 1e3,
+                 "ms_runs": [w * 1e3 for w in walls],
+                 "equals_reference": sha256(g.edge_dress) == gold["edge_dress_sha256"]}
+    g.resident_reset()
 
     # dominant kernel: the edge-phase bin with the most time
     vms, ems = g.time_phases(reps=5)
@@ -255,12 +379,21 @@ def main():
     launch_bytes = info["bin_terms"][b] * B_ENTRY + info["edge_bins"][b] * B_EDGE
     achieved = launch_bytes / (ems[b] * 1e-3) / 1e9
     group = [1, 4, 16, 64, 256][b]
-    iters_per_fit = r64["iterations_executed"] / args.steps
     S, T, Nn = info["S"], info["T"], info["N"]
     b_iter = B_ENTRY * T + B_HALF * S + B_EDGE * E + B_VERTEX * Nn
     iter_gbs = b_iter / (ms_per_step * 1e-3 / iters_per_fit) / 1e9
-    copy_ms = dh.copy_probe(1 << 30, 5)
+    copy_ms = median([dh.copy_probe(1 << 30, 5) for _ in range(3)])
     copy_gbs = 2 * (1 << 30) / (copy_ms * 1e-3) / 1e9
+    gather_ms = dh.gather_probe(1 << 30, 1 << 26, 5)
+    kname = f"k_edge<{group},packed,double> (edge bin {b})"
+    bins = []
+    for i in range(5):
+        if info["edge_bins"][i] == 0:
+            continue
+        nb = info["bin_terms"][i] * B_ENTRY + info["edge_bins"][i] * B_EDGE
+        bins.append({"bin": i, "edges": info["edge_bins"][i], "entries": info["bin_terms"][i],
+                     "launch_ms": float(ems[i]), "algorithmic_bytes": int(nb),
+                     "frac": nb / (ems[i] * 1e-3) / 1e9 / HBM_PEAK_GBS})
 
     out = {
         "metric": METRIC,
@@ -279,39 +412,81 @@ def main():
                                "gamma=2.1 cap 20,000 seed 1, eps 1e-6, max 100 iterations",
                    "N": N, "E": int(E), "S": S, "T": T, "iterations": iters_per_fit,
                    "parallelism": "single GPU"},
+        "parity": {"fp64_edge_values_equal_reference_digest": equals_ref,
+                   "fp32_fp64_max_edge_err": err32},
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBS,
-                     "traffic": (TRAFFIC["bytes_per_launch"]
-                                 if TRAFFIC and TRAFFIC["kernel"] == f"k_edge<{group},packed,double> (edge bin {b})"
-                                 else None),
-                     "kernel": f"k_edge<{group},packed,double> (edge bin {b})",
+                     "traffic": TRAFFIC["bytes_per_launch"] if TRAFFIC["kernel"] == kname else None,
+                     "traffic_source": TRAFFIC["source"] if TRAFFIC["kernel"] == kname else None,
+                     "kernel": kname,
                      "launch_ms": float(ems[b]), "algorithmic_bytes": int(launch_bytes),
                      "iteration_algorithmic_GBs": iter_gbs, "iteration_frac": iter_gbs / HBM_PEAK_GBS,
-                     "copy_kernel_GBs": copy_gbs},
+                     "iteration_ms": ms_per_step / iters_per_fit,
+                     "edge_bins": bins,
+                     "copy_kernel_GBs": copy_gbs,
+                     "gather_probe_ns_per_8B_read": gather_ms * 1e6 / (1 << 26)},
+        "fit_c_abi": fit_c_abi,
         "fp32_fp64": {"value": value32, "unit": "edge-updates/s",
                       "ms_per_step": r32["seconds"] / args.steps * 1e3,
                       "fp32_iterations_per_fit": r32["fp32_iterations"] / args.steps,
-                      "speedup_vs_fp64": value32 / value},
+                      "speedup_vs_fp64": value32 / value, "max_edge_err": err32},
         "phase_ms": {"vertex": [float(x) for x in vms], "edge": [float(x) for x in ems]},
         "dress_get": get_res,
+        "er2000": er2000(dh, gen),
     }
+    del d64
 
     if not args.no_delta:
         graphs = gen.delta1_batch_graphs(512)
         res = {}
+        hists = {}
         for name, prec in (("fp64", dh.FP64), ("fp32_fp64", dh.FP32_FP64)):
             dh.lifted_batch(graphs[:8], k=1, precision=prec)            # warm-up
-            best = None
-            for _ in range(3):
+            runs = []
+            for _ in range(5):
                 t0 = time.perf_counter()
                 r = dh.lifted_batch(graphs, k=1, precision=prec)
-                wall = time.perf_counter() - t0
-                if best is None or wall < best[0]:
-                    best = (wall, r)
-            wall, r = best
-            res[name] = {"solves_per_s": r.subgraphs / wall, "device_solves_per_s": r.subgraphs / (r.device_ms * 1e-3),
+                runs.append((time.perf_counter() - t0, r.device_ms))
+            wall = median([w for w, _ in runs])
+            dev = median([d for _, d in runs])
+            hists[name] = [(np.array(v), np.array(c)) for v, c in r.histograms]
+            res[name] = {"solves_per_s": r.subgraphs / wall, "device_solves_per_s": r.subgraphs / (dev * 1e-3),
                          "subgraphs": int(r.subgraphs), "values": int(r.values), "fp64_reruns": int(r.fp64_reruns),
-                         "wall_ms": wall * 1e3, "device_ms": r.device_ms}
+                         "wall_ms": wall * 1e3, "device_ms": dev,
+                         "wall_ms_runs": [w * 1e3 for w, _ in runs]}
+            del r
+        with open(os.path.join(GOLDEN, "delta1_batch512.json")) as f:
+            gold3 = json.load(f)
+        res["fp64"]["equals_reference"] = all(
+            hashlib.sha256(np.ascontiguousarray(v).tobytes() + np.ascontiguousarray(c).tobytes()).hexdigest() ==
+            rec["sha256"] for (v, c), rec in zip(hists["fp64"], gold3))
+        res["fp32_fp64"]["max_edge_err"] = max(sorted_err(a[0], a[1], m[0], m[1])
+                                               for a, m in zip(hists["fp64"], hists["fp32_fp64"]))
+        del hists
+        # LDS roofline of the batched kernel: the subgraphs' work counted by
+        # the kernel in one more FP64 call (DRESS_HIP_LIFT_WORK=1; results
+        # unchanged), priced at the algorithmic bytes per iteration, over the
+        # median device time of the timed FP64 calls
+        os.environ["DRESS_HIP_LIFT_WORK"] = "1"
+        try:
+            w = dh.lifted_batch(graphs, k=1, precision=dh.FP64).work
+        finally:
+            del os.environ["DRESS_HIP_LIFT_WORK"]
+        wT, wS, wE, wN = w
+        lds_ms, lds_bytes = dh.lds_probe(4096)
+        lds_peak = lds_bytes / (lds_ms * 1e-3) / 1e9
+        alg = B_ENTRY * wT + B_HALF * wS + B_EDGE * wE + B_VERTEX * wN
+        ldsb = L_ENTRY * wT + L_HALF * wS + L_EDGE * wE + L_VERTEX * wN
+        dev_s = res["fp64"]["device_ms"] * 1e-3
+        res["lds_roofline"] = {
+            "kernel": "k_lifted (batched delta^1, FP64)",
+            "work": {"entry_iterations": wT, "half_edge_iterations": wS, "edge_iterations": wE,
+                     "vertex_iterations": wN},
+            "algorithmic_bytes": alg, "algorithmic_GBs": alg / dev_s / 1e9,
+            "algorithmic_vs_hbm_peak": alg / dev_s / 1e9 / HBM_PEAK_GBS,
+            "lds_bytes": ldsb, "lds_GBs": ldsb / dev_s / 1e9,
+            "lds_peak_GBs_measured": lds_peak, "lds_frac": ldsb / dev_s / 1e9 / lds_peak,
+            "edge_updates_per_s": wE / dev_s}
         # plain FP64 GPU path: one device fit per subgraph (sample of graphs)
         dh.set_lifted_mode(1)
         t0 = time.perf_counter()
@@ -327,6 +502,9 @@ def main():
         res["speedup_fp64_batched_vs_plain_fp64_gpu"] = res["fp64"]["solves_per_s"] / plain
         res["workload"] = "delta^1 of 512 random 6-regular graphs, n ~ U{16..128}, seeds 0..511, eps 1e-6"
         out["delta1"] = res
+        log(f"delta^1 batch: {res['fp64']['solves_per_s']:.3g} solves/s FP64 "
+            f"(wall {res['fp64']['wall_ms']:.1f} ms, device {res['fp64']['device_ms']:.2f} ms), "
+            f"LDS {res['lds_roofline']['lds_frac']:.2f} of {lds_peak:.0f} GB/s")
 
     if not args.no_delta:
         out["delta1_large"] = delta1_large(dh, gen)
@@ -335,66 +513,87 @@ def main():
         n4, U4, V4 = gen.regular_switch(256, 12, 7)
         c4 = {"workload": "delta^2 (32,640 subgraphs of 254 vertices) and nabla^1 (256 marked graphs) of a "
                           "random 12-regular graph, n=256 (switch chain, seed 7), eps 1e-6"}
+        kept = {}
         with dh.Graph(n4, U4, V4) as g4:
-            for name, prec in (("delta2_fp64", dh.FP64), ("delta2_fp32_fp64", dh.FP32_FP64)):
+            for name, kind, k, prec in (("delta2_fp64", "delta", 2, dh.FP64),
+                                        ("delta2_fp32_fp64", "delta", 2, dh.FP32_FP64),
+                                        ("nabla1_fp64", "nabla", 1, dh.FP64),
+                                        ("nabla1_fp32_fp64", "nabla", 1, dh.FP32_FP64)):
                 dh.set_precision(prec)
                 try:
-                    g4.lifted("delta", 2)                               # warm-up: the same call
-                    t0 = time.perf_counter()
-                    res2 = g4.lifted("delta", 2)
-                    wall = time.perf_counter() - t0
-                    st = dh.last_lifted_stats()
+                    g4.lifted(kind, k)                                  # warm-up: the same call
+                    walls = []
+                    for _ in range(3):
+                        t0 = time.perf_counter()
+                        res2 = g4.lifted(kind, k)
+                        walls.append(time.perf_counter() - t0)
+                        st = dh.last_lifted_stats()
+                        if len(walls) < 3:
+                            del res2
                 finally:
                     dh.set_precision(dh.FP64)
+                wall = median(walls)
                 num, nvals = res2[3], int(res2[1].sum())
-                # the 0.79 GB result goes before the next call allocates its own
+                kept[name] = (np.array(res2[0]), np.array(res2[1]))
                 del res2
                 c4[name] = {"solves_per_s": num / wall, "subgraphs": int(num), "values": nvals,
-                            "wall_ms": wall * 1e3, "device_ms": st["device_ms"], "fp64_reruns": st["fp64_reruns"]}
-            t0 = time.perf_counter()
-            _, _, _, num = g4.lifted("nabla", 1)
-            wall = time.perf_counter() - t0
-            c4["nabla1_fp64"] = {"solves_per_s": num / wall, "subgraphs": int(num), "wall_ms": wall * 1e3}
+                            "wall_ms": wall * 1e3, "wall_ms_runs": [x * 1e3 for x in walls],
+                            "device_ms": st["device_ms"], "fp64_reruns": st["fp64_reruns"]}
+        with open(os.path.join(GOLDEN, "regular12_n256_delta2.json")) as f:
+            gd2 = json.load(f)
+        h = hashlib.sha256()
+        h.update(np.ascontiguousarray(kept["delta2_fp64"][0]).tobytes())
+        h.update(np.ascontiguousarray(kept["delta2_fp64"][1]).tobytes())
+        c4["delta2_fp64"]["equals_reference"] = h.hexdigest() == gd2["sha256"]
+        c4["delta2_fp32_fp64"]["max_edge_err"] = sorted_err(*kept["delta2_fp64"], *kept["delta2_fp32_fp64"])
+        c4["nabla1_fp32_fp64"]["max_edge_err"] = sorted_err(*kept["nabla1_fp64"], *kept["nabla1_fp32_fp64"])
+        del kept
         out["regular12_n256"] = c4
 
     if args.serve_seconds > 0:
         from dress_hip import serve
         rep = serve.run_stream(duration=args.serve_seconds, rate=2000.0, seed=0, workers=8, max_batch=64,
-                               drain=args.serve_seconds)
+                               drain=10.0)
         out["serving"] = rep
         log(f"serving: {rep['completed']} of {rep['offered']['requests']} offered; "
             f"p50/p99 {rep['latency']['all']['p50_ms']:.0f}/{rep['latency']['all']['p99_ms']:.0f} ms")
 
     if not args.no_cpu_baseline:
-        cores = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
+        cpu = host_cpu()
+        cores = omp_threads(cpu)
         with dh.Graph(N, U, V, precompute_intercepts=True) as gi:
             csr = gi.csr()
-        cpu_v, dt, get_rate = cpu_baseline_fit(N, U, V, csr, iterations=2)
+        cpu_v, times, get_rate = cpu_baseline_fit(N, U, V, csr)
         del csr
         if cpu_v is not None:
             out["cpu_baseline"] = {"value": cpu_v, "unit": "edge-updates/s", "cores": cores, "kind": "reference",
-                                   "sample": f"2 of the {iters_per_fit:.0f} iterations of the same fit, libdress "
-                                             f"dress_fit_omp with its intercept index prebuilt; the 2 iterations "
-                                             f"took {dt:.1f} s"}
+                                   "host": cpu,
+                                   "sample": f"median of 5 runs of one of the {iters_per_fit:.0f} iterations of the "
+                                             f"same fit, libdress dress_fit_omp on {cores} OpenMP threads with its "
+                                             f"intercept index prebuilt; runs took "
+                                             f"{', '.join(f'{x:.2f}' for x in times)} s"}
         if get_rate is not None:
             out["dress_get"]["cpu_baseline"] = {"virtual_pairs_per_s": get_rate, "cores": 1, "kind": "reference",
                                                 "sample": f"the first {GET_SAMPLE:,} virtual-edge pairs, libdress "
-                                                          "dress_get on the graph after the 2 iterations above"}
+                                                          "dress_get on the graph after the runs above"}
+        sys.path.insert(0, os.path.join(ROOT, "oracle"))
+        import pyoracle
+        lib = pyoracle.Lib("ref") if pyoracle.available("ref") else pyoracle.Lib("oracle")
+        kind_of = "reference" if lib.which == "ref" else "port"
         if "regular12_n256" in out:
-            sys.path.insert(0, os.path.join(ROOT, "oracle"))
-            import pyoracle
-            lib4 = pyoracle.Lib("ref") if pyoracle.available("ref") else pyoracle.Lib("oracle")
-            t0 = time.perf_counter()
-            _, _, _, num = lib4.lifted("delta", n4, U4, V4, k=2, n_samples=1500, seed=1, use_omp=True)
-            dt = time.perf_counter() - t0
-            out["regular12_n256"]["cpu_baseline"] = {"solves_per_s": 1500 / dt, "cores": cores,
-                                              "kind": "reference" if lib4.which == "ref" else "port",
-                                              "sample": "1,500 sampled delta^2 subgraphs, dress_delta_fit_omp"}
+            v, rates = cpu_lifted_rate(lib, 5, lambda L: L.lifted("delta", n4, U4, V4, k=2, n_samples=300, seed=1,
+                                                                  use_omp=True)[3])
+            out["regular12_n256"]["cpu_baseline"] = {"solves_per_s": v, "cores": cores, "kind": kind_of,
+                                                     "runs": rates,
+                                                     "sample": "median of 5 runs of 300 sampled delta^2 subgraphs, "
+                                                               "dress_delta_fit_omp"}
         if "delta1" in out:
-            v, kind = cpu_baseline_delta1(graphs[:64])
-            out["delta1"]["cpu_baseline"] = {"solves_per_s": v, "cores": cores,
-                                            "kind": "reference" if kind == "ref" else "port",
-                                            "sample": "delta^1 of the first 64 graphs, dress_delta_fit_omp"}
+            def d1(L):
+                return sum(L.lifted("delta", n, Ug, Vg, k=1, use_omp=True)[3] for n, Ug, Vg in graphs[:64])
+            v, rates = cpu_lifted_rate(lib, 5, d1)
+            out["delta1"]["cpu_baseline"] = {"solves_per_s": v, "cores": cores, "kind": kind_of, "runs": rates,
+                                             "sample": "median of 5 runs of delta^1 of the first 64 graphs, "
+                                                       "dress_delta_fit_omp"}
     print(json.dumps(out), flush=True)
 
 

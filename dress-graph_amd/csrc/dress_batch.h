@@ -60,6 +60,9 @@ struct LiftArgs {
     int64_t qcap;
     double qtol;                   // values within qtol of a bin edge are boundary values
     const int *parent_graph;       // graph id of each parent (for the key)
+    // work counters (or null): sums over items of executed iterations x the
+    // subgraph's index entries, half-edges, edges and vertices
+    unsigned long long *work;
 };
 
 constexpr unsigned long long QEMPTY = ~0ull;

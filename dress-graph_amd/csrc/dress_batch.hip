@@ -123,12 +123,38 @@ __global__ void __launch_bounds__(B) k_lifted(LiftArgs a)
             else { dA[e] = live ? 1.0 : 0.0; dB[e] = 0.0; }
         }
         __syncthreads();
+        // the subgraph's own size (live entries, half-edges, edges,
+        // vertices) for the work counters; live edges hold 1 here, dead 0
+        unsigned long long w_t = 0, w_s = 0, w_e = 0, w_n = 0;
+        if (a.work) {
+            for (int u = threadIdx.x; u < N; u += B) {
+                if (!alive[u]) continue;
+                w_n++;
+                for (int i = row_off[u]; i < row_off[u + 1]; i++) {
+                    const int e = adj_e[i];
+                    w_s += (fp32 ? fA[e] != 0.0f : dA[e] != 0.0) ? 1 : 0;
+                }
+            }
+            for (int e = threadIdx.x; e < E; e += B) {
+                if (!(fp32 ? fA[e] != 0.0f : dA[e] != 0.0)) continue;
+                w_e++;
+                for (int q = ioff[e]; q < ioff[e + 1]; q++) {
+                    const uint32_t pr = pairs[q];
+                    const int ea = (int)(pr & 0xffffu), eb = (int)(pr >> 16);
+                    const bool la = fp32 ? fA[ea] != 0.0f : dA[ea] != 0.0;
+                    const bool lb = fp32 ? fA[eb] != 0.0f : dA[eb] != 0.0;
+                    w_t += (la && lb) ? 1 : 0;
+                }
+            }
+        }
+        int executed = 0;       // iterations run
         const double thr = 3000.0 * a.eps;
         int cur = 0;            // 0: A holds the current values, 1: B
         int iters = a.max_it;   // reference convention result
         double last = 0.0;
         int ambiguous = 0;
         for (int iter = 0; iter < a.max_it; iter++) {
+            executed = iter + 1;
             // ---- vertex phase
             for (int u = threadIdx.x; u < N; u += B) {
                 if (!alive[u]) continue;
@@ -289,6 +315,22 @@ __global__ void __launch_bounds__(B) k_lifted(LiftArgs a)
             // vertex_dress of dress.c:555-574: norms of the last vertex phase
             for (int u = threadIdx.x; u < N; u += B)
                 a.vnorm[I.vn_ofs + u] = alive[u] ? vd[u] : __longlong_as_double(0x7ff8000000000000ll);
+        }
+        if (a.work) {
+            // wave sums, one atomic per wave and counter
+            for (int o = 32; o > 0; o >>= 1) {
+                w_t += __shfl_xor(w_t, o);
+                w_s += __shfl_xor(w_s, o);
+                w_e += __shfl_xor(w_e, o);
+                w_n += __shfl_xor(w_n, o);
+            }
+            if ((threadIdx.x & 63) == 0) {
+                const unsigned long long x = (unsigned long long)executed;
+                if (w_t) atomicAdd(&a.work[0], w_t * x);
+                if (w_s) atomicAdd(&a.work[1], w_s * x);
+                if (w_e) atomicAdd(&a.work[2], w_e * x);
+                if (w_n) atomicAdd(&a.work[3], w_n * x);
+            }
         }
         if (threadIdx.x == 0) {
             if (a.iters) a.iters[it] = iters;

@@ -797,12 +797,73 @@ int fit_impl(p_dress_graph_t g, int max_iterations, double epsilon, int precisio
     return 0;
 }
 
-// plain streaming copy: the calibration workload for HBM byte counters
-__global__ void k_stream_copy(const double2 *__restrict__ a, double2 *__restrict__ b, int64_t n)
+// plain streaming copy: the calibration workload for HBM byte counters and
+// the practical bandwidth ceiling.  Each thread keeps four 16-byte loads in
+// flight (non-temporal: the data is touched once) before its stores.
+typedef double dvec2 __attribute__((ext_vector_type(2)));
+__global__ void __launch_bounds__(256) k_stream_copy(const double2 *__restrict__ a_, double2 *__restrict__ b_,
+                                                     int64_t n)
 {
-    for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n;
-         i += (int64_t)gridDim.x * blockDim.x)
-        b[i] = a[i];
+    const dvec2 *__restrict__ a = reinterpret_cast<const dvec2 *>(a_);
+    dvec2 *__restrict__ b = reinterpret_cast<dvec2 *>(b_);
+    const int64_t stride = (int64_t)gridDim.x * blockDim.x;
+    int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+    for (; i + 3 * stride < n; i += 4 * stride) {
+        const dvec2 x0 = __builtin_nontemporal_load(a + i);
+        const dvec2 x1 = __builtin_nontemporal_load(a + i + stride);
+        const dvec2 x2 = __builtin_nontemporal_load(a + i + 2 * stride);
+        const dvec2 x3 = __builtin_nontemporal_load(a + i + 3 * stride);
+        __builtin_nontemporal_store(x0, b + i);
+        __builtin_nontemporal_store(x1, b + i + stride);
+        __builtin_nontemporal_store(x2, b + i + 2 * stride);
+        __builtin_nontemporal_store(x3, b + i + 3 * stride);
+    }
+    for (; i < n; i += stride) __builtin_nontemporal_store(__builtin_nontemporal_load(a + i), b + i);
+}
+
+// random 8-byte gathers: out[i] = src[idx[i]] -- the access pattern of the
+// edge kernels' common-neighbour reads, for calibrating the byte counters
+// on scattered requests
+__global__ void __launch_bounds__(256) k_gather_probe(const double *__restrict__ src, const uint32_t *__restrict__ idx,
+                                                      double *__restrict__ out, int64_t n)
+{
+    for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x)
+        out[i] = src[__builtin_nontemporal_load(idx + i)];
+}
+
+__global__ void k_fill_random_idx(uint32_t *idx, int64_t n, uint32_t range, uint64_t seed)
+{
+    for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) {
+        uint64_t z = seed + (uint64_t)i * 0x9E3779B97F4A7C15ull;       // splitmix64
+        z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ull;
+        z = (z ^ (z >> 27)) * 0x94D049BB133111EBull;
+        z ^= z >> 31;
+        idx[i] = (uint32_t)(z % range);
+    }
+}
+
+// LDS read bandwidth: every thread reads 16-byte words of a 32 KiB buffer,
+// consecutive lanes at consecutive words (no bank conflicts), rotating the
+// start each round so no read can be hoisted
+constexpr int LDS_PROBE_TPB = 256;
+constexpr int LDS_PROBE_WORDS = 2048;        // 32 KiB of double2
+__global__ void __launch_bounds__(LDS_PROBE_TPB) k_lds_probe(int rounds, double *out)
+{
+    __shared__ double2 buf[LDS_PROBE_WORDS];
+    for (int i = threadIdx.x; i < LDS_PROBE_WORDS; i += LDS_PROBE_TPB)
+        buf[i] = make_double2((double)i, (double)(i ^ 7));
+    __syncthreads();
+    double sx = 0.0, sy = 0.0;
+    for (int r = 0; r < rounds; r++) {
+        const int base = (threadIdx.x + r * 64) & (LDS_PROBE_WORDS - 1);
+#pragma unroll
+        for (int j = 0; j < 8; j++) {
+            const double2 v = buf[(base + j * LDS_PROBE_TPB) & (LDS_PROBE_WORDS - 1)];
+            sx += v.x;
+            sy += v.y;
+        }
+    }
+    if (sx + sy == -1.0) out[blockIdx.x] = sx;      // never true: keeps the reads
 }
 
 }  // namespace
@@ -1045,6 +1106,86 @@ int dress_hip_copy_probe(int64_t n_bytes, int reps, double *ms)
         return -1;
     }
     *ms = t / reps;
+    return 0;
+}
+
+int dress_hip_gather_probe(int64_t src_bytes, int64_t n_gathers, int reps, double *ms)
+{
+    clear_error();
+    if (src_bytes < 8 || src_bytes / 8 > (int64_t)UINT32_MAX || n_gathers < 1 || reps < 1 || !ms) {
+        set_error("dress_hip_gather_probe: bad arguments");
+        return -1;
+    }
+    double *src = nullptr, *out = nullptr;
+    uint32_t *idx = nullptr;
+    if (!dalloc(&src, (size_t)(src_bytes / 8), "gather probe source") ||
+        !dalloc(&idx, (size_t)n_gathers, "gather probe indices") || !dalloc(&out, (size_t)n_gathers, "gather probe output")) {
+        dfree(src); dfree(idx);
+        return -1;
+    }
+    (void)hipMemset(src, 0, (size_t)src_bytes);
+    const int grid = num_cus() * 16;
+    hipLaunchKernelGGL(k_fill_random_idx, dim3(grid), dim3(256), 0, 0, idx, n_gathers, (uint32_t)(src_bytes / 8),
+                       (uint64_t)12345);
+    hipEvent_t e0, e1;
+    (void)hipEventCreate(&e0);
+    (void)hipEventCreate(&e1);
+    hipLaunchKernelGGL(k_gather_probe, dim3(grid), dim3(256), 0, 0, (const double *)src, (const uint32_t *)idx, out,
+                       n_gathers);
+    (void)hipEventRecord(e0, 0);
+    for (int r = 0; r < reps; r++)
+        hipLaunchKernelGGL(k_gather_probe, dim3(grid), dim3(256), 0, 0, (const double *)src, (const uint32_t *)idx,
+                           out, n_gathers);
+    (void)hipEventRecord(e1, 0);
+    hipError_t err = hipEventSynchronize(e1);
+    float t = 0.f;
+    (void)hipEventElapsedTime(&t, e0, e1);
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    dfree(src); dfree(idx); dfree(out);
+    if (err != hipSuccess) {
+        set_error(std::string("dress_hip_gather_probe: ") + hipGetErrorString(err));
+        return -1;
+    }
+    *ms = t / reps;
+    return 0;
+}
+
+int dress_hip_lds_probe(int rounds, double *ms, double *bytes)
+{
+    clear_error();
+    if (rounds < 1 || !ms || !bytes) {
+        set_error("dress_hip_lds_probe: bad arguments");
+        return -1;
+    }
+    double *out = nullptr;
+    int per_cu = 0;
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_lds_probe, LDS_PROBE_TPB, 0) != hipSuccess ||
+        per_cu < 1) {
+        (void)hipGetLastError();
+        per_cu = 4;
+    }
+    const int grid = num_cus() * per_cu;
+    if (!dalloc(&out, (size_t)grid, "lds probe output")) return -1;
+    hipEvent_t e0, e1;
+    (void)hipEventCreate(&e0);
+    (void)hipEventCreate(&e1);
+    hipLaunchKernelGGL(k_lds_probe, dim3(grid), dim3(LDS_PROBE_TPB), 0, 0, rounds, out);
+    (void)hipEventRecord(e0, 0);
+    hipLaunchKernelGGL(k_lds_probe, dim3(grid), dim3(LDS_PROBE_TPB), 0, 0, rounds, out);
+    (void)hipEventRecord(e1, 0);
+    hipError_t err = hipEventSynchronize(e1);
+    float t = 0.f;
+    (void)hipEventElapsedTime(&t, e0, e1);
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    dfree(out);
+    if (err != hipSuccess) {
+        set_error(std::string("dress_hip_lds_probe: ") + hipGetErrorString(err));
+        return -1;
+    }
+    *ms = t;
+    *bytes = (double)grid * LDS_PROBE_TPB * (double)rounds * 8.0 * 16.0;
     return 0;
 }
 

@@ -981,7 +981,17 @@ struct LiftOut {
     int reruns = 0;
     double device_ms = 0.0;
     int block = 0;
+    // with DRESS_HIP_LIFT_WORK=1: executed iterations x the subgraphs' index
+    // entries, half-edges, edges and vertices, summed over items (the work
+    // counted by k_lifted; a measurement aid, off by default)
+    unsigned long long work[4] = {0, 0, 0, 0};
 };
+
+bool lift_work_counting()
+{
+    const char *e = getenv("DRESS_HIP_LIFT_WORK");
+    return e && !strcmp(e, "1");
+}
 
 // per-item outputs of whole-graph fits run as k = 0 items
 struct FitOutputs {
@@ -1143,6 +1153,10 @@ bool run_items(const std::vector<const HostParent *> &ps, const std::vector<int>
     bool ok = pk.run(&D.blk, s) && dalloc(&d_queue, 1, "work queue") && dalloc(&d_flags, (size_t)n, "item flags") &&
               dalloc(&out.d_vals, (size_t)out.val_ofs.back() + 1, "subgraph values");
     if (ok && fo) ok = dalloc(&d_iters, (size_t)n, "item iterations") && dalloc(&d_deltas, (size_t)n, "item deltas");
+    unsigned long long *d_work = nullptr;
+    if (ok && lift_work_counting())
+        ok = dalloc(&d_work, 4, "work counters") &&
+             hipMemsetAsync(d_work, 0, 4 * sizeof(unsigned long long), s) == hipSuccess;
     hipEvent_t e0 = nullptr, e1 = nullptr;
     if (ok) ok = hipEventCreate(&e0) == hipSuccess && hipEventCreate(&e1) == hipSuccess;
     if (pt.on) { hipStreamSynchronize(s); pt.mark("uploads+allocs"); }
@@ -1173,6 +1187,7 @@ bool run_items(const std::vector<const HostParent *> &ps, const std::vector<int>
         a.qcap = qcap;
         a.qtol = qtol;
         a.parent_graph = D.parent_graph;
+        a.work = d_work;
         if (fo) {
             a.vnorm = fo->d_vnorm;
             a.iters = d_iters;
@@ -1300,10 +1315,13 @@ bool run_items(const std::vector<const HostParent *> &ps, const std::vector<int>
                  hipSuccess &&
              hipStreamSynchronize(s) == hipSuccess;
     }
+    if (ok && d_work)
+        ok = hipMemcpyAsync(out.work, d_work, sizeof(out.work), hipMemcpyDeviceToHost, s) == hipSuccess &&
+             hipStreamSynchronize(s) == hipSuccess;
     if (e0) (void)hipEventDestroy(e0);
     if (e1) (void)hipEventDestroy(e1);
     pt.mark("reruns");
-    dfree(d_queue); dfree(d_flags); dfree(d_iters); dfree(d_deltas);
+    dfree(d_queue); dfree(d_flags); dfree(d_iters); dfree(d_deltas); dfree(d_work);
     D.release();                   // items, order, vertices and marks share its block
     pt.mark("free");
     if (!ok) {
@@ -2072,15 +2090,17 @@ int dress_hip_lifted_batch(int kind, int k, int n_graphs, const int *N, const in
     for (auto &P : parents) ps.push_back(&P);
     int64_t total_vals = 0;
     for (int64_t x : item_live) total_vals += x;
+    // the floor(d / eps) tables only on request (qtol >= 0)
+    const bool quant = qtol >= 0.0;
     int64_t qcap = 1024;
-    while (qcap < 2 * total_vals) qcap <<= 1;
+    while (quant && qcap < 2 * total_vals) qcap <<= 1;
     unsigned long long *qk = nullptr, *qc = nullptr;
     LiftOut lo;
-    bool ok = dalloc(&qk, (size_t)qcap, "quantised table") && dalloc(&qc, (size_t)qcap, "quantised table");
+    bool ok = !quant || (dalloc(&qk, (size_t)qcap, "quantised table") && dalloc(&qc, (size_t)qcap, "quantised table"));
     if (ok) {
-        hipLaunchKernelGGL(k_qtable_clear, dim3(grid_for(qcap, 4096)), dim3(TPB), 0, s, qk, qc, qcap);
-        ok = run_items(ps, graph_of, item_parent, item_verts, item_live, J, nullptr, 0, ms_rows, s, lo, qk, qc, qcap,
-                       qtol);
+        if (quant) hipLaunchKernelGGL(k_qtable_clear, dim3(grid_for(qcap, 4096)), dim3(TPB), 0, s, qk, qc, qcap);
+        ok = run_items(ps, graph_of, item_parent, item_verts, item_live, J, nullptr, 0, ms_rows, s, lo, qk, qc,
+                       quant ? qcap : 0, quant ? qtol : 0.0);
     }
     pt.mark("run_items");
     // exact histogram per graph: items of a graph are contiguous
@@ -2105,7 +2125,7 @@ int dress_hip_lifted_batch(int kind, int k, int n_graphs, const int *N, const in
     }
     int64_t nq = 0, nb = 0;
     int64_t *qk2 = nullptr, *qc2 = nullptr, *bk2 = nullptr, *bc2 = nullptr, *qofs = nullptr, *bofs = nullptr;
-    if (ok) ok = qtable_split(qk, qc, qcap, n_graphs, s, &qk2, &qc2, &bk2, &bc2, &qofs, &bofs, &nq, &nb);
+    if (ok && quant) ok = qtable_split(qk, qc, qcap, n_graphs, s, &qk2, &qc2, &bk2, &bc2, &qofs, &bofs, &nq, &nb);
     pt.mark("qtable_split");
     if (pt.on)
         fprintf(stderr, "dress_hip_lifted_batch: %lld values, %lld exact pairs, %lld bins, %lld boundary bins\n",
@@ -2127,6 +2147,8 @@ int dress_hip_lifted_batch(int kind, int k, int n_graphs, const int *N, const in
     }
     if (ok) {
         for (int gi = 0; gi <= n_graphs; gi++) out->hist_offset[gi] = n_graphs > 0 ? run_ofs[(size_t)gi] : 0;
+        if (!quant)
+            for (int gi = 0; gi <= n_graphs; gi++) out->qhist_offset[gi] = out->qbound_offset[gi] = 0;
         // (a copy stream taking the exact histogram down during the
         // quantised assembly measured slower: 7.3-8.6 ms against 5.2 ms)
         auto d2h = [&](void *dst, const void *src, size_t bytes) {
@@ -2139,8 +2161,8 @@ int dress_hip_lifted_batch(int kind, int k, int n_graphs, const int *N, const in
              d2h(out->qhist_count, qc2, (size_t)nq * sizeof(int64_t)) &&
              d2h(out->qbound_key, bk2, (size_t)nb * sizeof(int64_t)) &&
              d2h(out->qbound_count, bc2, (size_t)nb * sizeof(int64_t)) &&
-             d2h(out->qhist_offset, qofs, ((size_t)n_graphs + 1) * sizeof(int64_t)) &&
-             d2h(out->qbound_offset, bofs, ((size_t)n_graphs + 1) * sizeof(int64_t)) &&
+             (!quant || d2h(out->qhist_offset, qofs, ((size_t)n_graphs + 1) * sizeof(int64_t))) &&
+             (!quant || d2h(out->qbound_offset, bofs, ((size_t)n_graphs + 1) * sizeof(int64_t))) &&
              hipStreamSynchronize(s) == hipSuccess;
         pt2.mark("download");
         if (!ok && !has_error()) set_error("batch result download failed");
@@ -2156,6 +2178,7 @@ int dress_hip_lifted_batch(int kind, int k, int n_graphs, const int *N, const in
     out->values = lo.val_ofs.back();
     out->fp64_reruns = lo.reruns;
     out->device_ms = lo.device_ms;
+    for (int i = 0; i < 4; i++) out->work[i] = (int64_t)lo.work[i];
     pt.mark("assemble");
     out->total_ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
     record_lifted(lo.reruns, lo.device_ms, out->subgraphs, out->values);

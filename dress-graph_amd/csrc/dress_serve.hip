@@ -266,7 +266,7 @@ void run_lifted_batch(const std::vector<ServeReq *> &qs, std::vector<dress_hip_r
     int kind = q0.r.kind == DRESS_HIP_REQ_DELTA ? 0 : 1;
     int rc = dress_hip_lifted_batch(kind, q0.r.k, n, N.data(), off.data(), U.data(), V.data(),
                                     any_w ? W.data() : nullptr, q0.r.variant, q0.r.max_iterations, q0.r.epsilon,
-                                    q0.r.precision, 0.0, &r);
+                                    q0.r.precision, -1.0, &r);
     if (rc != 0) {
         std::string msg = std::string("batched lifted fit failed: ") + dress_hip_last_error();
         for (auto &o : outs) fail(o, msg);

@@ -101,6 +101,8 @@ class LiftedBatchResult:
     fp64_reruns: int
     device_ms: float
     total_ms: float
+    work: Tuple[int, int, int, int] = (0, 0, 0, 0)   # DRESS_HIP_LIFT_WORK=1: iterations x entries,
+                                                     # half-edges, edges, vertices
 
 
 def copy_probe(n_bytes=1 << 30, reps=5):
@@ -109,6 +111,23 @@ def copy_probe(n_bytes=1 << 30, reps=5):
     _lib.check(_lib.lib().dress_hip_copy_probe(int(n_bytes), int(reps), ctypes.byref(ms)) == 0,
                "dress_hip_copy_probe")
     return ms.value
+
+
+def gather_probe(src_bytes=1 << 30, n_gathers=1 << 26, reps=5):
+    """Mean ms of n_gathers random 8-byte device reads from src_bytes."""
+    ms = ctypes.c_double()
+    _lib.check(_lib.lib().dress_hip_gather_probe(int(src_bytes), int(n_gathers), int(reps), ctypes.byref(ms)) == 0,
+               "dress_hip_gather_probe")
+    return ms.value
+
+
+def lds_probe(rounds=4096):
+    """(ms, bytes) of one LDS read kernel over the whole device."""
+    ms = ctypes.c_double()
+    nb = ctypes.c_double()
+    _lib.check(_lib.lib().dress_hip_lds_probe(int(rounds), ctypes.byref(ms), ctypes.byref(nb)) == 0,
+               "dress_hip_lds_probe")
+    return ms.value, nb.value
 
 
 def set_lifted_mode(mode):
@@ -141,11 +160,14 @@ def last_lifted_stats():
 
 
 def lifted_batch(graphs, k=1, kind="delta", variant=UNDIRECTED, max_iterations=100, epsilon=1e-6,
-                 precision=FP64, qtol=1e-9, weights=None):
+                 precision=FP64, qtol=None, weights=None):
     """delta^k / nabla^k of many graphs in one batched launch.
 
     ``graphs`` is a list of ``(n, U, V)``; ``weights`` an optional list of
-    per-graph weight arrays.  Returns a :class:`LiftedBatchResult`."""
+    per-graph weight arrays.  ``qtol`` (e.g. 1e-9) asks for the
+    floor(d/epsilon) tables as well, with values within ``qtol`` of a bin
+    edge counted apart; ``None`` leaves them empty.  Returns a
+    :class:`LiftedBatchResult`."""
     L = _lib.lib()
     n = len(graphs)
     Ns = np.array([g[0] for g in graphs], dtype=np.int32)
@@ -165,7 +187,7 @@ def lifted_batch(graphs, k=1, kind="delta", variant=UNDIRECTED, max_iterations=1
                                   U.ctypes.data_as(c_int_p), V.ctypes.data_as(c_int_p),
                                   W.ctypes.data_as(c_dbl_p) if W is not None else c_dbl_p(),
                                   int(variant), int(max_iterations), float(epsilon), int(precision),
-                                  float(qtol), ctypes.byref(r))
+                                  -1.0 if qtol is None else float(qtol), ctypes.byref(r))
     _lib.check(rc == 0, "dress_hip_lifted_batch")
     # the result arrays stay where the library wrote them: numpy views own
     # the block and release it (dress_hip_lifted_batch_free) with the last view
@@ -178,7 +200,8 @@ def lifted_batch(graphs, k=1, kind="delta", variant=UNDIRECTED, max_iterations=1
     hists = split(r.hist_offset, r.hist_value, r.hist_count, "<f8")
     quant = split(r.qhist_offset, r.qhist_key, r.qhist_count, "<i8")
     bound = split(r.qbound_offset, r.qbound_key, r.qbound_count, "<i8")
-    return LiftedBatchResult(hists, quant, bound, r.subgraphs, r.values, r.fp64_reruns, r.device_ms, r.total_ms)
+    return LiftedBatchResult(hists, quant, bound, r.subgraphs, r.values, r.fp64_reruns, r.device_ms, r.total_ms,
+                             tuple(int(x) for x in r.work))
 
 
 class _PerGraph(Sequence):

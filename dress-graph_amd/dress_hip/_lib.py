@@ -63,7 +63,7 @@ class LiftedBatch(ctypes.Structure):
                 ("qbound_count", ctypes.POINTER(ctypes.c_int64)),
                 ("subgraphs", ctypes.c_int64), ("values", ctypes.c_int64),
                 ("fp64_reruns", ctypes.c_int), ("device_ms", ctypes.c_double),
-                ("total_ms", ctypes.c_double)]
+                ("total_ms", ctypes.c_double), ("work", ctypes.c_int64 * 4)]
 
 
 class FitBatch(ctypes.Structure):
@@ -148,6 +148,10 @@ def _bind(L):
     L.dress_hip_time_phases.argtypes = [PG, ctypes.c_int, c_dbl_p]
     L.dress_hip_copy_probe.restype = ctypes.c_int
     L.dress_hip_copy_probe.argtypes = [ctypes.c_int64, ctypes.c_int, c_dbl_p]
+    L.dress_hip_gather_probe.restype = ctypes.c_int
+    L.dress_hip_gather_probe.argtypes = [ctypes.c_int64, ctypes.c_int64, ctypes.c_int, c_dbl_p]
+    L.dress_hip_lds_probe.restype = ctypes.c_int
+    L.dress_hip_lds_probe.argtypes = [ctypes.c_int, c_dbl_p, c_dbl_p]
     L.dress_hip_resident_download.restype = ctypes.c_int
     L.dress_hip_resident_download.argtypes = [PG]
     L.dress_hip_set_lifted_mode.argtypes = [ctypes.c_int]

@@ -186,6 +186,13 @@ int dress_hip_time_phases(p_dress_graph_t g, int reps, double *ms);
  * bandwidth, and the reference reading for hardware byte counters.  No
  * counterpart in libdress.  Returns 0 or -1. */
 int dress_hip_copy_probe(int64_t n_bytes, int reps, double *ms);
+/* Measurement probes (no counterpart in the reference).  Mean ms of a
+ * kernel of n_gathers random 8-byte reads from a src_bytes array (the
+ * access pattern of the edge kernels' gathers, for calibrating byte
+ * counters); and ms of one launch of an LDS read kernel filling the device,
+ * with the LDS bytes it reads in *bytes. */
+int dress_hip_gather_probe(int64_t src_bytes, int64_t n_gathers, int reps, double *ms);
+int dress_hip_lds_probe(int rounds, double *ms, double *bytes);
 
 /* dress_get for many pairs at once (libdress/src/dress.c:651-739 per
  * query), against g's current edge_dress / vertex_dress.  out[i] receives
@@ -248,10 +255,11 @@ void dress_hip_last_lifted_stats(dress_hip_lifted_stats_t *out);
  * `variant`.  Every subgraph of every graph is fitted (no sampling or
  * striding).  Per graph the result holds
  *   - the exact histogram: distinct values ascending with their counts;
- *   - the quantised histogram: counts of floor(d / epsilon), accumulated on
- *     the device in integer atomics, for values farther than qtol from a bin
- *     edge; values within qtol of an edge are counted separately in
- *     qbound (their bin depends on the last bits of d).
+ *   - on request (qtol >= 0; qtol < 0 leaves them empty): the quantised
+ *     histogram, counts of floor(d / epsilon), accumulated on the device in
+ *     integer atomics, for values farther than qtol from a bin edge; values
+ *     within qtol of an edge are counted separately in qbound (their bin
+ *     depends on the last bits of d).
  * Large arrays are pinned host memory kept in a pool for later calls;
  * release every array only with dress_hip_lifted_batch_free.
  * Returns 0, or -1 (see dress_hip_last_error). */
@@ -271,6 +279,10 @@ typedef struct {
     int      fp64_reruns;
     double   device_ms;           /* batched launch(es)                       */
     double   total_ms;            /* whole call, wall clock                   */
+    int64_t  work[4];             /* with DRESS_HIP_LIFT_WORK=1: sums over the
+                                     subgraphs of iterations run x their index
+                                     entries, half-edges, edges, vertices;
+                                     else zeros                               */
 } dress_hip_lifted_batch_t;
 int dress_hip_lifted_batch(int kind, int k, int n_graphs, const int *N, const int64_t *edge_offset,
                            const int *U, const int *V, const double *W, dress_variant_t variant,

@@ -216,3 +216,27 @@ def test_block_size_does_not_change_results(gpu, gen, monkeypatch):
         else:
             for (v0, c0), (v1, c1) in zip(ref, cur):
                 assert np.array_equal(v0, v1) and np.array_equal(c0, c1)
+
+
+def test_quantised_tables_on_request(gpu, gen, monkeypatch):
+    """Without qtol the floor(d/eps) tables stay empty and the exact
+    histograms are those of a call with them; DRESS_HIP_LIFT_WORK=1 counts
+    the subgraphs' work (iterations x live entries, half-edges, edges,
+    vertices) without changing results."""
+    graphs = gen.delta1_batch_graphs(20)
+    a = gpu.lifted_batch(graphs, k=1)
+    b = gpu.lifted_batch(graphs, k=1, qtol=1e-9)
+    assert all(len(k) == 0 for k, _ in a.quantised) and all(len(k) == 0 for k, _ in a.boundary)
+    assert sum(int(c.sum()) for _, c in b.quantised) + sum(int(c.sum()) for _, c in b.boundary) == b.values
+    for (v0, c0), (v1, c1) in zip(a.histograms, b.histograms):
+        assert np.array_equal(v0, v1) and np.array_equal(c0, c1)
+    assert a.work == (0, 0, 0, 0)
+    monkeypatch.setenv("DRESS_HIP_LIFT_WORK", "1")
+    w = gpu.lifted_batch(graphs, k=1)
+    for (v0, c0), (v1, c1) in zip(a.histograms, w.histograms):
+        assert np.array_equal(v0, v1) and np.array_equal(c0, c1)
+    T, S, E, N = w.work
+    # every subgraph of a 6-regular graph minus one vertex: n-1 vertices,
+    # 3n-6 edges, 2(3n-6) half-edges, each times its iterations (>= 1)
+    lo_e = sum(n * (3 * n - 6) for n, _, _ in graphs)
+    assert E >= lo_e and S == 2 * E and N >= sum(n * (n - 1) for n, _, _ in graphs) and T >= 0
