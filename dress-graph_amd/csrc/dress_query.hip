@@ -15,6 +15,8 @@
 
 #include <cmath>
 #include <cstring>
+#include <mutex>
+#include <string>
 #include <vector>
 
 namespace dh {
@@ -262,9 +264,114 @@ bool get_batch(p_dress_graph_t g, int n, const int *u, const int *v, const doubl
     return ok;
 }
 
+// ---------------------------------------------------------------------------
+// dress_get batches without allocation (dress_hip_get_batch_resident_noalloc):
+// the queries and answers are staged in this file-scope device array, whose
+// device address is looked up once per device (hipGetSymbolAddress); the
+// array's name is a host-side stand-in and is never passed to the device.
+// Every call on a device shares that device's copy, so calls that overlap on
+// one device race.
+
+constexpr int NOALLOC_MAX_QUERIES = 65536;
+// per query: u, v (int32), edge weight and answer (double)
+constexpr size_t NOALLOC_SCRATCH_BYTES = (size_t)NOALLOC_MAX_QUERIES * (2 * sizeof(int) + 2 * sizeof(double));
+static __device__ __align__(256) unsigned char scratch[NOALLOC_SCRATCH_BYTES];
+
+namespace {
+constexpr int NOALLOC_MAX_DEVICES = 64;
+std::mutex g_scratch_mu;
+unsigned char *g_scratch_ptr[NOALLOC_MAX_DEVICES] = {nullptr};
+
+// scratch's device address on the current device (nullptr with the error
+// set when it cannot be had)
+unsigned char *scratch_on_device()
+{
+    int dev = 0;
+    if (hipGetDevice(&dev) != hipSuccess || dev < 0 || dev >= NOALLOC_MAX_DEVICES) {
+        (void)hipGetLastError();
+        set_error("dress_hip_get_batch_resident_noalloc: no current device");
+        return nullptr;
+    }
+    std::lock_guard<std::mutex> lk(g_scratch_mu);
+    if (!g_scratch_ptr[dev]) {
+        void *p = nullptr;
+        const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+        if (e != hipSuccess || !p) {
+            (void)hipGetLastError();
+            set_error(std::string("dress_hip_get_batch_resident_noalloc: hipGetSymbolAddress: ") +
+                      hipGetErrorString(e));
+            return nullptr;
+        }
+        g_scratch_ptr[dev] = (unsigned char *)p;
+    }
+    return g_scratch_ptr[dev];
+}
+}  // namespace
+
 }  // namespace dh
 
 extern "C" {
+
+int dress_hip_get_batch_resident_noalloc(p_dress_graph_t g, int n, const int *u, const int *v,
+                                         const double *edge_weight, int max_iterations, double epsilon, double *out)
+{
+    using namespace dh;
+    clear_error();
+    if (!g || n < 0 || (n > 0 && (!u || !v || !edge_weight || !out))) {
+        set_error("dress_hip_get_batch_resident_noalloc: bad arguments");
+        return -1;
+    }
+    if (n > NOALLOC_MAX_QUERIES) {
+        set_error("dress_hip_get_batch_resident_noalloc: " + std::to_string(n) + " queries exceed the maximum of " +
+                  std::to_string(NOALLOC_MAX_QUERIES) + " per call");
+        return -1;
+    }
+    for (int i = 0; i < n; i++) {
+        if (u[i] < 0 || u[i] >= g->N || v[i] < 0 || v[i] >= g->N) {
+            set_error("dress_get: vertex out of range");
+            return -1;
+        }
+    }
+    DevGraph *G = lookup(g);
+    if (!G || !G->fit_state_valid) {
+        set_error("dress_hip_get_batch_resident_noalloc: no fitted state on the device (run dress_fit on the graph "
+                  "first)");
+        return -1;
+    }
+    if (!G->ref_row_off || !G->ref_adj_x || !G->ref_adj_e) {
+        set_error("dress_hip_get_batch_resident_noalloc: the graph's query rows are not on the device (one call "
+                  "of dress_hip_get_batch_resident places them)");
+        return -1;
+    }
+    if (n == 0) return 0;
+    unsigned char *base = scratch_on_device();
+    if (!base) return -1;
+    // scratch layout: answers and weights (8-byte aligned), then u and v
+    double *d_out = (double *)base;
+    double *d_w = d_out + NOALLOC_MAX_QUERIES;
+    int *d_u = (int *)(d_w + NOALLOC_MAX_QUERIES);
+    int *d_v = d_u + NOALLOC_MAX_QUERIES;
+    hipStream_t s = G->stream;
+    bool ok = hipMemcpyAsync(d_u, u, (size_t)n * sizeof(int), hipMemcpyHostToDevice, s) == hipSuccess &&
+              hipMemcpyAsync(d_v, v, (size_t)n * sizeof(int), hipMemcpyHostToDevice, s) == hipSuccess &&
+              hipMemcpyAsync(d_w, edge_weight, (size_t)n * sizeof(double), hipMemcpyHostToDevice, s) == hipSuccess;
+    if (ok) {
+        const int grid = (int)std::min<int64_t>(8192, ((int64_t)n * GQ + TPB - 1) / TPB);
+        hipLaunchKernelGGL(k_get, dim3(grid), dim3(TPB), 0, s, n, (const int *)d_u, (const int *)d_v,
+                           (const double *)d_w, max_iterations, epsilon, G->ref_row_off, G->ref_adj_x, G->ref_adj_e,
+                           (const int *)G->inv, (const double *)G->d[G->cur], (const double *)G->ew_in,
+                           (const double *)G->vd, (const double *)G->nw, (const int *)G->rank, G->variant, d_out);
+        ok = hipGetLastError() == hipSuccess &&
+             hipMemcpyAsync(out, d_out, (size_t)n * sizeof(double), hipMemcpyDeviceToHost, s) == hipSuccess &&
+             hipStreamSynchronize(s) == hipSuccess;
+    }
+    if (!ok) {
+        (void)hipStreamSynchronize(s);
+        set_error("dress_hip_get_batch_resident_noalloc: dress_get kernel failed");
+        return -1;
+    }
+    return 0;
+}
 
 // dress.h:105-107.  Returns NaN on error (dress_hip_last_error()).
 double dress_get(const p_dress_graph_t g, int u, int v, int max_iterations, double epsilon, double edge_weight)

@@ -179,6 +179,9 @@ def _bind(L):
     L.dress_hip_get_batch_resident.restype = ctypes.c_int
     L.dress_hip_get_batch_resident.argtypes = [PG, ctypes.c_int, c_int_p, c_int_p, c_dbl_p, ctypes.c_int,
                                                ctypes.c_double, c_dbl_p]
+    L.dress_hip_get_batch_resident_noalloc.restype = ctypes.c_int
+    L.dress_hip_get_batch_resident_noalloc.argtypes = [PG, ctypes.c_int, c_int_p, c_int_p, c_dbl_p, ctypes.c_int,
+                                                       ctypes.c_double, c_dbl_p]
     L.dress_hip_fit_batch_eligible.restype = ctypes.c_int
     L.dress_hip_fit_batch_eligible.argtypes = [ctypes.c_int, ctypes.c_int]
     L.dress_hip_fit_batch.restype = ctypes.c_int

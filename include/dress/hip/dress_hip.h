@@ -216,6 +216,20 @@ int dress_hip_get_batch_resident(p_dress_graph_t g, int n, const int *u, const i
                                  const double *edge_weight, int max_iterations, double epsilon,
                                  double *out);
 
+/* dress_hip_get_batch_resident without any device allocation, for callers
+ * that cannot allocate (code under HIP graph capture, tight serving loops,
+ * hosts that forbid hipMalloc).  No counterpart in libdress.  At most
+ * 65,536 queries per call (more is an error); they are staged in a fixed
+ * 1.5 MiB device array of the library, one per device, held until the
+ * process ends, so calls that overlap on one device race.  The graph must
+ * already hold a fitted state and its query rows on the device: one call of
+ * dress_hip_get_batch_resident after the fit places them.  Answers equal
+ * dress_hip_get_batch_resident's.  Returns 0, or -1 (see
+ * dress_hip_last_error). */
+int dress_hip_get_batch_resident_noalloc(p_dress_graph_t g, int n, const int *u, const int *v,
+                                         const double *edge_weight, int max_iterations, double epsilon,
+                                         double *out);
+
 /* How dress_{delta,nabla}_fit_hip* run their subgraphs:
  *   0 (default) one batched persistent launch when the graph fits the
  *     kernel's LDS state (up to 65535 edges and ~150 KB of state), otherwise
