@@ -965,6 +965,7 @@ bool build_bins(DevGraph *G)
 bool refresh_self_terms(DevGraph *G, const double *NW_host)
 {
     G->fit_state_valid = false;
+    G->uni_state = 0;
     hipStream_t s = G->stream;
     if (NW_host && G->N > 0) {
         if (!G->vstage && !dalloc(&G->vstage, (size_t)G->N + 1, "vertex staging")) return false;
@@ -1120,7 +1121,7 @@ void free_graph(DevGraph *G)
     dfree(G->rank); dfree(G->vstage); dfree(G->q_d); dfree(G->q_vd); dfree(G->q_nw);
     free_reference_arrays(G);
     dfree(G->ord); dfree(G->inv); dfree(G->uv); dfree(G->rs); dfree(G->pos); dfree(G->ew); dfree(G->c4);
-    dfree(G->ioff); dfree(G->ipk); dfree(G->ipair); dfree(G->cnt_in);
+    dfree(G->ioff); dfree(G->ipk); dfree(G->ipair); dfree(G->cnt_in); dfree(G->b0off); dfree(G->b0pair);
     dfree(G->vbins.list);
     for (int i = 0; i < 2; i++) { dfree(G->d[i]); dfree(G->wh[i]); dfree(G->d32[i]); dfree(G->wh32[i]); }
     dfree(G->vd); dfree(G->stage); dfree(G->state);

@@ -94,6 +94,9 @@ struct EdgeArgs {
     const uint32_t *ipk;
     const int2 *ipair;
     const double *vd;
+    const int *b0off;              // bin 0: offsets relative to the bin
+    const int2 *b0pair;            // bin 0: absolute half-edge pairs
+    double ew_c, c4_c;             // the uniform w-bar and self-term constant
     int directed;
 };
 
@@ -115,10 +118,10 @@ __device__ __forceinline__ void entry(const EdgeArgs &a, int64_t k, int2 r, int 
 // Vertex phase: vd[u] = sqrt(4nw[u] + sum of w-bar*d over row u).
 // VT = double (FP64 phase) or float (FP32 phase: values stored in FP32,
 // summed in FP64, exact for FP32 terms within 2^29 of each other).
-template <int G, typename VT>
+template <int G, typename VT, bool UNI>
 __global__ void __launch_bounds__(TPB) k_vertex(const int *__restrict__ list, int count,
                                                 const int *__restrict__ row_off, const VT *w0, const VT *w1,
-                                                const double *__restrict__ nw, double *__restrict__ vd,
+                                                const double *__restrict__ nw, double nw_c, double *__restrict__ vd,
                                                 const FitState *__restrict__ st)
 {
     const bool f32 = sizeof(VT) == 4;
@@ -130,7 +133,7 @@ __global__ void __launch_bounds__(TPB) k_vertex(const int *__restrict__ list, in
         int u = ntl(list + g);
         int b = row_off[u], e = row_off[u + 1];
         double s = 0.0, c = 0.0;
-        if (lane == 0) s = 4.0 * nw[u];
+        if (lane == 0) s = 4.0 * (UNI ? nw_c : nw[u]);
         int i = b + lane;
         for (; i + G < e; i += 2 * G) {
             double x0 = (double)ntl(wh + i), x1 = (double)ntl(wh + i + G);
@@ -170,7 +173,7 @@ __device__ __forceinline__ int slice_start(const int64_t *ioff, int lo, int hi, 
 // edges of one owner vertex are adjacent in a bin, so the owner's row stays
 // in that XCD's L2 while its edges are processed.  The grid is what fits on
 // the device at once, so the blocks of a slice sweep it together.
-template <int G, bool PK, typename VT>
+template <int G, bool PK, typename VT, bool UNI>
 __global__ void __launch_bounds__(TPB) k_edge(int lo, int hi, int nx, EdgeArgs a, VT *d0, VT *d1, VT *w0,
                                               VT *w1, FitState *st)
 {
@@ -217,9 +220,9 @@ __global__ void __launch_bounds__(TPB) k_edge(int lo, int hi, int nx, EdgeArgs a
         double de = 0.0, w = 0.0, c4 = 0.0;
         if (lane == 0) {
             de = (double)dc[e];
-            w = ntl(a.ew + e);
+            w = UNI ? a.ew_c : ntl(a.ew + e);
             double own = w * de;
-            c4 = ntl(a.c4 + e);
+            c4 = UNI ? a.c4_c : ntl(a.c4 + e);
             double t0 = a.directed ? (c4 + own) : (c4 + 2.0 * own);
             two_sum_add(s, c, t0);
         }
@@ -243,6 +246,116 @@ __global__ void __launch_bounds__(TPB) k_edge(int lo, int hi, int nx, EdgeArgs a
         }
     }
     wave_max_atomic(mymax, &st->maxd);
+}
+
+// first edge (relative to the bin) of slice j of nx over the bin's n edges,
+// cost as in slice_start, from the bin's 32-bit offsets
+__device__ __forceinline__ int slice_start32(const int *off, int n, int j, int nx)
+{
+    if (j <= 0) return 0;
+    if (j >= nx) return n;
+    const int64_t total = (int64_t)off[n] + SLICE_EDGE_COST * n;
+    const int64_t target = total / nx * j + (total % nx) * j / nx;
+    int a = 0, b = n;
+    while (a < b) {
+        int m = a + (b - a) / 2;
+        int64_t c = (int64_t)off[m] + SLICE_EDGE_COST * m;
+        if (c < target) a = m + 1;
+        else b = m;
+    }
+    return a;
+}
+
+// Edge phase of bin 0 (at most 3 common neighbours, one lane per edge):
+// the index entries are absolute half-edge pairs behind a 32-bit offset, so
+// an edge streams its ends, its half-edge slots, one offset and its value
+// (and, for graphs whose w-bar or self terms vary, those two), with no row
+// starts and no 64-bit offsets.  Same arithmetic, in the same order, as
+// k_edge<1>.
+template <typename VT, bool UNI>
+__global__ void __launch_bounds__(TPB) k_edge0(int lo, int hi, int nx, EdgeArgs a, VT *d0, VT *d1, VT *w0,
+                                               VT *w1, FitState *st)
+{
+    const bool f32 = sizeof(VT) == 4;
+    if (st->done || (st->fp32 != 0) != f32) return;
+    const int cur = st->cur;
+    const VT *__restrict__ dc = cur ? d1 : d0;
+    const VT *__restrict__ wc = cur ? w1 : w0;
+    VT *__restrict__ dn = cur ? d0 : d1;
+    VT *__restrict__ wn = cur ? w0 : w1;
+    const int n = hi - lo;
+    const int xs = blockIdx.x % nx;
+    const int s_lo = slice_start32(a.b0off, n, xs, nx);
+    const int s_hi = slice_start32(a.b0off, n, xs + 1, nx);
+    const int groups = (gridDim.x / nx) * TPB;
+    double mymax = 0.0;
+    for (int g = s_lo + (blockIdx.x / nx) * TPB + threadIdx.x; g < s_hi; g += groups) {
+        const int e = lo + g;
+        const int kb = ntl(a.b0off + g), ke = ntl(a.b0off + g + 1);
+        double s = 0.0, c = 0.0;
+        for (int k = kb; k < ke; k++) {
+            const int2 p = ntl2(a.b0pair + k);
+            two_sum_add(s, c, (double)wc[p.x] + (double)wc[p.y]);
+        }
+        const double de = (double)dc[e];
+        const double w = UNI ? a.ew_c : ntl(a.ew + e);
+        const double c4 = UNI ? a.c4_c : ntl(a.c4 + e);
+        const double own = w * de;
+        two_sum_add(s, c, a.directed ? (c4 + own) : (c4 + 2.0 * own));
+        // a NaN self term marks an edge removed from a masked subgraph
+        const bool dead = !(c4 == c4);
+        const double num = s + c;
+        const int2 p = ntl2(a.uv + e);
+        const double den = a.vd[p.x] * a.vd[p.y];
+        const double x = dead ? 0.0 : (den > 0.0 ? num / den : 0.0);
+        if (!dead) mymax = fmax(mymax, fabs(de - x));
+        dn[e] = (VT)x;
+        const VT wx = (VT)(w * x);
+        const int2 q = ntl2(a.pos + e);
+        wn[q.x] = wx;
+        if (q.y >= 0) wn[q.y] = wx;
+    }
+    wave_max_atomic(mymax, &st->maxd);
+}
+
+__global__ void k_fill_int(int *p, int n, int v)
+{
+    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) p[i] = v;
+}
+
+// bin 0's offsets relative to the bin and its entries as absolute pairs
+__global__ void k_pack_bin0(int lo, int hi, const int64_t *ioff, const int2 *rs, const uint32_t *ipk,
+                            const int2 *ipair, int *b0off, int2 *b0pair)
+{
+    const int64_t base = ioff[lo];
+    for (int e = lo + blockIdx.x * blockDim.x + threadIdx.x; e < hi; e += gridDim.x * blockDim.x) {
+        b0off[e - lo] = (int)(ioff[e] - base);
+        if (e == hi - 1) b0off[hi - lo] = (int)(ioff[hi] - base);
+        for (int64_t k = ioff[e]; k < ioff[e + 1]; k++) {
+            int2 p;
+            if (ipk) {
+                const uint32_t q = ipk[k];
+                const int2 r = rs[e];
+                p = make_int2(r.x + (int)(q & 0xffffu), r.y + (int)(q >> 16));
+            } else {
+                p = ipair[k];
+            }
+            b0pair[k - base] = p;
+        }
+    }
+}
+
+// sets *flag when ew, c4 or nw hold more than one bit pattern
+__global__ void k_check_uniform(const double *ew, const double *c4, int E, const double *nw, int N, int *flag)
+{
+    const long long e0 = E > 0 ? __double_as_longlong(ew[0]) : 0, c0 = E > 0 ? __double_as_longlong(c4[0]) : 0;
+    const long long n0 = N > 0 ? __double_as_longlong(nw[0]) : 0;
+    int bad = 0;
+    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < E || i < N; i += gridDim.x * blockDim.x) {
+        if (i < E && (__double_as_longlong(ew[i]) != e0 || __double_as_longlong(c4[i]) != c0)) bad = 1;
+        if (i < N && __double_as_longlong(nw[i]) != n0) bad = 1;
+    }
+    if (__any(bad) && (threadIdx.x & 63) == 0) atomicOr(flag, 1);
 }
 
 // ------------------------------------------------------------- glue kernels
@@ -340,64 +453,130 @@ inline int grid_for(int64_t threads, int cap)
     return (int)b;
 }
 
-template <int G, typename VT>
+template <int G, typename VT, bool UNI>
 void launch_vertex_t(DevGraph *Gr, int bin, int cap, VT *w0, VT *w1)
 {
     const Bins &B = Gr->vbins;
     int n = B.count[bin];
     if (!n) return;
     int grid = grid_for((int64_t)n * G, cap);
-    hipLaunchKernelGGL((k_vertex<G, VT>), dim3(grid), dim3(TPB), 0, Gr->stream, B.list + B.start[bin], n,
-                       Gr->row_off, w0, w1, Gr->nw, Gr->vd, (const FitState *)Gr->state);
+    hipLaunchKernelGGL((k_vertex<G, VT, UNI>), dim3(grid), dim3(TPB), 0, Gr->stream, B.list + B.start[bin], n,
+                       Gr->row_off, w0, w1, Gr->nw, Gr->nw_c, Gr->vd, (const FitState *)Gr->state);
 }
 
-// workgroups of k_edge<G, PK, VT> resident on the whole device at once
-template <int G, bool PK, typename VT>
+// workgroups of a kernel resident on the whole device at once
+template <typename K>
+int resident_blocks(K kernel)
+{
+    int nb = 0;
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, kernel, TPB, 0) != hipSuccess || nb < 1) {
+        (void)hipGetLastError();
+        nb = 4;
+    }
+    return nb * num_cus();
+}
+
+template <int G, bool PK, typename VT, bool UNI>
 int edge_capacity()
 {
-    static const int cap = [] {
-        int nb = 0;
-        if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, k_edge<G, PK, VT>, TPB, 0) != hipSuccess || nb < 1) {
-            hipGetLastError();
-            nb = 4;
-        }
-        return nb * num_cus();
-    }();
+    static const int cap = resident_blocks(k_edge<G, PK, VT, UNI>);
     return cap;
 }
 
-template <int G, bool PK, typename VT>
-void launch_edge_t(DevGraph *Gr, int bin, int cap, VT *d0, VT *d1, VT *w0, VT *w1)
+template <typename VT, bool UNI>
+int edge0_capacity()
 {
-    (void)cap;
-    int lo = Gr->ebin[bin], hi = Gr->ebin[bin + 1];
-    if (hi <= lo) return;
-    int grid = grid_for((int64_t)(hi - lo) * G, edge_capacity<G, PK, VT>());
-    int nx = num_xcds();
-    if (grid >= 4 * nx) grid -= grid % nx;
-    else nx = 1;
+    static const int cap = resident_blocks(k_edge0<VT, UNI>);
+    return cap;
+}
+
+EdgeArgs edge_args(DevGraph *Gr)
+{
     EdgeArgs a;
     a.uv = Gr->uv; a.rs = Gr->rs; a.pos = Gr->pos; a.ew = Gr->ew; a.c4 = Gr->c4; a.ioff = Gr->ioff;
     a.ipk = Gr->ipk; a.ipair = Gr->ipair; a.vd = Gr->vd;
+    a.b0off = Gr->b0off; a.b0pair = Gr->b0pair;
+    a.ew_c = Gr->ew_c; a.c4_c = Gr->c4_c;
     a.directed = (Gr->variant == DRESS_VARIANT_FORWARD || Gr->variant == DRESS_VARIANT_BACKWARD);
-    hipLaunchKernelGGL((k_edge<G, PK, VT>), dim3(grid), dim3(TPB), 0, Gr->stream, lo, hi, nx, a, d0, d1, w0, w1,
-                       (FitState *)Gr->state);
+    return a;
 }
 
-template <bool PK, typename VT>
-void launch_phase_t(DevGraph *Gr, VT *d0, VT *d1, VT *w0, VT *w1)
+// grid of an edge launch over n edges with g lanes each, and its XCD slices
+inline void edge_grid(int64_t n, int g, int capacity, int *grid, int *nx)
 {
-    int cap = num_cus() * 32;
-    launch_vertex_t<1, VT>(Gr, 0, cap, w0, w1);
-    launch_vertex_t<4, VT>(Gr, 1, cap, w0, w1);
-    launch_vertex_t<16, VT>(Gr, 2, cap, w0, w1);
-    launch_vertex_t<64, VT>(Gr, 3, cap, w0, w1);
-    launch_vertex_t<TPB, VT>(Gr, 4, cap, w0, w1);
-    launch_edge_t<1, PK, VT>(Gr, 0, cap, d0, d1, w0, w1);
-    launch_edge_t<4, PK, VT>(Gr, 1, cap, d0, d1, w0, w1);
-    launch_edge_t<16, PK, VT>(Gr, 2, cap, d0, d1, w0, w1);
-    launch_edge_t<64, PK, VT>(Gr, 3, cap, d0, d1, w0, w1);
-    launch_edge_t<TPB, PK, VT>(Gr, 4, cap, d0, d1, w0, w1);
+    *grid = grid_for(n * g, capacity);
+    *nx = num_xcds();
+    if (*grid >= 4 * *nx) *grid -= *grid % *nx;
+    else *nx = 1;
+}
+
+template <int G, bool PK, typename VT, bool UNI>
+void launch_edge_t(DevGraph *Gr, int bin, VT *d0, VT *d1, VT *w0, VT *w1)
+{
+    int lo = Gr->ebin[bin], hi = Gr->ebin[bin + 1];
+    if (hi <= lo) return;
+    int grid, nx;
+    if (G == 1 && Gr->b0off) {
+        edge_grid(hi - lo, 1, edge0_capacity<VT, UNI>(), &grid, &nx);
+        hipLaunchKernelGGL((k_edge0<VT, UNI>), dim3(grid), dim3(TPB), 0, Gr->stream, lo, hi, nx, edge_args(Gr), d0, d1,
+                           w0, w1, (FitState *)Gr->state);
+        return;
+    }
+    edge_grid(hi - lo, G, edge_capacity<G, PK, VT, UNI>(), &grid, &nx);
+    hipLaunchKernelGGL((k_edge<G, PK, VT, UNI>), dim3(grid), dim3(TPB), 0, Gr->stream, lo, hi, nx, edge_args(Gr), d0,
+                       d1, w0, w1, (FitState *)Gr->state);
+}
+
+// one kernel of one iteration: vertex bin b (b < NB) or edge bin b - NB
+template <bool PK, typename VT, bool UNI>
+void launch_one_t(DevGraph *Gr, int b, VT *d0, VT *d1, VT *w0, VT *w1)
+{
+    const int cap = num_cus() * 32;
+    switch (b) {
+    case 0: launch_vertex_t<1, VT, UNI>(Gr, 0, cap, w0, w1); break;
+    case 1: launch_vertex_t<4, VT, UNI>(Gr, 1, cap, w0, w1); break;
+    case 2: launch_vertex_t<16, VT, UNI>(Gr, 2, cap, w0, w1); break;
+    case 3: launch_vertex_t<64, VT, UNI>(Gr, 3, cap, w0, w1); break;
+    case 4: launch_vertex_t<TPB, VT, UNI>(Gr, 4, cap, w0, w1); break;
+    case 5: launch_edge_t<1, PK, VT, UNI>(Gr, 0, d0, d1, w0, w1); break;
+    case 6: launch_edge_t<4, PK, VT, UNI>(Gr, 1, d0, d1, w0, w1); break;
+    case 7: launch_edge_t<16, PK, VT, UNI>(Gr, 2, d0, d1, w0, w1); break;
+    case 8: launch_edge_t<64, PK, VT, UNI>(Gr, 3, d0, d1, w0, w1); break;
+    default: launch_edge_t<TPB, PK, VT, UNI>(Gr, 4, d0, d1, w0, w1); break;
+    }
+}
+
+// kernel b of an iteration in the graph's precision form
+void launch_one(DevGraph *Gr, int b, bool fp32)
+{
+    const bool uni = Gr->uni_state == 1;
+    if (fp32) {
+        float *d0 = Gr->d32[0], *d1 = Gr->d32[1], *w0 = Gr->wh32[0], *w1 = Gr->wh32[1];
+        if (Gr->packed) {
+            if (uni) launch_one_t<true, float, true>(Gr, b, d0, d1, w0, w1);
+            else launch_one_t<true, float, false>(Gr, b, d0, d1, w0, w1);
+        } else {
+            if (uni) launch_one_t<false, float, true>(Gr, b, d0, d1, w0, w1);
+            else launch_one_t<false, float, false>(Gr, b, d0, d1, w0, w1);
+        }
+    } else {
+        double *d0 = Gr->d[0], *d1 = Gr->d[1], *w0 = Gr->wh[0], *w1 = Gr->wh[1];
+        if (Gr->packed) {
+            if (uni) launch_one_t<true, double, true>(Gr, b, d0, d1, w0, w1);
+            else launch_one_t<true, double, false>(Gr, b, d0, d1, w0, w1);
+        } else {
+            if (uni) launch_one_t<false, double, true>(Gr, b, d0, d1, w0, w1);
+            else launch_one_t<false, double, false>(Gr, b, d0, d1, w0, w1);
+        }
+    }
+}
+
+template <bool PK, typename VT, bool UNI>
+void prepare_t()
+{
+    (void)edge_capacity<1, PK, VT, UNI>(); (void)edge_capacity<4, PK, VT, UNI>();
+    (void)edge_capacity<16, PK, VT, UNI>(); (void)edge_capacity<64, PK, VT, UNI>();
+    (void)edge_capacity<TPB, PK, VT, UNI>(); (void)edge0_capacity<VT, UNI>();
 }
 
 // resolves the occupancy of every edge kernel the iteration launches (a
@@ -407,34 +586,54 @@ void launch_phase_prepare(DevGraph *Gr)
     (void)num_xcds();
     (void)num_cus();
     if (Gr->packed) {
-        (void)edge_capacity<1, true, double>(); (void)edge_capacity<4, true, double>();
-        (void)edge_capacity<16, true, double>(); (void)edge_capacity<64, true, double>();
-        (void)edge_capacity<TPB, true, double>();
-        (void)edge_capacity<1, true, float>(); (void)edge_capacity<4, true, float>();
-        (void)edge_capacity<16, true, float>(); (void)edge_capacity<64, true, float>();
-        (void)edge_capacity<TPB, true, float>();
+        prepare_t<true, double, true>(); prepare_t<true, double, false>();
+        prepare_t<true, float, true>(); prepare_t<true, float, false>();
     } else {
-        (void)edge_capacity<1, false, double>(); (void)edge_capacity<4, false, double>();
-        (void)edge_capacity<16, false, double>(); (void)edge_capacity<64, false, double>();
-        (void)edge_capacity<TPB, false, double>();
-        (void)edge_capacity<1, false, float>(); (void)edge_capacity<4, false, float>();
-        (void)edge_capacity<16, false, float>(); (void)edge_capacity<64, false, float>();
-        (void)edge_capacity<TPB, false, float>();
+        prepare_t<false, double, true>(); prepare_t<false, double, false>();
+        prepare_t<false, float, true>(); prepare_t<false, float, false>();
     }
 }
 
 void launch_phase(DevGraph *Gr, bool fp32)
 {
-    if (fp32) {
-        if (Gr->packed) launch_phase_t<true, float>(Gr, Gr->d32[0], Gr->d32[1], Gr->wh32[0], Gr->wh32[1]);
-        else launch_phase_t<false, float>(Gr, Gr->d32[0], Gr->d32[1], Gr->wh32[0], Gr->wh32[1]);
-    } else {
-        if (Gr->packed) launch_phase_t<true, double>(Gr, Gr->d[0], Gr->d[1], Gr->wh[0], Gr->wh[1]);
-        else launch_phase_t<false, double>(Gr, Gr->d[0], Gr->d[1], Gr->wh[0], Gr->wh[1]);
-    }
+    for (int b = 0; b < 2 * Bins::NB; b++) launch_one(Gr, b, fp32);
 }
 
 }  // namespace
+
+// Are ew, c4 and nw uniform (G->uni_state 1) or not (2)?  Decided on the
+// device when unknown; the kernels then take the three constants as
+// arguments.
+bool settle_uniform(DevGraph *G)
+{
+    if (G->uni_state != 0) return true;
+    hipStream_t s = G->stream;
+    int *flag = nullptr;
+    if (!dalloc(&flag, 1, "uniformity flag")) return false;
+    int bad = 1;
+    double c[3] = {0.0, 0.0, 0.0};
+    hipLaunchKernelGGL(k_fill_int, dim3(1), dim3(64), 0, s, flag, 1, 0);
+    const int64_t n = std::max<int64_t>(G->E, G->N);
+    if (n > 0)
+        hipLaunchKernelGGL(k_check_uniform, dim3(grid_for(n, num_cus() * 8)), dim3(TPB), 0, s, (const double *)G->ew,
+                           (const double *)G->c4, G->E, (const double *)G->nw, G->N, flag);
+    bool ok = hipGetLastError() == hipSuccess &&
+              hipMemcpyAsync(&bad, flag, sizeof(int), hipMemcpyDeviceToHost, s) == hipSuccess &&
+              (G->E == 0 || (hipMemcpyAsync(&c[0], G->ew, sizeof(double), hipMemcpyDeviceToHost, s) == hipSuccess &&
+                             hipMemcpyAsync(&c[1], G->c4, sizeof(double), hipMemcpyDeviceToHost, s) == hipSuccess)) &&
+              (G->N == 0 || hipMemcpyAsync(&c[2], G->nw, sizeof(double), hipMemcpyDeviceToHost, s) == hipSuccess) &&
+              hipStreamSynchronize(s) == hipSuccess;
+    dfree(flag);
+    if (!ok) {
+        set_error("uniformity check failed");
+        return false;
+    }
+    G->uni_state = bad ? 2 : 1;
+    G->ew_c = c[0];
+    G->c4_c = c[1];
+    G->nw_c = c[2];
+    return true;
+}
 
 bool fit_alloc_state(DevGraph *G, bool fp32)
 {
@@ -446,6 +645,29 @@ bool fit_alloc_state(DevGraph *G, bool fp32)
         if (!G->wh[i] && !dalloc(&G->wh[i], (size_t)G->S + 1, "half-edge values")) return false;
     }
     if (!G->vd && !dalloc(&G->vd, (size_t)G->N + 1, "vertex norms")) return false;
+    if (!G->b0off && G->ebin[1] > G->ebin[0]) {
+        // bin 0's index as absolute pairs behind 32-bit offsets (its
+        // entries are at most 3 per edge, so the bin holds far fewer than
+        // 2^31 of them)
+        const int lo = G->ebin[0], hi = G->ebin[1];
+        int64_t t[2] = {0, 0};
+        DH_CHECK(hipMemcpyAsync(&t[0], G->ioff + lo, sizeof(int64_t), hipMemcpyDeviceToHost, G->stream));
+        DH_CHECK(hipMemcpyAsync(&t[1], G->ioff + hi, sizeof(int64_t), hipMemcpyDeviceToHost, G->stream));
+        DH_CHECK(hipStreamSynchronize(G->stream));
+        const int64_t t0 = t[1] - t[0];
+        if (t0 < (int64_t)INT32_MAX) {
+            if (!dalloc(&G->b0off, (size_t)(hi - lo) + 1, "bin-0 offsets") ||
+                !dalloc(&G->b0pair, (size_t)t0 + 1, "bin-0 entries")) {
+                dfree(G->b0off);
+                G->b0off = nullptr;
+                return false;
+            }
+            hipLaunchKernelGGL(k_pack_bin0, dim3(grid_for(hi - lo, num_cus() * 8)), dim3(TPB), 0, G->stream, lo, hi,
+                               (const int64_t *)G->ioff, (const int2 *)G->rs, (const uint32_t *)G->ipk,
+                               (const int2 *)G->ipair, G->b0off, G->b0pair);
+            DH_CHECK(hipGetLastError());
+        }
+    }
     if (fp32) {
         for (int i = 0; i < 2; i++) {
             if (!G->d32[i] && !dalloc(&G->d32[i], (size_t)G->E + 1, "edge values fp32")) return false;
@@ -510,7 +732,8 @@ hipGraphExec_t iteration_graph(DevGraph *G, bool mixed, double epsilon, int max_
     const char *env = getenv("DRESS_HIP_GRAPH");
     if (!(env && !strcmp(env, "1"))) return nullptr;
     const int m = mixed ? 1 : 0;
-    if (G->iter_graph[m] && G->graph_eps[m] == epsilon && G->graph_maxit[m] == max_iterations)
+    if (G->iter_graph[m] && G->graph_eps[m] == epsilon && G->graph_maxit[m] == max_iterations &&
+        G->graph_uni[m] == G->uni_state)
         return G->iter_graph[m];
     if (G->iter_graph[m]) {
         hipGraphExecDestroy(G->iter_graph[m]);
@@ -536,6 +759,7 @@ hipGraphExec_t iteration_graph(DevGraph *G, bool mixed, double epsilon, int max_
     G->iter_graph[m] = exec;
     G->graph_eps[m] = epsilon;
     G->graph_maxit[m] = max_iterations;
+    G->graph_uni[m] = G->uni_state;
     return exec;
 }
 
@@ -666,7 +890,7 @@ bool fit_run(DevGraph *G, int max_iterations, double epsilon, int precision, Fit
     bool mixed = (precision == PREC_FP32_FP64);
     res->ms = 0.f;
     res->reruns = 0;
-    if (!build_bins(G)) return false;
+    if (!build_bins(G) || !settle_uniform(G)) return false;
     if (max_iterations <= 0) {
         res->iterations = max_iterations;
         res->executed = 0;
@@ -721,7 +945,7 @@ bool fit_fill_values(DevGraph *G, double value)
 // bins, ms[5..9] edge bins (0 where the bin is empty).
 bool fit_time_phases(DevGraph *G, int reps, double *ms)
 {
-    if (!fit_alloc_state(G, false)) return false;
+    if (!fit_alloc_state(G, false) || !settle_uniform(G)) return false;
     hipStream_t s = G->stream;
     FitState h;
     memset(&h, 0, sizeof(h));
@@ -734,49 +958,11 @@ bool fit_time_phases(DevGraph *G, int reps, double *ms)
     std::vector<hipEvent_t> ev(22);
     for (auto &e : ev) DH_CHECK(hipEventCreate(&e));
     for (int i = 0; i < 10; i++) ms[i] = 0.0;
-    int cap = num_cus() * 32;
     for (int r = 0; r < reps; r++) {
-        for (int b = 0; b < Bins::NB; b++) {
+        for (int b = 0; b < 2 * Bins::NB; b++) {
             DH_CHECK(hipEventRecord(ev[2 * b], s));
-            if (G->packed) {
-                switch (b) {
-                case 0: launch_vertex_t<1, double>(G, 0, cap, G->wh[0], G->wh[1]); break;
-                case 1: launch_vertex_t<4, double>(G, 1, cap, G->wh[0], G->wh[1]); break;
-                case 2: launch_vertex_t<16, double>(G, 2, cap, G->wh[0], G->wh[1]); break;
-                case 3: launch_vertex_t<64, double>(G, 3, cap, G->wh[0], G->wh[1]); break;
-                default: launch_vertex_t<TPB, double>(G, 4, cap, G->wh[0], G->wh[1]); break;
-                }
-            } else {
-                switch (b) {
-                case 0: launch_vertex_t<1, double>(G, 0, cap, G->wh[0], G->wh[1]); break;
-                case 1: launch_vertex_t<4, double>(G, 1, cap, G->wh[0], G->wh[1]); break;
-                case 2: launch_vertex_t<16, double>(G, 2, cap, G->wh[0], G->wh[1]); break;
-                case 3: launch_vertex_t<64, double>(G, 3, cap, G->wh[0], G->wh[1]); break;
-                default: launch_vertex_t<TPB, double>(G, 4, cap, G->wh[0], G->wh[1]); break;
-                }
-            }
+            launch_one(G, b, false);
             DH_CHECK(hipEventRecord(ev[2 * b + 1], s));
-        }
-        for (int b = 0; b < Bins::NB; b++) {
-            DH_CHECK(hipEventRecord(ev[10 + 2 * b], s));
-            if (G->packed) {
-                switch (b) {
-                case 0: launch_edge_t<1, true, double>(G, 0, cap, G->d[0], G->d[1], G->wh[0], G->wh[1]); break;
-                case 1: launch_edge_t<4, true, double>(G, 1, cap, G->d[0], G->d[1], G->wh[0], G->wh[1]); break;
-                case 2: launch_edge_t<16, true, double>(G, 2, cap, G->d[0], G->d[1], G->wh[0], G->wh[1]); break;
-                case 3: launch_edge_t<64, true, double>(G, 3, cap, G->d[0], G->d[1], G->wh[0], G->wh[1]); break;
-                default: launch_edge_t<TPB, true, double>(G, 4, cap, G->d[0], G->d[1], G->wh[0], G->wh[1]); break;
-                }
-            } else {
-                switch (b) {
-                case 0: launch_edge_t<1, false, double>(G, 0, cap, G->d[0], G->d[1], G->wh[0], G->wh[1]); break;
-                case 1: launch_edge_t<4, false, double>(G, 1, cap, G->d[0], G->d[1], G->wh[0], G->wh[1]); break;
-                case 2: launch_edge_t<16, false, double>(G, 2, cap, G->d[0], G->d[1], G->wh[0], G->wh[1]); break;
-                case 3: launch_edge_t<64, false, double>(G, 3, cap, G->d[0], G->d[1], G->wh[0], G->wh[1]); break;
-                default: launch_edge_t<TPB, false, double>(G, 4, cap, G->d[0], G->d[1], G->wh[0], G->wh[1]); break;
-                }
-            }
-            DH_CHECK(hipEventRecord(ev[11 + 2 * b], s));
         }
         DH_CHECK(hipEventSynchronize(ev[19]));
         for (int i = 0; i < 10; i++) {

@@ -186,6 +186,18 @@ struct DevGraph {
     hipGraphExec_t iter_graph[2] = {nullptr, nullptr};   // one iteration, [mixed]
     double graph_eps[2] = {0.0, 0.0};                      // ... captured with these
     int graph_maxit[2] = {0, 0};
+    int graph_uni[2] = {0, 0};
+    // bin 0 (one lane per edge): index entries as absolute half-edge pairs
+    // behind 32-bit offsets relative to the bin (built with the fit state)
+    int *b0off = nullptr;
+    int2 *b0pair = nullptr;
+    // Uniform per-edge constants: when every w-bar (ew), every self-term
+    // constant (c4) and every vertex weight (nw) is the same -- unweighted
+    // graphs without vertex weights, except DIRECTED -- the kernels take the
+    // three values as arguments instead of reading the arrays.  uni_state:
+    // 0 unknown (every writer of ew / c4 / nw resets it), 1 uniform, 2 not.
+    int uni_state = 0;
+    double ew_c = 0.0, c4_c = 0.0, nw_c = 0.0;
     int cur = 0;                   // buffer holding the current values
 };
 

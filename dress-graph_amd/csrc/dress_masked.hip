@@ -135,6 +135,9 @@ bool masked_items(p_dress_graph_t g, DevGraph *G, int kind, int k, const std::ve
 {
     *n_vals = 0;
     if (!build_bins(G) || !fit_alloc_state(G, precision == PREC_FP32_FP64)) return false;
+    // items mark removed edges and vertex weights per fit: not uniform
+    // until the graph's own self terms are back
+    G->uni_state = 2;
     hipStream_t s = G->stream;
     const int E = G->E, N = G->N;
     const int directed = (G->variant == DRESS_VARIANT_FORWARD || G->variant == DRESS_VARIANT_BACKWARD) ? 1 : 0;
@@ -234,6 +237,7 @@ bool masked_items(p_dress_graph_t g, DevGraph *G, int kind, int k, const std::ve
              hipStreamSynchronize(s) == hipSuccess;
     // the graph's own vertex weights and self terms again
     bool restored = refresh_self_terms(G, g->NW) && hipStreamSynchronize(s) == hipSuccess;
+    G->uni_state = 0;
     dfree(d_verts); dfree(d_marks); dfree(d_count); dfree(d_row);
     dfree(d_pe); dfree(d_pw); dfree(d_ppw);
     if (ok && !restored) ok = false;
