@@ -373,25 +373,30 @@ def main():
                  "equals_reference": sha256(g.edge_dress) == gold["edge_dress_sha256"]}
     g.resident_reset()
 
-    # dominant kernel: the edge-phase bin with the most time
-    vms, ems = g.time_phases(reps=5)
+    # dominant kernel: the edge-phase bin with the most time, each bin
+    # launched alone (as fits launch them, back to back)
+    vms, ems, phase = g.time_phases(reps=5)
     b = int(np.argmax(ems))
-    launch_bytes = info["bin_terms"][b] * B_ENTRY + info["edge_bins"][b] * B_EDGE
-    achieved = launch_bytes / (ems[b] * 1e-3) / 1e9
-    group = [1, 4, 16, 64, 256][b]
     S, T, Nn = info["S"], info["T"], info["N"]
+    launch_bytes = info["bin_terms"][b] * B_ENTRY + info["edge_bins"][b] * B_EDGE
+    launch_ms = float(ems[b])
+    achieved = launch_bytes / (launch_ms * 1e-3) / 1e9
+    # lanes per edge of each bin (DRESS_HIP_EDGE_LANES overrides bins 1-3)
+    lanes = [1, 4, 8, 32, 256]
+    if os.environ.get("DRESS_HIP_EDGE_LANES"):
+        lanes[1:4] = [int(x) for x in os.environ["DRESS_HIP_EDGE_LANES"].split(",")]
     b_iter = B_ENTRY * T + B_HALF * S + B_EDGE * E + B_VERTEX * Nn
     iter_gbs = b_iter / (ms_per_step * 1e-3 / iters_per_fit) / 1e9
     copy_ms = median([dh.copy_probe(1 << 30, 5) for _ in range(3)])
     copy_gbs = 2 * (1 << 30) / (copy_ms * 1e-3) / 1e9
     gather_ms = dh.gather_probe(1 << 30, 1 << 26, 5)
-    kname = f"k_edge<{group},packed,double> (edge bin {b})"
+    kname = f"k_edge<{lanes[b]},packed,double> (edge bin {b})" if b > 0 else "k_edge0<double> (edge bin 0)"
     bins = []
     for i in range(5):
         if info["edge_bins"][i] == 0:
             continue
         nb = info["bin_terms"][i] * B_ENTRY + info["edge_bins"][i] * B_EDGE
-        bins.append({"bin": i, "edges": info["edge_bins"][i], "entries": info["bin_terms"][i],
+        bins.append({"bin": i, "lanes": lanes[i], "edges": info["edge_bins"][i], "entries": info["bin_terms"][i],
                      "launch_ms": float(ems[i]), "algorithmic_bytes": int(nb),
                      "frac": nb / (ems[i] * 1e-3) / 1e9 / HBM_PEAK_GBS})
 
@@ -419,7 +424,8 @@ def main():
                      "traffic": TRAFFIC["bytes_per_launch"] if TRAFFIC["kernel"] == kname else None,
                      "traffic_source": TRAFFIC["source"] if TRAFFIC["kernel"] == kname else None,
                      "kernel": kname,
-                     "launch_ms": float(ems[b]), "algorithmic_bytes": int(launch_bytes),
+                     "launch_ms": launch_ms, "algorithmic_bytes": int(launch_bytes),
+                     "vertex_phase_ms": float(phase[0]),
                      "iteration_algorithmic_GBs": iter_gbs, "iteration_frac": iter_gbs / HBM_PEAK_GBS,
                      "iteration_ms": ms_per_step / iters_per_fit,
                      "edge_bins": bins,
@@ -430,7 +436,8 @@ def main():
                       "ms_per_step": r32["seconds"] / args.steps * 1e3,
                       "fp32_iterations_per_fit": r32["fp32_iterations"] / args.steps,
                       "speedup_vs_fp64": value32 / value, "max_edge_err": err32},
-        "phase_ms": {"vertex": [float(x) for x in vms], "edge": [float(x) for x in ems]},
+        "phase_ms": {"vertex_bins_alone": [float(x) for x in vms], "edge_bins": [float(x) for x in ems],
+                     "vertex": float(phase[0]), "edge": float(phase[1])},
         "dress_get": get_res,
         "er2000": er2000(dh, gen),
     }

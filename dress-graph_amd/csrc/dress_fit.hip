@@ -22,15 +22,18 @@
 // under relabelling like the reference's.  All FP64 code is compiled with
 // -ffp-contract=off: the reference computes w*d + w*d with two roundings.
 //
-// Work is degree binned: rows/edges with <= 4, 64, 512, 8192 terms use groups
-// of 1, 4, 16, 64 lanes; larger ones a 256-lane workgroup.  Edges of one bin
-// are a contiguous range of internal ids.
+// Work is degree binned: rows with <= 4, 64, 512, 8192 terms use groups of
+// 1, 4, 16, 64 lanes, edges groups of 1, 4, 8, 32 lanes; larger ones a
+// 256-lane workgroup.  Edges of one bin are a contiguous range of internal
+// ids.  The vertex bins run as one launch.
 #include "dress_device.h"
 #include "dress_hip_internal.h"
 
 #include <algorithm>
 #include <cmath>
 #include <cstring>
+#include <cstdio>
+#include <cstdlib>
 
 namespace dh {
 
@@ -119,17 +122,13 @@ __device__ __forceinline__ void entry(const EdgeArgs &a, int64_t k, int2 r, int 
 // VT = double (FP64 phase) or float (FP32 phase: values stored in FP32,
 // summed in FP64, exact for FP32 terms within 2^29 of each other).
 template <int G, typename VT, bool UNI>
-__global__ void __launch_bounds__(TPB) k_vertex(const int *__restrict__ list, int count,
-                                                const int *__restrict__ row_off, const VT *w0, const VT *w1,
-                                                const double *__restrict__ nw, double nw_c, double *__restrict__ vd,
-                                                const FitState *__restrict__ st)
+__device__ __forceinline__ void vertex_rows(const int *__restrict__ list, int count, int blk, int nblk,
+                                            const int *__restrict__ row_off, const VT *__restrict__ wh,
+                                            const double *__restrict__ nw, double nw_c, double *__restrict__ vd)
 {
-    const bool f32 = sizeof(VT) == 4;
-    if (st->done || (st->fp32 != 0) != f32) return;
-    const VT *__restrict__ wh = st->cur ? w1 : w0;
     const int lane = (G == TPB) ? threadIdx.x : (threadIdx.x % G);
-    const int64_t groups = (int64_t)gridDim.x * (TPB / G);
-    for (int64_t g = (int64_t)blockIdx.x * (TPB / G) + threadIdx.x / G; g < count; g += groups) {
+    const int64_t groups = (int64_t)nblk * (TPB / G);
+    for (int64_t g = (int64_t)blk * (TPB / G) + threadIdx.x / G; g < count; g += groups) {
         int u = ntl(list + g);
         int b = row_off[u], e = row_off[u + 1];
         double s = 0.0, c = 0.0;
@@ -144,6 +143,52 @@ __global__ void __launch_bounds__(TPB) k_vertex(const int *__restrict__ list, in
         if (G == TPB) block_reduce(s, c);
         else group_reduce<G>(s, c);
         if (lane == 0) vd[u] = sqrt(s + c);
+    }
+}
+
+template <int G, typename VT, bool UNI>
+__global__ void __launch_bounds__(TPB) k_vertex(const int *__restrict__ list, int count,
+                                                const int *__restrict__ row_off, const VT *w0, const VT *w1,
+                                                const double *__restrict__ nw, double nw_c, double *__restrict__ vd,
+                                                const FitState *__restrict__ st)
+{
+    const bool f32 = sizeof(VT) == 4;
+    if (st->done || (st->fp32 != 0) != f32) return;
+    vertex_rows<G, VT, UNI>(list, count, blockIdx.x, gridDim.x, row_off, st->cur ? w1 : w0, nw, nw_c, vd);
+}
+
+// the five vertex bins of an iteration in one launch: blocks [first[b],
+// first[b+1]) serve bin b (rows of up to 4, 64, 512, 8192 and more entries
+// with 1, 4, 16, 64 and 256 lanes), the widest rows first
+struct VertexBins {
+    const int *list[Bins::NB];
+    int count[Bins::NB];
+    int first[Bins::NB + 1];       // in launch order: bins 4, 3, 2, 1, 0
+};
+
+template <typename VT, bool UNI>
+__global__ void __launch_bounds__(TPB) k_vertex_all(VertexBins vb, const int *__restrict__ row_off, const VT *w0,
+                                                    const VT *w1, const double *__restrict__ nw, double nw_c,
+                                                    double *__restrict__ vd, const FitState *__restrict__ st)
+{
+    const bool f32 = sizeof(VT) == 4;
+    if (st->done || (st->fp32 != 0) != f32) return;
+    const VT *wh = st->cur ? w1 : w0;
+    const int b = blockIdx.x;
+    if (b < vb.first[1]) {
+        vertex_rows<TPB, VT, UNI>(vb.list[4], vb.count[4], b, vb.first[1], row_off, wh, nw, nw_c, vd);
+    } else if (b < vb.first[2]) {
+        vertex_rows<64, VT, UNI>(vb.list[3], vb.count[3], b - vb.first[1], vb.first[2] - vb.first[1], row_off, wh, nw,
+                                 nw_c, vd);
+    } else if (b < vb.first[3]) {
+        vertex_rows<16, VT, UNI>(vb.list[2], vb.count[2], b - vb.first[2], vb.first[3] - vb.first[2], row_off, wh, nw,
+                                 nw_c, vd);
+    } else if (b < vb.first[4]) {
+        vertex_rows<4, VT, UNI>(vb.list[1], vb.count[1], b - vb.first[3], vb.first[4] - vb.first[3], row_off, wh, nw,
+                                nw_c, vd);
+    } else {
+        vertex_rows<1, VT, UNI>(vb.list[0], vb.count[0], b - vb.first[4], vb.first[5] - vb.first[4], row_off, wh, nw,
+                                nw_c, vd);
     }
 }
 
@@ -173,24 +218,18 @@ __device__ __forceinline__ int slice_start(const int64_t *ioff, int lo, int hi, 
 // edges of one owner vertex are adjacent in a bin, so the owner's row stays
 // in that XCD's L2 while its edges are processed.  The grid is what fits on
 // the device at once, so the blocks of a slice sweep it together.
+// Edges [lo, hi) of one bin, G lanes per edge, by block j of the K blocks
+// that work in slice xs of nx; the largest change into mymax.
 template <int G, bool PK, typename VT, bool UNI>
-__global__ void __launch_bounds__(TPB) k_edge(int lo, int hi, int nx, EdgeArgs a, VT *d0, VT *d1, VT *w0,
-                                              VT *w1, FitState *st)
+__device__ __forceinline__ void edge_range(int lo, int hi, int nx, int xs, int j, int K, const EdgeArgs &a,
+                                           const VT *__restrict__ dc, const VT *__restrict__ wc,
+                                           VT *__restrict__ dn, VT *__restrict__ wn, double &mymax)
 {
-    const bool f32 = sizeof(VT) == 4;
-    if (st->done || (st->fp32 != 0) != f32) return;
-    const int cur = st->cur;
-    const VT *__restrict__ dc = cur ? d1 : d0;
-    const VT *__restrict__ wc = cur ? w1 : w0;
-    VT *__restrict__ dn = cur ? d0 : d1;
-    VT *__restrict__ wn = cur ? w0 : w1;
     const int lane = (G == TPB) ? threadIdx.x : (threadIdx.x % G);
-    const int xs = blockIdx.x % nx;
     const int s_lo = slice_start(a.ioff, lo, hi, xs, nx);
     const int s_hi = slice_start(a.ioff, lo, hi, xs + 1, nx);
-    const int64_t groups = (int64_t)(gridDim.x / nx) * (TPB / G);
-    double mymax = 0.0;
-    for (int64_t g = s_lo + (int64_t)(blockIdx.x / nx) * (TPB / G) + threadIdx.x / G; g < s_hi; g += groups) {
+    const int64_t groups = (int64_t)K * (TPB / G);
+    for (int64_t g = s_lo + (int64_t)j * (TPB / G) + threadIdx.x / G; g < s_hi; g += groups) {
         const int e = (int)g;
         int64_t kb = ntl(a.ioff + e), ke = ntl(a.ioff + e + 1);
         int2 r = PK ? ntl2(a.rs + e) : make_int2(0, 0);
@@ -245,6 +284,18 @@ __global__ void __launch_bounds__(TPB) k_edge(int lo, int hi, int nx, EdgeArgs a
             if (q.y >= 0) wn[q.y] = wx;
         }
     }
+}
+
+template <int G, bool PK, typename VT, bool UNI>
+__global__ void __launch_bounds__(TPB) k_edge(int lo, int hi, int nx, EdgeArgs a, VT *d0, VT *d1, VT *w0,
+                                              VT *w1, FitState *st)
+{
+    const bool f32 = sizeof(VT) == 4;
+    if (st->done || (st->fp32 != 0) != f32) return;
+    const int cur = st->cur;
+    double mymax = 0.0;
+    edge_range<G, PK, VT, UNI>(lo, hi, nx, blockIdx.x % nx, blockIdx.x / nx, gridDim.x / nx, a, cur ? d1 : d0,
+                               cur ? w1 : w0, cur ? d0 : d1, cur ? w0 : w1, mymax);
     wave_max_atomic(mymax, &st->maxd);
 }
 
@@ -273,23 +324,15 @@ __device__ __forceinline__ int slice_start32(const int *off, int n, int j, int n
 // starts and no 64-bit offsets.  Same arithmetic, in the same order, as
 // k_edge<1>.
 template <typename VT, bool UNI>
-__global__ void __launch_bounds__(TPB) k_edge0(int lo, int hi, int nx, EdgeArgs a, VT *d0, VT *d1, VT *w0,
-                                               VT *w1, FitState *st)
+__device__ __forceinline__ void edge0_range(int lo, int hi, int nx, int xs, int j, int K, const EdgeArgs &a,
+                                            const VT *__restrict__ dc, const VT *__restrict__ wc,
+                                            VT *__restrict__ dn, VT *__restrict__ wn, double &mymax)
 {
-    const bool f32 = sizeof(VT) == 4;
-    if (st->done || (st->fp32 != 0) != f32) return;
-    const int cur = st->cur;
-    const VT *__restrict__ dc = cur ? d1 : d0;
-    const VT *__restrict__ wc = cur ? w1 : w0;
-    VT *__restrict__ dn = cur ? d0 : d1;
-    VT *__restrict__ wn = cur ? w0 : w1;
     const int n = hi - lo;
-    const int xs = blockIdx.x % nx;
     const int s_lo = slice_start32(a.b0off, n, xs, nx);
     const int s_hi = slice_start32(a.b0off, n, xs + 1, nx);
-    const int groups = (gridDim.x / nx) * TPB;
-    double mymax = 0.0;
-    for (int g = s_lo + (blockIdx.x / nx) * TPB + threadIdx.x; g < s_hi; g += groups) {
+    const int groups = K * TPB;
+    for (int g = s_lo + j * TPB + threadIdx.x; g < s_hi; g += groups) {
         const int e = lo + g;
         const int kb = ntl(a.b0off + g), ke = ntl(a.b0off + g + 1);
         double s = 0.0, c = 0.0;
@@ -315,6 +358,18 @@ __global__ void __launch_bounds__(TPB) k_edge0(int lo, int hi, int nx, EdgeArgs 
         wn[q.x] = wx;
         if (q.y >= 0) wn[q.y] = wx;
     }
+}
+
+template <typename VT, bool UNI>
+__global__ void __launch_bounds__(TPB) k_edge0(int lo, int hi, int nx, EdgeArgs a, VT *d0, VT *d1, VT *w0,
+                                               VT *w1, FitState *st)
+{
+    const bool f32 = sizeof(VT) == 4;
+    if (st->done || (st->fp32 != 0) != f32) return;
+    const int cur = st->cur;
+    double mymax = 0.0;
+    edge0_range<VT, UNI>(lo, hi, nx, blockIdx.x % nx, blockIdx.x / nx, gridDim.x / nx, a, cur ? d1 : d0,
+                         cur ? w1 : w0, cur ? d0 : d1, cur ? w0 : w1, mymax);
     wave_max_atomic(mymax, &st->maxd);
 }
 
@@ -464,6 +519,27 @@ void launch_vertex_t(DevGraph *Gr, int bin, int cap, VT *w0, VT *w1)
                        Gr->row_off, w0, w1, Gr->nw, Gr->nw_c, Gr->vd, (const FitState *)Gr->state);
 }
 
+// all vertex bins in one launch (the block counts per bin the separate
+// launches would use)
+template <typename VT, bool UNI>
+void launch_vertex_all_t(DevGraph *Gr, VT *w0, VT *w1)
+{
+    const Bins &B = Gr->vbins;
+    const int cap = num_cus() * 32;
+    static const int lanes[Bins::NB] = {1, 4, 16, 64, TPB};
+    VertexBins vb;
+    vb.first[0] = 0;
+    for (int j = 0; j < Bins::NB; j++) {
+        const int bin = Bins::NB - 1 - j;      // launch order: widest rows first
+        vb.list[bin] = B.list + B.start[bin];
+        vb.count[bin] = B.count[bin];
+        vb.first[j + 1] = vb.first[j] + (B.count[bin] ? grid_for((int64_t)B.count[bin] * lanes[bin], cap) : 0);
+    }
+    if (vb.first[Bins::NB] == 0) return;
+    hipLaunchKernelGGL((k_vertex_all<VT, UNI>), dim3(vb.first[Bins::NB]), dim3(TPB), 0, Gr->stream, vb, Gr->row_off,
+                       w0, w1, Gr->nw, Gr->nw_c, Gr->vd, (const FitState *)Gr->state);
+}
+
 // workgroups of a kernel resident on the whole device at once
 template <typename K>
 int resident_blocks(K kernel)
@@ -527,46 +603,97 @@ void launch_edge_t(DevGraph *Gr, int bin, VT *d0, VT *d1, VT *w0, VT *w1)
                        d1, w0, w1, (FitState *)Gr->state);
 }
 
-// one kernel of one iteration: vertex bin b (b < NB) or edge bin b - NB
+// DRESS_HIP_PHASE_SPLIT=1 launches the vertex bins of an iteration one by
+// one instead of in one kernel (0.19 against 0.14 ms on the 4M-vertex graph)
+bool phases_split()
+{
+    static const bool split = [] {
+        const char *e = getenv("DRESS_HIP_PHASE_SPLIT");
+        return e && !strcmp(e, "1");
+    }();
+    return split;
+}
+
+// lanes per edge of edge bins 1-3 (4, 8, 32 by default: on the 4M-vertex
+// graph 139.8 ms per fit against 141.1 with 4, 16, 64; bin 0 takes one lane
+// per edge, bin 4 a workgroup).  DRESS_HIP_EDGE_LANES="l1,l2,l3"
+// overrides them with powers of two from 2 to 64 (a tuning knob: results
+// do not depend on it).
+const int *edge_lanes()
+{
+    static int lanes[3] = {4, 8, 32};
+    static bool once = [] {
+        if (const char *e = getenv("DRESS_HIP_EDGE_LANES")) {
+            int v[3];
+            if (sscanf(e, "%d,%d,%d", &v[0], &v[1], &v[2]) == 3)
+                for (int i = 0; i < 3; i++)
+                    if (v[i] >= 2 && v[i] <= 64 && (v[i] & (v[i] - 1)) == 0) lanes[i] = v[i];
+        }
+        return true;
+    }();
+    (void)once;
+    return lanes;
+}
+
 template <bool PK, typename VT, bool UNI>
-void launch_one_t(DevGraph *Gr, int b, VT *d0, VT *d1, VT *w0, VT *w1)
+void launch_edge_lanes(DevGraph *Gr, int bin, int lanes, VT *d0, VT *d1, VT *w0, VT *w1)
+{
+    switch (lanes) {
+    case 2: launch_edge_t<2, PK, VT, UNI>(Gr, bin, d0, d1, w0, w1); break;
+    case 4: launch_edge_t<4, PK, VT, UNI>(Gr, bin, d0, d1, w0, w1); break;
+    case 8: launch_edge_t<8, PK, VT, UNI>(Gr, bin, d0, d1, w0, w1); break;
+    case 16: launch_edge_t<16, PK, VT, UNI>(Gr, bin, d0, d1, w0, w1); break;
+    case 32: launch_edge_t<32, PK, VT, UNI>(Gr, bin, d0, d1, w0, w1); break;
+    default: launch_edge_t<64, PK, VT, UNI>(Gr, bin, d0, d1, w0, w1); break;
+    }
+}
+
+// Kernel b of an iteration: b < NB vertex bin b, b < 2 NB edge bin b - NB.
+// Unless `split`, the vertex phase is one launch over every bin (b = 0) and
+// b = 1..4 are empty.  Edge bins are always launched one by one: the edge
+// phase as one launch (each block through its share of every bin in turn)
+// took 5.87 ms per iteration on the 4M-vertex graph against 5.10 ms for the
+// five launches, whose boundaries keep each bin's rows together in L2.
+template <bool PK, typename VT, bool UNI>
+void launch_one_t(DevGraph *Gr, int b, bool split, VT *d0, VT *d1, VT *w0, VT *w1)
 {
     const int cap = num_cus() * 32;
     switch (b) {
-    case 0: launch_vertex_t<1, VT, UNI>(Gr, 0, cap, w0, w1); break;
-    case 1: launch_vertex_t<4, VT, UNI>(Gr, 1, cap, w0, w1); break;
-    case 2: launch_vertex_t<16, VT, UNI>(Gr, 2, cap, w0, w1); break;
-    case 3: launch_vertex_t<64, VT, UNI>(Gr, 3, cap, w0, w1); break;
-    case 4: launch_vertex_t<TPB, VT, UNI>(Gr, 4, cap, w0, w1); break;
+    case 0:
+        if (split) launch_vertex_t<1, VT, UNI>(Gr, 0, cap, w0, w1);
+        else launch_vertex_all_t<VT, UNI>(Gr, w0, w1);
+        break;
+    case 1: if (split) launch_vertex_t<4, VT, UNI>(Gr, 1, cap, w0, w1); break;
+    case 2: if (split) launch_vertex_t<16, VT, UNI>(Gr, 2, cap, w0, w1); break;
+    case 3: if (split) launch_vertex_t<64, VT, UNI>(Gr, 3, cap, w0, w1); break;
+    case 4: if (split) launch_vertex_t<TPB, VT, UNI>(Gr, 4, cap, w0, w1); break;
     case 5: launch_edge_t<1, PK, VT, UNI>(Gr, 0, d0, d1, w0, w1); break;
-    case 6: launch_edge_t<4, PK, VT, UNI>(Gr, 1, d0, d1, w0, w1); break;
-    case 7: launch_edge_t<16, PK, VT, UNI>(Gr, 2, d0, d1, w0, w1); break;
-    case 8: launch_edge_t<64, PK, VT, UNI>(Gr, 3, d0, d1, w0, w1); break;
+    case 6: case 7: case 8: launch_edge_lanes<PK, VT, UNI>(Gr, b - 5, edge_lanes()[b - 6], d0, d1, w0, w1); break;
     default: launch_edge_t<TPB, PK, VT, UNI>(Gr, 4, d0, d1, w0, w1); break;
     }
 }
 
 // kernel b of an iteration in the graph's precision form
-void launch_one(DevGraph *Gr, int b, bool fp32)
+void launch_one(DevGraph *Gr, int b, bool fp32, bool split)
 {
     const bool uni = Gr->uni_state == 1;
     if (fp32) {
         float *d0 = Gr->d32[0], *d1 = Gr->d32[1], *w0 = Gr->wh32[0], *w1 = Gr->wh32[1];
         if (Gr->packed) {
-            if (uni) launch_one_t<true, float, true>(Gr, b, d0, d1, w0, w1);
-            else launch_one_t<true, float, false>(Gr, b, d0, d1, w0, w1);
+            if (uni) launch_one_t<true, float, true>(Gr, b, split, d0, d1, w0, w1);
+            else launch_one_t<true, float, false>(Gr, b, split, d0, d1, w0, w1);
         } else {
-            if (uni) launch_one_t<false, float, true>(Gr, b, d0, d1, w0, w1);
-            else launch_one_t<false, float, false>(Gr, b, d0, d1, w0, w1);
+            if (uni) launch_one_t<false, float, true>(Gr, b, split, d0, d1, w0, w1);
+            else launch_one_t<false, float, false>(Gr, b, split, d0, d1, w0, w1);
         }
     } else {
         double *d0 = Gr->d[0], *d1 = Gr->d[1], *w0 = Gr->wh[0], *w1 = Gr->wh[1];
         if (Gr->packed) {
-            if (uni) launch_one_t<true, double, true>(Gr, b, d0, d1, w0, w1);
-            else launch_one_t<true, double, false>(Gr, b, d0, d1, w0, w1);
+            if (uni) launch_one_t<true, double, true>(Gr, b, split, d0, d1, w0, w1);
+            else launch_one_t<true, double, false>(Gr, b, split, d0, d1, w0, w1);
         } else {
-            if (uni) launch_one_t<false, double, true>(Gr, b, d0, d1, w0, w1);
-            else launch_one_t<false, double, false>(Gr, b, d0, d1, w0, w1);
+            if (uni) launch_one_t<false, double, true>(Gr, b, split, d0, d1, w0, w1);
+            else launch_one_t<false, double, false>(Gr, b, split, d0, d1, w0, w1);
         }
     }
 }
@@ -574,8 +701,10 @@ void launch_one(DevGraph *Gr, int b, bool fp32)
 template <bool PK, typename VT, bool UNI>
 void prepare_t()
 {
-    (void)edge_capacity<1, PK, VT, UNI>(); (void)edge_capacity<4, PK, VT, UNI>();
-    (void)edge_capacity<16, PK, VT, UNI>(); (void)edge_capacity<64, PK, VT, UNI>();
+    (void)edge_capacity<1, PK, VT, UNI>(); (void)edge_capacity<2, PK, VT, UNI>();
+    (void)edge_capacity<4, PK, VT, UNI>(); (void)edge_capacity<8, PK, VT, UNI>();
+    (void)edge_capacity<16, PK, VT, UNI>(); (void)edge_capacity<32, PK, VT, UNI>();
+    (void)edge_capacity<64, PK, VT, UNI>();
     (void)edge_capacity<TPB, PK, VT, UNI>(); (void)edge0_capacity<VT, UNI>();
 }
 
@@ -596,7 +725,8 @@ void launch_phase_prepare(DevGraph *Gr)
 
 void launch_phase(DevGraph *Gr, bool fp32)
 {
-    for (int b = 0; b < 2 * Bins::NB; b++) launch_one(Gr, b, fp32);
+    const bool split = phases_split();
+    for (int b = 0; b < 2 * Bins::NB; b++) launch_one(Gr, b, fp32, split);
 }
 
 }  // namespace
@@ -942,7 +1072,9 @@ bool fit_fill_values(DevGraph *G, double value)
 
 // Average device time of each phase kernel of one FP64 iteration over reps
 // iterations, bracketed by events on the graph's stream.  ms[0..4] vertex
-// bins, ms[5..9] edge bins (0 where the bin is empty).
+// bins, ms[5..9] edge bins, each launched alone (0 where the bin is empty);
+// ms[10] the vertex phase and ms[11] the edge phase as the fit launches
+// them (one kernel for the vertex phase, one per edge bin).
 bool fit_time_phases(DevGraph *G, int reps, double *ms)
 {
     if (!fit_alloc_state(G, false) || !settle_uniform(G)) return false;
@@ -955,24 +1087,36 @@ bool fit_time_phases(DevGraph *G, int reps, double *ms)
         hipLaunchKernelGGL((k_init_wh<double>), dim3(grid_for(G->E, num_cus() * 8)), dim3(TPB), 0, s, G->E, G->ew,
                            G->pos, G->d[0], G->d[1], (double *)nullptr, (double *)nullptr, G->wh[0], G->wh[1],
                            (const FitState *)G->state);
-    std::vector<hipEvent_t> ev(22);
+    std::vector<hipEvent_t> ev(24);
     for (auto &e : ev) DH_CHECK(hipEventCreate(&e));
-    for (int i = 0; i < 10; i++) ms[i] = 0.0;
+    for (int i = 0; i < 12; i++) ms[i] = 0.0;
     for (int r = 0; r < reps; r++) {
         for (int b = 0; b < 2 * Bins::NB; b++) {
             DH_CHECK(hipEventRecord(ev[2 * b], s));
-            launch_one(G, b, false);
+            launch_one(G, b, false, true);
             DH_CHECK(hipEventRecord(ev[2 * b + 1], s));
         }
-        DH_CHECK(hipEventSynchronize(ev[19]));
-        for (int i = 0; i < 10; i++) {
+        // the vertex phase in one launch, then the edge phase as fits launch it
+        DH_CHECK(hipEventRecord(ev[20], s));
+        launch_one(G, 0, false, false);
+        DH_CHECK(hipEventRecord(ev[21], s));
+        for (int b = Bins::NB; b < 2 * Bins::NB; b++) launch_one(G, b, false, false);
+        DH_CHECK(hipEventRecord(ev[22], s));
+        DH_CHECK(hipEventSynchronize(ev[22]));
+        for (int i = 0; i < 12; i++) {
+            if (i >= 10) {
+                float t = 0.f;
+                DH_CHECK(hipEventElapsedTime(&t, ev[20 + i - 10], ev[21 + i - 10]));
+                ms[i] += t;
+                continue;
+            }
             float t = 0.f;
             DH_CHECK(hipEventElapsedTime(&t, ev[2 * i], ev[2 * i + 1]));
             ms[i] += t;
         }
     }
     for (auto &e : ev) (void)hipEventDestroy(e);
-    for (int i = 0; i < 10; i++) ms[i] /= std::max(reps, 1);
+    for (int i = 0; i < 12; i++) ms[i] /= std::max(reps, 1);
     // the timing iterations only wrote the other buffer: the current values
     // (buffer G->cur) are unchanged
     return true;

@@ -429,11 +429,13 @@ class Graph:
         _lib.check(_lib.lib().dress_hip_resident_fill(self.g, float(value)) == 0, "dress_hip_resident_fill")
 
     def time_phases(self, reps=3):
-        """Average ms per kernel of one FP64 iteration: (vertex[5], edge[5])."""
-        ms = np.zeros(10, dtype=np.float64)
+        """Average ms per kernel of one FP64 iteration: (vertex[5], edge[5])
+        with every bin launched alone, and (vertex, edge) as fits launch
+        them (the vertex phase one kernel, the edge bins back to back)."""
+        ms = np.zeros(12, dtype=np.float64)
         r = _lib.lib().dress_hip_time_phases(self.g, int(reps), ms.ctypes.data_as(c_dbl_p))
         _lib.check(r == 0, "dress_hip_time_phases")
-        return ms[:5], ms[5:]
+        return ms[:5], ms[5:10], ms[10:12]
 
     def resident_download(self):
         _lib.check(_lib.lib().dress_hip_resident_download(self.g) == 0, "dress_hip_resident_download")

@@ -174,10 +174,13 @@ int dress_hip_resident_download(p_dress_graph_t g);
  * value = 1.0, the state dress_init_graph creates).  Returns 0 or -1. */
 int dress_hip_resident_fill(p_dress_graph_t g, double value);
 
-/* Device time of each kernel of one FP64 iteration, averaged over `reps`
- * iterations run on the resident state: ms[0..4] the vertex-phase kernels,
- * ms[5..9] the edge-phase kernels of the five work bins (0 for an empty
- * bin).  Leaves the resident values as they were.  Returns 0 or -1. */
+/* Device time of the kernels of one FP64 iteration, averaged over `reps`
+ * iterations run on the resident state (ms holds 12 entries): ms[0..4] the
+ * vertex phase and ms[5..9] the edge phase of each of the five work bins
+ * launched on its own (0 for an empty bin); ms[10] the vertex phase and
+ * ms[11] the edge phase as fits launch them (one kernel for the vertex
+ * phase, one per edge bin, back to back).  Leaves the
+ * resident values as they were.  Returns 0 or -1. */
 int dress_hip_time_phases(p_dress_graph_t g, int reps, double *ms);
 
 /* Streams n_bytes (a multiple of 16) from one device buffer to another with
