@@ -54,19 +54,20 @@ B_ENTRY, B_HALF, B_EDGE, B_VERTEX = 24, 12, 44, 20
 # two 8 B value reads, per half-edge one, per edge its value, new value, two
 # norms and two alive flags, per vertex its norm and alive flag
 L_ENTRY, L_HALF, L_EDGE, L_VERTEX = 16, 8, 40, 12
-# HBM bytes per launch of the dominant kernel from hardware counters, from
-# the run recorded in profiles/fit_power_law_4M_fp64.md (rocprofv3 --pmc
-# FETCH_SIZE and --pmc WRITE_SIZE over `python bench.py --traffic-run`), each
-# scaled by what the same counter reads for probes of known bytes: the
-# streaming copy (FETCH_SIZE x2: it counts half of a copy's reads) and random
-# 8-byte gathers (the pattern of the edge kernels' gathers).  Counters
-# cannot be read inside this process, so the figure is labelled with its
-# source and reported only when that kernel is the dominant one.
-TRAFFIC = {"kernel": "k_edge<16,packed,double> (edge bin 2)", "fetch_kib": 5418725, "write_kib": 117693,
-           "fetch_scale": 2.0, "write_scale": 1.0,
+# Bytes per launch of the dominant kernel moved beyond L2, from hardware
+# counters of the run recorded in profiles/fit_power_law_4M_fp64.md
+# (rocprofv3 --pmc TCC_HIT_sum TCC_MISS_sum, then --pmc WRITE_SIZE, over
+# `python bench.py --traffic-run`): 64 B per L2 miss plus WRITE_SIZE KiB.
+# Calibrated on two probes of known bytes in the same runs: the 1 GiB
+# streaming copy reads 16,777,328 misses x 64 B = 1 GiB and writes
+# 1,048,600 KiB; 2^26 random 8-byte gathers from 1 GiB take one miss each
+# (FETCH_SIZE counts a streaming read at half its bytes and a random one at
+# 64 B, so it is not used).  Counters cannot be read inside this process, so
+# the figure is labelled with its source and reported only when that kernel
+# is the dominant one.
+TRAFFIC = {"kernel": "k_edge<8,packed,double> (edge bin 2)", "tcc_miss": 92966314, "write_kib": 104202,
            "source": "profiles/fit_power_law_4M_fp64.md, counter runs of bench.py --traffic-run"}
-TRAFFIC["bytes_per_launch"] = int((TRAFFIC["fetch_kib"] * TRAFFIC["fetch_scale"] +
-                                   TRAFFIC["write_kib"] * TRAFFIC["write_scale"]) * 1024)
+TRAFFIC["bytes_per_launch"] = int(TRAFFIC["tcc_miss"] * 64 + TRAFFIC["write_kib"] * 1024)
 GOLDEN = os.path.join(ROOT, "tests", "golden")
 
 
@@ -430,7 +431,10 @@ def main():
                      "iteration_ms": ms_per_step / iters_per_fit,
                      "edge_bins": bins,
                      "copy_kernel_GBs": copy_gbs,
-                     "gather_probe_ns_per_8B_read": gather_ms * 1e6 / (1 << 26)},
+                     # the ceiling of random 8-byte reads (one 64 B request each)
+                     "random_read_ceiling_GBs": (1 << 26) * 64 / (gather_ms * 1e-3) / 1e9,
+                     "traffic_GBs": (TRAFFIC["bytes_per_launch"] / (launch_ms * 1e-3) / 1e9
+                                     if TRAFFIC["kernel"] == kname else None)},
         "fit_c_abi": fit_c_abi,
         "fp32_fp64": {"value": value32, "unit": "edge-updates/s",
                       "ms_per_step": r32["seconds"] / args.steps * 1e3,
