@@ -81,7 +81,7 @@ def test_max_iterations_edges(ora, ref):
 
 
 @pytest.mark.parametrize("kind", ["delta", "nabla"])
-@pytest.mark.parametrize("variant", [0, 1, 2])
+@pytest.mark.parametrize("variant", [0, 1, 2, 3])
 def test_lifted_bitwise_vs_reference(ora, ref, kind, variant):
     rng = np.random.default_rng(900 + variant)
     U, V, W, _ = _rand_multigraph(rng, 9, 22)
@@ -98,6 +98,36 @@ def test_lifted_bitwise_vs_reference(ora, ref, kind, variant):
                     assert a[2] is None and b[2] is None
                 else:
                     assert np.array_equal(a[2], b[2], equal_nan=True)
+
+
+@pytest.mark.parametrize("variant", [0, 1, 2, 3])
+@pytest.mark.parametrize("weighted", [False, True])
+def test_get_bitwise_vs_reference(ora, ref, variant, weighted):
+    """dress_get (dress.c:651-739) in every variant, with edge and vertex
+    weights: existing edges (the binary search's copy of a repeated edge)
+    and virtual edges (common neighbours of both rows, self term, cross
+    factor and the 1-D fixed point), on a fitted multigraph with loops."""
+    rng = np.random.default_rng(1200 + variant + 10 * weighted)
+    n = 14
+    U, V, W, _ = _rand_multigraph(rng, n, 40)
+    NW = rng.random(n) + 0.5 if weighted else None
+    Wg = W if weighted else None
+    ga = ora.graph(n, U, V, Wg, NW, variant, 0)
+    gb = ref.graph(n, U, V, Wg, NW, variant, 0)
+    try:
+        ora.fit_graph(ga, 100, 1e-8)
+        ref.fit_graph(gb, 100, 1e-8)
+        qu = rng.integers(0, n, 300)
+        qv = rng.integers(0, n, 300)
+        qw = rng.random(300) + 0.25
+        for a, b, w in zip(qu.tolist(), qv.tolist(), qw.tolist()):
+            for it, eps in ((100, 1e-8), (3, 0.0), (0, 1e-6)):
+                x = ora.get(ga, a, b, it, eps, w)
+                y = ref.get(gb, a, b, it, eps, w)
+                assert (x == y) or (x != x and y != y), (variant, a, b, w, it, x, y)
+    finally:
+        ora.free_graph(ga)
+        ref.free_graph(gb)
 
 
 def test_lifted_omp_vs_reference(ora, ref, gen):
