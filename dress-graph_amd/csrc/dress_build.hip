@@ -669,6 +669,13 @@ bool build_graph(DevGraph *G, int N, int E, const int *U, const int *V, const do
         }
     }
     hipStream_t s = G->stream;
+    // the uploads below read the caller's arrays asynchronously: whatever
+    // path leaves this function, they are complete before it returns (the
+    // caller may free the arrays at once)
+    struct SyncOnExit {
+        hipStream_t s;
+        ~SyncOnExit() { (void)hipStreamSynchronize(s); }
+    } sync_on_exit{s};
 
     if (!dalloc(&G->U, E, "U") || !dalloc(&G->V, E, "V") || !dalloc(&G->ew_in, E, "edge_weight") ||
         !dalloc(&G->nw, (size_t)N, "NW") || !dalloc(&G->c4_in, E, "self term") ||
