@@ -509,15 +509,28 @@ namespace {
 struct BatchCache {
     AllocCtx ctx;
     hipStream_t stream = nullptr;
+    hipStream_t side[2] = {nullptr, nullptr};      // work that overlaps the main stream's
     ~BatchCache()
     {
         for (auto &kv : ctx.free_blocks)
             for (void *q : kv.second) hipFree(q);
         if (stream) hipStreamDestroy(stream);
+        for (hipStream_t x : side)
+            if (x) hipStreamDestroy(x);
     }
 };
 thread_local BatchCache t_batch;
 }  // namespace
+
+hipStream_t batch_side_stream(int i)
+{
+    if (!t_ctx.active || t_ctx.serving || i < 0 || i > 1) return nullptr;
+    if (!t_batch.side[i] && hipStreamCreateWithFlags(&t_batch.side[i], hipStreamNonBlocking) != hipSuccess) {
+        t_batch.side[i] = nullptr;
+        hipGetLastError();
+    }
+    return t_batch.side[i];
+}
 
 hipStream_t batch_ctx_enter()
 {

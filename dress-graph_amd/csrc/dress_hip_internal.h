@@ -94,6 +94,11 @@ bool ctx_serving();                // the calling thread is a server worker
 // enter returns its stream, or nullptr when a context is already active
 hipStream_t batch_ctx_enter();
 void batch_ctx_leave();
+// two more streams of the calling thread's batch context (i = 0, 1;
+// nullptr outside one, or inside a server worker), for work that overlaps
+// the main stream's.  The block cache reuses a freed block on any stream:
+// free a block only once the work that reads it has finished.
+hipStream_t batch_side_stream(int i);
 // The calling thread's batch block cache for the length of one batch call
 // (a no-op inside a server worker, which has its own).  Enter it after the
 // graph's own device arrays exist: blocks taken inside come back to the
@@ -235,7 +240,11 @@ bool fit_time_phases(DevGraph *G, int reps, double *ms);
 // dress_prim.hip (rocPRIM wrappers, stream-ordered, scratch allocated inside)
 bool prim_sort_pairs_u64_i32(const uint64_t *kin, uint64_t *kout, const int *vin, int *vout,
                              size_t n, int end_bit, hipStream_t s);
-bool prim_sort_u64(const uint64_t *kin, uint64_t *kout, size_t n, hipStream_t s);
+bool prim_sort_u64(const uint64_t *kin, uint64_t *kout, size_t n, hipStream_t s, int end_bit = 64);
+// runs of equal keys of a sorted array: the distinct keys, their counts, and
+// the number of runs (device int64)
+bool prim_run_length_encode_u64(const uint64_t *in, size_t n, uint64_t *unique, int64_t *counts, int64_t *runs,
+                                hipStream_t s);
 bool prim_sort_pairs_u64_u64(const uint64_t *kin, uint64_t *kout, const uint64_t *vin, uint64_t *vout,
                              size_t n, hipStream_t s);
 bool prim_sort_pairs_i32_u64(const int *kin, int *kout, const uint64_t *vin, uint64_t *vout, size_t n,

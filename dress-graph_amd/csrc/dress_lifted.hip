@@ -18,11 +18,14 @@
 #include <algorithm>
 #include <chrono>
 #include <cmath>
+#include <condition_variable>
 #include <cstdio>
 #include <cstdlib>
 #include <cstring>
 #include <functional>
 #include <map>
+#include <string>
+#include <memory>
 #include <mutex>
 #include <unordered_map>
 #include <vector>
@@ -356,8 +359,8 @@ void enumerate_items(int kind, int N, int k, int n_samples, unsigned int seed, i
 // set by run().
 struct PackedUpload {
     struct Part {
-        const void *src;
-        size_t bytes, at;
+        std::vector<std::pair<const void *, size_t>> pieces;     // copied one after another
+        size_t at;
         std::function<void(char *)> set;
     };
     std::vector<Part> parts;
@@ -365,26 +368,70 @@ struct PackedUpload {
     template <typename T>
     void add(T **dst, const std::vector<T> &v)
     {
-        const size_t bytes = v.size() * sizeof(T), at = total;
-        parts.push_back({v.data(), bytes, at, [dst, at](char *b) { *dst = reinterpret_cast<T *>(b + at); }});
+        add_concat(dst, std::vector<const std::vector<T> *>{&v});
+    }
+    // one device array holding the vectors vs one after another
+    template <typename T>
+    void add_concat(T **dst, const std::vector<const std::vector<T> *> &vs)
+    {
+        const size_t at = total;
+        Part p;
+        p.at = at;
+        size_t bytes = 0;
+        for (const std::vector<T> *v : vs) {
+            p.pieces.push_back({v->data(), v->size() * sizeof(T)});
+            bytes += v->size() * sizeof(T);
+        }
+        p.set = [dst, at](char *b) { *dst = reinterpret_cast<T *>(b + at); };
+        parts.push_back(std::move(p));
         total += (bytes + 256) & ~(size_t)255;        // at least one byte past the end, as upload() gives
     }
-    bool run(char **blk, hipStream_t s)
+    char *stage = nullptr;
+    // host side: the arrays copied into the staging block (after this the
+    // source arrays may go)
+    bool pack()
     {
-        if (!dalloc(blk, total + 256, "batch inputs")) return false;
-        char *stage = (char *)pinned_alloc(total);
+        stage = (char *)pinned_alloc(total);
         if (!stage) {
             set_error("host staging of the batch inputs failed");
             return false;
         }
         for (Part &p : parts) {
-            if (p.bytes) memcpy(stage + p.at, p.src, p.bytes);
-            p.set(*blk);
+            size_t off = p.at;
+            for (auto &pc : p.pieces) {
+                if (pc.second) memcpy(stage + off, pc.first, pc.second);
+                off += pc.second;
+            }
         }
-        const bool ok = hipMemcpyAsync(*blk, stage, total, hipMemcpyHostToDevice, s) == hipSuccess &&
-                        hipStreamSynchronize(s) == hipSuccess;
+        return true;
+    }
+    // device side: the block allocated, the pointers set and the copy
+    // queued on s; the staging block is released by release() once s has
+    // passed the copy
+    bool upload(char **blk, hipStream_t s)
+    {
+        if (!dalloc(blk, total + 256, "batch inputs")) return false;
+        for (Part &p : parts) p.set(*blk);
+        if (hipMemcpyAsync(*blk, stage, total, hipMemcpyHostToDevice, s) != hipSuccess) {
+            set_error("upload of the batch inputs failed");
+            return false;
+        }
+        return true;
+    }
+    void release()
+    {
         pinned_free(stage);
-        if (!ok) set_error("upload of the batch inputs failed");
+        stage = nullptr;
+    }
+    ~PackedUpload() { release(); }
+    bool run(char **blk, hipStream_t s)
+    {
+        bool ok = pack() && upload(blk, s);
+        if (ok && hipStreamSynchronize(s) != hipSuccess) {
+            set_error("upload of the batch inputs failed");
+            ok = false;
+        }
+        release();
         return ok;
     }
 };
@@ -413,56 +460,48 @@ bool upload(T **dst, const std::vector<T> &src, hipStream_t s, const char *what)
     return true;
 }
 
-// the parents' concatenated arrays, host side
-struct ParentArrays {
-    std::vector<int> row_off, adj_e, ioff;
-    std::vector<int2> uv;
-    std::vector<double> ew;
-    std::vector<uint32_t> pairs;
-};
-
-// concatenates the parents' arrays (DIRECTED patched weight copies after the
-// parents' own weights) and registers them, the per-parent offsets (meta)
-// and graph_of with pk for D
+// registers the parents' arrays with pk, each kind concatenated over the
+// parents (DIRECTED patched weight copies after the parents' own weights),
+// with the per-parent offsets (meta) and graph_of, for D
 void pack_parents(const std::vector<const HostParent *> &ps, const std::vector<int> &graph_of,
-                  std::vector<LiftParent> &meta, ParentArrays &A, DevPools &D, PackedUpload &pk,
+                  std::vector<LiftParent> &meta, DevPools &D, PackedUpload &pk,
                   const std::vector<double> *extra_ew = nullptr)
 {
     meta.clear();
-    size_t nr = 0, na = 0, ne = 0, ni = 0, np = 0;
-    for (const HostParent *P : ps) {
-        nr += P->row_off.size(); na += P->adj_e.size(); ne += P->uv.size(); ni += P->ioff.size();
-        np += P->pairs.size();
-    }
-    A.row_off.reserve(nr); A.adj_e.reserve(na); A.uv.reserve(ne); A.ioff.reserve(ni); A.pairs.reserve(np);
-    A.ew.reserve(ne + (extra_ew ? extra_ew->size() : 0));
+    std::vector<const std::vector<int> *> row_off, adj_e, ioff;
+    std::vector<const std::vector<int2> *> uv;
+    std::vector<const std::vector<double> *> ew;
+    std::vector<const std::vector<uint32_t> *> pairs;
+    int64_t nr = 0, na = 0, ne = 0, ni = 0, np = 0;
     for (const HostParent *P : ps) {
         LiftParent m;
         m.N = P->N; m.E = P->E; m.S = P->S; m.T = P->T;
-        m.row_ofs = (int64_t)A.row_off.size();
-        m.adj_ofs = (int64_t)A.adj_e.size();
-        m.edge_ofs = (int64_t)A.uv.size();
-        m.ioff_ofs = (int64_t)A.ioff.size();
-        m.pair_ofs = (int64_t)A.pairs.size();
+        m.row_ofs = nr;
+        m.adj_ofs = na;
+        m.edge_ofs = ne;
+        m.ioff_ofs = ni;
+        m.pair_ofs = np;
         m.directed = P->directed;
         m.pad = 0;
-        A.row_off.insert(A.row_off.end(), P->row_off.begin(), P->row_off.end());
-        A.adj_e.insert(A.adj_e.end(), P->adj_e.begin(), P->adj_e.end());
-        A.uv.insert(A.uv.end(), P->uv.begin(), P->uv.end());
-        A.ew.insert(A.ew.end(), P->ew.begin(), P->ew.end());
-        A.ioff.insert(A.ioff.end(), P->ioff.begin(), P->ioff.end());
-        A.pairs.insert(A.pairs.end(), P->pairs.begin(), P->pairs.end());
         meta.push_back(m);
+        nr += (int64_t)P->row_off.size(); na += (int64_t)P->adj_e.size(); ne += (int64_t)P->uv.size();
+        ni += (int64_t)P->ioff.size(); np += (int64_t)P->pairs.size();
+        row_off.push_back(&P->row_off);
+        adj_e.push_back(&P->adj_e);
+        uv.push_back(&P->uv);
+        ew.push_back(&P->ew);
+        ioff.push_back(&P->ioff);
+        pairs.push_back(&P->pairs);
     }
-    if (extra_ew) A.ew.insert(A.ew.end(), extra_ew->begin(), extra_ew->end());
+    if (extra_ew) ew.push_back(extra_ew);
     pk.add(&D.parents, meta);
     pk.add(&D.parent_graph, graph_of);
-    pk.add(&D.row_off, A.row_off);
-    pk.add(&D.adj_e, A.adj_e);
-    pk.add(&D.uv, A.uv);
-    pk.add(&D.ew, A.ew);
-    pk.add(&D.ioff, A.ioff);
-    pk.add(&D.pairs, A.pairs);
+    pk.add_concat(&D.row_off, row_off);
+    pk.add_concat(&D.adj_e, adj_e);
+    pk.add_concat(&D.uv, uv);
+    pk.add_concat(&D.ew, ew);
+    pk.add_concat(&D.ioff, ioff);
+    pk.add_concat(&D.pairs, pairs);
 }
 
 // ------------------------------------------------------------ histogram
@@ -472,11 +511,68 @@ __global__ void k_zero_int(int *p, int n)
     for (int i = threadIdx.x; i < n; i += blockDim.x) p[i] = 0;
 }
 
-__global__ void k_sort_keys(const double *v, int64_t n, uint64_t *key)
+// ascending sort keys of doubles (the bit pattern, sign-folded), and each
+// block's smallest and largest key into part[2b], part[2b + 1]
+constexpr int RANGE_BLOCKS = 1024;
+__global__ void __launch_bounds__(TPB) k_sort_keys(const double *v, int64_t n, uint64_t *key,
+                                                   unsigned long long *part)
 {
+    unsigned long long lo = ~0ull, hi = 0ull;
     for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) {
-        uint64_t b = (uint64_t)__double_as_longlong(v[i]);
-        key[i] = (b >> 63) ? ~b : (b | 0x8000000000000000ull);
+        const uint64_t b = (uint64_t)__double_as_longlong(v[i]);
+        const uint64_t k = (b >> 63) ? ~b : (b | 0x8000000000000000ull);
+        key[i] = k;
+        lo = k < lo ? k : lo;
+        hi = k > hi ? k : hi;
+    }
+    for (int o = 32; o > 0; o >>= 1) {
+        const unsigned long long x = __shfl_xor(lo, o), y = __shfl_xor(hi, o);
+        lo = x < lo ? x : lo;
+        hi = y > hi ? y : hi;
+    }
+    __shared__ unsigned long long slo[TPB / 64], shi[TPB / 64];
+    if ((threadIdx.x & 63) == 0) {
+        slo[threadIdx.x >> 6] = lo;
+        shi[threadIdx.x >> 6] = hi;
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        for (int w = 1; w < TPB / 64; w++) {
+            lo = slo[w] < lo ? slo[w] : lo;
+            hi = shi[w] > hi ? shi[w] : hi;
+        }
+        part[2 * blockIdx.x] = lo;
+        part[2 * blockIdx.x + 1] = hi;
+    }
+}
+
+// range[0], range[1] = the smallest and largest of nb block results (one
+// workgroup)
+__global__ void __launch_bounds__(TPB) k_key_range(const unsigned long long *part, int nb, unsigned long long *range)
+{
+    unsigned long long lo = ~0ull, hi = 0ull;
+    for (int i = threadIdx.x; i < nb; i += TPB) {
+        lo = part[2 * i] < lo ? part[2 * i] : lo;
+        hi = part[2 * i + 1] > hi ? part[2 * i + 1] : hi;
+    }
+    for (int o = 32; o > 0; o >>= 1) {
+        const unsigned long long x = __shfl_xor(lo, o), y = __shfl_xor(hi, o);
+        lo = x < lo ? x : lo;
+        hi = y > hi ? y : hi;
+    }
+    __shared__ unsigned long long slo[TPB / 64], shi[TPB / 64];
+    if ((threadIdx.x & 63) == 0) {
+        slo[threadIdx.x >> 6] = lo;
+        shi[threadIdx.x >> 6] = hi;
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        for (int w = 1; w < TPB / 64; w++) {
+            lo = slo[w] < lo ? slo[w] : lo;
+            hi = shi[w] > hi ? shi[w] : hi;
+        }
+        range[0] = lo;
+        range[1] = hi;
     }
 }
 
@@ -501,19 +597,58 @@ __global__ void k_run_emit(const uint64_t *key, const int *seg, const int *flag,
     }
 }
 
+// the segment holding value i: the last g with seg_ofs[g] <= i
+__device__ __forceinline__ int segment_of(const int64_t *seg_ofs, int nseg, int64_t i)
+{
+    int lo = 0, hi = nseg - 1;
+    while (lo < hi) {
+        const int m = (lo + hi + 1) >> 1;
+        if (seg_ofs[m] <= i) lo = m;
+        else hi = m - 1;
+    }
+    return lo;
+}
+
+// one 64-bit key per value ordered as (segment, value): the segment above
+// bit vbits, the value's key minus the smallest below it.  Up to
+// PACK_LDS_SEGS segment offsets are searched in LDS.
+constexpr int PACK_LDS_SEGS = 4096;
+__global__ void __launch_bounds__(TPB) k_pack_keys(const uint64_t *key, const int64_t *seg_ofs, int nseg, int64_t n,
+                                                   const unsigned long long *range, int vbits, uint64_t *out)
+{
+    extern __shared__ int64_t s_ofs[];
+    const bool lds = nseg < PACK_LDS_SEGS;
+    if (lds) {
+        for (int i = threadIdx.x; i <= nseg; i += TPB) s_ofs[i] = seg_ofs[i];
+        __syncthreads();
+    }
+    const int64_t *so = lds ? s_ofs : seg_ofs;
+    const uint64_t lo = range[0];
+    for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) {
+        const uint64_t seg = nseg > 1 ? (uint64_t)segment_of(so, nseg, i) : 0;
+        out[i] = (vbits < 64 ? seg << vbits : 0) | (key[i] - lo);
+    }
+}
+
+// the runs' packed keys back to values, in place, with their segments
+__global__ void k_unpack_runs(uint64_t *ukey, int64_t runs, const unsigned long long *range, int vbits, int *useg)
+{
+    const uint64_t lo = range[0];
+    const uint64_t mask = vbits < 64 ? (((uint64_t)1 << vbits) - 1) : ~(uint64_t)0;
+    for (int64_t j = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; j < runs; j += (int64_t)gridDim.x * blockDim.x) {
+        const uint64_t key = ukey[j];
+        const uint64_t k = (key & mask) + lo;
+        ukey[j] = (k >> 63) ? (k & 0x7fffffffffffffffull) : ~k;
+        useg[j] = vbits < 64 ? (int)(key >> vbits) : 0;
+    }
+}
+
 // seg[i] = the segment holding value i (the last g with seg_ofs[g] <= i),
 // one thread per value
 __global__ void k_fill_seg(const int64_t *seg_ofs, int nseg, int64_t n, int *seg)
 {
-    for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) {
-        int lo = 0, hi = nseg - 1;
-        while (lo < hi) {
-            const int m = (lo + hi + 1) >> 1;
-            if (seg_ofs[m] <= i) lo = m;
-            else hi = m - 1;
-        }
-        seg[i] = lo;
-    }
+    for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x)
+        seg[i] = segment_of(seg_ofs, nseg, i);
 }
 
 // run lengths from run starts (ustart ascending, n values in all)
@@ -546,11 +681,14 @@ __global__ void k_interleave(const double *v, const int64_t *c, int64_t runs, in
 }
 
 // Exact (value, count) histogram of vals[0..n) per segment (segments are
-// contiguous ranges, seg_ofs has nseg+1 entries): radix sort by value, then a
-// stable radix sort by segment, then run-length encoding, all on the device.
-// Leaves d_val / d_cnt (runs entries, ascending by (segment, value)) on the
-// device for the caller to download and dfree; run_ofs[g] is the first run
-// of segment g (nseg+1 entries).
+// contiguous ranges, seg_ofs has nseg+1 entries), on the device.  When the
+// values' sort keys span vbits bits and the segment ids fit in the bits
+// above, one 64-bit key per value holds (segment, value): one key sort and
+// a run-length encoding give the runs.  Otherwise the values are sorted
+// with their segment ids along, then stably by segment, and the runs are
+// found by flags and a scan.  Leaves d_val / d_cnt (runs entries, ascending
+// by (segment, value)) on the device for the caller to download and dfree;
+// run_ofs[g] is the first run of segment g (nseg+1 entries).
 bool exact_histogram_dev(const double *vals, int64_t n, const std::vector<int64_t> &seg_ofs, hipStream_t s,
                          double **d_val, int64_t **d_cnt, int64_t *runs_out, std::vector<int64_t> &run_ofs)
 {
@@ -562,21 +700,57 @@ bool exact_histogram_dev(const double *vals, int64_t n, const std::vector<int64_
     run_ofs.assign((size_t)nseg + 1, 0);
     if (n <= 0) return true;
     uint64_t *k0 = nullptr, *k1 = nullptr;
+    unsigned long long *part = nullptr, *range = nullptr;
     int *g0 = nullptr, *g1 = nullptr, *flag = nullptr, *usg = nullptr;
-    int64_t *pos = nullptr, *ustart = nullptr, *dofs = nullptr, *cnt = nullptr, *rofs = nullptr;
+    int64_t *pos = nullptr, *ustart = nullptr, *dofs = nullptr, *cnt = nullptr, *rofs = nullptr, *druns = nullptr;
     double *uval = nullptr;
-    bool ok = dalloc(&k0, (size_t)n, "histogram keys") && dalloc(&k1, (size_t)n, "histogram keys") &&
-              dalloc(&g0, (size_t)n, "segments") && dalloc(&g1, (size_t)n, "segments") &&
-              dalloc(&flag, (size_t)n + 1, "run flags") && dalloc(&pos, (size_t)n + 1, "run positions") &&
-              dalloc(&dofs, (size_t)nseg + 1, "segment offsets") && dalloc(&rofs, (size_t)nseg + 1, "run offsets");
+    unsigned long long *host = (unsigned long long *)snapshot_take();   // pinned read-backs
+    bool ok = host && dalloc(&k0, (size_t)n, "histogram keys") && dalloc(&k1, (size_t)n, "histogram keys") &&
+              dalloc(&flag, (size_t)n + 1, "run flags") && dalloc(&dofs, (size_t)nseg + 1, "segment offsets") &&
+              dalloc(&rofs, (size_t)nseg + 1, "run offsets") && dalloc(&part, 2 * RANGE_BLOCKS, "key range") &&
+              dalloc(&range, 2, "key range") && dalloc(&druns, 1, "run count");
     pt.mark("alloc");
-    int grid = grid_for(n, num_cus() * 8);
+    const int grid = grid_for(n, num_cus() * 8);
+    const int rgrid = grid_for(n, RANGE_BLOCKS);
     int64_t runs = 0;
+    int vbits = -1, sbits = 0;
+    while ((1 << sbits) < nseg) sbits++;
     if (ok) {
-        hipLaunchKernelGGL(k_sort_keys, dim3(grid), dim3(TPB), 0, s, vals, n, k0);
+        hipLaunchKernelGGL(k_sort_keys, dim3(rgrid), dim3(TPB), 0, s, vals, n, k0, part);
+        hipLaunchKernelGGL(k_key_range, dim3(1), dim3(TPB), 0, s, part, rgrid, range);
         ok = hipMemcpyAsync(dofs, seg_ofs.data(), seg_ofs.size() * sizeof(int64_t), hipMemcpyHostToDevice, s) ==
-             hipSuccess;
-        hipLaunchKernelGGL(k_fill_seg, dim3(grid), dim3(TPB), 0, s, dofs, nseg, n, g0);
+                 hipSuccess &&
+             hipMemcpyAsync(host, range, 2 * sizeof(unsigned long long), hipMemcpyDeviceToHost, s) == hipSuccess &&
+             hipStreamSynchronize(s) == hipSuccess;
+        const char *pe = getenv("DRESS_HIP_HIST_PACKED");
+        const unsigned long long span = host[1] - host[0];
+        const int vb = span ? 64 - __builtin_clzll(span) : 0;
+        if (ok && vb + sbits <= 64 && n <= (int64_t)0xffffffffll && !(pe && !strcmp(pe, "0"))) vbits = vb;
+    }
+    pt.mark("key range");
+    if (ok && vbits >= 0) {
+        const size_t shm = nseg < PACK_LDS_SEGS ? ((size_t)nseg + 1) * sizeof(int64_t) : 0;
+        hipLaunchKernelGGL(k_pack_keys, dim3(grid), dim3(TPB), shm, s, k0, dofs, nseg, n, range, vbits, k1);
+        ok = prim_sort_u64(k1, k0, (size_t)n, s, std::max(1, vbits + sbits)) &&
+             dalloc(&cnt, (size_t)n + 1, "histogram counts") &&
+             prim_run_length_encode_u64(k0, (size_t)n, k1, cnt, druns, s) &&
+             hipMemcpyAsync(host, druns, sizeof(int64_t), hipMemcpyDeviceToHost, s) == hipSuccess &&
+             hipStreamSynchronize(s) == hipSuccess;
+        if (pt.on) pt.mark("sort+runs");
+        if (ok) {
+            runs = (int64_t)host[0];
+            const int ugrid = grid_for(runs, num_cus() * 8);
+            usg = flag;                                          // (flags unused on this path)
+            hipLaunchKernelGGL(k_unpack_runs, dim3(ugrid), dim3(TPB), 0, s, k1, runs, range, vbits, usg);
+            hipLaunchKernelGGL(k_seg_run_starts, dim3(ugrid), dim3(TPB), 0, s, usg, runs, nseg, rofs);
+            uval = reinterpret_cast<double *>(k1);
+            k1 = nullptr;
+            usg = nullptr;
+        }
+    } else if (ok) {
+        ok = dalloc(&g0, (size_t)n, "segments") && dalloc(&g1, (size_t)n, "segments") &&
+             dalloc(&pos, (size_t)n + 1, "run positions");
+        if (ok) hipLaunchKernelGGL(k_fill_seg, dim3(grid), dim3(TPB), 0, s, dofs, nseg, n, g0);
         if (nseg > 1) {
             // by value (segment ids travel along) ...
             ok = ok && prim_sort_pairs_u64_i32(k0, k1, g0, g1, (size_t)n, 64, s);
@@ -590,36 +764,37 @@ bool exact_histogram_dev(const double *vals, int64_t n, const std::vector<int64_
             if (ok) std::swap(k0, k1);
         }
         // now k0 / g0 hold the sorted (segment, value) keys
-        if (pt.on) { hipStreamSynchronize(s); pt.mark("sorts"); }
         if (ok) {
             hipLaunchKernelGGL(k_run_flags, dim3(grid), dim3(TPB), 0, s, k0, g0, n, flag);
             hipLaunchKernelGGL(k_zero_int, dim3(1), dim3(64), 0, s, flag + n, 1);
-            ok = prim_exclusive_scan_i32_to_i64(flag, pos, (size_t)n + 1, s);
-        }
-        if (ok) {
-            ok = hipMemcpyAsync(&runs, pos + n, sizeof(int64_t), hipMemcpyDeviceToHost, s) == hipSuccess &&
+            ok = prim_exclusive_scan_i32_to_i64(flag, pos, (size_t)n + 1, s) &&
+                 hipMemcpyAsync(host, pos + n, sizeof(int64_t), hipMemcpyDeviceToHost, s) == hipSuccess &&
                  hipStreamSynchronize(s) == hipSuccess;
         }
-        pt.mark("flags+scan");
-        if (ok)
+        if (pt.on) pt.mark("sorts+flags+scan");
+        if (ok) {
+            runs = (int64_t)host[0];
             ok = dalloc(&uval, (size_t)runs + 1, "histogram values") &&
-                 dalloc(&ustart, (size_t)runs + 1, "histogram runs") && dalloc(&usg, (size_t)runs + 1, "histogram segs") &&
-                 dalloc(&cnt, (size_t)runs + 1, "histogram counts");
-        if (ok) {
-            int rgrid = grid_for(runs, num_cus() * 8);
-            hipLaunchKernelGGL(k_run_emit, dim3(grid), dim3(TPB), 0, s, k0, g0, flag, pos, n, uval, ustart, usg);
-            hipLaunchKernelGGL(k_run_counts, dim3(rgrid), dim3(TPB), 0, s, ustart, runs, n, cnt);
-            hipLaunchKernelGGL(k_seg_run_starts, dim3(rgrid), dim3(TPB), 0, s, usg, runs, nseg, rofs);
-            ok = hipGetLastError() == hipSuccess &&
-                 hipMemcpyAsync(run_ofs.data(), rofs, ((size_t)nseg + 1) * sizeof(int64_t), hipMemcpyDeviceToHost,
-                                s) == hipSuccess &&
-                 hipStreamSynchronize(s) == hipSuccess;
+                 dalloc(&ustart, (size_t)runs + 1, "histogram runs") &&
+                 dalloc(&usg, (size_t)runs + 1, "histogram segs") && dalloc(&cnt, (size_t)runs + 1, "histogram counts");
         }
-        if (!ok && !has_error()) set_error("histogram assembly failed");
+        if (ok) {
+            const int ugrid = grid_for(runs, num_cus() * 8);
+            hipLaunchKernelGGL(k_run_emit, dim3(grid), dim3(TPB), 0, s, k0, g0, flag, pos, n, uval, ustart, usg);
+            hipLaunchKernelGGL(k_run_counts, dim3(ugrid), dim3(TPB), 0, s, ustart, runs, n, cnt);
+            hipLaunchKernelGGL(k_seg_run_starts, dim3(ugrid), dim3(TPB), 0, s, usg, runs, nseg, rofs);
+        }
     }
+    if (ok)
+        ok = hipGetLastError() == hipSuccess &&
+             hipMemcpyAsync(run_ofs.data(), rofs, ((size_t)nseg + 1) * sizeof(int64_t), hipMemcpyDeviceToHost, s) ==
+                 hipSuccess &&
+             hipStreamSynchronize(s) == hipSuccess;
+    if (!ok && !has_error()) set_error("histogram assembly failed");
     pt.mark("runs");
     dfree(k0); dfree(k1); dfree(g0); dfree(g1); dfree(flag); dfree(pos); dfree(ustart); dfree(dofs);
-    dfree(usg); dfree(rofs);
+    dfree(usg); dfree(rofs); dfree(part); dfree(range); dfree(druns);
+    if (host) snapshot_give(host);
     pt.mark("free");
     if (!ok) {
         dfree(uval);
@@ -1062,21 +1237,57 @@ bool tuning_enabled()
 
 // Runs every item (parent index, vertex tuple) in one persistent launch (plus
 // an FP64 relaunch of items whose mixed-precision stop was ambiguous).
-bool run_items(const std::vector<const HostParent *> &ps, const std::vector<int> &graph_of,
-               const std::vector<int> &item_parent, const std::vector<int> &item_verts,
-               const std::vector<int64_t> &item_live, const LiftJob &J, double *d_ms, int64_t ms_stride,
-               const std::vector<int64_t> &ms_rows, hipStream_t s, LiftOut &out, unsigned long long *qkeys,
-               unsigned long long *qcounts, int64_t qcap, double qtol, const FitOutputs *fo = nullptr)
+// One run of items through k_lifted in three steps: run_prepare (host
+// only: items, queue order, DIRECTED weight patches, the parents packed
+// into a pinned staging block), run_launch (upload and launches queued on
+// the stream, no wait) and run_finish (waits; FP64 redo of ambiguous
+// mixed-precision items; per-item outputs; frees).  A RunState is filled
+// in place and not moved: the staging registrations point into it.
+struct RunState {
+    int n = 0;
+    std::vector<LiftItem> items;
+    std::vector<int> order;
+    std::vector<double> extra_ew, marks;
+    std::vector<LiftParent> meta;
+    PackedUpload pk;
+    DevPools D;
+    LiftItem *d_items = nullptr;
+    int *d_order = nullptr, *d_verts = nullptr, *d_queue = nullptr, *d_flags = nullptr, *d_iters = nullptr;
+    double *d_deltas = nullptr, *d_marks = nullptr;
+    unsigned long long *d_work = nullptr;
+    std::vector<hipEvent_t> ev;          // start / end pairs of the queued launches
+    LiftArgs a;
+    int block = 0;
+    RunState() = default;
+    RunState(const RunState &) = delete;
+    RunState &operator=(const RunState &) = delete;
+    ~RunState() { release(); }
+    void release()
+    {
+        for (hipEvent_t e : ev) (void)hipEventDestroy(e);
+        ev.clear();
+        dfree(d_queue); dfree(d_flags); dfree(d_iters); dfree(d_deltas); dfree(d_work);
+        d_queue = d_flags = d_iters = nullptr;
+        d_deltas = nullptr;
+        d_work = nullptr;
+        D.release();               // items, order, vertices and marks share its block
+        pk.release();
+    }
+};
+
+bool run_prepare(const std::vector<const HostParent *> &ps, const std::vector<int> &graph_of,
+                 const std::vector<int> &item_parent, const std::vector<int> &item_verts,
+                 const std::vector<int64_t> &item_live, const LiftJob &J, const std::vector<int64_t> &ms_rows,
+                 const FitOutputs *fo, RunState &R, LiftOut &out)
 {
     const int n = (int)item_parent.size();
+    R.n = n;
     out.val_ofs.assign((size_t)n + 1, 0);
     for (int i = 0; i < n; i++) out.val_ofs[(size_t)i + 1] = out.val_ofs[(size_t)i] + item_live[(size_t)i];
     if (n == 0) return true;
-    std::vector<LiftParent> meta;
-    DevPools D;
-    std::vector<LiftItem> items((size_t)n);
+    R.items.resize((size_t)n);
     for (int i = 0; i < n; i++) {
-        LiftItem &I = items[(size_t)i];
+        LiftItem &I = R.items[(size_t)i];
         I.parent = item_parent[(size_t)i];
         I.kind = J.kind;
         I.k = J.k;
@@ -1090,7 +1301,6 @@ bool run_items(const std::vector<const HostParent *> &ps, const std::vector<int>
     // DIRECTED delta items whose own reciprocal lookups give other weights
     // than the parent's get a patched copy of the weights (one per distinct
     // patch set), placed after the parents' weights
-    std::vector<double> extra_ew;
     if (J.kind == LIFT_DELTA && J.k > 0) {
         int64_t base = 0;
         for (const HostParent *P : ps) base += P->E;
@@ -1109,19 +1319,19 @@ bool run_items(const std::vector<const HostParent *> &ps, const std::vector<int>
             for (size_t j = 0; j < pe.size(); j++) key.push_back({pe[j], pw[j]});
             auto it = seen.find({pi, key});
             if (it != seen.end()) {
-                items[(size_t)i].ew_ofs = it->second;
+                R.items[(size_t)i].ew_ofs = it->second;
                 continue;
             }
-            int64_t at = base + (int64_t)extra_ew.size();
-            extra_ew.insert(extra_ew.end(), P.ew.begin(), P.ew.end());
-            for (size_t j = 0; j < pe.size(); j++) extra_ew[(size_t)(at - base) + (size_t)pe[j]] = pw[j];
+            int64_t at = base + (int64_t)R.extra_ew.size();
+            R.extra_ew.insert(R.extra_ew.end(), P.ew.begin(), P.ew.end());
+            for (size_t j = 0; j < pe.size(); j++) R.extra_ew[(size_t)(at - base) + (size_t)pe[j]] = pw[j];
             seen.emplace(std::make_pair(pi, std::move(key)), at);
-            items[(size_t)i].ew_ofs = at;
+            R.items[(size_t)i].ew_ofs = at;
         }
     }
     // queue order: larger parents first, items of equal size in index order
     // (a counting sort over the parents' distinct edge counts)
-    std::vector<int> order((size_t)n);
+    R.order.resize((size_t)n);
     {
         std::vector<int> sizes;
         sizes.reserve(ps.size());
@@ -1135,194 +1345,234 @@ bool run_items(const std::vector<const HostParent *> &ps, const std::vector<int>
         std::vector<int> start(sizes.size() + 1, 0);
         for (int i = 0; i < n; i++) start[(size_t)prank[(size_t)item_parent[(size_t)i]] + 1]++;
         for (size_t r = 0; r < sizes.size(); r++) start[r + 1] += start[r];
-        for (int i = 0; i < n; i++) order[(size_t)start[(size_t)prank[(size_t)item_parent[(size_t)i]]]++] = i;
+        for (int i = 0; i < n; i++) R.order[(size_t)start[(size_t)prank[(size_t)item_parent[(size_t)i]]]++] = i;
     }
-    LiftItem *d_items = nullptr;
-    int *d_order = nullptr, *d_verts = nullptr, *d_queue = nullptr, *d_flags = nullptr, *d_iters = nullptr;
-    double *d_deltas = nullptr;
-    double *d_marks = nullptr;
-    std::vector<double> marks(J.marks, J.marks + LIFT_KMAX);
-    PhaseTimer pt("  run_items");
-    ParentArrays A;
-    PackedUpload pk;
-    pack_parents(ps, graph_of, meta, A, D, pk, &extra_ew);
-    pk.add(&d_items, items);
-    pk.add(&d_order, order);
-    pk.add(&d_verts, item_verts);
-    pk.add(&d_marks, marks);
-    bool ok = pk.run(&D.blk, s) && dalloc(&d_queue, 1, "work queue") && dalloc(&d_flags, (size_t)n, "item flags") &&
-              dalloc(&out.d_vals, (size_t)out.val_ofs.back() + 1, "subgraph values");
-    if (ok && fo) ok = dalloc(&d_iters, (size_t)n, "item iterations") && dalloc(&d_deltas, (size_t)n, "item deltas");
-    unsigned long long *d_work = nullptr;
-    if (ok && lift_work_counting())
-        ok = dalloc(&d_work, 4, "work counters") &&
-             hipMemsetAsync(d_work, 0, 4 * sizeof(unsigned long long), s) == hipSuccess;
-    hipEvent_t e0 = nullptr, e1 = nullptr;
-    if (ok) ok = hipEventCreate(&e0) == hipSuccess && hipEventCreate(&e1) == hipSuccess;
-    if (pt.on) { hipStreamSynchronize(s); pt.mark("uploads+allocs"); }
-    if (ok) {
-        LiftArgs a;
-        memset(&a, 0, sizeof(a));
-        a.parents = D.parents;
-        a.items = d_items;
-        a.order = d_order;
-        a.n_items = n;
-        a.item_verts = d_verts;
-        a.marks = d_marks;
-        a.row_off = D.row_off;
-        a.adj_e = D.adj_e;
-        a.uv = D.uv;
-        a.ew = D.ew;
-        a.ioff = D.ioff;
-        a.pairs = D.pairs;
-        a.queue = d_queue;
-        a.max_it = J.max_it;
-        a.eps = J.eps;
-        a.vals = out.d_vals;
-        a.ms = d_ms;
-        a.ms_stride = ms_stride;
-        a.flags = d_flags;
-        a.qkeys = qkeys;
-        a.qcounts = qcounts;
-        a.qcap = qcap;
-        a.qtol = qtol;
-        a.parent_graph = D.parent_graph;
-        a.work = d_work;
-        if (fo) {
-            a.vnorm = fo->d_vnorm;
-            a.iters = d_iters;
-            a.deltas = d_deltas;
+    R.marks.assign(J.marks, J.marks + LIFT_KMAX);
+    PhaseTimer pt("    run_prepare");
+    pack_parents(ps, graph_of, R.meta, R.D, R.pk, &R.extra_ew);
+    pt.mark("concatenate");
+    R.pk.add(&R.d_items, R.items);
+    R.pk.add(&R.d_order, R.order);
+    R.pk.add(&R.d_verts, item_verts);
+    R.pk.add(&R.d_marks, R.marks);
+    // the arrays are in the staging block from here on
+    const bool ok = R.pk.pack();
+    pt.mark("staging");
+    return ok;
+}
+
+// Launches of the items ids[] (positions in the queue order), grouped so
+// that each launch's largest parent fits the LDS limit: the ids are in
+// order of decreasing edge count, and a group ends where the next item
+// would push the larger of the group's vertex counts past it.  Launches are
+// queued with start / end events; only a block-size calibration waits.
+bool launch_ids(RunState &R, const std::vector<const HostParent *> &ps, const std::vector<int> &item_parent,
+                const LiftJob &J, const int *d_ids, const std::vector<int> &ids, hipStream_t s, LiftOut &out)
+{
+    LiftArgs &a = R.a;
+    const size_t limit = lifted_lds_limit();
+    const size_t per_cu_bytes = lifted_lds_per_cu();
+    size_t g0 = 0;
+    while (g0 < ids.size()) {
+        int gE = 0, gN = 0;
+        size_t g1 = g0;
+        while (g1 < ids.size()) {
+            const HostParent *P = ps[(size_t)item_parent[(size_t)ids[g1]]];
+            int nE = std::max(gE, P->E), nN = std::max(gN, P->N);
+            if (lifted_lds_bytes((nE + 1) & ~1, (nN + 1) & ~1) > limit) {
+                if (g1 == g0) {
+                    set_error("a graph of " + std::to_string(P->N) + " vertices and " + std::to_string(P->E) +
+                              " edges exceeds the LDS of the batched kernel");
+                    return false;
+                }
+                break;
+            }
+            gE = nE;
+            gN = nN;
+            g1++;
         }
-        // One launch per group of items whose largest graph sizes fit the
-        // LDS limit together: the ids are in order of decreasing edge count,
-        // and a group ends where the next item would push the larger of the
-        // group's vertex counts past it.
-        const size_t limit = lifted_lds_limit();
-        const size_t per_cu_bytes = lifted_lds_per_cu();
-        auto launch_ids = [&](const int *d_ids, const std::vector<int> &ids) -> bool {
-            size_t g0 = 0;
-            while (g0 < ids.size()) {
-                int gE = 0, gN = 0;
-                size_t g1 = g0;
-                while (g1 < ids.size()) {
-                    const HostParent *P = ps[(size_t)item_parent[(size_t)ids[g1]]];
-                    int nE = std::max(gE, P->E), nN = std::max(gN, P->N);
-                    if (lifted_lds_bytes((nE + 1) & ~1, (nN + 1) & ~1) > limit) {
-                        if (g1 == g0) {
-                            set_error("a graph of " + std::to_string(P->N) + " vertices and " + std::to_string(P->E) +
-                                      " edges exceeds the LDS of the batched kernel");
-                            return false;
-                        }
-                        break;
-                    }
-                    gE = nE;
-                    gN = nN;
-                    g1++;
-                }
-                a.ecap = (gE + 1) & ~1;
-                a.ncap = (gN + 1) & ~1;
-                const size_t lds = lifted_lds_bytes(a.ecap, a.ncap);
-                // workgroups of `block` threads one CU holds at once
-                auto per_cu_of = [&](int block) {
-                    int per_cu_lds = (int)std::max<size_t>(1, per_cu_bytes / (lds + 1024));
-                    int per_cu_waves = std::max(1, 32 / (block / 64));
-                    return std::max(1, std::min(per_cu_lds, per_cu_waves));
-                };
-                // one launch of items [b0, b1) of the list with `block` threads
-                auto launch_range = [&](size_t b0, size_t b1, int block, float *ms_out) -> bool {
-                    a.order = d_ids + b0;
-                    a.n_items = (int)(b1 - b0);
-                    const int resident = std::max(1, num_cus() * per_cu_of(block) * thread_grid_share() / 100);
-                    int grid = std::max(1, std::min(a.n_items, resident));
-                    if (!(hipMemsetAsync(d_queue, 0, sizeof(int), s) == hipSuccess &&
-                          hipEventRecord(e0, s) == hipSuccess && lifted_launch(a, block, grid, s) &&
-                          hipEventRecord(e1, s) == hipSuccess && hipEventSynchronize(e1) == hipSuccess))
-                        return false;
-                    float ms = 0.f;
-                    (void)hipEventElapsedTime(&ms, e0, e1);
-                    out.device_ms += ms;
-                    if (ms_out) *ms_out = ms;
-                    return true;
-                };
-                int block = forced_block();
-                const int cls = BlockTuner::size_class(gE, J.mixed, J.kind);
-                if (!block && tuning_enabled()) block = g_tuner.lookup(cls);
-                size_t b0 = g0;
-                if (!block && tuning_enabled()) {
-                    // calibrate on the group's leading items, each candidate
-                    // on two full waves of its own resident workgroups, when
-                    // the group holds at least twice what that takes
-                    static const int cand[4] = {64, 128, 256, 512};
-                    size_t chunk[4], need = 0;
-                    for (int i = 0; i < 4; i++) {
-                        chunk[i] = 2 * (size_t)num_cus() * (size_t)per_cu_of(cand[i]);
-                        need += chunk[i];
-                    }
-                    if (g1 - g0 >= 2 * need) {
-                        double best_per_item = 0.0;
-                        int best = 0;
-                        for (int i = 0; i < 4; i++) {
-                            float ms = 0.f;
-                            if (!launch_range(b0, b0 + chunk[i], cand[i], &ms)) return false;
-                            double per_item = ms / (double)chunk[i];
-                            if (!best || per_item < best_per_item) { best = cand[i]; best_per_item = per_item; }
-                            b0 += chunk[i];
-                        }
-                        g_tuner.store(cls, best);
-                        block = best;
-                    }
-                }
-                if (!block) block = choose_block(gE);
-                out.block = std::max(out.block, block);
-                if (b0 < g1 && !launch_range(b0, g1, block, nullptr)) return false;
-                g0 = g1;
+        a.ecap = (gE + 1) & ~1;
+        a.ncap = (gN + 1) & ~1;
+        const size_t lds = lifted_lds_bytes(a.ecap, a.ncap);
+        // workgroups of `block` threads one CU holds at once
+        auto per_cu_of = [&](int block) {
+            int per_cu_lds = (int)std::max<size_t>(1, per_cu_bytes / (lds + 1024));
+            int per_cu_waves = std::max(1, 32 / (block / 64));
+            return std::max(1, std::min(per_cu_lds, per_cu_waves));
+        };
+        // one launch of items [b0, b1) of the list with `block` threads;
+        // with ms_out it waits and returns the launch's time
+        auto launch_range = [&](size_t b0, size_t b1, int block, float *ms_out) -> bool {
+            a.order = d_ids + b0;
+            a.n_items = (int)(b1 - b0);
+            const int resident = std::max(1, num_cus() * per_cu_of(block) * thread_grid_share() / 100);
+            int grid = std::max(1, std::min(a.n_items, resident));
+            hipEvent_t e0 = nullptr, e1 = nullptr;
+            if (hipEventCreate(&e0) != hipSuccess) return false;
+            if (hipEventCreate(&e1) != hipSuccess) {
+                (void)hipEventDestroy(e0);
+                return false;
+            }
+            R.ev.push_back(e0);
+            R.ev.push_back(e1);
+            if (!(hipMemsetAsync(R.d_queue, 0, sizeof(int), s) == hipSuccess && hipEventRecord(e0, s) == hipSuccess &&
+                  lifted_launch(a, block, grid, s) && hipEventRecord(e1, s) == hipSuccess))
+                return false;
+            if (ms_out) {
+                if (hipEventSynchronize(e1) != hipSuccess) return false;
+                (void)hipEventElapsedTime(ms_out, e0, e1);
             }
             return true;
         };
-        ok = hipMemsetAsync(d_flags, 0, (size_t)n * sizeof(int), s) == hipSuccess && launch_ids(d_order, order);
-        pt.mark("launches");
-        // FP64 redo of ambiguous mixed-precision items
-        if (ok && J.mixed) {
-            std::vector<int> flags((size_t)n);
-            ok = hipMemcpyAsync(flags.data(), d_flags, (size_t)n * sizeof(int), hipMemcpyDeviceToHost, s) ==
-                     hipSuccess && hipStreamSynchronize(s) == hipSuccess;
-            std::vector<int> redo;
-            for (int qi = 0; qi < n && ok; qi++)        // keeps the decreasing-size order
-                if (flags[(size_t)order[(size_t)qi]]) redo.push_back(order[(size_t)qi]);
-            if (ok && !redo.empty()) {
-                out.reruns = (int)redo.size();
-                std::vector<LiftItem> items2 = items;
-                for (int i : redo) items2[(size_t)i].mixed = 0;
-                int *d_redo = nullptr;
-                // the quantised counts of the redone items' mixed values come
-                // back out first; the FP64 redo counts their final values
-                ok = (!qkeys || unquantise(out.d_vals, out.val_ofs, redo, item_parent, graph_of, J.eps, qtol, qkeys,
-                                           qcounts, qcap, s)) &&
-                     hipMemcpyAsync(d_items, items2.data(), items2.size() * sizeof(LiftItem), hipMemcpyHostToDevice,
-                                    s) == hipSuccess &&
-                     upload(&d_redo, redo, s, "redo list");
-                if (ok) ok = launch_ids(d_redo, redo) && hipStreamSynchronize(s) == hipSuccess;
-                dfree(d_redo);
+        int block = forced_block();
+        const int cls = BlockTuner::size_class(gE, J.mixed, J.kind);
+        if (!block && tuning_enabled()) block = g_tuner.lookup(cls);
+        size_t b0 = g0;
+        if (!block && tuning_enabled()) {
+            // calibrate on the group's leading items, each candidate on two
+            // full waves of its own resident workgroups, when the group holds
+            // at least twice what that takes
+            static const int cand[4] = {64, 128, 256, 512};
+            size_t chunk[4], need = 0;
+            for (int i = 0; i < 4; i++) {
+                chunk[i] = 2 * (size_t)num_cus() * (size_t)per_cu_of(cand[i]);
+                need += chunk[i];
+            }
+            if (g1 - g0 >= 2 * need) {
+                double best_per_item = 0.0;
+                int best = 0;
+                for (int i = 0; i < 4; i++) {
+                    float ms = 0.f;
+                    if (!launch_range(b0, b0 + chunk[i], cand[i], &ms)) return false;
+                    double per_item = ms / (double)chunk[i];
+                    if (!best || per_item < best_per_item) { best = cand[i]; best_per_item = per_item; }
+                    b0 += chunk[i];
+                }
+                g_tuner.store(cls, best);
+                block = best;
             }
         }
+        if (!block) block = choose_block(gE);
+        out.block = std::max(out.block, block);
+        if (b0 < g1 && !launch_range(b0, g1, block, nullptr)) return false;
+        g0 = g1;
     }
-    if (ok && fo) {
+    return true;
+}
+
+bool run_launch(RunState &R, const std::vector<const HostParent *> &ps, const std::vector<int> &item_parent,
+                const LiftJob &J, double *d_ms, int64_t ms_stride, hipStream_t s, LiftOut &out,
+                unsigned long long *qkeys, unsigned long long *qcounts, int64_t qcap, double qtol,
+                const FitOutputs *fo)
+{
+    const int n = R.n;
+    if (n == 0) return true;
+    bool ok = R.pk.upload(&R.D.blk, s) && dalloc(&R.d_queue, 1, "work queue") &&
+              dalloc(&R.d_flags, (size_t)n, "item flags") &&
+              dalloc(&out.d_vals, (size_t)out.val_ofs.back() + 1, "subgraph values");
+    if (ok && fo) ok = dalloc(&R.d_iters, (size_t)n, "item iterations") && dalloc(&R.d_deltas, (size_t)n, "item deltas");
+    if (ok && lift_work_counting())
+        ok = dalloc(&R.d_work, 4, "work counters") &&
+             hipMemsetAsync(R.d_work, 0, 4 * sizeof(unsigned long long), s) == hipSuccess;
+    if (!ok) return false;
+    LiftArgs &a = R.a;
+    memset(&a, 0, sizeof(a));
+    a.parents = R.D.parents;
+    a.items = R.d_items;
+    a.order = R.d_order;
+    a.n_items = n;
+    a.item_verts = R.d_verts;
+    a.marks = R.d_marks;
+    a.row_off = R.D.row_off;
+    a.adj_e = R.D.adj_e;
+    a.uv = R.D.uv;
+    a.ew = R.D.ew;
+    a.ioff = R.D.ioff;
+    a.pairs = R.D.pairs;
+    a.queue = R.d_queue;
+    a.max_it = J.max_it;
+    a.eps = J.eps;
+    a.vals = out.d_vals;
+    a.ms = d_ms;
+    a.ms_stride = ms_stride;
+    a.flags = R.d_flags;
+    a.qkeys = qkeys;
+    a.qcounts = qcounts;
+    a.qcap = qcap;
+    a.qtol = qtol;
+    a.parent_graph = R.D.parent_graph;
+    a.work = R.d_work;
+    if (fo) {
+        a.vnorm = fo->d_vnorm;
+        a.iters = R.d_iters;
+        a.deltas = R.d_deltas;
+    }
+    return hipMemsetAsync(R.d_flags, 0, (size_t)n * sizeof(int), s) == hipSuccess &&
+           launch_ids(R, ps, item_parent, J, R.d_order, R.order, s, out);
+}
+
+bool run_finish(RunState &R, const std::vector<const HostParent *> &ps, const std::vector<int> &graph_of,
+                const std::vector<int> &item_parent, const LiftJob &J, hipStream_t s, LiftOut &out,
+                unsigned long long *qkeys, unsigned long long *qcounts, int64_t qcap, double qtol,
+                const FitOutputs *fo, bool ok)
+{
+    const int n = R.n;
+    PhaseTimer pt("  run_items");
+    // (the end of the last launch: later work may already be queued on s)
+    if (ok && n > 0)
+        ok = (R.ev.empty() ? hipStreamSynchronize(s) : hipEventSynchronize(R.ev.back())) == hipSuccess;
+    // launch times from the queued events
+    auto add_times = [&] {
+        for (size_t i = 0; i + 1 < R.ev.size(); i += 2) {
+            float ms = 0.f;
+            if (hipEventElapsedTime(&ms, R.ev[i], R.ev[i + 1]) == hipSuccess) out.device_ms += ms;
+        }
+        for (hipEvent_t e : R.ev) (void)hipEventDestroy(e);
+        R.ev.clear();
+    };
+    if (ok) add_times();
+    pt.mark("launches");
+    // FP64 redo of ambiguous mixed-precision items
+    if (ok && n > 0 && J.mixed) {
+        std::vector<int> flags((size_t)n);
+        ok = hipMemcpyAsync(flags.data(), R.d_flags, (size_t)n * sizeof(int), hipMemcpyDeviceToHost, s) ==
+                 hipSuccess && hipStreamSynchronize(s) == hipSuccess;
+        std::vector<int> redo;
+        for (int qi = 0; qi < n && ok; qi++)        // keeps the decreasing-size order
+            if (flags[(size_t)R.order[(size_t)qi]]) redo.push_back(R.order[(size_t)qi]);
+        if (ok && !redo.empty()) {
+            out.reruns = (int)redo.size();
+            std::vector<LiftItem> items2 = R.items;
+            for (int i : redo) items2[(size_t)i].mixed = 0;
+            int *d_redo = nullptr;
+            // the quantised counts of the redone items' mixed values come
+            // back out first; the FP64 redo counts their final values
+            ok = (!qkeys || unquantise(out.d_vals, out.val_ofs, redo, item_parent, graph_of, J.eps, qtol, qkeys,
+                                       qcounts, qcap, s)) &&
+                 hipMemcpyAsync(R.d_items, items2.data(), items2.size() * sizeof(LiftItem), hipMemcpyHostToDevice,
+                                s) == hipSuccess &&
+                 upload(&d_redo, redo, s, "redo list");
+            if (ok) ok = launch_ids(R, ps, item_parent, J, d_redo, redo, s, out) && hipStreamSynchronize(s) == hipSuccess;
+            if (ok) add_times();
+            dfree(d_redo);
+        }
+    }
+    if (ok && n > 0 && fo) {
         fo->iters->resize((size_t)n);
         fo->deltas->resize((size_t)n);
-        ok = hipMemcpyAsync(fo->iters->data(), d_iters, (size_t)n * sizeof(int), hipMemcpyDeviceToHost, s) ==
+        ok = hipMemcpyAsync(fo->iters->data(), R.d_iters, (size_t)n * sizeof(int), hipMemcpyDeviceToHost, s) ==
                  hipSuccess &&
-             hipMemcpyAsync(fo->deltas->data(), d_deltas, (size_t)n * sizeof(double), hipMemcpyDeviceToHost, s) ==
+             hipMemcpyAsync(fo->deltas->data(), R.d_deltas, (size_t)n * sizeof(double), hipMemcpyDeviceToHost, s) ==
                  hipSuccess &&
              hipStreamSynchronize(s) == hipSuccess;
     }
-    if (ok && d_work)
-        ok = hipMemcpyAsync(out.work, d_work, sizeof(out.work), hipMemcpyDeviceToHost, s) == hipSuccess &&
+    if (ok && R.d_work)
+        ok = hipMemcpyAsync(out.work, R.d_work, sizeof(out.work), hipMemcpyDeviceToHost, s) == hipSuccess &&
              hipStreamSynchronize(s) == hipSuccess;
-    if (e0) (void)hipEventDestroy(e0);
-    if (e1) (void)hipEventDestroy(e1);
     pt.mark("reruns");
-    dfree(d_queue); dfree(d_flags); dfree(d_iters); dfree(d_deltas); dfree(d_work);
-    D.release();                   // items, order, vertices and marks share its block
+    // (on a failure the stream may still hold work that reads these blocks)
+    if (!ok) (void)hipStreamSynchronize(s);
+    R.release();
     pt.mark("free");
     if (!ok) {
         if (!has_error()) set_error("lifted launch failed");
@@ -1330,6 +1580,20 @@ bool run_items(const std::vector<const HostParent *> &ps, const std::vector<int>
         out.d_vals = nullptr;
     }
     return ok;
+}
+
+bool run_items(const std::vector<const HostParent *> &ps, const std::vector<int> &graph_of,
+               const std::vector<int> &item_parent, const std::vector<int> &item_verts,
+               const std::vector<int64_t> &item_live, const LiftJob &J, double *d_ms, int64_t ms_stride,
+               const std::vector<int64_t> &ms_rows, hipStream_t s, LiftOut &out, unsigned long long *qkeys,
+               unsigned long long *qcounts, int64_t qcap, double qtol, const FitOutputs *fo = nullptr)
+{
+    RunState R;
+    PhaseTimer pt("  run_items");
+    bool ok = run_prepare(ps, graph_of, item_parent, item_verts, item_live, J, ms_rows, fo, R, out);
+    pt.mark("prepare");
+    ok = ok && run_launch(R, ps, item_parent, J, d_ms, ms_stride, s, out, qkeys, qcounts, qcap, qtol, fo);
+    return run_finish(R, ps, graph_of, item_parent, J, s, out, qkeys, qcounts, qcap, qtol, fo, ok);
 }
 
 }  // namespace
@@ -1837,6 +2101,111 @@ namespace dh {
 
 static thread_local dress_hip_lifted_stats_t t_lstats;
 
+namespace {
+
+// Host parents of a batch, built by worker threads in graph order while the
+// caller runs earlier graphs on the device.  wait_built(end) builds graphs
+// below `end` that no worker has taken yet and returns once every graph
+// below `end` is built.  The destructor lets the workers finish the graph in
+// hand and joins them.
+class ParentPool {
+public:
+    ParentPool(int n, int threads, std::function<void(int)> fn) : n_(n), fn_(std::move(fn)), done_((size_t)n, 0)
+    {
+        for (int t = 0; t < threads; t++) pool_.emplace_back([this] { work(); });
+    }
+    ~ParentPool()
+    {
+        stop_ = true;
+        for (auto &t : pool_) t.join();
+    }
+    ParentPool(const ParentPool &) = delete;
+    ParentPool &operator=(const ParentPool &) = delete;
+    void wait_built(int end)
+    {
+        for (int gi = next_.load(); gi < end; gi = next_.load())
+            if (next_.compare_exchange_weak(gi, gi + 1)) build(gi);
+        std::unique_lock<std::mutex> lk(mu_);
+        cv_.wait(lk, [&] { return built_ >= end; });
+    }
+
+private:
+    void work()
+    {
+        while (!stop_) {
+            const int gi = next_.fetch_add(1);
+            if (gi >= n_) return;
+            build(gi);
+        }
+    }
+    void build(int gi)
+    {
+        fn_(gi);
+        std::lock_guard<std::mutex> lk(mu_);
+        done_[(size_t)gi] = 1;
+        while (built_ < n_ && done_[(size_t)built_]) built_++;
+        cv_.notify_all();
+    }
+    const int n_;
+    std::function<void(int)> fn_;
+    std::vector<char> done_;
+    int built_ = 0;
+    std::atomic<int> next_{0};
+    std::atomic<bool> stop_{false};
+    std::mutex mu_;
+    std::condition_variable cv_;
+    std::vector<std::thread> pool_;
+};
+
+// upper bound of the live values of one graph's items (every edge of every
+// subgraph), -1 when it does not fit in 62 bits
+int64_t batch_value_bound(int kind, int N, int64_t E, int k)
+{
+    if (k == 0) return E;
+    if (k >= N || E == 0) return 0;
+    const int64_t c = kind == LIFT_DELTA ? dress_delta_binom(N, k) : dress_nabla_perm_count(N, k);
+    if (c <= 0) return -1;
+    if (c > ((int64_t)1 << 62) / E) return -1;
+    return c * E;
+}
+
+// values per group of a pipelined batch (DRESS_HIP_BATCH_GROUP_VALUES
+// overrides; 0 keeps a batch in one group)
+int64_t batch_group_values()
+{
+    if (const char *e = getenv("DRESS_HIP_BATCH_GROUP_VALUES")) return (int64_t)atoll(e);
+    return 1800000;
+}
+
+// consecutive graphs in at most max_groups groups of about per_group values
+// each: boundaries b[0] = 0 < b[1] < ... < b[G] = n (one group when the
+// values fit in one, or per_group <= 0).  With three groups or more the
+// first and the last hold half a group each, so that the device starts
+// sooner and the last download is shorter.
+std::vector<int> batch_groups(const std::vector<int64_t> &bound, int64_t per_group, int max_groups)
+{
+    const int n = (int)bound.size();
+    int64_t total = 0;
+    for (int64_t x : bound) total += x;
+    int G = 1;
+    if (per_group > 0 && total > per_group) G = (int)std::min<int64_t>(max_groups, (total + per_group - 1) / per_group);
+    G = std::max(1, std::min(G, n));
+    // group g ends where the values reach ends[g] / ends[G] of the total:
+    // shares 1, 2, ..., 2, 1 with three groups or more, else equal
+    std::vector<int64_t> ends((size_t)G + 1, 0);
+    for (int g = 1; g <= G; g++) ends[(size_t)g] = ends[(size_t)g - 1] + ((G >= 3 && (g == 1 || g == G)) ? 1 : 2);
+    std::vector<int> b{0};
+    int64_t acc = 0;
+    for (int i = 0; i + 1 < n && (int)b.size() < G; i++) {
+        acc += bound[(size_t)i];
+        if ((__int128)acc * ends[(size_t)G] >= (__int128)total * ends[b.size()]) b.push_back(i + 1);
+    }
+    b.push_back(n);
+    return b;
+}
+
+}  // namespace
+
 void record_lifted(int reruns, double device_ms, int64_t subgraphs, int64_t values)
 {
     t_lstats.subgraphs = subgraphs;
@@ -2014,6 +2383,13 @@ int dress_hip_lifted_batch(int kind, int k, int n_graphs, const int *N, const in
                               ("dress_hip_lifted_batch graph " + std::to_string(gi)).c_str()))
             return -1;
     }
+    for (int gi = 0; gi < n_graphs; gi++) {
+        const int64_t Eg = edge_offset[gi + 1] - edge_offset[gi];
+        if (Eg > INT32_MAX || !lds_eligible_size(N[gi], (int)Eg)) {
+            set_error("graph " + std::to_string(gi) + " is too large for the batched kernel");
+            return -1;
+        }
+    }
     auto t0 = std::chrono::steady_clock::now();
     PhaseTimer pt("dress_hip_lifted_batch");
     BatchScope scope;
@@ -2023,8 +2399,37 @@ int dress_hip_lifted_batch(int kind, int k, int n_graphs, const int *N, const in
         set_error("stream creation failed");
         return -1;
     }
-    // parents and items: each graph on its own (host worker threads), then
-    // concatenated in graph order
+    // the floor(d / eps) tables only on request (qtol >= 0)
+    const bool quant = qtol >= 0.0;
+    // Groups of consecutive graphs, run one after the other: while the
+    // device fits a group, host threads build the parents of later groups
+    // and a copy stream downloads the histograms of earlier ones.  The
+    // result arrays are then sized by the bound on the values before the
+    // histograms exist.  Batches with quantised tables, inside a server
+    // worker, or of one group's size run as one group.
+    std::vector<int64_t> bound((size_t)n_graphs);
+    bool bounded = true;
+    int64_t cap = 0;
+    for (int gi = 0; gi < n_graphs; gi++) {
+        bound[(size_t)gi] = batch_value_bound(kind, N[gi], edge_offset[gi + 1] - edge_offset[gi], k);
+        if (bound[(size_t)gi] < 0 || cap > ((int64_t)1 << 62) - bound[(size_t)gi]) bounded = false;
+        else cap += bound[(size_t)gi];
+    }
+    // several groups: group g + 1 is launched on s before group g's
+    // histogram runs on hs and its download on cs
+    hipStream_t cs = nullptr, hs = s;
+    std::vector<int> gb{0, n_graphs};
+    if (!quant && bounded && scope.on) {
+        std::vector<int> b = batch_groups(bound, batch_group_values(), 8);
+        hipStream_t c = batch_side_stream(0), h = batch_side_stream(1);
+        if (b.size() > 2 && c && h) {
+            gb = b;
+            cs = c;
+            hs = h;
+        }
+    }
+    const int G = (int)gb.size() - 1;
+    // parents and items of each graph (host worker threads, graph order)
     std::vector<HostParent> parents((size_t)n_graphs);
     std::vector<std::vector<int>> g_verts((size_t)n_graphs);
     std::vector<std::vector<int64_t>> g_live((size_t)n_graphs);
@@ -2049,35 +2454,6 @@ int dress_hip_lifted_batch(int kind, int k, int n_graphs, const int *N, const in
         g_live[(size_t)gi] = kind == LIFT_DELTA ? delta_live_counts(&tmp, g_verts[(size_t)gi], k, sel.size())
                                                 : std::vector<int64_t>(sel.size(), (int64_t)Eg);
     };
-    {
-        const int nt = std::max(1, std::min({n_graphs / 32, (int)std::thread::hardware_concurrency(), 16}));
-        std::atomic<int> next{0};
-        auto worker = [&] {
-            for (int gi = next++; gi < n_graphs; gi = next++) build_one(gi);
-        };
-        std::vector<std::thread> pool;
-        for (int t = 1; t < nt; t++) pool.emplace_back(worker);
-        worker();
-        for (auto &th : pool) th.join();
-    }
-    std::vector<int> graph_of((size_t)n_graphs), item_parent, item_verts;
-    std::vector<int64_t> item_live, ms_rows;
-    for (int gi = 0; gi < n_graphs; gi++) {
-        graph_of[(size_t)gi] = gi;
-        if (!lds_eligible(parents[(size_t)gi])) {
-            set_error("graph " + std::to_string(gi) + " is too large for the batched kernel");
-            if (own_stream) (void)hipStreamDestroy(s);
-            return -1;
-        }
-        for (int64_t x : g_live[(size_t)gi]) {
-            item_parent.push_back(gi);
-            item_live.push_back(x);
-        }
-        item_verts.insert(item_verts.end(), g_verts[(size_t)gi].begin(), g_verts[(size_t)gi].end());
-    }
-    pt.mark("parents+items");
-    const int kk = std::max(k, 0);
-    if (k == 0) item_verts.assign(item_parent.size() * (size_t)kk, 0);
     LiftJob J;
     J.kind = kind;
     J.k = (k == 0) ? 0 : k;
@@ -2086,102 +2462,234 @@ int dress_hip_lifted_batch(int kind, int k, int n_graphs, const int *N, const in
     J.mixed = precision == DRESS_HIP_FP32_FP64;
     memset(J.marks, 0, sizeof(J.marks));
     if (kind == LIFT_NABLA) marker_weights(J.marks, k);
-    std::vector<const HostParent *> ps;
-    for (auto &P : parents) ps.push_back(&P);
-    int64_t total_vals = 0;
-    for (int64_t x : item_live) total_vals += x;
-    // the floor(d / eps) tables only on request (qtol >= 0)
-    const bool quant = qtol >= 0.0;
-    int64_t qcap = 1024;
-    while (quant && qcap < 2 * total_vals) qcap <<= 1;
-    unsigned long long *qk = nullptr, *qc = nullptr;
-    LiftOut lo;
-    bool ok = !quant || (dalloc(&qk, (size_t)qcap, "quantised table") && dalloc(&qc, (size_t)qcap, "quantised table"));
-    if (ok) {
-        if (quant) hipLaunchKernelGGL(k_qtable_clear, dim3(grid_for(qcap, 4096)), dim3(TPB), 0, s, qk, qc, qcap);
-        ok = run_items(ps, graph_of, item_parent, item_verts, item_live, J, nullptr, 0, ms_rows, s, lo, qk, qc,
-                       quant ? qcap : 0, quant ? qtol : 0.0);
-    }
-    pt.mark("run_items");
-    // exact histogram per graph: items of a graph are contiguous
-    std::vector<int64_t> seg_ofs((size_t)n_graphs + 1, 0);
-    if (ok) {
-        size_t it = 0;
-        for (int gi = 0; gi < n_graphs; gi++) {
-            while (it < item_parent.size() && item_parent[it] == gi) it++;
-            seg_ofs[(size_t)gi + 1] = lo.val_ofs[it];
-        }
-    }
-    double *hv = nullptr;
-    int64_t *hc = nullptr, runs = 0;
-    std::vector<int64_t> run_ofs;
-    if (ok && n_graphs > 0) ok = exact_histogram_dev(lo.d_vals, lo.val_ofs.back(), seg_ofs, s, &hv, &hc, &runs, run_ofs);
-    pt.mark("exact_histogram");
-    if (ok) {
-        out->hist_value = (double *)pinned_alloc(((size_t)runs + 1) * sizeof(double));
-        out->hist_count = (int64_t *)pinned_alloc(((size_t)runs + 1) * sizeof(int64_t));
+    out->n_graphs = n_graphs;
+    out->hist_offset = (int64_t *)malloc(((size_t)n_graphs + 1) * sizeof(int64_t));
+    out->qhist_offset = (int64_t *)malloc(((size_t)n_graphs + 1) * sizeof(int64_t));
+    out->qbound_offset = (int64_t *)malloc(((size_t)n_graphs + 1) * sizeof(int64_t));
+    bool ok = out->hist_offset && out->qhist_offset && out->qbound_offset;
+    if (ok && G > 1) {
+        out->hist_value = (double *)pinned_alloc(((size_t)cap + 1) * sizeof(double));
+        out->hist_count = (int64_t *)pinned_alloc(((size_t)cap + 1) * sizeof(int64_t));
         ok = out->hist_value && out->hist_count;
-        if (!ok) set_error("host allocation of batch results failed");
     }
+    if (!ok) set_error("host allocation of batch results failed");
+    auto d2h = [](void *dst, const void *src, size_t bytes, hipStream_t st) {
+        return bytes == 0 || hipMemcpyAsync(dst, src, bytes, hipMemcpyDeviceToHost, st) == hipSuccess;
+    };
+    std::vector<void *> held;            // device blocks that downloads still read
+    unsigned long long *qk = nullptr, *qc = nullptr;
+    int64_t qcap = 1024, runs_total = 0, subgraphs = 0, values = 0;
+    int reruns = 0;
+    double device_ms = 0.0;
+    unsigned long long work[4] = {0, 0, 0, 0};
+    // the run of each group prepared on the host ahead of the device: by a
+    // planner thread when there are several groups
+    struct GroupPlan {
+        RunState R;
+        LiftOut lo;
+        std::vector<const HostParent *> ps;
+        std::vector<int> graph_of, item_parent, item_verts;
+        std::vector<int64_t> item_live, ms_rows;
+        bool ok = false;
+        std::string err;
+    };
+    std::vector<std::unique_ptr<GroupPlan>> plans((size_t)G);
+    {
+        // host threads building parents: with several groups only the first
+        // group's wait for them, and 8 leave the calling thread room (the
+        // 512-graph delta^1 batch: 5.5 ms a call against 5.9 with 16, whose
+        // times also spread twice as wide); DRESS_HIP_BATCH_THREADS overrides
+        int nt = std::max(1, std::min({n_graphs / 32, (int)std::thread::hardware_concurrency(), G > 1 ? 8 : 16}));
+        if (const char *e = getenv("DRESS_HIP_BATCH_THREADS")) nt = std::max(1, atoi(e));
+        ParentPool pool(n_graphs, nt - 1, build_one);
+        auto plan_group = [&](int gr) {
+            PhaseTimer ppt("  plan_group");
+            std::unique_ptr<GroupPlan> P(new GroupPlan());
+            const int ga = gb[(size_t)gr], gz = gb[(size_t)gr + 1];
+            pool.wait_built(gz);
+            ppt.mark("parents built");
+            for (int gi = ga; gi < gz; gi++) {
+                P->graph_of.push_back(gi);
+                P->ps.push_back(&parents[(size_t)gi]);
+                for (int64_t x : g_live[(size_t)gi]) {
+                    P->item_parent.push_back(gi - ga);
+                    P->item_live.push_back(x);
+                }
+                P->item_verts.insert(P->item_verts.end(), g_verts[(size_t)gi].begin(), g_verts[(size_t)gi].end());
+            }
+            ppt.mark("items");
+            P->ok = run_prepare(P->ps, P->graph_of, P->item_parent, P->item_verts, P->item_live, J, P->ms_rows,
+                                nullptr, P->R, P->lo);
+            ppt.mark("prepared");
+            if (!P->ok) P->err = dress_hip_last_error();
+            return P;
+        };
+        std::mutex pm;
+        std::condition_variable pcv;
+        int planned = 0;
+        std::atomic<bool> stop_planning{false};
+        std::thread planner;
+        if (G > 1)
+            planner = std::thread([&] {
+                for (int gr = 0; gr < G && !stop_planning; gr++) {
+                    std::unique_ptr<GroupPlan> P = plan_group(gr);
+                    std::lock_guard<std::mutex> lk(pm);
+                    plans[(size_t)gr] = std::move(P);
+                    planned = gr + 1;
+                    pcv.notify_all();
+                }
+            });
+        pt.mark("threads started");
+        // group gr's run queued on s once its plan is ready
+        auto launch_group = [&](int gr) -> bool {
+            if (G > 1) {
+                std::unique_lock<std::mutex> lk(pm);
+                pcv.wait(lk, [&] { return planned > gr; });
+            } else {
+                plans[0] = plan_group(0);
+            }
+            GroupPlan &P = *plans[(size_t)gr];
+            if (gr == 0) pt.mark("first group prepared");
+            if (!P.ok) {
+                set_error(P.err.empty() ? std::string("batch preparation failed") : P.err);
+                return false;
+            }
+            if (quant) {
+                while (qcap < 2 * P.lo.val_ofs.back()) qcap <<= 1;
+                if (!(dalloc(&qk, (size_t)qcap, "quantised table") && dalloc(&qc, (size_t)qcap, "quantised table")))
+                    return false;
+                hipLaunchKernelGGL(k_qtable_clear, dim3(grid_for(qcap, 4096)), dim3(TPB), 0, s, qk, qc, qcap);
+            }
+            return run_launch(P.R, P.ps, P.item_parent, J, nullptr, 0, s, P.lo, qk, qc, quant ? qcap : 0,
+                              quant ? qtol : 0.0, nullptr);
+        };
+        bool launched = ok && launch_group(0);
+        for (int gr = 0; ok && gr < G; gr++) {
+            const int ga = gb[(size_t)gr], gz = gb[(size_t)gr + 1];
+            GroupPlan &P = *plans[(size_t)gr];
+            // the next group goes into the queue behind this one
+            bool next = gr + 1 < G && launched && launch_group(gr + 1);
+            ok = run_finish(P.R, P.ps, P.graph_of, P.item_parent, J, s, P.lo, qk, qc, quant ? qcap : 0,
+                            quant ? qtol : 0.0, nullptr, launched);
+            launched = next;
+            if (ok && gr + 1 < G && !next) ok = false;
+            LiftOut &lo = P.lo;
+            // exact histogram per graph: the items of a graph are contiguous
+            std::vector<int64_t> seg_ofs((size_t)(gz - ga) + 1, 0);
+            if (ok) {
+                size_t it = 0;
+                for (int gi = ga; gi < gz; gi++) {
+                    while (it < P.item_parent.size() && P.item_parent[it] == gi - ga) it++;
+                    seg_ofs[(size_t)(gi - ga) + 1] = lo.val_ofs[it];
+                }
+            }
+            double *hv = nullptr;
+            int64_t *hc = nullptr, runs = 0;
+            std::vector<int64_t> run_ofs((size_t)(gz - ga) + 1, 0);
+            if (ok && gz > ga) ok = exact_histogram_dev(lo.d_vals, lo.val_ofs.back(), seg_ofs, hs, &hv, &hc, &runs, run_ofs);
+            if (gr == 0) pt.mark("first group on the device");
+            if (ok && G == 1) {
+                out->hist_value = (double *)pinned_alloc(((size_t)runs + 1) * sizeof(double));
+                out->hist_count = (int64_t *)pinned_alloc(((size_t)runs + 1) * sizeof(int64_t));
+                ok = out->hist_value && out->hist_count;
+                if (!ok) set_error("host allocation of batch results failed");
+            } else if (ok && runs_total + runs > cap) {
+                set_error("batch histogram larger than its bound");
+                ok = false;
+            }
+            if (ok) {
+                for (int gi = ga; gi < gz; gi++) out->hist_offset[gi] = runs_total + run_ofs[(size_t)(gi - ga)];
+                ok = d2h(out->hist_value + runs_total, hv, (size_t)runs * sizeof(double), cs ? cs : s) &&
+                     d2h(out->hist_count + runs_total, hc, (size_t)runs * sizeof(int64_t), cs ? cs : s);
+                if (!ok) set_error("batch result download failed");
+            }
+            held.push_back(hv);
+            held.push_back(hc);
+            if (!ok) break;
+            dfree(lo.d_vals);
+            lo.d_vals = nullptr;
+            runs_total += runs;
+            subgraphs += (int64_t)P.item_parent.size();
+            values += lo.val_ofs.back();
+            reruns += lo.reruns;
+            device_ms += lo.device_ms;
+            for (int i = 0; i < 4; i++) work[i] += lo.work[i];
+            plans[(size_t)gr].reset();
+            for (int gi = ga; gi < gz; gi++) {
+                parents[(size_t)gi] = HostParent();
+                std::vector<int>().swap(g_verts[(size_t)gi]);
+            }
+        }
+        if (!ok) {
+            // queued work may still read the groups' blocks
+            (void)hipStreamSynchronize(s);
+            if (hs != s) (void)hipStreamSynchronize(hs);
+            if (cs) (void)hipStreamSynchronize(cs);
+        }
+        stop_planning = true;
+        if (planner.joinable()) planner.join();
+        for (auto &P : plans)
+            if (P) {
+                dfree(P->lo.d_vals);
+                P->lo.d_vals = nullptr;
+            }
+        plans.clear();
+    }
+    pt.mark("groups");
+    if (ok) out->hist_offset[n_graphs] = runs_total;
     int64_t nq = 0, nb = 0;
     int64_t *qk2 = nullptr, *qc2 = nullptr, *bk2 = nullptr, *bc2 = nullptr, *qofs = nullptr, *bofs = nullptr;
     if (ok && quant) ok = qtable_split(qk, qc, qcap, n_graphs, s, &qk2, &qc2, &bk2, &bc2, &qofs, &bofs, &nq, &nb);
     pt.mark("qtable_split");
     if (pt.on)
-        fprintf(stderr, "dress_hip_lifted_batch: %lld values, %lld exact pairs, %lld bins, %lld boundary bins\n",
-                (long long)lo.val_ofs.back(), (long long)runs, (long long)nq, (long long)nb);
-    out->n_graphs = n_graphs;
+        fprintf(stderr, "dress_hip_lifted_batch: %d groups, %lld values, %lld exact pairs, %lld bins, %lld boundary bins\n",
+                G, (long long)values, (long long)runs_total, (long long)nq, (long long)nb);
     if (ok) {
-        out->hist_offset = (int64_t *)malloc(((size_t)n_graphs + 1) * sizeof(int64_t));
-        out->qhist_offset = (int64_t *)malloc(((size_t)n_graphs + 1) * sizeof(int64_t));
-        out->qbound_offset = (int64_t *)malloc(((size_t)n_graphs + 1) * sizeof(int64_t));
         out->qhist_key = (int64_t *)pinned_alloc(((size_t)nq + 1) * sizeof(int64_t));
         out->qhist_count = (int64_t *)pinned_alloc(((size_t)nq + 1) * sizeof(int64_t));
         out->qbound_key = (int64_t *)pinned_alloc(((size_t)nb + 1) * sizeof(int64_t));
         out->qbound_count = (int64_t *)pinned_alloc(((size_t)nb + 1) * sizeof(int64_t));
-        if (!out->hist_offset || !out->qhist_offset || !out->qbound_offset ||
-            !out->qhist_key || !out->qhist_count || !out->qbound_key || !out->qbound_count) {
+        if (!out->qhist_key || !out->qhist_count || !out->qbound_key || !out->qbound_count) {
             set_error("host allocation of batch results failed");
             ok = false;
         }
     }
     if (ok) {
-        for (int gi = 0; gi <= n_graphs; gi++) out->hist_offset[gi] = n_graphs > 0 ? run_ofs[(size_t)gi] : 0;
         if (!quant)
             for (int gi = 0; gi <= n_graphs; gi++) out->qhist_offset[gi] = out->qbound_offset[gi] = 0;
-        // (a copy stream taking the exact histogram down during the
-        // quantised assembly measured slower: 7.3-8.6 ms against 5.2 ms)
-        auto d2h = [&](void *dst, const void *src, size_t bytes) {
-            return bytes == 0 || hipMemcpyAsync(dst, src, bytes, hipMemcpyDeviceToHost, s) == hipSuccess;
-        };
-        PhaseTimer pt2("  assemble");
-        ok = d2h(out->hist_value, hv, (size_t)runs * sizeof(double)) &&
-             d2h(out->hist_count, hc, (size_t)runs * sizeof(int64_t)) &&
-             d2h(out->qhist_key, qk2, (size_t)nq * sizeof(int64_t)) &&
-             d2h(out->qhist_count, qc2, (size_t)nq * sizeof(int64_t)) &&
-             d2h(out->qbound_key, bk2, (size_t)nb * sizeof(int64_t)) &&
-             d2h(out->qbound_count, bc2, (size_t)nb * sizeof(int64_t)) &&
-             (!quant || d2h(out->qhist_offset, qofs, ((size_t)n_graphs + 1) * sizeof(int64_t))) &&
-             (!quant || d2h(out->qbound_offset, bofs, ((size_t)n_graphs + 1) * sizeof(int64_t))) &&
-             hipStreamSynchronize(s) == hipSuccess;
-        pt2.mark("download");
+        ok = d2h(out->qhist_key, qk2, (size_t)nq * sizeof(int64_t), s) &&
+             d2h(out->qhist_count, qc2, (size_t)nq * sizeof(int64_t), s) &&
+             d2h(out->qbound_key, bk2, (size_t)nb * sizeof(int64_t), s) &&
+             d2h(out->qbound_count, bc2, (size_t)nb * sizeof(int64_t), s) &&
+             (!quant || d2h(out->qhist_offset, qofs, ((size_t)n_graphs + 1) * sizeof(int64_t), s)) &&
+             (!quant || d2h(out->qbound_offset, bofs, ((size_t)n_graphs + 1) * sizeof(int64_t), s));
         if (!ok && !has_error()) set_error("batch result download failed");
     }
-    dfree(hv); dfree(hc); dfree(qk2); dfree(qc2); dfree(bk2); dfree(bc2); dfree(qofs); dfree(bofs);
-    dfree(qk); dfree(qc); dfree(lo.d_vals);
+    // every download has finished before its device blocks go back
+    if (cs && hipStreamSynchronize(cs) != hipSuccess && ok) {
+        set_error("batch result download failed");
+        ok = false;
+    }
+    if (hipStreamSynchronize(s) != hipSuccess && ok) {
+        set_error("batch result download failed");
+        ok = false;
+    }
+    pt.mark("download");
+    for (void *p : held) dfree(p);
+    dfree(qk2); dfree(qc2); dfree(bk2); dfree(bc2); dfree(qofs); dfree(bofs);
+    dfree(qk); dfree(qc);
     if (own_stream) (void)hipStreamDestroy(s);
     if (!ok) {
         dress_hip_lifted_batch_free(out);
         return -1;
     }
-    out->subgraphs = (int64_t)item_parent.size();
-    out->values = lo.val_ofs.back();
-    out->fp64_reruns = lo.reruns;
-    out->device_ms = lo.device_ms;
-    for (int i = 0; i < 4; i++) out->work[i] = (int64_t)lo.work[i];
-    pt.mark("assemble");
+    out->subgraphs = subgraphs;
+    out->values = values;
+    out->fp64_reruns = reruns;
+    out->device_ms = device_ms;
+    for (int i = 0; i < 4; i++) out->work[i] = (int64_t)work[i];
     out->total_ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
-    record_lifted(lo.reruns, lo.device_ms, out->subgraphs, out->values);
+    record_lifted(reruns, device_ms, subgraphs, values);
     return 0;
 }
 

@@ -159,6 +159,50 @@ def last_lifted_stats():
             "device_ms": s.device_ms}
 
 
+@dataclass(frozen=True)
+class GraphBatch:
+    """Graphs of one batch in the flat arrays the batch entry points take:
+    vertex counts ``N``, edge ``offsets`` (graph i owns edges
+    ``offsets[i]:offsets[i+1]``), endpoints ``U``/``V`` and optional weights
+    ``W``.  Made by :func:`pack_graphs`; pass it to :func:`lifted_batch` or
+    :func:`fit_batch` in place of the list of graphs."""
+    N: np.ndarray
+    offsets: np.ndarray
+    U: np.ndarray
+    V: np.ndarray
+    W: Optional[np.ndarray] = None
+
+    def __len__(self):
+        return len(self.N)
+
+
+def pack_graphs(graphs, weights=None):
+    """Packs a list of ``(n, U, V)`` (and optional per-graph weight arrays)
+    into a :class:`GraphBatch`."""
+    if isinstance(graphs, GraphBatch):
+        if weights is not None:
+            raise ValueError("a GraphBatch carries its own weights")
+        return graphs
+    n = len(graphs)
+    Ns = np.fromiter((g[0] for g in graphs), dtype=np.int32, count=n)
+    offs = np.zeros(n + 1, dtype=np.int64)
+    np.cumsum(np.fromiter((len(g[1]) for g in graphs), dtype=np.int64, count=n), out=offs[1:])
+    if n:
+        U = np.ascontiguousarray(np.concatenate([g[1] for g in graphs]), dtype=np.int32)
+        V = np.ascontiguousarray(np.concatenate([g[2] for g in graphs]), dtype=np.int32)
+    else:
+        U = V = np.zeros(0, np.int32)
+    if U.size != offs[-1] or V.size != offs[-1]:
+        raise ValueError("every graph needs as many targets as sources")
+    W = None
+    if weights is not None:
+        W = np.ascontiguousarray(np.concatenate([np.asarray(w, dtype=np.float64) for w in weights]),
+                                 dtype=np.float64)
+        if W.size != offs[-1]:
+            raise ValueError("every graph needs one weight per edge")
+    return GraphBatch(Ns, offs, U, V, W)
+
+
 def lifted_batch(graphs, k=1, kind="delta", variant=UNDIRECTED, max_iterations=100, epsilon=1e-6,
                  precision=FP64, qtol=None, weights=None):
     """delta^k / nabla^k of many graphs in one batched launch.
@@ -166,21 +210,12 @@ def lifted_batch(graphs, k=1, kind="delta", variant=UNDIRECTED, max_iterations=1
     ``graphs`` is a list of ``(n, U, V)``; ``weights`` an optional list of
     per-graph weight arrays.  ``qtol`` (e.g. 1e-9) asks for the
     floor(d/epsilon) tables as well, with values within ``qtol`` of a bin
-    edge counted apart; ``None`` leaves them empty.  Returns a
+    edge counted apart; ``None`` leaves them empty.  ``graphs`` may also be
+    a :class:`GraphBatch` (then ``weights`` are its own).  Returns a
     :class:`LiftedBatchResult`."""
     L = _lib.lib()
-    n = len(graphs)
-    Ns = np.array([g[0] for g in graphs], dtype=np.int32)
-    offs = np.zeros(n + 1, dtype=np.int64)
-    for i, g in enumerate(graphs):
-        offs[i + 1] = offs[i] + len(g[1])
-    U = np.ascontiguousarray(np.concatenate([np.asarray(g[1], dtype=np.int32) for g in graphs])
-                             if n else np.zeros(0, np.int32))
-    V = np.ascontiguousarray(np.concatenate([np.asarray(g[2], dtype=np.int32) for g in graphs])
-                             if n else np.zeros(0, np.int32))
-    W = None
-    if weights is not None:
-        W = np.ascontiguousarray(np.concatenate([np.asarray(w, dtype=np.float64) for w in weights]))
+    b = pack_graphs(graphs, weights)
+    n, Ns, offs, U, V, W = len(b), b.N, b.offsets, b.U, b.V, b.W
     r = _lib.LiftedBatch()
     rc = L.dress_hip_lifted_batch(0 if kind == "delta" else 1, int(k), n,
                                   Ns.ctypes.data_as(c_int_p), offs.ctypes.data_as(_lib.c_i64_p),
@@ -282,20 +317,11 @@ def fit_batch_eligible(n_vertices, n_edges):
 
 def fit_batch(graphs, variant=UNDIRECTED, max_iterations=100, epsilon=1e-6, precision=FP64, weights=None):
     """Whole-graph fits of many small graphs in one launch; ``graphs`` is a
-    list of ``(n, U, V)``.  Per graph the results equal a single fit."""
+    list of ``(n, U, V)`` or a :class:`GraphBatch`.  Per graph the results
+    equal a single fit."""
     L = _lib.lib()
-    n = len(graphs)
-    Ns = np.array([g[0] for g in graphs], dtype=np.int32)
-    offs = np.zeros(n + 1, dtype=np.int64)
-    for i, g in enumerate(graphs):
-        offs[i + 1] = offs[i] + len(g[1])
-    U = np.ascontiguousarray(np.concatenate([np.asarray(g[1], dtype=np.int32) for g in graphs])
-                             if n else np.zeros(0, np.int32))
-    V = np.ascontiguousarray(np.concatenate([np.asarray(g[2], dtype=np.int32) for g in graphs])
-                             if n else np.zeros(0, np.int32))
-    W = None
-    if weights is not None:
-        W = np.ascontiguousarray(np.concatenate([np.asarray(w, dtype=np.float64) for w in weights]))
+    b = pack_graphs(graphs, weights)
+    n, Ns, offs, U, V, W = len(b), b.N, b.offsets, b.U, b.V, b.W
     r = _lib.FitBatch()
     rc = L.dress_hip_fit_batch(n, Ns.ctypes.data_as(c_int_p), offs.ctypes.data_as(_lib.c_i64_p),
                                U.ctypes.data_as(c_int_p), V.ctypes.data_as(c_int_p),

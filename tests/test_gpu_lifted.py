@@ -240,3 +240,55 @@ def test_quantised_tables_on_request(gpu, gen, monkeypatch):
     # 3n-6 edges, 2(3n-6) half-edges, each times its iterations (>= 1)
     lo_e = sum(n * (3 * n - 6) for n, _, _ in graphs)
     assert E >= lo_e and S == 2 * E and N >= sum(n * (n - 1) for n, _, _ in graphs) and T >= 0
+
+
+@pytest.mark.parametrize("kind,k,precision", [("delta", 1, 0), ("delta", 1, 1), ("delta", 2, 0), ("nabla", 1, 0)])
+def test_grouped_batch_equals_one_group(gpu, ora, gen, monkeypatch, kind, k, precision):
+    """A batch run as groups of consecutive graphs (parents of later groups
+    built on the host and histograms of earlier ones downloaded while the
+    device works) returns what the same batch run as one group returns, with
+    graphs that have no subgraph (k >= n) at group edges; every graph's
+    histogram equals the oracle's in FP64."""
+    i32 = np.int32
+    graphs = list(gen.delta1_batch_graphs(40, n_lo=8, n_hi=40))
+    graphs[0] = (2, np.zeros(1, i32), np.ones(1, i32))          # k >= n for k = 2
+    graphs[17] = (1, np.zeros(0, i32), np.zeros(0, i32))
+    graphs[39] = (2, np.zeros(1, i32), np.ones(1, i32))
+    monkeypatch.setenv("DRESS_HIP_BATCH_GROUP_VALUES", "0")
+    one = gpu.lifted_batch(graphs, k=k, kind=kind, precision=precision)
+    for per_group in ("1", "60000"):                             # 8 groups, then groups of ~60k values
+        monkeypatch.setenv("DRESS_HIP_BATCH_GROUP_VALUES", per_group)
+        r = gpu.lifted_batch(graphs, k=k, kind=kind, precision=precision)
+        assert (r.subgraphs, r.values) == (one.subgraphs, one.values)
+        for (v0, c0), (v1, c1) in zip(one.histograms, r.histograms):
+            assert np.array_equal(v0, v1) and np.array_equal(c0, c1)
+    if precision == 0:
+        for (hv, hc), (n, U, V) in zip(r.histograms, graphs):
+            ov, oc, _, _ = ora.lifted(kind, n, U, V, k=k)
+            assert np.array_equal(hv, ov) and np.array_equal(hc, oc), (kind, k, n)
+
+
+def test_histogram_key_layouts_agree(gpu, ora, gen, monkeypatch):
+    """The exact histogram sorts one 64-bit key per value holding (graph,
+    value) when the values' keys span few enough bits to leave room for the
+    graph ids, and otherwise sorts by value and then stably by graph: both
+    give the same histograms, for a batch (many graphs) and for a reference-
+    API call (one), and a weighted graph whose values span many binades
+    equals the oracle."""
+    graphs = gen.delta1_batch_graphs(24, n_lo=8, n_hi=40)
+    n, U, V = gen.random_regular(30, 4, 2)
+    rng = np.random.default_rng(3)
+    W = 10.0 ** rng.uniform(-120, 120, U.size)                   # values over many binades
+    res = []
+    for packed in ("1", "0"):
+        monkeypatch.setenv("DRESS_HIP_HIST_PACKED", packed)
+        r = gpu.lifted_batch(graphs, k=1)
+        res.append(([(np.array(v), np.array(c)) for v, c in r.histograms],
+                    _lifted(gpu, "delta", n, U, V, k=2), _lifted(gpu, "delta", n, U, V, W, k=1)))
+    for (v0, c0), (v1, c1) in zip(res[0][0], res[1][0]):
+        assert np.array_equal(v0, v1) and np.array_equal(c0, c1)
+    for j in (1, 2):
+        assert np.array_equal(res[0][j][0], res[1][j][0]) and np.array_equal(res[0][j][1], res[1][j][1])
+    ov, oc, _, _ = ora.lifted("delta", n, U, V, W, k=1)
+    assert np.array_equal(res[0][2][0], ov) and np.array_equal(res[0][2][1], oc)
+    assert ov.min() < 1e-30 < ov.max()

@@ -1,8 +1,9 @@
-"""Host-side parts of the serving stream (no GPU): the arrival process and
-the G(n, p) request generator."""
+"""Host-side parts of the serving stream and the batch API (no GPU): the
+arrival process, the G(n, p) request generator and batch packing."""
 import math
 
 import numpy as np
+import pytest
 
 
 def test_request_stream_is_seeded_and_in_range(dh):
@@ -55,3 +56,28 @@ def test_gnp_pair_frequencies(gen):
     freq = cnt[iu] / reps
     assert abs(freq.mean() - p) < 0.01
     assert np.all(np.abs(freq - p) < 5 * math.sqrt(p * (1 - p) / reps))
+
+
+def test_pack_graphs_flattens_a_batch(dh, gen):
+    """pack_graphs gives the flat arrays the batch entry points take (vertex
+    counts, edge offsets, int32 endpoints, float64 weights) and checks the
+    per-graph lengths; a GraphBatch passes through unchanged."""
+    graphs = gen.delta1_batch_graphs(7, n_lo=8, n_hi=20)
+    graphs[3] = (5, np.zeros(0, np.int64), np.zeros(0, np.int64))        # no edges, other dtype
+    w = [np.linspace(0.5, 1.0, len(U)) for _, U, _ in graphs]
+    b = dh.pack_graphs(graphs, w)
+    assert b.N.tolist() == [n for n, _, _ in graphs] and len(b) == 7
+    assert b.offsets.tolist() == np.concatenate([[0], np.cumsum([len(U) for _, U, _ in graphs])]).tolist()
+    assert b.U.dtype == np.int32 and b.V.dtype == np.int32 and b.W.dtype == np.float64
+    for i, (n, U, V) in enumerate(graphs):
+        lo, hi = b.offsets[i], b.offsets[i + 1]
+        assert np.array_equal(b.U[lo:hi], U) and np.array_equal(b.V[lo:hi], V) and np.array_equal(b.W[lo:hi], w[i])
+    assert dh.pack_graphs(b) is b
+    with pytest.raises(ValueError):
+        dh.pack_graphs(b, w)
+    with pytest.raises(ValueError):
+        dh.pack_graphs([(3, np.array([0, 1], np.int32), np.array([1], np.int32))])
+    with pytest.raises(ValueError):
+        dh.pack_graphs(graphs, [x[:-1] for x in w])
+    empty = dh.pack_graphs([])
+    assert len(empty) == 0 and empty.offsets.tolist() == [0]
