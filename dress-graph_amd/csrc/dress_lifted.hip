@@ -2522,6 +2522,13 @@ int dress_hip_lifted_batch(int kind, int k, int n_graphs, const int *N, const in
                                 nullptr, P->R, P->lo);
             ppt.mark("prepared");
             if (!P->ok) P->err = dress_hip_last_error();
+            // a host-side test hook: DRESS_HIP_BATCH_FAIL_GROUP=g fails group g's
+            // preparation, while earlier groups may be on the device
+            if (const char *e = getenv("DRESS_HIP_BATCH_FAIL_GROUP"))
+                if (P->ok && atoi(e) == gr) {
+                    P->ok = false;
+                    P->err = "preparation of batch group " + std::to_string(gr) + " failed (DRESS_HIP_BATCH_FAIL_GROUP)";
+                }
             return P;
         };
         std::mutex pm;

@@ -292,3 +292,22 @@ def test_histogram_key_layouts_agree(gpu, ora, gen, monkeypatch):
     ov, oc, _, _ = ora.lifted("delta", n, U, V, W, k=1)
     assert np.array_equal(res[0][2][0], ov) and np.array_equal(res[0][2][1], oc)
     assert ov.min() < 1e-30 < ov.max()
+
+
+@pytest.mark.parametrize("fail_group", [0, 1, 4, 7])
+def test_grouped_batch_failure_leaves_the_library_usable(gpu, gen, monkeypatch, fail_group):
+    """A group whose host preparation fails (a host-side hook, no device
+    fault) while earlier groups are queued or running ends the call with that
+    error; the queued work finishes before its blocks are released, and the
+    next call returns the same histograms as an undisturbed one."""
+    graphs = gen.delta1_batch_graphs(48, n_lo=8, n_hi=40)
+    monkeypatch.setenv("DRESS_HIP_BATCH_GROUP_VALUES", "1")                 # 8 groups
+    ref = gpu.lifted_batch(graphs, k=1)
+    monkeypatch.setenv("DRESS_HIP_BATCH_FAIL_GROUP", str(fail_group))
+    with pytest.raises(gpu.HipError, match=f"group {fail_group} failed"):
+        gpu.lifted_batch(graphs, k=1)
+    monkeypatch.delenv("DRESS_HIP_BATCH_FAIL_GROUP")
+    again = gpu.lifted_batch(graphs, k=1)
+    assert (again.subgraphs, again.values) == (ref.subgraphs, ref.values)
+    for (v0, c0), (v1, c1) in zip(ref.histograms, again.histograms):
+        assert np.array_equal(v0, v1) and np.array_equal(c0, c1)
