@@ -47,6 +47,7 @@ struct LiftArgs {
     int max_it;
     double eps;
     double *vals;                  // packed live values (or null)
+    int sort_vals;                 // 1: each item's live values ascending (by sort key), else edge order
     double *ms;                    // multiset rows (or null)
     int64_t ms_stride;
     double *vnorm;                 // per item: the norms of the last vertex phase (or null)
@@ -66,6 +67,14 @@ struct LiftArgs {
 };
 
 constexpr unsigned long long QEMPTY = ~0ull;
+
+// ascending order of doubles as unsigned keys (sign-folded bit patterns):
+// the order of the device histograms' keys
+__device__ __forceinline__ unsigned long long sort_key(double v)
+{
+    const unsigned long long b = (unsigned long long)__double_as_longlong(v);
+    return (b >> 63) ? ~b : (b | 0x8000000000000000ull);
+}
 
 // quantised-table key of value v of graph gid: floor(v / eps) (clamped to
 // 40 bits), with bit 40 set for values within tol of a bin edge

@@ -262,7 +262,9 @@ __global__ void __launch_bounds__(B) k_lifted(LiftArgs a)
             }
         }
         if (a.vals) {
-            // live values packed in edge order at vals[I.val_ofs ...]
+            // live values packed in edge order at vals[I.val_ofs ...], or
+            // first packed into the free buffer and sorted there
+            double *sv = cur ? dA : dB;            // the previous values: free now
             __shared__ int s_base;
             if (threadIdx.x == 0) s_base = 0;
             __syncthreads();
@@ -282,7 +284,10 @@ __global__ void __launch_bounds__(B) k_lifted(LiftArgs a)
                 __syncthreads();
                 int off = s_base;
                 for (int i = 0; i < wv; i++) off += s_wave[i];
-                if (live) a.vals[I.val_ofs + off + before] = dc[e];
+                if (live) {
+                    if (a.sort_vals) sv[off + before] = dc[e];
+                    else a.vals[I.val_ofs + off + before] = dc[e];
+                }
                 __syncthreads();
                 if (threadIdx.x == 0) {
                     int tot = 0;
@@ -290,6 +295,43 @@ __global__ void __launch_bounds__(B) k_lifted(LiftArgs a)
                     s_base += tot;
                 }
                 __syncthreads();
+            }
+            if (a.sort_vals) {
+                // bitonic network over the next power of two P >= L with
+                // the slots from L on as +infinity: every compare-exchange
+                // puts the smaller at the lower index, so a pair reaching
+                // past L never exchanges
+                const int L = s_base;
+                int lgP = 0;
+                while ((1 << lgP) < L) lgP++;
+                const int half = (1 << lgP) >> 1;
+                for (int lk = 1; lk <= lgP; lk++) {
+                    // flip within blocks of 2^lk: i = block start + off,
+                    // j = block end - off
+                    for (int t = threadIdx.x; t < half; t += B) {
+                        const int off = t & ((1 << (lk - 1)) - 1);
+                        const int i = ((t >> (lk - 1)) << lk) + off, j = i + (1 << lk) - 1 - 2 * off;
+                        if (j < L && sort_key(sv[j]) < sort_key(sv[i])) {
+                            const double x = sv[i];
+                            sv[i] = sv[j];
+                            sv[j] = x;
+                        }
+                    }
+                    __syncthreads();
+                    // half-cleaners at distance 2^lh
+                    for (int lh = lk - 2; lh >= 0; lh--) {
+                        for (int t = threadIdx.x; t < half; t += B) {
+                            const int i = ((t >> lh) << (lh + 1)) + (t & ((1 << lh) - 1)), j = i + (1 << lh);
+                            if (j < L && sort_key(sv[j]) < sort_key(sv[i])) {
+                                const double x = sv[i];
+                                sv[i] = sv[j];
+                                sv[j] = x;
+                            }
+                        }
+                        __syncthreads();
+                    }
+                }
+                for (int i = threadIdx.x; i < L; i += B) a.vals[I.val_ofs + i] = sv[i];
             }
         }
         if (a.qkeys) {

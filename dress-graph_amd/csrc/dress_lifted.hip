@@ -597,16 +597,19 @@ __global__ void k_run_emit(const uint64_t *key, const int *seg, const int *flag,
     }
 }
 
-// the segment holding value i: the last g with seg_ofs[g] <= i
-__device__ __forceinline__ int segment_of(const int64_t *seg_ofs, int nseg, int64_t i)
+// the segment holding value i: the last g in [lo, hi] with seg_ofs[g] <= i
+__device__ __forceinline__ int segment_in(const int64_t *seg_ofs, int lo, int hi, int64_t i)
 {
-    int lo = 0, hi = nseg - 1;
     while (lo < hi) {
         const int m = (lo + hi + 1) >> 1;
         if (seg_ofs[m] <= i) lo = m;
         else hi = m - 1;
     }
     return lo;
+}
+__device__ __forceinline__ int segment_of(const int64_t *seg_ofs, int nseg, int64_t i)
+{
+    return segment_in(seg_ofs, 0, nseg - 1, i);
 }
 
 // one 64-bit key per value ordered as (segment, value): the segment above
@@ -624,8 +627,14 @@ __global__ void __launch_bounds__(TPB) k_pack_keys(const uint64_t *key, const in
     }
     const int64_t *so = lds ? s_ofs : seg_ofs;
     const uint64_t lo = range[0];
-    for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) {
-        const uint64_t seg = nseg > 1 ? (uint64_t)segment_of(so, nseg, i) : 0;
+    // a contiguous chunk per block: its values lie in the segments between
+    // those of its first and last value
+    const int64_t chunk = (n + gridDim.x - 1) / gridDim.x;
+    const int64_t c0 = blockIdx.x * chunk, c1 = c0 + chunk < n ? c0 + chunk : n;
+    if (c0 >= c1) return;
+    const int g0 = nseg > 1 ? segment_of(so, nseg, c0) : 0, g1 = nseg > 1 ? segment_of(so, nseg, c1 - 1) : 0;
+    for (int64_t i = c0 + threadIdx.x; i < c1; i += blockDim.x) {
+        const uint64_t seg = (uint64_t)segment_in(so, g0, g1, i);
         out[i] = (vbits < 64 ? seg << vbits : 0) | (key[i] - lo);
     }
 }
@@ -640,6 +649,106 @@ __global__ void k_unpack_runs(uint64_t *ukey, int64_t runs, const unsigned long 
         const uint64_t k = (key & mask) + lo;
         ukey[j] = (k >> 63) ? (k & 0x7fffffffffffffffull) : ~k;
         useg[j] = vbits < 64 ? (int)(key >> vbits) : 0;
+    }
+}
+
+// One tile of a round of pairwise merges of sorted runs: outputs [o0, o1)
+// (relative to a0) of the merge of A = in[a0, b0) and B = in[b0, b1), written
+// to out[a0 + o0, a0 + o1); an empty B copies A.  Ties take A first.
+struct MergeTile {
+    int64_t a0, b0, b1;
+    int o0, o1;
+};
+constexpr int MERGE_TPB = 128, MERGE_PER_THREAD = 8, MERGE_TILE = MERGE_TPB * MERGE_PER_THREAD;
+
+// number of A elements among the first d outputs of the merge of a[0, la)
+// and b[0, lb)
+template <typename P>
+__device__ __forceinline__ int64_t merge_split(P a, int64_t la, P b, int64_t lb, int64_t d)
+{
+    int64_t lo = d > lb ? d - lb : 0, hi = d < la ? d : la;
+    while (lo < hi) {
+        const int64_t m = (lo + hi) >> 1;
+        if (a[m] <= b[d - 1 - m]) lo = m + 1;
+        else hi = m;
+    }
+    return lo;
+}
+
+__global__ void __launch_bounds__(MERGE_TPB) k_merge_tiles(const uint64_t *in, uint64_t *out, const MergeTile *tiles)
+{
+    __shared__ uint64_t s[MERGE_TILE];
+    __shared__ int64_t s_ia[2];
+    const MergeTile T = tiles[blockIdx.x];
+    const int64_t la = T.b0 - T.a0, lb = T.b1 - T.b0;
+    if (threadIdx.x < 2)
+        s_ia[threadIdx.x] = merge_split(in + T.a0, la, in + T.b0, lb, threadIdx.x == 0 ? T.o0 : T.o1);
+    __syncthreads();
+    const int64_t ia0 = s_ia[0], ia1 = s_ia[1];
+    const int na = (int)(ia1 - ia0), nb = (T.o1 - T.o0) - na;
+    const int64_t ib0 = T.o0 - ia0;
+    for (int i = threadIdx.x; i < na; i += MERGE_TPB) s[i] = in[T.a0 + ia0 + i];
+    for (int i = threadIdx.x; i < nb; i += MERGE_TPB) s[na + i] = in[T.b0 + ib0 + i];
+    __syncthreads();
+    const int n = na + nb;
+    const int d0 = min(n, (int)threadIdx.x * MERGE_PER_THREAD), d1 = min(n, d0 + MERGE_PER_THREAD);
+    int ia = (int)merge_split(s, na, s + na, nb, d0), ib = d0 - ia;
+    uint64_t *o = out + T.a0 + T.o0;
+    for (int d = d0; d < d1; d++) {
+        const bool take_a = ib >= nb || (ia < na && s[ia] <= s[na + ib]);
+        o[d] = take_a ? s[ia++] : s[na + ib++];
+    }
+}
+
+// Tiles of the rounds that merge the sorted runs of every segment into one:
+// runs end at run_end (ascending, the last = the values), segments at
+// seg_ofs (each segment boundary is a run boundary).  Each round merges
+// neighbouring runs of a segment in pairs (a lone last run is copied), so a
+// segment of r runs is one after ceil(log2 r) rounds; segments already done
+// are copied, keeping every round's output whole.  emit(tile) receives the
+// tiles in order; round_ofs[i] is the number emitted before round i (and
+// the last entry the total).
+template <typename Emit>
+void merge_rounds(const std::vector<int64_t> &run_end, const std::vector<int64_t> &seg_ofs, Emit &&emit,
+                  std::vector<size_t> &round_ofs)
+{
+    round_ofs.assign(1, 0);
+    const size_t nseg = seg_ofs.size() - 1;
+    // each segment's run boundaries at bnd[first[g], first[g] + cnt[g])
+    std::vector<int64_t> bnd;
+    bnd.reserve(run_end.size() + 2 * nseg);
+    std::vector<size_t> first(nseg);
+    std::vector<int> cnt(nseg);
+    size_t r = 0;
+    int most = 0;
+    for (size_t g = 0; g < nseg; g++) {
+        first[g] = bnd.size();
+        bnd.push_back(seg_ofs[g]);
+        while (r < run_end.size() && run_end[r] <= seg_ofs[g + 1]) {
+            if (run_end[r] > bnd.back()) bnd.push_back(run_end[r]);
+            r++;
+        }
+        if (bnd.back() < seg_ofs[g + 1]) bnd.push_back(seg_ofs[g + 1]);
+        cnt[g] = (int)(bnd.size() - first[g]);
+        most = std::max(most, cnt[g]);
+    }
+    size_t emitted = 0;
+    for (; most > 2; most = (most + 2) / 2) {
+        for (size_t g = 0; g < nseg; g++) {
+            int64_t *b = bnd.data() + first[g];
+            const int c = cnt[g];
+            int w = 1;                 // boundaries of the next round, written behind the reads
+            for (int j = 0; j + 1 < c; j += 2) {
+                const int64_t a0 = b[j], b0 = b[j + 1], b1 = j + 2 < c ? b[j + 2] : b[j + 1];
+                for (int64_t o = 0; o < b1 - a0; o += MERGE_TILE) {
+                    emit(MergeTile{a0, b0, b1, (int)o, (int)std::min<int64_t>(b1 - a0, o + MERGE_TILE)});
+                    emitted++;
+                }
+                b[w++] = b1;
+            }
+            cnt[g] = w;
+        }
+        round_ofs.push_back(emitted);
     }
 }
 
@@ -690,7 +799,8 @@ __global__ void k_interleave(const double *v, const int64_t *c, int64_t runs, in
 // by (segment, value)) on the device for the caller to download and dfree;
 // run_ofs[g] is the first run of segment g (nseg+1 entries).
 bool exact_histogram_dev(const double *vals, int64_t n, const std::vector<int64_t> &seg_ofs, hipStream_t s,
-                         double **d_val, int64_t **d_cnt, int64_t *runs_out, std::vector<int64_t> &run_ofs)
+                         double **d_val, int64_t **d_cnt, int64_t *runs_out, std::vector<int64_t> &run_ofs,
+                         const std::vector<int64_t> *sorted_runs = nullptr)
 {
     PhaseTimer pt("  exact_histogram_dev");
     *d_val = nullptr;
@@ -700,6 +810,7 @@ bool exact_histogram_dev(const double *vals, int64_t n, const std::vector<int64_
     run_ofs.assign((size_t)nseg + 1, 0);
     if (n <= 0) return true;
     uint64_t *k0 = nullptr, *k1 = nullptr;
+    MergeTile *dtiles = nullptr, *tile_host = nullptr;   // merge tiles, uploaded from a pinned block
     unsigned long long *part = nullptr, *range = nullptr;
     int *g0 = nullptr, *g1 = nullptr, *flag = nullptr, *usg = nullptr;
     int64_t *pos = nullptr, *ustart = nullptr, *dofs = nullptr, *cnt = nullptr, *rofs = nullptr, *druns = nullptr;
@@ -731,9 +842,39 @@ bool exact_histogram_dev(const double *vals, int64_t n, const std::vector<int64_
     if (ok && vbits >= 0) {
         const size_t shm = nseg < PACK_LDS_SEGS ? ((size_t)nseg + 1) * sizeof(int64_t) : 0;
         hipLaunchKernelGGL(k_pack_keys, dim3(grid), dim3(TPB), shm, s, k0, dofs, nseg, n, range, vbits, k1);
-        ok = prim_sort_u64(k1, k0, (size_t)n, s, std::max(1, vbits + sbits)) &&
-             dalloc(&cnt, (size_t)n + 1, "histogram counts") &&
-             prim_run_length_encode_u64(k0, (size_t)n, k1, cnt, druns, s) &&
+        if (pt.on) { hipStreamSynchronize(s); pt.mark("pack"); }
+        // values that arrive as sorted runs (each segment's runs merged in
+        // ceil(log2 runs) rounds) or in any order (one radix sort)
+        uint64_t *sorted = k1, *other = k0;
+        std::vector<size_t> round_ofs;
+        if (sorted_runs) merge_rounds(*sorted_runs, seg_ofs, [](const MergeTile &) {}, round_ofs);
+        if (sorted_runs && round_ofs.size() <= 9) {
+            // the tiles written straight into a pinned block (pooled, so
+            // already mapped), then one copy
+            const size_t nt = round_ofs.back();
+            if (nt) {
+                tile_host = (MergeTile *)pinned_alloc(nt * sizeof(MergeTile));
+                ok = tile_host && dalloc(&dtiles, nt, "merge tiles");
+                if (!tile_host) set_error("host staging of the merge tiles failed");
+                size_t at = 0;
+                if (ok) merge_rounds(*sorted_runs, seg_ofs, [&](const MergeTile &t) { tile_host[at++] = t; }, round_ofs);
+                ok = ok && hipMemcpyAsync(dtiles, tile_host, nt * sizeof(MergeTile), hipMemcpyHostToDevice, s) ==
+                               hipSuccess;
+            }
+            if (pt.on) pt.mark("merge plan");
+            for (size_t r = 0; ok && r + 1 < round_ofs.size(); r++) {
+                hipLaunchKernelGGL(k_merge_tiles, dim3((unsigned)(round_ofs[r + 1] - round_ofs[r])), dim3(MERGE_TPB),
+                                   0, s, sorted, other, dtiles + round_ofs[r]);
+                std::swap(sorted, other);
+            }
+            if (pt.on) { hipStreamSynchronize(s); pt.mark("merge rounds"); }
+        } else {
+            ok = prim_sort_u64(k1, k0, (size_t)n, s, std::max(1, vbits + sbits));
+            sorted = k0;
+            other = k1;
+        }
+        ok = ok && dalloc(&cnt, (size_t)n + 1, "histogram counts") &&
+             prim_run_length_encode_u64(sorted, (size_t)n, other, cnt, druns, s) &&
              hipMemcpyAsync(host, druns, sizeof(int64_t), hipMemcpyDeviceToHost, s) == hipSuccess &&
              hipStreamSynchronize(s) == hipSuccess;
         if (pt.on) pt.mark("sort+runs");
@@ -741,10 +882,11 @@ bool exact_histogram_dev(const double *vals, int64_t n, const std::vector<int64_
             runs = (int64_t)host[0];
             const int ugrid = grid_for(runs, num_cus() * 8);
             usg = flag;                                          // (flags unused on this path)
-            hipLaunchKernelGGL(k_unpack_runs, dim3(ugrid), dim3(TPB), 0, s, k1, runs, range, vbits, usg);
+            hipLaunchKernelGGL(k_unpack_runs, dim3(ugrid), dim3(TPB), 0, s, other, runs, range, vbits, usg);
             hipLaunchKernelGGL(k_seg_run_starts, dim3(ugrid), dim3(TPB), 0, s, usg, runs, nseg, rofs);
-            uval = reinterpret_cast<double *>(k1);
-            k1 = nullptr;
+            uval = reinterpret_cast<double *>(other);
+            if (other == k0) k0 = nullptr;
+            else k1 = nullptr;
             usg = nullptr;
         }
     } else if (ok) {
@@ -791,9 +933,11 @@ bool exact_histogram_dev(const double *vals, int64_t n, const std::vector<int64_
                  hipSuccess &&
              hipStreamSynchronize(s) == hipSuccess;
     if (!ok && !has_error()) set_error("histogram assembly failed");
+    if (!ok) (void)hipStreamSynchronize(s);     // queued kernels may still read the blocks
     pt.mark("runs");
     dfree(k0); dfree(k1); dfree(g0); dfree(g1); dfree(flag); dfree(pos); dfree(ustart); dfree(dofs);
-    dfree(usg); dfree(rofs); dfree(part); dfree(range); dfree(druns);
+    dfree(usg); dfree(rofs); dfree(part); dfree(range); dfree(druns); dfree(dtiles);
+    pinned_free(tile_host);
     if (host) snapshot_give(host);
     pt.mark("free");
     if (!ok) {
@@ -1463,7 +1607,7 @@ bool launch_ids(RunState &R, const std::vector<const HostParent *> &ps, const st
 bool run_launch(RunState &R, const std::vector<const HostParent *> &ps, const std::vector<int> &item_parent,
                 const LiftJob &J, double *d_ms, int64_t ms_stride, hipStream_t s, LiftOut &out,
                 unsigned long long *qkeys, unsigned long long *qcounts, int64_t qcap, double qtol,
-                const FitOutputs *fo)
+                const FitOutputs *fo, bool sort_vals = false)
 {
     const int n = R.n;
     if (n == 0) return true;
@@ -1502,6 +1646,7 @@ bool run_launch(RunState &R, const std::vector<const HostParent *> &ps, const st
     a.qtol = qtol;
     a.parent_graph = R.D.parent_graph;
     a.work = R.d_work;
+    a.sort_vals = sort_vals ? 1 : 0;
     if (fo) {
         a.vnorm = fo->d_vnorm;
         a.iters = R.d_iters;
@@ -2490,8 +2635,21 @@ int dress_hip_lifted_batch(int kind, int k, int n_graphs, const int *N, const in
         std::vector<const HostParent *> ps;
         std::vector<int> graph_of, item_parent, item_verts;
         std::vector<int64_t> item_live, ms_rows;
+        size_t max_items = 0;                 // most items of one graph
         bool ok = false;
         std::string err;
+    };
+    // In a pipelined batch, items leave k_lifted with their values sorted
+    // and each graph's histogram merges its items' runs, when that takes at
+    // most 8 rounds (256 items per graph); else the values are radix sorted.
+    // (The 512-graph delta^1 batch: 4.6 ms a call against 5.5.  In one
+    // group the sort's cost inside k_lifted, 0.18 ms, is not hidden and the
+    // radix sort is faster: 6.8 against 7.1 ms.)  DRESS_HIP_HIST_MERGE=0
+    // keeps the radix sort, =1 merges in one group too.
+    const char *hm = getenv("DRESS_HIP_HIST_MERGE");
+    const int merge_mode = hm ? atoi(hm) : -1;
+    auto merges = [&](const GroupPlan &P) {
+        return merge_mode != 0 && (G > 1 || merge_mode == 1) && P.max_items <= 256;
     };
     std::vector<std::unique_ptr<GroupPlan>> plans((size_t)G);
     {
@@ -2511,6 +2669,7 @@ int dress_hip_lifted_batch(int kind, int k, int n_graphs, const int *N, const in
             for (int gi = ga; gi < gz; gi++) {
                 P->graph_of.push_back(gi);
                 P->ps.push_back(&parents[(size_t)gi]);
+                P->max_items = std::max(P->max_items, g_live[(size_t)gi].size());
                 for (int64_t x : g_live[(size_t)gi]) {
                     P->item_parent.push_back(gi - ga);
                     P->item_live.push_back(x);
@@ -2568,18 +2727,27 @@ int dress_hip_lifted_batch(int kind, int k, int n_graphs, const int *N, const in
                 hipLaunchKernelGGL(k_qtable_clear, dim3(grid_for(qcap, 4096)), dim3(TPB), 0, s, qk, qc, qcap);
             }
             return run_launch(P.R, P.ps, P.item_parent, J, nullptr, 0, s, P.lo, qk, qc, quant ? qcap : 0,
-                              quant ? qtol : 0.0, nullptr);
+                              quant ? qtol : 0.0, nullptr, merges(P));
         };
-        bool launched = ok && launch_group(0);
+        // two groups stay queued on s ahead of the one being finished, so
+        // the device goes from one group's launches to the next while the
+        // host waits for a histogram
+        bool chain = ok;                                 // every launch so far queued
+        int queued = 0;                                  // groups [0, queued) are on s
+        auto queue_group = [&](int g) {
+            if (!chain || g >= G) return;
+            if (launch_group(g)) queued = g + 1;
+            else chain = false;
+        };
+        queue_group(0);
+        queue_group(1);
         for (int gr = 0; ok && gr < G; gr++) {
             const int ga = gb[(size_t)gr], gz = gb[(size_t)gr + 1];
             GroupPlan &P = *plans[(size_t)gr];
-            // the next group goes into the queue behind this one
-            bool next = gr + 1 < G && launched && launch_group(gr + 1);
             ok = run_finish(P.R, P.ps, P.graph_of, P.item_parent, J, s, P.lo, qk, qc, quant ? qcap : 0,
-                            quant ? qtol : 0.0, nullptr, launched);
-            launched = next;
-            if (ok && gr + 1 < G && !next) ok = false;
+                            quant ? qtol : 0.0, nullptr, gr < queued);
+            if (ok) queue_group(gr + 2);
+            if (!chain) ok = false;                      // a later group failed: stop after this one
             LiftOut &lo = P.lo;
             // exact histogram per graph: the items of a graph are contiguous
             std::vector<int64_t> seg_ofs((size_t)(gz - ga) + 1, 0);
@@ -2593,7 +2761,11 @@ int dress_hip_lifted_batch(int kind, int k, int n_graphs, const int *N, const in
             double *hv = nullptr;
             int64_t *hc = nullptr, runs = 0;
             std::vector<int64_t> run_ofs((size_t)(gz - ga) + 1, 0);
-            if (ok && gz > ga) ok = exact_histogram_dev(lo.d_vals, lo.val_ofs.back(), seg_ofs, hs, &hv, &hc, &runs, run_ofs);
+            std::vector<int64_t> item_end;
+            if (ok && merges(P)) item_end.assign(lo.val_ofs.begin() + 1, lo.val_ofs.end());
+            if (ok && gz > ga)
+                ok = exact_histogram_dev(lo.d_vals, lo.val_ofs.back(), seg_ofs, hs, &hv, &hc, &runs, run_ofs,
+                                         merges(P) ? &item_end : nullptr);
             if (gr == 0) pt.mark("first group on the device");
             if (ok && G == 1) {
                 out->hist_value = (double *)pinned_alloc(((size_t)runs + 1) * sizeof(double));

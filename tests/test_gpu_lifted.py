@@ -311,3 +311,25 @@ def test_grouped_batch_failure_leaves_the_library_usable(gpu, gen, monkeypatch, 
     assert (again.subgraphs, again.values) == (ref.subgraphs, ref.values)
     for (v0, c0), (v1, c1) in zip(ref.histograms, again.histograms):
         assert np.array_equal(v0, v1) and np.array_equal(c0, c1)
+
+
+@pytest.mark.parametrize("kind,k,precision", [("delta", 1, 0), ("delta", 2, 0), ("nabla", 1, 1), ("delta", 1, 1)])
+def test_histogram_merge_equals_radix(gpu, ora, gen, monkeypatch, kind, k, precision):
+    """With at most 256 items per graph, the batched kernel leaves each
+    item's values sorted (a bitonic network in LDS) and the histogram merges
+    each graph's sorted runs; the radix-sorted histogram of the same batch is
+    the same, in one group and in several, and FP64 equals the oracle."""
+    graphs = gen.delta1_batch_graphs(30, n_lo=8, n_hi=22 if k == 2 else 60)
+    res = []
+    for merge, groups in (("1", "0"), ("1", "1"), ("0", "0")):
+        monkeypatch.setenv("DRESS_HIP_HIST_MERGE", merge)
+        monkeypatch.setenv("DRESS_HIP_BATCH_GROUP_VALUES", groups)
+        r = gpu.lifted_batch(graphs, k=k, kind=kind, precision=precision)
+        res.append([(np.array(v), np.array(c)) for v, c in r.histograms])
+    for other in res[1:]:
+        for (v0, c0), (v1, c1) in zip(res[0], other):
+            assert np.array_equal(v0, v1) and np.array_equal(c0, c1)
+    if precision == 0:
+        for (hv, hc), (n, U, V) in zip(res[0], graphs):
+            ov, oc, _, _ = ora.lifted(kind, n, U, V, k=k)
+            assert np.array_equal(hv, ov) and np.array_equal(hc, oc)
