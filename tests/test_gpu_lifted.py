@@ -333,3 +333,31 @@ def test_histogram_merge_equals_radix(gpu, ora, gen, monkeypatch, kind, k, preci
         for (hv, hc), (n, U, V) in zip(res[0], graphs):
             ov, oc, _, _ = ora.lifted(kind, n, U, V, k=k)
             assert np.array_equal(hv, ov) and np.array_equal(hc, oc)
+
+
+@pytest.mark.parametrize("variant", [0, 1, 2, 3])
+def test_grouped_weighted_variants_equal_oracle(gpu, ora, gen, monkeypatch, variant):
+    """Weighted graphs of every variant (DIRECTED with repeated reciprocal
+    edges of different weights, whose subgraphs get patched weight copies)
+    in a pipelined batch of 8 groups: each graph's delta^1 histogram equals
+    the one-group batch's and the oracle's."""
+    i32 = np.int32
+    rng = np.random.default_rng(11 + variant)
+    graphs, weights = [], []
+    for i in range(24):
+        n, U, V = gen.random_regular(int(rng.integers(8, 30)) // 2 * 2, 4, 100 + i)
+        if variant == 1 and i % 3 == 0:
+            # a repeated reciprocal pair of another weight: u->v, v->u twice
+            U = np.concatenate([U, V[:1], V[:1]]).astype(i32)
+            V = np.concatenate([V, U[:1], U[:1]]).astype(i32)
+        graphs.append((n, U, V))
+        weights.append(rng.uniform(0.2, 2.0, U.size))
+    res = []
+    for groups in ("0", "1"):
+        monkeypatch.setenv("DRESS_HIP_BATCH_GROUP_VALUES", groups)
+        r = gpu.lifted_batch(graphs, k=1, variant=variant, weights=weights)
+        res.append([(np.array(v), np.array(c)) for v, c in r.histograms])
+    for (v0, c0), (v1, c1), (n, U, V), W in zip(res[0], res[1], graphs, weights):
+        ov, oc, _, _ = ora.lifted("delta", n, U, V, W, variant=variant, k=1)
+        assert np.array_equal(v0, ov) and np.array_equal(c0, oc), (variant, n)
+        assert np.array_equal(v1, ov) and np.array_equal(c1, oc), (variant, n)
