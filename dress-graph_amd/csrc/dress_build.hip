@@ -352,20 +352,30 @@ __global__ void k_order_positions(int2 *pos_in, int E)
 // endpoint of larger degree, so the edges of one hub share its row of values.
 // Ties go to the lower id.
 __global__ void k_order_keys(const int64_t *cnt_in, const int2 *pos_in, const int *U, const int *V,
-                             const int *row_off, int E, int low_bins, uint64_t *key, int *val)
+                             const int *row_off, int E, int low_bins, int tile_bins, int tile_log, uint64_t *key,
+                             int *val)
 {
     for (int e = blockIdx.x * blockDim.x + threadIdx.x; e < E; e += gridDim.x * blockDim.x) {
         int2 p = pos_in[e];
         int own = p.x;
         const int bin = term_bin(cnt_in[e] + 1);
+        // (bin, position) with the bin above bit 56
+        uint64_t k = ((uint64_t)bin << 56) | (uint32_t)own;
         if (p.y >= 0) {
             int u = U[e], v = V[e];
             int du = row_off[u + 1] - row_off[u], dv = row_off[v + 1] - row_off[v];
             bool u_high = du > dv || (du == dv && u < v);
             int o = (((low_bins >> bin) & 1) ? !u_high : u_high) ? u : v;
             own = (p.x >= row_off[o] && p.x < row_off[o + 1]) ? p.x : p.y;
+            k = ((uint64_t)bin << 56) | (uint32_t)own;
+            if ((tile_bins >> bin) & 1) {
+                // blocks of 2^tile_log owners, within a block by the other
+                // end (labels below 2^28: the host checks)
+                const int other = o == u ? v : u;
+                k = ((uint64_t)bin << 56) | ((uint64_t)((unsigned)o >> tile_log) << 28) | (uint64_t)(unsigned)other;
+            }
         }
-        key[e] = ((uint64_t)bin << 32) | (uint32_t)own;
+        key[e] = k;
         val[e] = e;
     }
 }
@@ -562,6 +572,27 @@ constexpr int64_t RELABEL_MIN_HALF_EDGES = 1 << 18;
 // low-degree vertex and a hub) by default, measured 12 % faster there on a
 // 4M-vertex power-law graph and slower in bin 1; DRESS_HIP_LOW_ORDER_BINS
 // overrides the mask
+// bins whose edges go in blocks of 2^tile_log consecutive owners (by
+// label), ordered by their other endpoint within a block, so that the
+// other endpoint's row serves the block's owners from L2: bin 3 (edges of
+// 513..8192 terms, between two hubs) in blocks of 32 owners by default.  On
+// the 4M-vertex power-law graph bin 3 took 1.115 -> 0.998 ms per iteration
+// (blocks of 2..64 owners: 1.085, 1.061, 1.023, 1.011, 0.998, 1.057 ms); the
+// same order slowed bins 1 and 2 at every block size tried.
+// DRESS_HIP_TILE_BINS=<mask> and DRESS_HIP_TILE_LOG=<log2 owners per block>
+// override.
+int tile_bins()
+{
+    const char *e = getenv("DRESS_HIP_TILE_BINS");
+    return (e && *e) ? (int)strtol(e, nullptr, 0) & 0x1f : 0x8;
+}
+int tile_log()
+{
+    const char *e = getenv("DRESS_HIP_TILE_LOG");
+    const int v = (e && *e) ? atoi(e) : 5;
+    return v < 0 ? 0 : (v > 22 ? 22 : v);
+}
+
 int low_order_bins()
 {
     const char *e = getenv("DRESS_HIP_LOW_ORDER_BINS");
@@ -912,7 +943,8 @@ bool build_index(DevGraph *G)
                    dalloc(&v0, (size_t)E + 1, "order vals") && dalloc(&cd, Bins::NB, "bin counts");
         if (ok2) {
             hipLaunchKernelGGL(k_order_keys, dim3(grid_for(E, 8192)), dim3(TPB), 0, s, G->cnt_in, G->pos_in, G->U, G->V,
-                               G->row_off, E, low_order_bins(), k0, v0);
+                               G->row_off, E, low_order_bins(), G->N < (1 << 28) ? tile_bins() : 0, tile_log(), k0,
+                               v0);
             ok2 = prim_sort_pairs_u64_i32(k0, k1, v0, G->ord, (size_t)E, 64, s);
         }
         ok2 = ok2 && hipMemsetAsync(cd, 0, Bins::NB * sizeof(int), s) == hipSuccess;
