@@ -352,7 +352,7 @@ __global__ void k_order_positions(int2 *pos_in, int E)
 // endpoint of larger degree, so the edges of one hub share its row of values.
 // Ties go to the lower id.
 __global__ void k_order_keys(const int64_t *cnt_in, const int2 *pos_in, const int *U, const int *V,
-                             const int *row_off, int E, int low_bins, int tile_bins, int tile_log, uint64_t *key,
+                             const int *row_off, int E, int low_bins, int tile_bins, int tile_logs, uint64_t *key,
                              int *val)
 {
     for (int e = blockIdx.x * blockDim.x + threadIdx.x; e < E; e += gridDim.x * blockDim.x) {
@@ -372,6 +372,7 @@ __global__ void k_order_keys(const int64_t *cnt_in, const int2 *pos_in, const in
                 // blocks of 2^tile_log owners, within a block by the other
                 // end (labels below 2^28: the host checks)
                 const int other = o == u ? v : u;
+                const int tile_log = (tile_logs >> (5 * bin)) & 31;
                 k = ((uint64_t)bin << 56) | ((uint64_t)((unsigned)o >> tile_log) << 28) | (uint64_t)(unsigned)other;
             }
         }
@@ -586,11 +587,20 @@ int tile_bins()
     const char *e = getenv("DRESS_HIP_TILE_BINS");
     return (e && *e) ? (int)strtol(e, nullptr, 0) & 0x1f : 0x8;
 }
-int tile_log()
+// log2 owners per block of each bin, 5 bits a bin (bin b at bits 5b..5b+4)
+int tile_logs()
 {
-    const char *e = getenv("DRESS_HIP_TILE_LOG");
-    const int v = (e && *e) ? atoi(e) : 5;
-    return v < 0 ? 0 : (v > 22 ? 22 : v);
+    int l[5] = {0, 0, 0, 5, 0};
+    if (const char *e = getenv("DRESS_HIP_TILE_LOG")) {
+        int v[5], n = sscanf(e, "%d,%d,%d,%d,%d", &v[0], &v[1], &v[2], &v[3], &v[4]);
+        if (n == 1)
+            for (int b = 0; b < 5; b++) l[b] = v[0];
+        else
+            for (int b = 0; b < n; b++) l[b] = v[b];
+    }
+    int packed = 0;
+    for (int b = 0; b < 5; b++) packed |= (l[b] < 0 ? 0 : (l[b] > 22 ? 22 : l[b])) << (5 * b);
+    return packed;
 }
 
 int low_order_bins()
@@ -943,7 +953,7 @@ bool build_index(DevGraph *G)
                    dalloc(&v0, (size_t)E + 1, "order vals") && dalloc(&cd, Bins::NB, "bin counts");
         if (ok2) {
             hipLaunchKernelGGL(k_order_keys, dim3(grid_for(E, 8192)), dim3(TPB), 0, s, G->cnt_in, G->pos_in, G->U, G->V,
-                               G->row_off, E, low_order_bins(), G->N < (1 << 28) ? tile_bins() : 0, tile_log(), k0,
+                               G->row_off, E, low_order_bins(), G->N < (1 << 28) ? tile_bins() : 0, tile_logs(), k0,
                                v0);
             ok2 = prim_sort_pairs_u64_i32(k0, k1, v0, G->ord, (size_t)E, 64, s);
         }
