@@ -579,9 +579,10 @@ constexpr int64_t RELABEL_MIN_HALF_EDGES = 1 << 18;
 // 513..8192 terms, between two hubs) in blocks of 32 owners by default.  On
 // the 4M-vertex power-law graph bin 3 took 1.115 -> 0.998 ms per iteration
 // (blocks of 2..64 owners: 1.085, 1.061, 1.023, 1.011, 0.998, 1.057 ms); the
-// same order slowed bins 1 and 2 at every block size tried.
-// DRESS_HIP_TILE_BINS=<mask> and DRESS_HIP_TILE_LOG=<log2 owners per block>
-// override.
+// same order slowed bins 0, 1 and 2 at every block size tried (bin 0 with
+// 4..65536 owners: 1.07..1.36 ms against 1.056).
+// DRESS_HIP_TILE_BINS=<mask> and DRESS_HIP_TILE_LOG=<log2 owners per block:
+// one value for every bin, or "l0,l1,l2,l3,l4"> override.
 int tile_bins()
 {
     const char *e = getenv("DRESS_HIP_TILE_BINS");
