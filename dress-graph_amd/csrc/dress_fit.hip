@@ -10,8 +10,17 @@
 // so the host only reads the state every few iterations.  wh[] holds one
 // copy of w-bar*d per CSR entry: the vertex phase streams rows, and the
 // common-neighbour terms of an edge are read at increasing positions of its
-// two rows, so neighbouring lanes share cache lines.  Index and per-edge
-// streams use non-temporal loads.
+// two rows, so neighbouring lanes share cache lines.
+//
+// What bounds these kernels is the number of L1 -> L2 requests: a CU keeps
+// about 100 of them in flight, so time ~ requests x latency.  The index and
+// the per-edge data of bins 1-4 and the vertex rows are therefore read
+// through the caches: the G lanes of an edge read its index in slices of G
+// entries, four slices a round, and one cached line serves all four.
+// Non-temporal loads kept no line in L1 and made each slice a request of
+// its own (4M-vertex fit 134.0 -> 125.4 ms, same values).  Bin 0 streams
+// each edge's data once and keeps non-temporal loads, which measured
+// slightly faster there.
 //
 // Sums are exact-rounding compensated sums (TwoSum per term, pairwise
 // TwoSum merges across lanes): the result is the correctly rounded sum of
@@ -108,11 +117,11 @@ template <bool PK>
 __device__ __forceinline__ void entry(const EdgeArgs &a, int64_t k, int2 r, int &x, int &y)
 {
     if (PK) {
-        uint32_t q = ntl(a.ipk + k);
+        uint32_t q = a.ipk[k];
         x = r.x + (int)(q & 0xffffu);
         y = r.y + (int)(q >> 16);
     } else {
-        int2 p = ntl2(a.ipair + k);
+        int2 p = a.ipair[k];
         x = p.x;
         y = p.y;
     }
@@ -129,17 +138,17 @@ __device__ __forceinline__ void vertex_rows(const int *__restrict__ list, int co
     const int lane = (G == TPB) ? threadIdx.x : (threadIdx.x % G);
     const int64_t groups = (int64_t)nblk * (TPB / G);
     for (int64_t g = (int64_t)blk * (TPB / G) + threadIdx.x / G; g < count; g += groups) {
-        int u = ntl(list + g);
+        int u = list[g];
         int b = row_off[u], e = row_off[u + 1];
         double s = 0.0, c = 0.0;
         if (lane == 0) s = 4.0 * (UNI ? nw_c : nw[u]);
         int i = b + lane;
         for (; i + G < e; i += 2 * G) {
-            double x0 = (double)ntl(wh + i), x1 = (double)ntl(wh + i + G);
+            double x0 = (double)wh[i], x1 = (double)wh[i + G];
             two_sum_add(s, c, x0);
             two_sum_add(s, c, x1);
         }
-        for (; i < e; i += G) two_sum_add(s, c, (double)ntl(wh + i));
+        for (; i < e; i += G) two_sum_add(s, c, (double)wh[i]);
         if (G == TPB) block_reduce(s, c);
         else group_reduce<G>(s, c);
         if (lane == 0) vd[u] = sqrt(s + c);
@@ -231,8 +240,8 @@ __device__ __forceinline__ void edge_range(int lo, int hi, int nx, int xs, int j
     const int64_t groups = (int64_t)K * (TPB / G);
     for (int64_t g = s_lo + (int64_t)j * (TPB / G) + threadIdx.x / G; g < s_hi; g += groups) {
         const int e = (int)g;
-        int64_t kb = ntl(a.ioff + e), ke = ntl(a.ioff + e + 1);
-        int2 r = PK ? ntl2(a.rs + e) : make_int2(0, 0);
+        int64_t kb = a.ioff[e], ke = a.ioff[e + 1];
+        int2 r = PK ? a.rs[e] : make_int2(0, 0);
         double s = 0.0, c = 0.0;
         int64_t k = kb + lane;
         // four index entries and eight gathers in flight per lane
@@ -259,9 +268,9 @@ __device__ __forceinline__ void edge_range(int lo, int hi, int nx, int xs, int j
         double de = 0.0, w = 0.0, c4 = 0.0;
         if (lane == 0) {
             de = (double)dc[e];
-            w = UNI ? a.ew_c : ntl(a.ew + e);
+            w = UNI ? a.ew_c : a.ew[e];
             double own = w * de;
-            c4 = UNI ? a.c4_c : ntl(a.c4 + e);
+            c4 = UNI ? a.c4_c : a.c4[e];
             double t0 = a.directed ? (c4 + own) : (c4 + 2.0 * own);
             two_sum_add(s, c, t0);
         }
@@ -273,13 +282,13 @@ __device__ __forceinline__ void edge_range(int lo, int hi, int nx, int xs, int j
             // other sum) and it takes no part in the change
             const bool dead = !(c4 == c4);
             double num = s + c;
-            int2 p = ntl2(a.uv + e);
+            int2 p = a.uv[e];
             double den = a.vd[p.x] * a.vd[p.y];
             double x = dead ? 0.0 : (den > 0.0 ? num / den : 0.0);
             if (!dead) mymax = fmax(mymax, fabs(de - x));
             dn[e] = (VT)x;
             VT wx = (VT)(w * x);
-            int2 q = ntl2(a.pos + e);
+            int2 q = a.pos[e];
             wn[q.x] = wx;
             if (q.y >= 0) wn[q.y] = wx;
         }
