@@ -1015,8 +1015,14 @@ bool build_bins(DevGraph *G)
 bool refresh_self_terms(DevGraph *G, const double *NW_host)
 {
     G->fit_state_valid = false;
+    if (!NW_host && G->self_terms_ones) return true;
+    G->self_terms_ones = false;
     G->uni_state = 0;
     hipStream_t s = G->stream;
+    // no vertex weights: every vertex weighs 1.0 (the device copy may still
+    // hold an earlier call's weights)
+    if (!NW_host && G->N > 0)
+        hipLaunchKernelGGL(k_fill_f64, dim3(grid_for(G->N, 4096)), dim3(TPB), 0, s, G->nw, (int64_t)G->N, 1.0);
     if (NW_host && G->N > 0) {
         if (!G->vstage && !dalloc(&G->vstage, (size_t)G->N + 1, "vertex staging")) return false;
         DH_CHECK(hipMemcpyAsync(G->vstage, NW_host, (size_t)G->N * sizeof(double), hipMemcpyHostToDevice, s));
@@ -1031,6 +1037,8 @@ bool refresh_self_terms(DevGraph *G, const double *NW_host)
                                G->c4);
     }
     DH_CHECK(hipGetLastError());
+    // the internal self terms exist only once the index is built
+    G->self_terms_ones = !NW_host && (G->have_index || G->E == 0);
     return true;
 }
 

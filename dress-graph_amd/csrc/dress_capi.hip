@@ -1071,8 +1071,10 @@ int dress_hip_resident_fill(p_dress_graph_t g, double value)
     clear_error();
     DevGraph *G = g ? device_graph(g) : nullptr;
     if (!G) return -1;
-    if (!refresh_self_terms(G, g->NW)) return -1;
-    return fit_fill_values(G, value) ? 0 : -1;
+    // only the values change: vertex weights and self terms stay as the
+    // last reset or fit loaded them
+    G->fit_state_valid = false;
+    return fit_fill_values(G, value, true) ? 0 : -1;
 }
 
 int dress_hip_time_phases(p_dress_graph_t g, int reps, double *ms)

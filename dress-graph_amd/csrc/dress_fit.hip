@@ -820,6 +820,7 @@ bool fit_alloc_state(DevGraph *G, bool fp32)
 bool fit_upload_values(DevGraph *G, const double *edge_dress)
 {
     G->fit_state_valid = false;
+    G->wh_ready = false;
     if (!fit_alloc_state(G, false)) return false;
     G->cur = 0;
     if (G->E == 0) return true;
@@ -926,7 +927,11 @@ bool fit_run_once(DevGraph *G, int max_iterations, double epsilon, bool mixed, F
     DH_CHECK(hipEventCreate(&ev1));
     int egrid = grid_for(G->E, num_cus() * 8);
     DH_CHECK(hipEventRecord(ev0, s));
-    if (G->E > 0) {
+    // half-edge copies prepared by a fill of the current values (FP64 only:
+    // a mixed fit starts from its FP32 copies)
+    const bool wh_ready = G->wh_ready && !mixed && G->cur == 0 && G->uni_state == 1;
+    G->wh_ready = false;
+    if (G->E > 0 && !wh_ready) {
         if (mixed)
             hipLaunchKernelGGL((k_init_wh<float>), dim3(egrid), dim3(TPB), 0, s, G->E, G->ew, G->pos, G->d[0],
                                G->d[1], G->d32[0], G->d32[1], G->wh32[0], G->wh32[1], st);
@@ -1061,19 +1066,27 @@ bool fit_run(DevGraph *G, int max_iterations, double epsilon, int precision, Fit
 }
 
 namespace {
-__global__ void k_fill_vals(double *d, int n, double v)
+__global__ void k_fill_vals(double *d, int64_t n, double v)
 {
-    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) d[i] = v;
+    for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x)
+        d[i] = v;
 }
 }  // namespace
 
-bool fit_fill_values(DevGraph *G, double value)
+bool fit_fill_values(DevGraph *G, double value, bool prepare_wh)
 {
+    G->wh_ready = false;
     if (!fit_alloc_state(G, false)) return false;
     G->cur = 0;
     if (G->E > 0)
-        hipLaunchKernelGGL(k_fill_vals, dim3(grid_for(G->E, num_cus() * 8)), dim3(TPB), 0, G->stream, G->d[0], G->E,
-                           value);
+        hipLaunchKernelGGL(k_fill_vals, dim3(grid_for(G->E, num_cus() * 8)), dim3(TPB), 0, G->stream, G->d[0],
+                           (int64_t)G->E, value);
+    if (prepare_wh && G->uni_state == 1 && G->E > 0) {
+        // every half-edge copy is w-bar * value, the product k_init_wh forms
+        hipLaunchKernelGGL(k_fill_vals, dim3(grid_for(G->S, num_cus() * 8)), dim3(TPB), 0, G->stream, G->wh[0],
+                           (int64_t)G->S, G->ew_c * value);
+        G->wh_ready = true;
+    }
     DH_CHECK(hipGetLastError());
     DH_CHECK(hipStreamSynchronize(G->stream));
     return true;

@@ -202,6 +202,13 @@ struct DevGraph {
     // three values as arguments instead of reading the arrays.  uni_state:
     // 0 unknown (every writer of ew / c4 / nw resets it), 1 uniform, 2 not.
     int uni_state = 0;
+    // the self terms (c4) were last computed from unit vertex weights and
+    // nothing has written nw or c4 since: a refresh without vertex weights
+    // has nothing to do
+    bool self_terms_ones = false;
+    // wh[cur] holds w-bar * d[cur] for the current values (set only by a
+    // fill that prepared the half-edge copies, taken by the next fit)
+    bool wh_ready = false;
     double ew_c = 0.0, c4_c = 0.0, nw_c = 0.0;
     int cur = 0;                   // buffer holding the current values
 };
@@ -234,7 +241,10 @@ bool fit_upload_values(DevGraph *G, const double *edge_dress);
 bool fit_run(DevGraph *G, int max_iterations, double epsilon, int precision,
              FitResult *res);
 bool fit_download(DevGraph *G, double *edge_dress, double *vertex_dress);
-bool fit_fill_values(DevGraph *G, double value);
+// Sets every current edge value to `value`.  With prepare_wh (graphs whose
+// w-bar is uniform) the half-edge copies are filled too, so the next FP64
+// fit skips their set-up; any later write of the values clears that.
+bool fit_fill_values(DevGraph *G, double value, bool prepare_wh = false);
 bool fit_time_phases(DevGraph *G, int reps, double *ms);
 
 // dress_prim.hip (rocPRIM wrappers, stream-ordered, scratch allocated inside)

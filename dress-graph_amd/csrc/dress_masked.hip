@@ -138,6 +138,7 @@ bool masked_items(p_dress_graph_t g, DevGraph *G, int kind, int k, const std::ve
     // items mark removed edges and vertex weights per fit: not uniform
     // until the graph's own self terms are back
     G->uni_state = 2;
+    G->self_terms_ones = false;
     hipStream_t s = G->stream;
     const int E = G->E, N = G->N;
     const int directed = (G->variant == DRESS_VARIANT_FORWARD || G->variant == DRESS_VARIANT_BACKWARD) ? 1 : 0;

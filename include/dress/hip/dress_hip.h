@@ -171,7 +171,9 @@ int dress_hip_fit_resident(p_dress_graph_t g, int max_iterations, double epsilon
 int dress_hip_resident_download(p_dress_graph_t g);
 
 /* Sets every resident edge value to `value` on the device (a fresh start is
- * value = 1.0, the state dress_init_graph creates).  Returns 0 or -1. */
+ * value = 1.0, the state dress_init_graph creates).  Vertex weights stay as
+ * the last dress_hip_resident_reset or fit read them from g->NW.  Returns 0
+ * or -1. */
 int dress_hip_resident_fill(p_dress_graph_t g, double value);
 
 /* Device time of the kernels of one FP64 iteration, averaged over `reps`

@@ -276,3 +276,47 @@ def test_graph_captured_iterations(gpu, ora, gen, monkeypatch):
             b = ora.fit(n, U, V, max_iterations=100, epsilon=eps)
             assert (it, dl) == (b["iterations"], b["delta"])
             assert np.array_equal(g.edge_dress, b["edge_dress"])
+
+
+def _oracle_from(ora, n, U, V, W, NW, variant, value, max_it, eps):
+    """oracle fit of the graph whose every edge value starts at `value`"""
+    og = ora.graph(n, U, V, W, NW, variant, 0)
+    try:
+        np.ctypeslib.as_array(og.contents.edge_dress, shape=(U.size,))[:] = value
+        it, dl = ora.fit_graph(og, max_it, eps)
+        d = np.ctypeslib.as_array(og.contents.edge_dress, shape=(U.size,)).copy()
+        vd = np.ctypeslib.as_array(og.contents.vertex_dress, shape=(n,)).copy()
+    finally:
+        ora.free_graph(og)
+    return it, dl, d, vd
+
+
+@pytest.mark.parametrize("case", ["uniform", "weighted", "vertex_weights", "directed"])
+def test_resident_fill_then_fit_equals_oracle(gpu, ora, gen, case):
+    """dress_hip_resident_fill + dress_hip_fit_resident: a fill of a uniform
+    graph prepares the half-edge copies itself (the next FP64 fit skips
+    their set-up); every case equals the oracle's fit from the same start,
+    also after an earlier fit moved the values, and a mixed fit after a fill
+    stays within the mixed tolerance."""
+    n, U, V = gen.er_single(300, 0.04, 9)
+    rng = np.random.default_rng(5)
+    W = rng.uniform(0.1, 1.0, U.size) if case == "weighted" else None
+    NW = rng.uniform(0.5, 2.0, n) if case == "vertex_weights" else None
+    variant = 1 if case == "directed" else 0
+    with gpu.Graph(n, U, V, W, NW, variant) as g:
+        g.resident_reset()
+        g.fit_resident(4, 1e-6, gpu.FP64)            # values away from the start
+        for value in (1.0, 0.5, 1.0):
+            g.resident_fill(value)
+            it, dl = g.fit_resident(100, 1e-6, gpu.FP64)
+            g.resident_download()
+            oit, odl, od, ovd = _oracle_from(ora, n, U, V, W, NW, variant, value, 100, 1e-6)
+            assert (it, dl) == (oit, odl), (case, value)
+            assert np.array_equal(g.edge_dress, od), (case, value)
+            assert np.array_equal(g.vertex_dress, ovd), (case, value)
+        g.resident_fill(1.0)
+        it, dl = g.fit_resident(100, 1e-6, gpu.FP32_FP64)
+        g.resident_download()
+        oit, odl, od, _ = _oracle_from(ora, n, U, V, W, NW, variant, 1.0, 100, 1e-6)
+        assert it == oit
+        assert np.max(np.abs(g.edge_dress - od)) <= TOL_MIXED
