@@ -174,11 +174,11 @@ template <bool PK>
 __device__ __forceinline__ void mentry(const MultiArgs &a, int64_t k, int2 r, int &x, int &y)
 {
     if (PK) {
-        uint32_t q = ntl(a.ipk + k);
+        uint32_t q = a.ipk[k];
         x = r.x + (int)(q & 0xffffu);
         y = r.y + (int)(q >> 16);
     } else {
-        int2 p = ntl2(a.ipair + k);
+        int2 p = a.ipair[k];
         x = p.x;
         y = p.y;
     }
