@@ -586,9 +586,34 @@ EdgeArgs edge_args(DevGraph *Gr)
     return a;
 }
 
-// grid of an edge launch over n edges with g lanes each, and its XCD slices
-inline void edge_grid(int64_t n, int g, int capacity, int *grid, int *nx)
+// Workgroups per CU of each edge bin's launch.  Fewer than fit (8) keep
+// fewer rows competing for each CU's L1 and each XCD's L2: on the 4M-vertex
+// graph 4 per CU took bins 0, 1 and 3 from 1.053, 1.015, 0.958 to 1.024,
+// 0.969, 0.943 ms and 3 took bin 2 from 1.723 to 1.546 ms (2 and 1 were
+// slower everywhere), the same values throughout.
+// DRESS_HIP_EDGE_BLOCKS="b0,b1,b2,b3,b4" overrides them (1..8 each; a
+// tuning knob: results do not depend on it).
+const int *edge_blocks_per_cu()
 {
+    static int blocks[Bins::NB] = {4, 4, 3, 4, 4};
+    static bool once = [] {
+        if (const char *e = getenv("DRESS_HIP_EDGE_BLOCKS")) {
+            int v[Bins::NB];
+            if (sscanf(e, "%d,%d,%d,%d,%d", &v[0], &v[1], &v[2], &v[3], &v[4]) == Bins::NB)
+                for (int i = 0; i < Bins::NB; i++)
+                    if (v[i] >= 1 && v[i] <= 8) blocks[i] = v[i];
+        }
+        return true;
+    }();
+    (void)once;
+    return blocks;
+}
+
+// grid of an edge launch of bin `bin` over n edges with g lanes each (at most
+// `capacity` workgroups, what fits on the device at once), and its XCD slices
+inline void edge_grid(int bin, int64_t n, int g, int capacity, int *grid, int *nx)
+{
+    capacity = std::min(capacity, edge_blocks_per_cu()[bin] * num_cus());
     *grid = grid_for(n * g, capacity);
     *nx = num_xcds();
     if (*grid >= 4 * *nx) *grid -= *grid % *nx;
@@ -602,12 +627,12 @@ void launch_edge_t(DevGraph *Gr, int bin, VT *d0, VT *d1, VT *w0, VT *w1)
     if (hi <= lo) return;
     int grid, nx;
     if (G == 1 && Gr->b0off) {
-        edge_grid(hi - lo, 1, edge0_capacity<VT, UNI>(), &grid, &nx);
+        edge_grid(bin, hi - lo, 1, edge0_capacity<VT, UNI>(), &grid, &nx);
         hipLaunchKernelGGL((k_edge0<VT, UNI>), dim3(grid), dim3(TPB), 0, Gr->stream, lo, hi, nx, edge_args(Gr), d0, d1,
                            w0, w1, (FitState *)Gr->state);
         return;
     }
-    edge_grid(hi - lo, G, edge_capacity<G, PK, VT, UNI>(), &grid, &nx);
+    edge_grid(bin, hi - lo, G, edge_capacity<G, PK, VT, UNI>(), &grid, &nx);
     hipLaunchKernelGGL((k_edge<G, PK, VT, UNI>), dim3(grid), dim3(TPB), 0, Gr->stream, lo, hi, nx, edge_args(Gr), d0,
                        d1, w0, w1, (FitState *)Gr->state);
 }
