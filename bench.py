@@ -59,13 +59,13 @@ L_ENTRY, L_HALF, L_EDGE, L_VERTEX = 16, 8, 40, 12
 # (rocprofv3 --pmc TCC_HIT_sum TCC_MISS_sum, then --pmc WRITE_SIZE, over
 # `python bench.py --traffic-run`): 64 B per L2 miss plus WRITE_SIZE KiB.
 # Calibrated on two probes of known bytes in the same runs: the 1 GiB
-# streaming copy reads 16,777,388 misses x 64 B = 1 GiB and writes
-# 1,048,622 KiB; 2^26 random 8-byte gathers from 1 GiB take one miss each
+# streaming copy reads 16,777,385 misses x 64 B = 1 GiB and writes
+# 1,048,618 KiB; 2^26 random 8-byte gathers from 1 GiB take one miss each
 # (FETCH_SIZE counts a streaming read at half its bytes and a random one at
 # 64 B, so it is not used).  Counters cannot be read inside this process, so
 # the figure is labelled with its source and reported only when that kernel
 # is the dominant one.
-TRAFFIC = {"kernel": "k_edge<8,packed,double> (edge bin 2)", "tcc_miss": 87916958, "write_kib": 103501,
+TRAFFIC = {"kernel": "k_edge<8,packed,double> (edge bin 2)", "tcc_miss": 77734136, "write_kib": 103452,
            "source": "profiles/fit_power_law_4M_fp64.md, counter runs of bench.py --traffic-run"}
 TRAFFIC["bytes_per_launch"] = int(TRAFFIC["tcc_miss"] * 64 + TRAFFIC["write_kib"] * 1024)
 GOLDEN = os.path.join(ROOT, "tests", "golden")
