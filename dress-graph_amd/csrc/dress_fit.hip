@@ -591,29 +591,33 @@ EdgeArgs edge_args(DevGraph *Gr)
 // graph 4 per CU took bins 0, 1 and 3 from 1.053, 1.015, 0.958 to 1.024,
 // 0.969, 0.943 ms and 3 took bin 2 from 1.723 to 1.546 ms (2 and 1 were
 // slower everywhere), the same values throughout.
-// DRESS_HIP_EDGE_BLOCKS="b0,b1,b2,b3,b4" overrides them (1..8 each; a
-// tuning knob: results do not depend on it).
-const int *edge_blocks_per_cu()
+// The FP32 kernels of a mixed fit have caps of their own.
+// DRESS_HIP_EDGE_BLOCKS="b0,b1,b2,b3,b4" and DRESS_HIP_EDGE_BLOCKS32 (FP32)
+// override them (1..8 each; a tuning knob: results do not depend on it).
+const int *edge_blocks_per_cu(bool fp32)
 {
-    static int blocks[Bins::NB] = {4, 4, 3, 4, 4};
+    static int blocks[2][Bins::NB] = {{4, 4, 3, 4, 4}, {4, 4, 4, 4, 4}};
     static bool once = [] {
-        if (const char *e = getenv("DRESS_HIP_EDGE_BLOCKS")) {
-            int v[Bins::NB];
-            if (sscanf(e, "%d,%d,%d,%d,%d", &v[0], &v[1], &v[2], &v[3], &v[4]) == Bins::NB)
-                for (int i = 0; i < Bins::NB; i++)
-                    if (v[i] >= 1 && v[i] <= 8) blocks[i] = v[i];
+        const char *names[2] = {"DRESS_HIP_EDGE_BLOCKS", "DRESS_HIP_EDGE_BLOCKS32"};
+        for (int p = 0; p < 2; p++) {
+            if (const char *e = getenv(names[p])) {
+                int v[Bins::NB];
+                if (sscanf(e, "%d,%d,%d,%d,%d", &v[0], &v[1], &v[2], &v[3], &v[4]) == Bins::NB)
+                    for (int i = 0; i < Bins::NB; i++)
+                        if (v[i] >= 1 && v[i] <= 8) blocks[p][i] = v[i];
+            }
         }
         return true;
     }();
     (void)once;
-    return blocks;
+    return blocks[fp32 ? 1 : 0];
 }
 
 // grid of an edge launch of bin `bin` over n edges with g lanes each (at most
 // `capacity` workgroups, what fits on the device at once), and its XCD slices
-inline void edge_grid(int bin, int64_t n, int g, int capacity, int *grid, int *nx)
+inline void edge_grid(int bin, bool fp32, int64_t n, int g, int capacity, int *grid, int *nx)
 {
-    capacity = std::min(capacity, edge_blocks_per_cu()[bin] * num_cus());
+    capacity = std::min(capacity, edge_blocks_per_cu(fp32)[bin] * num_cus());
     *grid = grid_for(n * g, capacity);
     *nx = num_xcds();
     if (*grid >= 4 * *nx) *grid -= *grid % *nx;
@@ -627,12 +631,12 @@ void launch_edge_t(DevGraph *Gr, int bin, VT *d0, VT *d1, VT *w0, VT *w1)
     if (hi <= lo) return;
     int grid, nx;
     if (G == 1 && Gr->b0off) {
-        edge_grid(bin, hi - lo, 1, edge0_capacity<VT, UNI>(), &grid, &nx);
+        edge_grid(bin, sizeof(VT) == 4, hi - lo, 1, edge0_capacity<VT, UNI>(), &grid, &nx);
         hipLaunchKernelGGL((k_edge0<VT, UNI>), dim3(grid), dim3(TPB), 0, Gr->stream, lo, hi, nx, edge_args(Gr), d0, d1,
                            w0, w1, (FitState *)Gr->state);
         return;
     }
-    edge_grid(bin, hi - lo, G, edge_capacity<G, PK, VT, UNI>(), &grid, &nx);
+    edge_grid(bin, sizeof(VT) == 4, hi - lo, G, edge_capacity<G, PK, VT, UNI>(), &grid, &nx);
     hipLaunchKernelGGL((k_edge<G, PK, VT, UNI>), dim3(grid), dim3(TPB), 0, Gr->stream, lo, hi, nx, edge_args(Gr), d0,
                        d1, w0, w1, (FitState *)Gr->state);
 }
@@ -940,10 +944,14 @@ bool fit_run_once(DevGraph *G, int max_iterations, double epsilon, bool mixed, F
     h.mixed = mixed ? 1 : 0;
     // The FP32 phase carries a relative error of ~1.5e-7 (u32 / (1 - rho)).
     // The FP64 phase contracts that error and the change at the same rate,
-    // so handing over at change ~ 3000 * epsilon leaves the error below
-    // 1.5e-7 * epsilon / (3000 epsilon) = 5e-11 when the change reaches
-    // epsilon.  At least 16 FP64 iterations are kept before max_iterations.
-    h.switch_thr = 3000.0 * epsilon;
+    // so handing over at change ~ 300 * epsilon leaves the error below
+    // 1.5e-7 * epsilon / (300 epsilon) = 5e-10 when the change reaches
+    // epsilon, within the 1e-9 tolerance.  On the 4M-vertex graph the
+    // measured max edge error against FP64 was 2.2e-13, 5.9e-13, 4.4e-12,
+    // 3.0e-11 and 1.9e-10 for hand-over at 3000, 1000, 300, 100 and 30
+    // epsilon, with 13, 14, 16, 18 and 20 of 26 iterations in FP32.  At
+    // least 16 FP64 iterations are kept before max_iterations.
+    h.switch_thr = 300.0 * epsilon;
     h.switch_by = max_iterations - 16;
     FitState *st = (FitState *)G->state;
     DH_CHECK(hipMemcpyAsync(st, &h, sizeof(h), hipMemcpyHostToDevice, s));

@@ -83,7 +83,7 @@ def test_power_law_4m_fp64_equals_reference(gpu, chung_lu_4m):
 def test_power_law_4m_mixed_precision_error(gpu, chung_lu_4m):
     """FP32->FP64 on the resident graph against the FP64 fit of the same
     graph (itself equal to the reference's): same stopping step, every edge
-    within 1e-9; values in (0, 2]."""
+    within 1e-9; values in (0, 2] up to that tolerance."""
     gold, n, U, V = chung_lu_4m
     with gpu.Graph(n, U, V) as g:
         g.resident_reset()
@@ -102,7 +102,10 @@ def test_power_law_4m_mixed_precision_error(gpu, chung_lu_4m):
           f"max edge error {err:.3e}")
     assert it32 == it64 == gold["iterations"]
     assert err <= TOL
-    assert np.all(d32 > 0) and np.all(d32 <= 2.0)
+    # the fixed point lies in (0, 2]; the FP64 values (the reference's) do,
+    # and the mixed ones within the tolerance of them
+    assert np.all(d64 > 0) and np.all(d64 <= 2.0)
+    assert np.all(d32 > 0) and np.all(d32 <= 2.0 + TOL)
 
 
 # --------------------------------------------------------------- config 3
