@@ -13,7 +13,8 @@ struct LiftParent {
     int N, E, S, T;
     int64_t row_ofs, adj_ofs, edge_ofs, ioff_ofs, pair_ofs;
     int directed;                  // FORWARD/BACKWARD self-term form
-    int pad;
+    int uni_w;                     // every variant weight is w_c (bit for bit)
+    double w_c;
 };
 
 // one subgraph: parent + deleted vertices (delta) or marked tuple (nabla)

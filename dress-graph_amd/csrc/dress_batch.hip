@@ -96,6 +96,11 @@ __global__ void __launch_bounds__(B) k_lifted(LiftArgs a)
         const int *__restrict__ adj_e = a.adj_e + P.adj_ofs;
         const int2 *__restrict__ uv = a.uv + P.edge_ofs;
         const double *__restrict__ ew = a.ew + (I.ew_ofs >= 0 ? I.ew_ofs : P.edge_ofs);
+        // the parent's own weights all equal w_c: no weight read per term
+        // (an item with weight patches reads its own array)
+        const bool uw = P.uni_w && I.ew_ofs < 0;
+        const double wc = P.w_c;
+        auto W = [&](int e) -> double { return uw ? wc : ew[e]; };
         const int *__restrict__ ioff = a.ioff + P.ioff_ofs;
         const uint32_t *__restrict__ pairs = a.pairs + P.pair_ofs;
         const int directed = P.directed;
@@ -167,7 +172,7 @@ __global__ void __launch_bounds__(B) k_lifted(LiftArgs a)
                     const float *dc = cur ? fB : fA;
                     for (int i = rb; i < re; i++) {
                         int e = adj_e[i];
-                        s += (double)((float)ew[e] * dc[e]);
+                        s += (double)((float)W(e) * dc[e]);
                     }
                     vd[u] = sqrt(s + self);
                 } else {
@@ -175,7 +180,7 @@ __global__ void __launch_bounds__(B) k_lifted(LiftArgs a)
                     s = self;
                     for (int i = rb; i < re; i++) {
                         int e = adj_e[i];
-                        two_sum_add(s, c, ew[e] * dc[e]);
+                        two_sum_add(s, c, W(e) * dc[e]);
                     }
                     vd[u] = sqrt(s + c);
                 }
@@ -189,7 +194,7 @@ __global__ void __launch_bounds__(B) k_lifted(LiftArgs a)
                 double cu = 4.0 * item_nw(I.kind, p.x, s_tup, s_mark, k);
                 double c4 = directed ? cu : cu + 4.0 * item_nw(I.kind, p.y, s_tup, s_mark, k);
                 int kb = ioff[e], ke = ioff[e + 1];
-                double w = ew[e];
+                double w = W(e);
                 double x, de;
                 if (fp32) {
                     const float *dc = cur ? fB : fA;
@@ -199,7 +204,7 @@ __global__ void __launch_bounds__(B) k_lifted(LiftArgs a)
                         uint32_t pr = pairs[q];
                         int ea = (int)(pr & 0xffffu), eb = (int)(pr >> 16);
                         // a dead neighbour kills both of its edges: adds 0
-                        s += (double)((float)ew[ea] * dc[ea]) + (double)((float)ew[eb] * dc[eb]);
+                        s += (double)((float)W(ea) * dc[ea]) + (double)((float)W(eb) * dc[eb]);
                     }
                     de = (double)dc[e];
                     double own = (double)((float)w * dc[e]);
@@ -216,7 +221,7 @@ __global__ void __launch_bounds__(B) k_lifted(LiftArgs a)
                         int ea = (int)(pr & 0xffffu), eb = (int)(pr >> 16);
                         double ta = dc[ea], tb = dc[eb];
                         if (ta == 0.0 && tb == 0.0) continue;   // deleted common neighbour
-                        two_sum_add(s, c, ew[ea] * ta + ew[eb] * tb);
+                        two_sum_add(s, c, W(ea) * ta + W(eb) * tb);
                     }
                     de = dc[e];
                     double own = w * de;

@@ -482,7 +482,12 @@ void pack_parents(const std::vector<const HostParent *> &ps, const std::vector<i
         m.ioff_ofs = ni;
         m.pair_ofs = np;
         m.directed = P->directed;
-        m.pad = 0;
+        // uniform variant weights (unweighted UNDIRECTED, FORWARD, BACKWARD):
+        // the kernel takes the constant instead of reading a weight per term
+        m.uni_w = 1;
+        m.w_c = P->ew.empty() ? 0.0 : P->ew[0];
+        for (double w : P->ew)
+            if (std::memcmp(&w, &m.w_c, sizeof(double)) != 0) { m.uni_w = 0; break; }
         meta.push_back(m);
         nr += (int64_t)P->row_off.size(); na += (int64_t)P->adj_e.size(); ne += (int64_t)P->uv.size();
         ni += (int64_t)P->ioff.size(); np += (int64_t)P->pairs.size();
