@@ -45,6 +45,7 @@ struct LiftArgs {
     const uint32_t *pairs;         // per parent: T (edge(u,x) | edge(v,x) << 16)
     int *queue;                    // work counter (zeroed before launch)
     int ecap, ncap;                // LDS capacity in edges / vertices
+    int scap;                      // ... and half-edges (structure staged in LDS)
     int max_it;
     double eps;
     double *vals;                  // packed live values (or null)
@@ -96,8 +97,10 @@ __device__ __forceinline__ int64_t quant_slot(unsigned long long key, int64_t ca
 }
 
 size_t lifted_lds_bytes(int ecap, int ncap);
+// with the parent's structure staged in LDS as well (scap half-edges)
+size_t lifted_lds_bytes_staged(int ecap, int ncap, int scap);
 size_t lifted_lds_limit();
 size_t lifted_lds_per_cu();
-bool lifted_launch(const LiftArgs &a, int block, int grid, hipStream_t s);
+bool lifted_launch(const LiftArgs &a, int block, int grid, bool staged, hipStream_t s);
 
 }  // namespace dh

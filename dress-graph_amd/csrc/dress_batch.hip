@@ -67,8 +67,15 @@ __device__ __forceinline__ double block_max(double m, double *red)
 
 // One persistent kernel over all items.  LDS layout (dynamic):
 //   double dA[ecap], dB[ecap], vd[ncap]; int alive[ncap] (bytes as int)
-// FP32 phase uses the first halves of dA / dB as float arrays.
-template <int B>
+// FP32 phase uses the first halves of dA / dB as float arrays.  With ST the
+// parent's structure follows, copied in once per parent a workgroup meets:
+//   int row[ncap + 2], ioff[ecap + 2]; ushort2 uv[ecap]; ushort adj[scap]
+// (edge and vertex ids below 2^16: the batched kernel takes E <= 65535,
+// and N is bounded by the LDS far below that), so each iteration reads the
+// structure from LDS instead of through L1, whose accesses bounded the
+// kernel.  Edge weights that differ per edge and the index pairs stay in
+// global memory.
+template <int B, bool ST>
 __global__ void __launch_bounds__(B) k_lifted(LiftArgs a)
 {
     extern __shared__ double smem[];
@@ -76,6 +83,12 @@ __global__ void __launch_bounds__(B) k_lifted(LiftArgs a)
     double *dB = dA + a.ecap;
     double *vd = dB + a.ecap;
     int *alive = (int *)(vd + a.ncap);
+    int *s_row = alive + a.ncap;
+    int *s_ioff = s_row + a.ncap + 2;
+    ushort2 *s_uv = (ushort2 *)(s_ioff + a.ecap + 2);
+    unsigned short *s_adj = (unsigned short *)(s_uv + a.ecap);
+    __shared__ int s_staged;               // parent whose structure is in LDS
+    if (ST && threadIdx.x == 0) s_staged = -1;
     __shared__ int s_item;
     __shared__ double s_red[B / 64];
     __shared__ int s_tup[LIFT_KMAX];
@@ -104,6 +117,39 @@ __global__ void __launch_bounds__(B) k_lifted(LiftArgs a)
         const int *__restrict__ ioff = a.ioff + P.ioff_ofs;
         const uint32_t *__restrict__ pairs = a.pairs + P.pair_ofs;
         const int directed = P.directed;
+        if (ST && s_staged != I.parent) {
+            // every thread has read s_staged above; the previous item is done
+            for (int u = threadIdx.x; u <= N; u += B) s_row[u] = row_off[u];
+            for (int e = threadIdx.x; e <= E; e += B) s_ioff[e] = ioff[e];
+            for (int e = threadIdx.x; e < E; e += B) {
+                const int2 p = uv[e];
+                s_uv[e] = make_ushort2((unsigned short)p.x, (unsigned short)p.y);
+            }
+            for (int i = threadIdx.x; i < P.S; i += B) s_adj[i] = (unsigned short)adj_e[i];
+            __syncthreads();
+            if (threadIdx.x == 0) s_staged = I.parent;
+        }
+        // the structure from LDS (ST) or through the cache
+        auto ROW = [&](int u) -> int {
+            if constexpr (ST) return s_row[u];
+            else return row_off[u];
+        };
+        auto ADJ = [&](int i) -> int {
+            if constexpr (ST) return (int)s_adj[i];
+            else return adj_e[i];
+        };
+        auto UV = [&](int e) -> int2 {
+            if constexpr (ST) {
+                const ushort2 p = s_uv[e];
+                return make_int2((int)p.x, (int)p.y);
+            } else {
+                return uv[e];
+            }
+        };
+        auto IOFF = [&](int e) -> int {
+            if constexpr (ST) return s_ioff[e];
+            else return ioff[e];
+        };
 
         if (threadIdx.x < k) {
             s_tup[threadIdx.x] = a.item_verts[I.vofs + threadIdx.x];
@@ -122,7 +168,7 @@ __global__ void __launch_bounds__(B) k_lifted(LiftArgs a)
         bool fp32 = I.mixed != 0 && a.max_it > 0;
         float *fA = (float *)dA, *fB = (float *)dB;
         for (int e = threadIdx.x; e < E; e += B) {
-            int2 p = uv[e];
+            int2 p = UV(e);
             bool live = alive[p.x] && alive[p.y];
             if (fp32) { fA[e] = live ? 1.0f : 0.0f; fB[e] = 0.0f; }
             else { dA[e] = live ? 1.0 : 0.0; dB[e] = 0.0; }
@@ -135,15 +181,15 @@ __global__ void __launch_bounds__(B) k_lifted(LiftArgs a)
             for (int u = threadIdx.x; u < N; u += B) {
                 if (!alive[u]) continue;
                 w_n++;
-                for (int i = row_off[u]; i < row_off[u + 1]; i++) {
-                    const int e = adj_e[i];
+                for (int i = ROW(u); i < ROW(u + 1); i++) {
+                    const int e = ADJ(i);
                     w_s += (fp32 ? fA[e] != 0.0f : dA[e] != 0.0) ? 1 : 0;
                 }
             }
             for (int e = threadIdx.x; e < E; e += B) {
                 if (!(fp32 ? fA[e] != 0.0f : dA[e] != 0.0)) continue;
                 w_e++;
-                for (int q = ioff[e]; q < ioff[e + 1]; q++) {
+                for (int q = IOFF(e); q < IOFF(e + 1); q++) {
                     const uint32_t pr = pairs[q];
                     const int ea = (int)(pr & 0xffffu), eb = (int)(pr >> 16);
                     const bool la = fp32 ? fA[ea] != 0.0f : dA[ea] != 0.0;
@@ -163,7 +209,7 @@ __global__ void __launch_bounds__(B) k_lifted(LiftArgs a)
             // ---- vertex phase
             for (int u = threadIdx.x; u < N; u += B) {
                 if (!alive[u]) continue;
-                int rb = row_off[u], re = row_off[u + 1];
+                int rb = ROW(u), re = ROW(u + 1);
                 const double self = 4.0 * item_nw(I.kind, u, s_tup, s_mark, k);
                 double s = 0.0, c = 0.0;
                 if (fp32) {
@@ -171,7 +217,7 @@ __global__ void __launch_bounds__(B) k_lifted(LiftArgs a)
                     // order), then the self term
                     const float *dc = cur ? fB : fA;
                     for (int i = rb; i < re; i++) {
-                        int e = adj_e[i];
+                        int e = ADJ(i);
                         s += (double)((float)W(e) * dc[e]);
                     }
                     vd[u] = sqrt(s + self);
@@ -179,7 +225,7 @@ __global__ void __launch_bounds__(B) k_lifted(LiftArgs a)
                     const double *dc = cur ? dB : dA;
                     s = self;
                     for (int i = rb; i < re; i++) {
-                        int e = adj_e[i];
+                        int e = ADJ(i);
                         two_sum_add(s, c, W(e) * dc[e]);
                     }
                     vd[u] = sqrt(s + c);
@@ -189,11 +235,11 @@ __global__ void __launch_bounds__(B) k_lifted(LiftArgs a)
             // ---- edge phase
             double mymax = 0.0;
             for (int e = threadIdx.x; e < E; e += B) {
-                int2 p = uv[e];
+                int2 p = UV(e);
                 if (!alive[p.x] || !alive[p.y]) continue;
                 double cu = 4.0 * item_nw(I.kind, p.x, s_tup, s_mark, k);
                 double c4 = directed ? cu : cu + 4.0 * item_nw(I.kind, p.y, s_tup, s_mark, k);
-                int kb = ioff[e], ke = ioff[e + 1];
+                int kb = IOFF(e), ke = IOFF(e + 1);
                 double w = W(e);
                 double x, de;
                 if (fp32) {
@@ -262,7 +308,7 @@ __global__ void __launch_bounds__(B) k_lifted(LiftArgs a)
         if (a.ms) {
             double *row = a.ms + (int64_t)I.ms_row * a.ms_stride;
             for (int e = threadIdx.x; e < E; e += B) {
-                int2 p = uv[e];
+                int2 p = UV(e);
                 row[e] = (alive[p.x] && alive[p.y]) ? dc[e] : __longlong_as_double(0x7ff8000000000000ll);
             }
         }
@@ -277,7 +323,7 @@ __global__ void __launch_bounds__(B) k_lifted(LiftArgs a)
                 int e = e0 + threadIdx.x;
                 bool live = false;
                 if (e < E) {
-                    int2 p = uv[e];
+                    int2 p = UV(e);
                     live = alive[p.x] && alive[p.y];
                 }
                 // block-wide exclusive prefix of live flags, wave ballots
@@ -344,7 +390,7 @@ __global__ void __launch_bounds__(B) k_lifted(LiftArgs a)
             // every live value; values within qtol of a bin edge counted apart
             const unsigned long long gid = (unsigned long long)a.parent_graph[I.parent];
             for (int e = threadIdx.x; e < E; e += B) {
-                int2 p = uv[e];
+                int2 p = UV(e);
                 if (!(alive[p.x] && alive[p.y])) continue;
                 const unsigned long long key = quant_key((double)dc[e], gid, a.eps, a.qtol);
                 int64_t slot = quant_slot(key, a.qcap);
@@ -395,6 +441,12 @@ size_t lifted_lds_bytes(int ecap, int ncap)
     return (size_t)(2 * ecap + ncap) * sizeof(double) + (size_t)ncap * sizeof(int) + 16;
 }
 
+size_t lifted_lds_bytes_staged(int ecap, int ncap, int scap)
+{
+    return lifted_lds_bytes(ecap, ncap) + (size_t)(ncap + 2 + ecap + 2) * sizeof(int) +
+           (size_t)ecap * sizeof(ushort2) + (size_t)scap * sizeof(unsigned short) + 16;
+}
+
 // Largest dynamic LDS one k_lifted launch may take on the current device:
 // the per-workgroup maximum less the kernel's own static LDS (and a margin),
 // at most 150 KiB.  Launches above it are refused: the workgroups would
@@ -411,8 +463,14 @@ size_t lifted_lds_limit()
             maxblk = 64 * 1024;
         }
         size_t stat = 0;
-        const void *fns[4] = {reinterpret_cast<const void *>(&k_lifted<64>), reinterpret_cast<const void *>(&k_lifted<128>),
-                              reinterpret_cast<const void *>(&k_lifted<256>), reinterpret_cast<const void *>(&k_lifted<512>)};
+        const void *fns[8] = {reinterpret_cast<const void *>(&k_lifted<64, false>),
+                              reinterpret_cast<const void *>(&k_lifted<128, false>),
+                              reinterpret_cast<const void *>(&k_lifted<256, false>),
+                              reinterpret_cast<const void *>(&k_lifted<512, false>),
+                              reinterpret_cast<const void *>(&k_lifted<64, true>),
+                              reinterpret_cast<const void *>(&k_lifted<128, true>),
+                              reinterpret_cast<const void *>(&k_lifted<256, true>),
+                              reinterpret_cast<const void *>(&k_lifted<512, true>)};
         for (const void *f : fns) {
             hipFuncAttributes fa;
             if (hipFuncGetAttributes(&fa, f) == hipSuccess) stat = std::max(stat, (size_t)fa.sharedSizeBytes);
@@ -444,22 +502,33 @@ size_t lifted_lds_per_cu()
     return cache[d];
 }
 
-bool lifted_launch(const LiftArgs &a, int block, int grid, hipStream_t s)
+bool lifted_launch(const LiftArgs &a, int block, int grid, bool staged, hipStream_t s)
 {
-    size_t lds = lifted_lds_bytes(a.ecap, a.ncap);
+    size_t lds = staged ? lifted_lds_bytes_staged(a.ecap, a.ncap, a.scap) : lifted_lds_bytes(a.ecap, a.ncap);
     if (lds > lifted_lds_limit()) {
         set_error("batched launch needs " + std::to_string(lds) + " B of LDS, above the device limit of " +
                   std::to_string(lifted_lds_limit()) + " B");
         return false;
     }
-    if (block == 64)
-        hipLaunchKernelGGL(k_lifted<64>, dim3(grid), dim3(64), lds, s, a);
-    else if (block == 128)
-        hipLaunchKernelGGL(k_lifted<128>, dim3(grid), dim3(128), lds, s, a);
-    else if (block == 256)
-        hipLaunchKernelGGL(k_lifted<256>, dim3(grid), dim3(256), lds, s, a);
-    else
-        hipLaunchKernelGGL(k_lifted<512>, dim3(grid), dim3(512), lds, s, a);
+    if (staged) {
+        if (block == 64)
+            hipLaunchKernelGGL((k_lifted<64, true>), dim3(grid), dim3(64), lds, s, a);
+        else if (block == 128)
+            hipLaunchKernelGGL((k_lifted<128, true>), dim3(grid), dim3(128), lds, s, a);
+        else if (block == 256)
+            hipLaunchKernelGGL((k_lifted<256, true>), dim3(grid), dim3(256), lds, s, a);
+        else
+            hipLaunchKernelGGL((k_lifted<512, true>), dim3(grid), dim3(512), lds, s, a);
+    } else {
+        if (block == 64)
+            hipLaunchKernelGGL((k_lifted<64, false>), dim3(grid), dim3(64), lds, s, a);
+        else if (block == 128)
+            hipLaunchKernelGGL((k_lifted<128, false>), dim3(grid), dim3(128), lds, s, a);
+        else if (block == 256)
+            hipLaunchKernelGGL((k_lifted<256, false>), dim3(grid), dim3(256), lds, s, a);
+        else
+            hipLaunchKernelGGL((k_lifted<512, false>), dim3(grid), dim3(512), lds, s, a);
+    }
     DH_CHECK(hipGetLastError());
     return true;
 }

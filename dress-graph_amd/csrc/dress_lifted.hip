@@ -1335,6 +1335,17 @@ int forced_block()
     return 0;
 }
 
+// Whether the batched kernel copies each parent's structure into LDS
+// (DRESS_HIP_LIFT_STAGE=0 turns it off; results do not depend on it).
+int lift_stage_mode()
+{
+    static const int mode = [] {
+        const char *e = getenv("DRESS_HIP_LIFT_STAGE");
+        return (e && !strcmp(e, "0")) ? 0 : 1;
+    }();
+    return mode;
+}
+
 // static block size by the launch group's largest edge count, where the
 // autotuner does not calibrate (delta^1 of 6-regular graphs with E <= 384:
 // 2.80 ms of device time with 128 threads, 3.06 ms with 64, 3.11 ms with
@@ -1523,11 +1534,12 @@ bool launch_ids(RunState &R, const std::vector<const HostParent *> &ps, const st
     const size_t per_cu_bytes = lifted_lds_per_cu();
     size_t g0 = 0;
     while (g0 < ids.size()) {
-        int gE = 0, gN = 0;
+        int gE = 0, gN = 0, gS = 0;
         size_t g1 = g0;
         while (g1 < ids.size()) {
             const HostParent *P = ps[(size_t)item_parent[(size_t)ids[g1]]];
             int nE = std::max(gE, P->E), nN = std::max(gN, P->N);
+            gS = std::max(gS, P->S);
             if (lifted_lds_bytes((nE + 1) & ~1, (nN + 1) & ~1) > limit) {
                 if (g1 == g0) {
                     set_error("a graph of " + std::to_string(P->N) + " vertices and " + std::to_string(P->E) +
@@ -1542,7 +1554,24 @@ bool launch_ids(RunState &R, const std::vector<const HostParent *> &ps, const st
         }
         a.ecap = (gE + 1) & ~1;
         a.ncap = (gN + 1) & ~1;
-        const size_t lds = lifted_lds_bytes(a.ecap, a.ncap);
+        a.scap = (gS + 1) & ~1;
+        // The parent's structure goes into LDS when it fits and the launch
+        // still holds 16 waves per CU (or as many as without it) at the
+        // static block size: delta^1 of 6-regular graphs of <= 128 vertices
+        // (13 instead of 16 workgroups of 128 per CU) took 1.30 instead of
+        // 1.50 ms; delta^2 of the 12-regular graph on 256 vertices (3
+        // instead of 5 workgroups of 256) would take 7.4 instead of 7.1 ms.
+        const size_t lds_plain = lifted_lds_bytes(a.ecap, a.ncap);
+        const size_t lds_staged = lifted_lds_bytes_staged(a.ecap, a.ncap, a.scap);
+        auto waves_per_cu = [&](size_t lds_b, int block) {
+            const int per_cu_lds = (int)std::max<size_t>(1, per_cu_bytes / (lds_b + 1024));
+            const int per_cu_waves = std::max(1, 32 / (block / 64));
+            return std::max(1, std::min(per_cu_lds, per_cu_waves)) * (block / 64);
+        };
+        const int sb = choose_block(gE);
+        const bool staged = lift_stage_mode() != 0 && lds_staged <= limit &&
+                            waves_per_cu(lds_staged, sb) >= std::min(16, waves_per_cu(lds_plain, sb));
+        const size_t lds = staged ? lds_staged : lds_plain;
         // workgroups of `block` threads one CU holds at once
         auto per_cu_of = [&](int block) {
             int per_cu_lds = (int)std::max<size_t>(1, per_cu_bytes / (lds + 1024));
@@ -1565,7 +1594,7 @@ bool launch_ids(RunState &R, const std::vector<const HostParent *> &ps, const st
             R.ev.push_back(e0);
             R.ev.push_back(e1);
             if (!(hipMemsetAsync(R.d_queue, 0, sizeof(int), s) == hipSuccess && hipEventRecord(e0, s) == hipSuccess &&
-                  lifted_launch(a, block, grid, s) && hipEventRecord(e1, s) == hipSuccess))
+                  lifted_launch(a, block, grid, staged, s) && hipEventRecord(e1, s) == hipSuccess))
                 return false;
             if (ms_out) {
                 if (hipEventSynchronize(e1) != hipSuccess) return false;
