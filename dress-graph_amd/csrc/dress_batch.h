@@ -15,6 +15,9 @@ struct LiftParent {
     int directed;                  // FORWARD/BACKWARD self-term form
     int uni_w;                     // every variant weight is w_c (bit for bit)
     double w_c;
+    int64_t ell_ofs;               // column-major edge ids at ell[ell_ofs ...], or -1
+    int ell_stride;                // rows per column there (N rounded up to 64)
+    int pad;
 };
 
 // one subgraph: parent + deleted vertices (delta) or marked tuple (nabla)
@@ -39,6 +42,7 @@ struct LiftArgs {
     const double *marks;           // nabla marker weights [LIFT_KMAX]
     const int *row_off;            // per parent: N+1 local offsets
     const int *adj_e;              // per parent: S local edge ids
+    const int *ell;                // per parent (or none): the same, column by column
     const int2 *uv;                // per parent: E (U, V)
     const double *ew;              // per parent: E variant weights
     const int *ioff;               // per parent: E+1 local offsets

@@ -150,6 +150,15 @@ __global__ void __launch_bounds__(B) k_lifted(LiftArgs a)
             if constexpr (ST) return s_ioff[e];
             else return ioff[e];
         };
+        // entry j of row u (starting at rb): from LDS (ST), column by
+        // column (lanes on consecutive rows read consecutive words), or
+        // from the row itself
+        const int *__restrict__ ell = P.ell_ofs >= 0 ? a.ell + P.ell_ofs : nullptr;
+        const int ell_stride = P.ell_stride;
+        auto ROWE = [&](int u, int rb, int j) -> int {
+            if constexpr (ST) return (int)s_adj[rb + j];
+            else return ell ? ell[j * ell_stride + u] : adj_e[rb + j];
+        };
 
         if (threadIdx.x < k) {
             s_tup[threadIdx.x] = a.item_verts[I.vofs + threadIdx.x];
@@ -216,16 +225,16 @@ __global__ void __launch_bounds__(B) k_lifted(LiftArgs a)
                     // float products, summed exactly in FP64 (independent of
                     // order), then the self term
                     const float *dc = cur ? fB : fA;
-                    for (int i = rb; i < re; i++) {
-                        int e = ADJ(i);
+                    for (int j = 0; j < re - rb; j++) {
+                        int e = ROWE(u, rb, j);
                         s += (double)((float)W(e) * dc[e]);
                     }
                     vd[u] = sqrt(s + self);
                 } else {
                     const double *dc = cur ? dB : dA;
                     s = self;
-                    for (int i = rb; i < re; i++) {
-                        int e = ADJ(i);
+                    for (int j = 0; j < re - rb; j++) {
+                        int e = ROWE(u, rb, j);
                         two_sum_add(s, c, W(e) * dc[e]);
                     }
                     vd[u] = sqrt(s + c);

@@ -73,6 +73,11 @@ struct HostParent {
     int N = 0, E = 0, S = 0, T = 0;
     int directed = 0;
     std::vector<int> row_off, adj_x, adj_e, ioff;
+    // the rows' edge ids column by column (entry i of row u at
+    // ell[i * ell_stride + u], ell_stride = N rounded up to 64), when that
+    // padded copy is at most twice the rows (empty otherwise)
+    std::vector<int> ell;
+    int ell_stride = 0;
     std::vector<int2> uv;
     std::vector<double> ew;
     std::vector<uint32_t> pairs;
@@ -181,6 +186,22 @@ bool host_parent_build(int N, int E, const int *U, const int *V, const double *W
     P.adj_x.resize(P.S);
     P.adj_e.resize(P.S);
     for (int i = 0; i < P.S; i++) { P.adj_x[i] = rows[i].x; P.adj_e[i] = rows[i].e; }
+    {
+        // a wave's lanes walk 64 consecutive rows together: column by
+        // column their edge ids are consecutive words
+        int maxdeg = 0;
+        for (int u = 0; u < N; u++) maxdeg = std::max(maxdeg, P.row_off[u + 1] - P.row_off[u]);
+        const int stride = (N + 63) & ~63;
+        P.ell.clear();
+        P.ell_stride = 0;
+        if (maxdeg > 0 && (int64_t)stride * maxdeg <= 2 * (int64_t)P.S) {
+            P.ell.assign((size_t)stride * maxdeg, -1);
+            for (int u = 0; u < N; u++)
+                for (int i = P.row_off[u]; i < P.row_off[u + 1]; i++)
+                    P.ell[(size_t)(i - P.row_off[u]) * stride + u] = P.adj_e[i];
+            P.ell_stride = stride;
+        }
+    }
     // intersections (dress.c:307-363 pairing of repeated neighbours)
     P.ioff.assign(E + 1, 0);
     P.pairs.clear();
@@ -441,7 +462,7 @@ struct DevPools {
     char *blk = nullptr;
     LiftParent *parents = nullptr;
     int *parent_graph = nullptr;
-    int *row_off = nullptr, *adj_e = nullptr, *ioff = nullptr;
+    int *row_off = nullptr, *adj_e = nullptr, *ioff = nullptr, *ell = nullptr;
     int2 *uv = nullptr;
     double *ew = nullptr;
     uint32_t *pairs = nullptr;
@@ -468,11 +489,11 @@ void pack_parents(const std::vector<const HostParent *> &ps, const std::vector<i
                   const std::vector<double> *extra_ew = nullptr)
 {
     meta.clear();
-    std::vector<const std::vector<int> *> row_off, adj_e, ioff;
+    std::vector<const std::vector<int> *> row_off, adj_e, ioff, ell;
     std::vector<const std::vector<int2> *> uv;
     std::vector<const std::vector<double> *> ew;
     std::vector<const std::vector<uint32_t> *> pairs;
-    int64_t nr = 0, na = 0, ne = 0, ni = 0, np = 0;
+    int64_t nr = 0, na = 0, ne = 0, ni = 0, np = 0, nl = 0;
     for (const HostParent *P : ps) {
         LiftParent m;
         m.N = P->N; m.E = P->E; m.S = P->S; m.T = P->T;
@@ -482,6 +503,10 @@ void pack_parents(const std::vector<const HostParent *> &ps, const std::vector<i
         m.ioff_ofs = ni;
         m.pair_ofs = np;
         m.directed = P->directed;
+        m.ell_ofs = P->ell.empty() ? -1 : nl;
+        m.ell_stride = P->ell_stride;
+        nl += (int64_t)P->ell.size();
+        ell.push_back(&P->ell);
         // uniform variant weights (unweighted UNDIRECTED, FORWARD, BACKWARD):
         // the kernel takes the constant instead of reading a weight per term
         m.uni_w = 1;
@@ -503,6 +528,7 @@ void pack_parents(const std::vector<const HostParent *> &ps, const std::vector<i
     pk.add(&D.parent_graph, graph_of);
     pk.add_concat(&D.row_off, row_off);
     pk.add_concat(&D.adj_e, adj_e);
+    pk.add_concat(&D.ell, ell);
     pk.add_concat(&D.uv, uv);
     pk.add_concat(&D.ew, ew);
     pk.add_concat(&D.ioff, ioff);
@@ -1663,6 +1689,7 @@ bool run_launch(RunState &R, const std::vector<const HostParent *> &ps, const st
     a.marks = R.d_marks;
     a.row_off = R.D.row_off;
     a.adj_e = R.D.adj_e;
+    a.ell = R.D.ell;
     a.uv = R.D.uv;
     a.ew = R.D.ew;
     a.ioff = R.D.ioff;
