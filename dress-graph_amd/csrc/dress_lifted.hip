@@ -74,8 +74,9 @@ struct HostParent {
     int directed = 0;
     std::vector<int> row_off, adj_x, adj_e, ioff;
     // the rows' edge ids column by column (entry i of row u at
-    // ell[i * ell_stride + u], ell_stride = N rounded up to 64), when that
-    // padded copy is at most twice the rows (empty otherwise)
+    // ell[i * ell_stride + u], ell_stride = N rounded up to 64), for
+    // parents of more than 1,024 edges whose padded copy is at most twice
+    // the rows (empty otherwise)
     std::vector<int> ell;
     int ell_stride = 0;
     std::vector<int2> uv;
@@ -194,7 +195,10 @@ bool host_parent_build(int N, int E, const int *U, const int *V, const double *W
         const int stride = (N + 63) & ~63;
         P.ell.clear();
         P.ell_stride = 0;
-        if (maxdeg > 0 && (int64_t)stride * maxdeg <= 2 * (int64_t)P.S) {
+        // (parents of up to 1,024 edges are read from the LDS copy of their
+        // rows, which the batched kernel makes when the launch still holds
+        // 16 waves per CU: no column copy for them)
+        if (E > 1024 && maxdeg > 0 && (int64_t)stride * maxdeg <= 2 * (int64_t)P.S) {
             P.ell.assign((size_t)stride * maxdeg, -1);
             for (int u = 0; u < N; u++)
                 for (int i = P.row_off[u]; i < P.row_off[u + 1]; i++)

@@ -361,3 +361,22 @@ def test_grouped_weighted_variants_equal_oracle(gpu, ora, gen, monkeypatch, vari
         ov, oc, _, _ = ora.lifted("delta", n, U, V, W, variant=variant, k=1)
         assert np.array_equal(v0, ov) and np.array_equal(c0, oc), (variant, n)
         assert np.array_equal(v1, ov) and np.array_equal(c1, oc), (variant, n)
+
+
+@pytest.mark.parametrize("variant", [0, 1, 2, 3])
+@pytest.mark.parametrize("weighted", [False, True])
+def test_parents_read_without_lds_copy_equal_oracle(gpu, ora, variant, weighted):
+    """A parent of 1,536 edges (the 12-regular graph on 256 vertices) is
+    run without the LDS copy of its structure (the copy would leave fewer
+    than 16 waves per CU) and, its rows being near-regular, with the
+    column-by-column copy of its rows; unweighted parents take the variant
+    weight as a constant, weighted ones read it per term.  delta^1 and
+    nabla^1 with multisets, bit for bit against the oracle."""
+    z = np.load(os.path.join(GOLDEN, "regular12_n256.npz"))
+    U, V = z["U"], z["V"]
+    W = np.random.default_rng(40 + variant).uniform(0.1, 1.0, U.size) if weighted else None
+    for kind in ("delta", "nabla"):
+        a = _lifted(gpu, kind, 256, U, V, W, variant, k=1, keep_multisets=True)
+        b = ora.lifted(kind, 256, U, V, W, variant=variant, k=1, keep_multisets=True)
+        assert np.array_equal(a[0], b[0]) and np.array_equal(a[1], b[1]), (kind, variant, weighted)
+        assert np.array_equal(a[2], b[2], equal_nan=True), (kind, variant, weighted)
