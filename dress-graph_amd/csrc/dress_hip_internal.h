@@ -253,8 +253,10 @@ bool prim_sort_pairs_u64_i32(const uint64_t *kin, uint64_t *kout, const int *vin
 bool prim_sort_u64(const uint64_t *kin, uint64_t *kout, size_t n, hipStream_t s, int end_bit = 64);
 // runs of equal keys of a sorted array: the distinct keys, their counts, and
 // the number of runs (device int64)
+// (with keep, the scratch is handed to the caller, to be freed once the
+// encoding has run, instead of being freed at once)
 bool prim_run_length_encode_u64(const uint64_t *in, size_t n, uint64_t *unique, int64_t *counts, int64_t *runs,
-                                hipStream_t s);
+                                hipStream_t s, void **keep = nullptr);
 bool prim_sort_pairs_u64_u64(const uint64_t *kin, uint64_t *kout, const uint64_t *vin, uint64_t *vout,
                              size_t n, hipStream_t s);
 bool prim_sort_pairs_i32_u64(const int *kin, int *kout, const uint64_t *vin, uint64_t *vout, size_t n,

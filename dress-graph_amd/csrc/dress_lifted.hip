@@ -651,9 +651,25 @@ __device__ __forceinline__ int segment_of(const int64_t *seg_ofs, int nseg, int6
 // bit vbits, the value's key minus the smallest below it.  Up to
 // PACK_LDS_SEGS segment offsets are searched in LDS.
 constexpr int PACK_LDS_SEGS = 4096;
-__global__ void __launch_bounds__(TPB) k_pack_keys(const uint64_t *key, const int64_t *seg_ofs, int nseg, int64_t n,
-                                                   const unsigned long long *range, int vbits, uint64_t *out)
+// vbits < 0: the bits of the key span, read from range on the device; when
+// they and the sbits of the segment ids exceed 64, *bad is set and nothing
+// is written (the caller takes another layout)
+__device__ __forceinline__ int span_bits(const unsigned long long *range)
 {
+    const unsigned long long span = range[1] - range[0];
+    return span ? 64 - __clzll((long long)span) : 0;
+}
+__global__ void __launch_bounds__(TPB) k_pack_keys(const uint64_t *key, const int64_t *seg_ofs, int nseg, int64_t n,
+                                                   const unsigned long long *range, int vbits, uint64_t *out,
+                                                   int sbits, int *bad)
+{
+    if (vbits < 0) {
+        vbits = span_bits(range);
+        if (vbits + sbits > 64) {
+            if (blockIdx.x == 0 && threadIdx.x == 0) *bad = 1;
+            return;
+        }
+    }
     extern __shared__ int64_t s_ofs[];
     const bool lds = nseg < PACK_LDS_SEGS;
     if (lds) {
@@ -675,8 +691,16 @@ __global__ void __launch_bounds__(TPB) k_pack_keys(const uint64_t *key, const in
 }
 
 // the runs' packed keys back to values, in place, with their segments
-__global__ void k_unpack_runs(uint64_t *ukey, int64_t runs, const unsigned long long *range, int vbits, int *useg)
+// (runs_dev: the run count and, vbits < 0, the layout from the device;
+// nothing to do after a layout failure)
+__global__ void k_unpack_runs(uint64_t *ukey, int64_t runs, const unsigned long long *range, int vbits, int *useg,
+                              const int64_t *runs_dev, const int *bad)
 {
+    if (runs_dev) {
+        if (*bad) return;
+        runs = *runs_dev;
+        if (vbits < 0) vbits = span_bits(range);
+    }
     const uint64_t lo = range[0];
     const uint64_t mask = vbits < 64 ? (((uint64_t)1 << vbits) - 1) : ~(uint64_t)0;
     for (int64_t j = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; j < runs; j += (int64_t)gridDim.x * blockDim.x) {
@@ -719,8 +743,10 @@ __global__ void __launch_bounds__(MERGE_TPB) k_merge_tiles(const uint64_t *in, u
     if (threadIdx.x < 2)
         s_ia[threadIdx.x] = merge_split(in + T.a0, la, in + T.b0, lb, threadIdx.x == 0 ? T.o0 : T.o1);
     __syncthreads();
+    // (clamped: on input that is not sorted runs the splits need not be
+    // monotone, and the tile must still read and write in bounds)
     const int64_t ia0 = s_ia[0], ia1 = s_ia[1];
-    const int na = (int)(ia1 - ia0), nb = (T.o1 - T.o0) - na;
+    const int na = (int)min(max(ia1 - ia0, (int64_t)0), (int64_t)(T.o1 - T.o0)), nb = (T.o1 - T.o0) - na;
     const int64_t ib0 = T.o0 - ia0;
     for (int i = threadIdx.x; i < na; i += MERGE_TPB) s[i] = in[T.a0 + ia0 + i];
     for (int i = threadIdx.x; i < nb; i += MERGE_TPB) s[na + i] = in[T.b0 + ib0 + i];
@@ -804,8 +830,13 @@ __global__ void k_run_counts(const int64_t *ustart, int64_t runs, int64_t n, int
 
 // first run of every segment: ofs[g] for g in [0, nseg], runs ordered by
 // segment (segments without runs get the next run's index)
-__global__ void k_seg_run_starts(const int *useg, int64_t runs, int nseg, int64_t *ofs)
+__global__ void k_seg_run_starts(const int *useg, int64_t runs, int nseg, int64_t *ofs, const int64_t *runs_dev,
+                                 const int *bad)
 {
+    if (runs_dev) {
+        if (*bad) return;
+        runs = *runs_dev;
+    }
     for (int64_t j = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; j < runs; j += (int64_t)gridDim.x * blockDim.x) {
         int prev = j ? useg[j - 1] : -1;
         for (int g = prev + 1; g <= useg[j]; g++) ofs[g] = j;
@@ -876,7 +907,8 @@ bool exact_histogram_dev(const double *vals, int64_t n, const std::vector<int64_
     pt.mark("key range");
     if (ok && vbits >= 0) {
         const size_t shm = nseg < PACK_LDS_SEGS ? ((size_t)nseg + 1) * sizeof(int64_t) : 0;
-        hipLaunchKernelGGL(k_pack_keys, dim3(grid), dim3(TPB), shm, s, k0, dofs, nseg, n, range, vbits, k1);
+        hipLaunchKernelGGL(k_pack_keys, dim3(grid), dim3(TPB), shm, s, k0, dofs, nseg, n, range, vbits, k1, 0,
+                           (int *)nullptr);
         if (pt.on) { hipStreamSynchronize(s); pt.mark("pack"); }
         // values that arrive as sorted runs (each segment's runs merged in
         // ceil(log2 runs) rounds) or in any order (one radix sort)
@@ -917,8 +949,10 @@ bool exact_histogram_dev(const double *vals, int64_t n, const std::vector<int64_
             runs = (int64_t)host[0];
             const int ugrid = grid_for(runs, num_cus() * 8);
             usg = flag;                                          // (flags unused on this path)
-            hipLaunchKernelGGL(k_unpack_runs, dim3(ugrid), dim3(TPB), 0, s, other, runs, range, vbits, usg);
-            hipLaunchKernelGGL(k_seg_run_starts, dim3(ugrid), dim3(TPB), 0, s, usg, runs, nseg, rofs);
+            hipLaunchKernelGGL(k_unpack_runs, dim3(ugrid), dim3(TPB), 0, s, other, runs, range, vbits, usg,
+                               (const int64_t *)nullptr, (const int *)nullptr);
+            hipLaunchKernelGGL(k_seg_run_starts, dim3(ugrid), dim3(TPB), 0, s, usg, runs, nseg, rofs,
+                               (const int64_t *)nullptr, (const int *)nullptr);
             uval = reinterpret_cast<double *>(other);
             if (other == k0) k0 = nullptr;
             else k1 = nullptr;
@@ -959,7 +993,8 @@ bool exact_histogram_dev(const double *vals, int64_t n, const std::vector<int64_
             const int ugrid = grid_for(runs, num_cus() * 8);
             hipLaunchKernelGGL(k_run_emit, dim3(grid), dim3(TPB), 0, s, k0, g0, flag, pos, n, uval, ustart, usg);
             hipLaunchKernelGGL(k_run_counts, dim3(ugrid), dim3(TPB), 0, s, ustart, runs, n, cnt);
-            hipLaunchKernelGGL(k_seg_run_starts, dim3(ugrid), dim3(TPB), 0, s, usg, runs, nseg, rofs);
+            hipLaunchKernelGGL(k_seg_run_starts, dim3(ugrid), dim3(TPB), 0, s, usg, runs, nseg, rofs,
+                               (const int64_t *)nullptr, (const int *)nullptr);
         }
     }
     if (ok)
@@ -984,6 +1019,135 @@ bool exact_histogram_dev(const double *vals, int64_t n, const std::vector<int64_
     *d_cnt = cnt;
     *runs_out = runs;
     return true;
+}
+
+// The exact histogram of one group of a pipelined batch, queued on s
+// behind the group's launches without the host waiting on the device: the
+// values come as sorted runs per item (the rounds of merges are planned
+// here, from sizes the host knows), the key layout is read from the key
+// range on the device (k_pack_keys), and the run count, a layout failure
+// and each segment's first run come back into a pinned block.  Every block
+// the queued work reads is held in H until the caller has waited for that
+// work (hist_collect, then hist_release).
+struct AsyncHist {
+    int64_t n = 0;
+    int nseg = 0;
+    uint64_t *k0 = nullptr, *k1 = nullptr;
+    uint64_t *uniq = nullptr;             // k0 or k1: unique keys, then values
+    int64_t *cnt = nullptr, *dofs = nullptr, *rofs = nullptr, *druns = nullptr;
+    int *flag = nullptr;                  // [0, runs): segment of each run; [n]: layout failure
+    unsigned long long *part = nullptr, *range = nullptr;
+    MergeTile *dtiles = nullptr;
+    void *scratch = nullptr;              // run-length encoding scratch
+    // pinned: [0] runs, [1] failure, [2, nseg + 3) each segment's first
+    // run, [nseg + 3, 2 nseg + 4) the segment offsets, then the tiles
+    int64_t *host = nullptr;
+};
+
+void hist_release(AsyncHist &H, bool keep_outputs)
+{
+    if (H.k0 != H.uniq) dfree(H.k0);
+    if (H.k1 != H.uniq) dfree(H.k1);
+    if (!keep_outputs) {
+        dfree(H.uniq);
+        dfree(H.cnt);
+    }
+    dfree(H.dofs); dfree(H.rofs); dfree(H.druns); dfree(H.flag); dfree(H.part); dfree(H.range);
+    dfree(H.dtiles); dfree(H.scratch);
+    pinned_free(H.host);
+    H = AsyncHist();
+}
+
+// 1 queued, 0 this form does not apply and nothing is queued (more than 8
+// rounds of merges, more than 2^32 values, or packed keys switched off),
+// -1 an error
+int hist_enqueue(AsyncHist &H, const double *vals, int64_t n, const std::vector<int64_t> &seg_ofs,
+                 const std::vector<int64_t> &item_end, hipStream_t s)
+{
+    H = AsyncHist();
+    const int nseg = (int)seg_ofs.size() - 1;
+    const char *pe = getenv("DRESS_HIP_HIST_PACKED");
+    if (n <= 0 || n > (int64_t)0xffffffffll || nseg < 1 || (pe && !strcmp(pe, "0"))) return 0;
+    std::vector<size_t> round_ofs;
+    merge_rounds(item_end, seg_ofs, [](const MergeTile &) {}, round_ofs);
+    if (round_ofs.size() > 9) return 0;
+    int sbits = 0;
+    while ((1 << sbits) < nseg) sbits++;
+    H.n = n;
+    H.nseg = nseg;
+    const size_t nt = round_ofs.back(), words = 2 * ((size_t)nseg + 1) + 2;
+    // at least the pool's smallest pinned size, so that the copies are
+    // asynchronous
+    H.host = (int64_t *)pinned_alloc(std::max<size_t>(words * sizeof(int64_t) + nt * sizeof(MergeTile), 1 << 20));
+    bool ok = H.host != nullptr;
+    if (!ok) set_error("host staging of the histogram failed");
+    int64_t *h_rofs = ok ? H.host + 2 : nullptr, *h_seg = ok ? H.host + 2 + (nseg + 1) : nullptr;
+    MergeTile *h_tiles = ok ? reinterpret_cast<MergeTile *>(H.host + words) : nullptr;
+    if (ok) {
+        H.host[0] = 0;
+        H.host[1] = 0;
+        for (int g = 0; g <= nseg; g++) h_seg[g] = seg_ofs[(size_t)g];
+        size_t at = 0;
+        merge_rounds(item_end, seg_ofs, [&](const MergeTile &t) { h_tiles[at++] = t; }, round_ofs);
+    }
+    ok = ok && dalloc(&H.k0, (size_t)n, "histogram keys") && dalloc(&H.k1, (size_t)n, "histogram keys") &&
+         dalloc(&H.flag, (size_t)n + 1, "run segments") && dalloc(&H.dofs, (size_t)nseg + 1, "segment offsets") &&
+         dalloc(&H.rofs, (size_t)nseg + 1, "run offsets") && dalloc(&H.part, 2 * RANGE_BLOCKS, "key range") &&
+         dalloc(&H.range, 2, "key range") && dalloc(&H.druns, 1, "run count") &&
+         dalloc(&H.cnt, (size_t)n + 1, "histogram counts") && (nt == 0 || dalloc(&H.dtiles, nt, "merge tiles"));
+    const int grid = grid_for(n, num_cus() * 8);
+    const int rgrid = grid_for(n, RANGE_BLOCKS);
+    if (ok) {
+        hipLaunchKernelGGL(k_zero_int, dim3(1), dim3(64), 0, s, H.flag + n, 1);
+        ok = hipMemcpyAsync(H.dofs, h_seg, ((size_t)nseg + 1) * sizeof(int64_t), hipMemcpyHostToDevice, s) ==
+                 hipSuccess &&
+             (nt == 0 ||
+              hipMemcpyAsync(H.dtiles, h_tiles, nt * sizeof(MergeTile), hipMemcpyHostToDevice, s) == hipSuccess);
+    }
+    uint64_t *sorted = H.k1, *other = H.k0;
+    if (ok) {
+        hipLaunchKernelGGL(k_sort_keys, dim3(rgrid), dim3(TPB), 0, s, vals, n, H.k0, H.part);
+        hipLaunchKernelGGL(k_key_range, dim3(1), dim3(TPB), 0, s, H.part, rgrid, H.range);
+        const size_t shm = nseg < PACK_LDS_SEGS ? ((size_t)nseg + 1) * sizeof(int64_t) : 0;
+        hipLaunchKernelGGL(k_pack_keys, dim3(grid), dim3(TPB), shm, s, H.k0, H.dofs, nseg, n, H.range, -1, H.k1,
+                           sbits, H.flag + n);
+        for (size_t r = 0; r + 1 < round_ofs.size(); r++) {
+            hipLaunchKernelGGL(k_merge_tiles, dim3((unsigned)(round_ofs[r + 1] - round_ofs[r])), dim3(MERGE_TPB), 0,
+                               s, sorted, other, H.dtiles + round_ofs[r]);
+            std::swap(sorted, other);
+        }
+        ok = prim_run_length_encode_u64(sorted, (size_t)n, other, H.cnt, H.druns, s, &H.scratch);
+    }
+    if (ok) {
+        H.uniq = other;
+        hipLaunchKernelGGL(k_unpack_runs, dim3(grid), dim3(TPB), 0, s, other, (int64_t)0, H.range, -1, H.flag,
+                           (const int64_t *)H.druns, (const int *)(H.flag + n));
+        hipLaunchKernelGGL(k_seg_run_starts, dim3(grid), dim3(TPB), 0, s, H.flag, (int64_t)0, nseg, H.rofs,
+                           (const int64_t *)H.druns, (const int *)(H.flag + n));
+        ok = hipGetLastError() == hipSuccess &&
+             hipMemcpyAsync(&H.host[0], H.druns, sizeof(int64_t), hipMemcpyDeviceToHost, s) == hipSuccess &&
+             hipMemcpyAsync(&H.host[1], H.flag + n, sizeof(int), hipMemcpyDeviceToHost, s) == hipSuccess &&
+             hipMemcpyAsync(h_rofs, H.rofs, ((size_t)nseg + 1) * sizeof(int64_t), hipMemcpyDeviceToHost, s) ==
+                 hipSuccess;
+    }
+    if (!ok) {
+        if (!has_error()) set_error("histogram assembly failed");
+        (void)hipStreamSynchronize(s);              // queued kernels may read the blocks
+        hist_release(H, false);
+        return -1;
+    }
+    return 1;
+}
+
+// after the queued work has finished: the run count, each segment's first
+// run, and whether the key layout failed (then neither is valid)
+void hist_collect(const AsyncHist &H, int64_t *runs, std::vector<int64_t> &run_ofs, bool *failed)
+{
+    int fail = 0;
+    memcpy(&fail, &H.host[1], sizeof(int));
+    *failed = fail != 0;
+    *runs = H.host[0];
+    run_ofs.assign(H.host + 2, H.host + 2 + H.nseg + 1);
 }
 
 // the same, downloaded into vectors (runs of all segments; useg unused here)
@@ -2794,6 +2958,28 @@ int dress_hip_lifted_batch(int kind, int k, int n_graphs, const int *N, const in
             return run_launch(P.R, P.ps, P.item_parent, J, nullptr, 0, s, P.lo, qk, qc, quant ? qcap : 0,
                               quant ? qtol : 0.0, nullptr, merges(P));
         };
+        // each graph's values are one segment of its group's histogram:
+        // the items of a graph are contiguous
+        auto group_segments = [&](int gr, const GroupPlan &P) {
+            const int ga = gb[(size_t)gr], gz = gb[(size_t)gr + 1];
+            std::vector<int64_t> seg_ofs((size_t)(gz - ga) + 1, 0);
+            size_t it = 0;
+            for (int gi = ga; gi < gz; gi++) {
+                while (it < P.item_parent.size() && P.item_parent[it] == gi - ga) it++;
+                seg_ofs[(size_t)(gi - ga) + 1] = P.lo.val_ofs[it];
+            }
+            return seg_ofs;
+        };
+        // With merged runs, each group's histogram is queued on s right
+        // behind its own launches and needs no wait on the host, so that a
+        // group's downloads run beside the next group's launches; the host
+        // waits for the histogram's event before it queues the downloads.
+        // DRESS_HIP_HIST_ASYNC=0 runs the histograms on hs after each group
+        // instead.
+        const char *ha = getenv("DRESS_HIP_HIST_ASYNC");
+        const bool async_hist = !(ha && !strcmp(ha, "0"));
+        std::vector<AsyncHist> ah((size_t)G);
+        std::vector<hipEvent_t> hev((size_t)G, nullptr);
         // two groups stay queued on s ahead of the one being finished, so
         // the device goes from one group's launches to the next while the
         // host waits for a histogram
@@ -2801,8 +2987,23 @@ int dress_hip_lifted_batch(int kind, int k, int n_graphs, const int *N, const in
         int queued = 0;                                  // groups [0, queued) are on s
         auto queue_group = [&](int g) {
             if (!chain || g >= G) return;
-            if (launch_group(g)) queued = g + 1;
-            else chain = false;
+            if (!launch_group(g)) {
+                chain = false;
+                return;
+            }
+            queued = g + 1;
+            GroupPlan &P = *plans[(size_t)g];
+            if (!(async_hist && G > 1 && !J.mixed && !quant && merges(P) && gb[(size_t)g + 1] > gb[(size_t)g]))
+                return;
+            const std::vector<int64_t> item_end(P.lo.val_ofs.begin() + 1, P.lo.val_ofs.end());
+            const int r = hist_enqueue(ah[(size_t)g], P.lo.d_vals, P.lo.val_ofs.back(), group_segments(g, P),
+                                       item_end, s);
+            if (r < 0) chain = false;
+            else if (r > 0 && (hipEventCreateWithFlags(&hev[(size_t)g], hipEventDisableTiming) != hipSuccess ||
+                               hipEventRecord(hev[(size_t)g], s) != hipSuccess)) {
+                set_error("histogram event failed");
+                chain = false;
+            }
         };
         queue_group(0);
         queue_group(1);
@@ -2814,23 +3015,32 @@ int dress_hip_lifted_batch(int kind, int k, int n_graphs, const int *N, const in
             if (ok) queue_group(gr + 2);
             if (!chain) ok = false;                      // a later group failed: stop after this one
             LiftOut &lo = P.lo;
-            // exact histogram per graph: the items of a graph are contiguous
-            std::vector<int64_t> seg_ofs((size_t)(gz - ga) + 1, 0);
-            if (ok) {
-                size_t it = 0;
-                for (int gi = ga; gi < gz; gi++) {
-                    while (it < P.item_parent.size() && P.item_parent[it] == gi - ga) it++;
-                    seg_ofs[(size_t)(gi - ga) + 1] = lo.val_ofs[it];
-                }
-            }
             double *hv = nullptr;
             int64_t *hc = nullptr, runs = 0;
             std::vector<int64_t> run_ofs((size_t)(gz - ga) + 1, 0);
-            std::vector<int64_t> item_end;
-            if (ok && merges(P)) item_end.assign(lo.val_ofs.begin() + 1, lo.val_ofs.end());
-            if (ok && gz > ga)
-                ok = exact_histogram_dev(lo.d_vals, lo.val_ofs.back(), seg_ofs, hs, &hv, &hc, &runs, run_ofs,
-                                         merges(P) ? &item_end : nullptr);
+            bool have = false;                           // the histogram is on the device
+            if (ok && hev[(size_t)gr]) {
+                AsyncHist &H = ah[(size_t)gr];
+                ok = hipEventSynchronize(hev[(size_t)gr]) == hipSuccess;
+                if (!ok) set_error("batch histogram failed");
+                bool failed = true;
+                if (ok) hist_collect(H, &runs, run_ofs, &failed);
+                have = ok && !failed;
+                if (have) {
+                    hv = reinterpret_cast<double *>(H.uniq);
+                    hc = H.cnt;
+                }
+                hist_release(H, have);
+                (void)hipEventDestroy(hev[(size_t)gr]);
+                hev[(size_t)gr] = nullptr;
+            }
+            // else (and when the packed keys did not fit) on hs, waited for
+            if (ok && gz > ga && !have) {
+                std::vector<int64_t> item_end;
+                if (merges(P)) item_end.assign(lo.val_ofs.begin() + 1, lo.val_ofs.end());
+                ok = exact_histogram_dev(lo.d_vals, lo.val_ofs.back(), group_segments(gr, P), hs, &hv, &hc, &runs,
+                                         run_ofs, merges(P) ? &item_end : nullptr);
+            }
             if (gr == 0) pt.mark("first group on the device");
             if (ok && G == 1) {
                 out->hist_value = (double *)pinned_alloc(((size_t)runs + 1) * sizeof(double));
@@ -2870,6 +3080,11 @@ int dress_hip_lifted_batch(int kind, int k, int n_graphs, const int *N, const in
             if (hs != s) (void)hipStreamSynchronize(hs);
             if (cs) (void)hipStreamSynchronize(cs);
         }
+        for (int g = 0; g < G; g++)                      // histograms of groups not reached
+            if (hev[(size_t)g]) {
+                hist_release(ah[(size_t)g], false);
+                (void)hipEventDestroy(hev[(size_t)g]);
+            }
         stop_planning = true;
         if (planner.joinable()) planner.join();
         for (auto &P : plans)

@@ -66,8 +66,9 @@ bool prim_sort_u64(const uint64_t *kin, uint64_t *kout, size_t n, hipStream_t s,
 }
 
 bool prim_run_length_encode_u64(const uint64_t *in, size_t n, uint64_t *unique, int64_t *counts, int64_t *runs,
-                                hipStream_t s)
+                                hipStream_t s, void **keep)
 {
+    if (keep) *keep = nullptr;
     if (n == 0) return true;
     if (n > 0xffffffffull) {
         set_error("run-length encoding of more than 2^32 keys");
@@ -78,7 +79,8 @@ bool prim_run_length_encode_u64(const uint64_t *in, size_t n, uint64_t *unique, 
     void *buf = nullptr;
     if (!dmalloc(&buf, tmp + 16, "run-length scratch")) return false;
     hipError_t e = rocprim::run_length_encode(buf, tmp, in, (unsigned)n, unique, counts, runs, s);
-    dfree(buf);
+    if (keep) *keep = buf;
+    else dfree(buf);
     DH_CHECK(e);
     return true;
 }
