@@ -547,11 +547,13 @@ __global__ void k_zero_int(int *p, int n)
 }
 
 // ascending sort keys of doubles (the bit pattern, sign-folded), and each
-// block's smallest and largest key into part[2b], part[2b + 1]
+// block's smallest and largest key into part[2b], part[2b + 1]; *zero (when
+// given) is cleared
 constexpr int RANGE_BLOCKS = 1024;
 __global__ void __launch_bounds__(TPB) k_sort_keys(const double *v, int64_t n, uint64_t *key,
-                                                   unsigned long long *part)
+                                                   unsigned long long *part, int64_t *zero)
 {
+    if (zero && blockIdx.x == 0 && threadIdx.x == 0) *zero = 0;
     unsigned long long lo = ~0ull, hi = 0ull;
     for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) {
         const uint64_t b = (uint64_t)__double_as_longlong(v[i]);
@@ -691,16 +693,8 @@ __global__ void __launch_bounds__(TPB) k_pack_keys(const uint64_t *key, const in
 }
 
 // the runs' packed keys back to values, in place, with their segments
-// (runs_dev: the run count and, vbits < 0, the layout from the device;
-// nothing to do after a layout failure)
-__global__ void k_unpack_runs(uint64_t *ukey, int64_t runs, const unsigned long long *range, int vbits, int *useg,
-                              const int64_t *runs_dev, const int *bad)
+__global__ void k_unpack_runs(uint64_t *ukey, int64_t runs, const unsigned long long *range, int vbits, int *useg)
 {
-    if (runs_dev) {
-        if (*bad) return;
-        runs = *runs_dev;
-        if (vbits < 0) vbits = span_bits(range);
-    }
     const uint64_t lo = range[0];
     const uint64_t mask = vbits < 64 ? (((uint64_t)1 << vbits) - 1) : ~(uint64_t)0;
     for (int64_t j = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; j < runs; j += (int64_t)gridDim.x * blockDim.x) {
@@ -708,6 +702,29 @@ __global__ void k_unpack_runs(uint64_t *ukey, int64_t runs, const unsigned long 
         const uint64_t k = (key & mask) + lo;
         ukey[j] = (k >> 63) ? (k & 0x7fffffffffffffffull) : ~k;
         useg[j] = vbits < 64 ? (int)(key >> vbits) : 0;
+    }
+}
+
+// The runs' packed keys (key layout from the key range, run count from
+// the device) to values, into val, and each segment's first run into ofs
+// (nseg + 1 entries, as k_seg_run_starts); nothing after a layout failure
+__global__ void k_runs_out(const uint64_t *ukey, const unsigned long long *range, const int64_t *runs_dev,
+                           const int *bad, int nseg, double *val, int64_t *ofs)
+{
+    if (*bad) return;
+    const int64_t runs = *runs_dev;
+    const int vbits = span_bits(range);
+    const uint64_t lo = range[0];
+    const uint64_t mask = vbits < 64 ? (((uint64_t)1 << vbits) - 1) : ~(uint64_t)0;
+    for (int64_t j = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; j < runs; j += (int64_t)gridDim.x * blockDim.x) {
+        const uint64_t key = ukey[j];
+        const uint64_t k = (key & mask) + lo;
+        val[j] = __longlong_as_double((long long)((k >> 63) ? (k & 0x7fffffffffffffffull) : ~k));
+        const int g = vbits < 64 ? (int)(key >> vbits) : 0;
+        const int prev = j ? (vbits < 64 ? (int)(ukey[j - 1] >> vbits) : 0) : -1;
+        for (int x = prev + 1; x <= g; x++) ofs[x] = j;
+        if (j == runs - 1)
+            for (int x = g + 1; x <= nseg; x++) ofs[x] = runs;
     }
 }
 
@@ -830,13 +847,8 @@ __global__ void k_run_counts(const int64_t *ustart, int64_t runs, int64_t n, int
 
 // first run of every segment: ofs[g] for g in [0, nseg], runs ordered by
 // segment (segments without runs get the next run's index)
-__global__ void k_seg_run_starts(const int *useg, int64_t runs, int nseg, int64_t *ofs, const int64_t *runs_dev,
-                                 const int *bad)
+__global__ void k_seg_run_starts(const int *useg, int64_t runs, int nseg, int64_t *ofs)
 {
-    if (runs_dev) {
-        if (*bad) return;
-        runs = *runs_dev;
-    }
     for (int64_t j = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; j < runs; j += (int64_t)gridDim.x * blockDim.x) {
         int prev = j ? useg[j - 1] : -1;
         for (int g = prev + 1; g <= useg[j]; g++) ofs[g] = j;
@@ -893,7 +905,7 @@ bool exact_histogram_dev(const double *vals, int64_t n, const std::vector<int64_
     int vbits = -1, sbits = 0;
     while ((1 << sbits) < nseg) sbits++;
     if (ok) {
-        hipLaunchKernelGGL(k_sort_keys, dim3(rgrid), dim3(TPB), 0, s, vals, n, k0, part);
+        hipLaunchKernelGGL(k_sort_keys, dim3(rgrid), dim3(TPB), 0, s, vals, n, k0, part, (int64_t *)nullptr);
         hipLaunchKernelGGL(k_key_range, dim3(1), dim3(TPB), 0, s, part, rgrid, range);
         ok = hipMemcpyAsync(dofs, seg_ofs.data(), seg_ofs.size() * sizeof(int64_t), hipMemcpyHostToDevice, s) ==
                  hipSuccess &&
@@ -949,10 +961,8 @@ bool exact_histogram_dev(const double *vals, int64_t n, const std::vector<int64_
             runs = (int64_t)host[0];
             const int ugrid = grid_for(runs, num_cus() * 8);
             usg = flag;                                          // (flags unused on this path)
-            hipLaunchKernelGGL(k_unpack_runs, dim3(ugrid), dim3(TPB), 0, s, other, runs, range, vbits, usg,
-                               (const int64_t *)nullptr, (const int *)nullptr);
-            hipLaunchKernelGGL(k_seg_run_starts, dim3(ugrid), dim3(TPB), 0, s, usg, runs, nseg, rofs,
-                               (const int64_t *)nullptr, (const int *)nullptr);
+            hipLaunchKernelGGL(k_unpack_runs, dim3(ugrid), dim3(TPB), 0, s, other, runs, range, vbits, usg);
+            hipLaunchKernelGGL(k_seg_run_starts, dim3(ugrid), dim3(TPB), 0, s, usg, runs, nseg, rofs);
             uval = reinterpret_cast<double *>(other);
             if (other == k0) k0 = nullptr;
             else k1 = nullptr;
@@ -993,8 +1003,7 @@ bool exact_histogram_dev(const double *vals, int64_t n, const std::vector<int64_
             const int ugrid = grid_for(runs, num_cus() * 8);
             hipLaunchKernelGGL(k_run_emit, dim3(grid), dim3(TPB), 0, s, k0, g0, flag, pos, n, uval, ustart, usg);
             hipLaunchKernelGGL(k_run_counts, dim3(ugrid), dim3(TPB), 0, s, ustart, runs, n, cnt);
-            hipLaunchKernelGGL(k_seg_run_starts, dim3(ugrid), dim3(TPB), 0, s, usg, runs, nseg, rofs,
-                               (const int64_t *)nullptr, (const int *)nullptr);
+            hipLaunchKernelGGL(k_seg_run_starts, dim3(ugrid), dim3(TPB), 0, s, usg, runs, nseg, rofs);
         }
     }
     if (ok)
@@ -1021,129 +1030,149 @@ bool exact_histogram_dev(const double *vals, int64_t n, const std::vector<int64_
     return true;
 }
 
-// The exact histogram of one group of a pipelined batch, queued on s
-// behind the group's launches without the host waiting on the device: the
-// values come as sorted runs per item (the rounds of merges are planned
-// here, from sizes the host knows), the key layout is read from the key
-// range on the device (k_pack_keys), and the run count, a layout failure
-// and each segment's first run come back into a pinned block.  Every block
-// the queued work reads is held in H until the caller has waited for that
-// work (hist_collect, then hist_release).
+// The exact histogram of one group of a pipelined batch, queued behind the
+// group's launches without the host waiting on the device.  The values come
+// as sorted runs per item, so the rounds of merges are planned on the host
+// from sizes known before the launch (hist_plan, on a planner thread); the
+// key layout is read from the key range on the device (k_pack_keys), and
+// the run count, a layout failure and each segment's first run come back
+// into the plan's pinned block.
+struct HistPlan {
+    int nseg = 0;
+    std::vector<size_t> round_ofs;        // tiles before each round of merges
+    // pinned: [0] runs, [1] failure, [2, nseg + 3) each segment's first
+    // run, [nseg + 3, 2 nseg + 4) the segment offsets, then the tiles
+    int64_t *host = nullptr;
+    HistPlan() = default;
+    HistPlan(const HistPlan &) = delete;
+    HistPlan &operator=(const HistPlan &) = delete;
+    ~HistPlan() { pinned_free(host); }
+    size_t words() const { return 2 * ((size_t)nseg + 1) + 2; }
+    MergeTile *tiles() const { return reinterpret_cast<MergeTile *>(host + words()); }
+};
+
+// 1 planned, 0 this form does not apply (more than 8 rounds of merges, more
+// than 2^32 values, or packed keys switched off), -1 an error
+int hist_plan(HistPlan &P, const std::vector<int64_t> &seg_ofs, const std::vector<int64_t> &item_end)
+{
+    const int nseg = (int)seg_ofs.size() - 1;
+    const int64_t n = seg_ofs.back();
+    const char *pe = getenv("DRESS_HIP_HIST_PACKED");
+    if (n <= 0 || n > (int64_t)0xffffffffll || nseg < 1 || (pe && !strcmp(pe, "0"))) return 0;
+    merge_rounds(item_end, seg_ofs, [](const MergeTile &) {}, P.round_ofs);
+    if (P.round_ofs.size() > 9) return 0;
+    P.nseg = nseg;
+    // at least the pool's smallest pinned size, so that the copies are
+    // asynchronous
+    const size_t nt = P.round_ofs.back();
+    P.host = (int64_t *)pinned_alloc(std::max<size_t>(P.words() * sizeof(int64_t) + nt * sizeof(MergeTile), 1 << 20));
+    if (!P.host) {
+        set_error("host staging of the histogram failed");
+        return -1;
+    }
+    P.host[0] = 0;
+    P.host[1] = 0;
+    for (int g = 0; g <= nseg; g++) P.host[2 + (nseg + 1) + g] = seg_ofs[(size_t)g];
+    MergeTile *t = P.tiles();
+    size_t at = 0;
+    merge_rounds(item_end, seg_ofs, [&](const MergeTile &x) { t[at++] = x; }, P.round_ofs);
+    return 1;
+}
+
+// Every block the queued work reads is held here until the caller has
+// waited for that work (hist_collect, then hist_release).
 struct AsyncHist {
     int64_t n = 0;
     int nseg = 0;
     uint64_t *k0 = nullptr, *k1 = nullptr;
-    uint64_t *uniq = nullptr;             // k0 or k1: unique keys, then values
-    int64_t *cnt = nullptr, *dofs = nullptr, *rofs = nullptr, *druns = nullptr;
-    int *flag = nullptr;                  // [0, runs): segment of each run; [n]: layout failure
+    double *val = nullptr;                // k0 or k1: the runs' values
+    int64_t *cnt = nullptr;
+    // [0] runs, [1] layout failure, [2, nseg + 3) each segment's first run:
+    // read back in one copy into the same words of the plan's block
+    int64_t *stat = nullptr;
+    int64_t *segt = nullptr;              // segment offsets, then the tiles: uploaded in one copy
     unsigned long long *part = nullptr, *range = nullptr;
-    MergeTile *dtiles = nullptr;
     void *scratch = nullptr;              // run-length encoding scratch
-    // pinned: [0] runs, [1] failure, [2, nseg + 3) each segment's first
-    // run, [nseg + 3, 2 nseg + 4) the segment offsets, then the tiles
-    int64_t *host = nullptr;
+    int64_t *host = nullptr;              // the plan's pinned block
 };
 
 void hist_release(AsyncHist &H, bool keep_outputs)
 {
-    if (H.k0 != H.uniq) dfree(H.k0);
-    if (H.k1 != H.uniq) dfree(H.k1);
+    if ((void *)H.k0 != (void *)H.val) dfree(H.k0);
+    if ((void *)H.k1 != (void *)H.val) dfree(H.k1);
     if (!keep_outputs) {
-        dfree(H.uniq);
+        dfree(H.val);
         dfree(H.cnt);
     }
-    dfree(H.dofs); dfree(H.rofs); dfree(H.druns); dfree(H.flag); dfree(H.part); dfree(H.range);
-    dfree(H.dtiles); dfree(H.scratch);
+    dfree(H.stat); dfree(H.segt); dfree(H.part); dfree(H.range); dfree(H.scratch);
     pinned_free(H.host);
     H = AsyncHist();
 }
 
-// 1 queued, 0 this form does not apply and nothing is queued (more than 8
-// rounds of merges, more than 2^32 values, or packed keys switched off),
-// -1 an error
-int hist_enqueue(AsyncHist &H, const double *vals, int64_t n, const std::vector<int64_t> &seg_ofs,
-                 const std::vector<int64_t> &item_end, hipStream_t s)
+// queues the planned histogram of vals[0, n) on s, taking the plan's
+// pinned block; false on an error (then nothing is left queued)
+bool hist_enqueue(AsyncHist &H, HistPlan &P, const double *vals, int64_t n, hipStream_t s)
 {
     H = AsyncHist();
-    const int nseg = (int)seg_ofs.size() - 1;
-    const char *pe = getenv("DRESS_HIP_HIST_PACKED");
-    if (n <= 0 || n > (int64_t)0xffffffffll || nseg < 1 || (pe && !strcmp(pe, "0"))) return 0;
-    std::vector<size_t> round_ofs;
-    merge_rounds(item_end, seg_ofs, [](const MergeTile &) {}, round_ofs);
-    if (round_ofs.size() > 9) return 0;
-    int sbits = 0;
-    while ((1 << sbits) < nseg) sbits++;
+    const int nseg = P.nseg;
     H.n = n;
     H.nseg = nseg;
-    const size_t nt = round_ofs.back(), words = 2 * ((size_t)nseg + 1) + 2;
-    // at least the pool's smallest pinned size, so that the copies are
-    // asynchronous
-    H.host = (int64_t *)pinned_alloc(std::max<size_t>(words * sizeof(int64_t) + nt * sizeof(MergeTile), 1 << 20));
-    bool ok = H.host != nullptr;
-    if (!ok) set_error("host staging of the histogram failed");
-    int64_t *h_rofs = ok ? H.host + 2 : nullptr, *h_seg = ok ? H.host + 2 + (nseg + 1) : nullptr;
-    MergeTile *h_tiles = ok ? reinterpret_cast<MergeTile *>(H.host + words) : nullptr;
-    if (ok) {
-        H.host[0] = 0;
-        H.host[1] = 0;
-        for (int g = 0; g <= nseg; g++) h_seg[g] = seg_ofs[(size_t)g];
-        size_t at = 0;
-        merge_rounds(item_end, seg_ofs, [&](const MergeTile &t) { h_tiles[at++] = t; }, round_ofs);
-    }
-    ok = ok && dalloc(&H.k0, (size_t)n, "histogram keys") && dalloc(&H.k1, (size_t)n, "histogram keys") &&
-         dalloc(&H.flag, (size_t)n + 1, "run segments") && dalloc(&H.dofs, (size_t)nseg + 1, "segment offsets") &&
-         dalloc(&H.rofs, (size_t)nseg + 1, "run offsets") && dalloc(&H.part, 2 * RANGE_BLOCKS, "key range") &&
-         dalloc(&H.range, 2, "key range") && dalloc(&H.druns, 1, "run count") &&
-         dalloc(&H.cnt, (size_t)n + 1, "histogram counts") && (nt == 0 || dalloc(&H.dtiles, nt, "merge tiles"));
+    H.host = P.host;
+    P.host = nullptr;
+    int sbits = 0;
+    while ((1 << sbits) < nseg) sbits++;
+    const size_t nt = P.round_ofs.back();
+    const size_t seg_words = (size_t)nseg + 1, tile_words = nt * (sizeof(MergeTile) / sizeof(int64_t));
+    static_assert(sizeof(MergeTile) % sizeof(int64_t) == 0, "tiles are whole words");
+    const int64_t *h_seg = H.host + 2 + seg_words;           // the tiles follow
+    bool ok = h_seg[nseg] == n && dalloc(&H.k0, (size_t)n, "histogram keys") &&
+              dalloc(&H.k1, (size_t)n, "histogram keys") && dalloc(&H.stat, seg_words + 2, "run offsets") &&
+              dalloc(&H.segt, seg_words + tile_words, "segment offsets and merge tiles") &&
+              dalloc(&H.part, 2 * RANGE_BLOCKS, "key range") && dalloc(&H.range, 2, "key range") &&
+              dalloc(&H.cnt, (size_t)n + 1, "histogram counts");
     const int grid = grid_for(n, num_cus() * 8);
     const int rgrid = grid_for(n, RANGE_BLOCKS);
-    if (ok) {
-        hipLaunchKernelGGL(k_zero_int, dim3(1), dim3(64), 0, s, H.flag + n, 1);
-        ok = hipMemcpyAsync(H.dofs, h_seg, ((size_t)nseg + 1) * sizeof(int64_t), hipMemcpyHostToDevice, s) ==
-                 hipSuccess &&
-             (nt == 0 ||
-              hipMemcpyAsync(H.dtiles, h_tiles, nt * sizeof(MergeTile), hipMemcpyHostToDevice, s) == hipSuccess);
-    }
+    int *bad = reinterpret_cast<int *>(H.stat + 1);
+    const MergeTile *dtiles = reinterpret_cast<const MergeTile *>(H.segt + seg_words);
+    ok = ok && hipMemcpyAsync(H.segt, h_seg, (seg_words + tile_words) * sizeof(int64_t), hipMemcpyHostToDevice, s) ==
+                   hipSuccess;
     uint64_t *sorted = H.k1, *other = H.k0;
     if (ok) {
-        hipLaunchKernelGGL(k_sort_keys, dim3(rgrid), dim3(TPB), 0, s, vals, n, H.k0, H.part);
+        hipLaunchKernelGGL(k_sort_keys, dim3(rgrid), dim3(TPB), 0, s, vals, n, H.k0, H.part, H.stat + 1);
         hipLaunchKernelGGL(k_key_range, dim3(1), dim3(TPB), 0, s, H.part, rgrid, H.range);
-        const size_t shm = nseg < PACK_LDS_SEGS ? ((size_t)nseg + 1) * sizeof(int64_t) : 0;
-        hipLaunchKernelGGL(k_pack_keys, dim3(grid), dim3(TPB), shm, s, H.k0, H.dofs, nseg, n, H.range, -1, H.k1,
-                           sbits, H.flag + n);
-        for (size_t r = 0; r + 1 < round_ofs.size(); r++) {
-            hipLaunchKernelGGL(k_merge_tiles, dim3((unsigned)(round_ofs[r + 1] - round_ofs[r])), dim3(MERGE_TPB), 0,
-                               s, sorted, other, H.dtiles + round_ofs[r]);
+        const size_t shm = nseg < PACK_LDS_SEGS ? seg_words * sizeof(int64_t) : 0;
+        hipLaunchKernelGGL(k_pack_keys, dim3(grid), dim3(TPB), shm, s, H.k0, (const int64_t *)H.segt, nseg, n,
+                           H.range, -1, H.k1, sbits, bad);
+        for (size_t r = 0; r + 1 < P.round_ofs.size(); r++) {
+            hipLaunchKernelGGL(k_merge_tiles, dim3((unsigned)(P.round_ofs[r + 1] - P.round_ofs[r])), dim3(MERGE_TPB),
+                               0, s, sorted, other, dtiles + P.round_ofs[r]);
             std::swap(sorted, other);
         }
-        ok = prim_run_length_encode_u64(sorted, (size_t)n, other, H.cnt, H.druns, s, &H.scratch);
+        ok = prim_run_length_encode_u64(sorted, (size_t)n, other, H.cnt, H.stat, s, &H.scratch);
     }
     if (ok) {
-        H.uniq = other;
-        hipLaunchKernelGGL(k_unpack_runs, dim3(grid), dim3(TPB), 0, s, other, (int64_t)0, H.range, -1, H.flag,
-                           (const int64_t *)H.druns, (const int *)(H.flag + n));
-        hipLaunchKernelGGL(k_seg_run_starts, dim3(grid), dim3(TPB), 0, s, H.flag, (int64_t)0, nseg, H.rofs,
-                           (const int64_t *)H.druns, (const int *)(H.flag + n));
+        // the values go into the merged keys' block, free after the encoding
+        H.val = reinterpret_cast<double *>(sorted);
+        hipLaunchKernelGGL(k_runs_out, dim3(grid), dim3(TPB), 0, s, (const uint64_t *)other,
+                           (const unsigned long long *)H.range, (const int64_t *)H.stat, (const int *)bad, nseg,
+                           H.val, H.stat + 2);
         ok = hipGetLastError() == hipSuccess &&
-             hipMemcpyAsync(&H.host[0], H.druns, sizeof(int64_t), hipMemcpyDeviceToHost, s) == hipSuccess &&
-             hipMemcpyAsync(&H.host[1], H.flag + n, sizeof(int), hipMemcpyDeviceToHost, s) == hipSuccess &&
-             hipMemcpyAsync(h_rofs, H.rofs, ((size_t)nseg + 1) * sizeof(int64_t), hipMemcpyDeviceToHost, s) ==
+             hipMemcpyAsync(H.host, H.stat, (seg_words + 2) * sizeof(int64_t), hipMemcpyDeviceToHost, s) ==
                  hipSuccess;
     }
     if (!ok) {
         if (!has_error()) set_error("histogram assembly failed");
         (void)hipStreamSynchronize(s);              // queued kernels may read the blocks
         hist_release(H, false);
-        return -1;
     }
-    return 1;
+    return ok;
 }
 
 // after the queued work has finished: the run count, each segment's first
 // run, and whether the key layout failed (then neither is valid)
 void hist_collect(const AsyncHist &H, int64_t *runs, std::vector<int64_t> &run_ofs, bool *failed)
 {
-    int fail = 0;
+    int fail = 0;                                   // (the slot's first four bytes)
     memcpy(&fail, &H.host[1], sizeof(int));
     *failed = fail != 0;
     *runs = H.host[0];
@@ -2544,11 +2573,13 @@ int64_t batch_value_bound(int kind, int N, int64_t E, int k)
 }
 
 // values per group of a pipelined batch (DRESS_HIP_BATCH_GROUP_VALUES
-// overrides; 0 keeps a batch in one group)
+// overrides; 0 keeps a batch in one group).  The 512-graph delta^1 batch
+// (9.6M values, 8 groups) took 4.10-4.16 ms a call against 4.24-4.31 with
+// 1.8M (6 groups); ramps of growing groups gave 4.12-4.27.
 int64_t batch_group_values()
 {
     if (const char *e = getenv("DRESS_HIP_BATCH_GROUP_VALUES")) return (int64_t)atoll(e);
-    return 1800000;
+    return 1200000;
 }
 
 // consecutive graphs in at most max_groups groups of about per_group values
@@ -2865,6 +2896,8 @@ int dress_hip_lifted_batch(int kind, int k, int n_graphs, const int *N, const in
         std::vector<int> graph_of, item_parent, item_verts;
         std::vector<int64_t> item_live, ms_rows;
         size_t max_items = 0;                 // most items of one graph
+        HistPlan hp;                          // the histogram's merges, when queued without a wait
+        bool hist_planned = false;
         bool ok = false;
         std::string err;
     };
@@ -2880,6 +2913,26 @@ int dress_hip_lifted_batch(int kind, int k, int n_graphs, const int *N, const in
     auto merges = [&](const GroupPlan &P) {
         return merge_mode != 0 && (G > 1 || merge_mode == 1) && P.max_items <= 256;
     };
+    // each graph's values are one segment of its group's histogram: the
+    // items of a graph are contiguous
+    auto group_segments = [&](int gr, const GroupPlan &P) {
+        const int ga = gb[(size_t)gr], gz = gb[(size_t)gr + 1];
+        std::vector<int64_t> seg_ofs((size_t)(gz - ga) + 1, 0);
+        size_t it = 0;
+        for (int gi = ga; gi < gz; gi++) {
+            while (it < P.item_parent.size() && P.item_parent[it] == gi - ga) it++;
+            seg_ofs[(size_t)(gi - ga) + 1] = P.lo.val_ofs[it];
+        }
+        return seg_ofs;
+    };
+    // With merged runs, each group's histogram is planned with the group
+    // and queued on hs behind an event after the group's launches, with no
+    // wait on the host: it runs beside the next group's launches, and the
+    // host waits for its event only to queue the group's downloads.
+    // DRESS_HIP_HIST_ASYNC=0 makes the host wait for each group's launches,
+    // then run its histogram on hs.
+    const char *ha = getenv("DRESS_HIP_HIST_ASYNC");
+    const bool async_hist = !(ha && !strcmp(ha, "0")) && G > 1 && hs != s && !J.mixed && !quant;
     std::vector<std::unique_ptr<GroupPlan>> plans((size_t)G);
     {
         // host threads building parents: with several groups only the first
@@ -2909,6 +2962,13 @@ int dress_hip_lifted_batch(int kind, int k, int n_graphs, const int *N, const in
             P->ok = run_prepare(P->ps, P->graph_of, P->item_parent, P->item_verts, P->item_live, J, P->ms_rows,
                                 nullptr, P->R, P->lo);
             ppt.mark("prepared");
+            if (P->ok && async_hist && merges(*P) && gz > ga) {
+                const std::vector<int64_t> item_end(P->lo.val_ofs.begin() + 1, P->lo.val_ofs.end());
+                const int r = hist_plan(P->hp, group_segments(gr, *P), item_end);
+                P->ok = r >= 0;
+                P->hist_planned = r > 0;
+                ppt.mark("histogram planned");
+            }
             if (!P->ok) P->err = dress_hip_last_error();
             // a host-side test hook: DRESS_HIP_BATCH_FAIL_GROUP=g fails group g's
             // preparation, while earlier groups may be on the device
@@ -2958,28 +3018,13 @@ int dress_hip_lifted_batch(int kind, int k, int n_graphs, const int *N, const in
             return run_launch(P.R, P.ps, P.item_parent, J, nullptr, 0, s, P.lo, qk, qc, quant ? qcap : 0,
                               quant ? qtol : 0.0, nullptr, merges(P));
         };
-        // each graph's values are one segment of its group's histogram:
-        // the items of a graph are contiguous
-        auto group_segments = [&](int gr, const GroupPlan &P) {
-            const int ga = gb[(size_t)gr], gz = gb[(size_t)gr + 1];
-            std::vector<int64_t> seg_ofs((size_t)(gz - ga) + 1, 0);
-            size_t it = 0;
-            for (int gi = ga; gi < gz; gi++) {
-                while (it < P.item_parent.size() && P.item_parent[it] == gi - ga) it++;
-                seg_ofs[(size_t)(gi - ga) + 1] = P.lo.val_ofs[it];
-            }
-            return seg_ofs;
-        };
-        // With merged runs, each group's histogram is queued on s right
-        // behind its own launches and needs no wait on the host, so that a
-        // group's downloads run beside the next group's launches; the host
-        // waits for the histogram's event before it queues the downloads.
-        // DRESS_HIP_HIST_ASYNC=0 runs the histograms on hs after each group
-        // instead.
-        const char *ha = getenv("DRESS_HIP_HIST_ASYNC");
-        const bool async_hist = !(ha && !strcmp(ha, "0"));
         std::vector<AsyncHist> ah((size_t)G);
         std::vector<hipEvent_t> hev((size_t)G, nullptr);
+        hipEvent_t kev = nullptr;                        // end of a group's launches on s
+        if (async_hist && hipEventCreateWithFlags(&kev, hipEventDisableTiming) != hipSuccess) {
+            set_error("histogram event failed");
+            ok = false;
+        }
         // two groups stay queued on s ahead of the one being finished, so
         // the device goes from one group's launches to the next while the
         // host waits for a histogram
@@ -2993,17 +3038,23 @@ int dress_hip_lifted_batch(int kind, int k, int n_graphs, const int *N, const in
             }
             queued = g + 1;
             GroupPlan &P = *plans[(size_t)g];
-            if (!(async_hist && G > 1 && !J.mixed && !quant && merges(P) && gb[(size_t)g + 1] > gb[(size_t)g]))
+            if (!P.hist_planned) return;
+            if (hipEventRecord(kev, s) != hipSuccess || hipStreamWaitEvent(hs, kev, 0) != hipSuccess) {
+                set_error("histogram event failed");
+                chain = false;
                 return;
-            const std::vector<int64_t> item_end(P.lo.val_ofs.begin() + 1, P.lo.val_ofs.end());
-            const int r = hist_enqueue(ah[(size_t)g], P.lo.d_vals, P.lo.val_ofs.back(), group_segments(g, P),
-                                       item_end, s);
-            if (r < 0) chain = false;
-            else if (r > 0 && (hipEventCreateWithFlags(&hev[(size_t)g], hipEventDisableTiming) != hipSuccess ||
-                               hipEventRecord(hev[(size_t)g], s) != hipSuccess)) {
+            }
+            if (!hist_enqueue(ah[(size_t)g], P.hp, P.lo.d_vals, P.lo.val_ofs.back(), hs)) chain = false;
+            else if (hipEventCreateWithFlags(&hev[(size_t)g], hipEventDisableTiming) != hipSuccess ||
+                     hipEventRecord(hev[(size_t)g], hs) != hipSuccess) {
                 set_error("histogram event failed");
                 chain = false;
             }
+        };
+        // whether group g's plan is ready (then queueing it does not wait)
+        auto plan_ready = [&](int g) {
+            std::lock_guard<std::mutex> lk(pm);
+            return planned > g;
         };
         queue_group(0);
         queue_group(1);
@@ -3012,8 +3063,16 @@ int dress_hip_lifted_batch(int kind, int k, int n_graphs, const int *N, const in
             GroupPlan &P = *plans[(size_t)gr];
             ok = run_finish(P.R, P.ps, P.graph_of, P.item_parent, J, s, P.lo, qk, qc, quant ? qcap : 0,
                             quant ? qtol : 0.0, nullptr, gr < queued);
-            if (ok) queue_group(gr + 2);
-            if (!chain) ok = false;                      // a later group failed: stop after this one
+            pt.mark("group launches done");
+            // the group after next: now when its plan is ready, so that s
+            // always holds a group; else once this group's downloads are
+            // queued
+            bool next_queued = false;
+            if (ok && gr + 2 < G && plan_ready(gr + 2)) {
+                queue_group(gr + 2);
+                next_queued = true;
+                pt.mark("next launch queued");
+            }
             LiftOut &lo = P.lo;
             double *hv = nullptr;
             int64_t *hc = nullptr, runs = 0;
@@ -3027,12 +3086,13 @@ int dress_hip_lifted_batch(int kind, int k, int n_graphs, const int *N, const in
                 if (ok) hist_collect(H, &runs, run_ofs, &failed);
                 have = ok && !failed;
                 if (have) {
-                    hv = reinterpret_cast<double *>(H.uniq);
+                    hv = H.val;
                     hc = H.cnt;
                 }
                 hist_release(H, have);
                 (void)hipEventDestroy(hev[(size_t)gr]);
                 hev[(size_t)gr] = nullptr;
+                pt.mark("group histogram");
             }
             // else (and when the packed keys did not fit) on hs, waited for
             if (ok && gz > ga && !have) {
@@ -3060,6 +3120,15 @@ int dress_hip_lifted_batch(int kind, int k, int n_graphs, const int *N, const in
             held.push_back(hv);
             held.push_back(hc);
             if (!ok) break;
+            pt.mark("downloads queued");
+            if (!next_queued) {
+                queue_group(gr + 2);
+                pt.mark("next launch queued");
+            }
+            if (!chain) {                                // a later group failed
+                ok = false;
+                break;
+            }
             dfree(lo.d_vals);
             lo.d_vals = nullptr;
             runs_total += runs;
@@ -3085,6 +3154,7 @@ int dress_hip_lifted_batch(int kind, int k, int n_graphs, const int *N, const in
                 hist_release(ah[(size_t)g], false);
                 (void)hipEventDestroy(hev[(size_t)g]);
             }
+        if (kev) (void)hipEventDestroy(kev);
         stop_planning = true;
         if (planner.joinable()) planner.join();
         for (auto &P : plans)
