@@ -290,6 +290,41 @@ def er2000(dh, gen):
     return out
 
 
+def lifted_run(dh, gen):
+    """The batched lifted workloads alone, for kernel traces and counter
+    runs: delta^1 of the 512-graph batch in one group and pipelined, then
+    delta^2 (FP64 and FP32->FP64) and nabla^1 of the 12-regular graph on 256
+    vertices; untimed calls of each (three, two), then the timed ones."""
+    batch = dh.pack_graphs(gen.delta1_batch_graphs(512))
+    for name, gv in (("delta1 one group", "0"), ("delta1 pipelined", None)):
+        if gv is not None:
+            os.environ["DRESS_HIP_BATCH_GROUP_VALUES"] = gv
+        try:
+            for _ in range(3):
+                dh.lifted_batch(batch, k=1, precision=dh.FP64)
+            for _ in range(3):
+                t0 = time.perf_counter()
+                r = dh.lifted_batch(batch, k=1, precision=dh.FP64)
+                log(f"{name}: wall {(time.perf_counter() - t0) * 1e3:.2f} ms, device {r.device_ms:.3f} ms")
+        finally:
+            os.environ.pop("DRESS_HIP_BATCH_GROUP_VALUES", None)
+    n4, U4, V4 = gen.regular_switch(256, 12, 7)
+    with dh.Graph(n4, U4, V4) as g4:
+        for name, kind, k, prec in (("delta2 FP64", "delta", 2, dh.FP64), ("delta2 FP32->FP64", "delta", 2, dh.FP32_FP64),
+                                    ("nabla1 FP64", "nabla", 1, dh.FP64)):
+            dh.set_precision(prec)
+            try:
+                for i in range(4):
+                    t0 = time.perf_counter()
+                    res = g4.lifted(kind, k)
+                    wall = (time.perf_counter() - t0) * 1e3
+                    del res
+                    if i >= 2:
+                        log(f"{name}: wall {wall:.2f} ms, device {dh.last_lifted_stats()['device_ms']:.3f} ms")
+            finally:
+                dh.set_precision(dh.FP64)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -303,6 +338,8 @@ def main():
                     help="one FP64 fit and the copy and gather probes only, for hardware-counter runs")
     ap.add_argument("--large-delta-only", action="store_true",
                     help="only the delta^1 section of the 2,000-vertex graph (for kernel traces)")
+    ap.add_argument("--lifted-run", action="store_true",
+                    help="only the batched delta^1, delta^2 and nabla^1 calls (for kernel traces and counters)")
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
         print("bench.py runs on one GPU: the DRESS engine keeps the whole edge set on a single "
@@ -315,6 +352,9 @@ def main():
         raise SystemExit("no GPU visible to libdress_hip.so: " + dh._lib.last_error())
     if args.large_delta_only:
         print(json.dumps({"delta1_large": delta1_large(dh, gen)}), flush=True)
+        return
+    if args.lifted_run:
+        lifted_run(dh, gen)
         return
 
     t = time.perf_counter()
