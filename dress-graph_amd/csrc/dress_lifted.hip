@@ -22,6 +22,7 @@
 #include <cstdio>
 #include <cstdlib>
 #include <cstring>
+#include <deque>
 #include <functional>
 #include <map>
 #include <string>
@@ -547,13 +548,13 @@ __global__ void k_zero_int(int *p, int n)
 }
 
 // ascending sort keys of doubles (the bit pattern, sign-folded), and each
-// block's smallest and largest key into part[2b], part[2b + 1]; *zero (when
-// given) is cleared
+// block's smallest and largest key into part[2b], part[2b + 1]; zero[0] and
+// zero[1] (when given) are cleared
 constexpr int RANGE_BLOCKS = 1024;
 __global__ void __launch_bounds__(TPB) k_sort_keys(const double *v, int64_t n, uint64_t *key,
                                                    unsigned long long *part, int64_t *zero)
 {
-    if (zero && blockIdx.x == 0 && threadIdx.x == 0) *zero = 0;
+    if (zero && blockIdx.x == 0 && threadIdx.x < 2) zero[threadIdx.x] = 0;
     unsigned long long lo = ~0ull, hi = 0ull;
     for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) {
         const uint64_t b = (uint64_t)__double_as_longlong(v[i]);
@@ -706,16 +707,19 @@ __global__ void k_unpack_runs(uint64_t *ukey, int64_t runs, const unsigned long 
 }
 
 // The runs' packed keys (key layout from the key range, run count from
-// the device) to values, into val, and each segment's first run into ofs
-// (nseg + 1 entries, as k_seg_run_starts); nothing after a layout failure
-__global__ void k_runs_out(const uint64_t *ukey, const unsigned long long *range, const int64_t *runs_dev,
-                           const int *bad, int nseg, double *val, int64_t *ofs)
+// the device) to values, into val, each segment's first run into ofs
+// (nseg + 1 entries, as k_seg_run_starts), and the largest count into
+// *maxc (cleared before); nothing after a layout failure
+__global__ void k_runs_out(const uint64_t *ukey, const int64_t *cnt, const unsigned long long *range,
+                           const int64_t *runs_dev, const int *bad, int nseg, double *val, int64_t *ofs,
+                           unsigned long long *maxc)
 {
     if (*bad) return;
     const int64_t runs = *runs_dev;
     const int vbits = span_bits(range);
     const uint64_t lo = range[0];
     const uint64_t mask = vbits < 64 ? (((uint64_t)1 << vbits) - 1) : ~(uint64_t)0;
+    unsigned long long most = 0;
     for (int64_t j = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; j < runs; j += (int64_t)gridDim.x * blockDim.x) {
         const uint64_t key = ukey[j];
         const uint64_t k = (key & mask) + lo;
@@ -725,6 +729,36 @@ __global__ void k_runs_out(const uint64_t *ukey, const unsigned long long *range
         for (int x = prev + 1; x <= g; x++) ofs[x] = j;
         if (j == runs - 1)
             for (int x = g + 1; x <= nseg; x++) ofs[x] = runs;
+        const unsigned long long c = (unsigned long long)cnt[j];
+        most = c > most ? c : most;
+    }
+    for (int o = 32; o > 0; o >>= 1) {
+        const unsigned long long x = __shfl_xor(most, o);
+        most = x > most ? x : most;
+    }
+    if ((threadIdx.x & 63) == 0 && most) atomicMax(maxc, most);
+}
+
+// bytes per count that hold every count up to most
+__host__ __device__ inline int count_width(unsigned long long most)
+{
+    return most <= 0xffull ? 1 : most <= 0xffffull ? 2 : most <= 0xffffffffull ? 4 : 8;
+}
+
+// the counts narrowed to count_width(*maxc) bytes into out (nothing to do
+// at 8 bytes, or after a layout failure)
+__global__ void k_pack_counts(const int64_t *cnt, const int64_t *runs_dev, const int *bad,
+                              const unsigned long long *maxc, void *out)
+{
+    if (*bad) return;
+    const int w = count_width(*maxc);
+    if (w == 8) return;
+    const int64_t runs = *runs_dev;
+    for (int64_t j = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; j < runs; j += (int64_t)gridDim.x * blockDim.x) {
+        const uint64_t c = (uint64_t)cnt[j];
+        if (w == 1) static_cast<uint8_t *>(out)[j] = (uint8_t)c;
+        else if (w == 2) static_cast<uint16_t *>(out)[j] = (uint16_t)c;
+        else static_cast<uint32_t *>(out)[j] = (uint32_t)c;
     }
 }
 
@@ -1040,14 +1074,16 @@ bool exact_histogram_dev(const double *vals, int64_t n, const std::vector<int64_
 struct HistPlan {
     int nseg = 0;
     std::vector<size_t> round_ofs;        // tiles before each round of merges
-    // pinned: [0] runs, [1] failure, [2, nseg + 3) each segment's first
-    // run, [nseg + 3, 2 nseg + 4) the segment offsets, then the tiles
+    // pinned: [0] runs, [1] failure, [2] the largest count, [3, nseg + 4)
+    // each segment's first run (read back in one copy), [nseg + 4,
+    // 2 nseg + 5) the segment offsets, then the tiles (uploaded in one)
     int64_t *host = nullptr;
+    static constexpr int HEAD = 3;
     HistPlan() = default;
     HistPlan(const HistPlan &) = delete;
     HistPlan &operator=(const HistPlan &) = delete;
     ~HistPlan() { pinned_free(host); }
-    size_t words() const { return 2 * ((size_t)nseg + 1) + 2; }
+    size_t words() const { return 2 * ((size_t)nseg + 1) + HEAD; }
     MergeTile *tiles() const { return reinterpret_cast<MergeTile *>(host + words()); }
 };
 
@@ -1070,9 +1106,8 @@ int hist_plan(HistPlan &P, const std::vector<int64_t> &seg_ofs, const std::vecto
         set_error("host staging of the histogram failed");
         return -1;
     }
-    P.host[0] = 0;
-    P.host[1] = 0;
-    for (int g = 0; g <= nseg; g++) P.host[2 + (nseg + 1) + g] = seg_ofs[(size_t)g];
+    for (int i = 0; i < HistPlan::HEAD; i++) P.host[i] = 0;
+    for (int g = 0; g <= nseg; g++) P.host[HistPlan::HEAD + (nseg + 1) + g] = seg_ofs[(size_t)g];
     MergeTile *t = P.tiles();
     size_t at = 0;
     merge_rounds(item_end, seg_ofs, [&](const MergeTile &x) { t[at++] = x; }, P.round_ofs);
@@ -1086,9 +1121,11 @@ struct AsyncHist {
     int nseg = 0;
     uint64_t *k0 = nullptr, *k1 = nullptr;
     double *val = nullptr;                // k0 or k1: the runs' values
+    void *packed = nullptr;               // the other: the counts at their narrowest width
     int64_t *cnt = nullptr;
-    // [0] runs, [1] layout failure, [2, nseg + 3) each segment's first run:
-    // read back in one copy into the same words of the plan's block
+    // [0] runs, [1] layout failure, [2] the largest count, [3, nseg + 4)
+    // each segment's first run: read back in one copy into the same words
+    // of the plan's block
     int64_t *stat = nullptr;
     int64_t *segt = nullptr;              // segment offsets, then the tiles: uploaded in one copy
     unsigned long long *part = nullptr, *range = nullptr;
@@ -1096,12 +1133,13 @@ struct AsyncHist {
     int64_t *host = nullptr;              // the plan's pinned block
 };
 
+// keep_outputs: the values, counts and packed counts stay for the caller
+// to download and free
 void hist_release(AsyncHist &H, bool keep_outputs)
 {
-    if ((void *)H.k0 != (void *)H.val) dfree(H.k0);
-    if ((void *)H.k1 != (void *)H.val) dfree(H.k1);
     if (!keep_outputs) {
-        dfree(H.val);
+        dfree(H.k0);
+        dfree(H.k1);
         dfree(H.cnt);
     }
     dfree(H.stat); dfree(H.segt); dfree(H.part); dfree(H.range); dfree(H.scratch);
@@ -1124,9 +1162,10 @@ bool hist_enqueue(AsyncHist &H, HistPlan &P, const double *vals, int64_t n, hipS
     const size_t nt = P.round_ofs.back();
     const size_t seg_words = (size_t)nseg + 1, tile_words = nt * (sizeof(MergeTile) / sizeof(int64_t));
     static_assert(sizeof(MergeTile) % sizeof(int64_t) == 0, "tiles are whole words");
-    const int64_t *h_seg = H.host + 2 + seg_words;           // the tiles follow
+    const int64_t *h_seg = H.host + HistPlan::HEAD + seg_words;           // the tiles follow
     bool ok = h_seg[nseg] == n && dalloc(&H.k0, (size_t)n, "histogram keys") &&
-              dalloc(&H.k1, (size_t)n, "histogram keys") && dalloc(&H.stat, seg_words + 2, "run offsets") &&
+              dalloc(&H.k1, (size_t)n, "histogram keys") &&
+              dalloc(&H.stat, seg_words + HistPlan::HEAD, "run offsets") &&
               dalloc(&H.segt, seg_words + tile_words, "segment offsets and merge tiles") &&
               dalloc(&H.part, 2 * RANGE_BLOCKS, "key range") && dalloc(&H.range, 2, "key range") &&
               dalloc(&H.cnt, (size_t)n + 1, "histogram counts");
@@ -1151,14 +1190,19 @@ bool hist_enqueue(AsyncHist &H, HistPlan &P, const double *vals, int64_t n, hipS
         ok = prim_run_length_encode_u64(sorted, (size_t)n, other, H.cnt, H.stat, s, &H.scratch);
     }
     if (ok) {
-        // the values go into the merged keys' block, free after the encoding
+        // the values go into the merged keys' block, free after the
+        // encoding, and the narrowed counts into the unique keys' block
         H.val = reinterpret_cast<double *>(sorted);
-        hipLaunchKernelGGL(k_runs_out, dim3(grid), dim3(TPB), 0, s, (const uint64_t *)other,
+        H.packed = other;
+        unsigned long long *maxc = reinterpret_cast<unsigned long long *>(H.stat + 2);
+        hipLaunchKernelGGL(k_runs_out, dim3(grid), dim3(TPB), 0, s, (const uint64_t *)other, (const int64_t *)H.cnt,
                            (const unsigned long long *)H.range, (const int64_t *)H.stat, (const int *)bad, nseg,
-                           H.val, H.stat + 2);
+                           H.val, H.stat + HistPlan::HEAD, maxc);
+        hipLaunchKernelGGL(k_pack_counts, dim3(grid), dim3(TPB), 0, s, (const int64_t *)H.cnt,
+                           (const int64_t *)H.stat, (const int *)bad, (const unsigned long long *)maxc, H.packed);
         ok = hipGetLastError() == hipSuccess &&
-             hipMemcpyAsync(H.host, H.stat, (seg_words + 2) * sizeof(int64_t), hipMemcpyDeviceToHost, s) ==
-                 hipSuccess;
+             hipMemcpyAsync(H.host, H.stat, (seg_words + HistPlan::HEAD) * sizeof(int64_t), hipMemcpyDeviceToHost,
+                            s) == hipSuccess;
     }
     if (!ok) {
         if (!has_error()) set_error("histogram assembly failed");
@@ -1169,14 +1213,16 @@ bool hist_enqueue(AsyncHist &H, HistPlan &P, const double *vals, int64_t n, hipS
 }
 
 // after the queued work has finished: the run count, each segment's first
-// run, and whether the key layout failed (then neither is valid)
-void hist_collect(const AsyncHist &H, int64_t *runs, std::vector<int64_t> &run_ofs, bool *failed)
+// run, the bytes per count of H.packed (8: use H.cnt), and whether the key
+// layout failed (then none is valid)
+void hist_collect(const AsyncHist &H, int64_t *runs, std::vector<int64_t> &run_ofs, int *width, bool *failed)
 {
     int fail = 0;                                   // (the slot's first four bytes)
     memcpy(&fail, &H.host[1], sizeof(int));
     *failed = fail != 0;
     *runs = H.host[0];
-    run_ofs.assign(H.host + 2, H.host + 2 + H.nseg + 1);
+    *width = count_width((unsigned long long)H.host[2]);
+    run_ofs.assign(H.host + HistPlan::HEAD, H.host + HistPlan::HEAD + H.nseg + 1);
 }
 
 // the same, downloaded into vectors (runs of all segments; useg unused here)
@@ -2511,6 +2557,108 @@ namespace {
 // below `end` that no worker has taken yet and returns once every graph
 // below `end` is built.  The destructor lets the workers finish the graph in
 // hand and joins them.
+// Host threads for work that follows device copies: started on first use,
+// shared by every caller, and left waiting on their queue until the
+// process ends.
+class HostTeam {
+public:
+    static HostTeam &get()
+    {
+        static HostTeam *t = new HostTeam();   // lives to process end
+        return *t;
+    }
+    void submit(std::function<void()> f)
+    {
+        {
+            std::lock_guard<std::mutex> lk(mu_);
+            q_.push_back(std::move(f));
+        }
+        cv_.notify_one();
+    }
+
+private:
+    HostTeam()
+    {
+        const int n = std::max(1, std::min(8, (int)std::thread::hardware_concurrency()));
+        for (int i = 0; i < n; i++) std::thread([this] { loop(); }).detach();
+    }
+    void loop()
+    {
+        for (;;) {
+            std::function<void()> f;
+            {
+                std::unique_lock<std::mutex> lk(mu_);
+                cv_.wait(lk, [&] { return !q_.empty(); });
+                f = std::move(q_.front());
+                q_.pop_front();
+            }
+            f();
+        }
+    }
+    std::mutex mu_;
+    std::condition_variable cv_;
+    std::deque<std::function<void()>> q_;
+};
+
+// Counts that came down narrowed (1, 2 or 4 bytes each, into pinned
+// staging blocks), widened into the int64 result by the host team in
+// chunks, each chunk once the event after its copy has passed.  wait()
+// returns when every chunk is done; the staging blocks and events go with
+// the set.
+class CountWidening {
+public:
+    CountWidening() = default;
+    CountWidening(const CountWidening &) = delete;
+    CountWidening &operator=(const CountWidening &) = delete;
+    ~CountWidening()
+    {
+        wait();
+        for (void *p : staging) pinned_free(p);
+        for (hipEvent_t e : events) (void)hipEventDestroy(e);
+    }
+    std::vector<void *> staging;
+    std::vector<hipEvent_t> events;
+    void add(const void *src, int64_t *dst, int64_t n, int width, hipEvent_t ev)
+    {
+        constexpr int64_t CHUNK = (int64_t)1 << 18;
+        for (int64_t a = 0; a < n; a += CHUNK) {
+            const int64_t b = std::min(n, a + CHUNK);
+            {
+                std::lock_guard<std::mutex> lk(mu_);
+                pending_++;
+            }
+            HostTeam::get().submit([this, src, dst, a, b, width, ev] {
+                const bool ok = hipEventSynchronize(ev) == hipSuccess;
+                if (ok) {
+                    if (width == 1)
+                        for (int64_t i = a; i < b; i++) dst[i] = static_cast<const uint8_t *>(src)[i];
+                    else if (width == 2)
+                        for (int64_t i = a; i < b; i++) dst[i] = static_cast<const uint16_t *>(src)[i];
+                    else
+                        for (int64_t i = a; i < b; i++) dst[i] = static_cast<const uint32_t *>(src)[i];
+                }
+                // (notified under the lock: the set may go as soon as the
+                // waiter sees pending_ reach 0)
+                std::lock_guard<std::mutex> lk(mu_);
+                if (!ok) failed_ = true;
+                if (--pending_ == 0) cv_.notify_all();
+            });
+        }
+    }
+    bool wait()
+    {
+        std::unique_lock<std::mutex> lk(mu_);
+        cv_.wait(lk, [&] { return pending_ == 0; });
+        return !failed_;
+    }
+
+private:
+    std::mutex mu_;
+    std::condition_variable cv_;
+    int64_t pending_ = 0;
+    bool failed_ = false;
+};
+
 class ParentPool {
 public:
     ParentPool(int n, int threads, std::function<void(int)> fn) : n_(n), fn_(std::move(fn)), done_((size_t)n, 0)
@@ -2767,6 +2915,8 @@ int dress_hip_lifted_batch(int kind, int k, int n_graphs, const int *N, const in
                            dress_hip_lifted_batch_t *out)
 {
     using namespace dh;
+    auto t0 = std::chrono::steady_clock::now();
+    PhaseTimer pt("dress_hip_lifted_batch");
     clear_error();
     if (!out) { set_error("NULL result"); return -1; }
     memset(out, 0, sizeof(*out));
@@ -2795,8 +2945,7 @@ int dress_hip_lifted_batch(int kind, int k, int n_graphs, const int *N, const in
             return -1;
         }
     }
-    auto t0 = std::chrono::steady_clock::now();
-    PhaseTimer pt("dress_hip_lifted_batch");
+    pt.mark("arguments checked");
     BatchScope scope;
     hipStream_t s = ctx_stream();
     const bool own_stream = s == nullptr;
@@ -2834,6 +2983,7 @@ int dress_hip_lifted_batch(int kind, int k, int n_graphs, const int *N, const in
         }
     }
     const int G = (int)gb.size() - 1;
+    pt.mark("groups chosen");
     // parents and items of each graph (host worker threads, graph order)
     std::vector<HostParent> parents((size_t)n_graphs);
     std::vector<std::vector<int>> g_verts((size_t)n_graphs);
@@ -2878,10 +3028,18 @@ int dress_hip_lifted_batch(int kind, int k, int n_graphs, const int *N, const in
         ok = out->hist_value && out->hist_count;
     }
     if (!ok) set_error("host allocation of batch results failed");
+    pt.mark("results allocated");
     auto d2h = [](void *dst, const void *src, size_t bytes, hipStream_t st) {
         return bytes == 0 || hipMemcpyAsync(dst, src, bytes, hipMemcpyDeviceToHost, st) == hipSuccess;
     };
     std::vector<void *> held;            // device blocks that downloads still read
+    // A pipelined group's counts come down at the narrowest width that
+    // holds its largest count and are widened on host threads, so the link
+    // carries 9 rather than 16 bytes per distinct value when every count is
+    // below 256.  DRESS_HIP_NARROW_COUNTS=0 downloads them as int64.
+    const char *nc = getenv("DRESS_HIP_NARROW_COUNTS");
+    const bool narrow_counts = !(nc && !strcmp(nc, "0"));
+    CountWidening widen;
     unsigned long long *qk = nullptr, *qc = nullptr;
     int64_t qcap = 1024, runs_total = 0, subgraphs = 0, values = 0;
     int reruns = 0;
@@ -3056,38 +3214,33 @@ int dress_hip_lifted_batch(int kind, int k, int n_graphs, const int *N, const in
             std::lock_guard<std::mutex> lk(pm);
             return planned > g;
         };
-        queue_group(0);
-        queue_group(1);
-        for (int gr = 0; ok && gr < G; gr++) {
+        // group gr's histogram collected (or made on hs and waited for),
+        // its downloads queued on cs, then its host state released
+        auto finish_group = [&](int gr) -> bool {
             const int ga = gb[(size_t)gr], gz = gb[(size_t)gr + 1];
             GroupPlan &P = *plans[(size_t)gr];
-            ok = run_finish(P.R, P.ps, P.graph_of, P.item_parent, J, s, P.lo, qk, qc, quant ? qcap : 0,
-                            quant ? qtol : 0.0, nullptr, gr < queued);
-            pt.mark("group launches done");
-            // the group after next: now when its plan is ready, so that s
-            // always holds a group; else once this group's downloads are
-            // queued
-            bool next_queued = false;
-            if (ok && gr + 2 < G && plan_ready(gr + 2)) {
-                queue_group(gr + 2);
-                next_queued = true;
-                pt.mark("next launch queued");
-            }
             LiftOut &lo = P.lo;
             double *hv = nullptr;
             int64_t *hc = nullptr, runs = 0;
+            void *packed = nullptr;                      // the counts at width bytes each (width < 8)
+            int width = 8;
             std::vector<int64_t> run_ofs((size_t)(gz - ga) + 1, 0);
             bool have = false;                           // the histogram is on the device
-            if (ok && hev[(size_t)gr]) {
+            bool good = true;
+            if (hev[(size_t)gr]) {
                 AsyncHist &H = ah[(size_t)gr];
-                ok = hipEventSynchronize(hev[(size_t)gr]) == hipSuccess;
-                if (!ok) set_error("batch histogram failed");
+                good = hipEventSynchronize(hev[(size_t)gr]) == hipSuccess;
+                if (!good) set_error("batch histogram failed");
                 bool failed = true;
-                if (ok) hist_collect(H, &runs, run_ofs, &failed);
-                have = ok && !failed;
+                if (good) hist_collect(H, &runs, run_ofs, &width, &failed);
+                have = good && !failed;
                 if (have) {
                     hv = H.val;
                     hc = H.cnt;
+                    packed = H.packed;
+                    held.push_back(packed);              // (hv and hc below)
+                } else {
+                    width = 8;
                 }
                 hist_release(H, have);
                 (void)hipEventDestroy(hev[(size_t)gr]);
@@ -3095,40 +3248,45 @@ int dress_hip_lifted_batch(int kind, int k, int n_graphs, const int *N, const in
                 pt.mark("group histogram");
             }
             // else (and when the packed keys did not fit) on hs, waited for
-            if (ok && gz > ga && !have) {
+            if (good && gz > ga && !have) {
                 std::vector<int64_t> item_end;
                 if (merges(P)) item_end.assign(lo.val_ofs.begin() + 1, lo.val_ofs.end());
-                ok = exact_histogram_dev(lo.d_vals, lo.val_ofs.back(), group_segments(gr, P), hs, &hv, &hc, &runs,
-                                         run_ofs, merges(P) ? &item_end : nullptr);
+                good = exact_histogram_dev(lo.d_vals, lo.val_ofs.back(), group_segments(gr, P), hs, &hv, &hc, &runs,
+                                           run_ofs, merges(P) ? &item_end : nullptr);
             }
             if (gr == 0) pt.mark("first group on the device");
-            if (ok && G == 1) {
+            if (good && G == 1) {
                 out->hist_value = (double *)pinned_alloc(((size_t)runs + 1) * sizeof(double));
                 out->hist_count = (int64_t *)pinned_alloc(((size_t)runs + 1) * sizeof(int64_t));
-                ok = out->hist_value && out->hist_count;
-                if (!ok) set_error("host allocation of batch results failed");
-            } else if (ok && runs_total + runs > cap) {
+                good = out->hist_value && out->hist_count;
+                if (!good) set_error("host allocation of batch results failed");
+            } else if (good && runs_total + runs > cap) {
                 set_error("batch histogram larger than its bound");
-                ok = false;
+                good = false;
             }
-            if (ok) {
+            if (good) {
                 for (int gi = ga; gi < gz; gi++) out->hist_offset[gi] = runs_total + run_ofs[(size_t)(gi - ga)];
-                ok = d2h(out->hist_value + runs_total, hv, (size_t)runs * sizeof(double), cs ? cs : s) &&
-                     d2h(out->hist_count + runs_total, hc, (size_t)runs * sizeof(int64_t), cs ? cs : s);
-                if (!ok) set_error("batch result download failed");
+                good = d2h(out->hist_value + runs_total, hv, (size_t)runs * sizeof(double), cs ? cs : s);
+                if (good && packed && width < 8 && narrow_counts && cs && runs > 0) {
+                    // into a pinned staging block (pool-sized, so the copy
+                    // is asynchronous), widened after the event
+                    void *stg = pinned_alloc(std::max<size_t>((size_t)runs * (size_t)width, (size_t)1 << 20));
+                    hipEvent_t ev = nullptr;
+                    if (stg) widen.staging.push_back(stg);
+                    good = stg && hipEventCreateWithFlags(&ev, hipEventDisableTiming) == hipSuccess;
+                    if (ev) widen.events.push_back(ev);
+                    good = good && d2h(stg, packed, (size_t)runs * (size_t)width, cs) &&
+                           hipEventRecord(ev, cs) == hipSuccess;
+                    if (good) widen.add(stg, out->hist_count + runs_total, runs, width, ev);
+                } else {
+                    good = good && d2h(out->hist_count + runs_total, hc, (size_t)runs * sizeof(int64_t), cs ? cs : s);
+                }
+                if (!good && !has_error()) set_error("batch result download failed");
             }
             held.push_back(hv);
             held.push_back(hc);
-            if (!ok) break;
+            if (!good) return false;
             pt.mark("downloads queued");
-            if (!next_queued) {
-                queue_group(gr + 2);
-                pt.mark("next launch queued");
-            }
-            if (!chain) {                                // a later group failed
-                ok = false;
-                break;
-            }
             dfree(lo.d_vals);
             lo.d_vals = nullptr;
             runs_total += runs;
@@ -3142,7 +3300,35 @@ int dress_hip_lifted_batch(int kind, int k, int n_graphs, const int *N, const in
                 parents[(size_t)gi] = HostParent();
                 std::vector<int>().swap(g_verts[(size_t)gi]);
             }
+            return true;
+        };
+        queue_group(0);
+        queue_group(1);
+        // Group gr - 1's histogram is collected and its downloads queued
+        // once group gr's launches are done: by then the histogram, queued
+        // behind gr - 1's launches, has long finished, and the host waits
+        // only for launches.
+        for (int gr = 0; ok && gr < G; gr++) {
+            GroupPlan &P = *plans[(size_t)gr];
+            ok = run_finish(P.R, P.ps, P.graph_of, P.item_parent, J, s, P.lo, qk, qc, quant ? qcap : 0,
+                            quant ? qtol : 0.0, nullptr, gr < queued);
+            pt.mark("group launches done");
+            // the group after next: now when its plan is ready, so that s
+            // always holds a group; else after the downloads
+            bool next_queued = false;
+            if (ok && gr + 2 < G && plan_ready(gr + 2)) {
+                queue_group(gr + 2);
+                next_queued = true;
+                pt.mark("next launch queued");
+            }
+            if (ok && gr > 0) ok = finish_group(gr - 1);
+            if (ok && !next_queued) {
+                queue_group(gr + 2);
+                pt.mark("next launch queued");
+            }
+            if (!chain) ok = false;                      // a later group failed
         }
+        if (ok && G > 0) ok = finish_group(G - 1);
         if (!ok) {
             // queued work may still read the groups' blocks
             (void)hipStreamSynchronize(s);
@@ -3200,6 +3386,10 @@ int dress_hip_lifted_batch(int kind, int k, int n_graphs, const int *N, const in
         ok = false;
     }
     if (hipStreamSynchronize(s) != hipSuccess && ok) {
+        set_error("batch result download failed");
+        ok = false;
+    }
+    if (!widen.wait() && ok) {                    // before the results may be freed below
         set_error("batch result download failed");
         ok = false;
     }

@@ -268,6 +268,36 @@ def test_grouped_batch_equals_one_group(gpu, ora, gen, monkeypatch, kind, k, pre
             assert np.array_equal(hv, ov) and np.array_equal(hc, oc), (kind, k, n)
 
 
+def _complete_graph(n):
+    iu = np.triu_indices(n, 1)
+    return (n, iu[0].astype(np.int32), iu[1].astype(np.int32))
+
+
+def test_grouped_batch_narrow_counts(gpu, ora, gen, monkeypatch):
+    """A pipelined group's counts travel at the narrowest width that holds
+    its largest count (1, 2 or 4 bytes) and are widened on host threads.
+    Complete graphs, whose delta^1 subgraphs are all the same graph, put
+    counts past 255 (K_12) and past 65,535 (K_64) into groups beside graphs
+    of distinct values; every histogram equals the one-group call's and the
+    int64 download's, and the oracle's."""
+    graphs = list(gen.delta1_batch_graphs(12, n_lo=8, n_hi=30))
+    graphs[3], graphs[7], graphs[10] = _complete_graph(12), _complete_graph(64), _complete_graph(5)
+    oracle = [ora.lifted("delta", n, U, V, k=1)[:2] for n, U, V in graphs]
+    assert 255 < oracle[3][1].max() <= 65535 and oracle[7][1].max() > 65535 and oracle[10][1].max() <= 255
+    monkeypatch.setenv("DRESS_HIP_BATCH_GROUP_VALUES", "0")
+    one = gpu.lifted_batch(graphs, k=1)
+    monkeypatch.setenv("DRESS_HIP_BATCH_GROUP_VALUES", "1")                 # 8 groups
+    runs = []
+    for narrow in ("1", "0"):
+        monkeypatch.setenv("DRESS_HIP_NARROW_COUNTS", narrow)
+        for _ in range(2):                                                  # staging blocks reused
+            runs.append(gpu.lifted_batch(graphs, k=1))
+    for r in runs:
+        for (v0, c0), (v1, c1), (ov, oc) in zip(one.histograms, r.histograms, oracle):
+            assert np.array_equal(v0, v1) and np.array_equal(c0, c1)
+            assert np.array_equal(v1, ov) and np.array_equal(c1, oc)
+
+
 def test_histogram_key_layouts_agree(gpu, ora, gen, monkeypatch):
     """The exact histogram sorts one 64-bit key per value holding (graph,
     value) when the values' keys span few enough bits to leave room for the
