@@ -732,11 +732,19 @@ __global__ void k_runs_out(const uint64_t *ukey, const int64_t *cnt, const unsig
         const unsigned long long c = (unsigned long long)cnt[j];
         most = c > most ? c : most;
     }
+    // one atomic per workgroup: one per wave on the one address cost
+    // 100 us a group of 1.2M runs
     for (int o = 32; o > 0; o >>= 1) {
         const unsigned long long x = __shfl_xor(most, o);
         most = x > most ? x : most;
     }
-    if ((threadIdx.x & 63) == 0 && most) atomicMax(maxc, most);
+    __shared__ unsigned long long s_most[TPB / 64];
+    if ((threadIdx.x & 63) == 0) s_most[threadIdx.x >> 6] = most;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        for (int w = 1; w < (int)(blockDim.x >> 6); w++) most = s_most[w] > most ? s_most[w] : most;
+        if (most) atomicMax(maxc, most);
+    }
 }
 
 // bytes per count that hold every count up to most
