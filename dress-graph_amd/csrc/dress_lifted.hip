@@ -3277,11 +3277,14 @@ int dress_hip_lifted_batch(int kind, int k, int n_graphs, const int *N, const in
                 good = d2h(out->hist_value + runs_total, hv, (size_t)runs * sizeof(double), cs ? cs : s);
                 if (good && packed && width < 8 && narrow_counts && cs && runs > 0) {
                     // into a pinned staging block (pool-sized, so the copy
-                    // is asynchronous), widened after the event
+                    // is asynchronous), widened after the event (a blocking
+                    // one: the host threads that wait for it sleep instead
+                    // of spinning beside the calling thread)
                     void *stg = pinned_alloc(std::max<size_t>((size_t)runs * (size_t)width, (size_t)1 << 20));
                     hipEvent_t ev = nullptr;
                     if (stg) widen.staging.push_back(stg);
-                    good = stg && hipEventCreateWithFlags(&ev, hipEventDisableTiming) == hipSuccess;
+                    good = stg && hipEventCreateWithFlags(&ev, hipEventDisableTiming | hipEventBlockingSync) ==
+                                      hipSuccess;
                     if (ev) widen.events.push_back(ev);
                     good = good && d2h(stg, packed, (size_t)runs * (size_t)width, cs) &&
                            hipEventRecord(ev, cs) == hipSuccess;
