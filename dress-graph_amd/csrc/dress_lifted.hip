@@ -1968,11 +1968,14 @@ bool run_launch(RunState &R, const std::vector<const HostParent *> &ps, const st
            launch_ids(R, ps, item_parent, J, R.d_order, R.order, s, out);
 }
 
+// rs: the stream for the FP64 redo of ambiguous mixed-precision items
+// (default s), so that the redo need not wait for later work queued on s
 bool run_finish(RunState &R, const std::vector<const HostParent *> &ps, const std::vector<int> &graph_of,
                 const std::vector<int> &item_parent, const LiftJob &J, hipStream_t s, LiftOut &out,
                 unsigned long long *qkeys, unsigned long long *qcounts, int64_t qcap, double qtol,
-                const FitOutputs *fo, bool ok)
+                const FitOutputs *fo, bool ok, hipStream_t rs = nullptr)
 {
+    hipStream_t xs = rs ? rs : s;
     const int n = R.n;
     PhaseTimer pt("  run_items");
     // (the end of the last launch: later work may already be queued on s)
@@ -1992,8 +1995,8 @@ bool run_finish(RunState &R, const std::vector<const HostParent *> &ps, const st
     // FP64 redo of ambiguous mixed-precision items
     if (ok && n > 0 && J.mixed) {
         std::vector<int> flags((size_t)n);
-        ok = hipMemcpyAsync(flags.data(), R.d_flags, (size_t)n * sizeof(int), hipMemcpyDeviceToHost, s) ==
-                 hipSuccess && hipStreamSynchronize(s) == hipSuccess;
+        ok = hipMemcpyAsync(flags.data(), R.d_flags, (size_t)n * sizeof(int), hipMemcpyDeviceToHost, xs) ==
+                 hipSuccess && hipStreamSynchronize(xs) == hipSuccess;
         std::vector<int> redo;
         for (int qi = 0; qi < n && ok; qi++)        // keeps the decreasing-size order
             if (flags[(size_t)R.order[(size_t)qi]]) redo.push_back(R.order[(size_t)qi]);
@@ -2005,11 +2008,11 @@ bool run_finish(RunState &R, const std::vector<const HostParent *> &ps, const st
             // the quantised counts of the redone items' mixed values come
             // back out first; the FP64 redo counts their final values
             ok = (!qkeys || unquantise(out.d_vals, out.val_ofs, redo, item_parent, graph_of, J.eps, qtol, qkeys,
-                                       qcounts, qcap, s)) &&
+                                       qcounts, qcap, xs)) &&
                  hipMemcpyAsync(R.d_items, items2.data(), items2.size() * sizeof(LiftItem), hipMemcpyHostToDevice,
-                                s) == hipSuccess &&
-                 upload(&d_redo, redo, s, "redo list");
-            if (ok) ok = launch_ids(R, ps, item_parent, J, d_redo, redo, s, out) && hipStreamSynchronize(s) == hipSuccess;
+                                xs) == hipSuccess &&
+                 upload(&d_redo, redo, xs, "redo list");
+            if (ok) ok = launch_ids(R, ps, item_parent, J, d_redo, redo, xs, out) && hipStreamSynchronize(xs) == hipSuccess;
             if (ok) add_times();
             dfree(d_redo);
         }
@@ -2027,8 +2030,11 @@ bool run_finish(RunState &R, const std::vector<const HostParent *> &ps, const st
         ok = hipMemcpyAsync(out.work, R.d_work, sizeof(out.work), hipMemcpyDeviceToHost, s) == hipSuccess &&
              hipStreamSynchronize(s) == hipSuccess;
     pt.mark("reruns");
-    // (on a failure the stream may still hold work that reads these blocks)
-    if (!ok) (void)hipStreamSynchronize(s);
+    // (on a failure the streams may still hold work that reads these blocks)
+    if (!ok) {
+        (void)hipStreamSynchronize(s);
+        if (rs) (void)hipStreamSynchronize(rs);
+    }
     R.release();
     pt.mark("free");
     if (!ok) {
@@ -3092,13 +3098,14 @@ int dress_hip_lifted_batch(int kind, int k, int n_graphs, const int *N, const in
         return seg_ofs;
     };
     // With merged runs, each group's histogram is planned with the group
-    // and queued on hs behind an event after the group's launches, with no
-    // wait on the host: it runs beside the next group's launches, and the
-    // host waits for its event only to queue the group's downloads.
+    // and queued on hs behind an event after the group's launches (in
+    // FP32->FP64, after the FP64 redo of its ambiguous items), with no wait
+    // on the host: it runs beside the next group's launches, and the host
+    // waits for its event only to queue the group's downloads.
     // DRESS_HIP_HIST_ASYNC=0 makes the host wait for each group's launches,
     // then run its histogram on hs.
     const char *ha = getenv("DRESS_HIP_HIST_ASYNC");
-    const bool async_hist = !(ha && !strcmp(ha, "0")) && G > 1 && hs != s && !J.mixed && !quant;
+    const bool async_hist = !(ha && !strcmp(ha, "0")) && G > 1 && hs != s && !quant;
     std::vector<std::unique_ptr<GroupPlan>> plans((size_t)G);
     {
         // host threads building parents: with several groups only the first
@@ -3196,16 +3203,12 @@ int dress_hip_lifted_batch(int kind, int k, int n_graphs, const int *N, const in
         // host waits for a histogram
         bool chain = ok;                                 // every launch so far queued
         int queued = 0;                                  // groups [0, queued) are on s
-        auto queue_group = [&](int g) {
-            if (!chain || g >= G) return;
-            if (!launch_group(g)) {
-                chain = false;
-                return;
-            }
-            queued = g + 1;
+        // group g's planned histogram queued on hs, behind the work on s so
+        // far when after_s
+        auto enqueue_hist = [&](int g, bool after_s) {
             GroupPlan &P = *plans[(size_t)g];
             if (!P.hist_planned) return;
-            if (hipEventRecord(kev, s) != hipSuccess || hipStreamWaitEvent(hs, kev, 0) != hipSuccess) {
+            if (after_s && (hipEventRecord(kev, s) != hipSuccess || hipStreamWaitEvent(hs, kev, 0) != hipSuccess)) {
                 set_error("histogram event failed");
                 chain = false;
                 return;
@@ -3216,6 +3219,15 @@ int dress_hip_lifted_batch(int kind, int k, int n_graphs, const int *N, const in
                 set_error("histogram event failed");
                 chain = false;
             }
+        };
+        auto queue_group = [&](int g) {
+            if (!chain || g >= G) return;
+            if (!launch_group(g)) {
+                chain = false;
+                return;
+            }
+            queued = g + 1;
+            if (!J.mixed) enqueue_hist(g, true);
         };
         // whether group g's plan is ready (then queueing it does not wait)
         auto plan_ready = [&](int g) {
@@ -3321,9 +3333,15 @@ int dress_hip_lifted_batch(int kind, int k, int n_graphs, const int *N, const in
         // only for launches.
         for (int gr = 0; ok && gr < G; gr++) {
             GroupPlan &P = *plans[(size_t)gr];
+            // (FP32->FP64: the FP64 redo on hs, and the histogram behind it
+            // there, since the group's launches are done)
             ok = run_finish(P.R, P.ps, P.graph_of, P.item_parent, J, s, P.lo, qk, qc, quant ? qcap : 0,
-                            quant ? qtol : 0.0, nullptr, gr < queued);
+                            quant ? qtol : 0.0, nullptr, gr < queued, J.mixed && async_hist ? hs : nullptr);
             pt.mark("group launches done");
+            if (ok && J.mixed) {                         // the values are final after the FP64 redo
+                enqueue_hist(gr, false);
+                if (!chain) ok = false;
+            }
             // the group after next: now when its plan is ready, so that s
             // always holds a group; else after the downloads
             bool next_queued = false;
