@@ -1203,7 +1203,9 @@ bool hist_enqueue(AsyncHist &H, HistPlan &P, const double *vals, int64_t n, hipS
         H.val = reinterpret_cast<double *>(sorted);
         H.packed = other;
         unsigned long long *maxc = reinterpret_cast<unsigned long long *>(H.stat + 2);
-        hipLaunchKernelGGL(k_runs_out, dim3(grid), dim3(TPB), 0, s, (const uint64_t *)other, (const int64_t *)H.cnt,
+        // (2 workgroups per CU: each ends with one atomic on the same word)
+        hipLaunchKernelGGL(k_runs_out, dim3(grid_for(n, num_cus() * 2)), dim3(TPB), 0, s, (const uint64_t *)other,
+                           (const int64_t *)H.cnt,
                            (const unsigned long long *)H.range, (const int64_t *)H.stat, (const int *)bad, nseg,
                            H.val, H.stat + HistPlan::HEAD, maxc);
         hipLaunchKernelGGL(k_pack_counts, dim3(grid), dim3(TPB), 0, s, (const int64_t *)H.cnt,
