@@ -301,7 +301,7 @@ def lifted_run(dh, gen):
             os.environ["DRESS_HIP_BATCH_GROUP_VALUES"] = gv
         try:
             for _ in range(6):
-                dh.lifted_batch(batch, k=1, precision=dh.FP64)
+                r = dh.lifted_batch(batch, k=1, precision=dh.FP64)
             for _ in range(3):
                 t0 = time.perf_counter()
                 r = dh.lifted_batch(batch, k=1, precision=dh.FP64)
@@ -500,10 +500,11 @@ def main():
         hists = {}
         for name, prec in (("fp64", dh.FP64), ("fp32_fp64", dh.FP32_FP64)):
             # repeated calls are the workload: the first calls fill the
-            # device-block and pinned-host caches (up to the sixth call of
-            # a process ran 6-14 ms instead of 3.7-4.3), so six are untimed
+            # device-block and pinned-host caches, so six are untimed, each
+            # kept until the next returns as the timed calls are (two sets
+            # of result blocks)
             for _ in range(6):
-                dh.lifted_batch(batch, k=1, precision=prec)
+                r = dh.lifted_batch(batch, k=1, precision=prec)
             runs = []
             for _ in range(11):
                 t0 = time.perf_counter()

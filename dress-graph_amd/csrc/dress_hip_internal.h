@@ -71,7 +71,9 @@ void ctx_begin(hipStream_t stream);
 void *host_alloc(size_t bytes);    // malloc-compatible (free() releases it)
 void ctx_end();                    // returns the cached blocks to the device
 // pinned host blocks for batch results; pinned_free also takes malloc'd ones
-void *pinned_alloc(size_t bytes);
+// host_writes: the caller fills the block from host threads (its pages are
+// touched once, the first time it is handed out so)
+void *pinned_alloc(size_t bytes, bool host_writes = false);
 void pinned_free(void *p);
 // large copies between pageable host memory and the device, host-parallel
 bool d2h_parallel(void *dst, const void *src, size_t bytes);

@@ -2646,12 +2646,18 @@ public:
             HostTeam::get().submit([this, src, dst, a, b, width, ev] {
                 const bool ok = hipEventSynchronize(ev) == hipSuccess;
                 if (ok) {
+                    // streaming stores: the result is not read here again,
+                    // and ordinary stores would first read every line
                     if (width == 1)
-                        for (int64_t i = a; i < b; i++) dst[i] = static_cast<const uint8_t *>(src)[i];
+                        for (int64_t i = a; i < b; i++)
+                            __builtin_nontemporal_store((int64_t)static_cast<const uint8_t *>(src)[i], dst + i);
                     else if (width == 2)
-                        for (int64_t i = a; i < b; i++) dst[i] = static_cast<const uint16_t *>(src)[i];
+                        for (int64_t i = a; i < b; i++)
+                            __builtin_nontemporal_store((int64_t)static_cast<const uint16_t *>(src)[i], dst + i);
                     else
-                        for (int64_t i = a; i < b; i++) dst[i] = static_cast<const uint32_t *>(src)[i];
+                        for (int64_t i = a; i < b; i++)
+                            __builtin_nontemporal_store((int64_t)static_cast<const uint32_t *>(src)[i], dst + i);
+                    __builtin_ia32_sfence();              // before the chunk counts as done
                 }
                 // (notified under the lock: the set may go as soon as the
                 // waiter sees pending_ reach 0)
@@ -3038,9 +3044,16 @@ int dress_hip_lifted_batch(int kind, int k, int n_graphs, const int *N, const in
     out->qhist_offset = (int64_t *)malloc(((size_t)n_graphs + 1) * sizeof(int64_t));
     out->qbound_offset = (int64_t *)malloc(((size_t)n_graphs + 1) * sizeof(int64_t));
     bool ok = out->hist_offset && out->qhist_offset && out->qbound_offset;
+    // A pipelined group's counts come down at the narrowest width that
+    // holds its largest count and are widened on host threads (into a
+    // block whose pages have been touched), so the link carries 9 rather
+    // than 16 bytes per distinct value when every count is below 256.
+    // DRESS_HIP_NARROW_COUNTS=0 downloads them as int64.
+    const char *nc = getenv("DRESS_HIP_NARROW_COUNTS");
+    const bool narrow_counts = !(nc && !strcmp(nc, "0"));
     if (ok && G > 1) {
         out->hist_value = (double *)pinned_alloc(((size_t)cap + 1) * sizeof(double));
-        out->hist_count = (int64_t *)pinned_alloc(((size_t)cap + 1) * sizeof(int64_t));
+        out->hist_count = (int64_t *)pinned_alloc(((size_t)cap + 1) * sizeof(int64_t), narrow_counts);
         ok = out->hist_value && out->hist_count;
     }
     if (!ok) set_error("host allocation of batch results failed");
@@ -3049,12 +3062,6 @@ int dress_hip_lifted_batch(int kind, int k, int n_graphs, const int *N, const in
         return bytes == 0 || hipMemcpyAsync(dst, src, bytes, hipMemcpyDeviceToHost, st) == hipSuccess;
     };
     std::vector<void *> held;            // device blocks that downloads still read
-    // A pipelined group's counts come down at the narrowest width that
-    // holds its largest count and are widened on host threads, so the link
-    // carries 9 rather than 16 bytes per distinct value when every count is
-    // below 256.  DRESS_HIP_NARROW_COUNTS=0 downloads them as int64.
-    const char *nc = getenv("DRESS_HIP_NARROW_COUNTS");
-    const bool narrow_counts = !(nc && !strcmp(nc, "0"));
     CountWidening widen;
     unsigned long long *qk = nullptr, *qc = nullptr;
     int64_t qcap = 1024, runs_total = 0, subgraphs = 0, values = 0;
@@ -3294,7 +3301,8 @@ int dress_hip_lifted_batch(int kind, int k, int n_graphs, const int *N, const in
                     // is asynchronous), widened after the event (a blocking
                     // one: the host threads that wait for it sleep instead
                     // of spinning beside the calling thread)
-                    void *stg = pinned_alloc(std::max<size_t>((size_t)runs * (size_t)width, (size_t)1 << 20));
+                    void *stg =
+                        pinned_alloc(std::max<size_t>((size_t)runs * (size_t)width, (size_t)1 << 20), true);
                     hipEvent_t ev = nullptr;
                     if (stg) widen.staging.push_back(stg);
                     good = stg && hipEventCreateWithFlags(&ev, hipEventDisableTiming | hipEventBlockingSync) ==
