@@ -1125,7 +1125,6 @@ int hist_plan(HistPlan &P, const std::vector<int64_t> &seg_ofs, const std::vecto
 // Every block the queued work reads is held here until the caller has
 // waited for that work (hist_collect, then hist_release).
 struct AsyncHist {
-    int64_t n = 0;
     int nseg = 0;
     uint64_t *k0 = nullptr, *k1 = nullptr;
     double *val = nullptr;                // k0 or k1: the runs' values
@@ -1161,7 +1160,6 @@ bool hist_enqueue(AsyncHist &H, HistPlan &P, const double *vals, int64_t n, hipS
 {
     H = AsyncHist();
     const int nseg = P.nseg;
-    H.n = n;
     H.nseg = nseg;
     H.host = P.host;
     P.host = nullptr;
