@@ -143,7 +143,6 @@ int copy_threads();
 // block from host threads asks for it touched: each page is written once,
 // by several threads for large blocks, the first time the block is handed
 // out that way, so that the faults do not land inside its later calls.
-namespace {
 void touch_pages(void *q, size_t bytes)
 {
     constexpr size_t PAGE = 4096;
@@ -164,7 +163,6 @@ void touch_pages(void *q, size_t bytes)
     touch(0);
     for (auto &t : team) t.join();
 }
-}  // namespace
 
 void *pinned_alloc(size_t bytes, bool host_writes)
 {
@@ -398,6 +396,81 @@ bool d2h_parallel(void *dst, const void *src, size_t bytes)
             good = good && hipEventSynchronize(ev[j]) == hipSuccess;
             if (good) memcpy((char *)dst + at, stage[j], std::min(CH, e - at));
         }
+        if (st && hipStreamSynchronize(st) != hipSuccess) good = false;
+        if (own) drop_lane(lane);
+        if (!good) ok = false;
+    };
+    std::vector<std::thread> pool;
+    for (int w = 1; w < T; w++) pool.emplace_back(slice, w);
+    slice(0);
+    for (auto &th : pool) th.join();
+    pinned_free(staging);
+    return ok.load();
+}
+
+// The flat histogram (value bits, count; 2 * runs int64) into pageable dst
+// from the device's values and its counts narrowed to `width` (1, 2 or 4)
+// bytes, as d2h_parallel moves a buffer: T host threads each take a slice of
+// the runs and move it in chunks of values and counts through two pinned
+// staging pairs of their own on a stream of their own, interleaving each
+// chunk into place while the next one crosses.  The link then carries
+// 8 + width bytes a run instead of 16.  vals and counts must be complete.
+bool d2h_flat_narrow(int64_t *dst, const double *vals, const void *counts, int width, int64_t runs)
+{
+    constexpr int64_t CH = (int64_t)1 << 17;                  // runs a chunk
+    if (runs <= 0) return true;
+    if (width != 1 && width != 2 && width != 4) return false;
+    const size_t bytes = (size_t)runs * 16;
+    const int T = (int)std::max<size_t>(1, std::min<size_t>({(size_t)copy_threads(), bytes / ((size_t)32 << 20),
+                                                              (size_t)std::max(1u, std::thread::hardware_concurrency())}));
+    int dev = 0;
+    hipGetDevice(&dev);
+    const size_t pair = (size_t)CH * (8 + (size_t)width);       // one chunk's values, then its counts
+    // (read by the host threads: pages touched)
+    char *staging = (char *)pinned_alloc((size_t)T * 2 * pair, true);
+    if (!staging) return false;
+    std::atomic<bool> ok{true};
+    LaneSet lanes(dev, T);
+    auto slice = [&](int w) {
+        hipSetDevice(dev);
+        const int64_t b = runs * w / T, e = runs * (w + 1) / T;
+        char *stage[2] = {staging + (size_t)w * 2 * pair, staging + (size_t)w * 2 * pair + pair};
+        CopyLane lane;
+        bool own = false;
+        bool good = lanes.get(w, lane, &own);
+        hipStream_t st = lane.st;
+        hipEvent_t *ev = lane.ev;
+        auto issue = [&](int64_t at, int j) {
+            const int64_t n = std::min(CH, e - at);
+            return hipMemcpyAsync(stage[j], vals + at, (size_t)n * 8, hipMemcpyDeviceToHost, st) == hipSuccess &&
+                   hipMemcpyAsync(stage[j] + (size_t)CH * 8, static_cast<const char *>(counts) + (size_t)at * width,
+                                  (size_t)n * width, hipMemcpyDeviceToHost, st) == hipSuccess &&
+                   hipEventRecord(ev[j], st) == hipSuccess;
+        };
+        if (good && b < e) good = issue(b, 0);
+        int j = 0;
+        for (int64_t at = b; good && at < e; at += CH, j ^= 1) {
+            if (at + CH < e) good = issue(at + CH, j ^ 1);
+            good = good && hipEventSynchronize(ev[j]) == hipSuccess;
+            if (!good) break;
+            const int64_t n = std::min(CH, e - at);
+            const int64_t *v = reinterpret_cast<const int64_t *>(stage[j]);
+            const char *c = stage[j] + (size_t)CH * 8;
+            // streaming stores: the destination is not read here, and
+            // ordinary stores would first read every line of it
+            int64_t *o = dst + 2 * at;
+            auto put = [o](int64_t i, int64_t value, int64_t count) {
+                __builtin_nontemporal_store(value, o + 2 * i);
+                __builtin_nontemporal_store(count, o + 2 * i + 1);
+            };
+            if (width == 1)
+                for (int64_t i = 0; i < n; i++) put(i, v[i], reinterpret_cast<const uint8_t *>(c)[i]);
+            else if (width == 2)
+                for (int64_t i = 0; i < n; i++) put(i, v[i], reinterpret_cast<const uint16_t *>(c)[i]);
+            else
+                for (int64_t i = 0; i < n; i++) put(i, v[i], reinterpret_cast<const uint32_t *>(c)[i]);
+        }
+        __builtin_ia32_sfence();                         // the stores visible before the join
         if (st && hipStreamSynchronize(st) != hipSuccess) good = false;
         if (own) drop_lane(lane);
         if (!good) ok = false;

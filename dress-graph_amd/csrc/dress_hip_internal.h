@@ -74,9 +74,14 @@ void ctx_end();                    // returns the cached blocks to the device
 // host_writes: the caller fills the block from host threads (its pages are
 // touched once, the first time it is handed out so)
 void *pinned_alloc(size_t bytes, bool host_writes = false);
+// every page of [p, p + bytes) written once (several threads from 32 MiB)
+void touch_pages(void *p, size_t bytes);
 void pinned_free(void *p);
 // large copies between pageable host memory and the device, host-parallel
 bool d2h_parallel(void *dst, const void *src, size_t bytes);
+// the flat (value bits, count) histogram of runs values into pageable dst,
+// from device values and counts narrowed to width (1, 2, 4) bytes
+bool d2h_flat_narrow(int64_t *dst, const double *vals, const void *counts, int width, int64_t runs);
 bool h2d_parallel(void *dst, const void *src, size_t bytes);
 // pinned blocks of SNAPSHOT_BYTES for the state snapshots of a running fit
 // or pass: take at its start, give back at its end (nullptr on failure,
