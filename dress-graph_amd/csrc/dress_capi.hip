@@ -414,10 +414,13 @@ bool d2h_parallel(void *dst, const void *src, size_t bytes)
 // the runs and move it in chunks of values and counts through two pinned
 // staging pairs of their own on a stream of their own, interleaving each
 // chunk into place while the next one crosses.  The link then carries
-// 8 + width bytes a run instead of 16.  vals and counts must be complete.
+// 8 + width bytes a run instead of 16.  Chunks of 2^20 runs (9 MB at 1 byte
+// a count; 18 MB of pooled staging a thread): the 49M runs of a delta^2
+// histogram took 9.2 ms so, against 10.1 ms with 2^19 and 15.2 ms with
+// 2^17.  vals and counts must be complete.
 bool d2h_flat_narrow(int64_t *dst, const double *vals, const void *counts, int width, int64_t runs)
 {
-    constexpr int64_t CH = (int64_t)1 << 17;                  // runs a chunk
+    constexpr int64_t CH = (int64_t)1 << 20;                  // runs a chunk
     if (runs <= 0) return true;
     if (width != 1 && width != 2 && width != 4) return false;
     const size_t bytes = (size_t)runs * 16;
