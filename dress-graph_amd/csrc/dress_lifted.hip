@@ -1310,6 +1310,24 @@ bool hist_size_fits(int64_t runs)
 // the result block in from host threads while the device worked made calls
 // slower, 790 MB results 32-36 -> 71-92 ms, most likely because the faults
 // contend with the runtime's own mappings for the address space.)
+// Pages of a host block touched by a host thread (several from 32 MiB)
+// while the caller goes on; join() waits, as does the destructor.
+class BackgroundTouch {
+public:
+    BackgroundTouch() = default;
+    BackgroundTouch(const BackgroundTouch &) = delete;
+    BackgroundTouch &operator=(const BackgroundTouch &) = delete;
+    ~BackgroundTouch() { join(); }
+    void start(void *p, size_t bytes) { th_ = std::thread([p, bytes] { touch_pages(p, bytes); }); }
+    void join()
+    {
+        if (th_.joinable()) th_.join();
+    }
+
+private:
+    std::thread th_;
+};
+
 // A host block for a result the caller frees with free(), allocated and
 // page-touched by host threads while the device computes: writing 0.79 GB
 // into fresh memory otherwise takes its page faults inside the download.
@@ -2302,12 +2320,27 @@ int64_t *lifted_flat(int kind, p_dress_graph_t g, int k, int iterations, double 
     const bool do_hist = hist_size != nullptr;
     const int64_t total = (k == 0) ? 1 : (kind == LIFT_DELTA ? dress_delta_binom(N, k) : dress_nabla_perm_count(N, k));
     if (num_items) *num_items = total;
+    int mode;
+    {
+        std::lock_guard<std::mutex> lk(g_lift_mu);
+        mode = g_lift_mode;
+    }
+    // the host parent (CSR, intersection pairs, DIRECTED reciprocal
+    // lookups) serves the LDS-resident kernel and the DIRECTED delta weight
+    // patches; the paths on the device graph need nothing else of it
+    const bool lds = mode == 0 && lds_eligible_size(N, E) && iterations > 0;
     double *host_ms = nullptr;
+    BackgroundTouch ms_touch;                    // (joined before the download, and on every return)
     if (keep_multisets && multisets) {
         size_t len = (size_t)std::max<int64_t>(total, 0) * (size_t)E;
         host_ms = (double *)host_alloc((len ? len : 1) * sizeof(double));
         if (host_ms) {
-            fill_nan_parallel(host_ms, len);
+            // The LDS kernel's path fills the device copy with NaN and
+            // downloads all of it, so the host block is only page-touched,
+            // beside the items (0.39 GB for delta^2 of a 256-vertex graph);
+            // the other paths write rows into a NaN-filled block.
+            if (lds && k > 0 && k < N && total > 0) ms_touch.start(host_ms, len * sizeof(double));
+            else fill_nan_parallel(host_ms, len);
             *multisets = host_ms;
         }
     }
@@ -2346,15 +2379,6 @@ int64_t *lifted_flat(int kind, p_dress_graph_t g, int k, int iterations, double 
     memset(J.marks, 0, sizeof(J.marks));
     if (kind == LIFT_NABLA) marker_weights(J.marks, k);
 
-    int mode;
-    {
-        std::lock_guard<std::mutex> lk(g_lift_mu);
-        mode = g_lift_mode;
-    }
-    // the host parent (CSR, intersection pairs, DIRECTED reciprocal
-    // lookups) serves the LDS-resident kernel and the DIRECTED delta weight
-    // patches; the paths on the device graph need nothing else of it
-    const bool lds = mode == 0 && lds_eligible_size(N, E) && iterations > 0;
     HostParent P;
     if (lds || (kind == LIFT_DELTA && g->variant == DRESS_VARIANT_DIRECTED && (mode == 0 || mode == 2 || mode == 3)))
         host_parent_build(N, E, g->U, g->V, g->W, (int)g->variant, P);
@@ -2451,9 +2475,11 @@ int64_t *lifted_flat(int kind, p_dress_graph_t g, int k, int iterations, double 
             double *d_vals = nullptr;
             ok = chunks.empty() || run_chunk(chunks[0], &d_vals, &n_values);
             pt.mark("items");
-            if (ok && lds && host_ms && total > 0)
+            if (ok && lds && host_ms && total > 0) {
+                ms_touch.join();
                 ok = hipStreamSynchronize(s) == hipSuccess &&
                      d2h_parallel(host_ms, d_ms, (size_t)total * (size_t)E * sizeof(double));
+            }
             if (ok && do_hist) ok = histogram_flat(d_vals, n_values, s, &flat, hist_size, &pre);
             pt.mark("histogram");
             dfree(d_vals);
@@ -2476,9 +2502,11 @@ int64_t *lifted_flat(int kind, p_dress_graph_t g, int k, int iterations, double 
                 n_values += nv;
                 if (do_hist) merge_histograms(acc_v, acc_c, v, c);
             }
-            if (ok && lds && host_ms && total > 0)
+            if (ok && lds && host_ms && total > 0) {
+                ms_touch.join();
                 ok = hipStreamSynchronize(s) == hipSuccess &&
                      d2h_parallel(host_ms, d_ms, (size_t)total * (size_t)E * sizeof(double));
+            }
             if (ok && do_hist) ok = flat_from_vectors(acc_v, acc_c, &flat, hist_size);
             pt.mark("chunks");
         }
