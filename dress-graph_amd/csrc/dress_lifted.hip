@@ -2090,8 +2090,9 @@ bool run_launch(RunState &R, const std::vector<const HostParent *> &ps, const st
         a.iters = R.d_iters;
         a.deltas = R.d_deltas;
     }
-    return hipMemsetAsync(R.d_flags, 0, (size_t)n * sizeof(int), s) == hipSuccess &&
-           launch_ids(R, ps, item_parent, J, R.d_order, R.order, s, out);
+    // (no clearing of the item flags: the launches run every item, and
+    // k_lifted writes each item's flag when it ends the item)
+    return launch_ids(R, ps, item_parent, J, R.d_order, R.order, s, out);
 }
 
 // rs: the stream for the FP64 redo of ambiguous mixed-precision items
