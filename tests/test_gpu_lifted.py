@@ -298,6 +298,23 @@ def test_grouped_batch_narrow_counts(gpu, ora, gen, monkeypatch):
             assert np.array_equal(v1, ov) and np.array_equal(c1, oc)
 
 
+@pytest.mark.parametrize("m,width", [(5, 2), (40, 4)])
+def test_large_flat_histogram_narrow_counts(gpu, ora, gen, m, width):
+    """A single-graph histogram of 2^20 runs or more comes down as values and
+    counts narrowed to the largest count's width, interleaved on the host:
+    G(700, p) beside a complete K_m, whose edges keep one value in every
+    subgraph that keeps K_m (a count past 255 for K_5, past 65,535 for
+    K_40), equals the oracle's histogram."""
+    n, U, V, _ = gen.gnp(700, 6.0, 3)
+    iu = np.triu_indices(m, 1)
+    U = np.concatenate([U, (iu[0] + n).astype(np.int32)])
+    V = np.concatenate([V, (iu[1] + n).astype(np.int32)])
+    ov, oc, _, _ = ora.lifted("delta", n + m, U, V, k=1)
+    assert ov.size >= (1 << 20) and 1 << (8 * (width // 2)) < oc.max() < 1 << (8 * width)
+    hv, hc = _lifted(gpu, "delta", n + m, U, V, k=1)[:2]
+    assert np.array_equal(hv, ov) and np.array_equal(hc, oc)
+
+
 def test_histogram_key_layouts_agree(gpu, ora, gen, monkeypatch):
     """The exact histogram sorts one 64-bit key per value holding (graph,
     value) when the values' keys span few enough bits to leave room for the
